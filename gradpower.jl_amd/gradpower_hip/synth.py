@@ -1,0 +1,391 @@
+"""Seeded synthetic cases for the benchmark and the parity tests.
+
+Each builder returns a `Case` holding the set-up PowerSystem, the initialised
+DynamicProblem, the flattened C-ABI descriptor and the per-scenario fault /
+horizon tables.  Random draws use numpy's PCG64 with seed 1000 + case index;
+the drawn values that the case definitions leave open are recorded in
+`case.meta`.
+
+  case_2bus_testdata   -- the reference's test/testdata/2bus.{raw,dyr} system
+  case0_2bus           -- 2-bus GENROU+IEESGO, constant-impedance load, 10 s
+  case1_9bus           -- 9-bus ring + 3 chords, 3 x GENROU+SEXS+TGOV1, M=64, 1 s
+  case2_grid           -- random geometric grid (default 2,000 buses), 400 machines
+                          with SEXS/ESDC1A + TGOV1/IEESGO, 100 IEEEST, 300 static
+                          generators, M=256 uneven faults and horizons
+  case3_200bus         -- 200 buses, 40 x GENROU+SEXS+TGOV1, M=128, 3 s
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .system import (Bus, Gen, Load, Branch, Shunt, PowerSystem, PowerSystemDynamics,
+                     Genrou, IEESGO, TGOV1, SEXS, ESDC1A, IEEEST, ContingencyEvent,
+                     esdc1a_sat_coefficients, set_dynamics, DynamicProblem, PQ, PV, SLACK)
+from .network import build_network, runpf_inplace
+from .init import initialize_dynamics
+from .desc import build_desc, SystemDesc
+
+DT = 1.0 / 120.0
+
+
+@dataclass
+class Case:
+    name: str
+    ps: PowerSystem
+    dp: DynamicProblem
+    desc: SystemDesc
+    M: int
+    dt: float
+    events: list                  # per scenario: list of ContingencyEvent
+    tf: np.ndarray                # per scenario horizon (s)
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nsteps(self):
+        return np.array([int(round(t / self.dt)) for t in self.tf], dtype=np.int32)
+
+
+# Machine data of the 2-bus spec (on machine base).
+C0_GENROU = dict(x_d=1.8, x_q=1.7, x_dp=0.3, x_qp=0.55, x_ddp=0.25, xl=0.2, H=3.5, D=0.0,
+                 T_d0p=8.0, T_q0p=0.4, T_d0dp=0.03, T_q0dp=0.05, S1=0.03, S2=0.4)
+
+
+def _genrou(bus, gid="1", **over):
+    p = dict(C0_GENROU)
+    p.update(over)
+    return Genrou(bus, gid, p["x_d"], p["x_q"], p["x_dp"], p["x_qp"], p["x_ddp"], p["xl"],
+                  p["H"], p["D"], p["T_d0p"], p["T_q0p"], p["T_d0dp"], p["T_q0dp"],
+                  p["S1"], p["S2"])
+
+
+def _sexs(bus, gid="1"):
+    return SEXS(bus, gid, 0.1, 10.0, 50.0, 0.05, -5.0, 5.0)
+
+
+def _tgov1(bus, gid="1"):
+    return TGOV1(bus, gid, 0.05, 0.5, 1.1, 0.0, 2.1, 7.0, 0.0)
+
+
+def _esdc1a(bus, gid="1"):
+    sa, sb = esdc1a_sat_coefficients(2.8, 0.04, 3.73, 0.33)
+    return ESDC1A(bus, gid, 46.0, 0.06, 0.1, 1.0, 1.0, 0.46, 0.0, sa, sb)
+
+
+def _ieesgo(bus, gid, rng, meta):
+    T = rng.uniform(0.05, 0.5, size=6)
+    meta.setdefault("ieesgo_T", []).append(T.tolist())
+    return IEESGO(bus, gid, *T, 20.0, 0.5, 0.5, 1.0, 0.0)
+
+
+def _ieeest(bus, gid="1"):
+    return IEEEST(bus, gid, 1, 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.15, 0.05, 0.15, 0.05,
+                  2.0, 2.0, 5.0, 0.1, -0.1, 0.0, 0.0)
+
+
+def _solve_pf_scaling(ps, meta):
+    """Power flow by continuation in the loading: solve with every load and
+    scheduled generation scaled by s, starting at s = 0 and raising s towards
+    1 from the previous solution (halving the increment after a failure).  If
+    s = 1 cannot be reached the largest solved s is kept and recorded in
+    meta["load_scale"]."""
+    from .network import runpf
+    pd0 = [(ld.pd, ld.qd) for ld in ps.loads]
+    pg0 = [g.psch for g in ps.gens]
+    vset = [b.v0m for b in ps.buses]
+
+    def apply(sc):
+        for ld, (pd, qd) in zip(ps.loads, pd0):
+            ld.pd, ld.qd = pd * sc, qd * sc
+        for g, p0 in zip(ps.gens, pg0):
+            g.psch = p0 * sc
+
+    s_ok, step, sol = None, 1.0, None
+    s_try = 1.0
+    while True:
+        apply(s_try)
+        try:
+            vm, va, _, _ = runpf(ps)
+            s_ok, sol = s_try, (vm, va)
+        except RuntimeError:
+            if s_try == 0.0:
+                raise
+            if s_ok is None:
+                s_try, step = 0.0, 0.25
+                continue
+            step *= 0.5
+            if step < 1.0 / 256:
+                break
+            s_try = s_ok + step
+            continue
+        if s_ok >= 1.0:
+            break
+        for b, m, a, vs in zip(ps.buses, sol[0], sol[1], vset):
+            b.v0a = float(a)
+            b.v0m = float(m) if b.type == PQ else vs
+        s_try = min(1.0, s_ok + step)
+    apply(s_ok)
+    for b, m, a, vs in zip(ps.buses, sol[0], sol[1], vset):
+        b.v0a = float(a)
+        b.v0m = float(m) if b.type == PQ else vs
+    runpf_inplace(ps)
+    meta["load_scale"] = s_ok
+
+
+def _finish(name, ps, psd, M, events, tf, meta, zip_alpha=None):
+    set_dynamics(ps, psd)
+    if zip_alpha is not None:
+        for dev in psd.devices:
+            if dev.dtype.kind == "zipload":
+                dev.dtype.alpha = zip_alpha
+    build_network(ps)
+    _solve_pf_scaling(ps, meta)
+    # ZIP loads were created from ps.loads in order by set_dynamics; carry a
+    # load scaling applied by the power-flow fallback over to them.
+    zips = [dev.dtype for dev in psd.devices if dev.dtype.kind == "zipload"]
+    for z, ld in zip(zips, ps.loads):
+        z.pinj, z.qinj = ld.pd, ld.qd
+    dp = DynamicProblem(ps)
+    initialize_dynamics(dp, ps)
+    desc = build_desc(ps, dp)
+    return Case(name, ps, dp, desc, M, DT, events, np.asarray(tf, dtype=float), meta)
+
+
+def case_2bus_testdata(fault=True, tf=1.0):
+    """test/testdata/2bus.raw + 2bus.dyr: slack bus 1 with one GENROU, PQ bus 2
+    with a 103.13 MW / 37.617 Mvar load, one line.  Values typed from those
+    files (PSS/E v33 fields; gen mbase 100 MVA)."""
+    buses = [Bus(1, "BUS1", SLACK, 345.0, 1.04, 0.0),
+             Bus(2, "BUS2", PQ, 345.0, 1.01613, math.radians(-3.3252))]
+    gens = [Gen(0, "1", 106.496 / 100.0, 45.996 / 100.0, 100.0)]
+    loads = [Load(1, "11", 103.130 / 100.0, -37.617 / 100.0)]
+    branches = [Branch(0, 1, "1", 1.0e-4, 5.76e-2, 0.0001)]
+    ps = PowerSystem(100.0, buses, gens, loads, branches, [])
+    psd = PowerSystemDynamics()
+    psd.add_device(Genrou(1, "1", 1.575, 1.512, 0.291, 0.39, 0.1733, 0.0787, 3.38, 0.0,
+                          6.1, 1.0, 0.05, 0.15, 0.0, 0.0))
+    events = [[ContingencyEvent(1, 0.2, 0.2, 0.3)]] if fault else [[]]
+    return _finish("2bus_testdata", ps, psd, 1, events, [tf], {})
+
+
+def case0_2bus(tf=10.0):
+    rng = np.random.default_rng(1000)
+    meta = {"seed": 1000, "ieesgo_K2": 0.5, "ieesgo_K3": 0.5, "ieesgo_pmin": 0.0}
+    buses = [Bus(1, "BUS1", SLACK, 100.0, 1.0, 0.0), Bus(2, "BUS2", PQ, 100.0, 1.0, 0.0)]
+    gens = [Gen(0, "1", 0.8, 0.2, 100.0)]
+    loads = [Load(1, "1", 0.8, -0.2)]
+    branches = [Branch(0, 1, "1", 0.01, 0.1, 0.0)]
+    ps = PowerSystem(100.0, buses, gens, loads, branches, [])
+    psd = PowerSystemDynamics()
+    psd.add_device(_genrou(1))
+    psd.add_device(_ieesgo(1, "1", rng, meta))
+    events = [[ContingencyEvent(1, 0.2, 0.2, 0.3)]]
+    return _finish("c0_2bus", ps, psd, 1, events, [tf], meta)
+
+
+def case1_9bus(M=64, tf=1.0):
+    rng = np.random.default_rng(1001)
+    meta = {"seed": 1001, "zip_alpha": 0.4,
+            "zip_note": "Z/I/P 0.4/0.3/0.3 mapped to alpha=0.4 constant-Z, 0.6 constant-P"}
+    nb = 9
+    buses = [Bus(i + 1, f"B{i+1}", PQ, 230.0, 1.0, 0.0) for i in range(nb)]
+    edges = [(i, (i + 1) % nb) for i in range(nb)]
+    have = {tuple(sorted(e)) for e in edges}
+    while len(edges) < nb + 3:
+        a, b = rng.choice(nb, 2, replace=False)
+        e = tuple(sorted((int(a), int(b))))
+        if e not in have:
+            have.add(e)
+            edges.append(e)
+    branches = []
+    for (a, b) in edges:
+        x = rng.uniform(0.05, 0.2)
+        r = x / rng.uniform(5.0, 15.0)
+        branches.append(Branch(a, b, "1", r, x, rng.uniform(0.0, 0.2)))
+    gen_buses = [0, 3, 6]
+    loads = []
+    for i in range(nb):
+        if i in gen_buses:
+            continue
+        P = rng.uniform(0.5, 1.5)
+        loads.append(Load(i, "1", P, -0.3 * P))
+    total = sum(l.pd for l in loads)
+    gens = []
+    for k, b in enumerate(gen_buses):
+        buses[b].type = SLACK if k == 0 else PV
+        buses[b].v0m = 1.02
+        gens.append(Gen(b, "1", total / 3.0, 0.0, 300.0))
+    ps = PowerSystem(100.0, buses, gens, loads, branches, [])
+    psd = PowerSystemDynamics()
+    for b in gen_buses:
+        psd.add_device(_genrou(b + 1))
+        psd.add_device(_sexs(b + 1))
+        psd.add_device(_tgov1(b + 1))
+    events = []
+    for m in range(M):
+        events.append([ContingencyEvent(m % nb, float(rng.uniform(0.01, 0.2)), 0.02,
+                                        float(rng.uniform(0.1, 0.2)))])
+    return _finish("c1_9bus", ps, psd, M, events, [tf] * M, meta, zip_alpha=0.4)
+
+
+def _rgg_edges(rng, n, mean_degree):
+    from scipy.spatial import cKDTree
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import minimum_spanning_tree
+    pts = rng.random((n, 2))
+    tree = cKDTree(pts)
+    k = 6
+    dd, ii = tree.query(pts, k + 1)
+    G = coo_matrix((dd[:, 1:].ravel(), (np.repeat(np.arange(n), k), ii[:, 1:].ravel())),
+                   shape=(n, n)).tocsr()
+    G = G.maximum(G.T)
+    T = minimum_spanning_tree(G).tocoo()
+    mst = {tuple(sorted((int(a), int(b)))) for a, b in zip(T.row, T.col)}
+    need = max(0, int(round(mean_degree * n / 2)) - len(mst))
+    r = math.sqrt(mean_degree / (n * math.pi))
+    pairs = tree.query_pairs(r, output_type="ndarray")
+    cand = [tuple(sorted((int(a), int(b)))) for a, b in pairs]
+    cand = [e for e in cand if e not in mst]
+    d = np.array([np.linalg.norm(pts[a] - pts[b]) for a, b in cand]) if cand else np.zeros(0)
+    order = np.argsort(d)
+    extra = [cand[i] for i in order[:need]]
+    return sorted(mst | set(extra)), pts
+
+
+def _spread_buses(rng, edges, n, k):
+    """Pick k buses spread over the graph: greedy farthest-first in hop
+    distance from a random start, so every load bus has generation a few hops
+    away (random placement leaves long generator-free corridors on sparse
+    random geometric graphs, where the power flow has no solution)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import shortest_path
+    e = np.array(edges)
+    A = coo_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(n, n)).tocsr()
+    chosen = [int(rng.integers(n))]
+    dist = shortest_path(A, directed=False, unweighted=True, indices=chosen[0])
+    while len(chosen) < k:
+        far = np.flatnonzero(dist == dist.max())
+        nxt = int(rng.choice(far))
+        chosen.append(nxt)
+        dist = np.minimum(dist, shortest_path(A, directed=False, unweighted=True, indices=nxt))
+    return np.array(chosen)
+
+
+def _regional_dispatch(edges, n, gen_buses, loads, floor=0.1):
+    """Scheduled generation that serves nearby load: each load is assigned to
+    the generator bus nearest in hops (multi-source BFS) and a generator's
+    output is its assigned load (at least `floor`), rescaled so the total
+    matches the total load.  Flows then stay regional, as in planned grids."""
+    from collections import deque
+    adj = [[] for _ in range(n)]
+    for a, b in edges:
+        adj[a].append(b)
+        adj[b].append(a)
+    owner = np.full(n, -1)
+    dq = deque()
+    for k, g in enumerate(gen_buses):
+        owner[g] = k
+        dq.append(g)
+    while dq:
+        u = dq.popleft()
+        for v in adj[u]:
+            if owner[v] < 0:
+                owner[v] = owner[u]
+                dq.append(v)
+    share = np.zeros(len(gen_buses))
+    for ld in loads:
+        share[owner[ld.bus]] += ld.pd
+    share = np.maximum(share, floor)
+    total = sum(ld.pd for ld in loads)
+    return share * (total / share.sum())
+
+
+def case2_grid(M=256, nbus=2000, n_genrou=400, n_pss=100, n_static=300, seed=1002,
+               load_fraction=0.2, tf_range=(1.0, 5.0), name="c2_grid", charging_scale=1.0):
+    rng = np.random.default_rng(seed)
+    meta = {"seed": seed, "zip_alpha": 0.4, "nbus": nbus,
+            "zip_note": "Z/I/P 0.4/0.3/0.3 mapped to alpha=0.4 constant-Z, 0.6 constant-P"}
+    edges, pts = _rgg_edges(rng, nbus, 2.6)
+    meta["mean_degree"] = 2 * len(edges) / nbus
+    meta["charging_scale"] = charging_scale
+    buses = [Bus(i + 1, f"B{i+1}", PQ, 230.0, 1.0, 0.0) for i in range(nbus)]
+    branches = []
+    for (a, b) in edges:
+        x = rng.uniform(0.05, 0.2)
+        r = x / rng.uniform(5.0, 15.0)
+        branches.append(Branch(a, b, "1", r, x, charging_scale * rng.uniform(0.0, 0.2)))
+    spread = _spread_buses(rng, edges, nbus, n_genrou + n_static)
+    spread = spread[rng.permutation(len(spread))]
+    gbus = spread[:n_genrou]
+    sbus = spread[n_genrou:]
+    rest = np.setdiff1d(np.arange(nbus), spread)
+    lbus = rest[rng.random(len(rest)) < load_fraction / max(1e-9, len(rest) / nbus)]
+    loads = []
+    for b in sorted(int(x) for x in lbus):
+        P = rng.uniform(0.5, 1.5)
+        loads.append(Load(b, "1", P, -0.3 * P))
+    allg = [int(b) for b in gbus] + [int(b) for b in sbus]
+    pg = _regional_dispatch(edges, nbus, allg, loads)
+    gens = []
+    for k, b in enumerate(allg):
+        buses[b].type = SLACK if k == 0 else PV
+        buses[b].v0m = 1.02 if k < n_genrou else 1.01
+        gens.append(Gen(b, "1", float(pg[k]), 0.0, max(100.0, 100.0 * pg[k] / 0.7)))
+    ps = PowerSystem(100.0, buses, gens, loads, branches, [])
+    psd = PowerSystemDynamics()
+    exc_kind = rng.random(n_genrou) < 0.5
+    gov_kind = rng.random(n_genrou) < 0.5
+    pss_on = np.zeros(n_genrou, bool)
+    pss_on[rng.choice(n_genrou, n_pss, replace=False)] = True
+    H = rng.uniform(2.0, 8.0, n_genrou)
+    D = rng.uniform(0.0, 2.0, n_genrou)
+    for k, b in enumerate(gbus):
+        eb = int(b) + 1
+        psd.add_device(_genrou(eb, H=float(H[k]), D=float(D[k])))
+        psd.add_device(_ieesgo(eb, "1", rng, meta) if gov_kind[k] else _tgov1(eb))
+        psd.add_device(_sexs(eb) if exc_kind[k] else _esdc1a(eb))
+        if pss_on[k]:
+            psd.add_device(_ieeest(eb))
+    meta.pop("ieesgo_T", None)
+    events, tf = [], []
+    for m in range(M):
+        events.append([ContingencyEvent(int(rng.integers(nbus)), float(rng.uniform(0.01, 0.2)),
+                                        0.02, float(rng.uniform(0.05, 0.25)))])
+        tf.append(float(rng.uniform(*tf_range)))
+    return _finish(name, ps, psd, M, events, tf, meta, zip_alpha=0.4)
+
+
+def case3_200bus(M=128, tf=3.0):
+    rng = np.random.default_rng(1003)
+    meta = {"seed": 1003, "zip_alpha": 0.4}
+    nbus, ng = 200, 40
+    edges, _ = _rgg_edges(rng, nbus, 2.6)
+    buses = [Bus(i + 1, f"B{i+1}", PQ, 230.0, 1.0, 0.0) for i in range(nbus)]
+    branches = []
+    for (a, b) in edges:
+        x = rng.uniform(0.05, 0.2)
+        branches.append(Branch(a, b, "1", x / rng.uniform(5.0, 15.0), x, rng.uniform(0.0, 0.2)))
+    gbus = _spread_buses(rng, edges, nbus, ng)
+    loads = []
+    for b in sorted(int(x) for x in np.setdiff1d(np.arange(nbus), gbus) if rng.random() < 0.25):
+        P = rng.uniform(0.5, 1.5)
+        loads.append(Load(b, "1", P, -0.3 * P))
+    pg = _regional_dispatch(edges, nbus, [int(b) for b in gbus], loads)
+    gens = []
+    for k, b in enumerate(gbus):
+        b = int(b)
+        buses[b].type = SLACK if k == 0 else PV
+        buses[b].v0m = 1.02
+        gens.append(Gen(b, "1", float(pg[k]), 0.0, max(100.0, 100.0 * pg[k] / 0.7)))
+    ps = PowerSystem(100.0, buses, gens, loads, branches, [])
+    psd = PowerSystemDynamics()
+    for b in gbus:
+        eb = int(b) + 1
+        psd.add_device(_genrou(eb, H=float(rng.uniform(2, 8)), D=float(rng.uniform(0, 2))))
+        psd.add_device(_sexs(eb))
+        psd.add_device(_tgov1(eb))
+    events = [[ContingencyEvent(int(rng.integers(nbus)), float(rng.uniform(0.01, 0.2)), 0.02,
+                                float(rng.uniform(0.05, 0.25)))] for _ in range(M)]
+    return _finish("c3_200bus", ps, psd, M, events, [tf] * M, meta, zip_alpha=0.4)

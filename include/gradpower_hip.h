@@ -1,0 +1,195 @@
+/*
+ * gradpower_hip.h -- C ABI of the MI355X batched backward-Euler DAE integrator.
+ *
+ * Replaces the GPU entry points of GradPower.jl's CUDA extension
+ * (ext/GradPowerCUDAExt/GradPowerCUDAExt.jl):
+ *   GpuBatchedLayout(dp, ps, M)                 ext:466          -> gp_create + gp_set_state
+ *   integrate_gpu_cudss!(gbl, ps, tf; dt, tol)  ext:2924-2996    -> gp_integrate(GP_METHOD_MONO_FP64)
+ *   integrate_gpu_schur_cudss!(gbl, ps, tf; ..) ext:3008-3080    -> gp_integrate(GP_METHOD_SCHUR_*)
+ *   integrate_gpu_shared_multi!(..., fault_buses, rfaults, ton, toff)
+ *                                               ext:2640-2708    -> gp_set_events (per-scenario faults)
+ *   trajs[m] = z_hist[:, m, :]                  ext:3075-3079    -> gp_get_trajectory
+ *   _residual_all_scenarios_gpu! / _jacobian_all_scenarios_gpu!
+ *                                               ext:1318-1516    -> gp_eval_residual / gp_eval_jacobian
+ *
+ * Conventions
+ *   - All indices are 0-based (the Julia wrapper subtracts 1).
+ *   - The system descriptor holds borrowed host pointers, read only during
+ *     gp_create; the library copies what it needs.
+ *   - Every call returns GP_OK (0) or a negative gp_status; gp_last_error()
+ *     returns a thread-local message for the last failure on this thread.
+ *     No C++ exception crosses the ABI.
+ *   - A context owns its device memory and one HIP stream on the device that
+ *     was current at gp_create.  Calls on one context must be serialised by
+ *     the caller; distinct contexts may be used concurrently.
+ */
+#ifndef GRADPOWER_HIP_H
+#define GRADPOWER_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+enum gp_status {
+  GP_OK = 0,
+  GP_ERR_BADARG = -1,
+  GP_ERR_HIP = -2,
+  GP_ERR_OOM = -3,
+  GP_ERR_SINGULAR_PIVOT = -4,
+  GP_ERR_NEWTON_NOT_CONVERGED = -5,
+  GP_ERR_NAN = -6,
+  GP_ERR_UNSUPPORTED = -7
+};
+
+/* ---- device tables ------------------------------------------------------- */
+/* Each table is a row-major int32 array [n_dev[t] x GP_COLS_<t>] of absolute
+ * 0-based z indices (-1 = not wired), parameter offsets into p and 0-based bus
+ * indices.  They correspond to the reference SoA tables after reorder_state!
+ * (src/layout.jl:61-179, src/clusters.jl:339-466). */
+enum gp_devtype {
+  GP_GENROU = 0,   /* diff_ptr, alg_ptr, par_ptr, bus, pm_idx, efd_idx        */
+  GP_IEESGO = 1,   /* diff_ptr, alg_ptr, par_ptr, w_idx                        */
+  GP_TGOV1 = 2,    /* diff_ptr, alg_ptr, par_ptr, w_idx                        */
+  GP_SEXS = 3,     /* diff_ptr, par_ptr, vr_idx, vs_idx                        */
+  GP_ESDC1A = 4,   /* diff_ptr, par_ptr, vr_idx, vs_idx                        */
+  GP_IEEEST = 5,   /* diff_ptr, alg_ptr, par_ptr, w_idx                        */
+  GP_ZIPLOAD = 6,  /* bus, par_ptr                                             */
+  GP_STATICGEN = 7,/* bus, alg_ptr (-1 for PQ), par_ptr, bus_type (1 PQ,2 PV,3 slack) */
+  GP_NDEVTYPES = 8
+};
+#define GP_COLS_GENROU 6
+#define GP_COLS_IEESGO 4
+#define GP_COLS_TGOV1 4
+#define GP_COLS_SEXS 4
+#define GP_COLS_ESDC1A 4
+#define GP_COLS_IEEEST 4
+#define GP_COLS_ZIPLOAD 2
+#define GP_COLS_STATICGEN 4
+
+/* Cluster table row (src/clusters.jl:16-28): a generator with its controllers,
+ * occupying z[w_start .. w_start+w_size).  Controller/generator fields are
+ * row indices into the device tables (-1 = absent). */
+enum gp_cluster_col {
+  GP_CL_WSTART = 0, GP_CL_WSIZE, GP_CL_BUS, GP_CL_TRIVIAL, GP_CL_GENROU,
+  GP_CL_GOVTYPE,   /* 0 none, GP_IEESGO or GP_TGOV1 */
+  GP_CL_GOV, GP_CL_EXCTYPE /* 0 none, GP_SEXS or GP_ESDC1A */, GP_CL_EXC,
+  GP_CL_PSS, GP_CL_STATICGEN, GP_CLUSTER_COLS
+};
+
+typedef struct gp_system_desc {
+  int32_t n;          /* n_sys = n_diff + n_alg + 2*nbus                      */
+  int32_t nbus;
+  int32_t net_ptr;    /* z index of vr of bus 0; vr_b = net_ptr+2b, vi_b = +1  */
+  int32_t n_par;
+  const uint8_t *is_diff;      /* [n] 1 = differential state                   */
+  /* realified network admittance (2nbus x 2nbus), CSR (src/network.jl:87)   */
+  int32_t y_nnz;
+  const int32_t *y_rowptr;     /* [2*nbus+1] */
+  const int32_t *y_colind;     /* [y_nnz]    */
+  const double *y_val;         /* [y_nnz]    */
+  int32_t n_dev[GP_NDEVTYPES];
+  const int32_t *dev[GP_NDEVTYPES];
+  const double *genrou_u0;     /* [n_genrou*2] e_fd, p_m used when unwired   */
+  int32_t n_clusters;
+  const int32_t *clusters;     /* [n_clusters*GP_CLUSTER_COLS]              */
+} gp_system_desc;
+
+/* One bus fault of one scenario (src/GradPower.jl:117-128, ext:2640-2650):
+ * conductance 1/r_fault on the bus's vr/vi rows from the end of step
+ * step_on until the end of step step_off (steps counted from 1; both >= 1). */
+typedef struct gp_event {
+  int32_t scenario;
+  int32_t bus;
+  double r_fault;
+  int32_t step_on;
+  int32_t step_off;
+} gp_event;
+
+enum gp_method {
+  GP_METHOD_MONO_FP64 = 0,    /* monolithic Newton, sparse LU of J refactored every iteration */
+  GP_METHOD_SCHUR_FP64 = 1,   /* cluster Schur reduction, FP64 factors                        */
+  GP_METHOD_SCHUR_FP32 = 2    /* cluster Schur reduction, FP32 block/network factors,
+                                 FP64 residual, update and states (inexact Newton)           */
+};
+
+typedef struct gp_options {
+  int32_t record_all;      /* 1: record every state each step; 0: only record_idx  */
+  int32_t n_record;        /* number of recorded z indices when record_all == 0     */
+  const int32_t *record_idx;
+  int32_t reserved[8];
+} gp_options;
+
+typedef struct gp_stats {
+  int64_t newton_iters;        /* total Newton iterations summed over scenarios   */
+  int64_t scenario_steps;      /* sum over scenarios of integrated steps          */
+  int64_t event_resolves;      /* dt = 0 re-solves                                 */
+  int64_t refactorizations;    /* numeric factorizations of S or J (x scenarios)  */
+  int32_t n_not_converged;     /* scenarios with at least one failed Newton solve */
+  int32_t n_nan;               /* scenarios that produced a non-finite residual   */
+  double max_final_residual;   /* max over scenarios/steps of the accepted |F|inf */
+  double elapsed_ms;           /* device time of the integration (HIP events)     */
+  double solve_ms;             /* device time inside the integrator's Newton loop */
+} gp_stats;
+
+typedef struct gp_ctx gp_ctx;
+
+/* Build a context for M scenarios: copies tables, builds the Jacobian / Schur
+ * patterns, the symbolic LU analyses (ordering, pivot order, level schedule)
+ * and allocates device memory. */
+int gp_create(const gp_system_desc *desc, int32_t M, const gp_options *opt,
+              gp_ctx **out);
+void gp_destroy(gp_ctx *ctx);
+
+/* z0: [n] (shared) or [M*n] scenario-major rows when per_scenario != 0;
+ * p: [n_par], shared by all scenarios. */
+int gp_set_state(gp_ctx *ctx, const double *z0, int32_t per_scenario,
+                 const double *p);
+
+/* Per-scenario faults and horizons (nsteps[m] = round(tf_m/dt)). */
+int gp_set_events(gp_ctx *ctx, const gp_event *ev, int32_t n_ev,
+                  const int32_t *nsteps);
+
+/* Integrate all scenarios from the current state to their horizons.
+ * tol: Newton tolerance on |F|inf per scenario; itermax per Newton solve. */
+int gp_integrate(gp_ctx *ctx, int32_t method, double dt, double tol,
+                 int32_t itermax, gp_stats *stats);
+
+/* Recorded trajectory of scenario m: out[j*ld + k] for recorded channel j and
+ * step k = 0..nsteps[m]; ld >= nsteps[m]+1. */
+int gp_get_trajectory(gp_ctx *ctx, int32_t m, double *out, int64_t ld);
+
+/* Final state of every scenario, out[m*n + j]. */
+int gp_get_state(gp_ctx *ctx, double *out);
+
+/* Residual F(z) of the backward-Euler system at z (scenario-major [M*n]) with
+ * zold and step dt, with scenario m's faults active when fault_on[m] != 0.
+ * Output f is [M*n]. */
+int gp_eval_residual(gp_ctx *ctx, const double *z, const double *zold,
+                     double dt, const uint8_t *fault_on, double *f);
+
+/* Jacobian values of the backward-Euler system at z, in the library's CSR
+ * pattern (gp_jacobian_pattern), out [M*nnz]. */
+int gp_jacobian_pattern(gp_ctx *ctx, int32_t *nnz, int32_t *rowptr,
+                        int32_t *colind);
+int gp_eval_jacobian(gp_ctx *ctx, const double *z, double dt,
+                     const uint8_t *fault_on, double *jval);
+
+/* Sizes of the symbolic analyses (for the roofline byte counts). */
+typedef struct gp_symbolic_info {
+  int32_t n, nnz_j, nnz_lu_j, levels_j;
+  int32_t n_red, nnz_s, nnz_lu_s, levels_s;
+  int32_t n_clusters_nt, w_total_nt;
+  int32_t reserved[6];
+} gp_symbolic_info;
+int gp_get_symbolic_info(gp_ctx *ctx, gp_symbolic_info *info);
+
+const char *gp_last_error(void);
+const char *gp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRADPOWER_HIP_H */
