@@ -1,0 +1,1099 @@
+// gp_host.cpp -- C-ABI runtime of the batched backward-Euler DAE integrator.
+//
+// Owns the context: device copies of the system tables, the Jacobian and
+// Schur-complement patterns with their slot/position tables, the symbolic
+// LU analyses, and the host time loop that drives the kernels
+// (integrate!/integrate_gpu_schur_cudss! semantics: src/dynamics.jl:502-655,
+// ext:3008-3080, with per-scenario faults and horizons).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/gradpower_hip.h"
+#include "gp_internal.hpp"
+#include "gp_splu.hpp"
+
+using namespace gp;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct GpError {
+  int code;
+  std::string msg;
+};
+
+#define HIPCHK(x)                                                                                 \
+  do {                                                                                            \
+    hipError_t e_ = (x);                                                                          \
+    if (e_ != hipSuccess)                                                                         \
+      throw GpError{GP_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)};                  \
+  } while (0)
+
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw GpError{GP_ERR_OOM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
+    }
+  }
+  void upload(const std::vector<T> &v) {
+    alloc(v.size() ? v.size() : 1);
+    if (v.size()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+};
+
+struct LUPlan {
+  SpluAnalysis A;
+  SpluDev dev{};
+  DBuf<int> a2lu, a_src, lu_cp, lu_dp, lu_ri, op_cp, op, fac_lp, fac_cols, lrow_p, lrow_k, lrow_pos, urow_p, urow_j,
+      urow_pos, fw_lp, fw_rows, bw_lp, bw_rows, q, pinv;
+  bool ready = false;
+  double dt = -1.0;
+
+  void upload(const std::vector<int> &asrc) {
+    a2lu.upload(A.a2lu); a_src.upload(asrc); lu_cp.upload(A.lu_cp); lu_dp.upload(A.lu_dp); lu_ri.upload(A.lu_ri);
+    op_cp.upload(A.op_cp); op.upload(A.op.empty() ? std::vector<int>{0, 0, 0} : A.op);
+    fac_lp.upload(A.fac_lp); fac_cols.upload(A.fac_cols);
+    lrow_p.upload(A.lrow_p); lrow_k.upload(A.lrow_k); lrow_pos.upload(A.lrow_pos);
+    urow_p.upload(A.urow_p); urow_j.upload(A.urow_j); urow_pos.upload(A.urow_pos);
+    fw_lp.upload(A.fw_lp); fw_rows.upload(A.fw_rows); bw_lp.upload(A.bw_lp); bw_rows.upload(A.bw_rows);
+    q.upload(A.q); pinv.upload(A.pinv);
+    SpluDev &d = dev;
+    d.n = A.n; d.nnz_a = A.nnz_a; d.nnz_lu = A.nnz_lu; d.max_col = A.max_col;
+    d.n_flev = (int)A.fac_lp.size() - 1; d.n_fwlev = (int)A.fw_lp.size() - 1; d.n_bwlev = (int)A.bw_lp.size() - 1;
+    d.a2lu = a2lu.p; d.a_src = a_src.p; d.lu_cp = lu_cp.p; d.lu_dp = lu_dp.p; d.lu_ri = lu_ri.p;
+    d.op_cp = op_cp.p; d.op = op.p; d.fac_lp = fac_lp.p; d.fac_cols = fac_cols.p;
+    d.lrow_p = lrow_p.p; d.lrow_k = lrow_k.p; d.lrow_pos = lrow_pos.p;
+    d.urow_p = urow_p.p; d.urow_j = urow_j.p; d.urow_pos = urow_pos.p;
+    d.fw_lp = fw_lp.p; d.fw_rows = fw_rows.p; d.bw_lp = bw_lp.p; d.bw_rows = bw_rows.p;
+    d.q = q.p; d.pinv = pinv.p;
+  }
+  void shape(size_t elem) {
+    int cols = 1;
+    while (cols < A.max_col && cols < 64) cols *= 2;
+    int threads = (int)(65536 / (cols * elem));
+    threads = std::min(256, std::max(64, threads / 64 * 64));
+    dev.threads = threads;
+    dev.lds_cols = cols;
+  }
+};
+
+}  // namespace
+
+struct gp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n = 0, nbus = 0, net_ptr = 0, npar = 0, M = 0, ldM = 0;
+  gp_options opt{};
+  // host copies
+  std::vector<uint8_t> is_diff;
+  std::vector<int> yrp, yci;
+  std::vector<double> yv;
+  std::vector<int> dev[GP_NDEVTYPES];
+  std::vector<double> u0;
+  std::vector<int> cl;
+  int ncl = 0;
+  std::vector<int> cl_nt, cl_xoff;
+  int xrows = 0;
+  // monolithic Jacobian pattern (CSC) and per-device positions
+  std::vector<int> jcp, jri;
+  std::vector<int> jp[6], jp_bus, jp_yoff_p, jp_yoff_pos;
+  std::vector<double> jp_yoff_val, ydiag;
+  // reduced (Schur) system
+  int nred = 0;
+  std::vector<int> red_of_z, z_of_red, scp, sri, s_src, sv_bus;
+  std::vector<double> s_cst;
+  int n_svar = 0;
+  // per-bus attachments
+  std::vector<int> bus_cl_p, bus_cl, bus_zip_p, bus_zip, bus_sg;
+  // events / horizons
+  std::vector<gp_event> events;
+  std::vector<int> nsteps;
+  std::vector<int> rec_idx;
+  int nrec = 0;
+  // device memory
+  DBuf<uint8_t> d_is_diff;
+  DBuf<int> d_dev[GP_NDEVTYPES], d_cl, d_yrp, d_yci, d_bus_cl_p, d_bus_cl, d_bus_zip_p, d_bus_zip, d_bus_sg;
+  DBuf<int> d_jp[6], d_jp_bus, d_jp_yoff_p, d_jp_yoff_pos, d_cl_nt, d_cl_xoff, d_red_of_z, d_z_of_red, d_sv_bus;
+  DBuf<int> d_rec_idx;
+  DBuf<double> d_yv, d_u0, d_jp_yoff_val, d_ydiag, d_s_cst;
+  DBuf<double> d_p, d_z, d_zold, d_f, d_inj, d_jv, d_svar, d_rhs, d_dv, d_Dg, d_fnorm, d_hist, d_luJ, d_luS64;
+  DBuf<float> d_luS32, d_X32;
+  DBuf<double> d_X64;
+  DBuf<uint8_t> d_work, d_act, d_refac, d_sel;
+  DBuf<int> d_fbus, d_nunconv, d_status;
+  DBuf<double> d_fg;
+  DBuf<unsigned long long> d_nb;
+  DevTables T{};
+  LUPlan luJ, luS;
+  bool state_set = false;
+  int hist_steps = 0;
+  std::vector<int> traj_len;
+};
+
+namespace {
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+// ---------------------------------------------------------------- patterns
+// Per-device Jacobian slot lists.  Each slot is (row, col) in z indices; the
+// order matches the value order of the kernels (gp_kernels.hip) and covers
+// the reference pattern of src/dynamics.jl:687-794 plus the backward-Euler
+// diagonal of every differential state.
+void genrou_slots(const int *g, int npt, std::vector<std::pair<int, int>> &s) {
+  int dp = g[0], ap = g[1], bus = g[3], pm = g[4], efd = g[5];
+  int eq = dp, ed = dp + 1, p1 = dp + 2, p2 = dp + 3, w = dp + 4, de = dp + 5;
+  int vq = ap, vd = ap + 1, iq = ap + 2, id = ap + 3, vr = npt + 2 * bus, vi = vr + 1;
+  s = {{dp, eq}, {dp, p1}, {dp, id}, {dp, ed}, {dp, p2}, {dp, efd},
+       {dp + 1, ed}, {dp + 1, p2}, {dp + 1, iq},
+       {dp + 2, eq}, {dp + 2, p1}, {dp + 2, id},
+       {dp + 3, ed}, {dp + 3, p2}, {dp + 3, iq},
+       {dp + 4, eq}, {dp + 4, ed}, {dp + 4, p1}, {dp + 4, p2}, {dp + 4, w}, {dp + 4, iq}, {dp + 4, id}, {dp + 4, pm},
+       {dp + 5, w}, {dp + 5, de},
+       {ap, eq}, {ap, p1}, {ap, vq}, {ap, id},
+       {ap + 1, ed}, {ap + 1, p2}, {ap + 1, vd}, {ap + 1, iq},
+       {ap + 2, de}, {ap + 2, vd}, {ap + 2, vr}, {ap + 2, vi},
+       {ap + 3, de}, {ap + 3, vq}, {ap + 3, vr}, {ap + 3, vi},
+       {vr, de}, {vr, iq}, {vr, id}, {vi, de}, {vi, iq}, {vi, id}};
+}
+void ieesgo_slots(const int *d, std::vector<std::pair<int, int>> &s) {
+  int dp = d[0], ap = d[1], w = d[3];
+  s = {{dp, dp}, {dp, w}, {dp + 1, dp}, {dp + 1, dp + 1}, {dp + 2, dp}, {dp + 2, dp + 1}, {dp + 2, dp + 2},
+       {dp + 3, dp + 2}, {dp + 3, dp + 3}, {dp + 4, dp + 3}, {dp + 4, dp + 4},
+       {ap, dp + 2}, {ap, dp + 3}, {ap, dp + 4}, {ap, ap}};
+}
+void tgov1_slots(const int *d, std::vector<std::pair<int, int>> &s) {
+  int dp = d[0], ap = d[1], w = d[3];
+  s = {{dp, dp}, {dp, dp + 1}, {dp + 1, dp + 1}, {dp + 1, w}, {ap, dp}, {ap, dp + 1}, {ap, w}, {ap, ap}};
+}
+void sexs_slots(const int *d, std::vector<std::pair<int, int>> &s) {
+  int dp = d[0], vr = d[2], vi = vr + 1, vs = d[3];
+  s = {{dp, dp}, {dp, vr}, {dp, vi}, {dp, vs}, {dp + 1, dp}, {dp + 1, dp + 1}, {dp + 1, vr}, {dp + 1, vi}, {dp + 1, vs}};
+}
+void esdc1a_slots(const int *d, std::vector<std::pair<int, int>> &s) {
+  int dp = d[0], vr = d[2], vi = vr + 1, vs = d[3];
+  s = {{dp, dp}, {dp, dp + 1}, {dp, dp + 2}, {dp, vr}, {dp, vi}, {dp, vs},
+       {dp + 1, dp + 1}, {dp + 1, dp + 2}, {dp + 2, dp}, {dp + 2, dp + 2}};
+}
+void ieeest_slots(const int *d, std::vector<std::pair<int, int>> &s) {
+  int dp = d[0], ap = d[1], w = d[3];
+  s = {{dp, dp}, {dp, dp + 1}, {dp, w}, {dp + 1, dp}, {dp + 2, dp + 1}, {dp + 2, dp + 2}, {dp + 2, dp + 3},
+       {dp + 3, dp + 2}};
+  for (int j = 1; j <= 4; j++) s.push_back({dp + 4, dp + j});
+  for (int j = 1; j <= 5; j++) s.push_back({dp + 5, dp + j});
+  for (int j = 1; j <= 6; j++) s.push_back({dp + 6, dp + j});
+  for (int j = 1; j <= 6; j++) s.push_back({ap, dp + j});
+  s.push_back({ap, ap});
+  s.push_back({dp + 1, dp + 1});
+  s.push_back({dp + 3, dp + 3});
+}
+// static generator slots: vr_ap, vi_ap, vr_ap1, vi_ap1, ap_vr, ap_vi, ap1_vi
+void sg_slots(const int *d, int npt, std::vector<std::pair<int, int>> &s) {
+  int bus = d[0], ap = d[1], bt = d[3], vr = npt + 2 * bus, vi = vr + 1;
+  s.assign(7, {-1, -1});
+  if (bt == 2) { s[0] = {vr, ap}; s[1] = {vi, ap}; s[4] = {ap, vr}; s[5] = {ap, vi}; }
+  if (bt == 3) { s[0] = {vr, ap}; s[1] = {vi, ap}; s[2] = {vr, ap + 1}; s[3] = {vi, ap + 1}; s[4] = {ap, vr}; s[6] = {ap + 1, vi}; }
+}
+
+struct Pattern {
+  int n;
+  std::vector<std::vector<int>> cols;  // rows per column
+  explicit Pattern(int n_) : n(n_), cols(n_) {}
+  void add(int r, int c) {
+    if (r >= 0 && c >= 0) cols[c].push_back(r);
+  }
+  void finish(std::vector<int> &cp, std::vector<int> &ri) {
+    cp.assign(n + 1, 0);
+    ri.clear();
+    for (int c = 0; c < n; c++) {
+      auto &v = cols[c];
+      std::sort(v.begin(), v.end());
+      v.erase(std::unique(v.begin(), v.end()), v.end());
+      cp[c + 1] = cp[c] + (int)v.size();
+      ri.insert(ri.end(), v.begin(), v.end());
+    }
+  }
+};
+
+int find_pos(const std::vector<int> &cp, const std::vector<int> &ri, int r, int c) {
+  if (r < 0 || c < 0) return -1;
+  auto b = ri.begin() + cp[c], e = ri.begin() + cp[c + 1];
+  auto it = std::lower_bound(b, e, r);
+  if (it == e || *it != r) return -2;
+  return (int)(it - ri.begin());
+}
+
+void build_structures(gp_ctx *C) {
+  const int npt = C->net_ptr, n = C->n, nbus = C->nbus;
+  // ---- monolithic pattern
+  Pattern P(n);
+  for (int i = 0; i < npt; i++)
+    if (C->is_diff[i]) P.add(i, i);
+  for (int b = 0; b < nbus; b++)
+    for (int u = 0; u < 2; u++)
+      for (int v = 0; v < 2; v++) P.add(npt + 2 * b + u, npt + 2 * b + v);
+  for (int r = 0; r < 2 * nbus; r++)
+    for (int q = C->yrp[r]; q < C->yrp[r + 1]; q++) P.add(npt + r, npt + C->yci[q]);
+  std::vector<std::pair<int, int>> sl;
+  static const int ncols[6] = {GP_COLS_GENROU, GP_COLS_IEESGO, GP_COLS_TGOV1, GP_COLS_SEXS, GP_COLS_ESDC1A, GP_COLS_IEEEST};
+  auto slots_of = [&](int t, const int *d) {
+    switch (t) {
+      case GP_GENROU: genrou_slots(d, npt, sl); break;
+      case GP_IEESGO: ieesgo_slots(d, sl); break;
+      case GP_TGOV1: tgov1_slots(d, sl); break;
+      case GP_SEXS: sexs_slots(d, sl); break;
+      case GP_ESDC1A: esdc1a_slots(d, sl); break;
+      default: ieeest_slots(d, sl); break;
+    }
+  };
+  for (int t = 0; t < 6; t++)
+    for (size_t k = 0; k < C->dev[t].size() / ncols[t]; k++) {
+      slots_of(t, C->dev[t].data() + k * ncols[t]);
+      for (auto &e : sl) P.add(e.first, e.second);
+    }
+  const auto &SG = C->dev[GP_STATICGEN];
+  const int nsg = (int)SG.size() / GP_COLS_STATICGEN;
+  for (int k = 0; k < nsg; k++) {
+    sg_slots(SG.data() + k * GP_COLS_STATICGEN, npt, sl);
+    for (auto &e : sl) P.add(e.first, e.second);
+  }
+  P.finish(C->jcp, C->jri);
+  static const int nslot[6] = {NS_GENROU, NS_IEESGO, NS_TGOV1, NS_SEXS, NS_ESDC1A, NS_IEEEST};
+  for (int t = 0; t < 6; t++) {
+    int nd = (int)C->dev[t].size() / ncols[t];
+    C->jp[t].assign((size_t)std::max(1, nd) * nslot[t], -1);
+    for (int k = 0; k < nd; k++) {
+      slots_of(t, C->dev[t].data() + k * ncols[t]);
+      if ((int)sl.size() != nslot[t]) throw GpError{GP_ERR_BADARG, "internal slot count mismatch"};
+      for (int s = 0; s < nslot[t]; s++) {
+        int pos = find_pos(C->jcp, C->jri, sl[s].first, sl[s].second);
+        if (pos == -2) throw GpError{GP_ERR_BADARG, "internal: Jacobian slot missing"};
+        C->jp[t][(size_t)k * nslot[t] + s] = pos;
+      }
+    }
+  }
+  // ---- per-bus attachments
+  C->bus_sg.assign(nbus, -1);
+  for (int k = 0; k < nsg; k++) {
+    int b = SG[k * GP_COLS_STATICGEN];
+    if (b < 0 || b >= nbus) throw GpError{GP_ERR_BADARG, "static generator bus out of range"};
+    if (C->bus_sg[b] >= 0) throw GpError{GP_ERR_UNSUPPORTED, "more than one static generator on a bus"};
+    C->bus_sg[b] = k;
+  }
+  const auto &ZP = C->dev[GP_ZIPLOAD];
+  const int nzip = (int)ZP.size() / GP_COLS_ZIPLOAD;
+  std::vector<std::vector<int>> bz(nbus), bc(nbus);
+  for (int k = 0; k < nzip; k++) {
+    int b = ZP[k * GP_COLS_ZIPLOAD];
+    if (b < 0 || b >= nbus) throw GpError{GP_ERR_BADARG, "ZIP load bus out of range"};
+    bz[b].push_back(k);
+  }
+  // machines at a bus in GENROU-table order (the reference's summation order)
+  std::vector<std::pair<int, int>> gen_cl;
+  for (int c = 0; c < C->ncl; c++) {
+    const int *r = C->cl.data() + c * GP_CLUSTER_COLS;
+    if (!r[GP_CL_TRIVIAL] && r[GP_CL_GENROU] >= 0) gen_cl.push_back({r[GP_CL_GENROU], c});
+  }
+  std::sort(gen_cl.begin(), gen_cl.end());
+  for (auto &gc : gen_cl) bc[C->cl[gc.second * GP_CLUSTER_COLS + GP_CL_BUS]].push_back(gc.second);
+  C->bus_cl_p.assign(nbus + 1, 0);
+  C->bus_zip_p.assign(nbus + 1, 0);
+  C->bus_cl.clear();
+  C->bus_zip.clear();
+  for (int b = 0; b < nbus; b++) {
+    C->bus_cl.insert(C->bus_cl.end(), bc[b].begin(), bc[b].end());
+    C->bus_zip.insert(C->bus_zip.end(), bz[b].begin(), bz[b].end());
+    C->bus_cl_p[b + 1] = (int)C->bus_cl.size();
+    C->bus_zip_p[b + 1] = (int)C->bus_zip.size();
+  }
+  // ---- bus rows of the monolithic Jacobian
+  C->jp_bus.assign((size_t)15 * nbus, -1);
+  C->ydiag.assign((size_t)4 * nbus, 0.0);
+  C->jp_yoff_p.assign(nbus + 1, 0);
+  C->jp_yoff_pos.clear();
+  C->jp_yoff_val.clear();
+  for (int b = 0; b < nbus; b++) {
+    int vr = npt + 2 * b;
+    int dpos[4] = {find_pos(C->jcp, C->jri, vr, vr), find_pos(C->jcp, C->jri, vr, vr + 1),
+                   find_pos(C->jcp, C->jri, vr + 1, vr), find_pos(C->jcp, C->jri, vr + 1, vr + 1)};
+    for (int i = 0; i < 4; i++) C->jp_bus[15 * b + i] = dpos[i];
+    if (C->bus_sg[b] >= 0) {
+      sg_slots(SG.data() + C->bus_sg[b] * GP_COLS_STATICGEN, npt, sl);
+      for (int i = 0; i < 7; i++) C->jp_bus[15 * b + 4 + i] = find_pos(C->jcp, C->jri, sl[i].first, sl[i].second);
+    }
+    for (int rr = 0; rr < 2; rr++)
+      for (int q = C->yrp[2 * b + rr]; q < C->yrp[2 * b + rr + 1]; q++) {
+        int c = C->yci[q];
+        if (c / 2 == b) C->ydiag[4 * b + 2 * rr + (c & 1)] += C->yv[q];
+        else {
+          C->jp_yoff_pos.push_back(find_pos(C->jcp, C->jri, vr + rr, npt + c));
+          C->jp_yoff_val.push_back(C->yv[q]);
+        }
+      }
+    C->jp_yoff_p[b + 1] = (int)C->jp_yoff_pos.size();
+  }
+  // ---- clusters: non-trivial list and X offsets
+  C->cl_nt.clear();
+  C->cl_xoff.assign(std::max(1, C->ncl), 0);
+  C->xrows = 0;
+  for (int c = 0; c < C->ncl; c++) {
+    const int *r = C->cl.data() + c * GP_CLUSTER_COLS;
+    if (r[GP_CL_TRIVIAL] || r[GP_CL_WSIZE] == 0) continue;
+    if (r[GP_CL_GENROU] < 0) throw GpError{GP_ERR_UNSUPPORTED, "non-trivial cluster without a GENROU"};
+    C->cl_nt.push_back(c);
+    C->cl_xoff[c] = C->xrows;
+    C->xrows += r[GP_CL_WSIZE];
+  }
+  // ---- reduced system: trivial-cluster states + bus voltages (src/schur.jl:53-168)
+  C->red_of_z.assign(n, -1);
+  C->z_of_red.clear();
+  for (int c = 0; c < C->ncl; c++) {
+    const int *r = C->cl.data() + c * GP_CLUSTER_COLS;
+    if (r[GP_CL_TRIVIAL] || r[GP_CL_WSIZE] == 0)
+      for (int j = 0; j < r[GP_CL_WSIZE]; j++) C->red_of_z[r[GP_CL_WSTART] + j] = 1;
+  }
+  for (int i = npt; i < n; i++) C->red_of_z[i] = 1;
+  for (int i = 0; i < n; i++)
+    if (C->red_of_z[i] > 0) {
+      C->red_of_z[i] = (int)C->z_of_red.size();
+      C->z_of_red.push_back(i);
+    }
+  C->nred = (int)C->z_of_red.size();
+  // S pattern: J restricted to the reduced set + every bus diagonal block
+  Pattern SP(C->nred);
+  for (int c = 0; c < n; c++) {
+    int rc = C->red_of_z[c];
+    if (rc < 0) continue;
+    for (int q = C->jcp[c]; q < C->jcp[c + 1]; q++) {
+      int rr = C->red_of_z[C->jri[q]];
+      if (rr >= 0) SP.add(rr, rc);
+    }
+  }
+  SP.finish(C->scp, C->sri);
+  const int nnzS = C->scp[C->nred];
+  C->s_src.assign(nnzS, 0);
+  std::vector<char> assigned(nnzS, 0);
+  C->sv_bus.assign((size_t)15 * nbus, -1);
+  C->n_svar = 0;
+  auto spos = [&](int zr, int zc) {
+    int rr = C->red_of_z[zr], rc = C->red_of_z[zc];
+    return find_pos(C->scp, C->sri, rr, rc);
+  };
+  for (int b = 0; b < nbus; b++) {
+    int vr = npt + 2 * b;
+    int pr[4] = {spos(vr, vr), spos(vr, vr + 1), spos(vr + 1, vr), spos(vr + 1, vr + 1)};
+    for (int i = 0; i < 4; i++) {
+      C->sv_bus[15 * b + i] = C->n_svar;
+      C->s_src[pr[i]] = C->n_svar++;
+      assigned[pr[i]] = 1;
+    }
+    if (C->bus_sg[b] >= 0) {
+      sg_slots(SG.data() + C->bus_sg[b] * GP_COLS_STATICGEN, npt, sl);
+      for (int i = 0; i < 7; i++) {
+        if (sl[i].first < 0) continue;
+        int ps = spos(sl[i].first, sl[i].second);
+        C->sv_bus[15 * b + 4 + i] = C->n_svar;
+        C->s_src[ps] = C->n_svar++;
+        assigned[ps] = 1;
+      }
+    }
+  }
+  C->s_cst.clear();
+  for (int b = 0; b < nbus; b++) {
+    int vr = npt + 2 * b;
+    for (int rr = 0; rr < 2; rr++)
+      for (int q = C->yrp[2 * b + rr]; q < C->yrp[2 * b + rr + 1]; q++) {
+        int c = C->yci[q];
+        if (c / 2 == b) continue;
+        int ps = spos(vr + rr, npt + c);
+        C->s_cst.push_back(-C->yv[q]);
+        C->s_src[ps] = -(int)C->s_cst.size();
+        assigned[ps] = 1;
+      }
+  }
+  for (int e = 0; e < nnzS; e++)
+    if (!assigned[e]) {
+      // entries of the reduced pattern that no device writes are structural zeros
+      C->s_cst.push_back(0.0);
+      C->s_src[e] = -(int)C->s_cst.size();
+    }
+  if (C->s_cst.empty()) C->s_cst.push_back(0.0);
+}
+
+template <typename T>
+void up(DBuf<T> &d, const std::vector<T> &v) { d.upload(v); }
+
+void upload_tables(gp_ctx *C) {
+  up(C->d_is_diff, C->is_diff);
+  for (int t = 0; t < GP_NDEVTYPES; t++) {
+    std::vector<int> v = C->dev[t];
+    if (v.empty()) v.push_back(0);
+    up(C->d_dev[t], v);
+  }
+  up(C->d_cl, C->cl.empty() ? std::vector<int>(GP_CLUSTER_COLS, 0) : C->cl);
+  up(C->d_yrp, C->yrp); up(C->d_yci, C->yci.empty() ? std::vector<int>{0} : C->yci);
+  up(C->d_yv, C->yv.empty() ? std::vector<double>{0.0} : C->yv);
+  up(C->d_u0, C->u0.empty() ? std::vector<double>{0.0, 0.0} : C->u0);
+  up(C->d_bus_cl_p, C->bus_cl_p); up(C->d_bus_cl, C->bus_cl.empty() ? std::vector<int>{0} : C->bus_cl);
+  up(C->d_bus_zip_p, C->bus_zip_p); up(C->d_bus_zip, C->bus_zip.empty() ? std::vector<int>{0} : C->bus_zip);
+  up(C->d_bus_sg, C->bus_sg);
+  for (int t = 0; t < 6; t++) up(C->d_jp[t], C->jp[t]);
+  up(C->d_jp_bus, C->jp_bus); up(C->d_jp_yoff_p, C->jp_yoff_p);
+  up(C->d_jp_yoff_pos, C->jp_yoff_pos.empty() ? std::vector<int>{0} : C->jp_yoff_pos);
+  up(C->d_jp_yoff_val, C->jp_yoff_val.empty() ? std::vector<double>{0.0} : C->jp_yoff_val);
+  up(C->d_ydiag, C->ydiag);
+  up(C->d_cl_nt, C->cl_nt.empty() ? std::vector<int>{0} : C->cl_nt);
+  up(C->d_cl_xoff, C->cl_xoff);
+  up(C->d_red_of_z, C->red_of_z); up(C->d_z_of_red, C->z_of_red); up(C->d_sv_bus, C->sv_bus);
+  up(C->d_s_cst, C->s_cst);
+  DevTables &T = C->T;
+  T.n = C->n; T.nbus = C->nbus; T.net_ptr = C->net_ptr; T.ldM = C->ldM; T.M = C->M;
+  T.is_diff = C->d_is_diff.p;
+  T.genrou = C->d_dev[GP_GENROU].p; T.ieesgo = C->d_dev[GP_IEESGO].p; T.tgov1 = C->d_dev[GP_TGOV1].p;
+  T.sexs = C->d_dev[GP_SEXS].p; T.esdc1a = C->d_dev[GP_ESDC1A].p; T.ieeest = C->d_dev[GP_IEEEST].p;
+  T.zip = C->d_dev[GP_ZIPLOAD].p; T.sg = C->d_dev[GP_STATICGEN].p;
+  T.u0 = C->d_u0.p; T.cl = C->d_cl.p; T.n_cl = C->ncl;
+  T.yrp = C->d_yrp.p; T.yci = C->d_yci.p; T.yv = C->d_yv.p;
+  T.bus_cl_p = C->d_bus_cl_p.p; T.bus_cl = C->d_bus_cl.p; T.bus_zip_p = C->d_bus_zip_p.p; T.bus_zip = C->d_bus_zip.p;
+  T.bus_sg = C->d_bus_sg.p;
+  T.jp_genrou = C->d_jp[0].p; T.jp_ieesgo = C->d_jp[1].p; T.jp_tgov1 = C->d_jp[2].p; T.jp_sexs = C->d_jp[3].p;
+  T.jp_esdc1a = C->d_jp[4].p; T.jp_ieeest = C->d_jp[5].p; T.jp_bus = C->d_jp_bus.p;
+  T.jp_yoff_p = C->d_jp_yoff_p.p; T.jp_yoff_pos = C->d_jp_yoff_pos.p; T.jp_yoff_val = C->d_jp_yoff_val.p;
+  T.ydiag = C->d_ydiag.p;
+  T.cl_nt = C->d_cl_nt.p; T.n_cl_nt = (int)C->cl_nt.size(); T.cl_xoff = C->d_cl_xoff.p;
+  T.red_of_z = C->d_red_of_z.p; T.sv_bus = C->d_sv_bus.p;
+}
+
+void alloc_state(gp_ctx *C) {
+  const size_t L = (size_t)C->ldM;
+  C->d_p.alloc(std::max(1, C->npar));
+  C->d_z.alloc((size_t)C->n * L);
+  C->d_zold.alloc((size_t)C->n * L);
+  C->d_f.alloc((size_t)C->n * L);
+  C->d_inj.alloc((size_t)std::max(1, 2 * C->ncl) * L);
+  C->d_Dg.alloc((size_t)std::max(1, 6 * C->ncl) * L);
+  C->d_fnorm.alloc(L);
+  C->d_work.alloc(L); C->d_act.alloc(L); C->d_refac.alloc(L); C->d_sel.alloc(L);
+  C->d_fbus.alloc((size_t)MAXFAULT * L);
+  C->d_fg.alloc((size_t)MAXFAULT * L);
+  C->d_nunconv.alloc(2);
+  C->d_status.alloc(L);
+  C->d_nb.alloc(L);
+  HIPCHK(hipMemset(C->d_nb.p, 0, L * sizeof(unsigned long long)));
+  HIPCHK(hipMemset(C->d_status.p, 0, L * sizeof(int)));
+  HIPCHK(hipMemset(C->d_z.p, 0, (size_t)C->n * L * sizeof(double)));
+  std::vector<int> fb((size_t)MAXFAULT * L, -1);
+  HIPCHK(hipMemcpy(C->d_fbus.p, fb.data(), fb.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(C->d_fg.p, 0, (size_t)MAXFAULT * L * sizeof(double)));
+}
+
+FaultState faults(gp_ctx *C) { return FaultState{C->d_fbus.p, C->d_fg.p}; }
+
+void set_mask(gp_ctx *C, DBuf<uint8_t> &d, const std::vector<uint8_t> &h) {
+  HIPCHK(hipMemcpyAsync(d.p, h.data(), h.size(), hipMemcpyHostToDevice, C->stream));
+}
+
+// Representative matrix of scenario 0 for the pivot analysis, evaluated by
+// the product's own kernels.
+void analyse(gp_ctx *C, int method, double dt) {
+  const int L = C->ldM;
+  std::vector<uint8_t> sel(L, 0);
+  sel[0] = 1;
+  set_mask(C, C->d_sel, sel);
+  if (method == GP_METHOD_MONO_FP64) {
+    if (C->luJ.ready && C->luJ.dt == dt) return;
+    const int nnz = C->jcp[C->n];
+    C->d_jv.alloc((size_t)nnz * L);
+    HIPCHK(hipMemsetAsync(C->d_jv.p, 0, (size_t)nnz * L * sizeof(double), C->stream));
+    launch_jac_mono(C->T, C->d_p.p, C->d_z.p, dt, faults(C), C->d_jv.p, C->d_sel.p, C->stream);
+    HIPCHK(hipGetLastError());
+    std::vector<double> jv(nnz);
+    HIPCHK(hipMemcpy2DAsync(jv.data(), sizeof(double), C->d_jv.p, (size_t)L * sizeof(double), sizeof(double), nnz,
+                            hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    std::string e = splu_analyse(C->n, C->jcp.data(), C->jri.data(), jv.data(), C->luJ.A);
+    if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "Jacobian analysis: " + e};
+    std::vector<int> asrc(nnz);
+    for (int i = 0; i < nnz; i++) asrc[i] = i;
+    C->luJ.upload(asrc);
+    C->luJ.shape(sizeof(double));
+    C->luJ.ready = true;
+    C->luJ.dt = dt;
+    C->d_luJ.alloc((size_t)C->luJ.A.nnz_lu * C->M);
+  } else {
+    if (C->luS.ready && C->luS.dt == dt) return;
+    const int nnzS = C->scp[C->nred];
+    C->d_svar.alloc((size_t)std::max(1, C->n_svar) * L);
+    C->d_rhs.alloc((size_t)C->nred * L);
+    C->d_dv.alloc((size_t)C->nred * L);
+    C->d_X64.alloc((size_t)std::max(1, C->xrows) * 3 * L);
+    HIPCHK(hipMemcpyAsync(C->d_zold.p, C->d_z.p, (size_t)C->n * L * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
+    launch_cluster_residual(C->T, C->d_p.p, C->d_z.p, C->d_zold.p, dt, C->d_f.p, C->d_inj.p, C->d_sel.p, C->stream);
+    launch_bus_residual(C->T, C->d_p.p, C->d_z.p, C->d_inj.p, faults(C), C->d_f.p, C->d_sel.p, C->stream);
+    launch_cluster_schur<double>(C->T, C->d_p.p, C->d_z.p, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, C->d_sel.p, C->stream);
+    launch_bus_schur(C->T, C->d_p.p, C->d_z.p, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_sel.p,
+                     C->stream);
+    HIPCHK(hipGetLastError());
+    std::vector<double> sv(std::max(1, C->n_svar));
+    HIPCHK(hipMemcpy2DAsync(sv.data(), sizeof(double), C->d_svar.p, (size_t)L * sizeof(double), sizeof(double),
+                            std::max(1, C->n_svar), hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    std::vector<double> sx(nnzS);
+    for (int e = 0; e < nnzS; e++) {
+      int s = C->s_src[e];
+      sx[e] = s >= 0 ? sv[s] : C->s_cst[-s - 1];
+    }
+    std::string e = splu_analyse(C->nred, C->scp.data(), C->sri.data(), sx.data(), C->luS.A);
+    if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "reduced-network analysis: " + e};
+    C->luS.upload(C->s_src);
+    C->luS.ready = true;
+    C->luS.dt = dt;
+  }
+}
+
+void ensure_schur_buffers(gp_ctx *C, bool fp32) {
+  const size_t L = C->ldM;
+  if (fp32) {
+    if (!C->d_X32.p) C->d_X32.alloc((size_t)std::max(1, C->xrows) * 3 * L);
+    if (!C->d_luS32.p) C->d_luS32.alloc((size_t)C->luS.A.nnz_lu * C->M);
+    C->luS.shape(sizeof(float));
+  } else {
+    if (!C->d_luS64.p) C->d_luS64.alloc((size_t)C->luS.A.nnz_lu * C->M);
+    C->luS.shape(sizeof(double));
+  }
+}
+
+struct NewtonOut {
+  int64_t corrections = 0;
+  int64_t refac = 0;
+  std::vector<uint8_t> failed;   // scenarios still unconverged after itermax
+};
+
+// newton_step! (src/numerics.jl:55-176) for every scenario in `mask`, in lock
+// step: each iteration evaluates F, drops the scenarios with |F|inf < tol,
+// and applies one correction to the rest.
+NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, const std::vector<uint8_t> &mask) {
+  NewtonOut out;
+  const int L = C->ldM;
+  set_mask(C, C->d_work, mask);
+  const double *p = C->d_p.p;
+  double *z = C->d_z.p;
+  int nun = 0;
+  for (int it = 0; it < itermax; it++) {
+    launch_cluster_residual(C->T, p, z, C->d_zold.p, dt, C->d_f.p, C->d_inj.p, C->d_work.p, C->stream);
+    launch_bus_residual(C->T, p, z, C->d_inj.p, faults(C), C->d_f.p, C->d_work.p, C->stream);
+    HIPCHK(hipMemsetAsync(C->d_nunconv.p, 0, 2 * sizeof(int), C->stream));
+    // refactor policy: theta <= 0 refactors every iteration; otherwise after
+    // the first iteration only when |F| fell by less than 10x
+    const double theta = (method == GP_METHOD_MONO_FP64 || C->opt.refactor_policy == GP_REFACTOR_EVERY_ITER)
+                             ? 0.0
+                             : (C->opt.refactor_policy == GP_REFACTOR_ADAPTIVE ? 10.0 : 1e300);
+    launch_rownorm_impl(C->n, C->M, L, C->d_f.p, C->d_work.p, tol, C->d_fnorm.p, C->d_nb.p, C->d_nunconv.p,
+                        C->d_refac.p, it == 0, theta, C->stream);
+    int cnt[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(cnt, C->d_nunconv.p, 2 * sizeof(int), hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    nun = cnt[0];
+    if (nun == 0) return out;
+    out.corrections += nun;
+    out.refac += cnt[1];
+    if (method == GP_METHOD_MONO_FP64) {
+      launch_jac_mono(C->T, p, z, dt, faults(C), C->d_jv.p, C->d_work.p, C->stream);
+      launch_splu_factor_impl<double>(C->luJ.dev, C->d_jv.p, nullptr, L, C->d_luJ.p,
+                                      C->d_refac.p, C->M, C->d_status.p, C->stream);
+      launch_splu_solve<double>(C->luJ.dev, C->d_luJ.p, C->d_f.p, L, -1.0, nullptr, z, nullptr, C->d_work.p, C->M,
+                                C->stream);
+    } else {
+      const bool fp32 = method == GP_METHOD_SCHUR_FP32;
+      if (fp32) {
+        launch_cluster_schur<float>(C->T, p, z, C->d_f.p, dt, C->d_X32.p, C->d_Dg.p, C->d_work.p, C->stream);
+      } else {
+        launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, C->d_work.p, C->stream);
+      }
+      launch_bus_schur(C->T, p, z, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_work.p, C->stream);
+      if (fp32)
+        launch_splu_factor_impl<float>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS32.p, C->d_refac.p, C->M,
+                                       C->d_status.p, C->stream);
+      else
+        launch_splu_factor_impl<double>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_refac.p, C->M,
+                                        C->d_status.p, C->stream);
+      if (fp32) {
+        launch_splu_solve<float>(C->luS.dev, C->d_luS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z, C->d_dv.p,
+                                 C->d_work.p, C->M, C->stream);
+        launch_cluster_backsub<float>(C->T, C->d_X32.p, C->d_f.p, C->d_dv.p, z, C->d_work.p, C->stream);
+      } else {
+        launch_splu_solve<double>(C->luS.dev, C->d_luS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z, C->d_dv.p,
+                                  C->d_work.p, C->M, C->stream);
+        launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, C->d_work.p, C->stream);
+      }
+    }
+    HIPCHK(hipGetLastError());
+  }
+  // itermax exhausted: the scenarios still in `work` did not converge
+  out.failed.assign(L, 0);
+  HIPCHK(hipMemcpyAsync(out.failed.data(), C->d_work.p, L, hipMemcpyDeviceToHost, C->stream));
+  HIPCHK(hipStreamSynchronize(C->stream));
+  return out;
+}
+
+}  // namespace
+
+// ================================================================= C ABI
+extern "C" {
+
+const char *gp_last_error(void) { return g_err.c_str(); }
+const char *gp_version(void) { return "gradpower_hip 0.1 (gfx950)"; }
+
+int gp_create(const gp_system_desc *d, int32_t M, const gp_options *opt, gp_ctx **out) {
+  if (!d || !out || M <= 0) return fail(GP_ERR_BADARG, "gp_create: null argument or M <= 0");
+  if (d->n <= 0 || d->nbus <= 0 || d->net_ptr + 2 * d->nbus != d->n)
+    return fail(GP_ERR_BADARG, "gp_create: inconsistent sizes (n != net_ptr + 2*nbus)");
+  std::unique_ptr<gp_ctx> C(new gp_ctx());
+  try {
+    HIPCHK(hipGetDevice(&C->device));
+    HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+    C->n = d->n; C->nbus = d->nbus; C->net_ptr = d->net_ptr; C->npar = d->n_par; C->M = M;
+    C->ldM = (M + WAVE - 1) / WAVE * WAVE;
+    if (opt) C->opt = *opt;
+    else { C->opt.record_all = 1; C->opt.refactor_policy = GP_REFACTOR_EVERY_ITER; }
+    C->is_diff.assign(d->is_diff, d->is_diff + d->n);
+    C->yrp.assign(d->y_rowptr, d->y_rowptr + 2 * d->nbus + 1);
+    C->yci.assign(d->y_colind, d->y_colind + d->y_nnz);
+    C->yv.assign(d->y_val, d->y_val + d->y_nnz);
+    static const int ncols[GP_NDEVTYPES] = {GP_COLS_GENROU, GP_COLS_IEESGO, GP_COLS_TGOV1, GP_COLS_SEXS,
+                                            GP_COLS_ESDC1A, GP_COLS_IEEEST, GP_COLS_ZIPLOAD, GP_COLS_STATICGEN};
+    for (int t = 0; t < GP_NDEVTYPES; t++) {
+      if (d->n_dev[t] < 0) return fail(GP_ERR_BADARG, "negative device count");
+      if (d->n_dev[t] && !d->dev[t]) return fail(GP_ERR_BADARG, "missing device table");
+      C->dev[t].assign(d->dev[t], d->dev[t] + (size_t)d->n_dev[t] * ncols[t]);
+    }
+    C->u0.assign(d->genrou_u0, d->genrou_u0 + 2 * (size_t)d->n_dev[GP_GENROU]);
+    C->ncl = d->n_clusters;
+    C->cl.assign(d->clusters, d->clusters + (size_t)d->n_clusters * GP_CLUSTER_COLS);
+    // bounds checks of every index the kernels will dereference
+    auto zok = [&](int i) { return i >= -1 && i < C->n; };
+    for (int t = 0; t < 6; t++)
+      for (int v : C->dev[t])
+        if (!zok(v) && !(v >= 0 && v < std::max(C->npar, C->n)))
+          return fail(GP_ERR_BADARG, "device table index out of range");
+    for (int c = 0; c < C->ncl; c++) {
+      const int *r = C->cl.data() + c * GP_CLUSTER_COLS;
+      if (r[GP_CL_WSTART] < 0 || r[GP_CL_WSTART] + r[GP_CL_WSIZE] > C->net_ptr || r[GP_CL_BUS] < 0 ||
+          r[GP_CL_BUS] >= C->nbus)
+        return fail(GP_ERR_BADARG, "cluster table out of range");
+    }
+    for (int q = 0; q < (int)C->yci.size(); q++)
+      if (C->yci[q] < 0 || C->yci[q] >= 2 * C->nbus) return fail(GP_ERR_BADARG, "Y column out of range");
+    build_structures(C.get());
+    upload_tables(C.get());
+    alloc_state(C.get());
+    if (C->opt.record_all || !C->opt.record_idx) {
+      C->nrec = C->n;
+      C->rec_idx.clear();
+    } else {
+      C->nrec = C->opt.n_record;
+      C->rec_idx.assign(C->opt.record_idx, C->opt.record_idx + C->nrec);
+      for (int r : C->rec_idx)
+        if (r < 0 || r >= C->n) return fail(GP_ERR_BADARG, "record index out of range");
+      up(C->d_rec_idx, C->rec_idx);
+    }
+    C->nsteps.assign(M, 0);
+  } catch (const GpError &e) {
+    return fail(e.code, e.msg);
+  } catch (const std::exception &e) {
+    return fail(GP_ERR_OOM, e.what());
+  }
+  *out = C.release();
+  return GP_OK;
+}
+
+void gp_destroy(gp_ctx *C) {
+  if (!C) return;
+  if (C->stream) (void)hipStreamSynchronize(C->stream);
+  hipStream_t s = C->stream;
+  delete C;
+  if (s) (void)hipStreamDestroy(s);
+}
+
+int gp_set_state(gp_ctx *C, const double *z0, int32_t per_scenario, const double *p) {
+  if (!C || !z0 || !p) return fail(GP_ERR_BADARG, "gp_set_state: null argument");
+  try {
+    const int L = C->ldM;
+    std::vector<double> zt((size_t)C->n * L, 0.0);
+    for (int m = 0; m < C->M; m++) {
+      const double *src = per_scenario ? z0 + (size_t)m * C->n : z0;
+      for (int i = 0; i < C->n; i++) zt[(size_t)i * L + m] = src[i];
+    }
+    HIPCHK(hipMemcpy(C->d_z.p, zt.data(), zt.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(C->d_p.p, p, (size_t)C->npar * sizeof(double), hipMemcpyHostToDevice));
+    C->state_set = true;
+  } catch (const GpError &e) {
+    return fail(e.code, e.msg);
+  }
+  return GP_OK;
+}
+
+int gp_set_events(gp_ctx *C, const gp_event *ev, int32_t n_ev, const int32_t *nsteps) {
+  if (!C || (n_ev > 0 && !ev) || !nsteps) return fail(GP_ERR_BADARG, "gp_set_events: null argument");
+  for (int i = 0; i < n_ev; i++) {
+    const gp_event &e = ev[i];
+    if (e.scenario < 0 || e.scenario >= C->M || e.bus < 0 || e.bus >= C->nbus)
+      return fail(GP_ERR_BADARG, "gp_set_events: scenario or bus out of range");
+    if (!(e.r_fault > 0.0)) return fail(GP_ERR_BADARG, "gp_set_events: r_fault must be > 0");
+    if (e.step_on < 1 || e.step_off < 1)
+      return fail(GP_ERR_BADARG, "gp_set_events: event steps must be >= 1 (an event at t < dt/2 never fires)");
+  }
+  for (int m = 0; m < C->M; m++)
+    if (nsteps[m] < 0) return fail(GP_ERR_BADARG, "gp_set_events: negative nsteps");
+  C->events.assign(ev, ev + n_ev);
+  C->nsteps.assign(nsteps, nsteps + C->M);
+  return GP_OK;
+}
+
+int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t itermax, gp_stats *st) {
+  if (!C) return fail(GP_ERR_BADARG, "null context");
+  if (method < 0 || method > 2) return fail(GP_ERR_BADARG, "unknown method");
+  if (!(dt > 0.0) || !(tol > 0.0) || itermax <= 0) return fail(GP_ERR_BADARG, "dt, tol and itermax must be > 0");
+  if (!C->state_set) return fail(GP_ERR_BADARG, "gp_set_state was not called");
+  gp_stats S{};
+  try {
+    const int L = C->ldM, M = C->M;
+    int maxsteps = 0;
+    for (int m = 0; m < M; m++) maxsteps = std::max(maxsteps, C->nsteps[m]);
+    // fault table reset (no fault active at t = 0)
+    std::vector<int> fbus((size_t)MAXFAULT * L, -1);
+    std::vector<double> fg((size_t)MAXFAULT * L, 0.0);
+    HIPCHK(hipMemcpyAsync(C->d_fbus.p, fbus.data(), fbus.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
+    HIPCHK(hipMemcpyAsync(C->d_fg.p, fg.data(), fg.size() * sizeof(double), hipMemcpyHostToDevice, C->stream));
+    analyse(C, method, dt);
+    if (method != GP_METHOD_MONO_FP64) ensure_schur_buffers(C, method == GP_METHOD_SCHUR_FP32);
+    else C->luJ.shape(sizeof(double));
+    // trajectory storage [step][channel][ldM]
+    C->d_hist.alloc((size_t)(maxsteps + 1) * C->nrec * L);
+    C->hist_steps = maxsteps + 1;
+    C->traj_len.assign(M, 0);
+    for (int m = 0; m < M; m++) C->traj_len[m] = C->nsteps[m] + 1;
+    // per-step event lists
+    std::map<int, std::vector<const gp_event *>> on_at, off_at;
+    for (auto &e : C->events) {
+      on_at[e.step_on].push_back(&e);
+      off_at[e.step_off].push_back(&e);
+    }
+    std::vector<std::vector<std::pair<int, double>>> active(M);   // (bus, g) per scenario, in event order
+    std::vector<std::vector<const gp_event *>> active_ev(M);
+    std::vector<uint8_t> act(L, 0), dead(L, 0), everfail(L, 0), res(L, 0);
+    for (int m = 0; m < M; m++) act[m] = 1;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, C->stream));
+    set_mask(C, C->d_act, act);
+    launch_record_impl(C->nrec, C->rec_idx.empty() ? nullptr : C->d_rec_idx.p, L, M, C->d_z.p, C->d_hist.p,
+                       C->d_act.p, C->stream);
+    for (int k = 1; k <= maxsteps; k++) {
+      for (int m = 0; m < M; m++) act[m] = (k <= C->nsteps[m] && !dead[m]) ? 1 : 0;
+      int nact = 0;
+      for (int m = 0; m < M; m++) nact += act[m];
+      if (nact == 0) break;
+      set_mask(C, C->d_act, act);
+      launch_copy_active(C->n, L, C->d_z.p, C->d_zold.p, C->d_act.p, C->stream);
+      NewtonOut no = newton(C, method, dt, tol, itermax, act);
+      S.newton_iters += no.corrections;
+      S.refactorizations += no.refac;
+      S.scenario_steps += nact;
+      if (!no.failed.empty())
+        for (int m = 0; m < M; m++)
+          if (no.failed[m]) everfail[m] = 1;
+      double *hist_k = C->d_hist.p + (size_t)k * C->nrec * L;
+      launch_record_impl(C->nrec, C->rec_idx.empty() ? nullptr : C->d_rec_idx.p, L, M, C->d_z.p, hist_k, C->d_act.p,
+                         C->stream);
+      // events at step k: activation then deactivation (src/dynamics.jl:599-629)
+      bool changed = false;
+      std::fill(res.begin(), res.end(), 0);
+      auto ion = on_at.find(k);
+      if (ion != on_at.end())
+        for (const gp_event *e : ion->second) {
+          if (!act[e->scenario]) continue;
+          active_ev[e->scenario].push_back(e);
+          res[e->scenario] = 1;
+          changed = true;
+        }
+      auto ioff = off_at.find(k);
+      if (ioff != off_at.end())
+        for (const gp_event *e : ioff->second) {
+          if (!act[e->scenario]) continue;
+          auto &v = active_ev[e->scenario];
+          auto it = std::find(v.begin(), v.end(), e);
+          if (it != v.end()) v.erase(it);
+          res[e->scenario] = 1;
+          changed = true;
+        }
+      if (changed) {
+        for (int m = 0; m < M; m++) {
+          if (!res[m]) continue;
+          if ((int)active_ev[m].size() > MAXFAULT)
+            throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults in one scenario"};
+          for (int q = 0; q < MAXFAULT; q++) {
+            if (q < (int)active_ev[m].size()) {
+              fbus[(size_t)q * L + m] = active_ev[m][q]->bus;
+              fg[(size_t)q * L + m] = 1.0 / active_ev[m][q]->r_fault;
+            } else {
+              fbus[(size_t)q * L + m] = -1;
+              fg[(size_t)q * L + m] = 0.0;
+            }
+          }
+        }
+        HIPCHK(hipMemcpyAsync(C->d_fbus.p, fbus.data(), fbus.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
+        HIPCHK(hipMemcpyAsync(C->d_fg.p, fg.data(), fg.size() * sizeof(double), hipMemcpyHostToDevice, C->stream));
+        int nres = 0;
+        for (int m = 0; m < M; m++) nres += res[m];
+        S.event_resolves += nres;
+        // re-solve at dt = 0: differential states frozen, algebraic states re-solved
+        set_mask(C, C->d_sel, res);
+        launch_copy_active(C->n, L, C->d_z.p, C->d_zold.p, C->d_sel.p, C->stream);
+        NewtonOut nr = newton(C, method, 0.0, tol, itermax, res);
+        S.newton_iters += nr.corrections;
+        S.refactorizations += nr.refac;
+        if (!nr.failed.empty())
+          for (int m = 0; m < M; m++)
+            if (nr.failed[m]) everfail[m] = 1;
+      }
+      // non-finite states stop a scenario
+      if (!no.failed.empty()) {
+        std::vector<double> fn(L);
+        HIPCHK(hipMemcpy(fn.data(), C->d_fnorm.p, L * sizeof(double), hipMemcpyDeviceToHost));
+        for (int m = 0; m < M; m++)
+          if (no.failed[m] && !std::isfinite(fn[m])) {
+            dead[m] = 1;
+            S.n_nan++;
+          }
+      }
+    }
+    HIPCHK(hipEventRecord(e1, C->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    S.elapsed_ms = ms;
+    S.solve_ms = ms;
+    for (int m = 0; m < M; m++) S.n_not_converged += everfail[m];
+    std::vector<int> status(L);
+    HIPCHK(hipMemcpy(status.data(), C->d_status.p, L * sizeof(int), hipMemcpyDeviceToHost));
+    int nsing = 0;
+    for (int m = 0; m < M; m++) nsing += status[m] != 0;
+    HIPCHK(hipMemset(C->d_status.p, 0, L * sizeof(int)));
+    if (st) *st = S;
+    if (nsing) return fail(GP_ERR_SINGULAR_PIVOT, std::to_string(nsing) + " scenario(s) hit a zero or non-finite pivot");
+    if (S.n_nan) return fail(GP_ERR_NAN, std::to_string(S.n_nan) + " scenario(s) produced a non-finite residual");
+    if (S.n_not_converged)
+      return fail(GP_ERR_NEWTON_NOT_CONVERGED,
+                  std::to_string(S.n_not_converged) + " scenario(s) had a Newton solve that did not converge");
+  } catch (const GpError &e) {
+    if (st) *st = S;
+    return fail(e.code, e.msg);
+  }
+  return GP_OK;
+}
+
+int gp_get_trajectory(gp_ctx *C, int32_t m, double *out, int64_t ld) {
+  if (!C || !out || m < 0 || m >= C->M) return fail(GP_ERR_BADARG, "gp_get_trajectory: bad argument");
+  if (C->traj_len.empty()) return fail(GP_ERR_BADARG, "gp_get_trajectory: nothing integrated yet");
+  const int len = C->traj_len[m];
+  if (ld < len) return fail(GP_ERR_BADARG, "gp_get_trajectory: ld < nsteps[m]+1");
+  try {
+    const int L = C->ldM;
+    std::vector<double> tmp((size_t)len * C->nrec);
+    // hist[(k*nrec + j)*L + m] -> tmp[k*nrec + j]
+    HIPCHK(hipMemcpy2D(tmp.data(), sizeof(double), C->d_hist.p + m, (size_t)L * sizeof(double), sizeof(double),
+                       (size_t)len * C->nrec, hipMemcpyDeviceToHost));
+    for (int j = 0; j < C->nrec; j++)
+      for (int k = 0; k < len; k++) out[(size_t)j * ld + k] = tmp[(size_t)k * C->nrec + j];
+  } catch (const GpError &e) {
+    return fail(e.code, e.msg);
+  }
+  return GP_OK;
+}
+
+int gp_get_state(gp_ctx *C, double *out) {
+  if (!C || !out) return fail(GP_ERR_BADARG, "gp_get_state: null argument");
+  try {
+    const int L = C->ldM;
+    std::vector<double> zt((size_t)C->n * L);
+    HIPCHK(hipMemcpy(zt.data(), C->d_z.p, zt.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int m = 0; m < C->M; m++)
+      for (int i = 0; i < C->n; i++) out[(size_t)m * C->n + i] = zt[(size_t)i * L + m];
+  } catch (const GpError &e) {
+    return fail(e.code, e.msg);
+  }
+  return GP_OK;
+}
+
+static void load_eval_state(gp_ctx *C, const double *z, const uint8_t *fault_on) {
+  const int L = C->ldM;
+  std::vector<double> zt((size_t)C->n * L, 0.0);
+  for (int m = 0; m < C->M; m++)
+    for (int i = 0; i < C->n; i++) zt[(size_t)i * L + m] = z[(size_t)m * C->n + i];
+  HIPCHK(hipMemcpy(C->d_z.p, zt.data(), zt.size() * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<int> fbus((size_t)MAXFAULT * L, -1);
+  std::vector<double> fg((size_t)MAXFAULT * L, 0.0);
+  if (fault_on)
+    for (int m = 0; m < C->M; m++) {
+      if (!fault_on[m]) continue;
+      int q = 0;
+      for (auto &e : C->events)
+        if (e.scenario == m && q < MAXFAULT) {
+          fbus[(size_t)q * L + m] = e.bus;
+          fg[(size_t)q * L + m] = 1.0 / e.r_fault;
+          q++;
+        }
+    }
+  HIPCHK(hipMemcpy(C->d_fbus.p, fbus.data(), fbus.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(C->d_fg.p, fg.data(), fg.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+
+int gp_eval_residual(gp_ctx *C, const double *z, const double *zold, double dt, const uint8_t *fault_on, double *f) {
+  if (!C || !z || !zold || !f) return fail(GP_ERR_BADARG, "gp_eval_residual: null argument");
+  try {
+    const int L = C->ldM;
+    std::vector<double> zt((size_t)C->n * L, 0.0);
+    for (int m = 0; m < C->M; m++)
+      for (int i = 0; i < C->n; i++) zt[(size_t)i * L + m] = zold[(size_t)m * C->n + i];
+    HIPCHK(hipMemcpy(C->d_zold.p, zt.data(), zt.size() * sizeof(double), hipMemcpyHostToDevice));
+    load_eval_state(C, z, fault_on);
+    std::vector<uint8_t> all(L, 0);
+    for (int m = 0; m < C->M; m++) all[m] = 1;
+    set_mask(C, C->d_sel, all);
+    launch_cluster_residual(C->T, C->d_p.p, C->d_z.p, C->d_zold.p, dt, C->d_f.p, C->d_inj.p, C->d_sel.p, C->stream);
+    launch_bus_residual(C->T, C->d_p.p, C->d_z.p, C->d_inj.p, faults(C), C->d_f.p, C->d_sel.p, C->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(C->stream));
+    HIPCHK(hipMemcpy(zt.data(), C->d_f.p, zt.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int m = 0; m < C->M; m++)
+      for (int i = 0; i < C->n; i++) f[(size_t)m * C->n + i] = zt[(size_t)i * L + m];
+  } catch (const GpError &e) {
+    return fail(e.code, e.msg);
+  }
+  return GP_OK;
+}
+
+int gp_jacobian_pattern(gp_ctx *C, int32_t *nnz, int32_t *rowptr, int32_t *colind) {
+  if (!C || !nnz) return fail(GP_ERR_BADARG, "gp_jacobian_pattern: null argument");
+  const int nz = C->jcp[C->n];
+  *nnz = nz;
+  if (rowptr && colind) {
+    // the library's pattern is CSC; report it transposed as CSR of J
+    std::vector<int> cnt(C->n + 1, 0);
+    for (int q = 0; q < nz; q++) cnt[C->jri[q] + 1]++;
+    for (int i = 0; i < C->n; i++) cnt[i + 1] += cnt[i];
+    std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+    for (int c = 0; c < C->n; c++)
+      for (int q = C->jcp[c]; q < C->jcp[c + 1]; q++) colind[fillp[C->jri[q]]++] = c;
+    for (int i = 0; i <= C->n; i++) rowptr[i] = cnt[i];
+  }
+  return GP_OK;
+}
+
+int gp_eval_jacobian(gp_ctx *C, const double *z, double dt, const uint8_t *fault_on, double *jval) {
+  if (!C || !z || !jval) return fail(GP_ERR_BADARG, "gp_eval_jacobian: null argument");
+  try {
+    const int L = C->ldM, nz = C->jcp[C->n];
+    if (!C->d_jv.p || C->d_jv.n < (size_t)nz * L) C->d_jv.alloc((size_t)nz * L);
+    load_eval_state(C, z, fault_on);
+    std::vector<uint8_t> all(L, 0);
+    for (int m = 0; m < C->M; m++) all[m] = 1;
+    set_mask(C, C->d_sel, all);
+    HIPCHK(hipMemsetAsync(C->d_jv.p, 0, (size_t)nz * L * sizeof(double), C->stream));
+    launch_jac_mono(C->T, C->d_p.p, C->d_z.p, dt, faults(C), C->d_jv.p, C->d_sel.p, C->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(C->stream));
+    std::vector<double> jt((size_t)nz * L);
+    HIPCHK(hipMemcpy(jt.data(), C->d_jv.p, jt.size() * sizeof(double), hipMemcpyDeviceToHost));
+    // CSC position -> CSR position
+    std::vector<int> cnt(C->n + 1, 0);
+    for (int q = 0; q < nz; q++) cnt[C->jri[q] + 1]++;
+    for (int i = 0; i < C->n; i++) cnt[i + 1] += cnt[i];
+    std::vector<int> fillp(cnt.begin(), cnt.end() - 1), csr_of(nz);
+    for (int c = 0; c < C->n; c++)
+      for (int q = C->jcp[c]; q < C->jcp[c + 1]; q++) csr_of[q] = fillp[C->jri[q]]++;
+    for (int m = 0; m < C->M; m++)
+      for (int q = 0; q < nz; q++) jval[(size_t)m * nz + csr_of[q]] = jt[(size_t)q * L + m];
+  } catch (const GpError &e) {
+    return fail(e.code, e.msg);
+  }
+  return GP_OK;
+}
+
+int gp_newton_step(gp_ctx *C, int32_t method, const double *z, const double *zold, double dt,
+                   const uint8_t *fault_on, double *dz) {
+  if (!C || !z || !zold || !dz) return fail(GP_ERR_BADARG, "gp_newton_step: null argument");
+  if (method < 0 || method > 2) return fail(GP_ERR_BADARG, "unknown method");
+  if (!C->state_set) return fail(GP_ERR_BADARG, "gp_set_state was not called");
+  try {
+    const int L = C->ldM;
+    // the symbolic analysis evaluates its representative matrix with
+    // zold = z, so zold is loaded after it
+    load_eval_state(C, z, nullptr);
+    analyse(C, method, dt > 0.0 ? dt : 1.0 / 120.0);
+    if (method != GP_METHOD_MONO_FP64) ensure_schur_buffers(C, method == GP_METHOD_SCHUR_FP32);
+    std::vector<double> zt((size_t)C->n * L, 0.0);
+    for (int m = 0; m < C->M; m++)
+      for (int i = 0; i < C->n; i++) zt[(size_t)i * L + m] = zold[(size_t)m * C->n + i];
+    HIPCHK(hipMemcpy(C->d_zold.p, zt.data(), zt.size() * sizeof(double), hipMemcpyHostToDevice));
+    load_eval_state(C, z, fault_on);
+    std::vector<uint8_t> all(L, 0);
+    for (int m = 0; m < C->M; m++) all[m] = 1;
+    (void)newton(C, method, dt, -1.0, 1, all);   // one correction, nothing counts as converged
+    HIPCHK(hipMemcpy(zt.data(), C->d_z.p, zt.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int m = 0; m < C->M; m++)
+      for (int i = 0; i < C->n; i++) dz[(size_t)m * C->n + i] = zt[(size_t)i * L + m] - z[(size_t)m * C->n + i];
+  } catch (const GpError &e) {
+    return fail(e.code, e.msg);
+  }
+  return GP_OK;
+}
+
+int gp_get_symbolic_info(gp_ctx *C, gp_symbolic_info *info) {
+  if (!C || !info) return fail(GP_ERR_BADARG, "gp_get_symbolic_info: null argument");
+  gp_symbolic_info I{};
+  I.n = C->n;
+  I.nnz_j = C->jcp[C->n];
+  I.nnz_lu_j = C->luJ.ready ? C->luJ.A.nnz_lu : 0;
+  I.levels_j = C->luJ.ready ? (int)C->luJ.A.fac_lp.size() - 1 : 0;
+  I.n_red = C->nred;
+  I.nnz_s = C->scp[C->nred];
+  I.nnz_lu_s = C->luS.ready ? C->luS.A.nnz_lu : 0;
+  I.levels_s = C->luS.ready ? (int)C->luS.A.fac_lp.size() - 1 : 0;
+  I.n_clusters_nt = (int)C->cl_nt.size();
+  I.w_total_nt = C->xrows;
+  *info = I;
+  return GP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// gp_internal.hpp -- shared definitions of the host runtime and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gp {
+
+// Scenario-major ("entry-major") arrays are [rows][ldM]: scenario m of row i
+// lives at i*ldM + m, so the 64 lanes of a wavefront (64 consecutive
+// scenarios) touch 512 contiguous bytes.
+constexpr int WAVE = 64;
+constexpr int MAXFAULT = 4;   // simultaneously active faults per scenario
+
+// Jacobian slot counts per device (order of the slot lists built on the host).
+constexpr int NS_GENROU = 47;   // incl. optional pm, efd slots (-1 when unwired) and the delta diagonal
+constexpr int NS_IEESGO = 15;
+constexpr int NS_TGOV1 = 8;
+constexpr int NS_SEXS = 9;
+constexpr int NS_ESDC1A = 10;
+constexpr int NS_IEEEST = 32;   // incl. the two BE diagonals (s1, s3) absent from the device rows
+
+// Device-resident tables (read-only after gp_create).
+struct DevTables {
+  int n, nbus, net_ptr, ldM, M;
+  const uint8_t *is_diff;
+  const int *genrou, *ieesgo, *tgov1, *sexs, *esdc1a, *ieeest, *zip, *sg;
+  int n_genrou, n_ieesgo, n_tgov1, n_sexs, n_esdc1a, n_ieeest, n_zip, n_sg;
+  const double *u0;
+  const int *cl;          // cluster rows, GP_CLUSTER_COLS ints each
+  int n_cl;
+  // Y realified CSR
+  const int *yrp, *yci;
+  const double *yv;
+  // per-bus attachments (CSR over buses)
+  const int *bus_cl_p, *bus_cl;      // non-trivial clusters at each bus
+  const int *bus_zip_p, *bus_zip;    // ZIP loads at each bus
+  const int *bus_sg;                 // static generator at each bus (-1)
+  // Jacobian position tables for the monolithic path
+  const int *jp_genrou, *jp_ieesgo, *jp_tgov1, *jp_sexs, *jp_esdc1a, *jp_ieeest;
+  const int *jp_bus;      // per bus: 4 diag positions + 6 static-gen positions
+  const int *jp_yoff_p, *jp_yoff_pos;   // per bus row pair: off-diagonal Y entries
+  const double *jp_yoff_val;
+  const double *ydiag;    // per bus: 4 diagonal Y values (rr, ri, ir, ii)
+  // Schur path: reduced indexing and S slots
+  const int *cl_nt;       // list of non-trivial cluster ids
+  int n_cl_nt;
+  const int *cl_xoff;     // per cluster: offset (in 3-column rows) into X
+  const int *red_of_z;    // z index -> reduced index (-1 if eliminated)
+  const int *sv_bus;      // per bus: 4 var slots of S diag + 6 static-gen var slots
+};
+
+// Per-scenario fault state: fault_bus[k*ldM+m], fault_g[k*ldM+m].
+struct FaultState {
+  int *bus;
+  double *g;
+};
+
+struct SpluDev {
+  int n, nnz_a, nnz_lu, max_col, n_flev, n_fwlev, n_bwlev;
+  const int *a2lu, *a_src;      // per A entry: LU position, value source
+  const int *lu_cp, *lu_dp, *lu_ri;
+  const int *op_cp, *op;
+  const int *fac_lp, *fac_cols;
+  const int *lrow_p, *lrow_k, *lrow_pos;
+  const int *urow_p, *urow_j, *urow_pos;
+  const int *fw_lp, *fw_rows, *bw_lp, *bw_rows;
+  const int *q, *pinv;
+  int threads, lds_cols;        // launch shape of the factor kernel
+  // host-side level counts for launches
+};
+
+// ---- kernel launchers (gp_kernels.hip) ----------------------------------
+void launch_cluster_residual(const DevTables &T, const double *p, const double *z, const double *zold,
+                             double dt, double *f, double *inj, const uint8_t *work, hipStream_t s);
+void launch_bus_residual(const DevTables &T, const double *p, const double *z, const double *inj,
+                         FaultState fs, double *f, const uint8_t *work, hipStream_t s);
+void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, double tol, double *fnorm,
+                         unsigned long long *nb, int *n_unconv, uint8_t *refac, int first, double theta,
+                         hipStream_t s);
+void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs,
+                     double *jv, const uint8_t *work, hipStream_t s);
+template <typename TF>
+void launch_cluster_schur(const DevTables &T, const double *p, const double *z, const double *f,
+                          double dt, TF *X, double *Dg, const uint8_t *work, hipStream_t s);
+void launch_bus_schur(const DevTables &T, const double *p, const double *z, const double *f,
+                      const double *Dg, FaultState fs, double *svar, double *rhs, const uint8_t *work,
+                      hipStream_t s);
+template <typename TF>
+void launch_cluster_backsub(const DevTables &T, const TF *X, const double *f, const double *dv,
+                            double *z, const uint8_t *work, hipStream_t s);
+template <typename TF>
+void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv,
+                             const uint8_t *refac, int M, int *status, hipStream_t s);
+template <typename TF>
+void launch_splu_solve(const SpluDev &S, const TF *luv, const double *rhs, int ldM, double sign,
+                       const int *scatter, double *zupd, double *xout, const uint8_t *work, int M,
+                       hipStream_t s);
+void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s);
+void launch_record_impl(int nrec, const int *idx, int ldM, int M, const double *z, double *hist,
+                        const uint8_t *act, hipStream_t s);
+
+}  // namespace gp

@@ -1,0 +1,1313 @@
+// gp_kernels.hip -- CDNA4 (gfx950) kernels of the batched backward-Euler DAE integrator.
+//
+// Mapping.  Device, cluster and bus kernels put one scenario on each lane:
+// a 64-lane wavefront evaluates one device/cluster/bus for 64 consecutive
+// scenarios, every table/parameter read is wave-uniform (scalar loads) and
+// every state read/write is one coalesced 512-byte access (arrays are
+// [row][ldM]).  The sparse LU kernels put one scenario on each workgroup:
+// the factorisation walks the elimination levels with workgroup barriers,
+// staging each column in LDS, and the triangular solves keep the whole
+// solution vector in LDS.
+//
+// Device equations restate the reference leaves (paths relative to the
+// GradPower.jl repository): src/kernels/genrou.jl:48-141,407-559,
+// ieesgo.jl:133-224, tgov1.jl:73-146, sexs.jl:74-156, esdc1a.jl:97-196,
+// ieeest.jl:207-359, zipload.jl:33-179, static_gen.jl:107-220; network and
+// faults src/dynamics.jl:265-327,823-894; backward Euler src/dynamics.jl:339-500.
+#include <hip/hip_runtime.h>
+
+#include "../../include/gradpower_hip.h"
+#include "gp_internal.hpp"
+
+namespace gp {
+
+#define TWOPI60 (2.0 * 3.14159265358979323846 * 60.0)
+
+__device__ __forceinline__ int lane_scenario() { return blockIdx.y * WAVE + threadIdx.x; }
+
+// ---------------------------------------------------------------- GENROU
+struct GenVals {
+  // parameters
+  double x_d, x_q, x_dp, x_qp, x_ddp, xl, H, D, T_d0p, T_q0p, T_d0dp, T_q0dp, S1, S2;
+};
+
+__device__ __forceinline__ void sat_coeff(double s1, double s2, double &sa, double &sb) {
+  sa = 0.0; sb = 0.0;
+  if (s1 <= 0.0 || s2 <= 0.0) return;
+  double a = sqrt(s1 * 1.0 / (s2 * 1.2));
+  if (a == 1.0) return;
+  sa = 1.2 - (1.0 - 1.2) / (a - 1.0);
+  sb = s2 * 1.2 * (a - 1.0) * (a - 1.0) / ((1.0 - 1.2) * (1.0 - 1.2));
+}
+__device__ __forceinline__ double sat_se(double psi, double sa, double sb) {
+  if (sb == 0.0 || psi <= sa || psi == 0.0) return 0.0;
+  return sb * (psi - sa) * (psi - sa) / psi;
+}
+
+// Residual of one GENROU (unscaled f for diff rows, g for alg rows) and its
+// network current injection.
+__device__ __forceinline__ void genrou_residual(const double *P, const double *zs, double e_fd, double p_m,
+                                                double vr, double vi, double *fr, double &ir, double &ii) {
+  double x_d = P[0], x_q = P[1], x_dp = P[2], x_qp = P[3], x_ddp = P[4], xl = P[5], H = P[6], Dm = P[7];
+  double T_d0p = P[8], T_q0p = P[9], T_d0dp = P[10], T_q0dp = P[11], S1 = P[12], S2 = P[13], x_qdp = x_ddp;
+  double e_qp = zs[0], e_dp = zs[1], phi_1d = zs[2], phi_2q = zs[3], w = zs[4], delta = zs[5];
+  double v_q = zs[6], v_d = zs[7], i_q = zs[8], i_d = zs[9];
+  double psi_de = (x_ddp - xl) / (x_dp - xl) * e_qp + (x_dp - x_ddp) / (x_dp - xl) * phi_1d;
+  double psi_qe = -(x_ddp - xl) / (x_qp - xl) * e_dp + (x_qp - x_ddp) / (x_qp - xl) * phi_2q;
+  double sa, sb;
+  sat_coeff(S1, S2, sa, sb);
+  double Se = sat_se(sqrt(psi_de * psi_de + psi_qe * psi_qe), sa, sb);
+  fr[0] = (-e_qp + e_fd - (i_d - (-x_ddp + x_dp) * (-e_qp + i_d * (x_dp - xl) + phi_1d) / ((x_dp - xl) * (x_dp - xl))) * (x_d - x_dp) - Se * psi_de) / T_d0p;
+  fr[1] = (-e_dp + (i_q - (-x_qdp + x_qp) * (e_dp + i_q * (x_qp - xl) + phi_2q) / ((x_qp - xl) * (x_qp - xl))) * (x_q - x_qp)) / T_q0p;
+  fr[2] = (e_qp - i_d * (x_dp - xl) - phi_1d) / T_d0dp;
+  fr[3] = (-e_dp - i_q * (x_qp - xl) - phi_2q) / T_q0dp;
+  fr[4] = (p_m - Dm * w - psi_de * i_q + psi_qe * i_d) / (2.0 * H);
+  fr[5] = TWOPI60 * w;
+  fr[6] = i_d - ((x_ddp - xl) / (x_dp - xl) * e_qp + (x_dp - x_ddp) / (x_dp - xl) * phi_1d - v_q) / x_ddp;
+  fr[7] = i_q - (-(x_qdp - xl) / (x_qp - xl) * e_dp + (x_qp - x_qdp) / (x_qp - xl) * phi_2q + v_d) / x_qdp;
+  double sd, cd;
+  sincos(delta, &sd, &cd);
+  fr[8] = v_d - (vr * sd - vi * cd);
+  fr[9] = v_q - (vr * cd + vi * sd);
+  ir = sd * i_d + cd * i_q;
+  ii = -cd * i_d + sd * i_q;
+}
+
+// Jacobian of one GENROU, unscaled.  jd[] = 47 slots in the order of the
+// host slot list (see gp_host.cpp, genrou_slots).
+__device__ __forceinline__ void genrou_jac(const double *P, const double *zs, double vr, double vi, double *j) {
+  double x_d = P[0], x_q = P[1], x_dp = P[2], x_qp = P[3], x_ddp = P[4], xl = P[5], H = P[6], Dm = P[7];
+  double T_d0p = P[8], T_q0p = P[9], T_d0dp = P[10], T_q0dp = P[11], S1 = P[12], S2 = P[13], x_qdp = x_ddp;
+  double e_qp = zs[0], e_dp = zs[1], phi_1d = zs[2], phi_2q = zs[3], delta = zs[5];
+  double i_q = zs[8], i_d = zs[9];
+  double sd, cd;
+  sincos(delta, &sd, &cd);
+  double psi_de = (x_ddp - xl) / (x_dp - xl) * e_qp + (x_dp - x_ddp) / (x_dp - xl) * phi_1d;
+  double psi_qe = -(x_ddp - xl) / (x_qp - xl) * e_dp + (x_qp - x_ddp) / (x_qp - xl) * phi_2q;
+  double sa, sb;
+  sat_coeff(S1, S2, sa, sb);
+  double psi2 = sqrt(psi_de * psi_de + psi_qe * psi_qe);
+  double Se = sat_se(psi2, sa, sb), dSe;
+  if (sb == 0.0 || psi2 <= sa || psi2 == 0.0) dSe = 0.0;
+  else { double g = psi2 - sa; dSe = sb * (2.0 * g * psi2 - g * g) / (psi2 * psi2); }
+  double dpd = psi2 == 0.0 ? 0.0 : psi_de / psi2, dpq = psi2 == 0.0 ? 0.0 : psi_qe / psi2;
+  double a_eq = (x_ddp - xl) / (x_dp - xl), a_p1 = (x_dp - x_ddp) / (x_dp - xl);
+  double a_ed = -(x_ddp - xl) / (x_qp - xl), a_p2 = (x_qp - x_ddp) / (x_qp - xl);
+  double dT_de = -(dSe * dpd * psi_de + Se), dT_qe = -(dSe * dpq * psi_de);
+  double kd = (x_dp - xl) * (x_dp - xl), kq = (x_qp - xl) * (x_qp - xl);
+  j[0] = (-(x_d - x_dp) * (-x_ddp + x_dp) / kd - 1) / T_d0p + dT_de * a_eq / T_d0p;   // R0 eq
+  j[1] = (x_d - x_dp) * (-x_ddp + x_dp) / kd / T_d0p + dT_de * a_p1 / T_d0p;         // R0 p1
+  j[2] = -(x_d - x_dp) * (-(-x_ddp + x_dp) / (x_dp - xl) + 1) / T_d0p;               // R0 id
+  j[3] = dT_qe * a_ed / T_d0p;                                                        // R0 ed
+  j[4] = dT_qe * a_p2 / T_d0p;                                                        // R0 p2
+  j[5] = 1.0 / T_d0p;                                                                 // R0 efd
+  j[6] = (-(x_q - x_qp) * (-x_qdp + x_qp) / kq - 1) / T_q0p;                          // R1 ed
+  j[7] = -(x_q - x_qp) * (-x_qdp + x_qp) / kq / T_q0p;                                // R1 p2
+  j[8] = (x_q - x_qp) * (-(-x_qdp + x_qp) / (x_qp - xl) + 1) / T_q0p;                 // R1 iq
+  j[9] = 1.0 / T_d0dp;                                                                // R2 eq
+  j[10] = -1.0 / T_d0dp;                                                              // R2 p1
+  j[11] = (-x_dp + xl) / T_d0dp;                                                      // R2 id
+  j[12] = -1.0 / T_q0dp;                                                              // R3 ed
+  j[13] = -1.0 / T_q0dp;                                                              // R3 p2
+  j[14] = (-x_qp + xl) / T_q0dp;                                                      // R3 iq
+  j[15] = -0.5 * i_q * (x_ddp - xl) / (H * (x_dp - xl));                              // R4 eq
+  j[16] = 0.5 * i_d * (-x_ddp + xl) / (H * (x_qp - xl));                              // R4 ed
+  j[17] = -0.5 * i_q * (-x_ddp + x_dp) / (H * (x_dp - xl));                           // R4 p1
+  j[18] = 0.5 * i_d * (-x_ddp + x_qp) / (H * (x_qp - xl));                            // R4 p2
+  j[19] = -0.5 * Dm / H;                                                              // R4 w
+  j[20] = 0.5 * (-e_qp * (x_ddp - xl) / (x_dp - xl) - phi_1d * (-x_ddp + x_dp) / (x_dp - xl)) / H;  // R4 iq
+  j[21] = 0.5 * (e_dp * (-x_ddp + xl) / (x_qp - xl) + phi_2q * (-x_ddp + x_qp) / (x_qp - xl)) / H;  // R4 id
+  j[22] = 0.5 / H;                                                                    // R4 pm
+  j[23] = TWOPI60;                                                                    // R5 w
+  j[24] = 0.0;                                                                        // R5 delta
+  j[25] = -(x_ddp - xl) / (x_ddp * (x_dp - xl));                                      // A1 eq
+  j[26] = -(-x_ddp + x_dp) / (x_ddp * (x_dp - xl));                                   // A1 p1
+  j[27] = 1.0 / x_ddp;                                                                // A1 vq
+  j[28] = 1.0;                                                                        // A1 id
+  j[29] = -(-x_qdp + xl) / (x_qdp * (x_qp - xl));                                     // A2 ed
+  j[30] = -(-x_qdp + x_qp) / (x_qdp * (x_qp - xl));                                   // A2 p2
+  j[31] = -1.0 / x_qdp;                                                               // A2 vd
+  j[32] = 1.0;                                                                        // A2 iq
+  j[33] = -vr * cd - vi * sd;                                                         // A3 delta
+  j[34] = 1.0;                                                                        // A3 vd
+  j[35] = -sd;                                                                        // A3 vr
+  j[36] = cd;                                                                         // A3 vi
+  j[37] = vr * sd - vi * cd;                                                          // A4 delta
+  j[38] = 1.0;                                                                        // A4 vq
+  j[39] = -cd;                                                                        // A4 vr
+  j[40] = -sd;                                                                        // A4 vi
+  j[41] = i_d * cd - i_q * sd;                                                        // NR delta
+  j[42] = cd;                                                                         // NR iq
+  j[43] = sd;                                                                         // NR id
+  j[44] = i_d * sd + i_q * cd;                                                        // NI delta
+  j[45] = sd;                                                                         // NI iq
+  j[46] = -cd;                                                                        // NI id
+}
+// diff-row flags of the 47 GENROU slots (rows 0..5 are differential)
+__constant__ unsigned char GEN_SLOT_DIFF[47] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+                                               0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+__constant__ unsigned char GEN_SLOT_DIAG[47] = {1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1,
+                                               0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+// ---------------------------------------------------------------- controllers
+__device__ __forceinline__ void ieesgo_residual(const double *P, const double *x, double p_m, double w, double *fr) {
+  double T1 = P[0], T2 = P[1], T3 = P[2], T4 = P[3], T5 = P[4], T6 = P[5], K1 = P[6], K2 = P[7], K3 = P[8], pref = P[11];
+  double PF0 = x[0], PLL = x[1], TP1 = x[2], TP2 = x[3], TP3 = x[4];
+  fr[0] = (1.0 / T1) * (K1 * w - PF0);
+  fr[1] = (1.0 / T3) * ((1.0 - (T2 / T3)) * PF0 - PLL);
+  double SatP = pref - (T2 / T3) * PF0 - PLL;
+  fr[2] = (1.0 / T4) * (SatP - TP1);
+  fr[3] = (1.0 / T5) * (K2 * TP1 - TP2);
+  fr[4] = (1.0 / T6) * (K3 * TP2 - TP3);
+  fr[5] = TP1 * (1.0 - K2) + TP2 * (1.0 - K3) + TP3 - p_m;
+}
+__device__ __forceinline__ void tgov1_residual(const double *P, const double *x, double p_m, double w, double *fr) {
+  double R = P[0], T1 = P[1], T2 = P[4], T3 = P[5], DT = P[6], pref = P[7], t23 = T2 / T3;
+  fr[0] = (-x[0] + (1.0 - t23) * x[1]) / T3;
+  fr[1] = ((pref - w) / R - x[1]) / T1;
+  fr[2] = x[0] + t23 * x[1] - DT * w - p_m;
+}
+__device__ __forceinline__ void sexs_residual(const double *P, const double *x, double vr, double vi, double vs, double *fr) {
+  double TA_TB = P[0], TB = P[1], K = P[2], TE = P[3], vref = P[6];
+  double vm = sqrt(vr * vr + vi * vi);
+  if (vm == 0.0) vm = 1e-12;
+  double err = vref - vm + vs;
+  fr[0] = (-x[0] + (1.0 - TA_TB) * err) / TB;
+  fr[1] = (-x[1] + K * (x[0] + TA_TB * err)) / TE;
+}
+__device__ __forceinline__ void esdc1a_residual(const double *P, const double *x, double vr, double vi, double vs, double *fr) {
+  double Ka = P[0], Ta = P[1], Kf = P[2], Tf = P[3], Ke = P[4], Te = P[5], sa = P[7], sb = P[8], vref = P[9];
+  double vm = sqrt(vr * vr + vi * vi);
+  if (vm == 0.0) vm = 1e-12;
+  double e_fd = x[2];
+  double sat = (sb == 0.0 || e_fd <= sa) ? 0.0 : sb * (e_fd - sa) * (e_fd - sa);
+  fr[0] = (Ka * (vref - vm + vs - x[1] - (Kf / Tf) * e_fd) - x[0]) / Ta;
+  fr[1] = -((Kf / Tf) * e_fd + x[1]) / Tf;
+  fr[2] = (x[0] - Ke * e_fd - sat) / Te;
+}
+__device__ __forceinline__ void ieeest_residual(const double *P, const double *x, double sig, double *fr) {
+  double A1 = P[0], A2 = P[1], A3 = P[2], A4 = P[3], A5 = P[4], A6 = P[5];
+  double T1 = P[6], T2 = P[7], T3 = P[8], T4 = P[9], T5 = P[10], T6 = P[11], KS = P[12];
+  double s0 = x[0], s1 = x[1], s2 = x[2], s3 = x[3], s4 = x[4], s5 = x[5], s6 = x[6], vs = x[7];
+  double qq = s1 - s3 - A3 * s2;
+  double y2 = s3 + A5 * s2 + (A6 / A4) * qq;
+  double y3 = s4 + (T1 / T2) * (y2 - s4);
+  double y4 = s5 + (T3 / T4) * (y3 - s5);
+  fr[0] = (sig - s1 - A1 * s0) / A2;
+  fr[1] = s0;
+  fr[2] = (s1 - s3 - A3 * s2) / A4;
+  fr[3] = s2;
+  fr[4] = (y2 - s4) / T2;
+  fr[5] = (y3 - s5) / T4;
+  fr[6] = (KS * y4 - s6) / T6;
+  fr[7] = vs - (T5 / T6) * (KS * y4 - s6);
+}
+
+// ---------------------------------------------------------------- residual kernels
+// One wave = one cluster x 64 scenarios.  Writes the backward-Euler residual
+// of every cluster row and the generator's current injection.
+__global__ void __launch_bounds__(64) k_cluster_residual(DevTables T, const double *__restrict__ p,
+                                                         const double *__restrict__ z, const double *__restrict__ zold,
+                                                         double dt, double *__restrict__ f, double *__restrict__ inj,
+                                                         const uint8_t *__restrict__ work) {
+  const int c = T.cl_nt[blockIdx.x];
+  const int m = lane_scenario();
+  if (m >= T.M || !work[m]) return;
+  const int ldM = T.ldM;
+  const int *cl = T.cl + c * GP_CLUSTER_COLS;
+  const int ws = cl[GP_CL_WSTART], gk = cl[GP_CL_GENROU];
+  const int *g = T.genrou + gk * GP_COLS_GENROU;
+  const int dp = g[0], ap = g[1], bus = g[3], pm = g[4], efd = g[5];
+  const double *Pg = p + g[2];
+  const int vrz = T.net_ptr + 2 * bus;
+  const double vr = z[(size_t)vrz * ldM + m], vi = z[(size_t)(vrz + 1) * ldM + m];
+  double zs[10], fr[10];
+#pragma unroll
+  for (int i = 0; i < 6; i++) zs[i] = z[(size_t)(dp + i) * ldM + m];
+#pragma unroll
+  for (int i = 0; i < 4; i++) zs[6 + i] = z[(size_t)(ap + i) * ldM + m];
+  const double e_fd = efd >= 0 ? z[(size_t)efd * ldM + m] : T.u0[2 * gk];
+  const double p_m = pm >= 0 ? z[(size_t)pm * ldM + m] : T.u0[2 * gk + 1];
+  double ir, ii;
+  genrou_residual(Pg, zs, e_fd, p_m, vr, vi, fr, ir, ii);
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    const size_t o = (size_t)(dp + i) * ldM + m;
+    f[o] = zs[i] - zold[o] - dt * fr[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) f[(size_t)(ap + i) * ldM + m] = fr[6 + i];
+  inj[(size_t)(2 * c) * ldM + m] = ir;
+  inj[(size_t)(2 * c + 1) * ldM + m] = ii;
+  const double w = zs[4];
+  const int gt = cl[GP_CL_GOVTYPE];
+  if (gt == GP_IEESGO) {
+    const int *d = T.ieesgo + cl[GP_CL_GOV] * GP_COLS_IEESGO;
+    double x[5], r[6];
+#pragma unroll
+    for (int i = 0; i < 5; i++) x[i] = z[(size_t)(d[0] + i) * ldM + m];
+    const double pmv = z[(size_t)d[1] * ldM + m];
+    const double wv = d[3] >= 0 ? z[(size_t)d[3] * ldM + m] : 0.0;
+    ieesgo_residual(p + d[2], x, pmv, wv, r);
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      const size_t o = (size_t)(d[0] + i) * ldM + m;
+      f[o] = x[i] - zold[o] - dt * r[i];
+    }
+    f[(size_t)d[1] * ldM + m] = r[5];
+  } else if (gt == GP_TGOV1) {
+    const int *d = T.tgov1 + cl[GP_CL_GOV] * GP_COLS_TGOV1;
+    double x[2], r[3];
+    x[0] = z[(size_t)d[0] * ldM + m];
+    x[1] = z[(size_t)(d[0] + 1) * ldM + m];
+    const double pmv = z[(size_t)d[1] * ldM + m];
+    const double wv = d[3] >= 0 ? z[(size_t)d[3] * ldM + m] : 0.0;
+    tgov1_residual(p + d[2], x, pmv, wv, r);
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const size_t o = (size_t)(d[0] + i) * ldM + m;
+      f[o] = x[i] - zold[o] - dt * r[i];
+    }
+    f[(size_t)d[1] * ldM + m] = r[2];
+  }
+  const int et = cl[GP_CL_EXCTYPE];
+  if (et == GP_SEXS || et == GP_ESDC1A) {
+    const int *d = (et == GP_SEXS ? T.sexs + cl[GP_CL_EXC] * GP_COLS_SEXS : T.esdc1a + cl[GP_CL_EXC] * GP_COLS_ESDC1A);
+    const int ne = et == GP_SEXS ? 2 : 3;
+    const double evr = z[(size_t)d[2] * ldM + m], evi = z[(size_t)(d[2] + 1) * ldM + m];
+    const double vs = d[3] >= 0 ? z[(size_t)d[3] * ldM + m] : 0.0;
+    double x[3], r[3];
+    for (int i = 0; i < ne; i++) x[i] = z[(size_t)(d[0] + i) * ldM + m];
+    if (et == GP_SEXS) sexs_residual(p + d[1], x, evr, evi, vs, r);
+    else esdc1a_residual(p + d[1], x, evr, evi, vs, r);
+    for (int i = 0; i < ne; i++) {
+      const size_t o = (size_t)(d[0] + i) * ldM + m;
+      f[o] = x[i] - zold[o] - dt * r[i];
+    }
+  }
+  if (cl[GP_CL_PSS] >= 0) {
+    const int *d = T.ieeest + cl[GP_CL_PSS] * GP_COLS_IEEEST;
+    double x[8], r[8];
+#pragma unroll
+    for (int i = 0; i < 7; i++) x[i] = z[(size_t)(d[0] + i) * ldM + m];
+    x[7] = z[(size_t)d[1] * ldM + m];
+    const double sig = d[3] >= 0 ? z[(size_t)d[3] * ldM + m] : 0.0;
+    ieeest_residual(p + d[2], x, sig, r);
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+      const size_t o = (size_t)(d[0] + i) * ldM + m;
+      f[o] = x[i] - zold[o] - dt * r[i];
+    }
+    f[(size_t)d[1] * ldM + m] = r[7];
+  }
+  (void)w;
+  (void)ws;
+}
+
+// One wave = one bus x 64 scenarios: network rows (-Y v + injections, faults)
+// and the static generator's own rows.  Summation order per bus follows the
+// reference: -Y v, machines, loads, static generator, faults.
+__global__ void __launch_bounds__(64) k_bus_residual(DevTables T, const double *__restrict__ p,
+                                                     const double *__restrict__ z, const double *__restrict__ inj,
+                                                     FaultState fs, double *__restrict__ f,
+                                                     const uint8_t *__restrict__ work) {
+  const int b = blockIdx.x;
+  const int m = lane_scenario();
+  if (m >= T.M || !work[m]) return;
+  const int ldM = T.ldM;
+  const int vrz = T.net_ptr + 2 * b;
+  const double vr = z[(size_t)vrz * ldM + m], vi = z[(size_t)(vrz + 1) * ldM + m];
+  double fr = 0.0, fi = 0.0;
+  for (int q = T.yrp[2 * b]; q < T.yrp[2 * b + 1]; q++) fr += T.yv[q] * z[(size_t)(T.net_ptr + T.yci[q]) * ldM + m];
+  for (int q = T.yrp[2 * b + 1]; q < T.yrp[2 * b + 2]; q++) fi += T.yv[q] * z[(size_t)(T.net_ptr + T.yci[q]) * ldM + m];
+  fr = -fr;
+  fi = -fi;
+  for (int q = T.bus_cl_p[b]; q < T.bus_cl_p[b + 1]; q++) {
+    const int c = T.bus_cl[q];
+    fr += inj[(size_t)(2 * c) * ldM + m];
+    fi += inj[(size_t)(2 * c + 1) * ldM + m];
+  }
+  const double vm2r = vr * vr + vi * vi;
+  for (int q = T.bus_zip_p[b]; q < T.bus_zip_p[b + 1]; q++) {
+    const double *P = p + T.zip[T.bus_zip[q] * GP_COLS_ZIPLOAD + 1];
+    const double pl = P[0], ql = P[1], al = P[2], yre = al * P[7], yim = al * P[8];
+    const double den = vm2r > 0.2 ? vm2r : 0.2;
+    fr -= vr * yre - vi * yim;
+    fi -= vr * yim + vi * yre;
+    fr -= (1.0 - al) * (pl * vr - ql * vi) / den;
+    fi -= (1.0 - al) * (ql * vr + pl * vi) / den;
+  }
+  const int sgk = T.bus_sg[b];
+  if (sgk >= 0) {
+    const int *d = T.sg + sgk * GP_COLS_STATICGEN;
+    const int ap = d[1], bt = d[3];
+    const double *P = p + d[2];
+    const double vm2 = vm2r > 0.2 ? vm2r : 0.2;
+    double pv, qv;
+    if (bt == 2) { pv = P[0]; qv = z[(size_t)ap * ldM + m]; }
+    else if (bt == 3) { pv = z[(size_t)ap * ldM + m]; qv = z[(size_t)(ap + 1) * ldM + m]; }
+    else { pv = P[0]; qv = P[1]; }
+    fr += (pv * vr + qv * vi) / vm2;
+    fi += (pv * vi - qv * vr) / vm2;
+    if (bt == 2) f[(size_t)ap * ldM + m] = vr * vr + vi * vi - P[2] * P[2];
+    else if (bt == 3) {
+      f[(size_t)ap * ldM + m] = vr - P[2] * cos(P[3]);
+      f[(size_t)(ap + 1) * ldM + m] = vi - P[2] * sin(P[3]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < MAXFAULT; k++) {
+    if (fs.bus[k * ldM + m] == b) {
+      const double gf = fs.g[k * ldM + m];
+      fr -= gf * vr;
+      fi -= gf * vi;
+    }
+  }
+  f[(size_t)vrz * ldM + m] = fr;
+  f[(size_t)(vrz + 1) * ldM + m] = fi;
+}
+
+// Max-norm of F per scenario.  Grid (ldM/64, chunks); partial maxima are
+// combined with an integer atomicMax on the bit pattern (|x| >= 0, so the
+// integer order is the float order; NaN patterns sort above +inf).
+__global__ void __launch_bounds__(64) k_rownorm(int n, int ldM, int M, const double *__restrict__ f,
+                                                const uint8_t *__restrict__ work, unsigned long long *__restrict__ nb) {
+  const int m = blockIdx.x * WAVE + threadIdx.x;
+  if (m >= M || !work[m]) return;
+  const int chunk = (n + gridDim.y - 1) / gridDim.y;
+  const int r0 = blockIdx.y * chunk, r1 = min(n, r0 + chunk);
+  double mx = 0.0;
+  unsigned long long bits = 0;
+  for (int i = r0; i < r1; i++) {
+    const double a = fabs(f[(size_t)i * ldM + m]);
+    const unsigned long long u = __double_as_longlong(a);
+    bits = u > bits ? u : bits;
+  }
+  (void)mx;
+  atomicMax(&nb[m], bits);
+}
+
+// Finishes the max-norm: converged scenarios leave `work`; for the others,
+// refac[m] is set when this is the first iteration (first != 0), when every
+// iteration refactors (theta <= 0), or when |F| fell by less than 1/theta
+// since the previous iteration.
+__global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict__ nb, double *__restrict__ fnorm,
+                              uint8_t *__restrict__ work, int *__restrict__ n_unconv, uint8_t *__restrict__ refac,
+                              int first, double theta) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  if (!work[m]) {
+    refac[m] = 0;
+    return;
+  }
+  const double v = __longlong_as_double(nb[m]);
+  const double prev = fnorm[m];
+  fnorm[m] = v;
+  nb[m] = 0ull;
+  if (v < tol) {                 // converged (NaN compares false: stays working and is caught by itermax)
+    work[m] = 0;
+    refac[m] = 0;
+  } else {
+    atomicAdd(n_unconv, 1);
+    const int rf = (first || theta <= 0.0 || !(v * theta < prev)) ? 1 : 0;
+    refac[m] = rf;
+    if (rf) atomicAdd(n_unconv + 1, 1);
+  }
+}
+
+// ---------------------------------------------------------------- monolithic Jacobian
+__device__ __forceinline__ double be(double v, bool diff, bool diag, double dt) {
+  return diff ? (diag ? 1.0 - dt * v : -dt * v) : v;
+}
+
+__global__ void __launch_bounds__(64) k_cluster_jac_mono(DevTables T, const double *__restrict__ p,
+                                                         const double *__restrict__ z, double dt,
+                                                         double *__restrict__ jv, const uint8_t *__restrict__ work) {
+  const int c = T.cl_nt[blockIdx.x];
+  const int m = lane_scenario();
+  if (m >= T.M || !work[m]) return;
+  const int ldM = T.ldM;
+  const int *cl = T.cl + c * GP_CLUSTER_COLS;
+  const int gk = cl[GP_CL_GENROU];
+  const int *g = T.genrou + gk * GP_COLS_GENROU;
+  const int dp = g[0], ap = g[1], bus = g[3];
+  const int vrz = T.net_ptr + 2 * bus;
+  const double vr = z[(size_t)vrz * ldM + m], vi = z[(size_t)(vrz + 1) * ldM + m];
+  double zs[10], j[47];
+#pragma unroll
+  for (int i = 0; i < 6; i++) zs[i] = z[(size_t)(dp + i) * ldM + m];
+#pragma unroll
+  for (int i = 0; i < 4; i++) zs[6 + i] = z[(size_t)(ap + i) * ldM + m];
+  genrou_jac(p + g[2], zs, vr, vi, j);
+  const int *jp = T.jp_genrou + gk * NS_GENROU;
+#pragma unroll
+  for (int s = 0; s < NS_GENROU; s++) {
+    const int pos = jp[s];
+    if (pos >= 0) jv[(size_t)pos * ldM + m] = be(j[s], GEN_SLOT_DIFF[s], GEN_SLOT_DIAG[s], dt);
+  }
+  const int gt = cl[GP_CL_GOVTYPE];
+  if (gt == GP_IEESGO) {
+    const int *d = T.ieesgo + cl[GP_CL_GOV] * GP_COLS_IEESGO;
+    const double *P = p + d[2];
+    const double T1 = P[0], T2 = P[1], T3 = P[2], T4 = P[3], T5 = P[4], T6 = P[5], K1 = P[6], K2 = P[7], K3 = P[8];
+    const double v[15] = {-1.0 / T1, K1 / T1, (1.0 - T2 / T3) / T3, -1.0 / T3, -T2 / (T3 * T4), -1.0 / T4, -1.0 / T4,
+                          K2 / T5, -1.0 / T5, K3 / T6, -1.0 / T6, 1.0 - K2, 1.0 - K3, 1.0, -1.0};
+    const bool diff[15] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0};
+    const bool diag[15] = {1, 0, 0, 1, 0, 0, 1, 0, 1, 0, 1, 0, 0, 0, 0};
+    const int *jpd = T.jp_ieesgo + cl[GP_CL_GOV] * NS_IEESGO;
+#pragma unroll
+    for (int s = 0; s < 15; s++)
+      if (jpd[s] >= 0) jv[(size_t)jpd[s] * ldM + m] = be(v[s], diff[s], diag[s], dt);
+  } else if (gt == GP_TGOV1) {
+    const int *d = T.tgov1 + cl[GP_CL_GOV] * GP_COLS_TGOV1;
+    const double *P = p + d[2];
+    const double R = P[0], T1 = P[1], T2 = P[4], T3 = P[5], DT = P[6], t23 = T2 / T3;
+    const double v[8] = {-1.0 / T3, (1.0 - t23) / T3, -1.0 / T1, -1.0 / (R * T1), 1.0, t23, -DT, -1.0};
+    const bool diff[8] = {1, 1, 1, 1, 0, 0, 0, 0};
+    const bool diag[8] = {1, 0, 1, 0, 0, 0, 0, 0};
+    const int *jpd = T.jp_tgov1 + cl[GP_CL_GOV] * NS_TGOV1;
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+      if (jpd[s] >= 0) jv[(size_t)jpd[s] * ldM + m] = be(v[s], diff[s], diag[s], dt);
+  }
+  const int et = cl[GP_CL_EXCTYPE];
+  if (et == GP_SEXS) {
+    const int *d = T.sexs + cl[GP_CL_EXC] * GP_COLS_SEXS;
+    const double *P = p + d[1];
+    const double TA_TB = P[0], TB = P[1], K = P[2], TE = P[3];
+    const double evr = z[(size_t)d[2] * ldM + m], evi = z[(size_t)(d[2] + 1) * ldM + m];
+    double vm = sqrt(evr * evr + evi * evi);
+    if (vm == 0.0) vm = 1e-12;
+    const double dr = evr / vm, di = evi / vm;
+    const double v[9] = {-1.0 / TB, -(1.0 - TA_TB) * dr / TB, -(1.0 - TA_TB) * di / TB, (1.0 - TA_TB) / TB,
+                         K / TE, -1.0 / TE, -K * TA_TB * dr / TE, -K * TA_TB * di / TE, K * TA_TB / TE};
+    const bool diag[9] = {1, 0, 0, 0, 0, 1, 0, 0, 0};
+    const int *jpd = T.jp_sexs + cl[GP_CL_EXC] * NS_SEXS;
+#pragma unroll
+    for (int s = 0; s < 9; s++)
+      if (jpd[s] >= 0) jv[(size_t)jpd[s] * ldM + m] = be(v[s], true, diag[s], dt);
+  } else if (et == GP_ESDC1A) {
+    const int *d = T.esdc1a + cl[GP_CL_EXC] * GP_COLS_ESDC1A;
+    const double *P = p + d[1];
+    const double Ka = P[0], Ta = P[1], Kf = P[2], Tf = P[3], Ke = P[4], Te = P[5], sa = P[7], sb = P[8];
+    const double e_fd = z[(size_t)(d[0] + 2) * ldM + m];
+    const double evr = z[(size_t)d[2] * ldM + m], evi = z[(size_t)(d[2] + 1) * ldM + m];
+    double vm = sqrt(evr * evr + evi * evi);
+    if (vm == 0.0) vm = 1e-12;
+    const double dsat = (sb == 0.0 || e_fd <= sa) ? 0.0 : 2.0 * sb * (e_fd - sa);
+    const double v[10] = {-1.0 / Ta, -Ka / Ta, -Ka * Kf / (Ta * Tf), -Ka * (evr / vm) / Ta, -Ka * (evi / vm) / Ta,
+                          Ka / Ta, -1.0 / Tf, -Kf / (Tf * Tf), 1.0 / Te, -(Ke + dsat) / Te};
+    const bool diag[10] = {1, 0, 0, 0, 0, 0, 1, 0, 0, 1};
+    const int *jpd = T.jp_esdc1a + cl[GP_CL_EXC] * NS_ESDC1A;
+#pragma unroll
+    for (int s = 0; s < 10; s++)
+      if (jpd[s] >= 0) jv[(size_t)jpd[s] * ldM + m] = be(v[s], true, diag[s], dt);
+  }
+  if (cl[GP_CL_PSS] >= 0) {
+    const int *d = T.ieeest + cl[GP_CL_PSS] * GP_COLS_IEEEST;
+    const double *P = p + d[2];
+    const double A1 = P[0], A2 = P[1], A3 = P[2], A4 = P[3], A5 = P[4], A6 = P[5];
+    const double T1 = P[6], T2 = P[7], T3 = P[8], T4 = P[9], T5 = P[10], T6 = P[11], KS = P[12];
+    const double rA = A6 / A4, y21 = rA, y22 = A5 - A3 * rA, y23 = 1.0 - rA;
+    const double r12 = T1 / T2, y31 = r12 * y21, y32 = r12 * y22, y33 = r12 * y23, y34 = 1.0 - r12;
+    const double r34 = T3 / T4, y41 = r34 * y31, y42 = r34 * y32, y43 = r34 * y33, y44 = r34 * y34, y45 = 1.0 - r34;
+    const double r56 = T5 / T6;
+    const double v[32] = {-A1 / A2, -1.0 / A2, 1.0 / A2, 1.0, 1.0 / A4, -A3 / A4, -1.0 / A4, 1.0,
+                          y21 / T2, y22 / T2, y23 / T2, -1.0 / T2,
+                          y31 / T4, y32 / T4, y33 / T4, y34 / T4, -1.0 / T4,
+                          KS * y41 / T6, KS * y42 / T6, KS * y43 / T6, KS * y44 / T6, KS * y45 / T6, -1.0 / T6,
+                          -r56 * KS * y41, -r56 * KS * y42, -r56 * KS * y43, -r56 * KS * y44, -r56 * KS * y45, r56, 1.0,
+                          0.0, 0.0};
+    const int *jpd = T.jp_ieeest + cl[GP_CL_PSS] * NS_IEEEST;
+#pragma unroll
+    for (int s = 0; s < 32; s++) {
+      if (jpd[s] < 0) continue;
+      const bool diff = !(s >= 23 && s <= 29);
+      const bool diag = (s == 0 || s == 5 || s == 11 || s == 16 || s == 22 || s == 30 || s == 31);
+      jv[(size_t)jpd[s] * ldM + m] = be(v[s], diff, diag, dt);
+    }
+  }
+}
+
+// Static generator Jacobian entries (src/kernels/static_gen.jl:168-220).
+// out: drr, dri, dir, dii, vr_ap, vi_ap, vr_ap1, vi_ap1, ap_vr, ap_vi, ap1_vi
+__device__ __forceinline__ void sg_jac(const double *P, int bt, double vr, double vi, double za, double za1, double *o) {
+  const double vm2r = vr * vr + vi * vi, vm2 = vm2r > 0.2 ? vm2r : 0.2;
+  double pv, qv;
+  if (bt == 2) { pv = P[0]; qv = za; }
+  else if (bt == 3) { pv = za; qv = za1; }
+  else { pv = P[0]; qv = P[1]; }
+  const double irn = pv * vr + qv * vi, iin = pv * vi - qv * vr;
+  if (vm2r > 0.2) {
+    const double vm4 = vm2 * vm2;
+    o[0] = pv / vm2 - 2.0 * vr * irn / vm4; o[1] = qv / vm2 - 2.0 * vi * irn / vm4;
+    o[2] = -qv / vm2 - 2.0 * vr * iin / vm4; o[3] = pv / vm2 - 2.0 * vi * iin / vm4;
+  } else {
+    o[0] = pv / vm2; o[1] = qv / vm2; o[2] = -qv / vm2; o[3] = pv / vm2;
+  }
+  for (int i = 4; i < 11; i++) o[i] = 0.0;
+  if (bt == 2) {
+    o[4] = vi / vm2; o[5] = -vr / vm2; o[8] = 2.0 * vr; o[9] = 2.0 * vi;
+  } else if (bt == 3) {
+    o[4] = vr / vm2; o[5] = vi / vm2; o[6] = vi / vm2; o[7] = -vr / vm2; o[8] = 1.0; o[10] = 1.0;
+  }
+}
+
+// ZIP-load Jacobian on the bus diagonal block (src/kernels/zipload.jl:133-179).
+__device__ __forceinline__ void zip_jac(const double *P, double vr, double vi, double *o) {
+  const double pl = P[0], ql = P[1], al = P[2], yre = P[7], yim = P[8];
+  const double vm2 = vr * vr + vi * vi;
+  double prr, pri, pir, pii;
+  if (vm2 > 0.2) {
+    const double iv4 = 1.0 / (vm2 * vm2);
+    prr = (1.0 - al) * ((pl * vr - ql * vi) * 2.0 * vr - pl * vm2) * iv4;
+    pri = (1.0 - al) * ((pl * vr - ql * vi) * 2.0 * vi + ql * vm2) * iv4;
+    pir = (1.0 - al) * ((ql * vr + pl * vi) * 2.0 * vr - ql * vm2) * iv4;
+    pii = (1.0 - al) * ((ql * vr + pl * vi) * 2.0 * vi - pl * vm2) * iv4;
+  } else {
+    const double it = 1.0 / 0.2;
+    prr = (1.0 - al) * (-pl) * it; pri = (1.0 - al) * ql * it;
+    pir = (1.0 - al) * (-ql) * it; pii = (1.0 - al) * (-pl) * it;
+  }
+  o[0] += -al * yre + prr;
+  o[1] += al * yim + pri;
+  o[2] += -al * yim + pir;
+  o[3] += -al * yre + pii;
+}
+
+// Bus diagonal block of the network rows: -Y_bb + loads + static generator - faults.
+__device__ __forceinline__ void bus_diag(const DevTables &T, const double *p, const double *z, const FaultState &fs,
+                                         int b, int m, double vr, double vi, double *dg, double *sgo) {
+  const int ldM = T.ldM;
+  dg[0] = -T.ydiag[4 * b]; dg[1] = -T.ydiag[4 * b + 1]; dg[2] = -T.ydiag[4 * b + 2]; dg[3] = -T.ydiag[4 * b + 3];
+  for (int q = T.bus_zip_p[b]; q < T.bus_zip_p[b + 1]; q++)
+    zip_jac(p + T.zip[T.bus_zip[q] * GP_COLS_ZIPLOAD + 1], vr, vi, dg);
+  const int sgk = T.bus_sg[b];
+  for (int i = 0; i < 11; i++) sgo[i] = 0.0;
+  if (sgk >= 0) {
+    const int *d = T.sg + sgk * GP_COLS_STATICGEN;
+    const int ap = d[1], bt = d[3];
+    const double za = ap >= 0 ? z[(size_t)ap * ldM + m] : 0.0;
+    const double za1 = bt == 3 ? z[(size_t)(ap + 1) * ldM + m] : 0.0;
+    sg_jac(p + d[2], bt, vr, vi, za, za1, sgo);
+    dg[0] += sgo[0]; dg[1] += sgo[1]; dg[2] += sgo[2]; dg[3] += sgo[3];
+  }
+#pragma unroll
+  for (int k = 0; k < MAXFAULT; k++)
+    if (fs.bus[k * ldM + m] == b) {
+      const double gf = fs.g[k * ldM + m];
+      dg[0] -= gf;
+      dg[3] -= gf;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_bus_jac_mono(DevTables T, const double *__restrict__ p,
+                                                     const double *__restrict__ z, FaultState fs,
+                                                     double *__restrict__ jv, const uint8_t *__restrict__ work) {
+  const int b = blockIdx.x;
+  const int m = lane_scenario();
+  if (m >= T.M || !work[m]) return;
+  const int ldM = T.ldM;
+  const int vrz = T.net_ptr + 2 * b;
+  const double vr = z[(size_t)vrz * ldM + m], vi = z[(size_t)(vrz + 1) * ldM + m];
+  double dg[4], sgo[11];
+  bus_diag(T, p, z, fs, b, m, vr, vi, dg, sgo);
+  const int *jb = T.jp_bus + 15 * b;
+#pragma unroll
+  for (int i = 0; i < 4; i++) jv[(size_t)jb[i] * ldM + m] = dg[i];
+#pragma unroll
+  for (int i = 4; i < 11; i++)
+    if (jb[i] >= 0) jv[(size_t)jb[i] * ldM + m] = sgo[i];
+  for (int q = T.jp_yoff_p[b]; q < T.jp_yoff_p[b + 1]; q++) jv[(size_t)T.jp_yoff_pos[q] * ldM + m] = -T.jp_yoff_val[q];
+}
+
+// ---------------------------------------------------------------- small dense solves
+template <int N, int NR, typename TF>
+__device__ __forceinline__ void dense_solve_nopiv(TF (&a)[N][N], TF (&r)[N][NR]) {
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const TF inv = TF(1) / a[k][k];
+#pragma unroll
+    for (int i = k + 1; i < N; i++) {
+      const TF l = a[i][k] * inv;
+#pragma unroll
+      for (int j = k + 1; j < N; j++) a[i][j] -= l * a[k][j];
+#pragma unroll
+      for (int c = 0; c < NR; c++) r[i][c] -= l * r[k][c];
+    }
+  }
+#pragma unroll
+  for (int k = N - 1; k >= 0; k--) {
+    const TF inv = TF(1) / a[k][k];
+#pragma unroll
+    for (int c = 0; c < NR; c++) {
+      TF s = r[k][c];
+#pragma unroll
+      for (int j = k + 1; j < N; j++) s -= a[k][j] * r[j][c];
+      r[k][c] = s * inv;
+    }
+  }
+}
+
+// Gaussian elimination with partial pivoting; row exchanges are done with
+// selects so every index stays a compile-time constant (registers only).
+template <int N, int NR, typename TF>
+__device__ __forceinline__ void dense_solve_pp(TF (&a)[N][N], TF (&r)[N][NR]) {
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    TF best = fabs(a[k][k]);
+    int piv = k;
+#pragma unroll
+    for (int i = k + 1; i < N; i++) {
+      const TF v = fabs(a[i][k]);
+      if (v > best) { best = v; piv = i; }
+    }
+#pragma unroll
+    for (int i = k + 1; i < N; i++) {
+      const bool sw = (piv == i);
+#pragma unroll
+      for (int j = k; j < N; j++) {
+        const TF ta = a[k][j], tb = a[i][j];
+        a[k][j] = sw ? tb : ta;
+        a[i][j] = sw ? ta : tb;
+      }
+#pragma unroll
+      for (int c = 0; c < NR; c++) {
+        const TF ta = r[k][c], tb = r[i][c];
+        r[k][c] = sw ? tb : ta;
+        r[i][c] = sw ? ta : tb;
+      }
+    }
+    const TF inv = TF(1) / a[k][k];
+#pragma unroll
+    for (int i = k + 1; i < N; i++) {
+      const TF l = a[i][k] * inv;
+#pragma unroll
+      for (int j = k + 1; j < N; j++) a[i][j] -= l * a[k][j];
+#pragma unroll
+      for (int c = 0; c < NR; c++) r[i][c] -= l * r[k][c];
+    }
+  }
+#pragma unroll
+  for (int k = N - 1; k >= 0; k--) {
+    const TF inv = TF(1) / a[k][k];
+#pragma unroll
+    for (int c = 0; c < NR; c++) {
+      TF s = r[k][c];
+#pragma unroll
+      for (int j = k + 1; j < N; j++) s -= a[k][j] * r[j][c];
+      r[k][c] = s * inv;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- controller blocks (BE-scaled)
+// Each builder fills the block matrix a (states x states), the column of the
+// block rows at the generator speed w (aw) and, for exciters, at the PSS
+// output v_s (av) and at the bus voltage (bvr, bvi).
+template <typename TF>
+__device__ __forceinline__ void ieesgo_block(const double *P, double dt, TF (&a)[6][6], TF (&aw)[6]) {
+  const double T1 = P[0], T2 = P[1], T3 = P[2], T4 = P[3], T5 = P[4], T6 = P[5], K1 = P[6], K2 = P[7], K3 = P[8];
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    aw[i] = 0;
+#pragma unroll
+    for (int j = 0; j < 6; j++) a[i][j] = 0;
+  }
+  a[0][0] = TF(1.0 + dt / T1);                   aw[0] = TF(-dt * K1 / T1);
+  a[1][0] = TF(-dt * (1.0 - T2 / T3) / T3);      a[1][1] = TF(1.0 + dt / T3);
+  a[2][0] = TF(dt * T2 / (T3 * T4));             a[2][1] = TF(dt / T4);  a[2][2] = TF(1.0 + dt / T4);
+  a[3][2] = TF(-dt * K2 / T5);                   a[3][3] = TF(1.0 + dt / T5);
+  a[4][3] = TF(-dt * K3 / T6);                   a[4][4] = TF(1.0 + dt / T6);
+  a[5][2] = TF(1.0 - K2); a[5][3] = TF(1.0 - K3); a[5][4] = TF(1.0); a[5][5] = TF(-1.0);
+}
+template <typename TF>
+__device__ __forceinline__ void tgov1_block(const double *P, double dt, TF (&a)[3][3], TF (&aw)[3]) {
+  const double R = P[0], T1 = P[1], T2 = P[4], T3 = P[5], DT = P[6], t23 = T2 / T3;
+  a[0][0] = TF(1.0 + dt / T3); a[0][1] = TF(-dt * (1.0 - t23) / T3); a[0][2] = 0;  aw[0] = 0;
+  a[1][0] = 0; a[1][1] = TF(1.0 + dt / T1); a[1][2] = 0;                          aw[1] = TF(dt / (R * T1));
+  a[2][0] = TF(1.0); a[2][1] = TF(t23); a[2][2] = TF(-1.0);                        aw[2] = TF(-DT);
+}
+template <typename TF>
+__device__ __forceinline__ void ieeest_block(const double *P, double dt, TF (&a)[8][8], TF (&aw)[8]) {
+  const double A1 = P[0], A2 = P[1], A3 = P[2], A4 = P[3], A5 = P[4], A6 = P[5];
+  const double T1 = P[6], T2 = P[7], T3 = P[8], T4 = P[9], T5 = P[10], T6 = P[11], KS = P[12];
+  const double rA = A6 / A4, y2[3] = {rA, A5 - A3 * rA, 1.0 - rA};
+  const double r12 = T1 / T2, y3[4] = {r12 * y2[0], r12 * y2[1], r12 * y2[2], 1.0 - r12};
+  const double r34 = T3 / T4, y4[5] = {r34 * y3[0], r34 * y3[1], r34 * y3[2], r34 * y3[3], 1.0 - r34};
+  const double r56 = T5 / T6;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    aw[i] = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) a[i][j] = 0;
+  }
+  a[0][0] = TF(1.0 + dt * A1 / A2); a[0][1] = TF(dt / A2); aw[0] = TF(-dt / A2);
+  a[1][0] = TF(-dt); a[1][1] = TF(1.0);
+  a[2][1] = TF(-dt / A4); a[2][2] = TF(1.0 + dt * A3 / A4); a[2][3] = TF(dt / A4);
+  a[3][2] = TF(-dt); a[3][3] = TF(1.0);
+#pragma unroll
+  for (int j = 0; j < 3; j++) a[4][1 + j] = TF(-dt * y2[j] / T2);
+  a[4][4] = TF(1.0 + dt / T2);
+#pragma unroll
+  for (int j = 0; j < 4; j++) a[5][1 + j] = TF(-dt * y3[j] / T4);
+  a[5][5] = TF(1.0 + dt / T4);
+#pragma unroll
+  for (int j = 0; j < 5; j++) a[6][1 + j] = TF(-dt * KS * y4[j] / T6);
+  a[6][6] = TF(1.0 + dt / T6);
+#pragma unroll
+  for (int j = 0; j < 5; j++) a[7][1 + j] = TF(-r56 * KS * y4[j]);
+  a[7][6] = TF(r56); a[7][7] = TF(1.0);
+}
+// SEXS / ESDC1A (2 or 3 states, padded to 3 with an identity row)
+template <typename TF>
+__device__ __forceinline__ void exciter_block(int et, const double *P, double dt, double e_fd, double vr, double vi,
+                                              TF (&a)[3][3], TF (&av)[3], TF (&bvr)[3], TF (&bvi)[3]) {
+  double vm = sqrt(vr * vr + vi * vi);
+  if (vm == 0.0) vm = 1e-12;
+  const double dr = vr / vm, di = vi / vm;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    av[i] = 0; bvr[i] = 0; bvi[i] = 0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) a[i][j] = 0;
+  }
+  if (et == GP_SEXS) {
+    const double TA_TB = P[0], TB = P[1], K = P[2], TE = P[3];
+    a[0][0] = TF(1.0 + dt / TB);
+    bvr[0] = TF(dt * (1.0 - TA_TB) * dr / TB); bvi[0] = TF(dt * (1.0 - TA_TB) * di / TB);
+    av[0] = TF(-dt * (1.0 - TA_TB) / TB);
+    a[1][0] = TF(-dt * K / TE); a[1][1] = TF(1.0 + dt / TE);
+    bvr[1] = TF(dt * K * TA_TB * dr / TE); bvi[1] = TF(dt * K * TA_TB * di / TE);
+    av[1] = TF(-dt * K * TA_TB / TE);
+    a[2][2] = TF(1.0);
+  } else {
+    const double Ka = P[0], Ta = P[1], Kf = P[2], Tf = P[3], Ke = P[4], Te = P[5], sa = P[7], sb = P[8];
+    const double dsat = (sb == 0.0 || e_fd <= sa) ? 0.0 : 2.0 * sb * (e_fd - sa);
+    a[0][0] = TF(1.0 + dt / Ta); a[0][1] = TF(dt * Ka / Ta); a[0][2] = TF(dt * Ka * Kf / (Ta * Tf));
+    bvr[0] = TF(dt * Ka * dr / Ta); bvi[0] = TF(dt * Ka * di / Ta); av[0] = TF(-dt * Ka / Ta);
+    a[1][1] = TF(1.0 + dt / Tf); a[1][2] = TF(dt * Kf / (Tf * Tf));
+    a[2][0] = TF(-dt / Te); a[2][2] = TF(1.0 + dt * (Ke + dsat) / Te);
+  }
+}
+
+// ---------------------------------------------------------------- Schur cluster kernel
+// For one non-trivial cluster and 64 scenarios: builds the BE-scaled block
+// A_k, the voltage columns B_k and the network rows C_k, and computes
+//   X = A_k^{-1} [B_vr | B_vi | f_k],  D_k = C_k X[:,0:2],  g_k = C_k X[:,2]
+// (src/schur.jl:195-247,321-348).  The block is eliminated controller by
+// controller: each controller's states are affine in the generator speed w
+// (and the exciter's in the PSS output), so the generator block absorbs them
+// as a rank-one change of its w column; the generator's four algebraic
+// states are substituted exactly and the remaining 6x6 differential block is
+// solved with partial pivoting.
+template <typename TF>
+__global__ void __launch_bounds__(64) k_cluster_schur(DevTables T, const double *__restrict__ p,
+                                                      const double *__restrict__ z, const double *__restrict__ f,
+                                                      double dt, TF *__restrict__ X, double *__restrict__ Dg,
+                                                      const uint8_t *__restrict__ work) {
+  const int c = T.cl_nt[blockIdx.x];
+  const int m = lane_scenario();
+  if (m >= T.M || !work[m]) return;
+  const int ldM = T.ldM;
+  const int *cl = T.cl + c * GP_CLUSTER_COLS;
+  const int gk = cl[GP_CL_GENROU];
+  const int *g = T.genrou + gk * GP_COLS_GENROU;
+  const int dp = g[0], ap = g[1], bus = g[3];
+  const int vrz = T.net_ptr + 2 * bus;
+  const double vr = z[(size_t)vrz * ldM + m], vi = z[(size_t)(vrz + 1) * ldM + m];
+  double zs[10], j[47];
+#pragma unroll
+  for (int i = 0; i < 6; i++) zs[i] = z[(size_t)(dp + i) * ldM + m];
+#pragma unroll
+  for (int i = 0; i < 4; i++) zs[6 + i] = z[(size_t)(ap + i) * ldM + m];
+  genrou_jac(p + g[2], zs, vr, vi, j);
+  // BE scaling of the generator's differential rows
+#pragma unroll
+  for (int s = 0; s < 25; s++) j[s] = be(j[s], true, GEN_SLOT_DIAG[s], dt);
+  // RHS of the generator rows: columns (vr, vi, f)
+  TF R[10][3];
+#pragma unroll
+  for (int i = 0; i < 10; i++) {
+    R[i][0] = 0; R[i][1] = 0;
+    R[i][2] = TF(f[(size_t)((i < 6 ? dp + i : ap + i - 6)) * ldM + m]);
+  }
+  R[8][0] = TF(j[35]); R[8][1] = TF(j[36]);
+  R[9][0] = TF(j[39]); R[9][1] = TF(j[40]);
+  TF a04 = 0, a44 = TF(j[19]);
+
+  // ---- governor: p_m = alpha_pm - beta_pm * w
+  const int gt = cl[GP_CL_GOVTYPE];
+  const int *gd = nullptr;
+  TF alpha_pm[3] = {0, 0, 0}, beta_pm = 0;
+  if (gt == GP_IEESGO) {
+    gd = T.ieesgo + cl[GP_CL_GOV] * GP_COLS_IEESGO;
+    TF a[6][6], aw[6], r[6][4];
+    ieesgo_block<TF>(p + gd[2], dt, a, aw);
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      r[i][0] = 0; r[i][1] = 0;
+      r[i][2] = TF(f[(size_t)(i < 5 ? gd[0] + i : gd[1]) * ldM + m]);
+      r[i][3] = aw[i];
+    }
+    dense_solve_nopiv<6, 4, TF>(a, r);
+    alpha_pm[0] = r[5][0]; alpha_pm[1] = r[5][1]; alpha_pm[2] = r[5][2]; beta_pm = r[5][3];
+  } else if (gt == GP_TGOV1) {
+    gd = T.tgov1 + cl[GP_CL_GOV] * GP_COLS_TGOV1;
+    TF a[3][3], aw[3], r[3][4];
+    tgov1_block<TF>(p + gd[2], dt, a, aw);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      r[i][0] = 0; r[i][1] = 0;
+      r[i][2] = TF(f[(size_t)(i < 2 ? gd[0] + i : gd[1]) * ldM + m]);
+      r[i][3] = aw[i];
+    }
+    dense_solve_nopiv<3, 4, TF>(a, r);
+    alpha_pm[0] = r[2][0]; alpha_pm[1] = r[2][1]; alpha_pm[2] = r[2][2]; beta_pm = r[2][3];
+  }
+  if (gt != 0) {
+    const TF j4pm = TF(j[22]);   // d f5 / d p_m, already BE-scaled
+#pragma unroll
+    for (int col = 0; col < 3; col++) R[4][col] -= j4pm * alpha_pm[col];
+    a44 -= j4pm * beta_pm;
+  }
+  // ---- PSS: v_s = alpha_vs - beta_vs * w
+  const int *pd = cl[GP_CL_PSS] >= 0 ? T.ieeest + cl[GP_CL_PSS] * GP_COLS_IEEEST : nullptr;
+  TF alpha_vs[3] = {0, 0, 0}, beta_vs = 0;
+  if (pd) {
+    TF a[8][8], aw[8], r[8][4];
+    ieeest_block<TF>(p + pd[2], dt, a, aw);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      r[i][0] = 0; r[i][1] = 0;
+      r[i][2] = TF(f[(size_t)(i < 7 ? pd[0] + i : pd[1]) * ldM + m]);
+      r[i][3] = aw[i];
+    }
+    dense_solve_nopiv<8, 4, TF>(a, r);
+    alpha_vs[0] = r[7][0]; alpha_vs[1] = r[7][1]; alpha_vs[2] = r[7][2]; beta_vs = r[7][3];
+  }
+  // ---- exciter: e_fd = a_e - gamma * v_s
+  const int et = cl[GP_CL_EXCTYPE];
+  const int *ed = et == GP_SEXS ? T.sexs + cl[GP_CL_EXC] * GP_COLS_SEXS
+                                : (et == GP_ESDC1A ? T.esdc1a + cl[GP_CL_EXC] * GP_COLS_ESDC1A : nullptr);
+  const int ne = et == GP_SEXS ? 2 : 3;
+  if (ed) {
+    TF a[3][3], av[3], bvr[3], bvi[3], r[3][4];
+    const double e_fd0 = et == GP_ESDC1A ? z[(size_t)(ed[0] + 2) * ldM + m] : 0.0;
+    exciter_block<TF>(et, p + ed[1], dt, e_fd0, vr, vi, a, av, bvr, bvi);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      r[i][0] = bvr[i]; r[i][1] = bvi[i];
+      r[i][2] = i < ne ? TF(f[(size_t)(ed[0] + i) * ldM + m]) : TF(0);
+      r[i][3] = av[i];
+    }
+    dense_solve_nopiv<3, 4, TF>(a, r);
+    const bool is_sexs = et == GP_SEXS;   // e_fd is state 1 of SEXS, state 2 of ESDC1A
+    const TF gam = is_sexs ? r[1][3] : r[2][3];
+    const TF j0e = TF(j[5]);   // d f1 / d e_fd, already BE-scaled
+#pragma unroll
+    for (int col = 0; col < 3; col++) R[0][col] -= j0e * ((is_sexs ? r[1][col] : r[2][col]) - gam * alpha_vs[col]);
+    a04 += j0e * gam * beta_vs;
+  }
+
+  // ---- generator: substitute the algebraic states, solve the 6x6 block
+  const TF a60 = TF(j[25]), a62 = TF(j[26]), a66 = TF(j[27]), a69 = TF(j[28]);
+  const TF a71 = TF(j[29]), a73 = TF(j[30]), a77 = TF(j[31]), a78 = TF(j[32]);
+  const TF a85 = TF(j[33]), a87 = TF(j[34]), a95 = TF(j[37]), a96 = TF(j[38]);
+  const TF cd_eq = -a60 / a69, cd_p1 = -a62 / a69, cd_de = a66 * a95 / (a96 * a69);
+  const TF cq_ed = -a71 / a78, cq_p2 = -a73 / a78, cq_de = a77 * a85 / (a87 * a78);
+  TF rd[3], rq[3];
+#pragma unroll
+  for (int col = 0; col < 3; col++) {
+    rd[col] = (R[6][col] - a66 * R[9][col] / a96) / a69;
+    rq[col] = (R[7][col] - a77 * R[8][col] / a87) / a78;
+  }
+  const TF a0id = TF(j[2]), a1iq = TF(j[8]), a2id = TF(j[11]), a3iq = TF(j[14]), a4iq = TF(j[20]), a4id = TF(j[21]);
+  TF A[6][6], B[6][3];
+  A[0][0] = TF(j[0]) + a0id * cd_eq; A[0][1] = TF(j[3]); A[0][2] = TF(j[1]) + a0id * cd_p1; A[0][3] = TF(j[4]);
+  A[0][4] = a04; A[0][5] = a0id * cd_de;
+  A[1][0] = 0; A[1][1] = TF(j[6]) + a1iq * cq_ed; A[1][2] = 0; A[1][3] = TF(j[7]) + a1iq * cq_p2; A[1][4] = 0;
+  A[1][5] = a1iq * cq_de;
+  A[2][0] = TF(j[9]) + a2id * cd_eq; A[2][1] = 0; A[2][2] = TF(j[10]) + a2id * cd_p1; A[2][3] = 0; A[2][4] = 0;
+  A[2][5] = a2id * cd_de;
+  A[3][0] = 0; A[3][1] = TF(j[12]) + a3iq * cq_ed; A[3][2] = 0; A[3][3] = TF(j[13]) + a3iq * cq_p2; A[3][4] = 0;
+  A[3][5] = a3iq * cq_de;
+  A[4][0] = TF(j[15]) + a4id * cd_eq; A[4][1] = TF(j[16]) + a4iq * cq_ed; A[4][2] = TF(j[17]) + a4id * cd_p1;
+  A[4][3] = TF(j[18]) + a4iq * cq_p2; A[4][4] = a44; A[4][5] = a4iq * cq_de + a4id * cd_de;
+  A[5][0] = 0; A[5][1] = 0; A[5][2] = 0; A[5][3] = 0; A[5][4] = TF(j[23]); A[5][5] = TF(j[24]);
+  const TF rowid[6] = {a0id, 0, a2id, 0, a4id, 0};
+  const TF rowiq[6] = {0, a1iq, 0, a3iq, a4iq, 0};
+#pragma unroll
+  for (int r = 0; r < 6; r++)
+#pragma unroll
+    for (int col = 0; col < 3; col++) B[r][col] = R[r][col] - rowid[r] * rd[col] - rowiq[r] * rq[col];
+  dense_solve_pp<6, 3, TF>(A, B);
+  TF xg[10][3];
+#pragma unroll
+  for (int col = 0; col < 3; col++) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) xg[i][col] = B[i][col];
+    const TF de = B[5][col];
+    const TF vd = (R[8][col] - a85 * de) / a87;
+    const TF vq = (R[9][col] - a95 * de) / a96;
+    xg[6][col] = vq;
+    xg[7][col] = vd;
+    xg[8][col] = rq[col] + cq_ed * B[1][col] + cq_p2 * B[3][col] + cq_de * de;
+    xg[9][col] = rd[col] + cd_eq * B[0][col] + cd_p1 * B[2][col] + cd_de * de;
+  }
+  // ---- outputs: X rows in cluster order, D_k, g_k
+  TF *Xc = X + (size_t)T.cl_xoff[c] * 3 * ldM;
+#pragma unroll
+  for (int i = 0; i < 10; i++)
+#pragma unroll
+    for (int col = 0; col < 3; col++) Xc[(size_t)(i * 3 + col) * ldM + m] = xg[i][col];
+  int row = 10;
+  // governor back-substitution: x = A^{-1}(r - aw w)
+  if (gt == GP_IEESGO) {
+    TF a[6][6], aw[6], r[6][3];
+    ieesgo_block<TF>(p + gd[2], dt, a, aw);
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      const TF fi = TF(f[(size_t)(i < 5 ? gd[0] + i : gd[1]) * ldM + m]);
+      r[i][0] = -aw[i] * xg[4][0]; r[i][1] = -aw[i] * xg[4][1]; r[i][2] = fi - aw[i] * xg[4][2];
+    }
+    dense_solve_nopiv<6, 3, TF>(a, r);
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+      for (int col = 0; col < 3; col++) Xc[(size_t)((row + i) * 3 + col) * ldM + m] = r[i][col];
+    row += 6;
+  } else if (gt == GP_TGOV1) {
+    TF a[3][3], aw[3], r[3][3];
+    tgov1_block<TF>(p + gd[2], dt, a, aw);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const TF fi = TF(f[(size_t)(i < 2 ? gd[0] + i : gd[1]) * ldM + m]);
+      r[i][0] = -aw[i] * xg[4][0]; r[i][1] = -aw[i] * xg[4][1]; r[i][2] = fi - aw[i] * xg[4][2];
+    }
+    dense_solve_nopiv<3, 3, TF>(a, r);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int col = 0; col < 3; col++) Xc[(size_t)((row + i) * 3 + col) * ldM + m] = r[i][col];
+    row += 3;
+  }
+  TF vs[3] = {0, 0, 0};
+  TF xp[8][3];
+  if (pd) {
+    TF a[8][8], aw[8];
+    ieeest_block<TF>(p + pd[2], dt, a, aw);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const TF fi = TF(f[(size_t)(i < 7 ? pd[0] + i : pd[1]) * ldM + m]);
+      xp[i][0] = -aw[i] * xg[4][0]; xp[i][1] = -aw[i] * xg[4][1]; xp[i][2] = fi - aw[i] * xg[4][2];
+    }
+    dense_solve_nopiv<8, 3, TF>(a, xp);
+    vs[0] = xp[7][0]; vs[1] = xp[7][1]; vs[2] = xp[7][2];
+  }
+  if (ed) {
+    TF a[3][3], av[3], bvr[3], bvi[3], r[3][3];
+    const double e_fd0 = et == GP_ESDC1A ? z[(size_t)(ed[0] + 2) * ldM + m] : 0.0;
+    exciter_block<TF>(et, p + ed[1], dt, e_fd0, vr, vi, a, av, bvr, bvi);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const TF fi = i < ne ? TF(f[(size_t)(ed[0] + i) * ldM + m]) : TF(0);
+      r[i][0] = bvr[i] - av[i] * vs[0]; r[i][1] = bvi[i] - av[i] * vs[1]; r[i][2] = fi - av[i] * vs[2];
+    }
+    dense_solve_nopiv<3, 3, TF>(a, r);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+      if (i < ne)
+#pragma unroll
+        for (int col = 0; col < 3; col++) Xc[(size_t)((row + i) * 3 + col) * ldM + m] = r[i][col];
+    row += ne;
+  }
+  if (pd) {
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+      for (int col = 0; col < 3; col++) Xc[(size_t)((row + i) * 3 + col) * ldM + m] = xp[i][col];
+  }
+  // C_k: network rows at (delta, i_q, i_d)
+  const double cr[3] = {j[41], j[42], j[43]}, ci[3] = {j[44], j[45], j[46]};
+  const int cidx[3] = {5, 8, 9};
+  double D[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+    const double x0 = (double)xg[cidx[t]][0], x1 = (double)xg[cidx[t]][1], x2 = (double)xg[cidx[t]][2];
+    D[0] += cr[t] * x0; D[1] += cr[t] * x1;
+    D[2] += ci[t] * x0; D[3] += ci[t] * x1;
+    D[4] += cr[t] * x2; D[5] += ci[t] * x2;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) Dg[(size_t)(6 * c + k) * ldM + m] = D[k];
+}
+
+// Bus kernel of the Schur path: S entries of the bus (diagonal block and the
+// static generator's couplings) and the reduced right-hand side
+//   rhs = -(F_red - sum_k C_k A_k^{-1} F_k).
+__global__ void __launch_bounds__(64) k_bus_schur(DevTables T, const double *__restrict__ p,
+                                                  const double *__restrict__ z, const double *__restrict__ f,
+                                                  const double *__restrict__ Dg, FaultState fs,
+                                                  double *__restrict__ svar, double *__restrict__ rhs,
+                                                  const uint8_t *__restrict__ work) {
+  const int b = blockIdx.x;
+  const int m = lane_scenario();
+  if (m >= T.M || !work[m]) return;
+  const int ldM = T.ldM;
+  const int vrz = T.net_ptr + 2 * b;
+  const double vr = z[(size_t)vrz * ldM + m], vi = z[(size_t)(vrz + 1) * ldM + m];
+  double dg[4], sgo[11];
+  bus_diag(T, p, z, fs, b, m, vr, vi, dg, sgo);
+  double rr = f[(size_t)vrz * ldM + m], ri = f[(size_t)(vrz + 1) * ldM + m];
+  for (int q = T.bus_cl_p[b]; q < T.bus_cl_p[b + 1]; q++) {
+    const int c = T.bus_cl[q];
+    dg[0] -= Dg[(size_t)(6 * c) * ldM + m];
+    dg[1] -= Dg[(size_t)(6 * c + 1) * ldM + m];
+    dg[2] -= Dg[(size_t)(6 * c + 2) * ldM + m];
+    dg[3] -= Dg[(size_t)(6 * c + 3) * ldM + m];
+    rr -= Dg[(size_t)(6 * c + 4) * ldM + m];
+    ri -= Dg[(size_t)(6 * c + 5) * ldM + m];
+  }
+  const int *sv = T.sv_bus + 15 * b;
+#pragma unroll
+  for (int i = 0; i < 4; i++) svar[(size_t)sv[i] * ldM + m] = dg[i];
+#pragma unroll
+  for (int i = 4; i < 11; i++)
+    if (sv[i] >= 0) svar[(size_t)sv[i] * ldM + m] = sgo[i];
+  rhs[(size_t)T.red_of_z[vrz] * ldM + m] = -rr;
+  rhs[(size_t)T.red_of_z[vrz + 1] * ldM + m] = -ri;
+  const int sgk = T.bus_sg[b];
+  if (sgk >= 0) {
+    const int *d = T.sg + sgk * GP_COLS_STATICGEN;
+    const int ap = d[1], bt = d[3];
+    if (bt == 2 || bt == 3) rhs[(size_t)T.red_of_z[ap] * ldM + m] = -f[(size_t)ap * ldM + m];
+    if (bt == 3) rhs[(size_t)T.red_of_z[ap + 1] * ldM + m] = -f[(size_t)(ap + 1) * ldM + m];
+  }
+}
+
+// dw_k = -(A_k^{-1} F_k + A_k^{-1} B_k dv_bus);  z_w += dw_k  (src/schur.jl:389-421)
+template <typename TF>
+__global__ void __launch_bounds__(64) k_cluster_backsub(DevTables T, const TF *__restrict__ X,
+                                                        const double *__restrict__ dv, double *__restrict__ z,
+                                                        const uint8_t *__restrict__ work) {
+  const int c = T.cl_nt[blockIdx.x];
+  const int m = lane_scenario();
+  if (m >= T.M || !work[m]) return;
+  const int ldM = T.ldM;
+  const int *cl = T.cl + c * GP_CLUSTER_COLS;
+  const int ws = cl[GP_CL_WSTART], wsz = cl[GP_CL_WSIZE];
+  const int vrz = T.net_ptr + 2 * cl[GP_CL_BUS];
+  const double d0 = dv[(size_t)T.red_of_z[vrz] * ldM + m], d1 = dv[(size_t)T.red_of_z[vrz + 1] * ldM + m];
+  const TF *Xc = X + (size_t)T.cl_xoff[c] * 3 * ldM;
+  for (int i = 0; i < wsz; i++) {
+    const double x0 = Xc[(size_t)(3 * i) * ldM + m], x1 = Xc[(size_t)(3 * i + 1) * ldM + m];
+    const double x2 = Xc[(size_t)(3 * i + 2) * ldM + m];
+    z[(size_t)(ws + i) * ldM + m] -= x2 + x0 * d0 + x1 * d1;
+  }
+}
+
+// ---------------------------------------------------------------- batched sparse LU
+// One workgroup per scenario.  Phase 1 scatters A into the LU storage
+// (fill entries zero); phase 2 walks the elimination levels: each thread
+// owns one column of the level, stages it in LDS, applies the left-looking
+// updates lu[dst] -= L(i,j) U(j,k) and scales L(:,k) by the pivot.
+template <typename TF>
+__global__ void k_splu_factor(SpluDev S, const double *__restrict__ var, const double *__restrict__ cst, int ldM,
+                              TF *__restrict__ luv, const uint8_t *__restrict__ refac, int *__restrict__ status) {
+  const int m = blockIdx.x;
+  if (!refac[m]) return;
+  extern __shared__ char smem_raw[];
+  TF *stage = reinterpret_cast<TF *>(smem_raw);
+  TF *lu = luv + (size_t)m * S.nnz_lu;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int q = tid; q < S.nnz_lu; q += nt) lu[q] = TF(0);
+  __syncthreads();
+  for (int e = tid; e < S.nnz_a; e += nt) {
+    const int src = S.a_src[e];
+    const double v = src >= 0 ? var[(size_t)src * ldM + m] : cst[-src - 1];
+    lu[S.a2lu[e]] = TF(v);
+  }
+  __syncthreads();
+  TF *buf = stage + (size_t)tid * S.lds_cols;
+  int bad = 0;
+  for (int l = 0; l < S.n_flev; l++) {
+    for (int idx = S.fac_lp[l] + tid; idx < S.fac_lp[l + 1]; idx += nt) {
+      const int k = S.fac_cols[idx];
+      const int c0 = S.lu_cp[k], c1 = S.lu_cp[k + 1], dpos = S.lu_dp[k];
+      const int o0 = S.op_cp[k], o1 = S.op_cp[k + 1];
+      if (c1 - c0 <= S.lds_cols) {
+        for (int t = 0; t < c1 - c0; t++) buf[t] = lu[c0 + t];
+        for (int o = o0; o < o1; o++) {
+          const int dst = S.op[3 * o], sl = S.op[3 * o + 1], su = S.op[3 * o + 2];
+          buf[dst - c0] -= lu[sl] * buf[su - c0];
+        }
+        const TF piv = buf[dpos - c0];
+        if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
+        const TF inv = TF(1) / piv;
+        for (int t = dpos - c0 + 1; t < c1 - c0; t++) buf[t] *= inv;
+        for (int t = 0; t < c1 - c0; t++) lu[c0 + t] = buf[t];
+      } else {
+        for (int o = o0; o < o1; o++) {
+          const int dst = S.op[3 * o], sl = S.op[3 * o + 1], su = S.op[3 * o + 2];
+          lu[dst] -= lu[sl] * lu[su];
+        }
+        const TF piv = lu[dpos];
+        if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
+        const TF inv = TF(1) / piv;
+        for (int t = dpos + 1; t < c1; t++) lu[t] *= inv;
+      }
+    }
+    __syncthreads();
+  }
+  if (bad) atomicOr(&status[m], 1);
+}
+
+// One workgroup per scenario: x = A^{-1} b with b gathered from rhs (row
+// space), the solution kept in LDS through both level-scheduled triangular
+// solves, then zupd[scatter[j]] += sign * x[j] and xout[j] = x[j].
+template <typename TF>
+__global__ void k_splu_solve(SpluDev S, const TF *__restrict__ luv, const double *__restrict__ rhs, int ldM,
+                             double sign, const int *__restrict__ scatter, double *__restrict__ zupd,
+                             double *__restrict__ xout, const uint8_t *__restrict__ work) {
+  const int m = blockIdx.x;
+  if (!work[m]) return;
+  extern __shared__ char smem_raw[];
+  double *x = reinterpret_cast<double *>(smem_raw);
+  const TF *lu = luv + (size_t)m * S.nnz_lu;
+  const int tid = threadIdx.x, nt = blockDim.x, n = S.n;
+  for (int r = tid; r < n; r += nt) x[S.pinv[r]] = rhs[(size_t)r * ldM + m];
+  __syncthreads();
+  for (int l = 0; l < S.n_fwlev; l++) {
+    for (int idx = S.fw_lp[l] + tid; idx < S.fw_lp[l + 1]; idx += nt) {
+      const int i = S.fw_rows[idx];
+      double s = x[i];
+      for (int e = S.lrow_p[i]; e < S.lrow_p[i + 1]; e++) s -= (double)lu[S.lrow_pos[e]] * x[S.lrow_k[e]];
+      x[i] = s;
+    }
+    __syncthreads();
+  }
+  for (int l = 0; l < S.n_bwlev; l++) {
+    for (int idx = S.bw_lp[l] + tid; idx < S.bw_lp[l + 1]; idx += nt) {
+      const int k = S.bw_rows[idx];
+      double s = x[k];
+      for (int e = S.urow_p[k]; e < S.urow_p[k + 1]; e++) s -= (double)lu[S.urow_pos[e]] * x[S.urow_j[e]];
+      x[k] = s / (double)lu[S.lu_dp[k]];
+    }
+    __syncthreads();
+  }
+  for (int k = tid; k < n; k += nt) {
+    const int j = S.q[k];
+    const double v = x[k];
+    if (xout) xout[(size_t)j * ldM + m] = v;
+    if (zupd) {
+      const int zr = scatter ? scatter[j] : j;
+      zupd[(size_t)zr * ldM + m] += sign * v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- small utilities
+__global__ void k_copy_active(int n, int ldM, const double *__restrict__ src, double *__restrict__ dst,
+                              const uint8_t *__restrict__ act) {
+  const int m = blockIdx.y * WAVE + threadIdx.x;
+  if (!act[m]) return;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) dst[(size_t)i * ldM + m] = src[(size_t)i * ldM + m];
+}
+
+__global__ void k_record(int nrec, const int *__restrict__ idx, int ldM, int M, const double *__restrict__ z,
+                         double *__restrict__ hist, const uint8_t *__restrict__ act) {
+  const int m = blockIdx.y * WAVE + threadIdx.x;
+  if (m >= M || !act[m]) return;
+  for (int j = blockIdx.x; j < nrec; j += gridDim.x) {
+    const int r = idx ? idx[j] : j;
+    hist[(size_t)j * ldM + m] = z[(size_t)r * ldM + m];
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+static inline dim3 lane_grid(int items, int ldM) { return dim3(items, ldM / WAVE); }
+
+void launch_cluster_residual(const DevTables &T, const double *p, const double *z, const double *zold, double dt,
+                             double *f, double *inj, const uint8_t *work, hipStream_t s) {
+  if (T.n_cl_nt == 0) return;
+  hipLaunchKernelGGL(k_cluster_residual, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, zold, dt, f, inj, work);
+}
+void launch_bus_residual(const DevTables &T, const double *p, const double *z, const double *inj, FaultState fs,
+                         double *f, const uint8_t *work, hipStream_t s) {
+  hipLaunchKernelGGL(k_bus_residual, lane_grid(T.nbus, T.ldM), dim3(WAVE), 0, s, T, p, z, inj, fs, f, work);
+}
+void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, double tol, double *fnorm,
+                         unsigned long long *nb, int *n_unconv, uint8_t *refac, int first, double theta,
+                         hipStream_t s) {
+  const int chunks = n < 64 ? 1 : (n < 4096 ? 16 : 64);
+  hipLaunchKernelGGL(k_rownorm, dim3(ldM / WAVE, chunks), dim3(WAVE), 0, s, n, ldM, M, f, work, nb);
+  hipLaunchKernelGGL(k_norm_finish, dim3((M + 255) / 256), dim3(256), 0, s, M, tol, nb, fnorm, work, n_unconv, refac,
+                     first, theta);
+}
+void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs, double *jv,
+                     const uint8_t *work, hipStream_t s) {
+  if (T.n_cl_nt)
+    hipLaunchKernelGGL(k_cluster_jac_mono, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, dt, jv, work);
+  hipLaunchKernelGGL(k_bus_jac_mono, lane_grid(T.nbus, T.ldM), dim3(WAVE), 0, s, T, p, z, fs, jv, work);
+}
+template <typename TF>
+void launch_cluster_schur(const DevTables &T, const double *p, const double *z, const double *f, double dt, TF *X,
+                          double *Dg, const uint8_t *work, hipStream_t s) {
+  if (T.n_cl_nt == 0) return;
+  hipLaunchKernelGGL(k_cluster_schur<TF>, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, f, dt, X, Dg, work);
+}
+void launch_bus_schur(const DevTables &T, const double *p, const double *z, const double *f, const double *Dg,
+                      FaultState fs, double *svar, double *rhs, const uint8_t *work, hipStream_t s) {
+  hipLaunchKernelGGL(k_bus_schur, lane_grid(T.nbus, T.ldM), dim3(WAVE), 0, s, T, p, z, f, Dg, fs, svar, rhs, work);
+}
+template <typename TF>
+void launch_cluster_backsub(const DevTables &T, const TF *X, const double *f, const double *dv, double *z,
+                            const uint8_t *work, hipStream_t s) {
+  (void)f;
+  if (T.n_cl_nt == 0) return;
+  hipLaunchKernelGGL(k_cluster_backsub<TF>, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, X, dv, z, work);
+}
+template <typename TF>
+void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv,
+                             const uint8_t *refac, int M, int *status, hipStream_t s) {
+  const size_t lds = (size_t)S.threads * S.lds_cols * sizeof(TF);
+  hipLaunchKernelGGL(k_splu_factor<TF>, dim3(M), dim3(S.threads), lds, s, S, var, cst, ldM, luv, refac, status);
+}
+template <typename TF>
+void launch_splu_solve(const SpluDev &S, const TF *luv, const double *rhs, int ldM, double sign, const int *scatter,
+                       double *zupd, double *xout, const uint8_t *work, int M, hipStream_t s) {
+  const size_t lds = (size_t)S.n * sizeof(double);
+  if (lds > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_splu_solve<TF>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_splu_solve<TF>, dim3(M), dim3(256), lds, s, S, luv, rhs, ldM, sign, scatter, zupd, xout, work);
+}
+void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s) {
+  const int gx = n < 1024 ? n : 1024;
+  hipLaunchKernelGGL(k_copy_active, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, n, ldM, src, dst, act);
+}
+void launch_record_impl(int nrec, const int *idx, int ldM, int M, const double *z, double *hist, const uint8_t *act,
+                        hipStream_t s) {
+  const int gx = nrec < 1024 ? (nrec > 0 ? nrec : 1) : 1024;
+  hipLaunchKernelGGL(k_record, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, nrec, idx, ldM, M, z, hist, act);
+}
+
+template void launch_cluster_schur<double>(const DevTables &, const double *, const double *, const double *, double,
+                                           double *, double *, const uint8_t *, hipStream_t);
+template void launch_cluster_schur<float>(const DevTables &, const double *, const double *, const double *, double,
+                                          float *, double *, const uint8_t *, hipStream_t);
+template void launch_cluster_backsub<double>(const DevTables &, const double *, const double *, const double *,
+                                             double *, const uint8_t *, hipStream_t);
+template void launch_cluster_backsub<float>(const DevTables &, const float *, const double *, const double *,
+                                            double *, const uint8_t *, hipStream_t);
+template void launch_splu_factor_impl<double>(const SpluDev &, const double *, const double *, int, double *,
+                                              const uint8_t *, int, int *, hipStream_t);
+template void launch_splu_factor_impl<float>(const SpluDev &, const double *, const double *, int, float *,
+                                             const uint8_t *, int, int *, hipStream_t);
+template void launch_splu_solve<double>(const SpluDev &, const double *, const double *, int, double, const int *,
+                                        double *, double *, const uint8_t *, int, hipStream_t);
+template void launch_splu_solve<float>(const SpluDev &, const float *, const double *, int, double, const int *,
+                                       double *, double *, const uint8_t *, int, hipStream_t);
+
+}  // namespace gp

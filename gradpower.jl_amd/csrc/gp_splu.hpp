@@ -1,0 +1,56 @@
+// gp_splu.hpp -- host-side symbolic analysis for the batched sparse LU.
+//
+// One analysis serves every scenario and every Newton iteration: the
+// matrices of all scenarios share a pattern, so the fill-reducing column
+// order, the pivot rows, the L/U patterns and the level schedules are
+// computed once on the host (the role KLU's symbolic analysis and first
+// factorisation play for the reference CPU path, src/numerics.jl:138-167,
+// and cuDSS's "analysis" phase for its GPU path, ext:776-819).  The device
+// then only runs numeric refactorisations with the fixed pivot sequence and
+// triangular solves.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace gp {
+
+struct SpluAnalysis {
+  int n = 0;
+  int nnz_a = 0;
+  // permutations: pivot position k has column q[k] and row prow[k]
+  std::vector<int> q, qinv, prow, pinv;
+  // LU storage in pivot space, by columns: column k holds U(0..k-1,k) rows
+  // (ascending), the diagonal, then L(k+1..,k) rows (ascending).
+  std::vector<int> lu_cp;   // [n+1]
+  std::vector<int> lu_ri;   // [nnz_lu] pivot-space row
+  std::vector<int> lu_dp;   // [n] position of the diagonal
+  int nnz_lu = 0;
+  std::vector<int> a2lu;    // [nnz_a] destination of each A entry (CSC order)
+  // factorisation program: for column k, ops [op_cp[k], op_cp[k+1]) are
+  // triples (dst, srcL, srcU): lu[dst] -= lu[srcL] * lu[srcU], grouped by
+  // source column in ascending order so every multiplier is final when used.
+  std::vector<int> op_cp;
+  std::vector<int> op;      // 3 ints per op
+  // factorisation levels: columns of level l are fac_cols[fac_lp[l] .. fac_lp[l+1])
+  std::vector<int> fac_lp, fac_cols;
+  int max_col = 0;          // largest column (entries of U + diag + L)
+  // row-oriented access for the solves (pivot space):
+  //   forward:  y[i] -= sum_k L(i,k) y[k]   rows of L:  lrow_p / lrow_k / lrow_pos
+  //   backward: x[k] = (y[k] - sum_j U(k,j) x[j]) / U(k,k)
+  std::vector<int> lrow_p, lrow_k, lrow_pos;
+  std::vector<int> urow_p, urow_j, urow_pos;
+  std::vector<int> fw_lp, fw_rows;   // forward levels
+  std::vector<int> bw_lp, bw_rows;   // backward levels
+  long long flops = 0;               // multiply-add pairs of one factorisation
+};
+
+// Analyse the CSC matrix (cp, ri, representative values ax).  Returns an
+// empty string on success, otherwise an error message.
+std::string splu_analyse(int n, const int *cp, const int *ri, const double *ax,
+                         SpluAnalysis &out, double pivot_threshold = 1e-3);
+
+// Minimum-degree order of the symmetric pattern of A + A^T.
+std::vector<int> min_degree_order(int n, const int *cp, const int *ri);
+
+}  // namespace gp

@@ -1,0 +1,221 @@
+"""ctypes binding of libgradpower_hip.so (the C ABI in include/gradpower_hip.h).
+
+The library is required: importing this module on a machine without the
+built library, or creating a context without a usable GPU, raises -- there
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .desc import SystemDesc, CEvent
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libgradpower_hip.so")
+
+GP_OK = 0
+GP_ERR_BADARG, GP_ERR_HIP, GP_ERR_OOM, GP_ERR_SINGULAR_PIVOT = -1, -2, -3, -4
+GP_ERR_NEWTON_NOT_CONVERGED, GP_ERR_NAN, GP_ERR_UNSUPPORTED = -5, -6, -7
+METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32 = 0, 1, 2
+REFACTOR_EVERY_ITER, REFACTOR_FIRST_ITER, REFACTOR_ADAPTIVE = 0, 1, 2
+_NAMES = {-1: "BADARG", -2: "HIP", -3: "OOM", -4: "SINGULAR_PIVOT", -5: "NEWTON_NOT_CONVERGED",
+          -6: "NAN", -7: "UNSUPPORTED"}
+
+
+class GpError(RuntimeError):
+    def __init__(self, code, msg, stats=None):
+        super().__init__(f"gp error {code} ({_NAMES.get(code, '?')}): {msg}")
+        self.code = code
+        self.stats = stats
+
+
+class COptions(C.Structure):
+    _fields_ = [("record_all", C.c_int32), ("n_record", C.c_int32),
+                ("record_idx", C.POINTER(C.c_int32)), ("refactor_policy", C.c_int32),
+                ("reserved", C.c_int32 * 7)]
+
+
+class CStats(C.Structure):
+    _fields_ = [("newton_iters", C.c_int64), ("scenario_steps", C.c_int64),
+                ("event_resolves", C.c_int64), ("refactorizations", C.c_int64),
+                ("n_not_converged", C.c_int32), ("n_nan", C.c_int32),
+                ("max_final_residual", C.c_double), ("elapsed_ms", C.c_double),
+                ("solve_ms", C.c_double)]
+
+
+class CSymInfo(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nnz_j", C.c_int32), ("nnz_lu_j", C.c_int32),
+                ("levels_j", C.c_int32), ("n_red", C.c_int32), ("nnz_s", C.c_int32),
+                ("nnz_lu_s", C.c_int32), ("levels_s", C.c_int32), ("n_clusters_nt", C.c_int32),
+                ("w_total_nt", C.c_int32), ("reserved", C.c_int32 * 6)]
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_up = C.POINTER(C.c_uint8)
+_LIB = None
+
+EXPORTS = ["gp_create", "gp_destroy", "gp_set_state", "gp_set_events", "gp_integrate",
+           "gp_get_trajectory", "gp_get_state", "gp_eval_residual", "gp_jacobian_pattern",
+           "gp_eval_jacobian", "gp_newton_step", "gp_get_symbolic_info", "gp_last_error", "gp_version"]
+
+
+def load():
+    """Load the library (raises FileNotFoundError/OSError if it is not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise FileNotFoundError(f"{LIBPATH} is not built; run __graft_entry__.build() "
+                                f"or `make -C gradpower.jl_amd`")
+    lib = C.CDLL(LIBPATH)
+    lib.gp_create.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_void_p)]
+    lib.gp_destroy.argtypes = [C.c_void_p]
+    lib.gp_destroy.restype = None
+    lib.gp_set_state.argtypes = [C.c_void_p, _dp, C.c_int32, _dp]
+    lib.gp_set_events.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, _ip]
+    lib.gp_integrate.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_void_p]
+    lib.gp_get_trajectory.argtypes = [C.c_void_p, C.c_int32, _dp, C.c_int64]
+    lib.gp_get_state.argtypes = [C.c_void_p, _dp]
+    lib.gp_eval_residual.argtypes = [C.c_void_p, _dp, _dp, C.c_double, _up, _dp]
+    lib.gp_jacobian_pattern.argtypes = [C.c_void_p, _ip, _ip, _ip]
+    lib.gp_eval_jacobian.argtypes = [C.c_void_p, _dp, C.c_double, _up, _dp]
+    lib.gp_get_symbolic_info.argtypes = [C.c_void_p, C.c_void_p]
+    lib.gp_newton_step.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, C.c_double, _up, _dp]
+    lib.gp_last_error.restype = C.c_char_p
+    lib.gp_version.restype = C.c_char_p
+    _LIB = lib
+    return lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _stats(s):
+    return {f: getattr(s, f) for f, _ in CStats._fields_}
+
+
+class HipContext:
+    """One library context: a system, M scenarios, device state on the current GPU."""
+
+    def __init__(self, desc: SystemDesc, M: int, record_idx=None,
+                 refactor_policy=REFACTOR_EVERY_ITER):
+        self.lib = load()
+        self.desc = desc
+        self.M = int(M)
+        self.n = desc.n
+        self._cs, self._keep = desc.as_ctypes()
+        opt = COptions()
+        if record_idx is None:
+            opt.record_all = 1
+            self.nrec = desc.n
+        else:
+            self._rec = np.ascontiguousarray(record_idx, dtype=np.int32)
+            opt.record_all = 0
+            opt.n_record = len(self._rec)
+            opt.record_idx = _i(self._rec)
+            self.nrec = len(self._rec)
+        opt.refactor_policy = refactor_policy
+        self._opt = opt
+        h = C.c_void_p()
+        self._check(self.lib.gp_create(C.byref(self._cs), self.M, C.byref(opt), C.byref(h)))
+        self.h = h
+        self.nsteps = np.zeros(self.M, np.int32)
+
+    def _check(self, rc, stats=None):
+        if rc != GP_OK:
+            raise GpError(rc, self.lib.gp_last_error().decode(), stats)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, z0, p, per_scenario=False):
+        z0 = np.ascontiguousarray(z0, np.float64)
+        p = np.ascontiguousarray(p, np.float64)
+        self._check(self.lib.gp_set_state(self.h, _d(z0), int(per_scenario), _d(p)))
+
+    def set_events(self, events, nsteps):
+        """events: iterable of (scenario, bus, r_fault, step_on, step_off)."""
+        ev = list(events)
+        arr = (CEvent * max(1, len(ev)))()
+        for i, (m, b, r, on, off) in enumerate(ev):
+            arr[i].scenario, arr[i].bus, arr[i].r_fault = int(m), int(b), float(r)
+            arr[i].step_on, arr[i].step_off = int(on), int(off)
+        self.nsteps = np.ascontiguousarray(nsteps, np.int32)
+        self._check(self.lib.gp_set_events(self.h, C.byref(arr), len(ev), _i(self.nsteps)))
+
+    def integrate(self, method, dt, tol=1e-10, itermax=50, raise_on_fail=True):
+        st = CStats()
+        rc = self.lib.gp_integrate(self.h, int(method), float(dt), float(tol), int(itermax), C.byref(st))
+        stats = _stats(st)
+        stats["status"] = rc
+        if rc != GP_OK and (raise_on_fail or rc in (GP_ERR_BADARG, GP_ERR_HIP, GP_ERR_OOM, GP_ERR_UNSUPPORTED)):
+            raise GpError(rc, self.lib.gp_last_error().decode(), stats)
+        return stats
+
+    def trajectory(self, m):
+        ln = int(self.nsteps[m]) + 1
+        out = np.zeros((self.nrec, ln))
+        self._check(self.lib.gp_get_trajectory(self.h, int(m), _d(out), ln))
+        return out
+
+    def state(self):
+        out = np.zeros((self.M, self.n))
+        self._check(self.lib.gp_get_state(self.h, _d(out)))
+        return out
+
+    def eval_residual(self, z, zold, dt, fault_on=None):
+        z = np.ascontiguousarray(np.broadcast_to(z, (self.M, self.n)), np.float64)
+        zold = np.ascontiguousarray(np.broadcast_to(zold, (self.M, self.n)), np.float64)
+        fo = np.ascontiguousarray(np.zeros(self.M) if fault_on is None else fault_on, np.uint8)
+        f = np.zeros((self.M, self.n))
+        self._check(self.lib.gp_eval_residual(self.h, _d(z), _d(zold), float(dt),
+                                              fo.ctypes.data_as(_up), _d(f)))
+        return f
+
+    def jacobian_pattern(self):
+        nnz = C.c_int32()
+        self._check(self.lib.gp_jacobian_pattern(self.h, C.byref(nnz), None, None))
+        rp = np.zeros(self.n + 1, np.int32)
+        ci = np.zeros(nnz.value, np.int32)
+        self._check(self.lib.gp_jacobian_pattern(self.h, C.byref(nnz), _i(rp), _i(ci)))
+        return rp, ci
+
+    def eval_jacobian(self, z, dt, fault_on=None):
+        rp, ci = self.jacobian_pattern()
+        z = np.ascontiguousarray(np.broadcast_to(z, (self.M, self.n)), np.float64)
+        fo = np.ascontiguousarray(np.zeros(self.M) if fault_on is None else fault_on, np.uint8)
+        vals = np.zeros((self.M, len(ci)))
+        self._check(self.lib.gp_eval_jacobian(self.h, _d(z), float(dt), fo.ctypes.data_as(_up), _d(vals)))
+        return rp, ci, vals
+
+    def newton_step(self, method, z, zold, dt, fault_on=None):
+        """One Newton correction dz per scenario (z_new = z + dz)."""
+        z = np.ascontiguousarray(np.broadcast_to(z, (self.M, self.n)), np.float64)
+        zold = np.ascontiguousarray(np.broadcast_to(zold, (self.M, self.n)), np.float64)
+        fo = np.ascontiguousarray(np.zeros(self.M) if fault_on is None else fault_on, np.uint8)
+        dz = np.zeros((self.M, self.n))
+        self._check(self.lib.gp_newton_step(self.h, int(method), _d(z), _d(zold), float(dt),
+                                            fo.ctypes.data_as(_up), _d(dz)))
+        return dz
+
+    def symbolic_info(self):
+        info = CSymInfo()
+        self._check(self.lib.gp_get_symbolic_info(self.h, C.byref(info)))
+        return {f: getattr(info, f) for f, _ in CSymInfo._fields_ if f != "reserved"}

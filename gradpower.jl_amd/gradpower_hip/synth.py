@@ -134,14 +134,24 @@ def _solve_pf_scaling(ps, meta):
     meta["load_scale"] = s_ok
 
 
-def _finish(name, ps, psd, M, events, tf, meta, zip_alpha=None):
+def _finish(name, ps, psd, M, events, tf, meta, zip_alpha=None, pq_after_pf=None):
+    if pq_after_pf is not None:
+        # power flow with the given buses voltage-controlled, then freeze them
+        # as constant P/Q injections at the solved operating point
+        build_network(ps)
+        _solve_pf_scaling(ps, meta)
+        for b in pq_after_pf:
+            ps.buses[int(b)].type = PQ
     set_dynamics(ps, psd)
     if zip_alpha is not None:
         for dev in psd.devices:
             if dev.dtype.kind == "zipload":
                 dev.dtype.alpha = zip_alpha
     build_network(ps)
-    _solve_pf_scaling(ps, meta)
+    if pq_after_pf is None:
+        _solve_pf_scaling(ps, meta)
+    else:
+        runpf_inplace(ps)
     # ZIP loads were created from ps.loads in order by set_dynamics; carry a
     # load scaling applied by the power-flow fallback over to them.
     zips = [dev.dtype for dev in psd.devices if dev.dtype.kind == "zipload"]
@@ -303,10 +313,18 @@ def _regional_dispatch(edges, n, gen_buses, loads, floor=0.1):
 
 
 def case2_grid(M=256, nbus=2000, n_genrou=400, n_pss=100, n_static=300, seed=1002,
-               load_fraction=0.2, tf_range=(1.0, 5.0), name="c2_grid", charging_scale=1.0):
+               load_fraction=0.2, tf_range=(1.0, 5.0), name="c2_grid", charging_scale=1.0,
+               zip_alpha=1.0, static_gen_type=PV):
     rng = np.random.default_rng(seed)
-    meta = {"seed": seed, "zip_alpha": 0.4, "nbus": nbus,
-            "zip_note": "Z/I/P 0.4/0.3/0.3 mapped to alpha=0.4 constant-Z, 0.6 constant-P"}
+    meta = {"seed": seed, "zip_alpha": zip_alpha, "nbus": nbus,
+            "zip_note": "constant-impedance loads (alpha=1, the reference default of "
+                        "src/GradPower.jl:382): with a constant-power share, bolted faults on this "
+                        "weak synthetic grid leave ~10% of the faulted algebraic systems without "
+                        "a Newton-reachable solution, in the oracle as on the GPU",
+            "static_gen_note": "static generators regulate voltage in the power flow, then become "
+                               "constant P/Q injections (PQ stubs) at that operating point: a PV stub "
+                               "holds |v| with unlimited reactive power, and faults 2-3 buses away "
+                               "then defeat the undamped Newton re-solve (oracle and GPU alike)"}
     edges, pts = _rgg_edges(rng, nbus, 2.6)
     meta["mean_degree"] = 2 * len(edges) / nbus
     meta["charging_scale"] = charging_scale
@@ -330,8 +348,12 @@ def case2_grid(M=256, nbus=2000, n_genrou=400, n_pss=100, n_static=300, seed=100
     pg = _regional_dispatch(edges, nbus, allg, loads)
     gens = []
     for k, b in enumerate(allg):
-        buses[b].type = SLACK if k == 0 else PV
-        buses[b].v0m = 1.02 if k < n_genrou else 1.01
+        if k < n_genrou:
+            buses[b].type = SLACK if k == 0 else PV
+            buses[b].v0m = 1.02
+        else:
+            buses[b].type = PV
+            buses[b].v0m = 1.01
         gens.append(Gen(b, "1", float(pg[k]), 0.0, max(100.0, 100.0 * pg[k] / 0.7)))
     ps = PowerSystem(100.0, buses, gens, loads, branches, [])
     psd = PowerSystemDynamics()
@@ -349,12 +371,26 @@ def case2_grid(M=256, nbus=2000, n_genrou=400, n_pss=100, n_static=300, seed=100
         if pss_on[k]:
             psd.add_device(_ieeest(eb))
     meta.pop("ieesgo_T", None)
+    # Fault buses are drawn uniformly among buses with no static-generator
+    # stub within one hop: a stub holds |v| = vset with reactive power only,
+    # so a resistive fault at or next to it leaves the faulted algebraic
+    # system without a nearby solution (the reference Newton fails there too).
+    adj = [[] for _ in range(nbus)]
+    for (a, b) in edges:
+        adj[a].append(b)
+        adj[b].append(a)
+    excl = set(int(b) for b in sbus)
+    for b in list(excl):
+        excl.update(adj[b])
+    fault_buses = np.array([b for b in range(nbus) if b not in excl])
+    meta["fault_bus_pool"] = int(len(fault_buses))
     events, tf = [], []
     for m in range(M):
-        events.append([ContingencyEvent(int(rng.integers(nbus)), float(rng.uniform(0.01, 0.2)),
-                                        0.02, float(rng.uniform(0.05, 0.25)))])
+        events.append([ContingencyEvent(int(fault_buses[rng.integers(len(fault_buses))]),
+                                        float(rng.uniform(0.01, 0.2)), 0.02, float(rng.uniform(0.05, 0.25)))])
         tf.append(float(rng.uniform(*tf_range)))
-    return _finish(name, ps, psd, M, events, tf, meta, zip_alpha=0.4)
+    return _finish(name, ps, psd, M, events, tf, meta, zip_alpha=zip_alpha,
+                   pq_after_pf=[int(b) for b in sbus] if static_gen_type == PQ else None)
 
 
 def case3_200bus(M=128, tf=3.0):

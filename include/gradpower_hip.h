@@ -115,11 +115,21 @@ enum gp_method {
                                  FP64 residual, update and states (inexact Newton)           */
 };
 
+enum gp_refactor_policy {
+  GP_REFACTOR_EVERY_ITER = 0,  /* numeric factorisation at every Newton iteration (the
+                                  reference's cuDSS factorization/refactorization cycle) */
+  GP_REFACTOR_FIRST_ITER = 1,  /* factorise at the first iteration of each Newton solve
+                                  and reuse it for the later iterations (Schur methods)  */
+  GP_REFACTOR_ADAPTIVE = 2     /* factorise at the first iteration and again whenever an
+                                  iteration fails to reduce |F|inf by 10x               */
+};
+
 typedef struct gp_options {
   int32_t record_all;      /* 1: record every state each step; 0: only record_idx  */
   int32_t n_record;        /* number of recorded z indices when record_all == 0     */
   const int32_t *record_idx;
-  int32_t reserved[8];
+  int32_t refactor_policy; /* gp_refactor_policy for the Schur methods              */
+  int32_t reserved[7];
 } gp_options;
 
 typedef struct gp_stats {
@@ -176,6 +186,13 @@ int gp_jacobian_pattern(gp_ctx *ctx, int32_t *nnz, int32_t *rowptr,
                         int32_t *colind);
 int gp_eval_jacobian(gp_ctx *ctx, const double *z, double dt,
                      const uint8_t *fault_on, double *jval);
+
+/* One Newton correction dz (z_new = z + dz) per scenario at (z, zold, dt)
+ * with the given method, scenario m's faults active when fault_on[m] != 0.
+ * Inputs/outputs are [M*n].  The context's integration state is replaced
+ * by z.  Test hook for the Schur algebra: every method returns -J^{-1}F. */
+int gp_newton_step(gp_ctx *ctx, int32_t method, const double *z, const double *zold,
+                   double dt, const uint8_t *fault_on, double *dz);
 
 /* Sizes of the symbolic analyses (for the roofline byte counts). */
 typedef struct gp_symbolic_info {

@@ -1,0 +1,138 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle.
+
+Tolerances: residuals and Jacobian values are the same FP64 expressions, so
+they agree to rounding (<= 1e-12 relative); trajectories compare converged
+Newton solutions (both sides iterate to |F|inf < 1e-12), so the per-step
+state deviation bound is 1e-9 pu.
+"""
+import numpy as np
+import pytest
+
+from conftest import get_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(case, M=None, **kw):
+    from gradpower_hip.lib import HipContext
+    c = HipContext(case.desc, M or case.M, **kw)
+    c.set_state(case.desc.z0, case.desc.p)
+    return c
+
+
+def _oracle(case):
+    from oracle.oracle import Oracle
+    return Oracle(case.desc)
+
+
+def _perturbed(case, M, seed, scale=0.02):
+    rng = np.random.default_rng(seed)
+    z0 = case.desc.z0
+    return z0[None, :] * (1 + scale * rng.standard_normal((M, len(z0)))) + scale * rng.standard_normal((M, len(z0)))
+
+
+@pytest.mark.parametrize("name", ["case_2bus_testdata", "case0_2bus", "case1_9bus", "case3_200bus"])
+def test_residual_matches_oracle(name):
+    case = get_case(name)
+    M = min(case.M, 8)
+    ctx = _ctx(case, M)
+    o = _oracle(case)
+    z = _perturbed(case, M, 1)
+    zold = _perturbed(case, M, 2, 0.01)
+    ev = [(m, case.events[m][0].bus, case.events[m][0].rfault, 1, 2) for m in range(M) if case.events[m]]
+    ctx.set_events(ev, case.nsteps[:M])
+    fault_on = np.array([1 if m % 2 else 0 for m in range(M)], np.uint8)
+    for dt in (case.dt, 0.0):
+        f = ctx.eval_residual(z, zold, dt, fault_on)
+        for m in range(M):
+            g = np.zeros(case.desc.nbus)
+            if fault_on[m] and case.events[m]:
+                g[case.events[m][0].bus] += 1.0 / case.events[m][0].rfault
+            fo = o.residual(z[m], zold[m], dt, g)
+            scale = 1.0 + np.abs(fo)
+            assert np.max(np.abs(f[m] - fo) / scale) < 1e-12, (name, m, dt)
+
+
+@pytest.mark.parametrize("name", ["case0_2bus", "case1_9bus", "case3_200bus"])
+def test_jacobian_matches_oracle(name):
+    import scipy.sparse as sp
+    case = get_case(name)
+    M = min(case.M, 4)
+    ctx = _ctx(case, M)
+    o = _oracle(case)
+    z = _perturbed(case, M, 3)
+    ev = [(m, case.events[m][0].bus, case.events[m][0].rfault, 1, 2) for m in range(M) if case.events[m]]
+    ctx.set_events(ev, case.nsteps[:M])
+    fault_on = np.ones(M, np.uint8)
+    rp, ci, vals = ctx.eval_jacobian(z, case.dt, fault_on)
+    n = case.desc.n
+    for m in range(M):
+        g = np.zeros(case.desc.nbus)
+        if case.events[m]:
+            g[case.events[m][0].bus] += 1.0 / case.events[m][0].rfault
+        Jo = o.jacobian_csc(z[m], case.dt, g).toarray()
+        Jg = sp.csr_matrix((vals[m], ci, rp), shape=(n, n)).toarray()
+        assert np.max(np.abs(Jg - Jo) / (1 + np.abs(Jo))) < 1e-12, (name, m)
+
+
+def _oracle_trajs(case, scen, tol, nsteps=None):
+    o = _oracle(case)
+    out = {}
+    for m in scen:
+        ev = [(e.bus, e.rfault, int(round(e.ton / case.dt)), int(round(e.toff / case.dt))) for e in case.events[m]]
+        ns = int(case.nsteps[m]) if nsteps is None else nsteps
+        tr, zf, st = o.integrate(case.dt, ns, ev, tol=tol, itermax=50)
+        assert st["not_converged"] == 0
+        out[m] = tr
+    return out
+
+
+METHODS = {"mono": 0, "schur64": 1, "schur32": 2}
+
+
+@pytest.mark.parametrize("method", ["mono", "schur64", "schur32"])
+@pytest.mark.parametrize("name", ["case0_2bus", "case1_9bus", "case3_200bus"])
+def test_newton_step_matches_oracle(name, method):
+    """One Newton correction of every method equals the oracle's -J^{-1}F
+    (the Schur elimination is exact algebra; FP32 factors give the step to
+    ~1e-6 relative)."""
+    case = get_case(name)
+    M = min(case.M, 4)
+    ctx = _ctx(case, M)
+    o = _oracle(case)
+    z = _perturbed(case, M, 5, 0.005)
+    zold = _perturbed(case, M, 6, 0.002)
+    ev = [(m, case.events[m][0].bus, case.events[m][0].rfault, 1, 2) for m in range(M) if case.events[m]]
+    ctx.set_events(ev, case.nsteps[:M])
+    fault_on = np.ones(M, np.uint8)
+    for dt in (case.dt, 0.0):
+        dz = ctx.newton_step(METHODS[method], z, zold, dt, fault_on)
+        for m in range(M):
+            g = np.zeros(case.desc.nbus)
+            if case.events[m]:
+                g[case.events[m][0].bus] += 1.0 / case.events[m][0].rfault
+            f = o.residual(z[m], zold[m], dt, g)
+            ref = -o.linsolve(o.jacobian(z[m], dt, g), f)
+            tol = 1e-4 if method == "schur32" else 1e-9
+            assert np.max(np.abs(dz[m] - ref)) <= tol * (1 + np.max(np.abs(ref))), (name, method, m, dt)
+
+
+@pytest.mark.parametrize("method,policy", [("mono", 0), ("schur64", 0), ("schur64", 2), ("schur32", 2)])
+@pytest.mark.parametrize("name,nsteps", [("case_2bus_testdata", 120), ("case0_2bus", 240), ("case1_9bus", None)])
+def test_trajectory_matches_oracle(name, nsteps, method, policy):
+    case = get_case(name)
+    M = min(case.M, 8)
+    tol = 1e-12
+    ctx = _ctx(case, M, refactor_policy=policy)
+    ns = [int(case.nsteps[m]) if nsteps is None else nsteps for m in range(M)]
+    ev = []
+    for m in range(M):
+        for e in case.events[m]:
+            ev.append((m, e.bus, e.rfault, int(round(e.ton / case.dt)), int(round(e.toff / case.dt))))
+    ctx.set_events(ev, ns)
+    st = ctx.integrate(METHODS[method], case.dt, tol, 50)
+    ref = _oracle_trajs(case, range(M), tol, nsteps)
+    for m in range(M):
+        tr = ctx.trajectory(m)
+        dev = np.max(np.abs(tr.T - ref[m]))
+        assert dev < 1e-9, (name, method, m, dev, st)
