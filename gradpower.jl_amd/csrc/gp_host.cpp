@@ -53,6 +53,10 @@ struct DBuf {
       throw GpError{GP_ERR_OOM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
     }
   }
+  // allocate unless a buffer of exactly this size is already held
+  void ensure(size_t count) {
+    if (!p || n != count) alloc(count);
+  }
   void upload(const std::vector<T> &v) {
     alloc(v.size() ? v.size() : 1);
     if (v.size()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
@@ -68,36 +72,54 @@ struct DBuf {
 struct LUPlan {
   SpluAnalysis A;
   SpluDev dev{};
-  DBuf<int> a2lu, a_src, lu_cp, lu_dp, lu_ri, op_cp, op, fac_lp, fac_cols, lrow_p, lrow_k, lrow_pos, urow_p, urow_j,
-      urow_pos, fw_lp, fw_rows, bw_lp, bw_rows, q, pinv;
+  DBuf<int> a2lu, a_src, lu_cp, lu_dp, op_cp, op_loc, op_sl, st_cp, st_o, fac_lp, fac_cols, fac_mode, lu2s, ls_k,
+      us_j, fw_lp, fw_info, fw_g, bw_lp, bw_info, bw_g, q, pinv;
   bool ready = false;
   double dt = -1.0;
+  static constexpr int kThreads = 512;
+  static constexpr int kLdsBudget = 120 * 1024;
+
+  static std::vector<int> nonempty(const std::vector<int> &v) { return v.empty() ? std::vector<int>{0} : v; }
 
   void upload(const std::vector<int> &asrc) {
-    a2lu.upload(A.a2lu); a_src.upload(asrc); lu_cp.upload(A.lu_cp); lu_dp.upload(A.lu_dp); lu_ri.upload(A.lu_ri);
-    op_cp.upload(A.op_cp); op.upload(A.op.empty() ? std::vector<int>{0, 0, 0} : A.op);
-    fac_lp.upload(A.fac_lp); fac_cols.upload(A.fac_cols);
-    lrow_p.upload(A.lrow_p); lrow_k.upload(A.lrow_k); lrow_pos.upload(A.lrow_pos);
-    urow_p.upload(A.urow_p); urow_j.upload(A.urow_j); urow_pos.upload(A.urow_pos);
-    fw_lp.upload(A.fw_lp); fw_rows.upload(A.fw_rows); bw_lp.upload(A.bw_lp); bw_rows.upload(A.bw_rows);
+    splu_schedule(A, kThreads);
+    a2lu.upload(A.a2lu); a_src.upload(asrc); lu_cp.upload(A.lu_cp); lu_dp.upload(A.lu_dp);
+    op_cp.upload(A.op_cp); op_loc.upload(nonempty(A.op_loc)); op_sl.upload(nonempty(A.op_sl));
+    st_cp.upload(A.st_cp); st_o.upload(A.st_o);
+    fac_lp.upload(A.fac_lp); fac_cols.upload(A.fac_cols); fac_mode.upload(nonempty(A.fac_mode));
+    lu2s.upload(A.lu2s); ls_k.upload(nonempty(A.ls_k)); us_j.upload(nonempty(A.us_j));
+    fw_lp.upload(A.fw_lp); fw_info.upload(A.fw_info); fw_g.upload(nonempty(A.fw_g));
+    bw_lp.upload(A.bw_lp); bw_info.upload(A.bw_info); bw_g.upload(nonempty(A.bw_g));
     q.upload(A.q); pinv.upload(A.pinv);
     SpluDev &d = dev;
-    d.n = A.n; d.nnz_a = A.nnz_a; d.nnz_lu = A.nnz_lu; d.max_col = A.max_col;
+    d.n = A.n; d.nnz_a = A.nnz_a; d.nnz_lu = A.nnz_lu; d.nnz_l = A.nnz_l; d.nnz_u = A.nnz_u;
     d.n_flev = (int)A.fac_lp.size() - 1; d.n_fwlev = (int)A.fw_lp.size() - 1; d.n_bwlev = (int)A.bw_lp.size() - 1;
-    d.a2lu = a2lu.p; d.a_src = a_src.p; d.lu_cp = lu_cp.p; d.lu_dp = lu_dp.p; d.lu_ri = lu_ri.p;
-    d.op_cp = op_cp.p; d.op = op.p; d.fac_lp = fac_lp.p; d.fac_cols = fac_cols.p;
-    d.lrow_p = lrow_p.p; d.lrow_k = lrow_k.p; d.lrow_pos = lrow_pos.p;
-    d.urow_p = urow_p.p; d.urow_j = urow_j.p; d.urow_pos = urow_pos.p;
-    d.fw_lp = fw_lp.p; d.fw_rows = fw_rows.p; d.bw_lp = bw_lp.p; d.bw_rows = bw_rows.p;
+    d.a2lu = a2lu.p; d.a_src = a_src.p; d.lu_cp = lu_cp.p; d.lu_dp = lu_dp.p;
+    d.op_cp = op_cp.p; d.op_loc = op_loc.p; d.op_sl = op_sl.p; d.st_cp = st_cp.p; d.st_o = st_o.p;
+    d.fac_lp = fac_lp.p; d.fac_cols = fac_cols.p; d.fac_mode = fac_mode.p;
+    d.lu2s = lu2s.p; d.ls_k = ls_k.p; d.us_j = us_j.p;
+    d.fw_lp = fw_lp.p; d.fw_info = fw_info.p; d.fw_g = fw_g.p;
+    d.bw_lp = bw_lp.p; d.bw_info = bw_info.p; d.bw_g = bw_g.p;
     d.q = q.p; d.pinv = pinv.p;
+    d.threads = kThreads;
   }
+  // LDS staging sizes for element size `elem`; columns (or op lists) larger
+  // than the staging buffers run in place in global memory.
   void shape(size_t elem) {
-    int cols = 1;
-    while (cols < A.max_col && cols < 64) cols *= 2;
-    int threads = (int)(65536 / (cols * elem));
-    threads = std::min(256, std::max(64, threads / 64 * 64));
-    dev.threads = threads;
-    dev.lds_cols = cols;
+    const int W = kThreads / 64;
+    int tc = std::min<long long>(A.t_cols_max, (long long)kLdsBudget / ((long long)kThreads * (long long)elem));
+    int wc = A.w_col_max, wo = A.w_ops_max;
+    auto wbytes = [&](int c, int o) { return (int)(((size_t)c * elem + (size_t)o * (elem + 4) + 15) / 16 * 16); };
+    while ((wc > 0 || wo > 0) && (long long)W * wbytes(wc, wo) > kLdsBudget) {
+      wc = wc * 3 / 4;
+      wo = wo * 3 / 4;
+    }
+    dev.t_cols = tc;
+    dev.w_col = wc;
+    dev.w_ops = wo;
+    dev.w_bytes = wbytes(wc, wo);
+    dev.lds_factor = std::max<int>((int)((size_t)kThreads * tc * elem), W * dev.w_bytes);
+    dev.lds_solve = (int)((size_t)A.n * sizeof(double));
   }
 };
 
@@ -141,9 +163,10 @@ struct gp_ctx {
   DBuf<int> d_rec_idx;
   DBuf<double> d_yv, d_u0, d_jp_yoff_val, d_ydiag, d_s_cst;
   DBuf<double> d_p, d_z, d_zold, d_f, d_inj, d_jv, d_svar, d_rhs, d_dv, d_Dg, d_fnorm, d_hist, d_luJ, d_luS64;
-  DBuf<float> d_luS32, d_X32;
+  DBuf<double> d_svJ, d_svS64;
+  DBuf<float> d_luS32, d_svS32, d_X32;
   DBuf<double> d_X64;
-  DBuf<uint8_t> d_work, d_act, d_refac, d_sel;
+  DBuf<uint8_t> d_work, d_act, d_refac, d_refac0, d_sel;
   DBuf<int> d_fbus, d_nunconv, d_status;
   DBuf<double> d_fg;
   DBuf<unsigned long long> d_nb;
@@ -152,6 +175,7 @@ struct gp_ctx {
   bool state_set = false;
   int hist_steps = 0;
   std::vector<int> traj_len;
+  std::vector<int> scen_status, scen_steps;   // per scenario, from the last gp_integrate
 };
 
 namespace {
@@ -499,7 +523,7 @@ void alloc_state(gp_ctx *C) {
   C->d_inj.alloc((size_t)std::max(1, 2 * C->ncl) * L);
   C->d_Dg.alloc((size_t)std::max(1, 6 * C->ncl) * L);
   C->d_fnorm.alloc(L);
-  C->d_work.alloc(L); C->d_act.alloc(L); C->d_refac.alloc(L); C->d_sel.alloc(L);
+  C->d_work.alloc(L); C->d_act.alloc(L); C->d_refac.alloc(L); C->d_refac0.alloc(L); C->d_sel.alloc(L);
   C->d_fbus.alloc((size_t)MAXFAULT * L);
   C->d_fg.alloc((size_t)MAXFAULT * L);
   C->d_nunconv.alloc(2);
@@ -546,6 +570,7 @@ void analyse(gp_ctx *C, int method, double dt) {
     C->luJ.ready = true;
     C->luJ.dt = dt;
     C->d_luJ.alloc((size_t)C->luJ.A.nnz_lu * C->M);
+    C->d_svJ.alloc((size_t)C->luJ.A.nnz_lu * C->M);
   } else {
     if (C->luS.ready && C->luS.dt == dt) return;
     const int nnzS = C->scp[C->nred];
@@ -580,11 +605,13 @@ void analyse(gp_ctx *C, int method, double dt) {
 void ensure_schur_buffers(gp_ctx *C, bool fp32) {
   const size_t L = C->ldM;
   if (fp32) {
-    if (!C->d_X32.p) C->d_X32.alloc((size_t)std::max(1, C->xrows) * 3 * L);
-    if (!C->d_luS32.p) C->d_luS32.alloc((size_t)C->luS.A.nnz_lu * C->M);
+    C->d_X32.ensure((size_t)std::max(1, C->xrows) * 3 * L);
+    C->d_luS32.ensure((size_t)C->luS.A.nnz_lu * C->M);
+    C->d_svS32.ensure((size_t)C->luS.A.nnz_lu * C->M);
     C->luS.shape(sizeof(float));
   } else {
-    if (!C->d_luS64.p) C->d_luS64.alloc((size_t)C->luS.A.nnz_lu * C->M);
+    C->d_luS64.ensure((size_t)C->luS.A.nnz_lu * C->M);
+    C->d_svS64.ensure((size_t)C->luS.A.nnz_lu * C->M);
     C->luS.shape(sizeof(double));
   }
 }
@@ -598,10 +625,13 @@ struct NewtonOut {
 // newton_step! (src/numerics.jl:55-176) for every scenario in `mask`, in lock
 // step: each iteration evaluates F, drops the scenarios with |F|inf < tol,
 // and applies one correction to the rest.
-NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, const std::vector<uint8_t> &mask) {
+NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, const std::vector<uint8_t> &mask,
+                 const std::vector<uint8_t> *refac0 = nullptr) {
   NewtonOut out;
   const int L = C->ldM;
   set_mask(C, C->d_work, mask);
+  const bool reuse = refac0 && method != GP_METHOD_MONO_FP64 && C->opt.refactor_policy == GP_REFACTOR_ADAPTIVE;
+  if (reuse) set_mask(C, C->d_refac0, *refac0);
   const double *p = C->d_p.p;
   double *z = C->d_z.p;
   int nun = 0;
@@ -615,7 +645,7 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
                              ? 0.0
                              : (C->opt.refactor_policy == GP_REFACTOR_ADAPTIVE ? 10.0 : 1e300);
     launch_rownorm_impl(C->n, C->M, L, C->d_f.p, C->d_work.p, tol, C->d_fnorm.p, C->d_nb.p, C->d_nunconv.p,
-                        C->d_refac.p, it == 0, theta, C->stream);
+                        C->d_refac.p, reuse ? C->d_refac0.p : nullptr, it == 0, theta, C->stream);
     int cnt[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(cnt, C->d_nunconv.p, 2 * sizeof(int), hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
@@ -625,9 +655,9 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
     out.refac += cnt[1];
     if (method == GP_METHOD_MONO_FP64) {
       launch_jac_mono(C->T, p, z, dt, faults(C), C->d_jv.p, C->d_work.p, C->stream);
-      launch_splu_factor_impl<double>(C->luJ.dev, C->d_jv.p, nullptr, L, C->d_luJ.p,
+      launch_splu_factor_impl<double>(C->luJ.dev, C->d_jv.p, nullptr, L, C->d_luJ.p, C->d_svJ.p,
                                       C->d_refac.p, C->M, C->d_status.p, C->stream);
-      launch_splu_solve<double>(C->luJ.dev, C->d_luJ.p, C->d_f.p, L, -1.0, nullptr, z, nullptr, C->d_work.p, C->M,
+      launch_splu_solve<double>(C->luJ.dev, C->d_svJ.p, C->d_f.p, L, -1.0, nullptr, z, nullptr, C->d_work.p, C->M,
                                 C->stream);
     } else {
       const bool fp32 = method == GP_METHOD_SCHUR_FP32;
@@ -637,18 +667,20 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
         launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, C->d_work.p, C->stream);
       }
       launch_bus_schur(C->T, p, z, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_work.p, C->stream);
-      if (fp32)
-        launch_splu_factor_impl<float>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS32.p, C->d_refac.p, C->M,
+      if (cnt[1] == 0) {
+        // every scenario keeps its factorisation
+      } else if (fp32)
+        launch_splu_factor_impl<float>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS32.p, C->d_svS32.p, C->d_refac.p, C->M,
                                        C->d_status.p, C->stream);
       else
-        launch_splu_factor_impl<double>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_refac.p, C->M,
+        launch_splu_factor_impl<double>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p, C->d_refac.p, C->M,
                                         C->d_status.p, C->stream);
       if (fp32) {
-        launch_splu_solve<float>(C->luS.dev, C->d_luS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z, C->d_dv.p,
+        launch_splu_solve<float>(C->luS.dev, C->d_svS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z, C->d_dv.p,
                                  C->d_work.p, C->M, C->stream);
         launch_cluster_backsub<float>(C->T, C->d_X32.p, C->d_f.p, C->d_dv.p, z, C->d_work.p, C->stream);
       } else {
-        launch_splu_solve<double>(C->luS.dev, C->d_luS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z, C->d_dv.p,
+        launch_splu_solve<double>(C->luS.dev, C->d_svS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z, C->d_dv.p,
                                   C->d_work.p, C->M, C->stream);
         launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, C->d_work.p, C->stream);
       }
@@ -808,7 +840,13 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
     std::vector<std::vector<std::pair<int, double>>> active(M);   // (bus, g) per scenario, in event order
     std::vector<std::vector<const gp_event *>> active_ev(M);
     std::vector<uint8_t> act(L, 0), dead(L, 0), everfail(L, 0), res(L, 0);
+    // factorisation reuse (GP_REFACTOR_ADAPTIVE): a scenario refactorises at
+    // the first iteration of a Newton solve when its stored factorisation was
+    // made with another dt or before a fault event (or it has none yet)
+    std::vector<uint8_t> stale(L, 1), need0(L, 0), need_res(L, 1);
     for (int m = 0; m < M; m++) act[m] = 1;
+    C->scen_status.assign(M, GP_OK);
+    C->scen_steps.assign(C->nsteps.begin(), C->nsteps.begin() + M);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -823,13 +861,23 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
       if (nact == 0) break;
       set_mask(C, C->d_act, act);
       launch_copy_active(C->n, L, C->d_z.p, C->d_zold.p, C->d_act.p, C->stream);
-      NewtonOut no = newton(C, method, dt, tol, itermax, act);
+      for (int m = 0; m < L; m++) need0[m] = act[m] && stale[m];
+      NewtonOut no = newton(C, method, dt, tol, itermax, act, &need0);
+      for (int m = 0; m < M; m++)
+        if (act[m]) stale[m] = 0;
       S.newton_iters += no.corrections;
       S.refactorizations += no.refac;
       S.scenario_steps += nact;
       if (!no.failed.empty())
+        for (int m = 0; m < M; m++) S.scenario_steps -= no.failed[m] ? 1 : 0;
+      if (!no.failed.empty())
         for (int m = 0; m < M; m++)
-          if (no.failed[m]) everfail[m] = 1;
+          if (no.failed[m]) {
+            everfail[m] = 1;
+            dead[m] = 1;   // a scenario whose Newton solve failed is stopped and reported
+            C->scen_status[m] = GP_ERR_NEWTON_NOT_CONVERGED;
+            C->scen_steps[m] = k - 1;
+          }
       double *hist_k = C->d_hist.p + (size_t)k * C->nrec * L;
       launch_record_impl(C->nrec, C->rec_idx.empty() ? nullptr : C->d_rec_idx.p, L, M, C->d_z.p, hist_k, C->d_act.p,
                          C->stream);
@@ -877,12 +925,19 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
         // re-solve at dt = 0: differential states frozen, algebraic states re-solved
         set_mask(C, C->d_sel, res);
         launch_copy_active(C->n, L, C->d_z.p, C->d_zold.p, C->d_sel.p, C->stream);
-        NewtonOut nr = newton(C, method, 0.0, tol, itermax, res);
+        NewtonOut nr = newton(C, method, 0.0, tol, itermax, res, &res);
+        for (int m = 0; m < M; m++)
+          if (res[m]) stale[m] = 1;   // factorised at dt = 0: the next step refactorises
         S.newton_iters += nr.corrections;
         S.refactorizations += nr.refac;
         if (!nr.failed.empty())
           for (int m = 0; m < M; m++)
-            if (nr.failed[m]) everfail[m] = 1;
+            if (nr.failed[m]) {
+              everfail[m] = 1;
+              dead[m] = 1;
+              C->scen_status[m] = GP_ERR_NEWTON_NOT_CONVERGED;
+              C->scen_steps[m] = k;
+            }
       }
       // non-finite states stop a scenario
       if (!no.failed.empty()) {
@@ -892,6 +947,7 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
           if (no.failed[m] && !std::isfinite(fn[m])) {
             dead[m] = 1;
             S.n_nan++;
+            C->scen_status[m] = GP_ERR_NAN;
           }
       }
     }
@@ -907,7 +963,11 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
     std::vector<int> status(L);
     HIPCHK(hipMemcpy(status.data(), C->d_status.p, L * sizeof(int), hipMemcpyDeviceToHost));
     int nsing = 0;
-    for (int m = 0; m < M; m++) nsing += status[m] != 0;
+    for (int m = 0; m < M; m++)
+      if (status[m] != 0) {
+        nsing++;
+        C->scen_status[m] = GP_ERR_SINGULAR_PIVOT;
+      }
     HIPCHK(hipMemset(C->d_status.p, 0, L * sizeof(int)));
     if (st) *st = S;
     if (nsing) return fail(GP_ERR_SINGULAR_PIVOT, std::to_string(nsing) + " scenario(s) hit a zero or non-finite pivot");
@@ -937,6 +997,16 @@ int gp_get_trajectory(gp_ctx *C, int32_t m, double *out, int64_t ld) {
       for (int k = 0; k < len; k++) out[(size_t)j * ld + k] = tmp[(size_t)k * C->nrec + j];
   } catch (const GpError &e) {
     return fail(e.code, e.msg);
+  }
+  return GP_OK;
+}
+
+int gp_get_scenario_status(gp_ctx *C, int32_t *status, int32_t *steps_done) {
+  if (!C) return fail(GP_ERR_BADARG, "gp_get_scenario_status: null context");
+  if (C->scen_status.empty()) return fail(GP_ERR_BADARG, "gp_get_scenario_status: nothing integrated yet");
+  for (int m = 0; m < C->M; m++) {
+    if (status) status[m] = C->scen_status[m];
+    if (steps_done) steps_done[m] = C->scen_steps[m];
   }
   return GP_OK;
 }

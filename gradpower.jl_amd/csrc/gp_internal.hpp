@@ -57,17 +57,21 @@ struct FaultState {
 };
 
 struct SpluDev {
-  int n, nnz_a, nnz_lu, max_col, n_flev, n_fwlev, n_bwlev;
+  int n, nnz_a, nnz_lu, nnz_l, nnz_u, n_flev, n_fwlev, n_bwlev;
   const int *a2lu, *a_src;      // per A entry: LU position, value source
-  const int *lu_cp, *lu_dp, *lu_ri;
-  const int *op_cp, *op;
-  const int *fac_lp, *fac_cols;
-  const int *lrow_p, *lrow_k, *lrow_pos;
-  const int *urow_p, *urow_j, *urow_pos;
-  const int *fw_lp, *fw_rows, *bw_lp, *bw_rows;
+  const int *lu_cp, *lu_dp;     // column starts, diagonal positions
+  const int *op_cp, *op_loc, *op_sl;   // update program (compact form)
+  const int *st_cp, *st_o;      // wavefront-mode steps
+  const int *fac_lp, *fac_cols, *fac_mode;
+  const int *lu2s;              // LU position -> solve-stream slot
+  const int *ls_k, *us_j;       // column of each L / U stream entry
+  const int *fw_lp, *fw_info, *fw_g;   // forward levels: slots (row, begin, end), lanes per row
+  const int *bw_lp, *bw_info, *bw_g;
   const int *q, *pinv;
-  int threads, lds_cols;        // launch shape of the factor kernel
-  // host-side level counts for launches
+  int threads;                  // workgroup size of both kernels
+  int t_cols;                   // LDS staging per thread (thread mode), 0 = none
+  int w_col, w_ops, w_bytes;    // LDS staging per wavefront (wavefront mode)
+  int lds_factor, lds_solve;    // dynamic LDS bytes
 };
 
 // ---- kernel launchers (gp_kernels.hip) ----------------------------------
@@ -76,8 +80,8 @@ void launch_cluster_residual(const DevTables &T, const double *p, const double *
 void launch_bus_residual(const DevTables &T, const double *p, const double *z, const double *inj,
                          FaultState fs, double *f, const uint8_t *work, hipStream_t s);
 void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, double tol, double *fnorm,
-                         unsigned long long *nb, int *n_unconv, uint8_t *refac, int first, double theta,
-                         hipStream_t s);
+                         unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
+                         double theta, hipStream_t s);
 void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs,
                      double *jv, const uint8_t *work, hipStream_t s);
 template <typename TF>
@@ -90,10 +94,10 @@ template <typename TF>
 void launch_cluster_backsub(const DevTables &T, const TF *X, const double *f, const double *dv,
                             double *z, const uint8_t *work, hipStream_t s);
 template <typename TF>
-void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv,
+void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv, TF *sv,
                              const uint8_t *refac, int M, int *status, hipStream_t s);
 template <typename TF>
-void launch_splu_solve(const SpluDev &S, const TF *luv, const double *rhs, int ldM, double sign,
+void launch_splu_solve(const SpluDev &S, const TF *sv, const double *rhs, int ldM, double sign,
                        const int *scatter, double *zupd, double *xout, const uint8_t *work, int M,
                        hipStream_t s);
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s);

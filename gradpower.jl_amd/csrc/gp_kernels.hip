@@ -370,21 +370,32 @@ __global__ void __launch_bounds__(64) k_bus_residual(DevTables T, const double *
 // Max-norm of F per scenario.  Grid (ldM/64, chunks); partial maxima are
 // combined with an integer atomicMax on the bit pattern (|x| >= 0, so the
 // integer order is the float order; NaN patterns sort above +inf).
-__global__ void __launch_bounds__(64) k_rownorm(int n, int ldM, int M, const double *__restrict__ f,
-                                                const uint8_t *__restrict__ work, unsigned long long *__restrict__ nb) {
-  const int m = blockIdx.x * WAVE + threadIdx.x;
-  if (m >= M || !work[m]) return;
+// |F|inf per scenario: block (x: 64 scenarios, y: a chunk of rows), its 4
+// wavefronts stride over the chunk's rows; the per-lane maxima meet in LDS
+// and one atomicMax per scenario and block publishes them.  |F| >= 0, so the
+// IEEE bit patterns order like the values.
+__global__ void __launch_bounds__(256) k_rownorm(int n, int ldM, int M, const double *__restrict__ f,
+                                                 const uint8_t *__restrict__ work,
+                                                 unsigned long long *__restrict__ nb) {
+  __shared__ unsigned long long red[4][WAVE];
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
+  const int m = blockIdx.x * WAVE + lane;
+  const bool on = m < M && work[m];
   const int chunk = (n + gridDim.y - 1) / gridDim.y;
   const int r0 = blockIdx.y * chunk, r1 = min(n, r0 + chunk);
-  double mx = 0.0;
   unsigned long long bits = 0;
-  for (int i = r0; i < r1; i++) {
-    const double a = fabs(f[(size_t)i * ldM + m]);
-    const unsigned long long u = __double_as_longlong(a);
-    bits = u > bits ? u : bits;
+  if (on)
+    for (int i = r0 + w; i < r1; i += 4) {
+      const unsigned long long u = __double_as_longlong(fabs(f[(size_t)i * ldM + m]));
+      bits = u > bits ? u : bits;
+    }
+  red[w][lane] = bits;
+  __syncthreads();
+  if (w == 0 && on) {
+    unsigned long long v = red[0][lane];
+    for (int q = 1; q < 4; q++) v = red[q][lane] > v ? red[q][lane] : v;
+    if (v) atomicMax(&nb[m], v);
   }
-  (void)mx;
-  atomicMax(&nb[m], bits);
 }
 
 // Finishes the max-norm: converged scenarios leave `work`; for the others,
@@ -393,7 +404,7 @@ __global__ void __launch_bounds__(64) k_rownorm(int n, int ldM, int M, const dou
 // since the previous iteration.
 __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict__ nb, double *__restrict__ fnorm,
                               uint8_t *__restrict__ work, int *__restrict__ n_unconv, uint8_t *__restrict__ refac,
-                              int first, double theta) {
+                              const uint8_t *__restrict__ refac0, int first, double theta) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
   if (!work[m]) {
@@ -409,7 +420,7 @@ __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict_
     refac[m] = 0;
   } else {
     atomicAdd(n_unconv, 1);
-    const int rf = (first || theta <= 0.0 || !(v * theta < prev)) ? 1 : 0;
+    const int rf = first ? (refac0 ? refac0[m] : 1) : ((theta <= 0.0 || !(v * theta < prev)) ? 1 : 0);
     refac[m] = rf;
     if (rf) atomicAdd(n_unconv + 1, 1);
   }
@@ -1108,20 +1119,72 @@ __global__ void __launch_bounds__(64) k_cluster_backsub(DevTables T, const TF *_
 }
 
 // ---------------------------------------------------------------- batched sparse LU
-// One workgroup per scenario.  Phase 1 scatters A into the LU storage
-// (fill entries zero); phase 2 walks the elimination levels: each thread
-// owns one column of the level, stages it in LDS, applies the left-looking
-// updates lu[dst] -= L(i,j) U(j,k) and scales L(:,k) by the pivot.
+// One workgroup per scenario.  Workgroups are dispatched to the 8 XCDs round
+// robin; the map below gives the workgroups of one XCD consecutive scenarios,
+// so the scenario-major vectors they gather from share that XCD's L2 lines.
+#ifdef GP_LU_TRACE
+// Level timestamps of workgroup 0 (microbenchmark builds only).
+__device__ long long gp_lu_trace[2][4096];
+#define GP_TRACE(which, idx)                                             \
+  do {                                                                   \
+    if (threadIdx.x == 0 && blockIdx.x == 0 && (idx) < 4096)             \
+      gp_lu_trace[which][idx] = (long long)__builtin_readcyclecounter(); \
+  } while (0)
+#else
+#define GP_TRACE(which, idx) \
+  do {                       \
+  } while (0)
+#endif
+
+__device__ __forceinline__ int scenario_of_block(int b, int nb) {
+  if (nb & 7) return b;
+  return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+// Compiler barrier plus wavefront-scope ordering: LDS operations of one
+// wavefront execute in order, so this is all one lane needs to see the
+// writes other lanes of its wavefront issued before.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Thread mode: one thread runs the whole update program of column k on x
+// (the column staged in LDS, or in place in global memory), 4 ops at a time
+// with their operand loads issued together.  The L multipliers it reads
+// belong to earlier columns and are final.
+template <typename TF, typename XP>
+__device__ __forceinline__ void col_ops_thread(const SpluDev &S, const TF *__restrict__ lu, XP x, int o0, int o1) {
+  int o = o0;
+  for (; o + 4 <= o1; o += 4) {
+    int loc[4];
+    TF lv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      loc[u] = S.op_loc[o + u];
+      lv[u] = lu[S.op_sl[o + u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) x[loc[u] & 0xffff] -= lv[u] * x[loc[u] >> 16];
+  }
+  for (; o < o1; o++) {
+    const int loc = S.op_loc[o];
+    x[loc & 0xffff] -= lu[S.op_sl[o]] * x[loc >> 16];
+  }
+}
+
 template <typename TF>
-__global__ void k_splu_factor(SpluDev S, const double *__restrict__ var, const double *__restrict__ cst, int ldM,
-                              TF *__restrict__ luv, const uint8_t *__restrict__ refac, int *__restrict__ status) {
-  const int m = blockIdx.x;
+__global__ void __launch_bounds__(512) k_splu_factor(SpluDev S, const double *__restrict__ var,
+                                                     const double *__restrict__ cst, int ldM, TF *__restrict__ luv,
+                                                     TF *__restrict__ svv, const uint8_t *__restrict__ refac,
+                                                     int *__restrict__ status) {
+  const int m = scenario_of_block(blockIdx.x, gridDim.x);
   if (!refac[m]) return;
-  extern __shared__ char smem_raw[];
-  TF *stage = reinterpret_cast<TF *>(smem_raw);
+  extern __shared__ __align__(16) char smem_raw[];
   TF *lu = luv + (size_t)m * S.nnz_lu;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int q = tid; q < S.nnz_lu; q += nt) lu[q] = TF(0);
+  TF *so = svv + (size_t)m * S.nnz_lu;
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+  for (int p = tid; p < S.nnz_lu; p += nt) lu[p] = TF(0);
   __syncthreads();
   for (int e = tid; e < S.nnz_a; e += nt) {
     const int src = S.a_src[e];
@@ -1129,72 +1192,169 @@ __global__ void k_splu_factor(SpluDev S, const double *__restrict__ var, const d
     lu[S.a2lu[e]] = TF(v);
   }
   __syncthreads();
-  TF *buf = stage + (size_t)tid * S.lds_cols;
+  GP_TRACE(0, 0);
   int bad = 0;
   for (int l = 0; l < S.n_flev; l++) {
-    for (int idx = S.fac_lp[l] + tid; idx < S.fac_lp[l + 1]; idx += nt) {
-      const int k = S.fac_cols[idx];
-      const int c0 = S.lu_cp[k], c1 = S.lu_cp[k + 1], dpos = S.lu_dp[k];
-      const int o0 = S.op_cp[k], o1 = S.op_cp[k + 1];
-      if (c1 - c0 <= S.lds_cols) {
-        for (int t = 0; t < c1 - c0; t++) buf[t] = lu[c0 + t];
-        for (int o = o0; o < o1; o++) {
-          const int dst = S.op[3 * o], sl = S.op[3 * o + 1], su = S.op[3 * o + 2];
-          buf[dst - c0] -= lu[sl] * buf[su - c0];
+    const int cb = S.fac_lp[l], ce = S.fac_lp[l + 1];
+    if (S.fac_mode[l] == 0) {
+      TF *buf = reinterpret_cast<TF *>(smem_raw) + (size_t)tid * S.t_cols;
+      for (int idx = cb + tid; idx < ce; idx += nt) {
+        const int k = S.fac_cols[idx];
+        const int c0 = S.lu_cp[k], len = S.lu_cp[k + 1] - c0, dl = S.lu_dp[k] - c0;
+        if (len <= S.t_cols) {
+          for (int t = 0; t < len; t++) buf[t] = lu[c0 + t];
+          col_ops_thread<TF>(S, lu, buf, S.op_cp[k], S.op_cp[k + 1]);
+          const TF piv = buf[dl];
+          if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
+          const TF inv = TF(1) / piv;
+          for (int t = 0; t < len; t++) {
+            TF v = buf[t];
+            if (t > dl) {
+              v *= inv;
+              lu[c0 + t] = v;
+            }
+            so[S.lu2s[c0 + t]] = v;
+          }
+        } else {
+          TF *x = lu + c0;
+          col_ops_thread<TF>(S, lu, x, S.op_cp[k], S.op_cp[k + 1]);
+          const TF piv = x[dl];
+          if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
+          const TF inv = TF(1) / piv;
+          for (int t = 0; t < len; t++) {
+            TF v = x[t];
+            if (t > dl) {
+              v *= inv;
+              x[t] = v;
+            }
+            so[S.lu2s[c0 + t]] = v;
+          }
         }
-        const TF piv = buf[dpos - c0];
-        if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
-        const TF inv = TF(1) / piv;
-        for (int t = dpos - c0 + 1; t < c1 - c0; t++) buf[t] *= inv;
-        for (int t = 0; t < c1 - c0; t++) lu[c0 + t] = buf[t];
-      } else {
-        for (int o = o0; o < o1; o++) {
-          const int dst = S.op[3 * o], sl = S.op[3 * o + 1], su = S.op[3 * o + 2];
-          lu[dst] -= lu[sl] * lu[su];
+      }
+    } else {
+      char *wb = smem_raw + (size_t)wave * S.w_bytes;
+      TF *col = reinterpret_cast<TF *>(wb);
+      TF *lvs = col + S.w_col;
+      int *locs = reinterpret_cast<int *>(lvs + S.w_ops);
+      for (int idx = cb + wave; idx < ce; idx += nw) {
+        const int k = S.fac_cols[idx];
+        const int c0 = S.lu_cp[k], len = S.lu_cp[k + 1] - c0, dl = S.lu_dp[k] - c0;
+        const int o0 = S.op_cp[k], nops = S.op_cp[k + 1] - o0;
+        const int s0 = S.st_cp[k], s1 = S.st_cp[k + 1];
+        TF piv;
+        if (len <= S.w_col && nops <= S.w_ops) {
+          // stage the column, its op locations and all its multipliers
+          for (int t = lane; t < len; t += WAVE) col[t] = lu[c0 + t];
+          for (int t = lane; t < nops; t += WAVE) {
+            locs[t] = S.op_loc[o0 + t];
+            lvs[t] = lu[S.op_sl[o0 + t]];
+          }
+          wave_sync();
+          for (int sb = s0; sb < s1; sb += WAVE) {
+            const int sn = min(WAVE, s1 - sb);
+            // lane t holds the first op of step sb + t (t <= sn)
+            const int bnd = lane <= sn ? S.st_o[sb + lane] - o0 : 0;
+            const int last = S.st_o[sb + sn] - o0;
+            for (int s = 0; s < sn; s++) {
+              const int ob = __builtin_amdgcn_readlane(bnd, s);
+              const int oe = s + 1 < WAVE ? __builtin_amdgcn_readlane(bnd, s + 1) : last;
+              const int t = ob + lane;
+              if (t < oe) {
+                const int loc = locs[t];
+                col[loc & 0xffff] -= lvs[t] * col[loc >> 16];
+              }
+              wave_sync();
+            }
+          }
+          piv = col[dl];
+          const TF inv = TF(1) / piv;
+          for (int t = lane; t < len; t += WAVE) {
+            TF v = col[t];
+            if (t > dl) {
+              v *= inv;
+              lu[c0 + t] = v;
+            }
+            so[S.lu2s[c0 + t]] = v;
+          }
+        } else {
+          // in place in global memory, operands loaded per step
+          TF *x = lu + c0;
+          for (int s = s0; s < s1; s++) {
+            const int t = S.st_o[s] + lane;
+            if (t < S.st_o[s + 1]) {
+              const int loc = S.op_loc[t];
+              x[loc & 0xffff] -= lu[S.op_sl[t]] * x[loc >> 16];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+          }
+          piv = x[dl];
+          const TF inv = TF(1) / piv;
+          for (int t = lane; t < len; t += WAVE) {
+            TF v = x[t];
+            if (t > dl) {
+              v *= inv;
+              x[t] = v;
+            }
+            so[S.lu2s[c0 + t]] = v;
+          }
         }
-        const TF piv = lu[dpos];
         if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
-        const TF inv = TF(1) / piv;
-        for (int t = dpos + 1; t < c1; t++) lu[t] *= inv;
+        wave_sync();
       }
     }
     __syncthreads();
+    GP_TRACE(0, l + 1);
   }
   if (bad) atomicOr(&status[m], 1);
 }
 
 // One workgroup per scenario: x = A^{-1} b with b gathered from rhs (row
 // space), the solution kept in LDS through both level-scheduled triangular
-// solves, then zupd[scatter[j]] += sign * x[j] and xout[j] = x[j].
+// solves, then zupd[scatter[j]] += sign * x[j] and xout[j] = x[j].  Each
+// row of a level is reduced by g cooperating lanes (g per level, a power of
+// two) reading the row's contiguous slice of the solve stream.
 template <typename TF>
-__global__ void k_splu_solve(SpluDev S, const TF *__restrict__ luv, const double *__restrict__ rhs, int ldM,
-                             double sign, const int *__restrict__ scatter, double *__restrict__ zupd,
-                             double *__restrict__ xout, const uint8_t *__restrict__ work) {
-  const int m = blockIdx.x;
+__global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restrict__ svv,
+                                                    const double *__restrict__ rhs, int ldM, double sign,
+                                                    const int *__restrict__ scatter, double *__restrict__ zupd,
+                                                    double *__restrict__ xout, const uint8_t *__restrict__ work) {
+  const int m = scenario_of_block(blockIdx.x, gridDim.x);
   if (!work[m]) return;
-  extern __shared__ char smem_raw[];
+  extern __shared__ __align__(16) char smem_raw[];
   double *x = reinterpret_cast<double *>(smem_raw);
-  const TF *lu = luv + (size_t)m * S.nnz_lu;
+  const TF *lv = svv + (size_t)m * S.nnz_lu;
+  const TF *uv = lv + S.nnz_l;
+  const TF *dg = uv + S.nnz_u;
   const int tid = threadIdx.x, nt = blockDim.x, n = S.n;
   for (int r = tid; r < n; r += nt) x[S.pinv[r]] = rhs[(size_t)r * ldM + m];
   __syncthreads();
+  GP_TRACE(1, 0);
   for (int l = 0; l < S.n_fwlev; l++) {
-    for (int idx = S.fw_lp[l] + tid; idx < S.fw_lp[l + 1]; idx += nt) {
-      const int i = S.fw_rows[idx];
-      double s = x[i];
-      for (int e = S.lrow_p[i]; e < S.lrow_p[i + 1]; e++) s -= (double)lu[S.lrow_pos[e]] * x[S.lrow_k[e]];
-      x[i] = s;
+    const int g = S.fw_g[l];
+    const int sub = tid & (g - 1), ng = nt / g;
+    for (int slot = S.fw_lp[l] + tid / g; slot < S.fw_lp[l + 1]; slot += ng) {
+      const int i = S.fw_info[3 * slot], b = S.fw_info[3 * slot + 1], e = S.fw_info[3 * slot + 2];
+      double acc = 0.0;
+      for (int p = b + sub; p < e; p += g) acc += (double)lv[p] * x[S.ls_k[p]];
+      for (int off = g >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+      if (sub == 0) x[i] -= acc;
     }
     __syncthreads();
+    GP_TRACE(1, l + 1);
   }
   for (int l = 0; l < S.n_bwlev; l++) {
-    for (int idx = S.bw_lp[l] + tid; idx < S.bw_lp[l + 1]; idx += nt) {
-      const int k = S.bw_rows[idx];
-      double s = x[k];
-      for (int e = S.urow_p[k]; e < S.urow_p[k + 1]; e++) s -= (double)lu[S.urow_pos[e]] * x[S.urow_j[e]];
-      x[k] = s / (double)lu[S.lu_dp[k]];
+    const int g = S.bw_g[l];
+    const int sub = tid & (g - 1), ng = nt / g;
+    for (int slot = S.bw_lp[l] + tid / g; slot < S.bw_lp[l + 1]; slot += ng) {
+      const int k = S.bw_info[3 * slot], b = S.bw_info[3 * slot + 1], e = S.bw_info[3 * slot + 2];
+      double acc = 0.0;
+      for (int p = b + sub; p < e; p += g) acc += (double)uv[p] * x[S.us_j[p]];
+      for (int off = g >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+      if (sub == 0) x[k] = (x[k] - acc) / (double)dg[k];
     }
     __syncthreads();
+    GP_TRACE(1, S.n_fwlev + l + 1);
   }
   for (int k = tid; k < n; k += nt) {
     const int j = S.q[k];
@@ -1238,12 +1398,12 @@ void launch_bus_residual(const DevTables &T, const double *p, const double *z, c
   hipLaunchKernelGGL(k_bus_residual, lane_grid(T.nbus, T.ldM), dim3(WAVE), 0, s, T, p, z, inj, fs, f, work);
 }
 void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, double tol, double *fnorm,
-                         unsigned long long *nb, int *n_unconv, uint8_t *refac, int first, double theta,
-                         hipStream_t s) {
-  const int chunks = n < 64 ? 1 : (n < 4096 ? 16 : 64);
-  hipLaunchKernelGGL(k_rownorm, dim3(ldM / WAVE, chunks), dim3(WAVE), 0, s, n, ldM, M, f, work, nb);
+                         unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
+                         double theta, hipStream_t s) {
+  const int chunks = std::max(1, std::min((n + 15) / 16, 1024 / (ldM / WAVE)));
+  hipLaunchKernelGGL(k_rownorm, dim3(ldM / WAVE, chunks), dim3(256), 0, s, n, ldM, M, f, work, nb);
   hipLaunchKernelGGL(k_norm_finish, dim3((M + 255) / 256), dim3(256), 0, s, M, tol, nb, fnorm, work, n_unconv, refac,
-                     first, theta);
+                     refac0, first, theta);
 }
 void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs, double *jv,
                      const uint8_t *work, hipStream_t s) {
@@ -1269,19 +1429,22 @@ void launch_cluster_backsub(const DevTables &T, const TF *X, const double *f, co
   hipLaunchKernelGGL(k_cluster_backsub<TF>, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, X, dv, z, work);
 }
 template <typename TF>
-void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv,
+void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv, TF *sv,
                              const uint8_t *refac, int M, int *status, hipStream_t s) {
-  const size_t lds = (size_t)S.threads * S.lds_cols * sizeof(TF);
-  hipLaunchKernelGGL(k_splu_factor<TF>, dim3(M), dim3(S.threads), lds, s, S, var, cst, ldM, luv, refac, status);
+  if (S.lds_factor > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_splu_factor<TF>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, S.lds_factor);
+  hipLaunchKernelGGL(k_splu_factor<TF>, dim3(M), dim3(S.threads), S.lds_factor, s, S, var, cst, ldM, luv, sv, refac,
+                     status);
 }
 template <typename TF>
-void launch_splu_solve(const SpluDev &S, const TF *luv, const double *rhs, int ldM, double sign, const int *scatter,
+void launch_splu_solve(const SpluDev &S, const TF *sv, const double *rhs, int ldM, double sign, const int *scatter,
                        double *zupd, double *xout, const uint8_t *work, int M, hipStream_t s) {
-  const size_t lds = (size_t)S.n * sizeof(double);
-  if (lds > 65536)
+  if (S.lds_solve > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_splu_solve<TF>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_splu_solve<TF>, dim3(M), dim3(256), lds, s, S, luv, rhs, ldM, sign, scatter, zupd, xout, work);
+                              hipFuncAttributeMaxDynamicSharedMemorySize, S.lds_solve);
+  hipLaunchKernelGGL(k_splu_solve<TF>, dim3(M), dim3(S.threads), S.lds_solve, s, S, sv, rhs, ldM, sign, scatter, zupd,
+                     xout, work);
 }
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s) {
   const int gx = n < 1024 ? n : 1024;
@@ -1302,8 +1465,8 @@ template void launch_cluster_backsub<double>(const DevTables &, const double *, 
 template void launch_cluster_backsub<float>(const DevTables &, const float *, const double *, const double *,
                                             double *, const uint8_t *, hipStream_t);
 template void launch_splu_factor_impl<double>(const SpluDev &, const double *, const double *, int, double *,
-                                              const uint8_t *, int, int *, hipStream_t);
-template void launch_splu_factor_impl<float>(const SpluDev &, const double *, const double *, int, float *,
+                                              double *, const uint8_t *, int, int *, hipStream_t);
+template void launch_splu_factor_impl<float>(const SpluDev &, const double *, const double *, int, float *, float *,
                                              const uint8_t *, int, int *, hipStream_t);
 template void launch_splu_solve<double>(const SpluDev &, const double *, const double *, int, double, const int *,
                                         double *, double *, const uint8_t *, int, hipStream_t);

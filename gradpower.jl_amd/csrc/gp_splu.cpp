@@ -223,6 +223,8 @@ std::string splu_analyse(int n, const int *cp, const int *ri, const double *ax, 
     for (auto &e : lr[i]) { A.lrow_k.push_back(e.first); A.lrow_pos.push_back(e.second); }
     for (auto &e : ur[i]) { A.urow_j.push_back(e.first); A.urow_pos.push_back(e.second); }
   }
+  A.nnz_l = A.lrow_p[n];
+  A.nnz_u = A.urow_p[n];
   std::vector<int> lv(n, 0);
   int nl = 0;
   for (int i = 0; i < n; i++) {
@@ -255,7 +257,115 @@ std::string splu_analyse(int n, const int *cp, const int *ri, const double *ax, 
     std::vector<int> f(A.bw_lp.begin(), A.bw_lp.end() - 1);
     for (int i = 0; i < n; i++) A.bw_rows[f[lv[i]]++] = i;
   }
+  // ---- solve streams in level order -----------------------------------------
+  A.lu2s.assign(A.nnz_lu, -1);
+  A.ls_k.clear();
+  A.us_j.clear();
+  A.fw_info.assign(3 * n, 0);
+  A.bw_info.assign(3 * n, 0);
+  for (int idx = 0; idx < n; idx++) {
+    int i = A.fw_rows[idx];
+    A.fw_info[3 * idx] = i;
+    A.fw_info[3 * idx + 1] = (int)A.ls_k.size();
+    for (int e = A.lrow_p[i]; e < A.lrow_p[i + 1]; e++) {
+      A.lu2s[A.lrow_pos[e]] = (int)A.ls_k.size();
+      A.ls_k.push_back(A.lrow_k[e]);
+    }
+    A.fw_info[3 * idx + 2] = (int)A.ls_k.size();
+  }
+  for (int idx = 0; idx < n; idx++) {
+    int k = A.bw_rows[idx];
+    A.bw_info[3 * idx] = k;
+    A.bw_info[3 * idx + 1] = (int)A.us_j.size();
+    for (int e = A.urow_p[k]; e < A.urow_p[k + 1]; e++) {
+      A.lu2s[A.urow_pos[e]] = A.nnz_l + (int)A.us_j.size();
+      A.us_j.push_back(A.urow_j[e]);
+    }
+    A.bw_info[3 * idx + 2] = (int)A.us_j.size();
+  }
+  for (int k = 0; k < n; k++) A.lu2s[A.lu_dp[k]] = A.nnz_l + A.nnz_u + k;
+  // ---- compact ops -------------------------------------------------------------
+  const int nop = (int)A.op.size() / 3;
+  A.op_loc.assign(nop, 0);
+  A.op_sl.assign(nop, 0);
+  for (int k = 0; k < n; k++) {
+    const int c0 = A.lu_cp[k];
+    for (int o = A.op_cp[k]; o < A.op_cp[k + 1]; o++) {
+      const int d = A.op[3 * o] - c0, su = A.op[3 * o + 2] - c0;
+      if (d < 0 || d > 65535 || su < 0 || su > 65535) return "column too large for compact ops";
+      A.op_loc[o] = d | (su << 16);
+      A.op_sl[o] = A.op[3 * o + 1];
+    }
+  }
   return "";
+}
+
+// Cost units below are one dependent update (an LDS read-modify-write, or a
+// gathered load followed by one).  Thread mode pays every op of its longest
+// column; wavefront mode pays one unit per step plus the column's staging.
+void splu_schedule(SpluAnalysis &A, int threads) {
+  const int n = A.n, W = threads / 64;
+  const int nl = (int)A.fac_lp.size() - 1;
+  // steps: maximal runs of ops with the same source entry, cut at 64
+  A.st_cp.assign(n + 1, 0);
+  A.st_o.clear();
+  for (int k = 0; k < n; k++) {
+    A.st_cp[k] = (int)A.st_o.size();
+    int o = A.op_cp[k];
+    while (o < A.op_cp[k + 1]) {
+      const int su = A.op_loc[o] >> 16;
+      A.st_o.push_back(o);
+      int cnt = 0;
+      while (o < A.op_cp[k + 1] && (A.op_loc[o] >> 16) == su && cnt < 64) { o++; cnt++; }
+    }
+  }
+  A.st_cp[n] = (int)A.st_o.size();
+  A.st_o.push_back(A.op_cp[n]);
+  A.fac_mode.assign(nl, 0);
+  A.t_cols_max = A.w_col_max = A.w_ops_max = 0;
+  std::vector<long long> wsum(W);
+  for (int l = 0; l < nl; l++) {
+    const int c0 = A.fac_lp[l], c1 = A.fac_lp[l + 1];
+    int maxops = 0;
+    std::fill(wsum.begin(), wsum.end(), 0);
+    for (int i = c0; i < c1; i++) {
+      const int k = A.fac_cols[i];
+      maxops = std::max(maxops, A.op_cp[k + 1] - A.op_cp[k]);
+      wsum[(i - c0) % W] += (A.st_cp[k + 1] - A.st_cp[k]) + 6;
+    }
+    const long long tcost = (long long)((c1 - c0 + threads - 1) / threads) * (maxops + 4);
+    const long long wcost = *std::max_element(wsum.begin(), wsum.end());
+    A.fac_mode[l] = wcost < tcost ? 1 : 0;
+    for (int i = c0; i < c1; i++) {
+      const int k = A.fac_cols[i];
+      const int len = A.lu_cp[k + 1] - A.lu_cp[k], nops = A.op_cp[k + 1] - A.op_cp[k];
+      if (A.fac_mode[l]) {
+        A.w_col_max = std::max(A.w_col_max, len);
+        A.w_ops_max = std::max(A.w_ops_max, nops);
+      } else {
+        A.t_cols_max = std::max(A.t_cols_max, len);
+      }
+    }
+  }
+  // solve levels: lanes per row g minimising passes * (ceil(len/g) + log2 g)
+  auto pick = [&](const std::vector<int> &lp, const std::vector<int> &rows, const std::vector<int> &rp,
+                  std::vector<int> &g) {
+    const int nlev = (int)lp.size() - 1;
+    g.assign(nlev, 1);
+    for (int l = 0; l < nlev; l++) {
+      int maxlen = 0;
+      for (int i = lp[l]; i < lp[l + 1]; i++) maxlen = std::max(maxlen, rp[rows[i] + 1] - rp[rows[i]]);
+      const int nr = lp[l + 1] - lp[l];
+      long long best = -1;
+      for (int gg = 1, lg = 0; gg <= 64; gg *= 2, lg++) {
+        const long long passes = ((long long)nr * gg + threads - 1) / threads;
+        const long long c = passes * ((maxlen + gg - 1) / gg + lg + 2);
+        if (best < 0 || c < best) { best = c; g[l] = gg; }
+      }
+    }
+  };
+  pick(A.fw_lp, A.fw_rows, A.lrow_p, A.fw_g);
+  pick(A.bw_lp, A.bw_rows, A.urow_p, A.bw_g);
 }
 
 }  // namespace gp

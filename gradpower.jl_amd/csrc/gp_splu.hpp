@@ -40,10 +40,36 @@ struct SpluAnalysis {
   //   backward: x[k] = (y[k] - sum_j U(k,j) x[j]) / U(k,k)
   std::vector<int> lrow_p, lrow_k, lrow_pos;
   std::vector<int> urow_p, urow_j, urow_pos;
+  // Solve streams.  The factorisation also writes every final value into a
+  // solve array laid out in the order the solves read it: L rows in forward
+  // level order, then U rows (off-diagonal) in backward level order, then
+  // the n diagonal entries.  lu2s maps an LU position to its solve slot.
+  std::vector<int> lu2s;             // [nnz_lu]
+  int nnz_l = 0, nnz_u = 0;
+  std::vector<int> ls_k;             // [nnz_l] column of each L stream entry
+  std::vector<int> us_j;             // [nnz_u] column of each U stream entry
+  std::vector<int> fw_info;          // [3n] per forward slot: row, stream begin, end
+  std::vector<int> bw_info;          // [3n] per backward slot: row, stream begin, end
+  // factorisation ops in compact form: op_loc[o] = (dst - c0) | (srcU - c0) << 16
+  // (offsets inside the target column), op_sl[o] = position of L(i,j)
+  std::vector<int> op_loc, op_sl;
   std::vector<int> fw_lp, fw_rows;   // forward levels
   std::vector<int> bw_lp, bw_rows;   // backward levels
   long long flops = 0;               // multiply-add pairs of one factorisation
+  // Launch schedule (splu_schedule).  A factorisation level runs either one
+  // thread per column (wide levels of short columns) or one wavefront per
+  // column (narrow levels of long columns).  In wavefront mode the updates of
+  // a column are split into steps: step s applies ops [st_o[s], st_o[s+1]),
+  // at most 64 of them, all with the same source column, one op per lane.
+  std::vector<int> fac_mode;         // [levels] 0 thread per column, 1 wavefront per column
+  std::vector<int> st_cp, st_o;      // [n+1] first step of column k; [steps+1] op bounds
+  std::vector<int> fw_g, bw_g;       // [levels] lanes cooperating on one row of a solve level
+  int t_cols_max = 0;                // largest column in a thread-mode level
+  int w_col_max = 0, w_ops_max = 0;  // largest column / op count in a wavefront-mode level
 };
+
+// Build the launch schedule for workgroups of `threads` threads.
+void splu_schedule(SpluAnalysis &A, int threads);
 
 // Analyse the CSC matrix (cp, ri, representative values ax).  Returns an
 // empty string on success, otherwise an error message.

@@ -60,7 +60,8 @@ _LIB = None
 
 EXPORTS = ["gp_create", "gp_destroy", "gp_set_state", "gp_set_events", "gp_integrate",
            "gp_get_trajectory", "gp_get_state", "gp_eval_residual", "gp_jacobian_pattern",
-           "gp_eval_jacobian", "gp_newton_step", "gp_get_symbolic_info", "gp_last_error", "gp_version"]
+           "gp_eval_jacobian", "gp_newton_step", "gp_get_symbolic_info", "gp_get_scenario_status",
+           "gp_last_error", "gp_version"]
 
 
 def load():
@@ -84,6 +85,7 @@ def load():
     lib.gp_jacobian_pattern.argtypes = [C.c_void_p, _ip, _ip, _ip]
     lib.gp_eval_jacobian.argtypes = [C.c_void_p, _dp, C.c_double, _up, _dp]
     lib.gp_get_symbolic_info.argtypes = [C.c_void_p, C.c_void_p]
+    lib.gp_get_scenario_status.argtypes = [C.c_void_p, _ip, _ip]
     lib.gp_newton_step.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, C.c_double, _up, _dp]
     lib.gp_last_error.restype = C.c_char_p
     lib.gp_version.restype = C.c_char_p
@@ -174,6 +176,13 @@ class HipContext:
         out = np.zeros((self.nrec, ln))
         self._check(self.lib.gp_get_trajectory(self.h, int(m), _d(out), ln))
         return out
+
+    def scenario_status(self):
+        """(status[M], steps_done[M]) of the last integrate call."""
+        st = np.zeros(self.M, np.int32)
+        sd = np.zeros(self.M, np.int32)
+        self._check(self.lib.gp_get_scenario_status(self.h, _i(st), _i(sd)))
+        return st, sd
 
     def state(self):
         out = np.zeros((self.M, self.n))

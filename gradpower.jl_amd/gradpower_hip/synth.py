@@ -314,7 +314,7 @@ def _regional_dispatch(edges, n, gen_buses, loads, floor=0.1):
 
 def case2_grid(M=256, nbus=2000, n_genrou=400, n_pss=100, n_static=300, seed=1002,
                load_fraction=0.2, tf_range=(1.0, 5.0), name="c2_grid", charging_scale=1.0,
-               zip_alpha=1.0, static_gen_type=PV):
+               zip_alpha=1.0, static_gen_type=PV, static_gen_p=1.0, fault_sg_hops=3):
     rng = np.random.default_rng(seed)
     meta = {"seed": seed, "zip_alpha": zip_alpha, "nbus": nbus,
             "zip_note": "constant-impedance loads (alpha=1, the reference default of "
@@ -346,6 +346,7 @@ def case2_grid(M=256, nbus=2000, n_genrou=400, n_pss=100, n_static=300, seed=100
         loads.append(Load(b, "1", P, -0.3 * P))
     allg = [int(b) for b in gbus] + [int(b) for b in sbus]
     pg = _regional_dispatch(edges, nbus, allg, loads)
+    pg[n_genrou:] *= static_gen_p
     gens = []
     for k, b in enumerate(allg):
         if k < n_genrou:
@@ -371,17 +372,19 @@ def case2_grid(M=256, nbus=2000, n_genrou=400, n_pss=100, n_static=300, seed=100
         if pss_on[k]:
             psd.add_device(_ieeest(eb))
     meta.pop("ieesgo_T", None)
-    # Fault buses are drawn uniformly among buses with no static-generator
-    # stub within one hop: a stub holds |v| = vset with reactive power only,
-    # so a resistive fault at or next to it leaves the faulted algebraic
-    # system without a nearby solution (the reference Newton fails there too).
+    # Fault buses are drawn uniformly among buses more than fault_sg_hops - 1
+    # hops from every static generator: a PV stub holds |v| = vset with
+    # unlimited reactive power, and a resistive fault at, next to or two hops
+    # from one leaves the faulted algebraic system without a Newton-reachable
+    # solution in about 8% of draws (the reference Newton fails there too).
     adj = [[] for _ in range(nbus)]
     for (a, b) in edges:
         adj[a].append(b)
         adj[b].append(a)
     excl = set(int(b) for b in sbus)
-    for b in list(excl):
-        excl.update(adj[b])
+    for _ in range(fault_sg_hops - 1):
+        for b in list(excl):
+            excl.update(adj[b])
     fault_buses = np.array([b for b in range(nbus) if b not in excl])
     meta["fault_bus_pool"] = int(len(fault_buses))
     events, tf = [], []

@@ -120,8 +120,9 @@ enum gp_refactor_policy {
                                   reference's cuDSS factorization/refactorization cycle) */
   GP_REFACTOR_FIRST_ITER = 1,  /* factorise at the first iteration of each Newton solve
                                   and reuse it for the later iterations (Schur methods)  */
-  GP_REFACTOR_ADAPTIVE = 2     /* factorise at the first iteration and again whenever an
-                                  iteration fails to reduce |F|inf by 10x               */
+  GP_REFACTOR_ADAPTIVE = 2     /* keep the factorisation across time steps; factorise
+                                  again after a fault event or a change of dt, and
+                                  whenever an iteration fails to reduce |F|inf by 10x  */
 };
 
 typedef struct gp_options {
@@ -170,6 +171,16 @@ int gp_integrate(gp_ctx *ctx, int32_t method, double dt, double tol,
 /* Recorded trajectory of scenario m: out[j*ld + k] for recorded channel j and
  * step k = 0..nsteps[m]; ld >= nsteps[m]+1. */
 int gp_get_trajectory(gp_ctx *ctx, int32_t m, double *out, int64_t ld);
+
+/* Outcome of the last gp_integrate per scenario: status[m] is GP_OK or the
+ * gp_status that stopped scenario m (a Newton solve that did not converge,
+ * a non-finite residual, a zero pivot); steps_done[m] is the number of time
+ * steps completed before it stopped (nsteps[m] when it finished), so columns
+ * 0..steps_done[m] of its trajectory are converged states.  The reference
+ * ignores a failed Newton solve and continues (src/dynamics.jl:593-597,
+ * src/batched.jl:459, ext:3044); this library stops the scenario and says so.
+ * Either output may be NULL. */
+int gp_get_scenario_status(gp_ctx *ctx, int32_t *status, int32_t *steps_done);
 
 /* Final state of every scenario, out[m*n + j]. */
 int gp_get_state(gp_ctx *ctx, double *out);
