@@ -34,9 +34,10 @@ def events_of(case):
     return ev
 
 
-def run_method(case, method, steps, warmup, record_idx, tol, refactor_policy):
+def run_method(case, method, steps, warmup, record_idx, tol, refactor_policy, stall_window, theta=0.0):
     from gradpower_hip.lib import HipContext
-    ctx = HipContext(case.desc, case.M, record_idx=record_idx, refactor_policy=refactor_policy)
+    ctx = HipContext(case.desc, case.M, record_idx=record_idx, refactor_policy=refactor_policy,
+                     stall_window=stall_window, refactor_theta=theta)
     ev = events_of(case)
     times, stats = [], None
     for it in range(warmup + steps):
@@ -48,6 +49,7 @@ def run_method(case, method, steps, warmup, record_idx, tol, refactor_policy):
         if it >= warmup:
             times.append(t1 - t0)
         log(f"  method {method} pass {it}: {t1 - t0:.3f} s, device {stats['elapsed_ms']:.1f} ms, "
+            f"rounds {stats['newton_rounds']} (factor {stats['factor_launches']}), "
             f"corrections {stats['newton_iters']}, refactorizations {stats['refactorizations']}, "
             f"not converged {stats['n_not_converged']}, nan {stats['n_nan']}, status {stats['status']}")
     speeds = None
@@ -91,6 +93,10 @@ def main():
     ap.add_argument("--horizon-cap", type=float, default=0.0,
                     help="cap every scenario horizon (s); profiling runs only, not the benchmark workload")
     ap.add_argument("--methods", default="schur32,schur64,mono")
+    ap.add_argument("--theta", type=float, default=10.0,
+                    help="adaptive refactorisation: refactor when |F| falls by less than this factor")
+    ap.add_argument("--stall-window", type=int, default=8,
+                    help="stop a scenario whose |F| has not halved in this many Newton iterations (0: run to itermax)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the integrator shards by scenario; "
@@ -119,7 +125,7 @@ def main():
             continue
         nst = args.steps if name != "mono" else max(1, args.steps // 3)
         runs[name] = run_method(case, meth, nst, args.warmup if name != "mono" else min(1, args.warmup),
-                                widx if name != "mono" else None, tol, pol)
+                                widx if name != "mono" else None, tol, pol, args.stall_window, args.theta)
     head = runs.get("schur32") or next(iter(runs.values()))
     info = head["info"]
     # value: completed contingency-steps per second of the pass (a scenario
@@ -141,12 +147,15 @@ def main():
         "config": {"workload": f"{args.nbus}-bus synthetic grid, M={args.M} uneven contingencies, dt=1/120 s, "
                                f"horizons U(1,5) s, Schur + FP32 factors",
                    "n_sys": case.desc.n, "n_red": info["n_red"], "contingency_steps": total_steps,
-                   "newton_tol": tol, "horizon_cap": args.horizon_cap or None},
+                   "newton_tol": tol, "itermax": 30, "stall_window": args.stall_window,
+                   "refactor_theta": args.theta,
+                   "horizon_cap": args.horizon_cap or None},
     }
     for k, r in runs.items():
         st = r["stats"]
         out[k] = {"value": v[k], "ms_per_pass": 1e3 * float(np.median(r["times"])),
-                  "completed_steps": st["scenario_steps"], "newton_corrections": st["newton_iters"],
+                  "completed_steps": st["scenario_steps"], "newton_rounds": st["newton_rounds"],
+                  "factor_launches": st["factor_launches"], "newton_corrections": st["newton_iters"],
                   "refactorizations": st["refactorizations"], "status": st["status"],
                   "failed_scenarios": int(np.sum(r["status"] != 0))}
     if "schur32" in runs and "schur64" in runs:

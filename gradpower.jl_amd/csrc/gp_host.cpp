@@ -72,34 +72,56 @@ struct DBuf {
 struct LUPlan {
   SpluAnalysis A;
   SpluDev dev{};
-  DBuf<int> a2lu, a_src, lu_cp, lu_dp, op_cp, op_loc, op_sl, st_cp, st_o, fac_lp, fac_cols, fac_mode, lu2s, ls_k,
-      us_j, fw_lp, fw_info, fw_g, bw_lp, bw_info, bw_g, q, pinv;
+  DBuf<int> a2lu, a_src, op_loc, op_sl, st_o, flp, fac_mode, crec, lu2s, dmap, fw_lvl, fw_chunk, fw_cols, fw_rows,
+      bw_lvl, bw_chunk, bw_cols, bw_rows, q, pinv;
   bool ready = false;
   double dt = -1.0;
   static constexpr int kThreads = 512;
-  static constexpr int kLdsBudget = 120 * 1024;
+  static constexpr int kLdsBudget = 120 * 1024;   // staging / dense block
+  static constexpr int kLdsMax = 160 * 1024;      // LDS of one CU
 
   static std::vector<int> nonempty(const std::vector<int> &v) { return v.empty() ? std::vector<int>{0} : v; }
+  // LDS bytes of the solve before its dense block: x, then the level and
+  // chunk tables, each section 16-byte aligned (the kernel's layout)
+  int table_bytes() const {
+    auto al = [](size_t b) { return (b + 15) / 16 * 16; };
+    return (int)(al((size_t)A.n * 8) + al(4 * A.sf.lvl.size()) + al(4 * A.sf.chunk.size()) + al(4 * A.sb.lvl.size()) +
+                 al(4 * A.sb.chunk.size()));
+  }
 
+  // Schedules and uploads the analysis.  The solve keeps x (8 bytes per row),
+  // its level tables and the dense block in LDS, which bounds the block.
   void upload(const std::vector<int> &asrc) {
-    splu_schedule(A, kThreads);
-    a2lu.upload(A.a2lu); a_src.upload(asrc); lu_cp.upload(A.lu_cp); lu_dp.upload(A.lu_dp);
-    op_cp.upload(A.op_cp); op_loc.upload(nonempty(A.op_loc)); op_sl.upload(nonempty(A.op_sl));
-    st_cp.upload(A.st_cp); st_o.upload(A.st_o);
-    fac_lp.upload(A.fac_lp); fac_cols.upload(A.fac_cols); fac_mode.upload(nonempty(A.fac_mode));
-    lu2s.upload(A.lu2s); ls_k.upload(nonempty(A.ls_k)); us_j.upload(nonempty(A.us_j));
-    fw_lp.upload(A.fw_lp); fw_info.upload(A.fw_info); fw_g.upload(nonempty(A.fw_g));
-    bw_lp.upload(A.bw_lp); bw_info.upload(A.bw_info); bw_g.upload(nonempty(A.bw_g));
+    const long long room = std::min<long long>(kLdsBudget, (long long)kLdsMax - 8LL * A.n - 16384);
+    int nd_max = 0;
+    while ((long long)(nd_max + 16) * (nd_max + 16) * 8 <= room) nd_max += 16;
+    splu_schedule(A, kThreads, nd_max);
+    if (table_bytes() + (long long)A.nd * A.nd * 8 > kLdsMax)
+      throw GpError{GP_ERR_UNSUPPORTED, "system too large for the in-LDS triangular solve (" +
+                                            std::to_string(A.n) + " rows)"};
+    a2lu.upload(A.a2lu); a_src.upload(asrc);
+    std::vector<int> pl = A.op_loc, ps = A.op_sl;   // one padding entry: clamped reads stay in bounds
+    pl.push_back(0);
+    ps.push_back(0);
+    op_loc.upload(pl); op_sl.upload(ps); st_o.upload(A.st_o);
+    flp.upload(A.flp); fac_mode.upload(nonempty(A.fac_mode)); crec.upload(nonempty(A.crec));
+    lu2s.upload(A.lu2s); dmap.upload(nonempty(A.dmap));
+    fw_lvl.upload(nonempty(A.sf.lvl)); fw_chunk.upload(nonempty(A.sf.chunk));
+    fw_cols.upload(nonempty(A.sf.cols)); fw_rows.upload(nonempty(A.sf.rows));
+    bw_lvl.upload(nonempty(A.sb.lvl)); bw_chunk.upload(nonempty(A.sb.chunk));
+    bw_cols.upload(nonempty(A.sb.cols)); bw_rows.upload(nonempty(A.sb.rows));
     q.upload(A.q); pinv.upload(A.pinv);
     SpluDev &d = dev;
-    d.n = A.n; d.nnz_a = A.nnz_a; d.nnz_lu = A.nnz_lu; d.nnz_l = A.nnz_l; d.nnz_u = A.nnz_u;
-    d.n_flev = (int)A.fac_lp.size() - 1; d.n_fwlev = (int)A.fw_lp.size() - 1; d.n_bwlev = (int)A.bw_lp.size() - 1;
-    d.a2lu = a2lu.p; d.a_src = a_src.p; d.lu_cp = lu_cp.p; d.lu_dp = lu_dp.p;
-    d.op_cp = op_cp.p; d.op_loc = op_loc.p; d.op_sl = op_sl.p; d.st_cp = st_cp.p; d.st_o = st_o.p;
-    d.fac_lp = fac_lp.p; d.fac_cols = fac_cols.p; d.fac_mode = fac_mode.p;
-    d.lu2s = lu2s.p; d.ls_k = ls_k.p; d.us_j = us_j.p;
-    d.fw_lp = fw_lp.p; d.fw_info = fw_info.p; d.fw_g = fw_g.p;
-    d.bw_lp = bw_lp.p; d.bw_info = bw_info.p; d.bw_g = bw_g.p;
+    d.n = A.n; d.nnz_a = A.nnz_a; d.nnz_lu = A.nnz_lu; d.ns = std::max(1, A.ns);
+    d.n_flev = (int)A.flp.size() - 1; d.nd = A.nd;
+    d.qb = A.lu_cp[A.n - A.nd]; d.nnz_top = A.lu_cp[A.n] - d.qb;
+    d.a2lu = a2lu.p; d.a_src = a_src.p; d.op_loc = op_loc.p; d.op_sl = op_sl.p; d.st_o = st_o.p;
+    d.flp = flp.p; d.fac_mode = fac_mode.p; d.crec = crec.p; d.lu2s = lu2s.p; d.dmap = dmap.p;
+    d.n_fwlev = (int)A.sf.lvl.size() / 4; d.n_bwlev = (int)A.sb.lvl.size() / 4;
+    d.n_fwch = (int)A.sf.chunk.size() / 2; d.n_bwch = (int)A.sb.chunk.size() / 2;
+    d.fw_nvals = A.sf.nvals; d.bw_nvals = A.sb.nvals;
+    d.fw_lvl = fw_lvl.p; d.fw_chunk = fw_chunk.p; d.fw_cols = fw_cols.p; d.fw_rows = fw_rows.p;
+    d.bw_lvl = bw_lvl.p; d.bw_chunk = bw_chunk.p; d.bw_cols = bw_cols.p; d.bw_rows = bw_rows.p;
     d.q = q.p; d.pinv = pinv.p;
     d.threads = kThreads;
   }
@@ -118,8 +140,9 @@ struct LUPlan {
     dev.w_col = wc;
     dev.w_ops = wo;
     dev.w_bytes = wbytes(wc, wo);
-    dev.lds_factor = std::max<int>((int)((size_t)kThreads * tc * elem), W * dev.w_bytes);
-    dev.lds_solve = (int)((size_t)A.n * sizeof(double));
+    dev.lds_factor = std::max<int>(std::max<int>((int)((size_t)kThreads * tc * elem), W * dev.w_bytes),
+                                   (int)((size_t)A.nd * A.nd * elem));
+    dev.lds_solve = table_bytes() + (int)((size_t)A.nd * A.nd * elem);
   }
 };
 
@@ -163,12 +186,13 @@ struct gp_ctx {
   DBuf<int> d_rec_idx;
   DBuf<double> d_yv, d_u0, d_jp_yoff_val, d_ydiag, d_s_cst;
   DBuf<double> d_p, d_z, d_zold, d_f, d_inj, d_jv, d_svar, d_rhs, d_dv, d_Dg, d_fnorm, d_hist, d_luJ, d_luS64;
-  DBuf<double> d_svJ, d_svS64;
-  DBuf<float> d_luS32, d_svS32, d_X32;
+  DBuf<double> d_svJ, d_svS64, d_dnJ, d_dnS64;
+  DBuf<float> d_luS32, d_svS32, d_dnS32, d_X32;
   DBuf<double> d_X64;
   DBuf<uint8_t> d_work, d_act, d_refac, d_refac0, d_sel;
-  DBuf<int> d_fbus, d_nunconv, d_status;
-  DBuf<double> d_fg;
+  DBuf<int> d_fbus, d_nunconv, d_status, d_nstall;
+  DBuf<double> d_fg, d_fbest;
+  DBuf<uint8_t> d_failed;
   DBuf<unsigned long long> d_nb;
   DevTables T{};
   LUPlan luJ, luS;
@@ -527,6 +551,9 @@ void alloc_state(gp_ctx *C) {
   C->d_fbus.alloc((size_t)MAXFAULT * L);
   C->d_fg.alloc((size_t)MAXFAULT * L);
   C->d_nunconv.alloc(2);
+  C->d_nstall.alloc(L);
+  C->d_fbest.alloc(L);
+  C->d_failed.alloc(L);
   C->d_status.alloc(L);
   C->d_nb.alloc(L);
   HIPCHK(hipMemset(C->d_nb.p, 0, L * sizeof(unsigned long long)));
@@ -570,7 +597,9 @@ void analyse(gp_ctx *C, int method, double dt) {
     C->luJ.ready = true;
     C->luJ.dt = dt;
     C->d_luJ.alloc((size_t)C->luJ.A.nnz_lu * C->M);
-    C->d_svJ.alloc((size_t)C->luJ.A.nnz_lu * C->M);
+    C->d_svJ.alloc((size_t)C->luJ.dev.ns * C->M);
+    HIPCHK(hipMemset(C->d_svJ.p, 0, C->d_svJ.n * sizeof(double)));   // padding slots stay zero
+    C->d_dnJ.alloc((size_t)std::max(1, C->luJ.A.nd * C->luJ.A.nd) * C->M);
   } else {
     if (C->luS.ready && C->luS.dt == dt) return;
     const int nnzS = C->scp[C->nred];
@@ -607,11 +636,19 @@ void ensure_schur_buffers(gp_ctx *C, bool fp32) {
   if (fp32) {
     C->d_X32.ensure((size_t)std::max(1, C->xrows) * 3 * L);
     C->d_luS32.ensure((size_t)C->luS.A.nnz_lu * C->M);
-    C->d_svS32.ensure((size_t)C->luS.A.nnz_lu * C->M);
+    if (C->d_svS32.n != (size_t)C->luS.dev.ns * C->M) {
+      C->d_svS32.alloc((size_t)C->luS.dev.ns * C->M);
+      HIPCHK(hipMemset(C->d_svS32.p, 0, C->d_svS32.n * sizeof(float)));   // padding slots stay zero
+    }
+    C->d_dnS32.ensure((size_t)std::max(1, C->luS.A.nd * C->luS.A.nd) * C->M);
     C->luS.shape(sizeof(float));
   } else {
     C->d_luS64.ensure((size_t)C->luS.A.nnz_lu * C->M);
-    C->d_svS64.ensure((size_t)C->luS.A.nnz_lu * C->M);
+    if (C->d_svS64.n != (size_t)C->luS.dev.ns * C->M) {
+      C->d_svS64.alloc((size_t)C->luS.dev.ns * C->M);
+      HIPCHK(hipMemset(C->d_svS64.p, 0, C->d_svS64.n * sizeof(double)));   // padding slots stay zero
+    }
+    C->d_dnS64.ensure((size_t)std::max(1, C->luS.A.nd * C->luS.A.nd) * C->M);
     C->luS.shape(sizeof(double));
   }
 }
@@ -619,7 +656,8 @@ void ensure_schur_buffers(gp_ctx *C, bool fp32) {
 struct NewtonOut {
   int64_t corrections = 0;
   int64_t refac = 0;
-  std::vector<uint8_t> failed;   // scenarios still unconverged after itermax
+  int64_t rounds = 0, factor_launches = 0;
+  std::vector<uint8_t> failed;   // scenarios that did not converge (empty: none)
 };
 
 // newton_step! (src/numerics.jl:55-176) for every scenario in `mask`, in lock
@@ -632,6 +670,23 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
   set_mask(C, C->d_work, mask);
   const bool reuse = refac0 && method != GP_METHOD_MONO_FP64 && C->opt.refactor_policy == GP_REFACTOR_ADAPTIVE;
   if (reuse) set_mask(C, C->d_refac0, *refac0);
+  const int stall = C->opt.stall_window;
+  if (stall > 0) HIPCHK(hipMemsetAsync(C->d_failed.p, 0, L, C->stream));
+  // scenarios stopped by the stall rule, plus those still working when
+  // itermax ran out
+  auto collect = [&](bool exhausted) {
+    if (!exhausted && stall <= 0) return;
+    std::vector<uint8_t> w(L, 0), fl(L, 0);
+    if (exhausted) HIPCHK(hipMemcpyAsync(w.data(), C->d_work.p, L, hipMemcpyDeviceToHost, C->stream));
+    if (stall > 0) HIPCHK(hipMemcpyAsync(fl.data(), C->d_failed.p, L, hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    bool any = false;
+    for (int m = 0; m < L; m++) {
+      w[m] = (w[m] || fl[m]) ? 1 : 0;
+      any = any || w[m];
+    }
+    if (any) out.failed = w;
+  };
   const double *p = C->d_p.p;
   double *z = C->d_z.p;
   int nun = 0;
@@ -643,21 +698,29 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
     // the first iteration only when |F| fell by less than 10x
     const double theta = (method == GP_METHOD_MONO_FP64 || C->opt.refactor_policy == GP_REFACTOR_EVERY_ITER)
                              ? 0.0
-                             : (C->opt.refactor_policy == GP_REFACTOR_ADAPTIVE ? 10.0 : 1e300);
+                             : (C->opt.refactor_policy == GP_REFACTOR_ADAPTIVE
+                                    ? (C->opt.refactor_theta > 0.0 ? C->opt.refactor_theta : 10.0)
+                                    : 1e300);
     launch_rownorm_impl(C->n, C->M, L, C->d_f.p, C->d_work.p, tol, C->d_fnorm.p, C->d_nb.p, C->d_nunconv.p,
-                        C->d_refac.p, reuse ? C->d_refac0.p : nullptr, it == 0, theta, C->stream);
+                        C->d_refac.p, reuse ? C->d_refac0.p : nullptr, it == 0, theta, stall, C->d_fbest.p,
+                        C->d_nstall.p, C->d_failed.p, C->stream);
     int cnt[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(cnt, C->d_nunconv.p, 2 * sizeof(int), hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
     nun = cnt[0];
-    if (nun == 0) return out;
+    if (nun == 0) {
+      collect(false);
+      return out;
+    }
     out.corrections += nun;
     out.refac += cnt[1];
+    out.rounds++;
+    if (cnt[1] > 0) out.factor_launches++;
     if (method == GP_METHOD_MONO_FP64) {
       launch_jac_mono(C->T, p, z, dt, faults(C), C->d_jv.p, C->d_work.p, C->stream);
-      launch_splu_factor_impl<double>(C->luJ.dev, C->d_jv.p, nullptr, L, C->d_luJ.p, C->d_svJ.p,
+      launch_splu_factor_impl<double>(C->luJ.dev, C->d_jv.p, nullptr, L, C->d_luJ.p, C->d_svJ.p, C->d_dnJ.p,
                                       C->d_refac.p, C->M, C->d_status.p, C->stream);
-      launch_splu_solve<double>(C->luJ.dev, C->d_svJ.p, C->d_f.p, L, -1.0, nullptr, z, nullptr, C->d_work.p, C->M,
+      launch_splu_solve<double>(C->luJ.dev, C->d_svJ.p, C->d_dnJ.p, C->d_f.p, L, -1.0, nullptr, z, nullptr, C->d_work.p, C->M,
                                 C->stream);
     } else {
       const bool fp32 = method == GP_METHOD_SCHUR_FP32;
@@ -670,17 +733,17 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
       if (cnt[1] == 0) {
         // every scenario keeps its factorisation
       } else if (fp32)
-        launch_splu_factor_impl<float>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS32.p, C->d_svS32.p, C->d_refac.p, C->M,
+        launch_splu_factor_impl<float>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS32.p, C->d_svS32.p, C->d_dnS32.p, C->d_refac.p, C->M,
                                        C->d_status.p, C->stream);
       else
-        launch_splu_factor_impl<double>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p, C->d_refac.p, C->M,
+        launch_splu_factor_impl<double>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p, C->d_dnS64.p, C->d_refac.p, C->M,
                                         C->d_status.p, C->stream);
       if (fp32) {
-        launch_splu_solve<float>(C->luS.dev, C->d_svS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z, C->d_dv.p,
+        launch_splu_solve<float>(C->luS.dev, C->d_svS32.p, C->d_dnS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z, C->d_dv.p,
                                  C->d_work.p, C->M, C->stream);
         launch_cluster_backsub<float>(C->T, C->d_X32.p, C->d_f.p, C->d_dv.p, z, C->d_work.p, C->stream);
       } else {
-        launch_splu_solve<double>(C->luS.dev, C->d_svS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z, C->d_dv.p,
+        launch_splu_solve<double>(C->luS.dev, C->d_svS64.p, C->d_dnS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z, C->d_dv.p,
                                   C->d_work.p, C->M, C->stream);
         launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, C->d_work.p, C->stream);
       }
@@ -688,9 +751,7 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
     HIPCHK(hipGetLastError());
   }
   // itermax exhausted: the scenarios still in `work` did not converge
-  out.failed.assign(L, 0);
-  HIPCHK(hipMemcpyAsync(out.failed.data(), C->d_work.p, L, hipMemcpyDeviceToHost, C->stream));
-  HIPCHK(hipStreamSynchronize(C->stream));
+  collect(true);
   return out;
 }
 
@@ -867,6 +928,8 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
         if (act[m]) stale[m] = 0;
       S.newton_iters += no.corrections;
       S.refactorizations += no.refac;
+      S.newton_rounds += no.rounds;
+      S.factor_launches += no.factor_launches;
       S.scenario_steps += nact;
       if (!no.failed.empty())
         for (int m = 0; m < M; m++) S.scenario_steps -= no.failed[m] ? 1 : 0;
@@ -930,6 +993,8 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
           if (res[m]) stale[m] = 1;   // factorised at dt = 0: the next step refactorises
         S.newton_iters += nr.corrections;
         S.refactorizations += nr.refac;
+        S.newton_rounds += nr.rounds;
+        S.factor_launches += nr.factor_launches;
         if (!nr.failed.empty())
           for (int m = 0; m < M; m++)
             if (nr.failed[m]) {

@@ -57,16 +57,18 @@ struct FaultState {
 };
 
 struct SpluDev {
-  int n, nnz_a, nnz_lu, nnz_l, nnz_u, n_flev, n_fwlev, n_bwlev;
+  int n, nnz_a, nnz_lu, ns, n_flev, nd, qb, nnz_top;
   const int *a2lu, *a_src;      // per A entry: LU position, value source
-  const int *lu_cp, *lu_dp;     // column starts, diagonal positions
-  const int *op_cp, *op_loc, *op_sl;   // update program (compact form)
-  const int *st_cp, *st_o;      // wavefront-mode steps
-  const int *fac_lp, *fac_cols, *fac_mode;
-  const int *lu2s;              // LU position -> solve-stream slot
-  const int *ls_k, *us_j;       // column of each L / U stream entry
-  const int *fw_lp, *fw_info, *fw_g;   // forward levels: slots (row, begin, end), lanes per row
-  const int *bw_lp, *bw_info, *bw_g;
+  const int *op_loc, *op_sl;    // update program (compact form)
+  const int *st_o;              // wavefront-mode step bounds
+  const int *flp, *fac_mode, *crec;   // factor levels: column records, mode
+  const int *lu2s;              // LU position -> solve slot (-1: not used by the solves)
+  const int *dmap;              // dense block: entry of the block's columns -> dense index
+  // solves, sliced ELL: per level (T, g, R, first chunk), per chunk (E, pf),
+  // per value slot its column, per lane its row
+  int n_fwlev, n_bwlev, n_fwch, n_bwch, fw_nvals, bw_nvals;
+  const int *fw_lvl, *fw_chunk, *fw_cols, *fw_rows;
+  const int *bw_lvl, *bw_chunk, *bw_cols, *bw_rows;
   const int *q, *pinv;
   int threads;                  // workgroup size of both kernels
   int t_cols;                   // LDS staging per thread (thread mode), 0 = none
@@ -81,7 +83,7 @@ void launch_bus_residual(const DevTables &T, const double *p, const double *z, c
                          FaultState fs, double *f, const uint8_t *work, hipStream_t s);
 void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, double tol, double *fnorm,
                          unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
-                         double theta, hipStream_t s);
+                         double theta, int stall, double *fbest, int *nstall, uint8_t *failed, hipStream_t s);
 void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs,
                      double *jv, const uint8_t *work, hipStream_t s);
 template <typename TF>
@@ -93,11 +95,13 @@ void launch_bus_schur(const DevTables &T, const double *p, const double *z, cons
 template <typename TF>
 void launch_cluster_backsub(const DevTables &T, const TF *X, const double *f, const double *dv,
                             double *z, const uint8_t *work, hipStream_t s);
+// luv: [M][nnz_lu] factor workspace; sv: [M][ns] solve values; dn: [M][nd*nd]
+// dense trailing block (column-major LU, unit lower part implied)
 template <typename TF>
 void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv, TF *sv,
-                             const uint8_t *refac, int M, int *status, hipStream_t s);
+                             TF *dn, const uint8_t *refac, int M, int *status, hipStream_t s);
 template <typename TF>
-void launch_splu_solve(const SpluDev &S, const TF *sv, const double *rhs, int ldM, double sign,
+void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const double *rhs, int ldM, double sign,
                        const int *scatter, double *zupd, double *xout, const uint8_t *work, int M,
                        hipStream_t s);
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s);

@@ -401,10 +401,13 @@ __global__ void __launch_bounds__(256) k_rownorm(int n, int ldM, int M, const do
 // Finishes the max-norm: converged scenarios leave `work`; for the others,
 // refac[m] is set when this is the first iteration (first != 0), when every
 // iteration refactors (theta <= 0), or when |F| fell by less than 1/theta
-// since the previous iteration.
+// since the previous iteration.  With stall > 0 a scenario whose |F| is not
+// finite, exceeds 1e10, or has not halved within `stall` iterations leaves
+// `work` with failed[m] = 1 (fbest/nstall carry its history; first resets it).
 __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict__ nb, double *__restrict__ fnorm,
                               uint8_t *__restrict__ work, int *__restrict__ n_unconv, uint8_t *__restrict__ refac,
-                              const uint8_t *__restrict__ refac0, int first, double theta) {
+                              const uint8_t *__restrict__ refac0, int first, double theta, int stall,
+                              double *__restrict__ fbest, int *__restrict__ nstall, uint8_t *__restrict__ failed) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
   if (!work[m]) {
@@ -418,12 +421,30 @@ __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict_
   if (v < tol) {                 // converged (NaN compares false: stays working and is caught by itermax)
     work[m] = 0;
     refac[m] = 0;
-  } else {
-    atomicAdd(n_unconv, 1);
-    const int rf = first ? (refac0 ? refac0[m] : 1) : ((theta <= 0.0 || !(v * theta < prev)) ? 1 : 0);
-    refac[m] = rf;
-    if (rf) atomicAdd(n_unconv + 1, 1);
+    return;
   }
+  if (stall > 0) {
+    double best = first ? 1e300 : fbest[m];
+    int ns = first ? 0 : nstall[m];
+    if (v < 0.5 * best) {
+      best = v;
+      ns = 0;
+    } else {
+      ns++;
+    }
+    fbest[m] = best;
+    nstall[m] = ns;
+    if (!(v <= 1e10) || ns >= stall) {
+      work[m] = 0;
+      refac[m] = 0;
+      failed[m] = 1;
+      return;
+    }
+  }
+  atomicAdd(n_unconv, 1);
+  const int rf = first ? (refac0 ? refac0[m] : 1) : ((theta <= 0.0 || !(v * theta < prev)) ? 1 : 0);
+  refac[m] = rf;
+  if (rf) atomicAdd(n_unconv + 1, 1);
 }
 
 // ---------------------------------------------------------------- monolithic Jacobian
@@ -1149,40 +1170,259 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Thread mode: one thread runs the whole update program of column k on x
+// LDS-only workgroup barrier: orders the workgroup's LDS accesses across the
+// barrier and leaves its global loads in flight.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+struct ColRec {
+  int k, c0, len, dl, o0, nops, s0, s1;
+};
+__device__ __forceinline__ ColRec load_rec(const int *__restrict__ crec, int idx) {
+  const int4 a = reinterpret_cast<const int4 *>(crec)[2 * idx];
+  const int4 b = reinterpret_cast<const int4 *>(crec)[2 * idx + 1];
+  return ColRec{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Thread mode: one thread runs the whole update program of one column on x
 // (the column staged in LDS, or in place in global memory), 4 ops at a time
 // with their operand loads issued together.  The L multipliers it reads
 // belong to earlier columns and are final.
+// Thread mode: one thread runs the whole update program of one column on x
+// (the column staged in LDS, or in place in global memory), 4 ops at a time;
+// the next batch's op words and multipliers are loaded while the current
+// batch is applied.  The multipliers belong to earlier columns and are final;
+// op indices are clamped to the column's last op (the op arrays carry one
+// padding entry), so every load is unconditional.
 template <typename TF, typename XP>
 __device__ __forceinline__ void col_ops_thread(const SpluDev &S, const TF *__restrict__ lu, XP x, int o0, int o1) {
-  int o = o0;
-  for (; o + 4 <= o1; o += 4) {
-    int loc[4];
-    TF lv[4];
+  if (o1 <= o0) return;
+  const int ol = o1 - 1;
+  int loc[4];
+  TF lv[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int o = min(o0 + u, ol);
+    loc[u] = S.op_loc[o];
+    lv[u] = lu[S.op_sl[o]];
+  }
+  for (int o = o0; o < o1; o += 4) {
+    int nloc[4], nsl[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      loc[u] = S.op_loc[o + u];
-      lv[u] = lu[S.op_sl[o + u]];
+      const int on = min(o + 4 + u, ol);
+      nloc[u] = S.op_loc[on];
+      nsl[u] = S.op_sl[on];
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) x[loc[u] & 0xffff] -= lv[u] * x[loc[u] >> 16];
+    for (int u = 0; u < 4; u++)
+      if (o + u < o1) x[loc[u] & 0xffff] -= lv[u] * x[loc[u] >> 16];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      loc[u] = nloc[u];
+      lv[u] = lu[nsl[u]];
+    }
   }
-  for (; o < o1; o++) {
-    const int loc = S.op_loc[o];
-    x[loc & 0xffff] -= lu[S.op_sl[o]] * x[loc >> 16];
+}
+
+template <typename TF>
+__device__ __forceinline__ int thread_column(const SpluDev &S, TF *__restrict__ lu, TF *__restrict__ so,
+                                             const ColRec &r, TF *buf) {
+  TF piv;
+  if (r.len <= S.t_cols) {
+    for (int t0 = 0; t0 < r.len; t0 += 4) {
+      TF v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) v[u] = lu[r.c0 + min(t0 + u, r.len - 1)];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (t0 + u < r.len) buf[t0 + u] = v[u];
+    }
+    col_ops_thread<TF>(S, lu, buf, r.o0, r.o0 + r.nops);
+    piv = buf[r.dl];
+    const TF inv = TF(1) / piv;
+    for (int t = 0; t < r.len; t++) {
+      TF v = buf[t];
+      if (t > r.dl) {
+        v *= inv;
+        lu[r.c0 + t] = v;
+      }
+      const int d = S.lu2s[r.c0 + t];
+      if (d >= 0) so[d] = v;
+    }
+  } else {
+    TF *x = lu + r.c0;
+    col_ops_thread<TF>(S, lu, x, r.o0, r.o0 + r.nops);
+    piv = x[r.dl];
+    const TF inv = TF(1) / piv;
+    for (int t = 0; t < r.len; t++) {
+      TF v = x[t];
+      if (t > r.dl) {
+        v *= inv;
+        x[t] = v;
+      }
+      const int d = S.lu2s[r.c0 + t];
+      if (d >= 0) so[d] = v;
+    }
   }
+  return (!(piv != TF(0)) || !isfinite((double)piv)) ? 1 : 0;
+}
+
+// Wavefront mode: the column, its op locations and every multiplier it needs
+// are loaded at once into the wavefront's LDS buffers; then each step applies
+// up to 64 independent updates (one per lane) with the next step's operands
+// already in registers.  ext: the column belongs to the dense trailing block
+// and only receives the updates from columns outside it (r.dl = number of
+// its rows above the block, which are final afterwards).
+template <typename TF>
+__device__ __forceinline__ int wave_column(const SpluDev &S, TF *__restrict__ lu, TF *__restrict__ so, const ColRec &r,
+                                           bool ext, TF *col, TF *lvs, int *locs, int lane) {
+  const int len = r.len, nops = r.nops, c0 = r.c0, o0 = r.o0, nst = r.s1 - r.s0;
+  TF piv = TF(1);
+  if (len <= S.w_col && len <= 128 && nops <= S.w_ops && nops <= 512) {
+    // every load of the column at once, at clamped (always valid) indices so
+    // that none of them waits before the LDS stores below
+    TF cv[2];
+    int l2s[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int t = min(lane + WAVE * j, len - 1);
+      cv[j] = lu[c0 + t];
+      l2s[j] = S.lu2s[c0 + t];
+    }
+    int locr[8], slr[8];
+    const int olast = max(nops - 1, 0);   // op arrays carry one padding entry at the end
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int t = min(lane + WAVE * j, olast);
+      locr[j] = S.op_loc[o0 + t];
+      slr[j] = S.op_sl[o0 + t];
+    }
+    const int sn0 = min(WAVE, nst);
+    int bnd = S.st_o[r.s0 + min(lane, sn0)] - o0;
+    TF lvr[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) lvr[j] = lu[slr[j]];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int t = lane + WAVE * j;
+      if (t < len) col[t] = cv[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int t = lane + WAVE * j;
+      if (t < nops) {
+        locs[t] = locr[j];
+        lvs[t] = lvr[j];
+      }
+    }
+    wave_sync();
+    for (int sb = 0; sb < nst; sb += WAVE) {
+      const int sn = min(WAVE, nst - sb);
+      if (sb) bnd = S.st_o[r.s0 + sb + min(lane, sn)] - o0;
+      const int last = S.st_o[r.s0 + sb + sn] - o0;
+      // step s covers op slots [bnd(s), bnd(s+1)); bnd(64) is `last`
+      int t = __builtin_amdgcn_readlane(bnd, 0) + lane;
+      bool v = t < __builtin_amdgcn_readlane(bnd, 1);
+      int loc = v ? locs[t] : 0;
+      TF lv = v ? lvs[t] : TF(0);
+      for (int s = 0; s < sn; s++) {
+        const bool vc = v;
+        const int locc = loc;
+        const TF lvc = lv;
+        if (s + 1 < sn) {
+          const int tb = __builtin_amdgcn_readlane(bnd, s + 1);
+          const int te = s + 2 < WAVE ? __builtin_amdgcn_readlane(bnd, s + 2) : last;
+          t = tb + lane;
+          v = t < te;
+          loc = v ? locs[t] : 0;
+          lv = v ? lvs[t] : TF(0);
+        }
+        if (vc) col[locc & 0xffff] -= lvc * col[locc >> 16];
+        wave_sync();
+      }
+    }
+    if (!ext) {
+      piv = col[r.dl];
+      const TF inv = TF(1) / piv;
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const int t = lane + WAVE * j;
+        if (t < len) {
+          TF x = col[t];
+          if (t > r.dl) {
+            x *= inv;
+            lu[c0 + t] = x;
+          }
+          if (l2s[j] >= 0) so[l2s[j]] = x;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const int t = lane + WAVE * j;
+        if (t < len) {
+          const TF x = col[t];
+          lu[c0 + t] = x;
+          if (t < r.dl && l2s[j] >= 0) so[l2s[j]] = x;
+        }
+      }
+    }
+  } else {
+    // in place in global memory, operands loaded per step
+    TF *x = lu + c0;
+    for (int s = r.s0; s < r.s1; s++) {
+      const int t = S.st_o[s] + lane;
+      if (t < S.st_o[s + 1]) {
+        const int loc = S.op_loc[t];
+        x[loc & 0xffff] -= lu[S.op_sl[t]] * x[loc >> 16];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (!ext) {
+      piv = x[r.dl];
+      const TF inv = TF(1) / piv;
+      for (int t = lane; t < len; t += WAVE) {
+        TF v = x[t];
+        if (t > r.dl) {
+          v *= inv;
+          x[t] = v;
+        }
+        const int d = S.lu2s[c0 + t];
+        if (d >= 0) so[d] = v;
+      }
+    } else {
+      for (int t = lane; t < r.dl; t += WAVE) {
+        const int d = S.lu2s[c0 + t];
+        if (d >= 0) so[d] = x[t];
+      }
+    }
+  }
+  wave_sync();
+  return (!(piv != TF(0)) || !isfinite((double)piv)) ? 1 : 0;
 }
 
 template <typename TF>
 __global__ void __launch_bounds__(512) k_splu_factor(SpluDev S, const double *__restrict__ var,
                                                      const double *__restrict__ cst, int ldM, TF *__restrict__ luv,
-                                                     TF *__restrict__ svv, const uint8_t *__restrict__ refac,
-                                                     int *__restrict__ status) {
+                                                     TF *__restrict__ svv, TF *__restrict__ dnv,
+                                                     const uint8_t *__restrict__ refac, int *__restrict__ status) {
   const int m = scenario_of_block(blockIdx.x, gridDim.x);
   if (!refac[m]) return;
   extern __shared__ __align__(16) char smem_raw[];
   TF *lu = luv + (size_t)m * S.nnz_lu;
-  TF *so = svv + (size_t)m * S.nnz_lu;
+  TF *so = svv + (size_t)m * S.ns;
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
   for (int p = tid; p < S.nnz_lu; p += nt) lu[p] = TF(0);
   __syncthreads();
@@ -1195,167 +1435,351 @@ __global__ void __launch_bounds__(512) k_splu_factor(SpluDev S, const double *__
   GP_TRACE(0, 0);
   int bad = 0;
   for (int l = 0; l < S.n_flev; l++) {
-    const int cb = S.fac_lp[l], ce = S.fac_lp[l + 1];
-    if (S.fac_mode[l] == 0) {
+    const int rb = S.flp[l], re = S.flp[l + 1], mode = S.fac_mode[l];
+    if (mode == 0) {
       TF *buf = reinterpret_cast<TF *>(smem_raw) + (size_t)tid * S.t_cols;
-      for (int idx = cb + tid; idx < ce; idx += nt) {
-        const int k = S.fac_cols[idx];
-        const int c0 = S.lu_cp[k], len = S.lu_cp[k + 1] - c0, dl = S.lu_dp[k] - c0;
-        if (len <= S.t_cols) {
-          for (int t = 0; t < len; t++) buf[t] = lu[c0 + t];
-          col_ops_thread<TF>(S, lu, buf, S.op_cp[k], S.op_cp[k + 1]);
-          const TF piv = buf[dl];
-          if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
-          const TF inv = TF(1) / piv;
-          for (int t = 0; t < len; t++) {
-            TF v = buf[t];
-            if (t > dl) {
-              v *= inv;
-              lu[c0 + t] = v;
-            }
-            so[S.lu2s[c0 + t]] = v;
-          }
-        } else {
-          TF *x = lu + c0;
-          col_ops_thread<TF>(S, lu, x, S.op_cp[k], S.op_cp[k + 1]);
-          const TF piv = x[dl];
-          if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
-          const TF inv = TF(1) / piv;
-          for (int t = 0; t < len; t++) {
-            TF v = x[t];
-            if (t > dl) {
-              v *= inv;
-              x[t] = v;
-            }
-            so[S.lu2s[c0 + t]] = v;
-          }
-        }
-      }
+      for (int idx = rb + tid; idx < re; idx += nt) bad |= thread_column<TF>(S, lu, so, load_rec(S.crec, idx), buf);
     } else {
       char *wb = smem_raw + (size_t)wave * S.w_bytes;
       TF *col = reinterpret_cast<TF *>(wb);
       TF *lvs = col + S.w_col;
       int *locs = reinterpret_cast<int *>(lvs + S.w_ops);
-      for (int idx = cb + wave; idx < ce; idx += nw) {
-        const int k = S.fac_cols[idx];
-        const int c0 = S.lu_cp[k], len = S.lu_cp[k + 1] - c0, dl = S.lu_dp[k] - c0;
-        const int o0 = S.op_cp[k], nops = S.op_cp[k + 1] - o0;
-        const int s0 = S.st_cp[k], s1 = S.st_cp[k + 1];
-        TF piv;
-        if (len <= S.w_col && nops <= S.w_ops) {
-          // stage the column, its op locations and all its multipliers
-          for (int t = lane; t < len; t += WAVE) col[t] = lu[c0 + t];
-          for (int t = lane; t < nops; t += WAVE) {
-            locs[t] = S.op_loc[o0 + t];
-            lvs[t] = lu[S.op_sl[o0 + t]];
-          }
-          wave_sync();
-          for (int sb = s0; sb < s1; sb += WAVE) {
-            const int sn = min(WAVE, s1 - sb);
-            // lane t holds the first op of step sb + t (t <= sn)
-            const int bnd = lane <= sn ? S.st_o[sb + lane] - o0 : 0;
-            const int last = S.st_o[sb + sn] - o0;
-            for (int s = 0; s < sn; s++) {
-              const int ob = __builtin_amdgcn_readlane(bnd, s);
-              const int oe = s + 1 < WAVE ? __builtin_amdgcn_readlane(bnd, s + 1) : last;
-              const int t = ob + lane;
-              if (t < oe) {
-                const int loc = locs[t];
-                col[loc & 0xffff] -= lvs[t] * col[loc >> 16];
-              }
-              wave_sync();
-            }
-          }
-          piv = col[dl];
-          const TF inv = TF(1) / piv;
-          for (int t = lane; t < len; t += WAVE) {
-            TF v = col[t];
-            if (t > dl) {
-              v *= inv;
-              lu[c0 + t] = v;
-            }
-            so[S.lu2s[c0 + t]] = v;
-          }
-        } else {
-          // in place in global memory, operands loaded per step
-          TF *x = lu + c0;
-          for (int s = s0; s < s1; s++) {
-            const int t = S.st_o[s] + lane;
-            if (t < S.st_o[s + 1]) {
-              const int loc = S.op_loc[t];
-              x[loc & 0xffff] -= lu[S.op_sl[t]] * x[loc >> 16];
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-          }
-          piv = x[dl];
-          const TF inv = TF(1) / piv;
-          for (int t = lane; t < len; t += WAVE) {
-            TF v = x[t];
-            if (t > dl) {
-              v *= inv;
-              x[t] = v;
-            }
-            so[S.lu2s[c0 + t]] = v;
-          }
-        }
-        if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
-        wave_sync();
-      }
+      for (int idx = rb + wave; idx < re; idx += nw)
+        bad |= wave_column<TF>(S, lu, so, load_rec(S.crec, idx), mode == 2, col, lvs, locs, lane);
     }
     __syncthreads();
     GP_TRACE(0, l + 1);
   }
+  if (S.nd) {
+    // dense trailing block: right-looking LU without pivoting, in LDS
+    const int nd = S.nd;
+    TF *D = reinterpret_cast<TF *>(smem_raw);
+    for (int i = tid; i < nd * nd; i += nt) D[i] = TF(0);
+    __syncthreads();
+    for (int q = tid; q < S.nnz_top; q += nt) {
+      const int d = S.dmap[q];
+      if (d >= 0) D[d] = lu[S.qb + q];
+    }
+    __syncthreads();
+    // thread (il, jl) owns rows i = p+1+il (+64) and columns j = p+1+jl (+8k) of
+    // the trailing update; its operands are read in one batch, and the
+    // multipliers of pivot p are written back after the next barrier, when
+    // nobody reads column p any more
+    const int il = tid & 63, jl = tid >> 6;
+    TF lkeep[2] = {TF(0), TF(0)};
+    int pkeep = -1;
+    for (int p = 0; p < nd; p++) {
+      if (jl == 0 && pkeep >= 0) {
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          const int i = pkeep + 1 + il + WAVE * a;
+          if (i < nd) D[i + pkeep * nd] = lkeep[a];
+        }
+      }
+      const TF piv = D[p + p * nd];
+      if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
+      const TF inv = TF(1) / piv;
+      TF li[2];
+      int ir[2];
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const int i = p + 1 + il + WAVE * a;
+        ir[a] = min(i, nd - 1);
+        li[a] = D[ir[a] + p * nd] * inv;
+        if (i >= nd) ir[a] = -1;
+      }
+      const int ncol = nd - p - 1 - jl;   // columns j = p+1+jl+8b, b < ceil(ncol/8)
+      for (int b0 = 0; 8 * b0 < ncol; b0 += 4) {
+        TF uj[4], d0[4], d1[4];
+        int jj[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int j = p + 1 + jl + 8 * (b0 + u);
+          jj[u] = j < nd ? j : -1;
+          const int jc = min(j, nd - 1);
+          uj[u] = D[p + jc * nd];
+          d0[u] = D[max(ir[0], 0) + jc * nd];
+          d1[u] = D[max(ir[1], 0) + jc * nd];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (jj[u] >= 0) {
+            if (ir[0] >= 0) D[ir[0] + jj[u] * nd] = d0[u] - li[0] * uj[u];
+            if (ir[1] >= 0) D[ir[1] + jj[u] * nd] = d1[u] - li[1] * uj[u];
+          }
+        }
+      }
+      lkeep[0] = li[0];
+      lkeep[1] = li[1];
+      pkeep = p;
+      __syncthreads();
+    }
+    if (jl == 0 && pkeep >= 0) {
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const int i = pkeep + 1 + il + WAVE * a;
+        if (i < nd) D[i + pkeep * nd] = lkeep[a];
+      }
+    }
+    __syncthreads();
+    TF *dn = dnv + (size_t)m * nd * nd;
+    for (int i = tid; i < nd * nd; i += nt) dn[i] = D[i];
+    GP_TRACE(0, S.n_flev + 1);
+  }
   if (bad) atomicOr(&status[m], 1);
 }
 
-// One workgroup per scenario: x = A^{-1} b with b gathered from rhs (row
-// space), the solution kept in LDS through both level-scheduled triangular
-// solves, then zupd[scatter[j]] += sign * x[j] and xout[j] = x[j].  Each
-// row of a level is reduced by g cooperating lanes (g per level, a power of
-// two) reading the row's contiguous slice of the solve stream.
+// One level of a sliced-ELL triangular solve for the lanes u = tid, tid+nt, ...
+// The registers prefetched for the first lane hold its first GP_PF entries
+// and the level's shape, so the level itself reads no table.
+#define GP_PF 4
 template <typename TF>
-__global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restrict__ svv,
+struct SolvePre {
+  TF v[GP_PF];
+  int c[GP_PF];
+  int row, pf, base, T, g, R, cb;
+  TF dg;
+};
+
+// DPP lane exchange of a double within a row of 16 lanes
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over aligned groups of g lanes (g uniform, a power of two <= 64);
+// every lane of a group ends with the group's sum.  Lanes 1, 2 apart swap by
+// quad permutes, 4 and 8 apart by the half-row and row mirrors (after the
+// previous steps each half holds one partial sum, so a mirror pairs halves).
+__device__ __forceinline__ double group_sum(double acc, int g) {
+  if (g > 1) acc += dpp_d<0xB1>(acc);    // quad_perm [1,0,3,2]
+  if (g > 2) acc += dpp_d<0x4E>(acc);    // quad_perm [2,3,0,1]
+  if (g > 4) acc += dpp_d<0x141>(acc);   // row_half_mirror
+  if (g > 8) acc += dpp_d<0x140>(acc);   // row_mirror
+  if (g > 16) acc += __shfl_xor(acc, 16);
+  if (g > 32) acc += __shfl_xor(acc, 32);
+  return acc;
+}
+
+// Branch-free so that nothing waits for these loads before they are used:
+// lanes past the level and entries past a chunk's pf read a valid slot of the
+// same chunk (every chunk has pf >= 1), masked out where they are used.
+template <typename TF, bool BW>
+__device__ __forceinline__ void solve_fetch(const int *__restrict__ lt, const int *__restrict__ ct, int l,
+                                            const TF *__restrict__ vals, const int *__restrict__ cols,
+                                            const int *__restrict__ rows, const TF *__restrict__ dg, int tid,
+                                            SolvePre<TF> &P) {
+  const int4 lv = reinterpret_cast<const int4 *>(lt)[l];
+  const int T = lv.x;
+  const int u = min(tid, T - 1);
+  const int c = lv.w + (u >> 6);
+  const int2 ch = reinterpret_cast<const int2 *>(ct)[c];
+  const int base = ch.x + (u & 63);
+#pragma unroll
+  for (int j = 0; j < GP_PF; j++) {
+    const int at = base + WAVE * min(j, ch.y - 1);
+    P.v[j] = vals[at];
+    P.c[j] = cols[at];
+  }
+  P.T = T;
+  P.g = lv.y;
+  P.R = lv.z;
+  P.cb = lv.w;
+  P.pf = ch.y;
+  P.base = base;
+  P.row = rows[lv.z + u];
+  if (BW) P.dg = dg[lv.z + u];
+}
+
+template <typename TF, bool BW>
+__device__ __forceinline__ void solve_level(const int *__restrict__ ct, const TF *__restrict__ vals,
+                                            const int *__restrict__ cols, const int *__restrict__ rows,
+                                            const TF *__restrict__ dg, double *x, int tid, int nt,
+                                            const SolvePre<TF> &P) {
+  const int T = P.T, g = P.g;
+  const int sub = tid & (g - 1);
+  // first lane: prefetched
+  {
+    double acc = 0.0;
+    if (tid < T) {
+#pragma unroll
+      for (int j = 0; j < GP_PF; j++)
+        if (j < P.pf) acc += (double)P.v[j] * x[P.c[j]];
+      for (int j = GP_PF; j < P.pf; j++) {
+        const int at = P.base + WAVE * j;
+        acc += (double)vals[at] * x[cols[at]];
+      }
+    }
+    acc = group_sum(acc, g);
+    if (tid < T && sub == 0) {
+      if (BW)
+        x[P.row] = (x[P.row] - acc) / (double)P.dg;
+      else
+        x[P.row] -= acc;
+    }
+  }
+  // further lanes (levels wider than the workgroup)
+  for (int u = tid + nt; u < ((T + nt - 1) / nt) * nt; u += nt) {
+    double acc = 0.0;
+    if (u < T) {
+      const int c = P.cb + (u >> 6);
+      const int E = ct[2 * c], pf = ct[2 * c + 1];
+      for (int j = 0; j < pf; j++) {
+        const int at = E + (u & 63) + WAVE * j;
+        acc += (double)vals[at] * x[cols[at]];
+      }
+    }
+    acc = group_sum(acc, g);
+    if (u < T && sub == 0) {
+      const int row = rows[P.R + u];
+      if (BW)
+        x[row] = (x[row] - acc) / (double)dg[P.R + u];
+      else
+        x[row] -= acc;
+    }
+  }
+}
+
+// One workgroup per scenario: x = A^{-1} b with b gathered from rhs (row
+// space), the solution kept in LDS: forward levels of the sparse part and of
+// the block rows' external entries, the dense block's forward and backward
+// substitution (one wavefront, x of the block in registers), then the sparse
+// backward levels; then zupd[scatter[j]] += sign * x[j] and xout[j] = x[j].
+// Each level's loads are issued while the previous level computes.
+template <typename TF>
+__global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restrict__ svv, const TF *__restrict__ dnv,
                                                     const double *__restrict__ rhs, int ldM, double sign,
                                                     const int *__restrict__ scatter, double *__restrict__ zupd,
                                                     double *__restrict__ xout, const uint8_t *__restrict__ work) {
   const int m = scenario_of_block(blockIdx.x, gridDim.x);
   if (!work[m]) return;
   extern __shared__ __align__(16) char smem_raw[];
+  const int n = S.n, nd = S.nd, B = n - nd;
   double *x = reinterpret_cast<double *>(smem_raw);
-  const TF *lv = svv + (size_t)m * S.nnz_lu;
-  const TF *uv = lv + S.nnz_l;
-  const TF *dg = uv + S.nnz_u;
-  const int tid = threadIdx.x, nt = blockDim.x, n = S.n;
+  // LDS: x, then the level tables (16-byte aligned for their vector reads),
+  // then the dense block
+  int *flt = reinterpret_cast<int *>(smem_raw + (((size_t)n * 8 + 15) & ~(size_t)15));
+  int *fct = flt + 4 * S.n_fwlev;
+  int *blt = fct + ((2 * S.n_fwch + 3) & ~3);
+  int *bct = blt + 4 * S.n_bwlev;
+  TF *Dl = reinterpret_cast<TF *>(reinterpret_cast<char *>(bct) + (((size_t)8 * S.n_bwch + 15) & ~(size_t)15));
+  const TF *fv = svv + (size_t)m * S.ns;
+  const TF *bv = fv + S.fw_nvals;
+  const TF *bd = bv + S.bw_nvals;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int i = tid; i < 4 * S.n_fwlev; i += nt) flt[i] = S.fw_lvl[i];
+  for (int i = tid; i < 2 * S.n_fwch; i += nt) fct[i] = S.fw_chunk[i];
+  for (int i = tid; i < 4 * S.n_bwlev; i += nt) blt[i] = S.bw_lvl[i];
+  for (int i = tid; i < 2 * S.n_bwch; i += nt) bct[i] = S.bw_chunk[i];
   for (int r = tid; r < n; r += nt) x[S.pinv[r]] = rhs[(size_t)r * ldM + m];
+  if (nd) {
+    const TF *dn = dnv + (size_t)m * nd * nd;
+    for (int i = tid; i < nd * nd; i += nt) Dl[i] = dn[i];
+  }
   __syncthreads();
   GP_TRACE(1, 0);
-  for (int l = 0; l < S.n_fwlev; l++) {
-    const int g = S.fw_g[l];
-    const int sub = tid & (g - 1), ng = nt / g;
-    for (int slot = S.fw_lp[l] + tid / g; slot < S.fw_lp[l + 1]; slot += ng) {
-      const int i = S.fw_info[3 * slot], b = S.fw_info[3 * slot + 1], e = S.fw_info[3 * slot + 2];
-      double acc = 0.0;
-      for (int p = b + sub; p < e; p += g) acc += (double)lv[p] * x[S.ls_k[p]];
-      for (int off = g >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-      if (sub == 0) x[i] -= acc;
+  // three register sets in rotation: the loads for level l + 2 are issued
+  // while level l computes, with no copy between the sets
+  SolvePre<TF> pa, pb, pc;
+  const int nf = S.n_fwlev, nb = S.n_bwlev;
+#define GP_FW_FETCH(L, P) solve_fetch<TF, false>(flt, fct, L, fv, S.fw_cols, S.fw_rows, bd, tid, P)
+#define GP_FW_LEVEL(L, P)                                                      \
+  do {                                                                         \
+    solve_level<TF, false>(fct, fv, S.fw_cols, S.fw_rows, bd, x, tid, nt, P);          \
+    lds_barrier();                                                             \
+    GP_TRACE(1, (L) + 1);                                                      \
+  } while (0)
+#define GP_BW_FETCH(L, P) solve_fetch<TF, true>(blt, bct, L, bv, S.bw_cols, S.bw_rows, bd, tid, P)
+#define GP_BW_LEVEL(L, P)                                                      \
+  do {                                                                         \
+    solve_level<TF, true>(bct, bv, S.bw_cols, S.bw_rows, bd, x, tid, nt, P);           \
+    lds_barrier();                                                             \
+    GP_TRACE(1, nf + 2 + (L));                                                 \
+  } while (0)
+  if (nf > 0) GP_FW_FETCH(0, pa);
+  if (nf > 1) GP_FW_FETCH(1, pb);
+  for (int l = 0; l < nf; l += 3) {
+    if (l + 2 < nf) GP_FW_FETCH(l + 2, pc);
+    GP_FW_LEVEL(l, pa);
+    if (l + 1 < nf) {
+      if (l + 3 < nf) GP_FW_FETCH(l + 3, pa);
+      GP_FW_LEVEL(l + 1, pb);
     }
-    __syncthreads();
-    GP_TRACE(1, l + 1);
-  }
-  for (int l = 0; l < S.n_bwlev; l++) {
-    const int g = S.bw_g[l];
-    const int sub = tid & (g - 1), ng = nt / g;
-    for (int slot = S.bw_lp[l] + tid / g; slot < S.bw_lp[l + 1]; slot += ng) {
-      const int k = S.bw_info[3 * slot], b = S.bw_info[3 * slot + 1], e = S.bw_info[3 * slot + 2];
-      double acc = 0.0;
-      for (int p = b + sub; p < e; p += g) acc += (double)uv[p] * x[S.us_j[p]];
-      for (int off = g >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-      if (sub == 0) x[k] = (x[k] - acc) / (double)dg[k];
+    if (l + 2 < nf) {
+      if (l + 4 < nf) GP_FW_FETCH(l + 4, pb);
+      GP_FW_LEVEL(l + 2, pc);
     }
-    __syncthreads();
-    GP_TRACE(1, S.n_fwlev + l + 1);
   }
+  if (nb > 0) GP_BW_FETCH(0, pa);
+  if (nb > 1) GP_BW_FETCH(1, pb);
+  if (nd && tid < WAVE) {
+    // dense block, one wavefront: lane i holds x[B + i] and x[B + 64 + i];
+    // the block's values for 8 pivots are read ahead of their use
+    const int i0 = tid, i1 = tid + WAVE;
+    double x0 = i0 < nd ? x[B + i0] : 0.0, x1 = i1 < nd ? x[B + i1] : 0.0;
+    for (int p0 = 0; p0 < nd; p0 += 8) {
+      TF a0[8], a1[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int p = min(p0 + u, nd - 1);
+        a0[u] = Dl[min(i0, nd - 1) + p * nd];
+        a1[u] = Dl[min(i1, nd - 1) + p * nd];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int p = p0 + u;
+        if (p < nd) {
+          const double xp = p < WAVE ? readlane_d(x0, p) : readlane_d(x1, p - WAVE);
+          if (i0 > p && i0 < nd) x0 -= (double)a0[u] * xp;
+          if (i1 > p && i1 < nd) x1 -= (double)a1[u] * xp;
+        }
+      }
+    }
+    for (int p0 = nd - 1; p0 >= 0; p0 -= 8) {
+      TF a0[8], a1[8], dg[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int p = max(p0 - u, 0);
+        a0[u] = Dl[min(i0, nd - 1) + p * nd];
+        a1[u] = Dl[min(i1, nd - 1) + p * nd];
+        dg[u] = Dl[p + p * nd];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int p = p0 - u;
+        if (p >= 0) {
+          const double xp = (p < WAVE ? readlane_d(x0, p) : readlane_d(x1, p - WAVE)) / (double)dg[u];
+          if (i0 == p) x0 = xp;
+          if (i1 == p) x1 = xp;
+          if (i0 < p) x0 -= (double)a0[u] * xp;
+          if (i1 < p && i1 < nd) x1 -= (double)a1[u] * xp;
+        }
+      }
+    }
+    if (i0 < nd) x[B + i0] = x0;
+    if (i1 < nd) x[B + i1] = x1;
+  }
+  lds_barrier();
+  GP_TRACE(1, nf + 1);
+  for (int l = 0; l < nb; l += 3) {
+    if (l + 2 < nb) GP_BW_FETCH(l + 2, pc);
+    GP_BW_LEVEL(l, pa);
+    if (l + 1 < nb) {
+      if (l + 3 < nb) GP_BW_FETCH(l + 3, pa);
+      GP_BW_LEVEL(l + 1, pb);
+    }
+    if (l + 2 < nb) {
+      if (l + 4 < nb) GP_BW_FETCH(l + 4, pb);
+      GP_BW_LEVEL(l + 2, pc);
+    }
+  }
+#undef GP_FW_FETCH
+#undef GP_FW_LEVEL
+#undef GP_BW_FETCH
+#undef GP_BW_LEVEL
   for (int k = tid; k < n; k += nt) {
     const int j = S.q[k];
     const double v = x[k];
@@ -1399,11 +1823,11 @@ void launch_bus_residual(const DevTables &T, const double *p, const double *z, c
 }
 void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, double tol, double *fnorm,
                          unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
-                         double theta, hipStream_t s) {
+                         double theta, int stall, double *fbest, int *nstall, uint8_t *failed, hipStream_t s) {
   const int chunks = std::max(1, std::min((n + 15) / 16, 1024 / (ldM / WAVE)));
   hipLaunchKernelGGL(k_rownorm, dim3(ldM / WAVE, chunks), dim3(256), 0, s, n, ldM, M, f, work, nb);
   hipLaunchKernelGGL(k_norm_finish, dim3((M + 255) / 256), dim3(256), 0, s, M, tol, nb, fnorm, work, n_unconv, refac,
-                     refac0, first, theta);
+                     refac0, first, theta, stall, fbest, nstall, failed);
 }
 void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs, double *jv,
                      const uint8_t *work, hipStream_t s) {
@@ -1429,22 +1853,22 @@ void launch_cluster_backsub(const DevTables &T, const TF *X, const double *f, co
   hipLaunchKernelGGL(k_cluster_backsub<TF>, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, X, dv, z, work);
 }
 template <typename TF>
-void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv, TF *sv,
+void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv, TF *sv, TF *dn,
                              const uint8_t *refac, int M, int *status, hipStream_t s) {
   if (S.lds_factor > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_splu_factor<TF>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, S.lds_factor);
-  hipLaunchKernelGGL(k_splu_factor<TF>, dim3(M), dim3(S.threads), S.lds_factor, s, S, var, cst, ldM, luv, sv, refac,
-                     status);
+  hipLaunchKernelGGL(k_splu_factor<TF>, dim3(M), dim3(S.threads), S.lds_factor, s, S, var, cst, ldM, luv, sv, dn,
+                     refac, status);
 }
 template <typename TF>
-void launch_splu_solve(const SpluDev &S, const TF *sv, const double *rhs, int ldM, double sign, const int *scatter,
-                       double *zupd, double *xout, const uint8_t *work, int M, hipStream_t s) {
+void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const double *rhs, int ldM, double sign,
+                       const int *scatter, double *zupd, double *xout, const uint8_t *work, int M, hipStream_t s) {
   if (S.lds_solve > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_splu_solve<TF>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, S.lds_solve);
-  hipLaunchKernelGGL(k_splu_solve<TF>, dim3(M), dim3(S.threads), S.lds_solve, s, S, sv, rhs, ldM, sign, scatter, zupd,
-                     xout, work);
+  hipLaunchKernelGGL(k_splu_solve<TF>, dim3(M), dim3(S.threads), S.lds_solve, s, S, sv, dn, rhs, ldM, sign, scatter,
+                     zupd, xout, work);
 }
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s) {
   const int gx = n < 1024 ? n : 1024;
@@ -1465,12 +1889,12 @@ template void launch_cluster_backsub<double>(const DevTables &, const double *, 
 template void launch_cluster_backsub<float>(const DevTables &, const float *, const double *, const double *,
                                             double *, const uint8_t *, hipStream_t);
 template void launch_splu_factor_impl<double>(const SpluDev &, const double *, const double *, int, double *,
-                                              double *, const uint8_t *, int, int *, hipStream_t);
+                                              double *, double *, const uint8_t *, int, int *, hipStream_t);
 template void launch_splu_factor_impl<float>(const SpluDev &, const double *, const double *, int, float *, float *,
-                                             const uint8_t *, int, int *, hipStream_t);
-template void launch_splu_solve<double>(const SpluDev &, const double *, const double *, int, double, const int *,
-                                        double *, double *, const uint8_t *, int, hipStream_t);
-template void launch_splu_solve<float>(const SpluDev &, const float *, const double *, int, double, const int *,
-                                       double *, double *, const uint8_t *, int, hipStream_t);
+                                             float *, const uint8_t *, int, int *, hipStream_t);
+template void launch_splu_solve<double>(const SpluDev &, const double *, const double *, const double *, int, double,
+                                        const int *, double *, double *, const uint8_t *, int, hipStream_t);
+template void launch_splu_solve<float>(const SpluDev &, const float *, const float *, const double *, int, double,
+                                       const int *, double *, double *, const uint8_t *, int, hipStream_t);
 
 }  // namespace gp

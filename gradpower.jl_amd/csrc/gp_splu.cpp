@@ -300,10 +300,19 @@ std::string splu_analyse(int n, const int *cp, const int *ri, const double *ax, 
   return "";
 }
 
-// Cost units below are one dependent update (an LDS read-modify-write, or a
-// gathered load followed by one).  Thread mode pays every op of its longest
-// column; wavefront mode pays one unit per step plus the column's staging.
-void splu_schedule(SpluAnalysis &A, int threads) {
+// ---- launch schedule ------------------------------------------------------------
+// Cost model in GPU cycles, fitted to level timings of the factor kernel on a
+// 4,300-row reduced network: a thread-mode round pays a fixed latency plus
+// ~220 cycles per op of its longest column; a wavefront-mode column pays its
+// staging plus ~40 cycles per op and ~150 per step; the dense block pays
+// ~400 cycles per pivot.
+namespace {
+long long thread_level_cost(long long rounds, long long maxops) { return rounds * (10000 + 220 * maxops); }
+long long wave_col_cost(long long nops, long long steps) { return 4000 + 40 * nops + 150 * steps; }
+long long dense_cost(int nd) { return nd ? 6000 + 400LL * nd : 0; }
+}  // namespace
+
+void splu_schedule(SpluAnalysis &A, int threads, int nd_max) {
   const int n = A.n, W = threads / 64;
   const int nl = (int)A.fac_lp.size() - 1;
   // steps: maximal runs of ops with the same source entry, cut at 64
@@ -321,51 +330,236 @@ void splu_schedule(SpluAnalysis &A, int threads) {
   }
   A.st_cp[n] = (int)A.st_o.size();
   A.st_o.push_back(A.op_cp[n]);
-  A.fac_mode.assign(nl, 0);
-  A.t_cols_max = A.w_col_max = A.w_ops_max = 0;
-  std::vector<long long> wsum(W);
-  for (int l = 0; l < nl; l++) {
-    const int c0 = A.fac_lp[l], c1 = A.fac_lp[l + 1];
-    int maxops = 0;
-    std::fill(wsum.begin(), wsum.end(), 0);
-    for (int i = c0; i < c1; i++) {
-      const int k = A.fac_cols[i];
-      maxops = std::max(maxops, A.op_cp[k + 1] - A.op_cp[k]);
-      wsum[(i - c0) % W] += (A.st_cp[k + 1] - A.st_cp[k]) + 6;
+  // external steps of column k >= B: the steps whose source column is < B
+  // (sources are visited in ascending order, so they form a prefix)
+  auto ext_steps = [&](int k, int B) {
+    const int c0 = A.lu_cp[k];
+    int s = A.st_cp[k];
+    while (s < A.st_cp[k + 1] && A.lu_ri[c0 + (A.op_loc[A.st_o[s]] >> 16)] < B) s++;
+    return s;
+  };
+  auto level_cost = [&](const std::vector<int> &cols, int B, bool ext, int &mode) {
+    long long maxops = 0;
+    std::vector<long long> wsum(W, 0);
+    for (size_t i = 0; i < cols.size(); i++) {
+      const int k = cols[i];
+      const int se = ext ? ext_steps(k, B) : A.st_cp[k + 1];
+      const int nops = A.st_o[se] - A.op_cp[k];
+      maxops = std::max<long long>(maxops, nops);
+      wsum[i % W] += wave_col_cost(nops, se - A.st_cp[k]);
     }
-    const long long tcost = (long long)((c1 - c0 + threads - 1) / threads) * (maxops + 4);
-    const long long wcost = *std::max_element(wsum.begin(), wsum.end());
-    A.fac_mode[l] = wcost < tcost ? 1 : 0;
-    for (int i = c0; i < c1; i++) {
-      const int k = A.fac_cols[i];
-      const int len = A.lu_cp[k + 1] - A.lu_cp[k], nops = A.op_cp[k + 1] - A.op_cp[k];
-      if (A.fac_mode[l]) {
+    const long long tc = thread_level_cost((long long)(cols.size() + threads - 1) / threads, maxops);
+    const long long wc = *std::max_element(wsum.begin(), wsum.end());
+    mode = ext ? 2 : (wc < tc ? 1 : 0);
+    return ext ? wc : std::min(tc, wc);
+  };
+  // sparse levels restricted to columns < B
+  auto sparse_levels = [&](int B) {
+    std::vector<std::vector<int>> lv;
+    for (int l = 0; l < nl; l++) {
+      std::vector<int> c;
+      for (int i = A.fac_lp[l]; i < A.fac_lp[l + 1]; i++)
+        if (A.fac_cols[i] < B) c.push_back(A.fac_cols[i]);
+      if (!c.empty()) lv.push_back(std::move(c));
+    }
+    return lv;
+  };
+  // dense block size: the trailing nd pivots, nd * nd * 8 bytes within LDS
+  int best_nd = 0;
+  long long best = -1;
+  for (int nd = 0; nd <= std::min(std::min(nd_max, 128), n / 2); nd += 16) {
+    const int B = n - nd;
+    long long cost = dense_cost(nd);
+    int mode;
+    for (auto &c : sparse_levels(B)) cost += 2000 + level_cost(c, B, false, mode);
+    if (nd) {
+      std::vector<int> top;
+      for (int k = B; k < n; k++) top.push_back(k);
+      cost += 2000 + level_cost(top, B, true, mode);
+    }
+    if (best < 0 || cost < best) { best = cost; best_nd = nd; }
+  }
+  A.nd = best_nd;
+  const int B = n - A.nd;
+  A.est_cycles = best;
+  // factor levels and column records
+  auto lvls = sparse_levels(B);
+  if (A.nd) {
+    std::vector<int> top;
+    for (int k = B; k < n; k++) top.push_back(k);
+    lvls.push_back(top);
+  }
+  A.flp.assign(1, 0);
+  A.fac_mode.clear();
+  A.crec.clear();
+  A.t_cols_max = A.w_col_max = A.w_ops_max = 0;
+  for (size_t l = 0; l < lvls.size(); l++) {
+    const bool ext = A.nd && l + 1 == lvls.size();
+    int mode;
+    level_cost(lvls[l], B, ext, mode);
+    A.fac_mode.push_back(mode);
+    for (int k : lvls[l]) {
+      const int c0 = A.lu_cp[k], len = A.lu_cp[k + 1] - c0;
+      const int se = ext ? ext_steps(k, B) : A.st_cp[k + 1];
+      const int nops = A.st_o[se] - A.op_cp[k];
+      int d = A.lu_dp[k] - c0;
+      if (ext) {   // rows above the block: final after the external updates
+        d = 0;
+        while (d < len && A.lu_ri[c0 + d] < B) d++;
+      }
+      const int r[8] = {k, c0, len, d, A.op_cp[k], nops, A.st_cp[k], se};
+      A.crec.insert(A.crec.end(), r, r + 8);
+      if (mode) {
         A.w_col_max = std::max(A.w_col_max, len);
         A.w_ops_max = std::max(A.w_ops_max, nops);
       } else {
         A.t_cols_max = std::max(A.t_cols_max, len);
       }
     }
+    A.flp.push_back((int)A.crec.size() / 8);
   }
-  // solve levels: lanes per row g minimising passes * (ceil(len/g) + log2 g)
-  auto pick = [&](const std::vector<int> &lp, const std::vector<int> &rows, const std::vector<int> &rp,
-                  std::vector<int> &g) {
+  // dense block map: entry q of the block's columns -> (r - B) + (k - B) * nd, or -1 above the block
+  A.dmap.clear();
+  for (int k = B; k < n; k++)
+    for (int q = A.lu_cp[k]; q < A.lu_cp[k + 1]; q++) {
+      const int r = A.lu_ri[q];
+      A.dmap.push_back(r >= B ? (r - B) + (k - B) * A.nd : -1);
+    }
+  // stream ranges per row
+  std::vector<int> lb(n), le(n), ub(n), ue(n);
+  for (int i = 0; i < n; i++) {
+    lb[A.fw_info[3 * i]] = A.fw_info[3 * i + 1];
+    le[A.fw_info[3 * i]] = A.fw_info[3 * i + 2];
+    ub[A.bw_info[3 * i]] = A.bw_info[3 * i + 1];
+    ue[A.bw_info[3 * i]] = A.bw_info[3 * i + 2];
+  }
+  // forward levels: rows < B keep their levels; the block rows form one more
+  // level that applies only their entries in columns < B
+  std::vector<int> flp{0}, finfo;
+  for (int l = 0; l + 1 < (int)A.fw_lp.size(); l++) {
+    bool any = false;
+    for (int i = A.fw_lp[l]; i < A.fw_lp[l + 1]; i++) {
+      const int r = A.fw_rows[i];
+      if (r >= B) continue;
+      finfo.insert(finfo.end(), {r, lb[r], le[r]});
+      any = true;
+    }
+    if (any) flp.push_back((int)finfo.size() / 3);
+  }
+  if (A.nd) {
+    for (int r = B; r < n; r++) {
+      int e = lb[r];
+      while (e < le[r] && A.ls_k[e] < B) e++;
+      finfo.insert(finfo.end(), {r, lb[r], e});
+    }
+    flp.push_back((int)finfo.size() / 3);
+  }
+  // backward levels of rows < B (the block rows are solved densely first)
+  std::vector<int> blev(n, -1);
+  int nbl = 0;
+  for (int k = B - 1; k >= 0; k--) {
+    int l = 0;
+    for (int e = ub[k]; e < ue[k]; e++) {
+      const int j = A.us_j[e];
+      if (j < B) l = std::max(l, blev[j] + 1);
+    }
+    blev[k] = l;
+    nbl = std::max(nbl, l + 1);
+  }
+  std::vector<int> blp(nbl + 1, 0), binfo(3 * (size_t)B);
+  for (int k = 0; k < B; k++) blp[blev[k] + 1]++;
+  for (int l = 0; l < nbl; l++) blp[l + 1] += blp[l];
+  {
+    std::vector<int> f(blp.begin(), blp.end() - 1);
+    for (int k = B - 1; k >= 0; k--) {
+      const int slot = f[blev[k]]++;
+      binfo[3 * slot] = k;
+      binfo[3 * slot + 1] = ub[k];
+      binfo[3 * slot + 2] = ue[k];
+    }
+  }
+  A.sfw_lp = flp;
+  A.sfw_info = finfo;
+  A.sbw_lp = blp;
+  A.sbw_info = binfo;
+  // ---- sliced-ELL solve layout ------------------------------------------------
+  // Each level's rows (longest first) get g lanes each; lanes are cut into
+  // chunks of 64 (one wavefront), and a chunk stores its entries lane-major,
+  // padded to its longest lane: entry j of lane u of chunk c sits at
+  // E_c + 64 j + (u mod 64).  Every address a lane reads is then known
+  // without reading the matrix, so a level's loads are issued during the
+  // previous level.
+  A.lu2s.assign(A.nnz_lu, -1);
+  // stream index -> LU position (streams were laid out from lrow/urow order)
+  std::vector<int> lpos(A.nnz_l), upos(A.nnz_u);
+  for (int i = 0; i < n; i++) {
+    int sb = lb[i];
+    for (int e = A.lrow_p[i]; e < A.lrow_p[i + 1]; e++) lpos[sb++] = A.lrow_pos[e];
+    sb = ub[i];
+    for (int e = A.urow_p[i]; e < A.urow_p[i + 1]; e++) upos[sb++] = A.urow_pos[e];
+  }
+  auto build = [&](const std::vector<int> &lp, const std::vector<int> &info, const std::vector<int> &scol,
+                   const std::vector<int> &spos, SpluAnalysis::SolveSide &S, int base) {
+    S = SpluAnalysis::SolveSide{};
     const int nlev = (int)lp.size() - 1;
-    g.assign(nlev, 1);
+    int R = 0;
     for (int l = 0; l < nlev; l++) {
-      int maxlen = 0;
-      for (int i = lp[l]; i < lp[l + 1]; i++) maxlen = std::max(maxlen, rp[rows[i] + 1] - rp[rows[i]]);
-      const int nr = lp[l + 1] - lp[l];
-      long long best = -1;
+      std::vector<int> slots;
+      for (int i = lp[l]; i < lp[l + 1]; i++) slots.push_back(i);
+      std::stable_sort(slots.begin(), slots.end(), [&](int a, int b) {
+        return info[3 * a + 2] - info[3 * a + 1] > info[3 * b + 2] - info[3 * b + 1];
+      });
+      const int nr = (int)slots.size();
+      const int maxlen = nr ? info[3 * slots[0] + 2] - info[3 * slots[0] + 1] : 0;
+      int g = 1;
+      long long bestc = -1;
       for (int gg = 1, lg = 0; gg <= 64; gg *= 2, lg++) {
         const long long passes = ((long long)nr * gg + threads - 1) / threads;
         const long long c = passes * ((maxlen + gg - 1) / gg + lg + 2);
-        if (best < 0 || c < best) { best = c; g[l] = gg; }
+        if (bestc < 0 || c < bestc) { bestc = c; g = gg; }
       }
+      const int T = nr * g;
+      const int nch = (T + 63) / 64;
+      S.lvl.insert(S.lvl.end(), {T, g, R, (int)S.chunk.size() / 2});
+      for (int c = 0; c < nch; c++) {
+        // longest lane of the chunk: its first row is the longest
+        const int r0 = (64 * c) / g;
+        const int len0 = info[3 * slots[r0] + 2] - info[3 * slots[r0] + 1];
+        const int pf = std::max(1, (len0 + g - 1) / g);   // >= 1: every lane has a readable slot
+        const int E = S.nvals;
+        S.chunk.insert(S.chunk.end(), {E, pf});
+        S.nvals += 64 * pf;
+        S.cols.resize(S.nvals, 0);
+        for (int u = 64 * c; u < std::min(T, 64 * c + 64); u++) {
+          const int r = u / g, sub = u % g;
+          const int slot = slots[r];
+          for (int e = info[3 * slot + 1] + sub, j = 0; e < info[3 * slot + 2]; e += g, j++) {
+            const int at = E + 64 * j + (u - 64 * c);
+            S.cols[at] = scol[e];
+            A.lu2s[spos[e]] = base + at;
+          }
+        }
+      }
+      for (int u = 0; u < T; u++) S.rows.push_back(info[3 * slots[u / g]]);
+      R += T;
     }
   };
-  pick(A.fw_lp, A.fw_rows, A.lrow_p, A.fw_g);
-  pick(A.bw_lp, A.bw_rows, A.urow_p, A.bw_g);
+  build(A.sfw_lp, A.sfw_info, A.ls_k, lpos, A.sf, 0);
+  build(A.sbw_lp, A.sbw_info, A.us_j, upos, A.sb, A.sf.nvals);
+  // diagonal of row k stored per backward lane (every lane of the row's group)
+  A.ns = A.sf.nvals + A.sb.nvals + (int)A.sb.rows.size();
+  {
+    // the group's first lane carries the diagonal
+    const int dbase = A.sf.nvals + A.sb.nvals;
+    std::vector<int> first(n, -1);
+    for (int u = (int)A.sb.rows.size() - 1; u >= 0; u--) first[A.sb.rows[u]] = u;
+    for (int k = 0; k < B; k++)
+      if (first[k] >= 0) A.lu2s[A.lu_dp[k]] = dbase + first[k];
+  }
+  // the dense block's own entries are read from the dense copy, not the streams
+  for (int k = B; k < n; k++)
+    for (int q = A.lu_cp[k]; q < A.lu_cp[k + 1]; q++)
+      if (A.lu_ri[q] >= B) A.lu2s[q] = -1;
 }
 
 }  // namespace gp

@@ -56,20 +56,44 @@ struct SpluAnalysis {
   std::vector<int> fw_lp, fw_rows;   // forward levels
   std::vector<int> bw_lp, bw_rows;   // backward levels
   long long flops = 0;               // multiply-add pairs of one factorisation
-  // Launch schedule (splu_schedule).  A factorisation level runs either one
-  // thread per column (wide levels of short columns) or one wavefront per
-  // column (narrow levels of long columns).  In wavefront mode the updates of
-  // a column are split into steps: step s applies ops [st_o[s], st_o[s+1]),
-  // at most 64 of them, all with the same source column, one op per lane.
-  std::vector<int> fac_mode;         // [levels] 0 thread per column, 1 wavefront per column
+  // Launch schedule (splu_schedule).  The trailing nd pivots form a dense
+  // block (B = n - nd): its columns first receive the updates from columns
+  // < B (one extra level), then a dense LU without pivoting finishes it.
+  // A factorisation level runs one thread per column (mode 0: wide levels of
+  // short columns), one wavefront per column (mode 1: narrow levels of long
+  // columns) or the block's external updates (mode 2, wavefront per column).
+  // In wavefront mode the updates of a column are split into steps: step s
+  // applies ops [st_o[s], st_o[s+1]), at most 64 of them, all with the same
+  // source column, one op per lane.
+  int nd = 0;
+  long long est_cycles = 0;          // cost-model estimate of one factorisation
   std::vector<int> st_cp, st_o;      // [n+1] first step of column k; [steps+1] op bounds
-  std::vector<int> fw_g, bw_g;       // [levels] lanes cooperating on one row of a solve level
+  std::vector<int> fac_mode;         // [levels]
+  std::vector<int> flp;              // [levels+1] first column record of each level
+  std::vector<int> crec;             // 8 ints per column record: k, c0, len, dl (mode 2: rows
+                                     // above the block), o0, nops, first step, end step
+  std::vector<int> dmap;             // per entry of the block's columns: dense index or -1
+  // solve levels: rows < B (forward, then one level of the block rows'
+  // entries in columns < B) and rows < B (backward); 3 ints per slot: row,
+  // stream begin, stream end; g = lanes cooperating on one row
+  std::vector<int> sfw_lp, sfw_info, sbw_lp, sbw_info;
+  // Sliced-ELL solve layout (forward side sf over L, backward side sb over U).
+  struct SolveSide {
+    std::vector<int> lvl;     // 4 ints per level: lanes T, lanes per row g, first lane R, first chunk
+    std::vector<int> chunk;   // 2 ints per chunk of 64 lanes: first value E, entries per lane pf
+    std::vector<int> cols;    // [nvals] column of each value slot (0 in padding)
+    std::vector<int> rows;    // per lane (all levels): its row
+    int nvals = 0;
+  };
+  SolveSide sf, sb;
+  int ns = 0;                        // solve values per scenario: sf values, sb values, sb diagonals
   int t_cols_max = 0;                // largest column in a thread-mode level
   int w_col_max = 0, w_ops_max = 0;  // largest column / op count in a wavefront-mode level
 };
 
-// Build the launch schedule for workgroups of `threads` threads.
-void splu_schedule(SpluAnalysis &A, int threads);
+// Build the launch schedule for workgroups of `threads` threads, with a
+// dense trailing block of at most nd_max pivots.
+void splu_schedule(SpluAnalysis &A, int threads, int nd_max = 112);
 
 // Analyse the CSC matrix (cp, ri, representative values ax).  Returns an
 // empty string on success, otherwise an error message.

@@ -35,7 +35,7 @@ class GpError(RuntimeError):
 class COptions(C.Structure):
     _fields_ = [("record_all", C.c_int32), ("n_record", C.c_int32),
                 ("record_idx", C.POINTER(C.c_int32)), ("refactor_policy", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("stall_window", C.c_int32), ("reserved", C.c_int32 * 5), ("refactor_theta", C.c_double)]
 
 
 class CStats(C.Structure):
@@ -43,7 +43,7 @@ class CStats(C.Structure):
                 ("event_resolves", C.c_int64), ("refactorizations", C.c_int64),
                 ("n_not_converged", C.c_int32), ("n_nan", C.c_int32),
                 ("max_final_residual", C.c_double), ("elapsed_ms", C.c_double),
-                ("solve_ms", C.c_double)]
+                ("solve_ms", C.c_double), ("newton_rounds", C.c_int64), ("factor_launches", C.c_int64)]
 
 
 class CSymInfo(C.Structure):
@@ -109,7 +109,7 @@ class HipContext:
     """One library context: a system, M scenarios, device state on the current GPU."""
 
     def __init__(self, desc: SystemDesc, M: int, record_idx=None,
-                 refactor_policy=REFACTOR_EVERY_ITER):
+                 refactor_policy=REFACTOR_EVERY_ITER, stall_window=0, refactor_theta=0.0):
         self.lib = load()
         self.desc = desc
         self.M = int(M)
@@ -126,6 +126,8 @@ class HipContext:
             opt.record_idx = _i(self._rec)
             self.nrec = len(self._rec)
         opt.refactor_policy = refactor_policy
+        opt.stall_window = int(stall_window)
+        opt.refactor_theta = float(refactor_theta)
         self._opt = opt
         h = C.c_void_p()
         self._check(self.lib.gp_create(C.byref(self._cs), self.M, C.byref(opt), C.byref(h)))

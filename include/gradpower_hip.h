@@ -122,7 +122,8 @@ enum gp_refactor_policy {
                                   and reuse it for the later iterations (Schur methods)  */
   GP_REFACTOR_ADAPTIVE = 2     /* keep the factorisation across time steps; factorise
                                   again after a fault event or a change of dt, and
-                                  whenever an iteration fails to reduce |F|inf by 10x  */
+                                  whenever an iteration fails to reduce |F|inf by
+                                  refactor_theta (default 10x)                         */
 };
 
 typedef struct gp_options {
@@ -130,7 +131,15 @@ typedef struct gp_options {
   int32_t n_record;        /* number of recorded z indices when record_all == 0     */
   const int32_t *record_idx;
   int32_t refactor_policy; /* gp_refactor_policy for the Schur methods              */
-  int32_t reserved[7];
+  int32_t stall_window;    /* 0 (default, the reference's behaviour): a Newton solve
+                              runs to itermax.  k > 0: a scenario whose |F|inf has
+                              not halved within its last k iterations, or exceeds
+                              1e10, or is not finite, stops at once and is reported
+                              as not converged (failed contingencies cost k rounds
+                              of the lock step instead of itermax)               */
+  int32_t reserved[5];
+  double refactor_theta;   /* GP_REFACTOR_ADAPTIVE: refactorise when an iteration
+                              reduces |F|inf by less than this factor (0: 10)   */
 } gp_options;
 
 typedef struct gp_stats {
@@ -143,6 +152,8 @@ typedef struct gp_stats {
   double max_final_residual;   /* max over scenarios/steps of the accepted |F|inf */
   double elapsed_ms;           /* device time of the integration (HIP events)     */
   double solve_ms;             /* device time inside the integrator's Newton loop */
+  int64_t newton_rounds;       /* lock-step Newton iterations (launch rounds)      */
+  int64_t factor_launches;     /* rounds that launched a numeric factorisation     */
 } gp_stats;
 
 typedef struct gp_ctx gp_ctx;
