@@ -34,10 +34,11 @@ def events_of(case):
     return ev
 
 
-def run_method(case, method, steps, warmup, record_idx, tol, refactor_policy, stall_window, theta=0.0):
+def run_method(case, method, steps, warmup, record_idx, tol, refactor_policy, stall_window, theta=0.0,
+               async_below=0.0):
     from gradpower_hip.lib import HipContext
     ctx = HipContext(case.desc, case.M, record_idx=record_idx, refactor_policy=refactor_policy,
-                     stall_window=stall_window, refactor_theta=theta)
+                     stall_window=stall_window, refactor_theta=theta, async_below=async_below)
     ev = events_of(case)
     times, stats = [], None
     for it in range(warmup + steps):
@@ -93,8 +94,12 @@ def main():
     ap.add_argument("--horizon-cap", type=float, default=0.0,
                     help="cap every scenario horizon (s); profiling runs only, not the benchmark workload")
     ap.add_argument("--methods", default="schur32,schur64,mono")
+    ap.add_argument("--policy", default="async", choices=["async", "adaptive"],
+                    help="refactorisation policy of the Schur methods")
     ap.add_argument("--theta", type=float, default=10.0,
                     help="adaptive refactorisation: refactor when |F| falls by less than this factor")
+    ap.add_argument("--async-below", type=float, default=0.0,
+                    help="async policy: refactorisations asked for below this |F| run on the second stream")
     ap.add_argument("--stall-window", type=int, default=8,
                     help="stop a scenario whose |F| has not halved in this many Newton iterations (0: run to itermax)")
     args = ap.parse_args()
@@ -104,7 +109,8 @@ def main():
         sys.exit(2)
     from gradpower_hip import synth
     from gradpower_hip.desc import gen_speed_indices
-    from gradpower_hip.lib import METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32, REFACTOR_ADAPTIVE, REFACTOR_EVERY_ITER
+    from gradpower_hip.lib import (METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32, REFACTOR_ADAPTIVE,
+                                   REFACTOR_ASYNC, REFACTOR_EVERY_ITER)
 
     t0 = time.perf_counter()
     case = synth.case2_grid(M=args.M, nbus=args.nbus)
@@ -118,14 +124,16 @@ def main():
 
     methods = args.methods.split(",")
     runs = {}
-    for name, meth, pol in (("schur32", METHOD_SCHUR_FP32, REFACTOR_ADAPTIVE),
-                            ("schur64", METHOD_SCHUR_FP64, REFACTOR_ADAPTIVE),
+    spol = {"async": REFACTOR_ASYNC, "adaptive": REFACTOR_ADAPTIVE}[args.policy]
+    for name, meth, pol in (("schur32", METHOD_SCHUR_FP32, spol),
+                            ("schur64", METHOD_SCHUR_FP64, spol),
                             ("mono", METHOD_MONO_FP64, REFACTOR_EVERY_ITER)):
         if name not in methods or (name == "mono" and args.no_mono):
             continue
         nst = args.steps if name != "mono" else max(1, args.steps // 3)
         runs[name] = run_method(case, meth, nst, args.warmup if name != "mono" else min(1, args.warmup),
-                                widx if name != "mono" else None, tol, pol, args.stall_window, args.theta)
+                                widx if name != "mono" else None, tol, pol, args.stall_window, args.theta,
+                                args.async_below)
     head = runs.get("schur32") or next(iter(runs.values()))
     info = head["info"]
     # value: completed contingency-steps per second of the pass (a scenario
@@ -148,7 +156,8 @@ def main():
                                f"horizons U(1,5) s, Schur + FP32 factors",
                    "n_sys": case.desc.n, "n_red": info["n_red"], "contingency_steps": total_steps,
                    "newton_tol": tol, "itermax": 30, "stall_window": args.stall_window,
-                   "refactor_theta": args.theta,
+                   "refactor_theta": args.theta, "refactor_policy": args.policy,
+                   "async_below": args.async_below,
                    "horizon_cap": args.horizon_cap or None},
     }
     for k, r in runs.items():

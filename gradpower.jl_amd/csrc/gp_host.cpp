@@ -190,9 +190,15 @@ struct gp_ctx {
   DBuf<float> d_luS32, d_svS32, d_dnS32, d_X32;
   DBuf<double> d_X64;
   DBuf<uint8_t> d_work, d_act, d_refac, d_refac0, d_sel;
-  DBuf<int> d_fbus, d_nunconv, d_status, d_nstall;
-  DBuf<double> d_fg, d_fbest;
-  DBuf<uint8_t> d_failed;
+  DBuf<int> d_fbus, d_nunconv, d_status, d_nstall, d_iters;
+  DBuf<double> d_fg, d_fbest, d_svar_snap, d_traj_tmp;
+  DBuf<uint8_t> d_failed, d_upd, d_cur, d_tgt, d_pend, d_req, d_areq, d_sch;
+  // asynchronous refactorisation (GP_REFACTOR_ASYNC): one batch at a time on
+  // a second stream, into the buffer the solves do not read
+  hipStream_t fstream = nullptr;
+  hipEvent_t ev_snap = nullptr, ev_done = nullptr;
+  bool inflight = false;
+  std::vector<uint8_t> cur, pend, batch, queue;
   DBuf<unsigned long long> d_nb;
   DevTables T{};
   LUPlan luJ, luS;
@@ -550,8 +556,23 @@ void alloc_state(gp_ctx *C) {
   C->d_work.alloc(L); C->d_act.alloc(L); C->d_refac.alloc(L); C->d_refac0.alloc(L); C->d_sel.alloc(L);
   C->d_fbus.alloc((size_t)MAXFAULT * L);
   C->d_fg.alloc((size_t)MAXFAULT * L);
-  C->d_nunconv.alloc(2);
+  C->d_nunconv.alloc(4);
+  C->d_areq.alloc(L);
+  C->d_sch.alloc(L);
   C->d_nstall.alloc(L);
+  C->d_iters.alloc(L);
+  C->d_upd.alloc(L);
+  C->d_cur.alloc(L);
+  C->d_tgt.alloc(L);
+  C->d_pend.alloc(L);
+  C->d_req.alloc(L);
+  HIPCHK(hipMemset(C->d_cur.p, 0, L));
+  HIPCHK(hipMemset(C->d_pend.p, 0, L));
+  HIPCHK(hipMemset(C->d_upd.p, 0, L));
+  C->cur.assign(L, 0);
+  C->pend.assign(L, 0);
+  C->batch.assign(L, 0);
+  C->queue.assign(L, 0);
   C->d_fbest.alloc(L);
   C->d_failed.alloc(L);
   C->d_status.alloc(L);
@@ -604,6 +625,7 @@ void analyse(gp_ctx *C, int method, double dt) {
     if (C->luS.ready && C->luS.dt == dt) return;
     const int nnzS = C->scp[C->nred];
     C->d_svar.alloc((size_t)std::max(1, C->n_svar) * L);
+    C->d_svar_snap.alloc((size_t)std::max(1, C->n_svar) * L);
     C->d_rhs.alloc((size_t)C->nred * L);
     C->d_dv.alloc((size_t)C->nred * L);
     C->d_X64.alloc((size_t)std::max(1, C->xrows) * 3 * L);
@@ -636,19 +658,19 @@ void ensure_schur_buffers(gp_ctx *C, bool fp32) {
   if (fp32) {
     C->d_X32.ensure((size_t)std::max(1, C->xrows) * 3 * L);
     C->d_luS32.ensure((size_t)C->luS.A.nnz_lu * C->M);
-    if (C->d_svS32.n != (size_t)C->luS.dev.ns * C->M) {
-      C->d_svS32.alloc((size_t)C->luS.dev.ns * C->M);
+    if (C->d_svS32.n != (size_t)2 * C->luS.dev.ns * C->M) {   // two buffers per scenario
+      C->d_svS32.alloc((size_t)2 * C->luS.dev.ns * C->M);
       HIPCHK(hipMemset(C->d_svS32.p, 0, C->d_svS32.n * sizeof(float)));   // padding slots stay zero
     }
-    C->d_dnS32.ensure((size_t)std::max(1, C->luS.A.nd * C->luS.A.nd) * C->M);
+    C->d_dnS32.ensure((size_t)2 * std::max(1, C->luS.A.nd * C->luS.A.nd) * C->M);
     C->luS.shape(sizeof(float));
   } else {
     C->d_luS64.ensure((size_t)C->luS.A.nnz_lu * C->M);
-    if (C->d_svS64.n != (size_t)C->luS.dev.ns * C->M) {
-      C->d_svS64.alloc((size_t)C->luS.dev.ns * C->M);
+    if (C->d_svS64.n != (size_t)2 * C->luS.dev.ns * C->M) {   // two buffers per scenario
+      C->d_svS64.alloc((size_t)2 * C->luS.dev.ns * C->M);
       HIPCHK(hipMemset(C->d_svS64.p, 0, C->d_svS64.n * sizeof(double)));   // padding slots stay zero
     }
-    C->d_dnS64.ensure((size_t)std::max(1, C->luS.A.nd * C->luS.A.nd) * C->M);
+    C->d_dnS64.ensure((size_t)2 * std::max(1, C->luS.A.nd * C->luS.A.nd) * C->M);
     C->luS.shape(sizeof(double));
   }
 }
@@ -660,98 +682,210 @@ struct NewtonOut {
   std::vector<uint8_t> failed;   // scenarios that did not converge (empty: none)
 };
 
+// Asynchronous refactorisation (GP_REFACTOR_ASYNC).  A batch copies the
+// requested scenarios' reduced-network values (made by this round's Schur
+// kernels) aside, then factorises them on the second stream into the buffer
+// their solves do not read; when the batch has finished, those scenarios
+// switch buffers.  The copy is taken on the main stream and the factorisation
+// waits for it, so it also waits for every solve that was still reading the
+// buffer it overwrites.
+template <typename TF>
+void launch_batch(gp_ctx *C, TF *luv, TF *sv, TF *dn) {
+  const int L = C->ldM;
+  std::vector<uint8_t> tgt(L, 0);
+  for (int m = 0; m < L; m++) {
+    C->batch[m] = C->queue[m];
+    tgt[m] = (uint8_t)(1 - C->cur[m]);
+    if (C->batch[m]) C->pend[m] = 1;
+    C->queue[m] = 0;
+  }
+  HIPCHK(hipMemcpyAsync(C->d_req.p, C->batch.data(), L, hipMemcpyHostToDevice, C->stream));
+  HIPCHK(hipMemcpyAsync(C->d_tgt.p, tgt.data(), L, hipMemcpyHostToDevice, C->stream));
+  HIPCHK(hipMemcpyAsync(C->d_pend.p, C->pend.data(), L, hipMemcpyHostToDevice, C->stream));
+  launch_copy_active(std::max(1, C->n_svar), L, C->d_svar.p, C->d_svar_snap.p, C->d_req.p, C->stream);
+  HIPCHK(hipEventRecord(C->ev_snap, C->stream));
+  HIPCHK(hipStreamWaitEvent(C->fstream, C->ev_snap, 0));
+  launch_splu_factor_impl<TF>(C->luS.dev, C->d_svar_snap.p, C->d_s_cst.p, L, luv, sv, dn, C->d_req.p, C->d_tgt.p,
+                              C->M, C->d_status.p, C->fstream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(C->ev_done, C->fstream));
+  // the host copies of batch/tgt/pend must outlive the copies above
+  HIPCHK(hipStreamSynchronize(C->stream));
+  C->inflight = true;
+}
+
+// The batch in flight has finished (or `wait`): its scenarios switch buffers.
+bool land_batch(gp_ctx *C, bool wait) {
+  if (!C->inflight) return false;
+  if (wait) {
+    HIPCHK(hipEventSynchronize(C->ev_done));
+  } else {
+    const hipError_t q = hipEventQuery(C->ev_done);
+    if (q == hipErrorNotReady) return false;
+    HIPCHK(q);
+  }
+  const int L = C->ldM;
+  for (int m = 0; m < L; m++)
+    if (C->batch[m]) {
+      C->cur[m] = (uint8_t)(1 - C->cur[m]);
+      C->pend[m] = 0;
+      C->batch[m] = 0;
+    }
+  HIPCHK(hipMemcpyAsync(C->d_cur.p, C->cur.data(), L, hipMemcpyHostToDevice, C->stream));
+  HIPCHK(hipMemcpyAsync(C->d_pend.p, C->pend.data(), L, hipMemcpyHostToDevice, C->stream));
+  HIPCHK(hipStreamSynchronize(C->stream));
+  C->inflight = false;
+  return true;
+}
+
 // newton_step! (src/numerics.jl:55-176) for every scenario in `mask`, in lock
-// step: each iteration evaluates F, drops the scenarios with |F|inf < tol,
-// and applies one correction to the rest.
+// step: each round evaluates F, drops the scenarios with |F|inf < tol (and
+// those that failed), and applies one correction to the rest.
 NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, const std::vector<uint8_t> &mask,
                  const std::vector<uint8_t> *refac0 = nullptr) {
   NewtonOut out;
   const int L = C->ldM;
   set_mask(C, C->d_work, mask);
-  const bool reuse = refac0 && method != GP_METHOD_MONO_FP64 && C->opt.refactor_policy == GP_REFACTOR_ADAPTIVE;
+  const int pol = C->opt.refactor_policy;
+  const bool schur = method != GP_METHOD_MONO_FP64;
+  const bool reuse = refac0 && schur && (pol == GP_REFACTOR_ADAPTIVE || pol == GP_REFACTOR_ASYNC);
+  const bool async = schur && pol == GP_REFACTOR_ASYNC;
   if (reuse) set_mask(C, C->d_refac0, *refac0);
   const int stall = C->opt.stall_window;
-  if (stall > 0) HIPCHK(hipMemsetAsync(C->d_failed.p, 0, L, C->stream));
-  // scenarios stopped by the stall rule, plus those still working when
-  // itermax ran out
-  auto collect = [&](bool exhausted) {
-    if (!exhausted && stall <= 0) return;
-    std::vector<uint8_t> w(L, 0), fl(L, 0);
-    if (exhausted) HIPCHK(hipMemcpyAsync(w.data(), C->d_work.p, L, hipMemcpyDeviceToHost, C->stream));
-    if (stall > 0) HIPCHK(hipMemcpyAsync(fl.data(), C->d_failed.p, L, hipMemcpyDeviceToHost, C->stream));
-    HIPCHK(hipStreamSynchronize(C->stream));
-    bool any = false;
-    for (int m = 0; m < L; m++) {
-      w[m] = (w[m] || fl[m]) ? 1 : 0;
-      any = any || w[m];
-    }
-    if (any) out.failed = w;
-  };
+  HIPCHK(hipMemsetAsync(C->d_failed.p, 0, L, C->stream));
+  // refactor policy: theta <= 0 refactors every iteration; otherwise after
+  // the first iteration only when |F| fell by less than theta
+  const double theta = (!schur || pol == GP_REFACTOR_EVERY_ITER)
+                           ? 0.0
+                           : ((pol == GP_REFACTOR_ADAPTIVE || pol == GP_REFACTOR_ASYNC)
+                                  ? (C->opt.refactor_theta > 0.0 ? C->opt.refactor_theta : 10.0)
+                                  : 1e300);
+  const bool fp32 = method == GP_METHOD_SCHUR_FP32;
+  const double async_below = C->opt.async_below > 0.0 ? C->opt.async_below : 1e300;
   const double *p = C->d_p.p;
   double *z = C->d_z.p;
-  int nun = 0;
-  for (int it = 0; it < itermax; it++) {
+  std::vector<uint8_t> req(L, 0);
+  // a scenario may sit out rounds while its factorisation is in flight, so
+  // rounds are bounded separately from each scenario's itermax corrections
+  const int max_rounds = 4 * itermax + 16;
+  bool done = false;
+  for (int it = 0; it < max_rounds; it++) {
     launch_cluster_residual(C->T, p, z, C->d_zold.p, dt, C->d_f.p, C->d_inj.p, C->d_work.p, C->stream);
     launch_bus_residual(C->T, p, z, C->d_inj.p, faults(C), C->d_f.p, C->d_work.p, C->stream);
-    HIPCHK(hipMemsetAsync(C->d_nunconv.p, 0, 2 * sizeof(int), C->stream));
-    // refactor policy: theta <= 0 refactors every iteration; otherwise after
-    // the first iteration only when |F| fell by less than 10x
-    const double theta = (method == GP_METHOD_MONO_FP64 || C->opt.refactor_policy == GP_REFACTOR_EVERY_ITER)
-                             ? 0.0
-                             : (C->opt.refactor_policy == GP_REFACTOR_ADAPTIVE
-                                    ? (C->opt.refactor_theta > 0.0 ? C->opt.refactor_theta : 10.0)
-                                    : 1e300);
+    HIPCHK(hipMemsetAsync(C->d_nunconv.p, 0, 4 * sizeof(int), C->stream));
     launch_rownorm_impl(C->n, C->M, L, C->d_f.p, C->d_work.p, tol, C->d_fnorm.p, C->d_nb.p, C->d_nunconv.p,
-                        C->d_refac.p, reuse ? C->d_refac0.p : nullptr, it == 0, theta, stall, C->d_fbest.p,
-                        C->d_nstall.p, C->d_failed.p, C->stream);
-    int cnt[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(cnt, C->d_nunconv.p, 2 * sizeof(int), hipMemcpyDeviceToHost, C->stream));
+                        C->d_refac.p, reuse ? C->d_refac0.p : nullptr, it == 0, theta, stall, itermax, C->d_fbest.p,
+                        C->d_nstall.p, C->d_iters.p, C->d_failed.p, async ? C->d_pend.p : nullptr, C->d_upd.p,
+                        async ? async_below : 0.0, C->d_areq.p, C->d_sch.p, C->stream);
+    int cnt[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(cnt, C->d_nunconv.p, 4 * sizeof(int), hipMemcpyDeviceToHost, C->stream));
+    if (async && it > 0) HIPCHK(hipMemcpyAsync(req.data(), C->d_areq.p, L, hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
-    nun = cnt[0];
+    const int nun = cnt[0];
     if (nun == 0) {
-      collect(false);
-      return out;
+      done = true;
+      break;
     }
-    out.corrections += nun;
-    out.refac += cnt[1];
+    out.corrections += cnt[2];
     out.rounds++;
-    if (cnt[1] > 0) out.factor_launches++;
-    if (method == GP_METHOD_MONO_FP64) {
-      launch_jac_mono(C->T, p, z, dt, faults(C), C->d_jv.p, C->d_work.p, C->stream);
-      launch_splu_factor_impl<double>(C->luJ.dev, C->d_jv.p, nullptr, L, C->d_luJ.p, C->d_svJ.p, C->d_dnJ.p,
-                                      C->d_refac.p, C->M, C->d_status.p, C->stream);
-      launch_splu_solve<double>(C->luJ.dev, C->d_svJ.p, C->d_dnJ.p, C->d_f.p, L, -1.0, nullptr, z, nullptr, C->d_work.p, C->M,
-                                C->stream);
-    } else {
-      const bool fp32 = method == GP_METHOD_SCHUR_FP32;
-      if (fp32) {
-        launch_cluster_schur<float>(C->T, p, z, C->d_f.p, dt, C->d_X32.p, C->d_Dg.p, C->d_work.p, C->stream);
-      } else {
-        launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, C->d_work.p, C->stream);
-      }
-      launch_bus_schur(C->T, p, z, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_work.p, C->stream);
-      if (cnt[1] == 0) {
-        // every scenario keeps its factorisation
-      } else if (fp32)
-        launch_splu_factor_impl<float>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS32.p, C->d_svS32.p, C->d_dnS32.p, C->d_refac.p, C->M,
-                                       C->d_status.p, C->stream);
+    auto queued = [&]() {
+      for (int m = 0; m < L; m++)
+        if (C->queue[m]) return true;
+      return false;
+    };
+    auto launch_queue = [&]() {
+      int nb = 0;
+      for (int m = 0; m < L; m++) nb += C->queue[m];
+      out.refac += nb;
+      out.factor_launches++;
+      if (fp32)
+        launch_batch<float>(C, C->d_luS32.p, C->d_svS32.p, C->d_dnS32.p);
       else
-        launch_splu_factor_impl<double>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p, C->d_dnS64.p, C->d_refac.p, C->M,
-                                        C->d_status.p, C->stream);
-      if (fp32) {
-        launch_splu_solve<float>(C->luS.dev, C->d_svS32.p, C->d_dnS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z, C->d_dv.p,
-                                 C->d_work.p, C->M, C->stream);
-        launch_cluster_backsub<float>(C->T, C->d_X32.p, C->d_f.p, C->d_dv.p, z, C->d_work.p, C->stream);
-      } else {
-        launch_splu_solve<double>(C->luS.dev, C->d_svS64.p, C->d_dnS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z, C->d_dv.p,
-                                  C->d_work.p, C->M, C->stream);
-        launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, C->d_work.p, C->stream);
+        launch_batch<double>(C, C->d_luS64.p, C->d_svS64.p, C->d_dnS64.p);
+    };
+    const bool landed = async && land_batch(C, false);
+    if (cnt[2] == 0 && cnt[3] == 0) {
+      // every remaining scenario waits for its factorisation
+      if (async && !C->inflight && queued()) launch_queue();
+      if (C->inflight)
+        land_batch(C, true);
+      else if (!landed)
+        break;   // nothing will change (cannot happen)
+      continue;
+    }
+    if (method == GP_METHOD_MONO_FP64) {
+      out.refac += cnt[1];
+      if (cnt[1] > 0) out.factor_launches++;
+      launch_jac_mono(C->T, p, z, dt, faults(C), C->d_jv.p, C->d_upd.p, C->stream);
+      launch_splu_factor_impl<double>(C->luJ.dev, C->d_jv.p, nullptr, L, C->d_luJ.p, C->d_svJ.p, C->d_dnJ.p,
+                                      C->d_refac.p, nullptr, C->M, C->d_status.p, C->stream);
+      launch_splu_solve<double>(C->luJ.dev, C->d_svJ.p, C->d_dnJ.p, C->d_f.p, L, -1.0, nullptr, z, nullptr,
+                                C->d_upd.p, nullptr, C->M, C->stream);
+      HIPCHK(hipGetLastError());
+      continue;
+    }
+    if (fp32)
+      launch_cluster_schur<float>(C->T, p, z, C->d_f.p, dt, C->d_X32.p, C->d_Dg.p, C->d_sch.p, C->stream);
+    else
+      launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, C->d_sch.p, C->stream);
+    launch_bus_schur(C->T, p, z, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_sch.p, C->stream);
+    const bool sync_fac = cnt[1] > 0;
+    if (sync_fac) {
+      // synchronous factorisation into the buffer the solves read.  Later
+      // iterations ask only for scenarios with nothing pending; the first one
+      // (a solve after an event or a change of dt) may include scenarios in
+      // the batch in flight, which lands first, and their queued requests
+      // (made for the old system) are dropped
+      if (async && it == 0) {
+        land_batch(C, true);
+        if (refac0) {
+          for (int m = 0; m < L; m++)
+            if ((*refac0)[m]) C->queue[m] = C->pend[m] = 0;
+          HIPCHK(hipMemcpyAsync(C->d_pend.p, C->pend.data(), L, hipMemcpyHostToDevice, C->stream));
+          HIPCHK(hipStreamSynchronize(C->stream));   // the host copy stays valid until then
+        }
       }
+      out.refac += cnt[1];
+      out.factor_launches++;
+      if (fp32)
+        launch_splu_factor_impl<float>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS32.p, C->d_svS32.p,
+                                       C->d_dnS32.p, C->d_refac.p, C->d_cur.p, C->M, C->d_status.p, C->stream);
+      else
+        launch_splu_factor_impl<double>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
+                                        C->d_dnS64.p, C->d_refac.p, C->d_cur.p, C->M, C->d_status.p, C->stream);
+    }
+    if (async && cnt[3] > 0) {
+      // asynchronous requests: queued (and pending: the scenario waits when
+      // its |F| stops decreasing), launched when no batch is in flight
+      for (int m = 0; m < L; m++)
+        if (req[m]) C->queue[m] = C->pend[m] = 1;
+      HIPCHK(hipMemcpyAsync(C->d_pend.p, C->pend.data(), L, hipMemcpyHostToDevice, C->stream));
+      HIPCHK(hipStreamSynchronize(C->stream));   // the host copy stays valid until then
+    }
+    if (async && !C->inflight && queued()) launch_queue();
+    if (fp32) {
+      launch_splu_solve<float>(C->luS.dev, C->d_svS32.p, C->d_dnS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
+                               C->d_dv.p, C->d_upd.p, C->d_cur.p, C->M, C->stream);
+      launch_cluster_backsub<float>(C->T, C->d_X32.p, C->d_f.p, C->d_dv.p, z, C->d_upd.p, C->stream);
+    } else {
+      launch_splu_solve<double>(C->luS.dev, C->d_svS64.p, C->d_dnS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
+                                C->d_dv.p, C->d_upd.p, C->d_cur.p, C->M, C->stream);
+      launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, C->d_upd.p, C->stream);
     }
     HIPCHK(hipGetLastError());
   }
-  // itermax exhausted: the scenarios still in `work` did not converge
-  collect(true);
+  // scenarios stopped by the itermax / stall rules, plus (round cap) any
+  // still working
+  std::vector<uint8_t> w(L, 0), fl(L, 0);
+  if (!done) HIPCHK(hipMemcpyAsync(w.data(), C->d_work.p, L, hipMemcpyDeviceToHost, C->stream));
+  HIPCHK(hipMemcpyAsync(fl.data(), C->d_failed.p, L, hipMemcpyDeviceToHost, C->stream));
+  HIPCHK(hipStreamSynchronize(C->stream));
+  bool any = false;
+  for (int m = 0; m < L; m++) {
+    w[m] = (w[m] || fl[m]) ? 1 : 0;
+    any = any || w[m];
+  }
+  if (any) out.failed = w;
   return out;
 }
 
@@ -771,6 +905,9 @@ int gp_create(const gp_system_desc *d, int32_t M, const gp_options *opt, gp_ctx 
   try {
     HIPCHK(hipGetDevice(&C->device));
     HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&C->fstream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&C->ev_snap, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&C->ev_done, hipEventDisableTiming));
     C->n = d->n; C->nbus = d->nbus; C->net_ptr = d->net_ptr; C->npar = d->n_par; C->M = M;
     C->ldM = (M + WAVE - 1) / WAVE * WAVE;
     if (opt) C->opt = *opt;
@@ -828,9 +965,14 @@ int gp_create(const gp_system_desc *d, int32_t M, const gp_options *opt, gp_ctx 
 
 void gp_destroy(gp_ctx *C) {
   if (!C) return;
+  if (C->fstream) (void)hipStreamSynchronize(C->fstream);
   if (C->stream) (void)hipStreamSynchronize(C->stream);
-  hipStream_t s = C->stream;
+  hipStream_t s = C->stream, fs = C->fstream;
+  hipEvent_t e1 = C->ev_snap, e2 = C->ev_done;
   delete C;
+  if (e1) (void)hipEventDestroy(e1);
+  if (e2) (void)hipEventDestroy(e2);
+  if (fs) (void)hipStreamDestroy(fs);
   if (s) (void)hipStreamDestroy(s);
 }
 
@@ -1016,6 +1158,7 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
           }
       }
     }
+    land_batch(C, true);   // no factorisation outlives the call
     HIPCHK(hipEventRecord(e1, C->stream));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.0f;
@@ -1041,6 +1184,11 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
       return fail(GP_ERR_NEWTON_NOT_CONVERGED,
                   std::to_string(S.n_not_converged) + " scenario(s) had a Newton solve that did not converge");
   } catch (const GpError &e) {
+    try {
+      land_batch(C, true);
+    } catch (const GpError &) {
+      C->inflight = false;
+    }
     if (st) *st = S;
     return fail(e.code, e.msg);
   }
@@ -1054,12 +1202,15 @@ int gp_get_trajectory(gp_ctx *C, int32_t m, double *out, int64_t ld) {
   if (ld < len) return fail(GP_ERR_BADARG, "gp_get_trajectory: ld < nsteps[m]+1");
   try {
     const int L = C->ldM;
-    std::vector<double> tmp((size_t)len * C->nrec);
-    // hist[(k*nrec + j)*L + m] -> tmp[k*nrec + j]
-    HIPCHK(hipMemcpy2D(tmp.data(), sizeof(double), C->d_hist.p + m, (size_t)L * sizeof(double), sizeof(double),
-                       (size_t)len * C->nrec, hipMemcpyDeviceToHost));
+    const size_t tot = (size_t)len * C->nrec;
+    C->d_traj_tmp.ensure(std::max<size_t>(1, tot));
+    launch_gather_traj(C->nrec, len, L, m, C->d_hist.p, C->d_traj_tmp.p, C->stream);
+    HIPCHK(hipGetLastError());
+    std::vector<double> tmp(tot);
+    HIPCHK(hipMemcpyAsync(tmp.data(), C->d_traj_tmp.p, tot * sizeof(double), hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
     for (int j = 0; j < C->nrec; j++)
-      for (int k = 0; k < len; k++) out[(size_t)j * ld + k] = tmp[(size_t)k * C->nrec + j];
+      for (int k = 0; k < len; k++) out[(size_t)j * ld + k] = tmp[(size_t)j * len + k];
   } catch (const GpError &e) {
     return fail(e.code, e.msg);
   }

@@ -83,7 +83,9 @@ void launch_bus_residual(const DevTables &T, const double *p, const double *z, c
                          FaultState fs, double *f, const uint8_t *work, hipStream_t s);
 void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, double tol, double *fnorm,
                          unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
-                         double theta, int stall, double *fbest, int *nstall, uint8_t *failed, hipStream_t s);
+                         double theta, int stall, int itermax, double *fbest, int *nstall, int *iters,
+                         uint8_t *failed, const uint8_t *pend, uint8_t *upd, double async_below, uint8_t *areq,
+                         uint8_t *sch, hipStream_t s);
 void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs,
                      double *jv, const uint8_t *work, hipStream_t s);
 template <typename TF>
@@ -95,16 +97,19 @@ void launch_bus_schur(const DevTables &T, const double *p, const double *z, cons
 template <typename TF>
 void launch_cluster_backsub(const DevTables &T, const TF *X, const double *f, const double *dv,
                             double *z, const uint8_t *work, hipStream_t s);
-// luv: [M][nnz_lu] factor workspace; sv: [M][ns] solve values; dn: [M][nd*nd]
-// dense trailing block (column-major LU, unit lower part implied)
+// luv: [M][nnz_lu] factor workspace; sv: [2][M][ns] solve values; dn:
+// [2][M][nd*nd] dense trailing block (column-major LU, unit lower part
+// implied).  The factorisation writes buffer tgt[m], the solve reads cur[m]
+// (NULL: buffer 0).
 template <typename TF>
 void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv, TF *sv,
-                             TF *dn, const uint8_t *refac, int M, int *status, hipStream_t s);
+                             TF *dn, const uint8_t *refac, const uint8_t *tgt, int M, int *status, hipStream_t s);
 template <typename TF>
 void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const double *rhs, int ldM, double sign,
-                       const int *scatter, double *zupd, double *xout, const uint8_t *work, int M,
+                       const int *scatter, double *zupd, double *xout, const uint8_t *work, const uint8_t *cur, int M,
                        hipStream_t s);
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s);
+void launch_gather_traj(int nrec, int len, int ldM, int m, const double *hist, double *out, hipStream_t s);
 void launch_record_impl(int nrec, const int *idx, int ldM, int M, const double *z, double *hist,
                         const uint8_t *act, hipStream_t s);
 

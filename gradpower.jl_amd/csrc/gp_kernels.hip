@@ -398,53 +398,90 @@ __global__ void __launch_bounds__(256) k_rownorm(int n, int ldM, int M, const do
   }
 }
 
-// Finishes the max-norm: converged scenarios leave `work`; for the others,
-// refac[m] is set when this is the first iteration (first != 0), when every
-// iteration refactors (theta <= 0), or when |F| fell by less than 1/theta
-// since the previous iteration.  With stall > 0 a scenario whose |F| is not
-// finite, exceeds 1e10, or has not halved within `stall` iterations leaves
-// `work` with failed[m] = 1 (fbest/nstall carry its history; first resets it).
+// Finishes the max-norm and decides, per working scenario, what this round
+// does.  Converged scenarios (|F| < tol) leave `work`.  A scenario that has
+// used its itermax corrections, or (stall > 0) whose |F| is not finite,
+// exceeds 1e10 or has not halved within `stall` of its corrections, leaves
+// `work` with failed[m] = 1.  refac[m] asks for a numeric factorisation: at
+// the first iteration refac0[m] (or 1 without refac0); later when every
+// iteration refactors (theta <= 0) or |F| fell by less than theta since the
+// previous iteration, except while one is pending (pend[m]).  With
+// async_below > 0 a later request from a scenario whose |F| is below it is
+// asynchronous (areq[m] = 1, refac[m] = 0).  upd[m] = 1 when the scenario
+// applies a correction this round: a scenario whose factorisation is pending,
+// or that asks for an asynchronous one, waits for it instead, so that its
+// iterates are those of the synchronous policy.  sch[m] = 1 where the Schur
+// kernels run (corrections and new asynchronous requests).  iters[m] counts
+// corrections; first resets the per-solve history.  n_unconv:
+// [0] working scenarios, [1] synchronous requests, [2] corrections, [3]
+// asynchronous requests.
 __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict__ nb, double *__restrict__ fnorm,
                               uint8_t *__restrict__ work, int *__restrict__ n_unconv, uint8_t *__restrict__ refac,
-                              const uint8_t *__restrict__ refac0, int first, double theta, int stall,
-                              double *__restrict__ fbest, int *__restrict__ nstall, uint8_t *__restrict__ failed) {
+                              const uint8_t *__restrict__ refac0, int first, double theta, int stall, int itermax,
+                              double *__restrict__ fbest, int *__restrict__ nstall, int *__restrict__ iters,
+                              uint8_t *__restrict__ failed, const uint8_t *__restrict__ pend,
+                              uint8_t *__restrict__ upd, double async_below, uint8_t *__restrict__ areq,
+                              uint8_t *__restrict__ sch) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
-  if (!work[m]) {
-    refac[m] = 0;
-    return;
-  }
+  const bool was_upd = first ? true : upd[m] != 0;
+  upd[m] = 0;
+  refac[m] = 0;
+  areq[m] = 0;
+  sch[m] = 0;
+  if (!work[m]) return;
   const double v = __longlong_as_double(nb[m]);
   const double prev = fnorm[m];
   fnorm[m] = v;
   nb[m] = 0ull;
-  if (v < tol) {                 // converged (NaN compares false: stays working and is caught by itermax)
+  if (v < tol) {                 // converged (NaN compares false)
     work[m] = 0;
-    refac[m] = 0;
     return;
   }
+  const int it = first ? 0 : iters[m];
+  bool fail = it >= itermax;
   if (stall > 0) {
     double best = first ? 1e300 : fbest[m];
     int ns = first ? 0 : nstall[m];
     if (v < 0.5 * best) {
       best = v;
       ns = 0;
-    } else {
+    } else if (was_upd) {
       ns++;
     }
     fbest[m] = best;
     nstall[m] = ns;
-    if (!(v <= 1e10) || ns >= stall) {
-      work[m] = 0;
-      refac[m] = 0;
-      failed[m] = 1;
-      return;
-    }
+    fail = fail || !(v <= 1e10) || ns >= stall;
+  }
+  if (fail) {
+    work[m] = 0;
+    failed[m] = 1;
+    return;
   }
   atomicAdd(n_unconv, 1);
-  const int rf = first ? (refac0 ? refac0[m] : 1) : ((theta <= 0.0 || !(v * theta < prev)) ? 1 : 0);
-  refac[m] = rf;
-  if (rf) atomicAdd(n_unconv + 1, 1);
+  const bool pending = pend && pend[m];
+  // a scenario that waited last round compares against an unchanged |F|:
+  // its new factorisation has just arrived, so it asks for none
+  const int rf = first ? (refac0 ? refac0[m] : 1)
+                       : ((pending || !was_upd) ? 0 : ((theta <= 0.0 || !(v * theta < prev)) ? 1 : 0));
+  bool wait = pending;
+  if (rf && !first && async_below > 0.0 && v < async_below) {
+    areq[m] = 1;
+    atomicAdd(n_unconv + 3, 1);
+    wait = true;
+  } else if (rf) {
+    refac[m] = 1;
+    atomicAdd(n_unconv + 1, 1);
+  }
+  const bool u = !wait;
+  sch[m] = (u || areq[m]) ? 1 : 0;   // the Schur kernels also run for a new request's values
+  if (u) {
+    upd[m] = 1;
+    iters[m] = it + 1;
+    atomicAdd(n_unconv + 2, 1);
+  } else {
+    iters[m] = it;
+  }
 }
 
 // ---------------------------------------------------------------- monolithic Jacobian
@@ -1413,16 +1450,21 @@ __device__ __forceinline__ int wave_column(const SpluDev &S, TF *__restrict__ lu
   return (!(piv != TF(0)) || !isfinite((double)piv)) ? 1 : 0;
 }
 
+// svv/dnv hold two buffers per scenario ([2][M][...]); tgt[m] selects the
+// one written (NULL: buffer 0).
 template <typename TF>
 __global__ void __launch_bounds__(512) k_splu_factor(SpluDev S, const double *__restrict__ var,
                                                      const double *__restrict__ cst, int ldM, TF *__restrict__ luv,
                                                      TF *__restrict__ svv, TF *__restrict__ dnv,
-                                                     const uint8_t *__restrict__ refac, int *__restrict__ status) {
+                                                     const uint8_t *__restrict__ refac, const uint8_t *__restrict__ tgt,
+                                                     int *__restrict__ status) {
   const int m = scenario_of_block(blockIdx.x, gridDim.x);
   if (!refac[m]) return;
   extern __shared__ __align__(16) char smem_raw[];
+  const int Mb = gridDim.x;
+  const int bsel = tgt ? tgt[m] : 0;
   TF *lu = luv + (size_t)m * S.nnz_lu;
-  TF *so = svv + (size_t)m * S.ns;
+  TF *so = svv + ((size_t)bsel * Mb + m) * S.ns;
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
   for (int p = tid; p < S.nnz_lu; p += nt) lu[p] = TF(0);
   __syncthreads();
@@ -1522,7 +1564,7 @@ __global__ void __launch_bounds__(512) k_splu_factor(SpluDev S, const double *__
       }
     }
     __syncthreads();
-    TF *dn = dnv + (size_t)m * nd * nd;
+    TF *dn = dnv + ((size_t)bsel * Mb + m) * nd * nd;
     for (int i = tid; i < nd * nd; i += nt) dn[i] = D[i];
     GP_TRACE(0, S.n_flev + 1);
   }
@@ -1653,11 +1695,14 @@ template <typename TF>
 __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restrict__ svv, const TF *__restrict__ dnv,
                                                     const double *__restrict__ rhs, int ldM, double sign,
                                                     const int *__restrict__ scatter, double *__restrict__ zupd,
-                                                    double *__restrict__ xout, const uint8_t *__restrict__ work) {
+                                                    double *__restrict__ xout, const uint8_t *__restrict__ work,
+                                                    const uint8_t *__restrict__ cur) {
   const int m = scenario_of_block(blockIdx.x, gridDim.x);
   if (!work[m]) return;
   extern __shared__ __align__(16) char smem_raw[];
   const int n = S.n, nd = S.nd, B = n - nd;
+  const int Mb = gridDim.x;
+  const int bsel = cur ? cur[m] : 0;
   double *x = reinterpret_cast<double *>(smem_raw);
   // LDS: x, then the level tables (16-byte aligned for their vector reads),
   // then the dense block
@@ -1666,7 +1711,7 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   int *blt = fct + ((2 * S.n_fwch + 3) & ~3);
   int *bct = blt + 4 * S.n_bwlev;
   TF *Dl = reinterpret_cast<TF *>(reinterpret_cast<char *>(bct) + (((size_t)8 * S.n_bwch + 15) & ~(size_t)15));
-  const TF *fv = svv + (size_t)m * S.ns;
+  const TF *fv = svv + ((size_t)bsel * Mb + m) * S.ns;
   const TF *bv = fv + S.fw_nvals;
   const TF *bd = bv + S.bw_nvals;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -1676,7 +1721,7 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   for (int i = tid; i < 2 * S.n_bwch; i += nt) bct[i] = S.bw_chunk[i];
   for (int r = tid; r < n; r += nt) x[S.pinv[r]] = rhs[(size_t)r * ldM + m];
   if (nd) {
-    const TF *dn = dnv + (size_t)m * nd * nd;
+    const TF *dn = dnv + ((size_t)bsel * Mb + m) * nd * nd;
     for (int i = tid; i < nd * nd; i += nt) Dl[i] = dn[i];
   }
   __syncthreads();
@@ -1809,6 +1854,15 @@ __global__ void k_record(int nrec, const int *__restrict__ idx, int ldM, int M, 
   }
 }
 
+// One scenario's recorded history, transposed: out[j*len + k] = hist[(k*nrec + j)*ldM + m].
+__global__ void k_gather_traj(int nrec, int len, int ldM, int m, const double *__restrict__ hist,
+                              double *__restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nrec * len) return;
+  const int j = (int)(i / len), k = (int)(i % len);
+  out[i] = hist[((size_t)k * nrec + j) * ldM + m];
+}
+
 // ---------------------------------------------------------------- launchers
 static inline dim3 lane_grid(int items, int ldM) { return dim3(items, ldM / WAVE); }
 
@@ -1823,11 +1877,14 @@ void launch_bus_residual(const DevTables &T, const double *p, const double *z, c
 }
 void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, double tol, double *fnorm,
                          unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
-                         double theta, int stall, double *fbest, int *nstall, uint8_t *failed, hipStream_t s) {
+                         double theta, int stall, int itermax, double *fbest, int *nstall, int *iters,
+                         uint8_t *failed, const uint8_t *pend, uint8_t *upd, double async_below, uint8_t *areq,
+                         uint8_t *sch, hipStream_t s) {
   const int chunks = std::max(1, std::min((n + 15) / 16, 1024 / (ldM / WAVE)));
   hipLaunchKernelGGL(k_rownorm, dim3(ldM / WAVE, chunks), dim3(256), 0, s, n, ldM, M, f, work, nb);
   hipLaunchKernelGGL(k_norm_finish, dim3((M + 255) / 256), dim3(256), 0, s, M, tol, nb, fnorm, work, n_unconv, refac,
-                     refac0, first, theta, stall, fbest, nstall, failed);
+                     refac0, first, theta, stall, itermax, fbest, nstall, iters, failed, pend, upd, async_below,
+                     areq, sch);
 }
 void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs, double *jv,
                      const uint8_t *work, hipStream_t s) {
@@ -1854,21 +1911,27 @@ void launch_cluster_backsub(const DevTables &T, const TF *X, const double *f, co
 }
 template <typename TF>
 void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv, TF *sv, TF *dn,
-                             const uint8_t *refac, int M, int *status, hipStream_t s) {
+                             const uint8_t *refac, const uint8_t *tgt, int M, int *status, hipStream_t s) {
   if (S.lds_factor > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_splu_factor<TF>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, S.lds_factor);
   hipLaunchKernelGGL(k_splu_factor<TF>, dim3(M), dim3(S.threads), S.lds_factor, s, S, var, cst, ldM, luv, sv, dn,
-                     refac, status);
+                     refac, tgt, status);
 }
 template <typename TF>
 void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const double *rhs, int ldM, double sign,
-                       const int *scatter, double *zupd, double *xout, const uint8_t *work, int M, hipStream_t s) {
+                       const int *scatter, double *zupd, double *xout, const uint8_t *work, const uint8_t *cur, int M,
+                       hipStream_t s) {
   if (S.lds_solve > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_splu_solve<TF>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, S.lds_solve);
   hipLaunchKernelGGL(k_splu_solve<TF>, dim3(M), dim3(S.threads), S.lds_solve, s, S, sv, dn, rhs, ldM, sign, scatter,
-                     zupd, xout, work);
+                     zupd, xout, work, cur);
+}
+void launch_gather_traj(int nrec, int len, int ldM, int m, const double *hist, double *out, hipStream_t s) {
+  const size_t tot = (size_t)nrec * len;
+  if (tot == 0) return;
+  hipLaunchKernelGGL(k_gather_traj, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nrec, len, ldM, m, hist, out);
 }
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s) {
   const int gx = n < 1024 ? n : 1024;
@@ -1889,12 +1952,15 @@ template void launch_cluster_backsub<double>(const DevTables &, const double *, 
 template void launch_cluster_backsub<float>(const DevTables &, const float *, const double *, const double *,
                                             double *, const uint8_t *, hipStream_t);
 template void launch_splu_factor_impl<double>(const SpluDev &, const double *, const double *, int, double *,
-                                              double *, double *, const uint8_t *, int, int *, hipStream_t);
+                                              double *, double *, const uint8_t *, const uint8_t *, int, int *,
+                                              hipStream_t);
 template void launch_splu_factor_impl<float>(const SpluDev &, const double *, const double *, int, float *, float *,
-                                             float *, const uint8_t *, int, int *, hipStream_t);
+                                             float *, const uint8_t *, const uint8_t *, int, int *, hipStream_t);
 template void launch_splu_solve<double>(const SpluDev &, const double *, const double *, const double *, int, double,
-                                        const int *, double *, double *, const uint8_t *, int, hipStream_t);
+                                        const int *, double *, double *, const uint8_t *, const uint8_t *, int,
+                                        hipStream_t);
 template void launch_splu_solve<float>(const SpluDev &, const float *, const float *, const double *, int, double,
-                                       const int *, double *, double *, const uint8_t *, int, hipStream_t);
+                                       const int *, double *, double *, const uint8_t *, const uint8_t *, int,
+                                       hipStream_t);
 
 }  // namespace gp

@@ -20,13 +20,13 @@ import numpy as np
 
 from .desc import build_desc
 from .lib import (HipContext, METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32,
-                  REFACTOR_ADAPTIVE, REFACTOR_EVERY_ITER)
+                  REFACTOR_ASYNC, REFACTOR_EVERY_ITER)
 
 
 class HipBatchedLayout:
     """Device layout of M scenarios of one system (all start from dp.zvec)."""
 
-    def __init__(self, dp, ps, M, record_idx=None, refactor_policy=REFACTOR_ADAPTIVE):
+    def __init__(self, dp, ps, M, record_idx=None, refactor_policy=REFACTOR_ASYNC):
         self.desc = build_desc(ps, dp)
         self.M = int(M)
         self.ctx = HipContext(self.desc, self.M, record_idx=record_idx,

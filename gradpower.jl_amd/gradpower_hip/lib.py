@@ -20,7 +20,7 @@ GP_OK = 0
 GP_ERR_BADARG, GP_ERR_HIP, GP_ERR_OOM, GP_ERR_SINGULAR_PIVOT = -1, -2, -3, -4
 GP_ERR_NEWTON_NOT_CONVERGED, GP_ERR_NAN, GP_ERR_UNSUPPORTED = -5, -6, -7
 METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32 = 0, 1, 2
-REFACTOR_EVERY_ITER, REFACTOR_FIRST_ITER, REFACTOR_ADAPTIVE = 0, 1, 2
+REFACTOR_EVERY_ITER, REFACTOR_FIRST_ITER, REFACTOR_ADAPTIVE, REFACTOR_ASYNC = 0, 1, 2, 3
 _NAMES = {-1: "BADARG", -2: "HIP", -3: "OOM", -4: "SINGULAR_PIVOT", -5: "NEWTON_NOT_CONVERGED",
           -6: "NAN", -7: "UNSUPPORTED"}
 
@@ -35,7 +35,8 @@ class GpError(RuntimeError):
 class COptions(C.Structure):
     _fields_ = [("record_all", C.c_int32), ("n_record", C.c_int32),
                 ("record_idx", C.POINTER(C.c_int32)), ("refactor_policy", C.c_int32),
-                ("stall_window", C.c_int32), ("reserved", C.c_int32 * 5), ("refactor_theta", C.c_double)]
+                ("stall_window", C.c_int32), ("reserved", C.c_int32 * 5), ("refactor_theta", C.c_double),
+                ("async_below", C.c_double)]
 
 
 class CStats(C.Structure):
@@ -109,7 +110,7 @@ class HipContext:
     """One library context: a system, M scenarios, device state on the current GPU."""
 
     def __init__(self, desc: SystemDesc, M: int, record_idx=None,
-                 refactor_policy=REFACTOR_EVERY_ITER, stall_window=0, refactor_theta=0.0):
+                 refactor_policy=REFACTOR_EVERY_ITER, stall_window=0, refactor_theta=0.0, async_below=0.0):
         self.lib = load()
         self.desc = desc
         self.M = int(M)
@@ -128,6 +129,7 @@ class HipContext:
         opt.refactor_policy = refactor_policy
         opt.stall_window = int(stall_window)
         opt.refactor_theta = float(refactor_theta)
+        opt.async_below = float(async_below)
         self._opt = opt
         h = C.c_void_p()
         self._check(self.lib.gp_create(C.byref(self._cs), self.M, C.byref(opt), C.byref(h)))

@@ -120,10 +120,15 @@ enum gp_refactor_policy {
                                   reference's cuDSS factorization/refactorization cycle) */
   GP_REFACTOR_FIRST_ITER = 1,  /* factorise at the first iteration of each Newton solve
                                   and reuse it for the later iterations (Schur methods)  */
-  GP_REFACTOR_ADAPTIVE = 2     /* keep the factorisation across time steps; factorise
+  GP_REFACTOR_ADAPTIVE = 2,    /* keep the factorisation across time steps; factorise
                                   again after a fault event or a change of dt, and
                                   whenever an iteration fails to reduce |F|inf by
                                   refactor_theta (default 10x)                         */
+  GP_REFACTOR_ASYNC = 3        /* the iterates of ADAPTIVE, but a refactorisation asked
+                                  for after the first iteration of a solve runs on a
+                                  second stream: the scenario waits for it while the
+                                  other scenarios keep iterating, instead of every
+                                  scenario waiting in the lock step                   */
 };
 
 typedef struct gp_options {
@@ -138,8 +143,11 @@ typedef struct gp_options {
                               as not converged (failed contingencies cost k rounds
                               of the lock step instead of itermax)               */
   int32_t reserved[5];
-  double refactor_theta;   /* GP_REFACTOR_ADAPTIVE: refactorise when an iteration
+  double refactor_theta;   /* ADAPTIVE / ASYNC: refactorise when an iteration
                               reduces |F|inf by less than this factor (0: 10)   */
+  double async_below;      /* ASYNC: a refactorisation asked for at |F|inf below
+                              this runs on the second stream, above it in the
+                              lock step (0: every one runs on the second stream) */
 } gp_options;
 
 typedef struct gp_stats {

@@ -117,8 +117,10 @@ def test_newton_step_matches_oracle(name, method):
             assert np.max(np.abs(dz[m] - ref)) <= tol * (1 + np.max(np.abs(ref))), (name, method, m, dt)
 
 
-@pytest.mark.parametrize("method,policy", [("mono", 0), ("schur64", 0), ("schur64", 2), ("schur32", 2)])
-@pytest.mark.parametrize("name,nsteps", [("case_2bus_testdata", 120), ("case0_2bus", 240), ("case1_9bus", None)])
+@pytest.mark.parametrize("method,policy", [("mono", 0), ("schur64", 0), ("schur64", 2), ("schur32", 2),
+                                           ("schur64", 3), ("schur32", 3)])
+@pytest.mark.parametrize("name,nsteps", [("case_2bus_testdata", 120), ("case0_2bus", 240), ("case1_9bus", None),
+                                         ("case3_200bus", 60)])
 def test_trajectory_matches_oracle(name, nsteps, method, policy):
     case = get_case(name)
     M = min(case.M, 8)
