@@ -5,8 +5,15 @@ beside the plain FP64 monolithic path and Schur FP64), one MI355X.
 A "step" of this benchmark is one pass of the integrator over the whole
 batch: all 256 scenarios integrated from t=0 to their own horizons
 (U(1,5) s at dt=1/120 s).  value = completed contingency-steps / wall
-seconds of that pass, inputs resident on the GPU, system setup and symbolic
-analysis excluded (done in warmup).  Prints one JSON line.
+seconds of that pass (median of the timed passes), inputs resident on the
+GPU; system setup and symbolic analysis are excluded (done in the warmup
+pass).  Prints one JSON line.
+
+The line also carries the roofline of the dominant kernel (the batched
+triangular solve or the batched numeric factorisation, whichever has the
+larger device time in the headline method), timed with HIP events by the
+library around every launch inside the timed passes, and the CPU oracle
+timed on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -21,6 +28,11 @@ for _p in (ROOT, os.path.join(ROOT, "gradpower.jl_amd")):
 
 import numpy as np  # noqa: E402
 
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, published
+KERNEL_NAMES = {("solve", 2): "k_splu_solve<float>", ("solve", 1): "k_splu_solve<double>",
+                ("solve", 0): "k_splu_solve<double>", ("factor", 2): "k_splu_factor<float>",
+                ("factor", 1): "k_splu_factor<double>", ("factor", 0): "k_splu_factor<double>"}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -34,37 +46,43 @@ def events_of(case):
     return ev
 
 
-def run_method(case, method, steps, warmup, record_idx, tol, refactor_policy, stall_window, theta=0.0,
-               async_below=0.0):
+def run_method(case, method, steps, warmup, record_idx, tol, itermax, opts):
     from gradpower_hip.lib import HipContext
-    ctx = HipContext(case.desc, case.M, record_idx=record_idx, refactor_policy=refactor_policy,
-                     stall_window=stall_window, refactor_theta=theta, async_below=async_below)
+    ctx = HipContext(case.desc, case.M, record_idx=record_idx, **opts)
     ev = events_of(case)
-    times, stats = [], None
+    times, stats, kern = [], None, {"solve_ms": 0.0, "solve_n": 0, "solve_sc": 0, "factor_ms": 0.0,
+                                    "factor_n": 0, "factor_sc": 0}
     for it in range(warmup + steps):
         ctx.set_state(case.desc.z0, case.desc.p)
         ctx.set_events(ev, case.nsteps)
         t0 = time.perf_counter()
-        stats = ctx.integrate(method, case.dt, tol, 30, raise_on_fail=False)
+        stats = ctx.integrate(method, case.dt, tol, itermax, raise_on_fail=False)
         t1 = time.perf_counter()
         if it >= warmup:
             times.append(t1 - t0)
+            kern["solve_ms"] += stats["solve_kernel_ms"]
+            kern["solve_n"] += stats["solve_kernel_launches"]
+            kern["solve_sc"] += stats["solve_kernel_scenarios"]
+            kern["factor_ms"] += stats["factor_kernel_ms"]
+            kern["factor_n"] += stats["factor_kernel_launches"]
+            kern["factor_sc"] += stats["factor_kernel_scenarios"]
         log(f"  method {method} pass {it}: {t1 - t0:.3f} s, device {stats['elapsed_ms']:.1f} ms, "
-            f"rounds {stats['newton_rounds']} (factor {stats['factor_launches']}), "
-            f"corrections {stats['newton_iters']}, refactorizations {stats['refactorizations']}, "
-            f"not converged {stats['n_not_converged']}, nan {stats['n_nan']}, status {stats['status']}")
-    speeds = None
-    if record_idx is not None:
-        speeds = [ctx.trajectory(m) for m in range(case.M)]
+            f"steps {stats['scenario_steps']}, rounds {stats['newton_rounds']} "
+            f"(factor {stats['factor_launches']}), corrections {stats['newton_iters']}, "
+            f"refactorizations {stats['refactorizations']}, not converged {stats['n_not_converged']}, "
+            f"solve kernel {stats['solve_kernel_ms']:.1f} ms / {stats['solve_kernel_launches']}, "
+            f"factor kernel {stats['factor_kernel_ms']:.1f} ms / {stats['factor_kernel_launches']}")
+    speeds = [ctx.trajectory(m) for m in range(case.M)] if record_idx is not None else None
     status, steps_done = ctx.scenario_status()
     info = ctx.symbolic_info()
     ctx.close()
     return {"times": np.array(times), "stats": stats, "speeds": speeds, "info": info,
-            "status": status, "steps_done": steps_done}
+            "status": status, "steps_done": steps_done, "kern": kern}
 
 
 def cpu_baseline(case, budget_s=20.0):
-    """The CPU oracle (restated reference CPU path, one core) on the first
+    """The CPU oracle (the reference CPU path restated in C: monolithic Newton
+    with a sparse LU, one core, scenarios one after another) on the first
     scenarios of the same batch, full horizons, until the time budget."""
     from oracle.oracle import Oracle
     o = Oracle(case.desc)
@@ -72,13 +90,39 @@ def cpu_baseline(case, budget_s=20.0):
     for m in range(case.M):
         ev = [(e.bus, e.rfault, int(round(e.ton / case.dt)), int(round(e.toff / case.dt))) for e in case.events[m]]
         t0 = time.perf_counter()
-        o.integrate(case.dt, int(case.nsteps[m]), ev, tol=1e-10, itermax=50, traj=False)
+        o.integrate(case.dt, int(case.nsteps[m]), ev, tol=1e-10, itermax=30, traj=False)
         t_total += time.perf_counter() - t0
         steps += int(case.nsteps[m])
         done += 1
+        log(f"  cpu oracle scenario {m}: {int(case.nsteps[m])} steps, {t_total:.1f} s so far")
         if t_total > budget_s:
             break
     return steps / t_total, done, steps, t_total
+
+
+def roofline(run, method):
+    """Dominant kernel of one method: HIP-event device time per launch and
+    algorithmic bytes per launch (bytes per scenario x scenarios served)."""
+    k, st = run["kern"], run["stats"]
+    which = "solve" if k["solve_ms"] >= k["factor_ms"] else "factor"
+    n = k[which + "_n"]
+    if n == 0:
+        return None
+    avg_s = k[which + "_ms"] / n * 1e-3
+    per_scen = st["solve_bytes_per_scenario" if which == "solve" else "factor_bytes_per_scenario"]
+    bytes_per_launch = per_scen * k[which + "_sc"] / n
+    achieved = bytes_per_launch / avg_s / 1e9
+    name = KERNEL_NAMES[(which, method)]
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_" + name.replace("<", "_").replace(">", "") + ".json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("traffic_bytes_per_launch")
+    return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_us": avg_s * 1e6,
+            "launches": n, "algorithmic_bytes_per_launch": bytes_per_launch,
+            "scenarios_per_launch": k[which + "_sc"] / n, "bytes_per_scenario": per_scen,
+            "share_of_wall": k[which + "_ms"] * 1e-3 / float(np.sum(run["times"]))}
 
 
 def main():
@@ -88,7 +132,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nbus", type=int, default=2000)
     ap.add_argument("--M", type=int, default=256)
-    ap.add_argument("--no-mono", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--horizon-cap", type=float, default=0.0,
@@ -97,11 +140,12 @@ def main():
     ap.add_argument("--policy", default="async", choices=["async", "adaptive"],
                     help="refactorisation policy of the Schur methods")
     ap.add_argument("--theta", type=float, default=10.0,
-                    help="adaptive refactorisation: refactor when |F| falls by less than this factor")
+                    help="refactorise when |F| falls by less than this factor in an iteration")
     ap.add_argument("--async-below", type=float, default=0.0,
                     help="async policy: refactorisations asked for below this |F| run on the second stream")
     ap.add_argument("--stall-window", type=int, default=8,
                     help="stop a scenario whose |F| has not halved in this many Newton iterations (0: run to itermax)")
+    ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the integrator shards by scenario; "
@@ -120,28 +164,28 @@ def main():
         f"steps/scenario {case.nsteps.min()}..{case.nsteps.max()} load_scale={case.meta.get('load_scale')}")
     total_steps = int(case.nsteps.sum())   # scheduled; the value counts completed steps
     widx = gen_speed_indices(case.desc).astype(np.int32)
-    tol = 1e-10
+    tol, itermax = 1e-10, 30
 
-    methods = args.methods.split(",")
-    runs = {}
     spol = {"async": REFACTOR_ASYNC, "adaptive": REFACTOR_ADAPTIVE}[args.policy]
-    for name, meth, pol in (("schur32", METHOD_SCHUR_FP32, spol),
-                            ("schur64", METHOD_SCHUR_FP64, spol),
+    base = {"stall_window": args.stall_window, "refactor_theta": args.theta, "async_below": args.async_below,
+            "profile": not args.no_profile}
+    methods = args.methods.split(",")
+    runs, mcode = {}, {}
+    for name, meth, pol in (("schur32", METHOD_SCHUR_FP32, spol), ("schur64", METHOD_SCHUR_FP64, spol),
                             ("mono", METHOD_MONO_FP64, REFACTOR_EVERY_ITER)):
-        if name not in methods or (name == "mono" and args.no_mono):
+        if name not in methods:
             continue
-        nst = args.steps if name != "mono" else max(1, args.steps // 3)
-        runs[name] = run_method(case, meth, nst, args.warmup if name != "mono" else min(1, args.warmup),
-                                widx if name != "mono" else None, tol, pol, args.stall_window, args.theta,
-                                args.async_below)
-    head = runs.get("schur32") or next(iter(runs.values()))
+        log(f"method {name}")
+        runs[name] = run_method(case, meth, args.steps, args.warmup, widx if name != "mono" else None, tol,
+                                itermax, dict(base, refactor_policy=pol))
+        mcode[name] = meth
+    head_name = "schur32" if "schur32" in runs else next(iter(runs))
+    head = runs[head_name]
     info = head["info"]
-    # value: completed contingency-steps per second of the pass (a scenario
-    # stopped by a failed Newton solve contributes the steps it completed)
     v = {k: r["stats"]["scenario_steps"] / float(np.median(r["times"])) for k, r in runs.items()}
     out = {
         "metric": "contingency-steps/sec at 2k-bus M=256 (Schur+FP32 LU vs plain FP64)",
-        "value": v.get("schur32", next(iter(v.values()))),
+        "value": v[head_name],
         "unit": "contingency-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -150,36 +194,45 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 residual/state, f32 factors",
+        "dtype": "f64 residual/state, f32 factors" if head_name == "schur32" else "f64",
         "data": "synthetic (seeded random geometric 2,000-bus grid, 256 uneven faults)",
         "config": {"workload": f"{args.nbus}-bus synthetic grid, M={args.M} uneven contingencies, dt=1/120 s, "
-                               f"horizons U(1,5) s, Schur + FP32 factors",
-                   "n_sys": case.desc.n, "n_red": info["n_red"], "contingency_steps": total_steps,
-                   "newton_tol": tol, "itermax": 30, "stall_window": args.stall_window,
-                   "refactor_theta": args.theta, "refactor_policy": args.policy,
-                   "async_below": args.async_below,
+                               f"horizons U(1,5) s, {head_name}",
+                   "n_sys": case.desc.n, "n_red": info["n_red"], "nnz_lu_s": info["nnz_lu_s"],
+                   "nnz_lu_j": info["nnz_lu_j"], "contingency_steps_scheduled": total_steps,
+                   "newton_tol": tol, "itermax": itermax, "stall_window": args.stall_window,
+                   "refactor_policy": args.policy, "refactor_theta": args.theta,
                    "horizon_cap": args.horizon_cap or None},
     }
+    rl = roofline(head, mcode[head_name]) if not args.no_profile else None
+    if rl:
+        out["roofline"] = rl
     for k, r in runs.items():
         st = r["stats"]
         out[k] = {"value": v[k], "ms_per_pass": 1e3 * float(np.median(r["times"])),
                   "completed_steps": st["scenario_steps"], "newton_rounds": st["newton_rounds"],
                   "factor_launches": st["factor_launches"], "newton_corrections": st["newton_iters"],
+                  "corrections_per_step": st["newton_iters"] / max(1, st["scenario_steps"]),
                   "refactorizations": st["refactorizations"], "status": st["status"],
                   "failed_scenarios": int(np.sum(r["status"] != 0))}
+        if not args.no_profile and k != head_name:
+            out[k]["roofline"] = roofline(r, mcode[k])
     if "schur32" in runs and "schur64" in runs:
         a, b = runs["schur32"], runs["schur64"]
         ok = [m for m in range(case.M) if a["status"][m] == 0 and b["status"][m] == 0]
         err = max((float(np.max(np.abs(a["speeds"][m] - b["speeds"][m]))) for m in ok), default=None)
         out["schur32"]["rotor_speed_err_vs_fp64_pu"] = err
         out["schur32"]["rotor_speed_err_scenarios"] = len(ok)
+        out["schur32"]["extra_corrections_vs_fp64"] = (a["stats"]["newton_iters"] - b["stats"]["newton_iters"])
     if "schur32" in v and "mono" in v:
         out["speedup_vs_mono_fp64"] = v["schur32"] / v["mono"]
     if not args.no_cpu:
-        v, done, steps, tt = cpu_baseline(case, args.cpu_budget)
-        out["cpu_baseline"] = {"value": v, "unit": "contingency-steps/s", "cores": 1, "kind": "port",
-                               "sample": f"oracle (restated reference CPU path, monolithic sparse LU) on the first "
-                                         f"{done} scenarios, full horizons ({steps} steps, {tt:.1f} s)"}
+        log("cpu oracle")
+        cv, done, steps, tt = cpu_baseline(case, args.cpu_budget)
+        out["cpu_baseline"] = {"value": cv, "unit": "contingency-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle (restated reference CPU path, monolithic sparse-LU Newton, "
+                                         f"1 thread) on the first {done} scenarios of the batch, full horizons "
+                                         f"({steps} steps, {tt:.1f} s)"}
     print(json.dumps(out))
 
 

@@ -199,6 +199,12 @@ struct gp_ctx {
   hipEvent_t ev_snap = nullptr, ev_done = nullptr;
   bool inflight = false;
   std::vector<uint8_t> cur, pend, batch, queue;
+  // kernel timing (gp_options.profile): event pairs around each launch
+  struct KProf {
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    int64_t launches = 0, scen = 0;
+  } prof_solve, prof_factor;
   DBuf<unsigned long long> d_nb;
   DevTables T{};
   LUPlan luJ, luS;
@@ -675,6 +681,42 @@ void ensure_schur_buffers(gp_ctx *C, bool fp32) {
   }
 }
 
+// Events around one launch of a timed kernel (gp_options.profile).
+int prof_begin(gp_ctx *C, gp_ctx::KProf &P, hipStream_t s) {
+  if (!C->opt.profile) return -1;
+  if (P.used + 2 > P.ev.size()) {
+    for (int k = 0; k < 64; k++) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      P.ev.push_back(e);
+    }
+  }
+  const int i = (int)P.used;
+  P.used += 2;
+  HIPCHK(hipEventRecord(P.ev[i], s));
+  return i;
+}
+void prof_end(gp_ctx::KProf &P, int i, hipStream_t s, int64_t scenarios) {
+  if (i < 0) return;
+  HIPCHK(hipEventRecord(P.ev[i + 1], s));
+  P.launches++;
+  P.scen += scenarios;
+}
+// Total milliseconds of the recorded pairs (all complete), then reset.
+double prof_collect(gp_ctx::KProf &P, int64_t &launches, int64_t &scen) {
+  double tot = 0.0;
+  for (size_t i = 0; i + 1 < P.used; i += 2) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, P.ev[i], P.ev[i + 1]));
+    tot += ms;
+  }
+  launches = P.launches;
+  scen = P.scen;
+  P.used = 0;
+  P.launches = P.scen = 0;
+  return tot;
+}
+
 struct NewtonOut {
   int64_t corrections = 0;
   int64_t refac = 0;
@@ -705,8 +747,12 @@ void launch_batch(gp_ctx *C, TF *luv, TF *sv, TF *dn) {
   launch_copy_active(std::max(1, C->n_svar), L, C->d_svar.p, C->d_svar_snap.p, C->d_req.p, C->stream);
   HIPCHK(hipEventRecord(C->ev_snap, C->stream));
   HIPCHK(hipStreamWaitEvent(C->fstream, C->ev_snap, 0));
+  int nb = 0;
+  for (int m = 0; m < L; m++) nb += C->batch[m];
+  const int pi = prof_begin(C, C->prof_factor, C->fstream);
   launch_splu_factor_impl<TF>(C->luS.dev, C->d_svar_snap.p, C->d_s_cst.p, L, luv, sv, dn, C->d_req.p, C->d_tgt.p,
                               C->M, C->d_status.p, C->fstream);
+  prof_end(C->prof_factor, pi, C->fstream, nb);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(C->ev_done, C->fstream));
   // the host copies of batch/tgt/pend must outlive the copies above
@@ -817,10 +863,14 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
       out.refac += cnt[1];
       if (cnt[1] > 0) out.factor_launches++;
       launch_jac_mono(C->T, p, z, dt, faults(C), C->d_jv.p, C->d_upd.p, C->stream);
+      int pi = prof_begin(C, C->prof_factor, C->stream);
       launch_splu_factor_impl<double>(C->luJ.dev, C->d_jv.p, nullptr, L, C->d_luJ.p, C->d_svJ.p, C->d_dnJ.p,
                                       C->d_refac.p, nullptr, C->M, C->d_status.p, C->stream);
+      prof_end(C->prof_factor, pi, C->stream, cnt[1]);
+      pi = prof_begin(C, C->prof_solve, C->stream);
       launch_splu_solve<double>(C->luJ.dev, C->d_svJ.p, C->d_dnJ.p, C->d_f.p, L, -1.0, nullptr, z, nullptr,
                                 C->d_upd.p, nullptr, C->M, C->stream);
+      prof_end(C->prof_solve, pi, C->stream, cnt[2]);
       HIPCHK(hipGetLastError());
       continue;
     }
@@ -847,12 +897,14 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
       }
       out.refac += cnt[1];
       out.factor_launches++;
+      const int pi = prof_begin(C, C->prof_factor, C->stream);
       if (fp32)
         launch_splu_factor_impl<float>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS32.p, C->d_svS32.p,
                                        C->d_dnS32.p, C->d_refac.p, C->d_cur.p, C->M, C->d_status.p, C->stream);
       else
         launch_splu_factor_impl<double>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
                                         C->d_dnS64.p, C->d_refac.p, C->d_cur.p, C->M, C->d_status.p, C->stream);
+      prof_end(C->prof_factor, pi, C->stream, cnt[1]);
     }
     if (async && cnt[3] > 0) {
       // asynchronous requests: queued (and pending: the scenario waits when
@@ -863,15 +915,18 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
       HIPCHK(hipStreamSynchronize(C->stream));   // the host copy stays valid until then
     }
     if (async && !C->inflight && queued()) launch_queue();
-    if (fp32) {
+    const int ps = cnt[2] > 0 ? prof_begin(C, C->prof_solve, C->stream) : -1;
+    if (fp32)
       launch_splu_solve<float>(C->luS.dev, C->d_svS32.p, C->d_dnS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
                                C->d_dv.p, C->d_upd.p, C->d_cur.p, C->M, C->stream);
-      launch_cluster_backsub<float>(C->T, C->d_X32.p, C->d_f.p, C->d_dv.p, z, C->d_upd.p, C->stream);
-    } else {
+    else
       launch_splu_solve<double>(C->luS.dev, C->d_svS64.p, C->d_dnS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
                                 C->d_dv.p, C->d_upd.p, C->d_cur.p, C->M, C->stream);
+    prof_end(C->prof_solve, ps, C->stream, cnt[2]);
+    if (fp32)
+      launch_cluster_backsub<float>(C->T, C->d_X32.p, C->d_f.p, C->d_dv.p, z, C->d_upd.p, C->stream);
+    else
       launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, C->d_upd.p, C->stream);
-    }
     HIPCHK(hipGetLastError());
   }
   // scenarios stopped by the itermax / stall rules, plus (round cap) any
@@ -969,6 +1024,8 @@ void gp_destroy(gp_ctx *C) {
   if (C->stream) (void)hipStreamSynchronize(C->stream);
   hipStream_t s = C->stream, fs = C->fstream;
   hipEvent_t e1 = C->ev_snap, e2 = C->ev_done;
+  for (hipEvent_t e : C->prof_solve.ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : C->prof_factor.ev) (void)hipEventDestroy(e);
   delete C;
   if (e1) (void)hipEventDestroy(e1);
   if (e2) (void)hipEventDestroy(e2);
@@ -1167,6 +1224,16 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
     (void)hipEventDestroy(e1);
     S.elapsed_ms = ms;
     S.solve_ms = ms;
+    if (C->opt.profile) {
+      HIPCHK(hipStreamSynchronize(C->fstream));
+      S.solve_kernel_ms = prof_collect(C->prof_solve, S.solve_kernel_launches, S.solve_kernel_scenarios);
+      S.factor_kernel_ms = prof_collect(C->prof_factor, S.factor_kernel_launches, S.factor_kernel_scenarios);
+      const bool mono = method == GP_METHOD_MONO_FP64;
+      const SpluAnalysis &A = mono ? C->luJ.A : C->luS.A;
+      const double bf = method == GP_METHOD_SCHUR_FP32 ? 4.0 : 8.0;
+      S.solve_bytes_per_scenario = bf * A.nnz_lu + (mono ? 24.0 : 32.0) * A.n;
+      S.factor_bytes_per_scenario = 8.0 * A.nnz_a + bf * A.nnz_lu;
+    }
     for (int m = 0; m < M; m++) S.n_not_converged += everfail[m];
     std::vector<int> status(L);
     HIPCHK(hipMemcpy(status.data(), C->d_status.p, L * sizeof(int), hipMemcpyDeviceToHost));
@@ -1211,6 +1278,18 @@ int gp_get_trajectory(gp_ctx *C, int32_t m, double *out, int64_t ld) {
     HIPCHK(hipStreamSynchronize(C->stream));
     for (int j = 0; j < C->nrec; j++)
       for (int k = 0; k < len; k++) out[(size_t)j * ld + k] = tmp[(size_t)j * len + k];
+  } catch (const GpError &e) {
+    return fail(e.code, e.msg);
+  }
+  return GP_OK;
+}
+
+int gp_copy_kernel(gp_ctx *C, const double *src, double *dst, int64_t n) {
+  if (!C || !src || !dst || n < 0) return fail(GP_ERR_BADARG, "gp_copy_kernel: bad argument");
+  try {
+    launch_copy_plain(src, dst, n, C->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(C->stream));
   } catch (const GpError &e) {
     return fail(e.code, e.msg);
   }

@@ -110,6 +110,7 @@ void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const doubl
                        hipStream_t s);
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s);
 void launch_gather_traj(int nrec, int len, int ldM, int m, const double *hist, double *out, hipStream_t s);
+void launch_copy_plain(const double *src, double *dst, long long n, hipStream_t s);
 void launch_record_impl(int nrec, const int *idx, int ldM, int M, const double *z, double *hist,
                         const uint8_t *act, hipStream_t s);
 

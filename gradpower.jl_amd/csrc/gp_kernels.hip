@@ -1863,6 +1863,12 @@ __global__ void k_gather_traj(int nrec, int len, int ldM, int m, const double *_
   out[i] = hist[((size_t)k * nrec + j) * ldM + m];
 }
 
+// Plain copy (calibration of byte counters).
+__global__ void k_copy_plain(const double *__restrict__ src, double *__restrict__ dst, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------- launchers
 static inline dim3 lane_grid(int items, int ldM) { return dim3(items, ldM / WAVE); }
 
@@ -1927,6 +1933,11 @@ void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const doubl
                               hipFuncAttributeMaxDynamicSharedMemorySize, S.lds_solve);
   hipLaunchKernelGGL(k_splu_solve<TF>, dim3(M), dim3(S.threads), S.lds_solve, s, S, sv, dn, rhs, ldM, sign, scatter,
                      zupd, xout, work, cur);
+}
+void launch_copy_plain(const double *src, double *dst, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  const long long blocks = std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_copy_plain, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
 }
 void launch_gather_traj(int nrec, int len, int ldM, int m, const double *hist, double *out, hipStream_t s) {
   const size_t tot = (size_t)nrec * len;

@@ -35,7 +35,8 @@ class GpError(RuntimeError):
 class COptions(C.Structure):
     _fields_ = [("record_all", C.c_int32), ("n_record", C.c_int32),
                 ("record_idx", C.POINTER(C.c_int32)), ("refactor_policy", C.c_int32),
-                ("stall_window", C.c_int32), ("reserved", C.c_int32 * 5), ("refactor_theta", C.c_double),
+                ("stall_window", C.c_int32), ("profile", C.c_int32), ("reserved", C.c_int32 * 4),
+                ("refactor_theta", C.c_double),
                 ("async_below", C.c_double)]
 
 
@@ -44,7 +45,11 @@ class CStats(C.Structure):
                 ("event_resolves", C.c_int64), ("refactorizations", C.c_int64),
                 ("n_not_converged", C.c_int32), ("n_nan", C.c_int32),
                 ("max_final_residual", C.c_double), ("elapsed_ms", C.c_double),
-                ("solve_ms", C.c_double), ("newton_rounds", C.c_int64), ("factor_launches", C.c_int64)]
+                ("solve_ms", C.c_double), ("newton_rounds", C.c_int64), ("factor_launches", C.c_int64),
+                ("solve_kernel_ms", C.c_double), ("solve_kernel_launches", C.c_int64),
+                ("solve_kernel_scenarios", C.c_int64), ("factor_kernel_ms", C.c_double),
+                ("factor_kernel_launches", C.c_int64), ("factor_kernel_scenarios", C.c_int64),
+                ("solve_bytes_per_scenario", C.c_double), ("factor_bytes_per_scenario", C.c_double)]
 
 
 class CSymInfo(C.Structure):
@@ -62,7 +67,7 @@ _LIB = None
 EXPORTS = ["gp_create", "gp_destroy", "gp_set_state", "gp_set_events", "gp_integrate",
            "gp_get_trajectory", "gp_get_state", "gp_eval_residual", "gp_jacobian_pattern",
            "gp_eval_jacobian", "gp_newton_step", "gp_get_symbolic_info", "gp_get_scenario_status",
-           "gp_last_error", "gp_version"]
+           "gp_copy_kernel", "gp_last_error", "gp_version"]
 
 
 def load():
@@ -87,6 +92,7 @@ def load():
     lib.gp_eval_jacobian.argtypes = [C.c_void_p, _dp, C.c_double, _up, _dp]
     lib.gp_get_symbolic_info.argtypes = [C.c_void_p, C.c_void_p]
     lib.gp_get_scenario_status.argtypes = [C.c_void_p, _ip, _ip]
+    lib.gp_copy_kernel.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
     lib.gp_newton_step.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, C.c_double, _up, _dp]
     lib.gp_last_error.restype = C.c_char_p
     lib.gp_version.restype = C.c_char_p
@@ -110,7 +116,8 @@ class HipContext:
     """One library context: a system, M scenarios, device state on the current GPU."""
 
     def __init__(self, desc: SystemDesc, M: int, record_idx=None,
-                 refactor_policy=REFACTOR_EVERY_ITER, stall_window=0, refactor_theta=0.0, async_below=0.0):
+                 refactor_policy=REFACTOR_EVERY_ITER, stall_window=0, refactor_theta=0.0, async_below=0.0,
+                 profile=False):
         self.lib = load()
         self.desc = desc
         self.M = int(M)
@@ -130,6 +137,7 @@ class HipContext:
         opt.stall_window = int(stall_window)
         opt.refactor_theta = float(refactor_theta)
         opt.async_below = float(async_below)
+        opt.profile = 1 if profile else 0
         self._opt = opt
         h = C.c_void_p()
         self._check(self.lib.gp_create(C.byref(self._cs), self.M, C.byref(opt), C.byref(h)))

@@ -1,0 +1,98 @@
+"""Summaries of rocprofv3 CSV output for the profiles/ folder.
+
+    python3 -m gradpower_hip.prof_summary stats DIR OUT.md --cmd "..."
+        per-kernel table (calls, total ms, average us, share) from *kernel_stats.csv
+
+    python3 -m gradpower_hip.prof_summary traffic FETCH_DIR WRITE_DIR KERNEL COPY_BYTES OUT.json --cmd "..."
+        HBM bytes per launch of KERNEL from a FETCH_SIZE run and a WRITE_SIZE run
+        (KiB per dispatch), each scaled by what the counter reads for the plain
+        copy kernel k_copy_plain, which reads and writes COPY_BYTES
+"""
+import csv
+import glob
+import json
+import os
+import sys
+
+
+def _rows(d, pattern):
+    files = sorted(glob.glob(os.path.join(d, "**", pattern), recursive=True))
+    if not files:
+        raise SystemExit(f"no {pattern} under {d}")
+    out = []
+    for f in files:
+        with open(f, newline="") as fh:
+            out.extend(csv.DictReader(fh))
+    return out
+
+
+def _col(row, *names):
+    low = {k.lower(): k for k in row}
+    for n in names:
+        if n.lower() in low:
+            return row[low[n.lower()]]
+    raise KeyError(f"none of {names} in {list(row)}")
+
+
+def short(name):
+    s = name.replace("void ", "").replace("gp::", "")
+    return s.split("(")[0]
+
+
+def stats(d, out, cmd):
+    rows = _rows(d, "*kernel_stats.csv")
+    tab = []
+    for r in rows:
+        tot = float(_col(r, "TotalDurationNs", "Total_Duration_Ns", "TotalDuration"))
+        tab.append((short(_col(r, "Name", "Kernel_Name", "KernelName")), int(_col(r, "Calls")), tot,
+                    float(_col(r, "AverageNs", "Average_Ns", "Average")), float(_col(r, "Percentage"))))
+    tab.sort(key=lambda t: -t[2])
+    with open(out, "w") as fh:
+        fh.write(f"# Kernel time summary\n\nCommand (from the repository root, under `cd /tmp && export TMPDIR=/tmp`):\n\n"
+                 f"```\n{cmd}\n```\n\n| kernel | calls | total ms | average us | share % |\n|---|---:|---:|---:|---:|\n")
+        for name, calls, tot, avg, pct in tab[:25]:
+            fh.write(f"| `{name}` | {calls} | {tot / 1e6:.2f} | {avg / 1e3:.2f} | {pct:.2f} |\n")
+    print(open(out).read())
+
+
+def _counter(d, kernel, counter):
+    vals = []
+    for r in _rows(d, "*counter_collection.csv"):
+        if kernel in _col(r, "Kernel_Name", "KernelName", "Name") and \
+                _col(r, "Counter_Name", "CounterName") == counter:
+            vals.append(float(_col(r, "Counter_Value", "CounterValue")))
+    if not vals:
+        raise SystemExit(f"no {counter} samples for {kernel} under {d}")
+    return sum(vals) / len(vals), len(vals)
+
+
+def traffic(fdir, wdir, kernel, copy_bytes, out, cmd):
+    copy_bytes = float(copy_bytes)
+    kf, nkf = _counter(fdir, kernel, "FETCH_SIZE")
+    kw, nkw = _counter(wdir, kernel, "WRITE_SIZE")
+    cf, _ = _counter(fdir, "k_copy_plain", "FETCH_SIZE")
+    cw, _ = _counter(wdir, "k_copy_plain", "WRITE_SIZE")
+    sf, sw = copy_bytes / cf, copy_bytes / cw   # bytes per counter unit
+    res = {"kernel": kernel, "fetch_size_per_dispatch": kf, "write_size_per_dispatch": kw,
+           "dispatches": [nkf, nkw], "copy_bytes": copy_bytes, "copy_fetch_size": cf, "copy_write_size": cw,
+           "fetch_scale_bytes_per_unit": sf, "write_scale_bytes_per_unit": sw,
+           "fetch_bytes_per_launch": kf * sf, "write_bytes_per_launch": kw * sw,
+           "traffic_bytes_per_launch": kf * sf + kw * sw, "command": cmd}
+    with open(out, "w") as fh:
+        json.dump(res, fh, indent=1)
+    print(json.dumps(res, indent=1))
+
+
+if __name__ == "__main__":
+    a = sys.argv[1:]
+    cmd = ""
+    if "--cmd" in a:
+        i = a.index("--cmd")
+        cmd = a[i + 1]
+        a = a[:i] + a[i + 2:]
+    if a[0] == "stats":
+        stats(a[1], a[2], cmd)
+    elif a[0] == "traffic":
+        traffic(a[1], a[2], a[3], a[4], a[5], cmd)
+    else:
+        raise SystemExit(__doc__)

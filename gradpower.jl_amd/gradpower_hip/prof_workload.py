@@ -1,0 +1,62 @@
+"""Workload driver for hardware-counter and kernel-trace runs (rocprofv3):
+a plain copy kernel of a known size (to calibrate the byte counters) and
+one pass of the 2,000-bus M=256 contingency workload, optionally with the
+scenario horizons capped so that a counter run stays short.
+
+    rocprofv3 --kernel-trace --stats --output-format csv -d OUT -o run -- \\
+        python3 -m gradpower_hip.prof_workload --horizon-cap 0.5
+"""
+import argparse
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import numpy as np  # noqa: E402
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-mb", type=float, default=256.0)
+    ap.add_argument("--horizon-cap", type=float, default=0.5)
+    ap.add_argument("--method", default="schur32", choices=["schur32", "schur64", "mono"])
+    ap.add_argument("--passes", type=int, default=1)
+    args = ap.parse_args()
+    import ctypes as C
+    from gradpower_hip import synth
+    from gradpower_hip.lib import (HipContext, METHOD_MONO_FP64, METHOD_SCHUR_FP32, METHOD_SCHUR_FP64,
+                                   REFACTOR_ASYNC, REFACTOR_EVERY_ITER)
+    t0 = time.perf_counter()
+    case = synth.case2_grid(M=256)
+    case.tf = np.minimum(case.tf, args.horizon_cap)
+    print(f"setup {time.perf_counter() - t0:.1f} s", flush=True)
+    meth = {"schur32": METHOD_SCHUR_FP32, "schur64": METHOD_SCHUR_FP64, "mono": METHOD_MONO_FP64}[args.method]
+    pol = REFACTOR_EVERY_ITER if args.method == "mono" else REFACTOR_ASYNC
+    ctx = HipContext(case.desc, case.M, record_idx=np.zeros(1, np.int32), refactor_policy=pol, stall_window=8)
+    # calibration copy: n doubles device to device through the library's copy kernel
+    n = int(args.copy_mb * 1e6 / 8)
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(8 * n)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(8 * n)) == 0
+    assert hip.hipMemset(src, 0, C.c_size_t(8 * n)) == 0
+    rc = ctx.lib.gp_copy_kernel(ctx.h, src, dst, n)
+    assert rc == 0, ctx.lib.gp_last_error()
+    print(f"copy kernel: {8 * n} bytes read, {8 * n} bytes written", flush=True)
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    ev = [(m, e.bus, e.rfault, int(round(e.ton / case.dt)), int(round(e.toff / case.dt)))
+          for m in range(case.M) for e in case.events[m]]
+    for it in range(args.passes):
+        ctx.set_state(case.desc.z0, case.desc.p)
+        ctx.set_events(ev, case.nsteps)
+        st = ctx.integrate(meth, case.dt, 1e-10, 30, raise_on_fail=False)
+        print(f"pass {it}: {st['scenario_steps']} steps, {st['newton_rounds']} rounds, "
+              f"{st['elapsed_ms']:.1f} ms", flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -61,8 +61,12 @@ def _genrou(bus, gid="1", **over):
                   p["S1"], p["S2"])
 
 
+SEXS_K = 50.0       # exciter gains of the synthetic grids
+ESDC1A_KA = 46.0
+
+
 def _sexs(bus, gid="1"):
-    return SEXS(bus, gid, 0.1, 10.0, 50.0, 0.05, -5.0, 5.0)
+    return SEXS(bus, gid, 0.1, 10.0, SEXS_K, 0.05, -5.0, 5.0)
 
 
 def _tgov1(bus, gid="1"):
@@ -71,7 +75,7 @@ def _tgov1(bus, gid="1"):
 
 def _esdc1a(bus, gid="1"):
     sa, sb = esdc1a_sat_coefficients(2.8, 0.04, 3.73, 0.33)
-    return ESDC1A(bus, gid, 46.0, 0.06, 0.1, 1.0, 1.0, 0.46, 0.0, sa, sb)
+    return ESDC1A(bus, gid, ESDC1A_KA, 0.06, 0.1, 1.0, 1.0, 0.46, 0.0, sa, sb)
 
 
 def _ieesgo(bus, gid, rng, meta):
@@ -134,21 +138,29 @@ def _solve_pf_scaling(ps, meta):
     meta["load_scale"] = s_ok
 
 
-def _finish(name, ps, psd, M, events, tf, meta, zip_alpha=None, pq_after_pf=None):
-    if pq_after_pf is not None:
-        # power flow with the given buses voltage-controlled, then freeze them
-        # as constant P/Q injections at the solved operating point
+def _finish(name, ps, psd, M, events, tf, meta, zip_alpha=None, pq_after_pf=None, size_machines=False):
+    pf_done = False
+    if pq_after_pf is not None or size_machines:
+        # power flow first: optionally freeze the given voltage-controlled
+        # buses as constant P/Q injections at the solved operating point, and
+        # size each generator's MVA base to its solved loading (at most 80%),
+        # which the machine models scale with (the slack machine in particular
+        # picks up the losses and the dispatch mismatch)
         build_network(ps)
         _solve_pf_scaling(ps, meta)
-        for b in pq_after_pf:
+        pf_done = True
+        for b in pq_after_pf or []:
             ps.buses[int(b)].type = PQ
+        if size_machines:
+            for g in ps.gens:
+                g.mbase = max(g.mbase, 100.0 * math.hypot(g.psch, g.qsch) / 0.8)
     set_dynamics(ps, psd)
     if zip_alpha is not None:
         for dev in psd.devices:
             if dev.dtype.kind == "zipload":
                 dev.dtype.alpha = zip_alpha
     build_network(ps)
-    if pq_after_pf is None:
+    if not pf_done:
         _solve_pf_scaling(ps, meta)
     else:
         runpf_inplace(ps)
@@ -314,7 +326,7 @@ def _regional_dispatch(edges, n, gen_buses, loads, floor=0.1):
 
 def case2_grid(M=256, nbus=2000, n_genrou=400, n_pss=100, n_static=300, seed=1002,
                load_fraction=0.2, tf_range=(1.0, 5.0), name="c2_grid", charging_scale=1.0,
-               zip_alpha=1.0, static_gen_type=PV, static_gen_p=1.0, fault_sg_hops=3):
+               zip_alpha=1.0, static_gen_type=PV, static_gen_p=1.0, fault_sg_hops=3, size_machines=True):
     rng = np.random.default_rng(seed)
     meta = {"seed": seed, "zip_alpha": zip_alpha, "nbus": nbus,
             "zip_note": "constant-impedance loads (alpha=1, the reference default of "
@@ -393,7 +405,8 @@ def case2_grid(M=256, nbus=2000, n_genrou=400, n_pss=100, n_static=300, seed=100
                                         float(rng.uniform(0.01, 0.2)), 0.02, float(rng.uniform(0.05, 0.25)))])
         tf.append(float(rng.uniform(*tf_range)))
     return _finish(name, ps, psd, M, events, tf, meta, zip_alpha=zip_alpha,
-                   pq_after_pf=[int(b) for b in sbus] if static_gen_type == PQ else None)
+                   pq_after_pf=[int(b) for b in sbus] if static_gen_type == PQ else None,
+                   size_machines=size_machines)
 
 
 def case3_200bus(M=128, tf=3.0):

@@ -142,7 +142,10 @@ typedef struct gp_options {
                               1e10, or is not finite, stops at once and is reported
                               as not converged (failed contingencies cost k rounds
                               of the lock step instead of itermax)               */
-  int32_t reserved[5];
+  int32_t profile;         /* 1: time every launch of the batched triangular solve
+                              and of the numeric factorisation with HIP events on
+                              the stream it runs on (gp_stats kernel fields)    */
+  int32_t reserved[4];
   double refactor_theta;   /* ADAPTIVE / ASYNC: refactorise when an iteration
                               reduces |F|inf by less than this factor (0: 10)   */
   double async_below;      /* ASYNC: a refactorisation asked for at |F|inf below
@@ -162,6 +165,17 @@ typedef struct gp_stats {
   double solve_ms;             /* device time inside the integrator's Newton loop */
   int64_t newton_rounds;       /* lock-step Newton iterations (launch rounds)      */
   int64_t factor_launches;     /* rounds that launched a numeric factorisation     */
+  /* with gp_options.profile: device time, launches and scenarios served of
+     the batched triangular solve and of the numeric factorisation, and their
+     algorithmic HBM bytes per scenario (solve: b_f*nnz(L+U) + 32*rows for the
+     factor values, rhs, solution and state update; factorisation:
+     8*nnz(A) + b_f*nnz(L+U) for the matrix values read and the factors
+     written; b_f = 4 or 8 bytes per factor value)                          */
+  double solve_kernel_ms;
+  int64_t solve_kernel_launches, solve_kernel_scenarios;
+  double factor_kernel_ms;
+  int64_t factor_kernel_launches, factor_kernel_scenarios;
+  double solve_bytes_per_scenario, factor_bytes_per_scenario;
 } gp_stats;
 
 typedef struct gp_ctx gp_ctx;
@@ -232,6 +246,11 @@ typedef struct gp_symbolic_info {
   int32_t reserved[6];
 } gp_symbolic_info;
 int gp_get_symbolic_info(gp_ctx *ctx, gp_symbolic_info *info);
+
+/* Measurement helper with no counterpart in the reference: a plain device
+ * copy kernel dst[i] = src[i], i < n (device pointers), on the context's
+ * stream; used to calibrate hardware byte counters against a known size. */
+int gp_copy_kernel(gp_ctx *ctx, const double *src, double *dst, int64_t n);
 
 const char *gp_last_error(void);
 const char *gp_version(void);
