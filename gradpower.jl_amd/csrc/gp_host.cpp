@@ -73,7 +73,11 @@ struct LUPlan {
   SpluAnalysis A;
   SpluDev dev{};
   DBuf<int> a2lu, a_src, op_loc, op_sl, st_o, flp, fac_mode, crec, lu2s, dmap, fw_lvl, fw_chunk, fw_cols, fw_rows,
-      bw_lvl, bw_chunk, bw_cols, bw_rows, q, pinv;
+      bw_lvl, bw_chunk, bw_cols, bw_rows, q, pinv, r_hdr, r_blob, r_a2x, r_omap;
+  ResPlan rp;                  // LDS-resident factorisation plan
+  bool rp_ok = false;
+  // LU values per scenario the factorisation kernels use
+  size_t lu_elems() const { return rp_ok ? std::max<size_t>(A.nnz_lu, rp.xtot) : (size_t)A.nnz_lu; }
   bool ready = false;
   double dt = -1.0;
   static constexpr int kThreads = 512;
@@ -111,6 +115,13 @@ struct LUPlan {
     bw_lvl.upload(nonempty(A.sb.lvl)); bw_chunk.upload(nonempty(A.sb.chunk));
     bw_cols.upload(nonempty(A.sb.cols)); bw_rows.upload(nonempty(A.sb.rows));
     q.upload(A.q); pinv.upload(A.pinv);
+    rp_ok = res_plan(A, kResCB * kThreads, kResCX * kThreads, rp);
+    if (rp_ok) {
+      r_hdr.upload(rp.hdr);
+      r_blob.upload(rp.blob);
+      r_a2x.upload(rp.a2x);
+      r_omap.upload(rp.omap);
+    }
     SpluDev &d = dev;
     d.n = A.n; d.nnz_a = A.nnz_a; d.nnz_lu = A.nnz_lu; d.ns = std::max(1, A.ns);
     d.n_flev = (int)A.flp.size() - 1; d.nd = A.nd;
@@ -141,8 +152,27 @@ struct LUPlan {
     dev.w_ops = wo;
     dev.w_bytes = wbytes(wc, wo);
     dev.lds_factor = std::max<int>(std::max<int>((int)((size_t)kThreads * tc * elem), W * dev.w_bytes),
-                                   (int)((size_t)A.nd * A.nd * elem));
+                                   (int)((((size_t)A.nd * A.nd + 3) / 4 * 4 + 512) * elem));   // dense block, buffers
     dev.lds_solve = table_bytes() + (int)((size_t)A.nd * A.nd * elem);
+    // LDS-resident factorisation when its multipliers (or the dense block),
+    // staging blob and column values fit
+    dev.res_nsteps = 0;
+    if (rp_ok) {
+      const int lv = std::max(rp.nL + 1, A.nd * A.nd);
+      const size_t need = ((size_t)lv * elem + 15) / 16 * 16 + (size_t)kResCB * kThreads * 4 +
+                          2 * (size_t)kResCX * kThreads * elem;
+      if (need <= (size_t)kLdsMax) {
+        dev.res_nsteps = rp.nsteps();
+        dev.res_lv = lv;
+        dev.res_nl = rp.nL;
+        dev.res_xtot = rp.xtot;
+        dev.res_hdr = r_hdr.p;
+        dev.res_blob = r_blob.p;
+        dev.res_a2x = r_a2x.p;
+        dev.res_omap = r_omap.p;
+        dev.lds_factor = (int)need;
+      }
+    }
   }
 };
 
@@ -623,7 +653,7 @@ void analyse(gp_ctx *C, int method, double dt) {
     C->luJ.shape(sizeof(double));
     C->luJ.ready = true;
     C->luJ.dt = dt;
-    C->d_luJ.alloc((size_t)C->luJ.A.nnz_lu * C->M);
+    C->d_luJ.alloc(C->luJ.lu_elems() * C->M);
     C->d_svJ.alloc((size_t)C->luJ.dev.ns * C->M);
     HIPCHK(hipMemset(C->d_svJ.p, 0, C->d_svJ.n * sizeof(double)));   // padding slots stay zero
     C->d_dnJ.alloc((size_t)std::max(1, C->luJ.A.nd * C->luJ.A.nd) * C->M);
@@ -663,7 +693,7 @@ void ensure_schur_buffers(gp_ctx *C, bool fp32) {
   const size_t L = C->ldM;
   if (fp32) {
     C->d_X32.ensure((size_t)std::max(1, C->xrows) * 3 * L);
-    C->d_luS32.ensure((size_t)C->luS.A.nnz_lu * C->M);
+    C->d_luS32.ensure(C->luS.lu_elems() * C->M);
     if (C->d_svS32.n != (size_t)2 * C->luS.dev.ns * C->M) {   // two buffers per scenario
       C->d_svS32.alloc((size_t)2 * C->luS.dev.ns * C->M);
       HIPCHK(hipMemset(C->d_svS32.p, 0, C->d_svS32.n * sizeof(float)));   // padding slots stay zero
@@ -671,7 +701,7 @@ void ensure_schur_buffers(gp_ctx *C, bool fp32) {
     C->d_dnS32.ensure((size_t)2 * std::max(1, C->luS.A.nd * C->luS.A.nd) * C->M);
     C->luS.shape(sizeof(float));
   } else {
-    C->d_luS64.ensure((size_t)C->luS.A.nnz_lu * C->M);
+    C->d_luS64.ensure(C->luS.lu_elems() * C->M);
     if (C->d_svS64.n != (size_t)2 * C->luS.dev.ns * C->M) {   // two buffers per scenario
       C->d_svS64.alloc((size_t)2 * C->luS.dev.ns * C->M);
       HIPCHK(hipMemset(C->d_svS64.p, 0, C->d_svS64.n * sizeof(double)));   // padding slots stay zero

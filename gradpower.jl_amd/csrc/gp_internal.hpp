@@ -74,7 +74,15 @@ struct SpluDev {
   int t_cols;                   // LDS staging per thread (thread mode), 0 = none
   int w_col, w_ops, w_bytes;    // LDS staging per wavefront (wavefront mode)
   int lds_factor, lds_solve;    // dynamic LDS bytes
+  // LDS-resident factorisation (res_nsteps > 0): step headers, blobs, A
+  // entry -> step-ordered position, LDS elements of the multiplier / dense
+  // block region
+  int res_nsteps, res_lv, res_nl, res_xtot;
+  const int *res_hdr, *res_blob, *res_a2x, *res_omap;
 };
+// staging of the LDS-resident factorisation, per thread of a 512-thread
+// workgroup: blob dwords and initial column values
+constexpr int kResCB = 24, kResCX = 8;
 
 // ---- kernel launchers (gp_kernels.hip) ----------------------------------
 void launch_cluster_residual(const DevTables &T, const double *p, const double *z, const double *zold,

@@ -18,6 +18,7 @@
 
 #include "../../include/gradpower_hip.h"
 #include "gp_internal.hpp"
+#include "gp_splu.hpp"
 
 namespace gp {
 
@@ -1450,6 +1451,103 @@ __device__ __forceinline__ int wave_column(const SpluDev &S, TF *__restrict__ lu
   return (!(piv != TF(0)) || !isfinite((double)piv)) ? 1 : 0;
 }
 
+// Right-looking LU without pivoting of the nd x nd block D (column-major,
+// LDS, nd <= 128) by a 512-thread workgroup, the block held in registers:
+// thread (i = lane, w = wavefront) owns rows i and i + 64 of columns
+// w, w + 8, ..., w + 120.  For pivot p the wavefront owning column p scales
+// it into multipliers and publishes them (zero for rows <= p), lane p % 64
+// of every wavefront publishes its part of row p (zero for columns <= p);
+// after one barrier everyone applies a -= l * u to its columns right of p
+// (a zero operand leaves an entry exactly as it is).  The multipliers and
+// the row go through double buffers (lbuf: 2 x 128 values, ubuf: 2 x 8 x 16
+// values), so one barrier per pivot suffices.  Same arithmetic as a scalar
+// right-looking LU (l = a * (1/piv), a -= l * u).  L below the diagonal, U
+// on and above it, written back to D.  Returns 1 if a pivot is zero or not
+// finite (a non-finite operand also spreads through the zero products).
+// Ends with a barrier.
+// Pivot p in register column K of its owner wavefront: scale the column
+// below the diagonal and publish the multipliers (zero for rows <= p).
+template <typename TF, int K>
+__device__ __forceinline__ void dense_pivot(TF (&a)[2][16], int p, int nd, int i0, TF *lb, int &bad) {
+  const TF piv = __shfl(p < 64 ? a[0][K] : a[1][K], p & 63);
+  if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
+  const TF inv = TF(1) / piv;
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int i = i0 + 64 * r;
+    const bool below = i > p && i < nd;
+    if (below) a[r][K] *= inv;
+    lb[i] = below ? a[r][K] : TF(0);
+  }
+}
+
+template <typename TF>
+__device__ __forceinline__ int dense_lu(TF *D, int nd, int tid, TF *lbuf, TF *ubuf) {
+  constexpr int NR = 2, NC = 16;
+  const int i0 = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  TF a[NR][NC];
+#pragma unroll
+  for (int r = 0; r < NR; r++)
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      const int i = i0 + 64 * r, j = w + 8 * k;
+      a[r][k] = (i < nd && j < nd) ? D[i + j * nd] : TF(0);
+    }
+  int bad = 0;
+  for (int p = 0; p < nd; p++) {
+    TF *lb = lbuf + (p & 1) * 128;
+    TF *ub = ubuf + (p & 1) * 128;
+    if (w == (p & 7)) {
+      switch (p >> 3) {
+        case 0: dense_pivot<TF, 0>(a, p, nd, i0, lb, bad); break;
+        case 1: dense_pivot<TF, 1>(a, p, nd, i0, lb, bad); break;
+        case 2: dense_pivot<TF, 2>(a, p, nd, i0, lb, bad); break;
+        case 3: dense_pivot<TF, 3>(a, p, nd, i0, lb, bad); break;
+        case 4: dense_pivot<TF, 4>(a, p, nd, i0, lb, bad); break;
+        case 5: dense_pivot<TF, 5>(a, p, nd, i0, lb, bad); break;
+        case 6: dense_pivot<TF, 6>(a, p, nd, i0, lb, bad); break;
+        case 7: dense_pivot<TF, 7>(a, p, nd, i0, lb, bad); break;
+        case 8: dense_pivot<TF, 8>(a, p, nd, i0, lb, bad); break;
+        case 9: dense_pivot<TF, 9>(a, p, nd, i0, lb, bad); break;
+        case 10: dense_pivot<TF, 10>(a, p, nd, i0, lb, bad); break;
+        case 11: dense_pivot<TF, 11>(a, p, nd, i0, lb, bad); break;
+        case 12: dense_pivot<TF, 12>(a, p, nd, i0, lb, bad); break;
+        case 13: dense_pivot<TF, 13>(a, p, nd, i0, lb, bad); break;
+        case 14: dense_pivot<TF, 14>(a, p, nd, i0, lb, bad); break;
+        default: dense_pivot<TF, 15>(a, p, nd, i0, lb, bad); break;
+      }
+    }
+    if (i0 == (p & 63)) {
+      // row p of this wavefront's columns right of p
+#pragma unroll
+      for (int k = 0; k < NC; k++) {
+        const int j = w + 8 * k;
+        ub[w * 16 + k] = (j > p && j < nd) ? (p < 64 ? a[0][k] : a[1][k]) : TF(0);
+      }
+    }
+    __syncthreads();
+    // a -= l * u everywhere: rows <= p have l = 0 and columns <= p have u = 0
+    TF l[NR], u[NC];
+#pragma unroll
+    for (int r = 0; r < NR; r++) l[r] = lb[i0 + 64 * r];
+#pragma unroll
+    for (int k = 0; k < NC; k++) u[k] = ub[w * 16 + k];
+#pragma unroll
+    for (int k = 0; k < NC; k++)
+#pragma unroll
+      for (int r = 0; r < NR; r++) a[r][k] = a[r][k] - l[r] * u[k];
+  }
+#pragma unroll
+  for (int r = 0; r < NR; r++)
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+      const int i = i0 + 64 * r, j = w + 8 * k;
+      if (i < nd && j < nd) D[i + j * nd] = a[r][k];
+    }
+  __syncthreads();
+  return bad;
+}
+
 // svv/dnv hold two buffers per scenario ([2][M][...]); tgt[m] selects the
 // one written (NULL: buffer 0).
 template <typename TF>
@@ -1503,70 +1601,210 @@ __global__ void __launch_bounds__(512) k_splu_factor(SpluDev S, const double *__
       if (d >= 0) D[d] = lu[S.qb + q];
     }
     __syncthreads();
-    // thread (il, jl) owns rows i = p+1+il (+64) and columns j = p+1+jl (+8k) of
-    // the trailing update; its operands are read in one batch, and the
-    // multipliers of pivot p are written back after the next barrier, when
-    // nobody reads column p any more
-    const int il = tid & 63, jl = tid >> 6;
-    TF lkeep[2] = {TF(0), TF(0)};
-    int pkeep = -1;
-    for (int p = 0; p < nd; p++) {
-      if (jl == 0 && pkeep >= 0) {
-#pragma unroll
-        for (int a = 0; a < 2; a++) {
-          const int i = pkeep + 1 + il + WAVE * a;
-          if (i < nd) D[i + pkeep * nd] = lkeep[a];
-        }
-      }
-      const TF piv = D[p + p * nd];
-      if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
-      const TF inv = TF(1) / piv;
-      TF li[2];
-      int ir[2];
-#pragma unroll
-      for (int a = 0; a < 2; a++) {
-        const int i = p + 1 + il + WAVE * a;
-        ir[a] = min(i, nd - 1);
-        li[a] = D[ir[a] + p * nd] * inv;
-        if (i >= nd) ir[a] = -1;
-      }
-      const int ncol = nd - p - 1 - jl;   // columns j = p+1+jl+8b, b < ceil(ncol/8)
-      for (int b0 = 0; 8 * b0 < ncol; b0 += 4) {
-        TF uj[4], d0[4], d1[4];
-        int jj[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int j = p + 1 + jl + 8 * (b0 + u);
-          jj[u] = j < nd ? j : -1;
-          const int jc = min(j, nd - 1);
-          uj[u] = D[p + jc * nd];
-          d0[u] = D[max(ir[0], 0) + jc * nd];
-          d1[u] = D[max(ir[1], 0) + jc * nd];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          if (jj[u] >= 0) {
-            if (ir[0] >= 0) D[ir[0] + jj[u] * nd] = d0[u] - li[0] * uj[u];
-            if (ir[1] >= 0) D[ir[1] + jj[u] * nd] = d1[u] - li[1] * uj[u];
-          }
-        }
-      }
-      lkeep[0] = li[0];
-      lkeep[1] = li[1];
-      pkeep = p;
-      __syncthreads();
-    }
-    if (jl == 0 && pkeep >= 0) {
-#pragma unroll
-      for (int a = 0; a < 2; a++) {
-        const int i = pkeep + 1 + il + WAVE * a;
-        if (i < nd) D[i + pkeep * nd] = lkeep[a];
-      }
-    }
-    __syncthreads();
+    TF *lbuf = D + (((size_t)nd * nd + 3) & ~(size_t)3);
+    bad |= dense_lu<TF>(D, nd, tid, lbuf, lbuf + 256);
     TF *dn = dnv + ((size_t)bsel * Mb + m) * nd * nd;
     for (int i = tid; i < nd * nd; i += nt) dn[i] = D[i];
     GP_TRACE(0, S.n_flev + 1);
+  }
+  if (bad) atomicOr(&status[m], 1);
+}
+
+// ---- LDS-resident factorisation -----------------------------------------
+// The multipliers (L entries of the columns below the dense block) stay in
+// LDS (Lv) once their column is final, so the updates read no global
+// memory.  The schedule is cut into steps; the static data of step s + 1
+// (column records, update program, step bounds, output map) and its initial
+// column values are loaded into registers while step s computes, and stored
+// into the LDS staging area between two barriers.
+
+// One column, a group of g lanes (j: lane within the group).  The column's
+// updates come in rows of g (one step's ops, padded with no-ops on the spare
+// slot x[len]); lane j applies entry j of each row, reading the next row's
+// op word while the current row updates.  Every group of the wavefront runs
+// the same number of rows R (the largest column of the wavefront in this
+// round); rows past the column's own are read clamped and not applied.
+// Then (sparse columns) the pivot and the scaling of L, whose entries also
+// go to their multiplier slots Lv[lbase + t - dl - 1].
+template <typename TF>
+__device__ __forceinline__ int res_group_column(const int *rec, const int2 *ops, TF *x, TF *Lv, bool ext, int j,
+                                                int g) {
+  const int len = rec[1], dl = rec[2], nrow = rec[4], lbase = rec[6];
+  const int R = __builtin_amdgcn_readfirstlane(rec[5]);
+  ops += rec[3] * g + j;
+  // straight-line rows: the operand reads first, then the next row's op
+  // word (its read completes behind the update), and a write in every row
+  // (rows past the column's own write the value back unchanged)
+  int2 w = ops[0];
+  for (int r = 0; r < R; r++) {
+    const int dst = w.x & 0xffff;
+    const TF xs = x[w.x >> 16];
+    const TF d = x[dst];
+    const TF l = Lv[w.y];
+    w = ops[min(r + 1, nrow - 1) * g];
+    const TF v = d - l * xs;
+    x[dst] = r < nrow ? v : d;
+  }
+  if (ext) return 0;
+  const TF piv = x[dl];
+  const TF inv = TF(1) / piv;
+  for (int t = dl + 1 + j; t < len; t += g) {
+    const TF v = x[t] * inv;
+    x[t] = v;
+    Lv[lbase + t - dl - 1] = v;
+  }
+  return (!(piv != TF(0)) || !isfinite((double)piv)) ? 1 : 0;
+}
+
+// Same interface as k_splu_factor; luv holds the step-ordered LU values.
+// Step s: its column values sit in xb[s & 1]; while it computes (LDS only),
+// the blob and initial values of step s + 1 and the output map of step s
+// are loaded into registers; after a barrier they go to the staging area and
+// xb[(s + 1) & 1], and the finished values of step s are written out (solve
+// array, dense copy), so that no wait for those stores ever holds up a step.
+template <typename TF>
+__global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double *__restrict__ var,
+                                                         const double *__restrict__ cst, int ldM,
+                                                         TF *__restrict__ luv, TF *__restrict__ svv,
+                                                         TF *__restrict__ dnv, const uint8_t *__restrict__ refac,
+                                                         const uint8_t *__restrict__ tgt, int *__restrict__ status) {
+  const int m = scenario_of_block(blockIdx.x, gridDim.x);
+  if (!refac[m]) return;
+  extern __shared__ __align__(16) char smem_raw[];
+  const int Mb = gridDim.x;
+  const int bsel = tgt ? tgt[m] : 0;
+  TF *lu = luv + (size_t)m * S.res_xtot;
+  TF *so = svv + ((size_t)bsel * Mb + m) * S.ns;
+  TF *dn = dnv + ((size_t)bsel * Mb + m) * (size_t)S.nd * S.nd;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  // LDS: multipliers (later the dense block), staging blob, two value buffers
+  TF *Lv = reinterpret_cast<TF *>(smem_raw);
+  const size_t lvb = ((size_t)S.res_lv * sizeof(TF) + 15) & ~(size_t)15;
+  int *stg = reinterpret_cast<int *>(smem_raw + lvb);
+  TF *xbuf = reinterpret_cast<TF *>(smem_raw + lvb + (size_t)kResCB * nt * 4);
+  const int xcap = kResCX * nt;
+  for (int p = tid; p < S.res_xtot; p += nt) lu[p] = TF(0);
+  if (tid == 0) Lv[S.res_nl] = TF(0);   // multiplier of the no-op updates
+  __syncthreads();
+  for (int e = tid; e < S.nnz_a; e += nt) {
+    const int src = S.a_src[e];
+    const double v = src >= 0 ? var[(size_t)src * ldM + m] : cst[-src - 1];
+    lu[S.res_a2x[e]] = TF(v);
+  }
+  __syncthreads();
+  // per thread: blob dwords 4 (tid + j nt) .. + 3 (j < kResCB / 4), value
+  // and output-map slots 4 (tid + j nt) .. + 3 (j < kResCX / 4), at clamped
+  // addresses (blobs and value ranges start at multiples of 4 and are
+  // followed by at least 4 readable entries); the LDS stores are not masked
+  // (slots past a step's length are never read)
+  constexpr int PB = kResCB / 4, PX = kResCX / 4;
+  int pb[kResCB];
+  TF px[kResCX];
+  int po[kResCX];
+  const int nst = S.res_nsteps;
+  auto fetch_blob = [&](int st) {
+    const int *h = S.res_hdr + kResHdr * st;
+    const int bo = h[2], bl = h[3], xo = h[4], xl = h[5];
+#pragma unroll
+    for (int j = 0; j < PB; j++) {
+      const int4 v = *reinterpret_cast<const int4 *>(S.res_blob + bo + min(4 * (tid + j * nt), bl - 4));
+      pb[4 * j] = v.x;
+      pb[4 * j + 1] = v.y;
+      pb[4 * j + 2] = v.z;
+      pb[4 * j + 3] = v.w;
+    }
+#pragma unroll
+    for (int j = 0; j < PX; j++) {
+      const TF *q = lu + xo + min(4 * (tid + j * nt), xl - 4);
+#pragma unroll
+      for (int u = 0; u < 4; u++) px[4 * j + u] = q[u];
+    }
+  };
+  auto fetch_omap = [&](int st) {
+    const int *h = S.res_hdr + kResHdr * st;
+    const int xo = h[4], xl = h[5];
+#pragma unroll
+    for (int j = 0; j < PX; j++) {
+      const int4 v = *reinterpret_cast<const int4 *>(S.res_omap + xo + min(4 * (tid + j * nt), xl - 4));
+      po[4 * j] = v.x;
+      po[4 * j + 1] = v.y;
+      po[4 * j + 2] = v.z;
+      po[4 * j + 3] = v.w;
+    }
+  };
+  auto store = [&](int st) {
+    TF *xd = xbuf + (st & 1) * xcap;
+#pragma unroll
+    for (int j = 0; j < PB; j++)
+      *reinterpret_cast<int4 *>(stg + 4 * (tid + j * nt)) =
+          make_int4(pb[4 * j], pb[4 * j + 1], pb[4 * j + 2], pb[4 * j + 3]);
+#pragma unroll
+    for (int j = 0; j < PX; j++)
+#pragma unroll
+      for (int u = 0; u < 4; u++) xd[4 * (tid + j * nt) + u] = px[4 * j + u];
+  };
+  auto flush = [&](int st) {
+    const int xl = S.res_hdr[kResHdr * st + 5];
+    const TF *xs = xbuf + (st & 1) * xcap;
+    // every load of this step has landed by now (the staging stores waited
+    // for the younger ones); waiting here, before the first store, keeps
+    // the output stores from being waited for until the next step
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt 0, expcnt 7, lgkmcnt 15
+#pragma unroll
+    for (int j = 0; j < PX; j++)
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int t = 4 * (tid + j * nt) + u, o = po[4 * j + u];
+        if (t < xl && o != -1) {
+          if (o >= 0)
+            so[o] = xs[t];
+          else
+            dn[-2 - o] = xs[t];
+        }
+      }
+  };
+  fetch_blob(0);
+  store(0);
+  __syncthreads();
+  GP_TRACE(0, 0);
+  int bad = 0;
+  for (int st = 0; st < nst; st++) {
+    const int *h = S.res_hdr + kResHdr * st;
+    const int mode = h[0], nc = h[1];
+    const int2 *ops = reinterpret_cast<const int2 *>(stg + h[6]);
+    TF *xb = xbuf + (st & 1) * xcap;
+    fetch_omap(st);
+    if (st + 1 < nst) fetch_blob(st + 1);
+    {
+      const int g = h[9], G = nt / g;
+      for (int c = tid / g; c < nc; c += G) {
+        const int *rec = stg + 8 * c;
+        bad |= res_group_column<TF>(rec, ops, xb + rec[0], Lv, mode == 2, tid & (g - 1), g);
+      }
+    }
+    lds_barrier();
+    if (st + 1 < nst) store(st + 1);
+    flush(st);
+    lds_barrier();
+    GP_TRACE(0, st + 1);
+  }
+  if (S.nd) {
+    // dense trailing block (over the multipliers, no longer needed): the
+    // block columns' values as the external updates left them in dn
+    __syncthreads();
+    const int nd = S.nd;
+    TF *D = Lv;
+    for (int i = tid; i < nd * nd; i += nt) D[i] = TF(0);
+    __syncthreads();
+    for (int q = tid; q < S.nnz_top; q += nt) {
+      const int d = S.dmap[q];
+      if (d >= 0) D[d] = dn[d];
+    }
+    __syncthreads();
+    TF *lbuf = reinterpret_cast<TF *>(stg);   // the staging area is free now
+    bad |= dense_lu<TF>(D, nd, tid, lbuf, lbuf + 256);
+    for (int i = tid; i < nd * nd; i += nt) dn[i] = D[i];
+    GP_TRACE(0, nst + 1);
   }
   if (bad) atomicOr(&status[m], 1);
 }
@@ -1918,6 +2156,14 @@ void launch_cluster_backsub(const DevTables &T, const TF *X, const double *f, co
 template <typename TF>
 void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv, TF *sv, TF *dn,
                              const uint8_t *refac, const uint8_t *tgt, int M, int *status, hipStream_t s) {
+  if (S.res_nsteps > 0) {
+    if (S.lds_factor > 65536)
+      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_splu_factor_res<TF>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, S.lds_factor);
+    hipLaunchKernelGGL(k_splu_factor_res<TF>, dim3(M), dim3(S.threads), S.lds_factor, s, S, var, cst, ldM, luv, sv,
+                       dn, refac, tgt, status);
+    return;
+  }
   if (S.lds_factor > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_splu_factor<TF>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, S.lds_factor);

@@ -562,4 +562,168 @@ void splu_schedule(SpluAnalysis &A, int threads, int nd_max) {
       if (A.lu_ri[q] >= B) A.lu2s[q] = -1;
 }
 
+bool res_plan(const SpluAnalysis &A, int bcap, int xcap, ResPlan &R) {
+  R = ResPlan{};
+  const int n = A.n, B = n - A.nd;
+  const int qb = A.lu_cp[B];
+  // Lv slots: the L entries of the columns below the block; slot nL is zero
+  std::vector<int> lslot(A.nnz_lu, -1);
+  for (int k = 0; k < B; k++)
+    for (int p = A.lu_dp[k] + 1; p < A.lu_cp[k + 1]; p++) lslot[p] = R.nL++;
+  std::vector<int> xstart(n, -1);
+  const int nlev = (int)A.flp.size() - 1;
+  auto al4 = [](size_t v) { return (v + 3) & ~(size_t)3; };
+  // Rows of a column with g lanes: its ops (in program order) packed into
+  // rows of at most g, an op going to the first row after every op that
+  // writes its source entry, with room left and no other op writing its
+  // destination.  Ops on one destination commute and the sources ascend, so
+  // any such packing gives the sequential result.  At least one row.
+  auto pack_rows = [&](int i2, int g, std::vector<std::vector<int>> *out) {
+    const int *r = &A.crec[8 * i2];
+    const int len = r[2], o0 = r[4], nops = r[5];
+    std::vector<int> ready(len, 0);           // first row that may read entry t
+    std::vector<int> cnt;                     // ops per row
+    std::vector<std::vector<int>> dsts;       // destinations per row
+    if (out) out->clear();
+    for (int o = o0; o < o0 + nops; o++) {
+      const int d = A.op_loc[o] & 0xffff, src = A.op_loc[o] >> 16;
+      int row = ready[src];
+      while (row < (int)cnt.size() &&
+             (cnt[row] >= g || std::find(dsts[row].begin(), dsts[row].end(), d) != dsts[row].end()))
+        row++;
+      if (row == (int)cnt.size()) {
+        cnt.push_back(0);
+        dsts.emplace_back();
+        if (out) out->emplace_back();
+      }
+      cnt[row]++;
+      dsts[row].push_back(d);
+      if (out) (*out)[row].push_back(o);
+      ready[d] = std::max(ready[d], row + 1);
+    }
+    return std::max<long long>((long long)cnt.size(), 1);
+  };
+  auto col_rows = [&](int i2, int g) { return pack_rows(i2, g, nullptr); };
+  auto col_cost = [&](int i2, int g) { return col_rows(i2, g) + (A.crec[8 * i2 + 2] + g - 1) / g + 4; };
+  auto col_blob = [&](int i2, int g) { return 8 + 2 * (size_t)g * col_rows(i2, g); };
+  int xtot = 0;
+  for (int l = 0; l < nlev; l++) {
+    const int mode = A.fac_mode[l];
+    std::vector<int> lev;
+    for (int i = A.flp[l]; i < A.flp[l + 1]; i++) lev.push_back(i);
+    // lanes per column g: the groups of the workgroup take the columns in
+    // rounds, longest first; cost in row latencies, ties to the smaller blob
+    std::stable_sort(lev.begin(), lev.end(), [&](int x, int y) { return col_cost(x, 1) > col_cost(y, 1); });
+    int g = 1;
+    long long cbest = -1;
+    size_t bbest = 0;
+    for (int gg = 1; gg <= 64; gg *= 2) {
+      const int G = 512 / gg;
+      long long t = 0;
+      size_t bl = 0;
+      for (size_t c = 0; c < lev.size(); c += G) {
+        long long mx = 0;
+        for (size_t c2 = c; c2 < std::min(lev.size(), c + G); c2++) mx = std::max(mx, col_cost(lev[c2], gg));
+        t += mx + 8;
+      }
+      for (int i2 : lev) bl += col_blob(i2, gg);
+      if (cbest < 0 || t < cbest || (t == cbest && bl < bbest)) { cbest = t; g = gg; bbest = bl; }
+    }
+    const int G = 512 / g, gw = 64 / g;
+    size_t i = 0;
+    while (i < lev.size()) {
+      // one step: as many of the level's columns as fit
+      std::vector<int> cols;
+      size_t blen = 0;
+      int xlen = 0;
+      for (; i < lev.size(); i++) {
+        const int len = A.crec[8 * lev[i] + 2];
+        const size_t cb = col_blob(lev[i], g);
+        if (al4(blen + cb) > (size_t)bcap || (int)al4(xlen + len + 1) > xcap) {
+          if (cols.empty()) { R = ResPlan{}; return false; }
+          break;
+        }
+        cols.push_back(lev[i]);
+        blen += cb;
+        xlen += len + 1;
+      }
+      const int nc = (int)cols.size();
+      // rows every group of a wavefront runs: the largest among the columns
+      // the wavefront holds in the same round
+      std::vector<int> rows(nc), wrows(nc);
+      for (int c = 0; c < nc; c++) rows[c] = (int)col_rows(cols[c], g);
+      for (int c = 0; c < nc; c++) {
+        const int round = c / G, wave = (c % G) / gw;
+        int mx = 0;
+        for (int c2 = round * G + wave * gw; c2 < std::min(nc, round * G + (wave + 1) * gw); c2++)
+          mx = std::max(mx, rows[c2]);
+        wrows[c] = mx;
+      }
+      const int boff = (int)R.blob.size();
+      std::vector<int> recs, ops;
+      int xo = 0;
+      for (int c = 0; c < nc; c++) {
+        const int *r = &A.crec[8 * cols[c]];
+        const int k = r[0], c0 = r[1], len = r[2], dl = r[3];
+        xstart[k] = xtot + xo;
+        const int lbase = (mode != 2 && dl + 1 < len) ? lslot[c0 + dl + 1] : 0;
+        recs.insert(recs.end(), {xo, len, dl, (int)ops.size() / (2 * g), rows[c], wrows[c], lbase, 0});
+        std::vector<std::vector<int>> prow;
+        pack_rows(cols[c], g, &prow);
+        int nrow = (int)prow.size();
+        for (const auto &row : prow)
+          for (int jj = 0; jj < g; jj++) {
+            if (jj < (int)row.size()) {
+              const int o = row[jj];
+              const int li = lslot[A.op_sl[o]];
+              if (li < 0) { R = ResPlan{}; return false; }   // multiplier outside the kept L entries
+              ops.insert(ops.end(), {A.op_loc[o], li});
+            } else {
+              ops.insert(ops.end(), {len, R.nL});   // no-op on the spare slot
+            }
+          }
+        if (nrow == 0) {   // a column without updates still has one (no-op) row
+          for (int jj = 0; jj < g; jj++) ops.insert(ops.end(), {len, R.nL});
+          nrow = 1;
+        }
+        if (nrow != rows[c]) { R = ResPlan{}; return false; }
+        for (int t = 0; t < len; t++) {
+          const int p = c0 + t;
+          if (mode != 2 && t > dl && lslot[p] != lbase + (t - dl - 1)) { R = ResPlan{}; return false; }
+          int o = A.lu2s[p];
+          if (mode == 2 && t >= dl) o = A.dmap[p - qb] >= 0 ? -2 - A.dmap[p - qb] : -1;
+          R.omap.push_back(o);
+        }
+        R.omap.push_back(-1);   // spare slot
+        xo += len + 1;
+      }
+      const int oo = 8 * nc;
+      R.blob.insert(R.blob.end(), recs.begin(), recs.end());
+      R.blob.insert(R.blob.end(), ops.begin(), ops.end());
+      R.blob.resize(al4(R.blob.size()), 0);
+      const int blen2 = (int)R.blob.size() - boff;
+      const int xl = (int)al4(xo);
+      R.omap.resize(xtot + xl, -1);
+      R.hdr.insert(R.hdr.end(), {mode, nc, boff, blen2, xtot, xl, oo, 0, 0, g, 0, 0});
+      R.maxb = std::max(R.maxb, blen2);
+      R.maxx = std::max(R.maxx, xl);
+      xtot += xl;
+    }
+  }
+  R.blob.resize(R.blob.size() + 4, 0);   // vector loads of the last step may read past it
+  R.xtot = xtot + 4;
+  R.omap.resize(R.xtot, -1);
+  // A entry -> step-ordered position
+  std::vector<int> colof(A.nnz_lu);
+  for (int k = 0; k < n; k++)
+    for (int p = A.lu_cp[k]; p < A.lu_cp[k + 1]; p++) colof[p] = k;
+  R.a2x.resize(A.nnz_a);
+  for (int e = 0; e < A.nnz_a; e++) {
+    const int p = A.a2lu[e], k = colof[p];
+    if (xstart[k] < 0) { R = ResPlan{}; return false; }
+    R.a2x[e] = xstart[k] + (p - A.lu_cp[k]);
+  }
+  return true;
+}
+
 }  // namespace gp

@@ -100,6 +100,40 @@ void splu_schedule(SpluAnalysis &A, int threads, int nd_max = 112);
 std::string splu_analyse(int n, const int *cp, const int *ri, const double *ax,
                          SpluAnalysis &out, double pivot_threshold = 1e-3);
 
+// Plan of the LDS-resident factorisation.  Every L entry of the columns
+// below the dense block (B = n - nd) that an update reads is kept in LDS
+// (Lv, nL values, slot nL holding zero) once its column is final, so an
+// update reads only LDS (a column's L entries take consecutive slots from
+// its lbase).  The levels of the schedule are cut into steps whose static
+// data (a blob of dwords) and initial column values fit the staging
+// capacities; each column has len + 1 value slots (the last one a spare that
+// padding updates write), stored contiguously per step in a step-ordered
+// copy of the LU values (a2x: A entry -> position, xtot values).  A level
+// runs g lanes per column (a power of two): the updates of one step (one
+// source entry, distinct destinations) are laid out in rows of g, padded
+// with no-op updates (spare slot, multiplier nL), and a group of g lanes
+// applies one row at a time.  Blob of a step, in dwords:
+//   8 per column: x offset, len, dl, first row, rows, rows of its
+//                 wavefront in this round (the loop bound), lbase, 0
+//   2 per op:     op_loc, Lv index of the multiplier (rows of g)
+// omap, per value slot: its solve slot (lu2s), -2 - d for entry d of the
+// dense block, or -1.  Step blob and x range start at multiples of 4.
+// Step header (kResHdr ints): mode, ncols, blob offset, blob length, x
+// offset, x length, ops offset in the blob, 0, 0, g.
+constexpr int kResHdr = 12;
+struct ResPlan {
+  int nL = 0;
+  int maxb = 0, maxx = 0;          // largest step blob (dwords) / x length
+  int xtot = 0;                    // step-ordered values per scenario
+  std::vector<int> a2x;            // [nnz_a]
+  std::vector<int> omap;           // [xtot]
+  std::vector<int> hdr;            // kResHdr per step
+  std::vector<int> blob;
+  int nsteps() const { return (int)hdr.size() / kResHdr; }
+};
+// Returns false (and leaves R empty) if a column alone exceeds a capacity.
+bool res_plan(const SpluAnalysis &A, int bcap, int xcap, ResPlan &R);
+
 // Minimum-degree order of the symmetric pattern of A + A^T.
 std::vector<int> min_degree_order(int n, const int *cp, const int *ri);
 
