@@ -73,7 +73,7 @@ struct LUPlan {
   SpluAnalysis A;
   SpluDev dev{};
   DBuf<int> a2lu, a_src, op_loc, op_sl, st_o, flp, fac_mode, crec, lu2s, dmap, fw_lvl, fw_chunk, fw_cols, fw_rows,
-      bw_lvl, bw_chunk, bw_cols, bw_rows, q, pinv, r_hdr, r_blob, r_a2x, r_omap;
+      bw_lvl, bw_chunk, bw_cols, bw_rows, q, pinv, r_hdr, r_blob, r_a2x, r_omap, sf_hdr, sf_blob, sb_hdr, sb_blob;
   ResPlan rp;                  // LDS-resident factorisation plan
   bool rp_ok = false;
   // LU values per scenario the factorisation kernels use
@@ -85,13 +85,8 @@ struct LUPlan {
   static constexpr int kLdsMax = 160 * 1024;      // LDS of one CU
 
   static std::vector<int> nonempty(const std::vector<int> &v) { return v.empty() ? std::vector<int>{0} : v; }
-  // LDS bytes of the solve before its dense block: x, then the level and
-  // chunk tables, each section 16-byte aligned (the kernel's layout)
-  int table_bytes() const {
-    auto al = [](size_t b) { return (b + 15) / 16 * 16; };
-    return (int)(al((size_t)A.n * 8) + al(4 * A.sf.lvl.size()) + al(4 * A.sf.chunk.size()) + al(4 * A.sb.lvl.size()) +
-                 al(4 * A.sb.chunk.size()));
-  }
+  // LDS bytes of the solve's x (16-byte aligned)
+  int x_bytes() const { return (int)(((size_t)A.n * 8 + 15) / 16 * 16); }
 
   // Schedules and uploads the analysis.  The solve keeps x (8 bytes per row),
   // its level tables and the dense block in LDS, which bounds the block.
@@ -100,7 +95,7 @@ struct LUPlan {
     int nd_max = 0;
     while ((long long)(nd_max + 16) * (nd_max + 16) * 8 <= room) nd_max += 16;
     splu_schedule(A, kThreads, nd_max);
-    if (table_bytes() + (long long)A.nd * A.nd * 8 > kLdsMax)
+    if (x_bytes() + (long long)A.nd * A.nd * 8 > kLdsMax)
       throw GpError{GP_ERR_UNSUPPORTED, "system too large for the in-LDS triangular solve (" +
                                             std::to_string(A.n) + " rows)"};
     a2lu.upload(A.a2lu); a_src.upload(asrc);
@@ -115,6 +110,22 @@ struct LUPlan {
     bw_lvl.upload(nonempty(A.sb.lvl)); bw_chunk.upload(nonempty(A.sb.chunk));
     bw_cols.upload(nonempty(A.sb.cols)); bw_rows.upload(nonempty(A.sb.rows));
     q.upload(A.q); pinv.upload(A.pinv);
+    {
+      SolvePlan pf, pb;
+      if (!solve_plan(A.sf, false, 0, kSolCB * kThreads, kSolCV * kThreads, pf) ||
+          !solve_plan(A.sb, true, 0, kSolCB * kThreads, kSolCV * kThreads, pb))
+        throw GpError{GP_ERR_UNSUPPORTED, "triangular-solve staging plan failed"};
+      sf_hdr.upload(nonempty(pf.hdr));
+      sf_blob.upload(nonempty(pf.blob));
+      sb_hdr.upload(nonempty(pb.hdr));
+      sb_blob.upload(nonempty(pb.blob));
+      dev.sol_nfw = pf.nsteps();
+      dev.sol_nbw = pb.nsteps();
+      dev.sol_fw_hdr = sf_hdr.p;
+      dev.sol_fw_blob = sf_blob.p;
+      dev.sol_bw_hdr = sb_hdr.p;
+      dev.sol_bw_blob = sb_blob.p;
+    }
     rp_ok = res_plan(A, kResCB * kThreads, kResCX * kThreads, rp);
     if (rp_ok) {
       r_hdr.upload(rp.hdr);
@@ -153,7 +164,13 @@ struct LUPlan {
     dev.w_bytes = wbytes(wc, wo);
     dev.lds_factor = std::max<int>(std::max<int>((int)((size_t)kThreads * tc * elem), W * dev.w_bytes),
                                    (int)((((size_t)A.nd * A.nd + 3) / 4 * 4 + 512) * elem));   // dense block, buffers
-    dev.lds_solve = table_bytes() + (int)((size_t)A.nd * A.nd * elem);
+    // solve: x, then the staging area, which the dense block reuses
+    dev.lds_solve = x_bytes() + (int)std::max<size_t>((size_t)kSolCB * kThreads * 4 +
+                                                          (size_t)(kSolCV + kSolCD) * kThreads * elem,
+                                                      (size_t)A.nd * A.nd * elem);
+    if (dev.lds_solve > kLdsMax)
+      throw GpError{GP_ERR_UNSUPPORTED, "system too large for the in-LDS triangular solve (" +
+                                            std::to_string(A.n) + " rows)"};
     // LDS-resident factorisation when its multipliers (or the dense block),
     // staging blob and column values fit
     dev.res_nsteps = 0;

@@ -78,11 +78,16 @@ struct SpluDev {
   // entry -> step-ordered position, LDS elements of the multiplier / dense
   // block region
   int res_nsteps, res_lv, res_nl, res_xtot;
+  // staged triangular solve: steps per side, headers and blobs
+  int sol_nfw, sol_nbw;
+  const int *sol_fw_hdr, *sol_fw_blob, *sol_bw_hdr, *sol_bw_blob;
   const int *res_hdr, *res_blob, *res_a2x, *res_omap;
 };
 // staging of the LDS-resident factorisation, per thread of a 512-thread
 // workgroup: blob dwords and initial column values
 constexpr int kResCB = 24, kResCX = 8;
+// staging of the triangular solve, per thread: blob dwords, values, diagonals
+constexpr int kSolCB = 8, kSolCV = 8, kSolCD = 4;
 
 // ---- kernel launchers (gp_kernels.hip) ----------------------------------
 void launch_cluster_residual(const DevTables &T, const double *p, const double *z, const double *zold,

@@ -1809,18 +1809,6 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
   if (bad) atomicOr(&status[m], 1);
 }
 
-// One level of a sliced-ELL triangular solve for the lanes u = tid, tid+nt, ...
-// The registers prefetched for the first lane hold its first GP_PF entries
-// and the level's shape, so the level itself reads no table.
-#define GP_PF 4
-template <typename TF>
-struct SolvePre {
-  TF v[GP_PF];
-  int c[GP_PF];
-  int row, pf, base, T, g, R, cb;
-  TF dg;
-};
-
 // DPP lane exchange of a double within a row of 16 lanes
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v) {
@@ -1844,91 +1832,121 @@ __device__ __forceinline__ double group_sum(double acc, int g) {
   return acc;
 }
 
-// Branch-free so that nothing waits for these loads before they are used:
-// lanes past the level and entries past a chunk's pf read a valid slot of the
-// same chunk (every chunk has pf >= 1), masked out where they are used.
-template <typename TF, bool BW>
-__device__ __forceinline__ void solve_fetch(const int *__restrict__ lt, const int *__restrict__ ct, int l,
-                                            const TF *__restrict__ vals, const int *__restrict__ cols,
-                                            const int *__restrict__ rows, const TF *__restrict__ dg, int tid,
-                                            SolvePre<TF> &P) {
-  const int4 lv = reinterpret_cast<const int4 *>(lt)[l];
-  const int T = lv.x;
-  const int u = min(tid, T - 1);
-  const int c = lv.w + (u >> 6);
-  const int2 ch = reinterpret_cast<const int2 *>(ct)[c];
-  const int base = ch.x + (u & 63);
+// ---- staged triangular solve ----------------------------------------------
+// The levels of each side are grouped into steps (host plan); the static
+// data of step s + 1 (level and chunk tables, value columns, lane rows) and
+// its values (and diagonals) are loaded into registers while step s
+// computes from LDS, and stored into the staging area between two barriers.
+// Per thread: kSolCB blob dwords, kSolCV values, kSolCD diagonals.
+struct SolStage {
+  int b[kSolCB];
+};
+
+// Loads of step st of a side into registers (clamped, always valid).
+template <typename TF>
+__device__ __forceinline__ void sol_fetch(const int *__restrict__ hdr, const int *__restrict__ blob, int st,
+                                          const TF *__restrict__ vals, const TF *__restrict__ dg, int tid, int nt,
+                                          SolStage &P, TF (&pv)[kSolCV], TF (&pd)[kSolCD]) {
+  const int *h = hdr + kSolHdr * st;
+  const int bo = h[1], bl = h[2], vo = h[3], vl = h[4], dgo = h[5], dl = h[6];
 #pragma unroll
-  for (int j = 0; j < GP_PF; j++) {
-    const int at = base + WAVE * min(j, ch.y - 1);
-    P.v[j] = vals[at];
-    P.c[j] = cols[at];
+  for (int j = 0; j < kSolCB / 4; j++) {
+    const int4 v = *reinterpret_cast<const int4 *>(blob + bo + min(4 * (tid + j * nt), bl - 4));
+    P.b[4 * j] = v.x;
+    P.b[4 * j + 1] = v.y;
+    P.b[4 * j + 2] = v.z;
+    P.b[4 * j + 3] = v.w;
   }
-  P.T = T;
-  P.g = lv.y;
-  P.R = lv.z;
-  P.cb = lv.w;
-  P.pf = ch.y;
-  P.base = base;
-  P.row = rows[lv.z + u];
-  if (BW) P.dg = dg[lv.z + u];
+  const int vend = h[10];   // values actually present (the rest of vl is padding)
+#pragma unroll
+  for (int j = 0; j < kSolCV; j++) pv[j] = vals[vo + min(tid + j * nt, max(vend - 1, 0))];
+  const int dend = h[11];
+#pragma unroll
+  for (int j = 0; j < kSolCD; j++) pd[j] = dl ? dg[dgo + min(tid + j * nt, max(dend - 1, 0))] : TF(0);
+  (void)vl;
 }
 
-template <typename TF, bool BW>
-__device__ __forceinline__ void solve_level(const int *__restrict__ ct, const TF *__restrict__ vals,
-                                            const int *__restrict__ cols, const int *__restrict__ rows,
-                                            const TF *__restrict__ dg, double *x, int tid, int nt,
-                                            const SolvePre<TF> &P) {
-  const int T = P.T, g = P.g;
-  const int sub = tid & (g - 1);
-  // first lane: prefetched
-  {
-    double acc = 0.0;
-    if (tid < T) {
+template <typename TF>
+__device__ __forceinline__ void sol_store(int *sb, TF *svl, TF *sdg, int tid, int nt, const SolStage &P,
+                                          const TF (&pv)[kSolCV], const TF (&pd)[kSolCD]) {
 #pragma unroll
-      for (int j = 0; j < GP_PF; j++)
-        if (j < P.pf) acc += (double)P.v[j] * x[P.c[j]];
-      for (int j = GP_PF; j < P.pf; j++) {
-        const int at = P.base + WAVE * j;
-        acc += (double)vals[at] * x[cols[at]];
+  for (int j = 0; j < kSolCB / 4; j++)
+    *reinterpret_cast<int4 *>(sb + 4 * (tid + j * nt)) = make_int4(P.b[4 * j], P.b[4 * j + 1], P.b[4 * j + 2],
+                                                                   P.b[4 * j + 3]);
+#pragma unroll
+  for (int j = 0; j < kSolCV; j++) svl[tid + j * nt] = pv[j];
+#pragma unroll
+  for (int j = 0; j < kSolCD; j++) sdg[tid + j * nt] = pd[j];
+}
+
+// The passes of one staged step.  A pass record gives each wavefront one
+// chunk:
+// lane i (i < nl) sums vals[E + i + 64 j] * x[col] over j < pf, the g lanes
+// of a row add up, and the row's first lane updates x[row] (backward: and
+// divides by the diagonal).  The row and x[row] are read together with the
+// entries.  A barrier follows the last pass of each level.
+template <typename TF, bool BW>
+__device__ __forceinline__ void sol_step(const int *h, const int *sb, const TF *svl,
+                                         const TF *sdg, double *x, int tid) {
+  const int np = h[0];
+  const int *cols = sb + h[8];
+  const int *rows = sb + h[9];
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  // pass records from the staged blob (LDS: in-order, unlike scalar loads,
+  // so waiting for them never waits for anything else); the next pass's
+  // records are read while the current pass runs
+  const int4 *L = reinterpret_cast<const int4 *>(sb);
+  int4 hd = L[0], rc = L[1 + w];
+  for (int k = 0; k < np; k++) {
+    const int4 hdc = hd, rcc = rc;
+    if (k + 1 < np) {
+      hd = L[9 * (k + 1)];
+      rc = L[9 * (k + 1) + 1 + w];
+    }
+    const int g = __builtin_amdgcn_readfirstlane(hdc.x), E = __builtin_amdgcn_readfirstlane(rcc.x);
+    const int pf = __builtin_amdgcn_readfirstlane(rcc.y), rs = __builtin_amdgcn_readfirstlane(rcc.z);
+    const int nl = __builtin_amdgcn_readfirstlane(rcc.w);
+    if (nl > 0) {
+      const int uc = min(lane, nl - 1);
+      const int base = E + uc;
+      const int row = rows[rs + uc];
+      double acc = 0.0;
+      double xr = 0.0;
+      for (int j0 = 0; j0 < pf; j0 += 4) {
+        int cc[4];
+        TF vv[4];
+        double xv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) cc[q] = cols[base + WAVE * min(j0 + q, pf - 1)];
+#pragma unroll
+        for (int q = 0; q < 4; q++) vv[q] = svl[base + WAVE * min(j0 + q, pf - 1)];
+#pragma unroll
+        for (int q = 0; q < 4; q++) xv[q] = x[cc[q]];
+        if (j0 == 0) xr = x[row];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const double t = (double)vv[q] * xv[q];
+          acc += j0 + q < pf ? t : 0.0;
+        }
+      }
+      acc = group_sum(acc, g);
+      if (lane < nl && (lane & (g - 1)) == 0) {
+        if (BW)
+          x[row] = (xr - acc) / (double)sdg[rs + uc];
+        else
+          x[row] = xr - acc;
       }
     }
-    acc = group_sum(acc, g);
-    if (tid < T && sub == 0) {
-      if (BW)
-        x[P.row] = (x[P.row] - acc) / (double)P.dg;
-      else
-        x[P.row] -= acc;
-    }
-  }
-  // further lanes (levels wider than the workgroup)
-  for (int u = tid + nt; u < ((T + nt - 1) / nt) * nt; u += nt) {
-    double acc = 0.0;
-    if (u < T) {
-      const int c = P.cb + (u >> 6);
-      const int E = ct[2 * c], pf = ct[2 * c + 1];
-      for (int j = 0; j < pf; j++) {
-        const int at = E + (u & 63) + WAVE * j;
-        acc += (double)vals[at] * x[cols[at]];
-      }
-    }
-    acc = group_sum(acc, g);
-    if (u < T && sub == 0) {
-      const int row = rows[P.R + u];
-      if (BW)
-        x[row] = (x[row] - acc) / (double)dg[P.R + u];
-      else
-        x[row] -= acc;
-    }
+    if (__builtin_amdgcn_readfirstlane(hdc.y)) lds_barrier();
   }
 }
 
 // One workgroup per scenario: x = A^{-1} b with b gathered from rhs (row
-// space), the solution kept in LDS: forward levels of the sparse part and of
+// space), the solution kept in LDS: forward steps over the sparse part and
 // the block rows' external entries, the dense block's forward and backward
-// substitution (one wavefront, x of the block in registers), then the sparse
-// backward levels; then zupd[scatter[j]] += sign * x[j] and xout[j] = x[j].
-// Each level's loads are issued while the previous level computes.
+// substitution (one wavefront, x of the block in registers), then the
+// backward steps; then zupd[scatter[j]] += sign * x[j] and xout[j] = x[j].
+// svv/dnv hold two buffers per scenario; cur[m] selects the one read (NULL: 0).
 template <typename TF>
 __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restrict__ svv, const TF *__restrict__ dnv,
                                                     const double *__restrict__ rhs, int ldM, double sign,
@@ -1941,63 +1959,56 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   const int n = S.n, nd = S.nd, B = n - nd;
   const int Mb = gridDim.x;
   const int bsel = cur ? cur[m] : 0;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  // LDS: x, then the staging area (blob, values, diagonals), which the dense
+  // block reuses between the two sides
   double *x = reinterpret_cast<double *>(smem_raw);
-  // LDS: x, then the level tables (16-byte aligned for their vector reads),
-  // then the dense block
-  int *flt = reinterpret_cast<int *>(smem_raw + (((size_t)n * 8 + 15) & ~(size_t)15));
-  int *fct = flt + 4 * S.n_fwlev;
-  int *blt = fct + ((2 * S.n_fwch + 3) & ~3);
-  int *bct = blt + 4 * S.n_bwlev;
-  TF *Dl = reinterpret_cast<TF *>(reinterpret_cast<char *>(bct) + (((size_t)8 * S.n_bwch + 15) & ~(size_t)15));
+  char *area = smem_raw + (((size_t)n * 8 + 15) & ~(size_t)15);
+  int *sb = reinterpret_cast<int *>(area);
+  TF *svl = reinterpret_cast<TF *>(area + (size_t)kSolCB * nt * 4);
+  TF *sdg = svl + (size_t)kSolCV * nt;
+  TF *Dl = reinterpret_cast<TF *>(area);
   const TF *fv = svv + ((size_t)bsel * Mb + m) * S.ns;
   const TF *bv = fv + S.fw_nvals;
   const TF *bd = bv + S.bw_nvals;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int i = tid; i < 4 * S.n_fwlev; i += nt) flt[i] = S.fw_lvl[i];
-  for (int i = tid; i < 2 * S.n_fwch; i += nt) fct[i] = S.fw_chunk[i];
-  for (int i = tid; i < 4 * S.n_bwlev; i += nt) blt[i] = S.bw_lvl[i];
-  for (int i = tid; i < 2 * S.n_bwch; i += nt) bct[i] = S.bw_chunk[i];
-  for (int r = tid; r < n; r += nt) x[S.pinv[r]] = rhs[(size_t)r * ldM + m];
+  const int nfs = S.sol_nfw, nbs = S.sol_nbw;
+  SolStage P;
+  TF pv[kSolCV], pd[kSolCD];
+  if (nfs) sol_fetch<TF>(S.sol_fw_hdr, S.sol_fw_blob, 0, fv, bd, tid, nt, P, pv, pd);
+  {
+    // gather b, four rows per thread in flight
+    int r = tid;
+    for (; r + 3 * nt < n; r += 4 * nt) {
+      double b4[4];
+      int p4[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        b4[u] = rhs[(size_t)(r + u * nt) * ldM + m];
+        p4[u] = S.pinv[r + u * nt];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) x[p4[u]] = b4[u];
+    }
+    for (; r < n; r += nt) x[S.pinv[r]] = rhs[(size_t)r * ldM + m];
+  }
+  if (nfs) sol_store<TF>(sb, svl, sdg, tid, nt, P, pv, pd);
+  __syncthreads();
+  GP_TRACE(1, 0);
+  for (int st = 0; st < nfs; st++) {
+    if (st + 1 < nfs) sol_fetch<TF>(S.sol_fw_hdr, S.sol_fw_blob, st + 1, fv, bd, tid, nt, P, pv, pd);
+    sol_step<TF, false>(S.sol_fw_hdr + kSolHdr * st, sb, svl, sdg, x, tid);
+    if (st + 1 < nfs) {
+      sol_store<TF>(sb, svl, sdg, tid, nt, P, pv, pd);
+      lds_barrier();
+    }
+    GP_TRACE(1, st + 1);
+  }
+  if (nbs) sol_fetch<TF>(S.sol_bw_hdr, S.sol_bw_blob, 0, bv, bd, tid, nt, P, pv, pd);
   if (nd) {
     const TF *dn = dnv + ((size_t)bsel * Mb + m) * nd * nd;
     for (int i = tid; i < nd * nd; i += nt) Dl[i] = dn[i];
+    __syncthreads();
   }
-  __syncthreads();
-  GP_TRACE(1, 0);
-  // three register sets in rotation: the loads for level l + 2 are issued
-  // while level l computes, with no copy between the sets
-  SolvePre<TF> pa, pb, pc;
-  const int nf = S.n_fwlev, nb = S.n_bwlev;
-#define GP_FW_FETCH(L, P) solve_fetch<TF, false>(flt, fct, L, fv, S.fw_cols, S.fw_rows, bd, tid, P)
-#define GP_FW_LEVEL(L, P)                                                      \
-  do {                                                                         \
-    solve_level<TF, false>(fct, fv, S.fw_cols, S.fw_rows, bd, x, tid, nt, P);          \
-    lds_barrier();                                                             \
-    GP_TRACE(1, (L) + 1);                                                      \
-  } while (0)
-#define GP_BW_FETCH(L, P) solve_fetch<TF, true>(blt, bct, L, bv, S.bw_cols, S.bw_rows, bd, tid, P)
-#define GP_BW_LEVEL(L, P)                                                      \
-  do {                                                                         \
-    solve_level<TF, true>(bct, bv, S.bw_cols, S.bw_rows, bd, x, tid, nt, P);           \
-    lds_barrier();                                                             \
-    GP_TRACE(1, nf + 2 + (L));                                                 \
-  } while (0)
-  if (nf > 0) GP_FW_FETCH(0, pa);
-  if (nf > 1) GP_FW_FETCH(1, pb);
-  for (int l = 0; l < nf; l += 3) {
-    if (l + 2 < nf) GP_FW_FETCH(l + 2, pc);
-    GP_FW_LEVEL(l, pa);
-    if (l + 1 < nf) {
-      if (l + 3 < nf) GP_FW_FETCH(l + 3, pa);
-      GP_FW_LEVEL(l + 1, pb);
-    }
-    if (l + 2 < nf) {
-      if (l + 4 < nf) GP_FW_FETCH(l + 4, pb);
-      GP_FW_LEVEL(l + 2, pc);
-    }
-  }
-  if (nb > 0) GP_BW_FETCH(0, pa);
-  if (nb > 1) GP_BW_FETCH(1, pb);
   if (nd && tid < WAVE) {
     // dense block, one wavefront: lane i holds x[B + i] and x[B + 64 + i];
     // the block's values for 8 pivots are read ahead of their use
@@ -2022,19 +2033,19 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
       }
     }
     for (int p0 = nd - 1; p0 >= 0; p0 -= 8) {
-      TF a0[8], a1[8], dg[8];
+      TF a0[8], a1[8], dgv[8];
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int p = max(p0 - u, 0);
         a0[u] = Dl[min(i0, nd - 1) + p * nd];
         a1[u] = Dl[min(i1, nd - 1) + p * nd];
-        dg[u] = Dl[p + p * nd];
+        dgv[u] = Dl[p + p * nd];
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int p = p0 - u;
         if (p >= 0) {
-          const double xp = (p < WAVE ? readlane_d(x0, p) : readlane_d(x1, p - WAVE)) / (double)dg[u];
+          const double xp = (p < WAVE ? readlane_d(x0, p) : readlane_d(x1, p - WAVE)) / (double)dgv[u];
           if (i0 == p) x0 = xp;
           if (i1 == p) x1 = xp;
           if (i0 < p) x0 -= (double)a0[u] * xp;
@@ -2045,24 +2056,21 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
     if (i0 < nd) x[B + i0] = x0;
     if (i1 < nd) x[B + i1] = x1;
   }
-  lds_barrier();
-  GP_TRACE(1, nf + 1);
-  for (int l = 0; l < nb; l += 3) {
-    if (l + 2 < nb) GP_BW_FETCH(l + 2, pc);
-    GP_BW_LEVEL(l, pa);
-    if (l + 1 < nb) {
-      if (l + 3 < nb) GP_BW_FETCH(l + 3, pa);
-      GP_BW_LEVEL(l + 1, pb);
-    }
-    if (l + 2 < nb) {
-      if (l + 4 < nb) GP_BW_FETCH(l + 4, pb);
-      GP_BW_LEVEL(l + 2, pc);
-    }
+  __syncthreads();
+  GP_TRACE(1, nfs + 1);
+  if (nbs) {
+    sol_store<TF>(sb, svl, sdg, tid, nt, P, pv, pd);
+    __syncthreads();
   }
-#undef GP_FW_FETCH
-#undef GP_FW_LEVEL
-#undef GP_BW_FETCH
-#undef GP_BW_LEVEL
+  for (int st = 0; st < nbs; st++) {
+    if (st + 1 < nbs) sol_fetch<TF>(S.sol_bw_hdr, S.sol_bw_blob, st + 1, bv, bd, tid, nt, P, pv, pd);
+    sol_step<TF, true>(S.sol_bw_hdr + kSolHdr * st, sb, svl, sdg, x, tid);
+    if (st + 1 < nbs) {
+      sol_store<TF>(sb, svl, sdg, tid, nt, P, pv, pd);
+      lds_barrier();
+    }
+    GP_TRACE(1, nfs + 2 + st);
+  }
   for (int k = tid; k < n; k += nt) {
     const int j = S.q[k];
     const double v = x[k];

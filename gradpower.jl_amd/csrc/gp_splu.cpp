@@ -726,4 +726,105 @@ bool res_plan(const SpluAnalysis &A, int bcap, int xcap, ResPlan &R) {
   return true;
 }
 
+bool solve_plan(const SpluAnalysis::SolveSide &S, bool diag, int dg_base, int bcap, int vcap, SolvePlan &P) {
+  P = SolvePlan{};
+  const int nlev = (int)S.lvl.size() / 4;
+  auto al4 = [](int v) { return (v + 3) & ~3; };
+  // (sub-)levels as chunk ranges [c0, c1) of a level
+  struct Sub { int l, c0, c1; };
+  std::vector<Sub> subs;
+  auto chunk_vals = [&](int c) { return 64 * S.chunk[2 * c + 1]; };
+  for (int l = 0; l < nlev; l++) {
+    const int T = S.lvl[4 * l], cb = S.lvl[4 * l + 3], nch = (T + 63) / 64;
+    int c = cb;
+    while (c < cb + nch) {
+      int c1 = c, v = 0;
+      // whole chunks while the values, columns and rows fit one step
+      while (c1 < cb + nch && v + chunk_vals(c1) + (diag ? 64 * (c1 - c + 1) : 0) <= vcap - 8 &&
+             36 * ((c1 - c + 8) / 8 + 1) + v + chunk_vals(c1) + 64 * (c1 - c + 1) <= bcap - 16) {
+        v += chunk_vals(c1);
+        c1++;
+      }
+      if (c1 == c) return false;   // one chunk does not fit
+      subs.push_back({l, c, c1});
+      c = c1;
+    }
+  }
+  // steps: consecutive sub-levels while their data fit
+  size_t i = 0;
+  while (i < subs.size()) {
+    size_t j = i;
+    int nch = 0, nv = 0, nl = 0;
+    while (j < subs.size()) {
+      const int ch = subs[j].c1 - subs[j].c0;
+      int v = 0;
+      for (int c = subs[j].c0; c < subs[j].c1; c++) v += chunk_vals(c);
+      const int lanes = 64 * (nch + ch);   // rows: at most 64 per chunk
+      const int blen = 36 * (nl + (nch + ch + 7) / 8 + 1) + al4(nv + v) + al4(lanes);
+      const int vlen = al4(nv + v) + (diag ? al4(lanes) : 0);
+      if (blen > bcap || vlen > vcap) break;
+      nch += ch;
+      nv += v;
+      nl++;
+      j++;
+    }
+    if (j == i) return false;
+    // the step's ranges
+    const int cfirst = subs[i].c0, clast = subs[j - 1].c1;   // chunks [cfirst, clast)
+    const int E0 = S.chunk[2 * cfirst];
+    const int E1 = E0 + nv;
+    auto lane_of = [&](const Sub &sb, int c) {   // first lane of chunk c of a sub-level
+      const int l = sb.l;
+      return S.lvl[4 * l + 2] + 64 * (c - S.lvl[4 * l + 3]);
+    };
+    const int R0 = lane_of(subs[i], cfirst);
+    const Sub &sl = subs[j - 1];
+    const int Tl = S.lvl[4 * sl.l];
+    const int R1 = std::min(lane_of(sl, sl.c1 - 1) + 64, S.lvl[4 * sl.l + 2] + Tl);
+    const int boff = (int)P.blob.size();
+    std::vector<int> lv, cols, rows;
+    int nlp = 0;
+    for (size_t k = i; k < j; k++) {
+      const Sub &sb = subs[k];
+      const int l = sb.l, T = S.lvl[4 * l], g = S.lvl[4 * l + 1], cb = S.lvl[4 * l + 3];
+      // passes of 8 chunks (one per wavefront); a barrier after the last
+      for (int c = sb.c0; c < sb.c1; c += 8, nlp++) {
+        lv.insert(lv.end(), {g, c + 8 >= sb.c1 ? 1 : 0, 0, 0});
+        for (int w = 0; w < 8; w++) {
+          const int cw = c + w;
+          if (cw < sb.c1) {
+            const int u0 = 64 * (cw - cb);   // first lane of the chunk in its level
+            lv.insert(lv.end(), {S.chunk[2 * cw] - E0, S.chunk[2 * cw + 1], S.lvl[4 * l + 2] + u0 - R0,
+                                 std::min(64, T - u0)});
+          } else {
+            lv.insert(lv.end(), {0, 0, 0, 0});
+          }
+        }
+      }
+    }
+    if (S.chunk[2 * (clast - 1)] + chunk_vals(clast - 1) != E1) return false;   // values contiguous
+    cols.assign(S.cols.begin() + E0, S.cols.begin() + E1);
+    rows.assign(S.rows.begin() + R0, S.rows.begin() + R1);
+    auto put = [&](std::vector<int> &v) {
+      v.resize(al4((int)v.size()), 0);
+      P.blob.insert(P.blob.end(), v.begin(), v.end());
+    };
+    put(lv);
+    const int o_ch = 0;
+    const int o_cols = (int)P.blob.size() - boff;
+    put(cols);
+    const int o_rows = (int)P.blob.size() - boff;
+    put(rows);
+    const int blen = (int)P.blob.size() - boff;
+    const int vlen = al4(E1 - E0), dlen = diag ? al4(R1 - R0) : 0;
+    P.hdr.insert(P.hdr.end(), {nlp, boff, blen, E0, vlen, dg_base + R0, dlen, o_ch, o_cols, o_rows, E1 - E0,
+                               diag ? R1 - R0 : 0});
+    P.maxb = std::max(P.maxb, blen);
+    P.maxv = std::max(P.maxv, vlen + dlen);
+    i = j;
+  }
+  P.blob.resize(P.blob.size() + 4, 0);
+  return true;
+}
+
 }  // namespace gp

@@ -131,6 +131,28 @@ struct ResPlan {
   std::vector<int> blob;
   int nsteps() const { return (int)hdr.size() / kResHdr; }
 };
+// Plan of the staged triangular solve (one side).  Consecutive levels are
+// grouped into steps whose data fit the staging capacities; a level larger
+// than that is cut into sub-levels of whole chunks (its rows are
+// independent).  A (sub-)level runs in passes of 8 chunks, one per
+// wavefront.  Step blob, in dwords: 36 per pass (lanes per row g, 1 if a
+// barrier follows, 0, 0, then per wavefront: first value of its chunk
+// relative to the step, entries per lane pf, first lane relative to the
+// step, active lanes (0: idle)), the step's value columns, then its lanes'
+// rows.  The step's values (and, for the backward side, diagonals) are the
+// ranges [val_off, + val_len) and [dg_off, + dg_len) of the solve array.
+// Step header (kSolHdr ints): passes, blob offset, blob length, val_off,
+// val_len, dg_off, dg_len, 0, cols offset, rows offset, values and
+// diagonals actually present (the lengths before padding to 4).  All
+// offsets and lengths of blob sections are multiples of 4.
+constexpr int kSolHdr = 12, kSolLvl = 4;
+struct SolvePlan {
+  std::vector<int> hdr, blob;
+  int maxb = 0, maxv = 0;
+  int nsteps() const { return (int)hdr.size() / kSolHdr; }
+};
+bool solve_plan(const SpluAnalysis::SolveSide &S, bool diag, int dg_base, int bcap, int vcap, SolvePlan &P);
+
 // Returns false (and leaves R empty) if a column alone exceeds a capacity.
 bool res_plan(const SpluAnalysis &A, int bcap, int xcap, ResPlan &R);
 
