@@ -139,7 +139,7 @@ def main():
     ap.add_argument("--methods", default="schur32,schur64,mono")
     ap.add_argument("--policy", default="async", choices=["async", "adaptive"],
                     help="refactorisation policy of the Schur methods")
-    ap.add_argument("--theta", type=float, default=10.0,
+    ap.add_argument("--theta", type=float, default=100.0,
                     help="refactorise when |F| falls by less than this factor in an iteration")
     ap.add_argument("--async-below", type=float, default=0.0,
                     help="async policy: refactorisations asked for below this |F| run on the second stream")

@@ -851,7 +851,7 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
   const double theta = (!schur || pol == GP_REFACTOR_EVERY_ITER)
                            ? 0.0
                            : ((pol == GP_REFACTOR_ADAPTIVE || pol == GP_REFACTOR_ASYNC)
-                                  ? (C->opt.refactor_theta > 0.0 ? C->opt.refactor_theta : 10.0)
+                                  ? (C->opt.refactor_theta > 0.0 ? C->opt.refactor_theta : 100.0)
                                   : 1e300);
   const bool fp32 = method == GP_METHOD_SCHUR_FP32;
   const double async_below = C->opt.async_below > 0.0 ? C->opt.async_below : 1e300;

@@ -1216,6 +1216,46 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// A's values into the LU storage, 8 entries per thread in flight (the
+// index loads, then the value loads, then the stores).
+template <typename TF>
+__device__ __forceinline__ void scatter_a(int nnz_a, const int *__restrict__ a_src, const int *__restrict__ a2,
+                                          const double *__restrict__ var, const double *__restrict__ cst, int ldM,
+                                          int m, TF *lu, int tid, int nt) {
+  constexpr int U = 8;
+  for (int e0 = 0; e0 < nnz_a; e0 += U * nt) {
+    int src[U], dst[U];
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int e = min(e0 + tid + u * nt, nnz_a - 1);
+      src[u] = a_src[e];
+      dst[u] = a2[e];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = src[u] >= 0 ? var[(size_t)src[u] * ldM + m] : cst[-src[u] - 1];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (e0 + tid + u * nt < nnz_a) lu[dst[u]] = TF(v[u]);
+  }
+}
+
+// Copy of `bytes` (a multiple of 16, both ends 16-byte aligned) in 16-byte
+// pieces, 8 per thread in flight.
+__device__ __forceinline__ void copy16(void *dst, const void *__restrict__ src, size_t bytes, int tid, int nt) {
+  const int n16 = (int)(bytes / 16);
+  const int4 *s4 = reinterpret_cast<const int4 *>(src);
+  int4 *d4 = reinterpret_cast<int4 *>(dst);
+  for (int i0 = 0; i0 < n16; i0 += 8 * nt) {
+    int4 t[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) t[u] = s4[min(i0 + tid + u * nt, n16 - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (i0 + tid + u * nt < n16) d4[i0 + tid + u * nt] = t[u];
+  }
+}
+
 struct ColRec {
   int k, c0, len, dl, o0, nops, s0, s1;
 };
@@ -1566,11 +1606,7 @@ __global__ void __launch_bounds__(512) k_splu_factor(SpluDev S, const double *__
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
   for (int p = tid; p < S.nnz_lu; p += nt) lu[p] = TF(0);
   __syncthreads();
-  for (int e = tid; e < S.nnz_a; e += nt) {
-    const int src = S.a_src[e];
-    const double v = src >= 0 ? var[(size_t)src * ldM + m] : cst[-src - 1];
-    lu[S.a2lu[e]] = TF(v);
-  }
+  scatter_a<TF>(S.nnz_a, S.a_src, S.a2lu, var, cst, ldM, m, lu, tid, nt);
   __syncthreads();
   GP_TRACE(0, 0);
   int bad = 0;
@@ -1596,9 +1632,18 @@ __global__ void __launch_bounds__(512) k_splu_factor(SpluDev S, const double *__
     TF *D = reinterpret_cast<TF *>(smem_raw);
     for (int i = tid; i < nd * nd; i += nt) D[i] = TF(0);
     __syncthreads();
-    for (int q = tid; q < S.nnz_top; q += nt) {
-      const int d = S.dmap[q];
-      if (d >= 0) D[d] = lu[S.qb + q];
+    for (int q0 = 0; q0 < S.nnz_top; q0 += 8 * nt) {
+      int d[8];
+      TF v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int q = min(q0 + tid + u * nt, S.nnz_top - 1);
+        d[u] = S.dmap[q];
+        v[u] = lu[S.qb + q];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (q0 + tid + u * nt < S.nnz_top && d[u] >= 0) D[d[u]] = v[u];
     }
     __syncthreads();
     TF *lbuf = D + (((size_t)nd * nd + 3) & ~(size_t)3);
@@ -1684,13 +1729,12 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
   TF *xbuf = reinterpret_cast<TF *>(smem_raw + lvb + (size_t)kResCB * nt * 4);
   const int xcap = kResCX * nt;
   for (int p = tid; p < S.res_xtot; p += nt) lu[p] = TF(0);
+  // the dense copy starts at zero: the external updates write only the
+  // block's pattern entries, and the block is read back whole
+  for (int i = tid; i < S.nd * S.nd; i += nt) dn[i] = TF(0);
   if (tid == 0) Lv[S.res_nl] = TF(0);   // multiplier of the no-op updates
   __syncthreads();
-  for (int e = tid; e < S.nnz_a; e += nt) {
-    const int src = S.a_src[e];
-    const double v = src >= 0 ? var[(size_t)src * ldM + m] : cst[-src - 1];
-    lu[S.res_a2x[e]] = TF(v);
-  }
+  scatter_a<TF>(S.nnz_a, S.a_src, S.res_a2x, var, cst, ldM, m, lu, tid, nt);
   __syncthreads();
   // per thread: blob dwords 4 (tid + j nt) .. + 3 (j < kResCB / 4), value
   // and output-map slots 4 (tid + j nt) .. + 3 (j < kResCX / 4), at clamped
@@ -1794,12 +1838,7 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
     __syncthreads();
     const int nd = S.nd;
     TF *D = Lv;
-    for (int i = tid; i < nd * nd; i += nt) D[i] = TF(0);
-    __syncthreads();
-    for (int q = tid; q < S.nnz_top; q += nt) {
-      const int d = S.dmap[q];
-      if (d >= 0) D[d] = dn[d];
-    }
+    copy16(D, dn, (size_t)nd * nd * sizeof(TF), tid, nt);
     __syncthreads();
     TF *lbuf = reinterpret_cast<TF *>(stg);   // the staging area is free now
     bad |= dense_lu<TF>(D, nd, tid, lbuf, lbuf + 256);
@@ -1976,20 +2015,20 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   TF pv[kSolCV], pd[kSolCD];
   if (nfs) sol_fetch<TF>(S.sol_fw_hdr, S.sol_fw_blob, 0, fv, bd, tid, nt, P, pv, pd);
   {
-    // gather b, four rows per thread in flight
-    int r = tid;
-    for (; r + 3 * nt < n; r += 4 * nt) {
-      double b4[4];
-      int p4[4];
+    // gather b, eight rows per thread in flight
+    for (int r0 = 0; r0 < n; r0 += 8 * nt) {
+      double b8[8];
+      int p8[8];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        b4[u] = rhs[(size_t)(r + u * nt) * ldM + m];
-        p4[u] = S.pinv[r + u * nt];
+      for (int u = 0; u < 8; u++) {
+        const int r = min(r0 + tid + u * nt, n - 1);
+        b8[u] = rhs[(size_t)r * ldM + m];
+        p8[u] = S.pinv[r];
       }
 #pragma unroll
-      for (int u = 0; u < 4; u++) x[p4[u]] = b4[u];
+      for (int u = 0; u < 8; u++)
+        if (r0 + tid + u * nt < n) x[p8[u]] = b8[u];
     }
-    for (; r < n; r += nt) x[S.pinv[r]] = rhs[(size_t)r * ldM + m];
   }
   if (nfs) sol_store<TF>(sb, svl, sdg, tid, nt, P, pv, pd);
   __syncthreads();
@@ -2006,7 +2045,7 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   if (nbs) sol_fetch<TF>(S.sol_bw_hdr, S.sol_bw_blob, 0, bv, bd, tid, nt, P, pv, pd);
   if (nd) {
     const TF *dn = dnv + ((size_t)bsel * Mb + m) * nd * nd;
-    for (int i = tid; i < nd * nd; i += nt) Dl[i] = dn[i];
+    copy16(Dl, dn, (size_t)nd * nd * sizeof(TF), tid, nt);
     __syncthreads();
   }
   if (nd && tid < WAVE) {
@@ -2071,14 +2110,28 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
     }
     GP_TRACE(1, nfs + 2 + st);
   }
-  for (int k = tid; k < n; k += nt) {
-    const int j = S.q[k];
-    const double v = x[k];
-    if (xout) xout[(size_t)j * ldM + m] = v;
-    if (zupd) {
-      const int zr = scatter ? scatter[j] : j;
-      zupd[(size_t)zr * ldM + m] += sign * v;
+  // outputs, eight rows per thread in flight
+  for (int k0 = 0; k0 < n; k0 += 8 * nt) {
+    int j[8], zr[8];
+    double v[8], z[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int k = min(k0 + tid + u * nt, n - 1);
+      j[u] = S.q[k];
+      v[u] = x[k];
     }
+#pragma unroll
+    for (int u = 0; u < 8; u++) zr[u] = scatter ? scatter[j[u]] : j[u];
+    if (zupd) {
+#pragma unroll
+      for (int u = 0; u < 8; u++) z[u] = zupd[(size_t)zr[u] * ldM + m];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (k0 + tid + u * nt < n) {
+        if (xout) xout[(size_t)j[u] * ldM + m] = v[u];
+        if (zupd) zupd[(size_t)zr[u] * ldM + m] = z[u] + sign * v[u];
+      }
   }
 }
 

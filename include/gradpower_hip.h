@@ -123,7 +123,7 @@ enum gp_refactor_policy {
   GP_REFACTOR_ADAPTIVE = 2,    /* keep the factorisation across time steps; factorise
                                   again after a fault event or a change of dt, and
                                   whenever an iteration fails to reduce |F|inf by
-                                  refactor_theta (default 10x)                         */
+                                  refactor_theta (default 100x)                        */
   GP_REFACTOR_ASYNC = 3        /* the iterates of ADAPTIVE, but a refactorisation asked
                                   for after the first iteration of a solve runs on a
                                   second stream: the scenario waits for it while the
@@ -147,7 +147,7 @@ typedef struct gp_options {
                               the stream it runs on (gp_stats kernel fields)    */
   int32_t reserved[4];
   double refactor_theta;   /* ADAPTIVE / ASYNC: refactorise when an iteration
-                              reduces |F|inf by less than this factor (0: 10)   */
+                              reduces |F|inf by less than this factor (0: 100)  */
   double async_below;      /* ASYNC: a refactorisation asked for at |F|inf below
                               this runs on the second stream, above it in the
                               lock step (0: every one runs on the second stream) */
