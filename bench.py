@@ -14,6 +14,13 @@ triangular solve or the batched numeric factorisation, whichever has the
 larger device time in the headline method), timed with HIP events by the
 library around every launch inside the timed passes, and the CPU oracle
 timed on a bounded sample of the same workload.
+
+The "adjoint" object measures the discrete adjoint on the 200-bus workload
+(40 GENROU+SEXS+TGOV1, M=128 faults, 3 s): one step is one gp_adjoint call
+(checkpointed forward sweep, recomputation, transposed solves) giving the
+gradient of the batch objective w.r.t. 120 parameters; it carries a central
+difference check on a few random parameters and the CPU oracle's adjoint
+timed on a bounded sample.
 """
 import argparse
 import json
@@ -30,7 +37,7 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, published
 KERNEL_NAMES = {("solve", 2): "k_splu_solve<float>", ("solve", 1): "k_splu_solve<double>",
-                ("solve", 0): "k_splu_solve<double>", ("factor", 2): "k_splu_factor<float>",
+                ("solve", 0): "k_splu_solve<double>", ("factor", 2): "k_splu_factor_res<float>",
                 ("factor", 1): "k_splu_factor<double>", ("factor", 0): "k_splu_factor<double>"}
 
 
@@ -100,6 +107,102 @@ def cpu_baseline(case, budget_s=20.0):
     return steps / t_total, done, steps, t_total
 
 
+def adjoint_params(case):
+    """All GENROU H and D and all SEXS K of a case (parameter indices)."""
+    d = case.desc.dev
+    return ([int(r[2] + 6) for r in d[0]] + [int(r[2] + 7) for r in d[0]] + [int(r[1] + 2) for r in d[3]])
+
+
+def adjoint_objective(case, p, rows, scale, tol, scen):
+    """The adjoint's objective from gp_integrate's trajectories (for the
+    central differences): scale * sum_m sum_{k>=1} dt sum_r z_k[r]^2."""
+    from gradpower_hip.lib import HipContext, METHOD_MONO_FP64
+    ctx = HipContext(case.desc, case.M, record_idx=rows)
+    ctx.set_state(case.desc.z0, p)
+    ctx.set_events(events_of(case), case.nsteps)
+    ctx.integrate(METHOD_MONO_FP64, case.dt, tol, 50, raise_on_fail=False)
+    st, _ = ctx.scenario_status()
+    if any(st[m] != 0 for m in scen):
+        raise RuntimeError("a scenario of the difference quotient did not converge")
+    tot = sum(scale * case.dt * float((ctx.trajectory(m)[:, 1:] ** 2).sum()) for m in scen)
+    ctx.close()
+    return tot
+
+
+def run_adjoint(steps, warmup, n_fd, cpu_budget, no_cpu):
+    """Discrete adjoint on the 200-bus workload: 40 GENROU+SEXS+TGOV1,
+    M=128 faults, 3 s at dt=1/120, objective sum_m sum_k dt mean_g w_g^2,
+    gradient w.r.t. all H, D and SEXS K (120 parameters), 30-step
+    checkpoint segments.  One "step" is one gp_adjoint call over the batch."""
+    from gradpower_hip import synth
+    from gradpower_hip.lib import HipContext, METHOD_MONO_FP64
+    case = synth.case3_200bus()
+    params = adjoint_params(case)
+    rows = [int(r[0] + 4) for r in case.desc.dev[0]]
+    scale = 1.0 / len(rows)
+    tol = 1e-12
+    ctx = HipContext(case.desc, case.M)
+    times, r = [], None
+    for it in range(warmup + steps):
+        ctx.set_state(case.desc.z0, case.desc.p)
+        ctx.set_events(events_of(case), case.nsteps)
+        t0 = time.perf_counter()
+        r = ctx.adjoint(METHOD_MONO_FP64, case.dt, params, tol=tol, itermax=50, segment=30, scale=scale,
+                        raise_on_fail=False)
+        t1 = time.perf_counter()
+        if it >= warmup:
+            times.append(t1 - t0)
+        a = r["stats"]
+        log(f"  adjoint pass {it}: {t1 - t0:.3f} s, device {a['elapsed_ms']:.1f} ms (forward "
+            f"{a['forward_ms']:.1f}, recompute+backward {a['backward_ms']:.1f}), adjoint steps "
+            f"{a['adjoint_steps']}, excluded {a['n_excluded']}")
+    st, _ = ctx.scenario_status()
+    ctx.close()
+    ok = [m for m in range(case.M) if st[m] == 0]
+    t = float(np.median(times))
+    out = {"workload": "200-bus synthetic grid, 40 GENROU+SEXS+TGOV1, M=128 faults, 3 s at dt=1/120, "
+                       "objective sum_m sum_k dt mean_g w_g^2, d/d(all H, D, SEXS K) = 120 parameters, "
+                       "30-step checkpoint segments, monolithic FP64 forward and transposed FP64 LU",
+           "grads_per_s": 1.0 / t, "unit": "batch gradients/s", "ms_per_gradient": 1e3 * t,
+           "scenario_grads_per_s": len(ok) / t, "adjoint_steps_per_s": r["stats"]["adjoint_steps"] / t,
+           "forward_ms": r["stats"]["forward_ms"], "backward_ms": r["stats"]["backward_ms"],
+           "excluded_scenarios": case.M - len(ok), "n_params": len(params), "newton_tol": tol,
+           "dtype": "f64"}
+    if n_fd > 0:
+        rng = np.random.default_rng(2024)
+        worst, p0 = 0.0, case.desc.p.copy()
+        for j in rng.choice(len(params), size=n_fd, replace=False):
+            q = params[j]
+            h = 1e-6 * max(abs(p0[q]), 1.0)
+            pp, pm = p0.copy(), p0.copy()
+            pp[q] += h
+            pm[q] -= h
+            fd = (adjoint_objective(case, pp, rows, scale, tol, ok) -
+                  adjoint_objective(case, pm, rows, scale, tol, ok)) / (2 * h)
+            worst = max(worst, abs(r["grad"][j] - fd) / abs(fd))
+            log(f"  fd check p[{q}]: adjoint {r['grad'][j]:.10e} fd {fd:.10e}")
+        out["fd_check"] = {"params": int(n_fd), "h": "1e-6 max(|p|, 1)", "max_rel_err": worst}
+    if not no_cpu:
+        from oracle.oracle import Oracle
+        o = Oracle(case.desc)
+        tt, done = 0.0, 0
+        for m in ok:
+            ev = [(e.bus, e.rfault, int(round(e.ton / case.dt)), int(round(e.toff / case.dt)))
+                  for e in case.events[m]]
+            t0 = time.perf_counter()
+            o.adjoint(case.dt, int(case.nsteps[m]), params, ev, rows=rows, scale=scale, tol=tol, itermax=50)
+            tt += time.perf_counter() - t0
+            done += 1
+            log(f"  cpu oracle adjoint scenario {m}: {tt:.1f} s so far")
+            if tt > cpu_budget:
+                break
+        out["cpu_baseline"] = {"value": done / tt, "unit": "scenario gradients/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle adjoint (restated reference CPU path, sparse LU of J^T, "
+                                         f"residual differences for dF/dp, 1 thread) on {done} scenarios of "
+                                         f"the batch, 120 parameters, full 3 s ({tt:.1f} s)"}
+    return out
+
+
 def roofline(run, method):
     """Dominant kernel of one method: HIP-event device time per launch and
     algorithmic bytes per launch (bytes per scenario x scenarios served)."""
@@ -146,6 +249,9 @@ def main():
     ap.add_argument("--stall-window", type=int, default=8,
                     help="stop a scenario whose |F| has not halved in this many Newton iterations (0: run to itermax)")
     ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete-adjoint measurement")
+    ap.add_argument("--adjoint-fd", type=int, default=4,
+                    help="parameters checked against central differences in the adjoint measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the integrator shards by scenario; "
@@ -184,7 +290,7 @@ def main():
     info = head["info"]
     v = {k: r["stats"]["scenario_steps"] / float(np.median(r["times"])) for k, r in runs.items()}
     out = {
-        "metric": "contingency-steps/sec at 2k-bus M=256 (Schur+FP32 LU vs plain FP64)",
+        "metric": "contingency-steps/sec at 2k-bus M=256 (Schur+FP32 LU vs plain FP64); adjoint grads/s",
         "value": v[head_name],
         "unit": "contingency-steps/s",
         "n_gpus": 1,
@@ -233,6 +339,9 @@ def main():
                                "sample": f"oracle (restated reference CPU path, monolithic sparse-LU Newton, "
                                          f"1 thread) on the first {done} scenarios of the batch, full horizons "
                                          f"({steps} steps, {tt:.1f} s)"}
+    if not args.no_adjoint:
+        log("adjoint (200-bus, M=128)")
+        out["adjoint"] = run_adjoint(args.steps, args.warmup, args.adjoint_fd, args.cpu_budget / 2, args.no_cpu)
     print(json.dumps(out))
 
 

@@ -255,6 +255,14 @@ struct gp_ctx {
   DBuf<unsigned long long> d_nb;
   DevTables T{};
   LUPlan luJ, luS;
+  // discrete adjoint: LU plan of J^T (values read from d_jv), its factors,
+  // the initial state of the last gp_set_state, checkpoints, the states of
+  // one segment, multipliers, parameter pullbacks and objective per scenario
+  LUPlan luJT;
+  DBuf<double> d_luJT, d_svJT, d_dnJT, d_z0, d_ckpt, d_seg, d_lam, d_gacc, d_val;
+  DBuf<int> d_hz, d_objrows;
+  DBuf<int2> d_ent;
+  DBuf<uint8_t> d_bact;
   bool state_set = false;
   int hist_steps = 0;
   std::vector<int> traj_len;
@@ -601,6 +609,7 @@ void alloc_state(gp_ctx *C) {
   const size_t L = (size_t)C->ldM;
   C->d_p.alloc(std::max(1, C->npar));
   C->d_z.alloc((size_t)C->n * L);
+  C->d_z0.alloc((size_t)C->n * L);
   C->d_zold.alloc((size_t)C->n * L);
   C->d_f.alloc((size_t)C->n * L);
   C->d_inj.alloc((size_t)std::max(1, 2 * C->ncl) * L);
@@ -991,6 +1000,306 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
   return out;
 }
 
+// Symbolic analysis and buffers of `method` at step dt.
+void prepare_method(gp_ctx *C, int method, double dt) {
+  analyse(C, method, dt);
+  if (method != GP_METHOD_MONO_FP64) ensure_schur_buffers(C, method == GP_METHOD_SCHUR_FP32);
+  else C->luJ.shape(sizeof(double));
+}
+
+// Column 0 of the recorded trajectory: the initial state of every scenario.
+void record_initial(gp_ctx *C) {
+  std::vector<uint8_t> all(C->ldM, 0);
+  std::fill(all.begin(), all.begin() + C->M, 1);
+  set_mask(C, C->d_act, all);
+  launch_record_impl(C->nrec, C->rec_idx.empty() ? nullptr : C->d_rec_idx.p, C->ldM, C->M, C->d_z.p, C->d_hist.p,
+                     C->d_act.p, C->stream);
+  HIPCHK(hipStreamSynchronize(C->stream));   // the host mask stays valid until then
+}
+
+// Device time between construction and ms() on one stream.
+struct DevTimer {
+  hipStream_t s;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  explicit DevTimer(hipStream_t s_) : s(s_) {
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+  }
+  double ms() {
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    return t;
+  }
+  ~DevTimer() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
+// Zero pivots reported by the factorisations of the sweep, then the
+// scenario outcome as the call's status.
+int finish_status(gp_ctx *C, const gp_stats &S) {
+  const int L = C->ldM, M = C->M;
+  std::vector<int> status(L);
+  HIPCHK(hipMemcpy(status.data(), C->d_status.p, L * sizeof(int), hipMemcpyDeviceToHost));
+  int nsing = 0;
+  for (int m = 0; m < M; m++)
+    if (status[m] != 0) {
+      nsing++;
+      C->scen_status[m] = GP_ERR_SINGULAR_PIVOT;
+    }
+  HIPCHK(hipMemset(C->d_status.p, 0, L * sizeof(int)));
+  if (nsing) return fail(GP_ERR_SINGULAR_PIVOT, std::to_string(nsing) + " scenario(s) hit a zero or non-finite pivot");
+  if (S.n_nan) return fail(GP_ERR_NAN, std::to_string(S.n_nan) + " scenario(s) produced a non-finite residual");
+  if (S.n_not_converged)
+    return fail(GP_ERR_NEWTON_NOT_CONVERGED,
+                std::to_string(S.n_not_converged) + " scenario(s) had a Newton solve that did not converge");
+  return GP_OK;
+}
+
+// The time loop of integrate! (src/dynamics.jl:548-640) one step at a time,
+// for every scenario still running: the Newton solve of step k, the record
+// of z_k, then the fault events at step k (activation, then deactivation)
+// and the dt = 0 re-solve of the scenarios they touch.
+struct Stepper {
+  gp_ctx *C;
+  int method, itermax;
+  double dt, tol;
+  std::map<int, std::vector<const gp_event *>> on_at, off_at;
+  // per-scenario state that a checkpoint restores
+  struct State {
+    std::vector<std::vector<const gp_event *>> active_ev;   // active faults, in event order
+    std::vector<uint8_t> dead, everfail;
+    std::vector<uint8_t> stale;   // factorisation made with another dt or before an event
+    std::vector<int> fbus;        // fault tables [MAXFAULT][ldM]
+    std::vector<double> fg;
+    int ftab = 0;                 // index of the fault tables in `tabs`
+  } st;
+  std::vector<uint8_t> act, res, need0;
+  gp_stats S{};
+  // every fault table the sweep used (version 0: no fault) and, per step k,
+  // the version in force during step k
+  std::vector<std::vector<int>> tab_bus;
+  std::vector<std::vector<double>> tab_g;
+  std::vector<int> step_tab;
+
+  Stepper(gp_ctx *c, int meth, double dt_, double tol_, int itmax, int maxsteps)
+      : C(c), method(meth), itermax(itmax), dt(dt_), tol(tol_) {
+    const int L = C->ldM, M = C->M;
+    for (auto &e : C->events) {
+      on_at[e.step_on].push_back(&e);
+      off_at[e.step_off].push_back(&e);
+    }
+    st.active_ev.assign(M, {});
+    st.dead.assign(L, 0);
+    st.everfail.assign(L, 0);
+    st.stale.assign(L, 1);
+    st.fbus.assign((size_t)MAXFAULT * L, -1);
+    st.fg.assign((size_t)MAXFAULT * L, 0.0);
+    act.assign(L, 0);
+    res.assign(L, 0);
+    need0.assign(L, 0);
+    tab_bus.push_back(st.fbus);
+    tab_g.push_back(st.fg);
+    step_tab.assign(maxsteps + 2, 0);
+    upload_faults();
+  }
+  void upload_faults() {
+    HIPCHK(hipMemcpyAsync(C->d_fbus.p, tab_bus[st.ftab].data(), st.fbus.size() * sizeof(int), hipMemcpyHostToDevice,
+                          C->stream));
+    HIPCHK(hipMemcpyAsync(C->d_fg.p, tab_g[st.ftab].data(), st.fg.size() * sizeof(double), hipMemcpyHostToDevice,
+                          C->stream));
+  }
+  void restore(const State &s) {
+    st = s;
+    upload_faults();
+  }
+  void add_stats(const NewtonOut &no) {
+    S.newton_iters += no.corrections;
+    S.refactorizations += no.refac;
+    S.newton_rounds += no.rounds;
+    S.factor_launches += no.factor_launches;
+  }
+  // Step k for the scenarios with k <= nsteps[m] that have not stopped.
+  // hist_k: recorded channels of z_k (NULL: none); zsave: all of z_k,
+  // scenario-major (NULL: none).  Returns false when no scenario ran.
+  bool step(int k, double *hist_k, double *zsave) {
+    const int L = C->ldM, M = C->M;
+    for (int m = 0; m < M; m++) act[m] = (k <= C->nsteps[m] && !st.dead[m]) ? 1 : 0;
+    int nact = 0;
+    for (int m = 0; m < M; m++) nact += act[m];
+    if (nact == 0) return false;
+    step_tab[k] = st.ftab;
+    set_mask(C, C->d_act, act);
+    launch_copy_active(C->n, L, C->d_z.p, C->d_zold.p, C->d_act.p, C->stream);
+    for (int m = 0; m < L; m++) need0[m] = act[m] && st.stale[m];
+    NewtonOut no = newton(C, method, dt, tol, itermax, act, &need0);
+    for (int m = 0; m < M; m++)
+      if (act[m]) st.stale[m] = 0;
+    add_stats(no);
+    S.scenario_steps += nact;
+    if (!no.failed.empty())
+      for (int m = 0; m < M; m++)
+        if (no.failed[m]) {
+          S.scenario_steps--;
+          st.everfail[m] = 1;
+          st.dead[m] = 1;   // a scenario whose Newton solve failed is stopped and reported
+          C->scen_status[m] = GP_ERR_NEWTON_NOT_CONVERGED;
+          C->scen_steps[m] = k - 1;
+        }
+    if (hist_k)
+      launch_record_impl(C->nrec, C->rec_idx.empty() ? nullptr : C->d_rec_idx.p, L, M, C->d_z.p, hist_k,
+                         C->d_act.p, C->stream);
+    if (zsave) launch_copy_active(C->n, L, C->d_z.p, zsave, C->d_act.p, C->stream);
+    // events at step k: activation then deactivation (src/dynamics.jl:599-629)
+    bool changed = false;
+    std::fill(res.begin(), res.end(), 0);
+    auto ion = on_at.find(k);
+    if (ion != on_at.end())
+      for (const gp_event *e : ion->second) {
+        if (!act[e->scenario]) continue;
+        st.active_ev[e->scenario].push_back(e);
+        res[e->scenario] = 1;
+        changed = true;
+      }
+    auto ioff = off_at.find(k);
+    if (ioff != off_at.end())
+      for (const gp_event *e : ioff->second) {
+        if (!act[e->scenario]) continue;
+        auto &v = st.active_ev[e->scenario];
+        auto it = std::find(v.begin(), v.end(), e);
+        if (it != v.end()) v.erase(it);
+        res[e->scenario] = 1;
+        changed = true;
+      }
+    if (changed) {
+      for (int m = 0; m < M; m++) {
+        if (!res[m]) continue;
+        if ((int)st.active_ev[m].size() > MAXFAULT)
+          throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults in one scenario"};
+        for (int q = 0; q < MAXFAULT; q++) {
+          const bool on = q < (int)st.active_ev[m].size();
+          st.fbus[(size_t)q * L + m] = on ? st.active_ev[m][q]->bus : -1;
+          st.fg[(size_t)q * L + m] = on ? 1.0 / st.active_ev[m][q]->r_fault : 0.0;
+        }
+      }
+      // a new version of the tables; kept, so that the asynchronous copies
+      // read memory that stays valid and a backward sweep can reuse them
+      tab_bus.push_back(st.fbus);
+      tab_g.push_back(st.fg);
+      st.ftab = (int)tab_bus.size() - 1;
+      upload_faults();
+      int nres = 0;
+      for (int m = 0; m < M; m++) nres += res[m];
+      S.event_resolves += nres;
+      // re-solve at dt = 0: differential states frozen, algebraic states re-solved
+      set_mask(C, C->d_sel, res);
+      launch_copy_active(C->n, L, C->d_z.p, C->d_zold.p, C->d_sel.p, C->stream);
+      NewtonOut nr = newton(C, method, 0.0, tol, itermax, res, &res);
+      for (int m = 0; m < M; m++)
+        if (res[m]) st.stale[m] = 1;   // factorised at dt = 0: the next step refactorises
+      add_stats(nr);
+      if (!nr.failed.empty())
+        for (int m = 0; m < M; m++)
+          if (nr.failed[m]) {
+            st.everfail[m] = 1;
+            st.dead[m] = 1;
+            C->scen_status[m] = GP_ERR_NEWTON_NOT_CONVERGED;
+            C->scen_steps[m] = k;
+          }
+    }
+    // non-finite states stop a scenario
+    if (!no.failed.empty()) {
+      std::vector<double> fn(L);
+      HIPCHK(hipMemcpy(fn.data(), C->d_fnorm.p, L * sizeof(double), hipMemcpyDeviceToHost));
+      for (int m = 0; m < M; m++)
+        if (no.failed[m] && !std::isfinite(fn[m])) {
+          st.dead[m] = 1;
+          S.n_nan++;
+          C->scen_status[m] = GP_ERR_NAN;
+        }
+    }
+    return true;
+  }
+};
+
+
+// LU plan of J^T for the adjoint's transposed solves: the CSC pattern of J^T
+// (the CSR pattern of J) whose values the factorisation reads from the
+// Jacobian value array in place (a_src), pivots chosen on scenario 0's matrix
+// at the current state.
+void analyse_transposed(gp_ctx *C, double dt) {
+  analyse(C, GP_METHOD_MONO_FP64, dt);
+  C->luJ.shape(sizeof(double));
+  if (C->luJT.ready && C->luJT.dt == dt) return;
+  const int n = C->n, nnz = C->jcp[n], L = C->ldM;
+  std::vector<uint8_t> sel(L, 0);
+  sel[0] = 1;
+  set_mask(C, C->d_sel, sel);
+  launch_jac_mono(C->T, C->d_p.p, C->d_z.p, dt, faults(C), C->d_jv.p, C->d_sel.p, C->stream);
+  HIPCHK(hipGetLastError());
+  std::vector<double> jv(nnz);
+  HIPCHK(hipMemcpy2DAsync(jv.data(), sizeof(double), C->d_jv.p, (size_t)L * sizeof(double), sizeof(double), nnz,
+                          hipMemcpyDeviceToHost, C->stream));
+  HIPCHK(hipStreamSynchronize(C->stream));
+  std::vector<int> tcp(n + 1, 0), tri(nnz), tsrc(nnz);
+  for (int q = 0; q < nnz; q++) tcp[C->jri[q] + 1]++;
+  for (int i = 0; i < n; i++) tcp[i + 1] += tcp[i];
+  std::vector<int> next(tcp.begin(), tcp.end() - 1);
+  for (int c = 0; c < n; c++)
+    for (int q = C->jcp[c]; q < C->jcp[c + 1]; q++) {
+      const int t = next[C->jri[q]]++;
+      tri[t] = c;   // rows of each J^T column ascend: c ascends
+      tsrc[t] = q;
+    }
+  std::vector<double> tv(nnz);
+  for (int t = 0; t < nnz; t++) tv[t] = jv[tsrc[t]];
+  std::string e = splu_analyse(n, tcp.data(), tri.data(), tv.data(), C->luJT.A);
+  if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "transposed Jacobian analysis: " + e};
+  C->luJT.upload(tsrc);
+  C->luJT.shape(sizeof(double));
+  C->luJT.ready = true;
+  C->luJT.dt = dt;
+  C->d_luJT.alloc(C->luJT.lu_elems() * C->M);
+  C->d_svJT.alloc((size_t)C->luJT.dev.ns * C->M);
+  HIPCHK(hipMemset(C->d_svJT.p, 0, C->d_svJT.n * sizeof(double)));   // padding slots stay zero
+  C->d_dnJT.alloc((size_t)std::max(1, C->luJT.A.nd * C->luJT.A.nd) * C->M);
+}
+
+// Parameter index -> pullback entry (kind, device row) of k_adj_param.
+bool adj_entry(const gp_ctx *C, int q, int2 &e) {
+  struct Slot {
+    int type, pcol, off, kind;
+  };
+  static const Slot slots[] = {{GP_GENROU, 2, 6, GP_ADJ_GENROU_H}, {GP_GENROU, 2, 7, GP_ADJ_GENROU_D},
+                               {GP_SEXS, 1, 2, GP_ADJ_SEXS_K},     {GP_ESDC1A, 1, 0, GP_ADJ_ESDC1A_KA},
+                               {GP_TGOV1, 2, 0, GP_ADJ_TGOV1_R},   {GP_IEESGO, 2, 6, GP_ADJ_IEESGO_K1},
+                               {GP_IEEEST, 2, 12, GP_ADJ_IEEEST_KS}};
+  static const int ncols[GP_NDEVTYPES] = {GP_COLS_GENROU, GP_COLS_IEESGO, GP_COLS_TGOV1, GP_COLS_SEXS,
+                                          GP_COLS_ESDC1A, GP_COLS_IEEEST, GP_COLS_ZIPLOAD, GP_COLS_STATICGEN};
+  for (const Slot &s : slots) {
+    const auto &d = C->dev[s.type];
+    const int nc = ncols[s.type], rows = (int)d.size() / nc;
+    for (int r = 0; r < rows; r++)
+      if (d[(size_t)r * nc + s.pcol] + s.off == q) {
+        e = make_int2(s.kind, r);
+        return true;
+      }
+  }
+  return false;
+}
+
+// Restores the refactorisation policy on scope exit.
+struct PolicyGuard {
+  gp_ctx *C;
+  int saved;
+  PolicyGuard(gp_ctx *c, int pol) : C(c), saved(c->opt.refactor_policy) { C->opt.refactor_policy = pol; }
+  ~PolicyGuard() { C->opt.refactor_policy = saved; }
+};
+
 }  // namespace
 
 // ================================================================= C ABI
@@ -1090,6 +1399,7 @@ int gp_set_state(gp_ctx *C, const double *z0, int32_t per_scenario, const double
       for (int i = 0; i < C->n; i++) zt[(size_t)i * L + m] = src[i];
     }
     HIPCHK(hipMemcpy(C->d_z.p, zt.data(), zt.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(C->d_z0.p, zt.data(), zt.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(C->d_p.p, p, (size_t)C->npar * sizeof(double), hipMemcpyHostToDevice));
     C->state_set = true;
   } catch (const GpError &e) {
@@ -1125,152 +1435,22 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
     const int L = C->ldM, M = C->M;
     int maxsteps = 0;
     for (int m = 0; m < M; m++) maxsteps = std::max(maxsteps, C->nsteps[m]);
-    // fault table reset (no fault active at t = 0)
-    std::vector<int> fbus((size_t)MAXFAULT * L, -1);
-    std::vector<double> fg((size_t)MAXFAULT * L, 0.0);
-    HIPCHK(hipMemcpyAsync(C->d_fbus.p, fbus.data(), fbus.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
-    HIPCHK(hipMemcpyAsync(C->d_fg.p, fg.data(), fg.size() * sizeof(double), hipMemcpyHostToDevice, C->stream));
-    analyse(C, method, dt);
-    if (method != GP_METHOD_MONO_FP64) ensure_schur_buffers(C, method == GP_METHOD_SCHUR_FP32);
-    else C->luJ.shape(sizeof(double));
+    Stepper sp(C, method, dt, tol, itermax, maxsteps);   // no fault active at t = 0
+    prepare_method(C, method, dt);
     // trajectory storage [step][channel][ldM]
     C->d_hist.alloc((size_t)(maxsteps + 1) * C->nrec * L);
     C->hist_steps = maxsteps + 1;
     C->traj_len.assign(M, 0);
     for (int m = 0; m < M; m++) C->traj_len[m] = C->nsteps[m] + 1;
-    // per-step event lists
-    std::map<int, std::vector<const gp_event *>> on_at, off_at;
-    for (auto &e : C->events) {
-      on_at[e.step_on].push_back(&e);
-      off_at[e.step_off].push_back(&e);
-    }
-    std::vector<std::vector<std::pair<int, double>>> active(M);   // (bus, g) per scenario, in event order
-    std::vector<std::vector<const gp_event *>> active_ev(M);
-    std::vector<uint8_t> act(L, 0), dead(L, 0), everfail(L, 0), res(L, 0);
-    // factorisation reuse (GP_REFACTOR_ADAPTIVE): a scenario refactorises at
-    // the first iteration of a Newton solve when its stored factorisation was
-    // made with another dt or before a fault event (or it has none yet)
-    std::vector<uint8_t> stale(L, 1), need0(L, 0), need_res(L, 1);
-    for (int m = 0; m < M; m++) act[m] = 1;
     C->scen_status.assign(M, GP_OK);
     C->scen_steps.assign(C->nsteps.begin(), C->nsteps.begin() + M);
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, C->stream));
-    set_mask(C, C->d_act, act);
-    launch_record_impl(C->nrec, C->rec_idx.empty() ? nullptr : C->d_rec_idx.p, L, M, C->d_z.p, C->d_hist.p,
-                       C->d_act.p, C->stream);
-    for (int k = 1; k <= maxsteps; k++) {
-      for (int m = 0; m < M; m++) act[m] = (k <= C->nsteps[m] && !dead[m]) ? 1 : 0;
-      int nact = 0;
-      for (int m = 0; m < M; m++) nact += act[m];
-      if (nact == 0) break;
-      set_mask(C, C->d_act, act);
-      launch_copy_active(C->n, L, C->d_z.p, C->d_zold.p, C->d_act.p, C->stream);
-      for (int m = 0; m < L; m++) need0[m] = act[m] && stale[m];
-      NewtonOut no = newton(C, method, dt, tol, itermax, act, &need0);
-      for (int m = 0; m < M; m++)
-        if (act[m]) stale[m] = 0;
-      S.newton_iters += no.corrections;
-      S.refactorizations += no.refac;
-      S.newton_rounds += no.rounds;
-      S.factor_launches += no.factor_launches;
-      S.scenario_steps += nact;
-      if (!no.failed.empty())
-        for (int m = 0; m < M; m++) S.scenario_steps -= no.failed[m] ? 1 : 0;
-      if (!no.failed.empty())
-        for (int m = 0; m < M; m++)
-          if (no.failed[m]) {
-            everfail[m] = 1;
-            dead[m] = 1;   // a scenario whose Newton solve failed is stopped and reported
-            C->scen_status[m] = GP_ERR_NEWTON_NOT_CONVERGED;
-            C->scen_steps[m] = k - 1;
-          }
-      double *hist_k = C->d_hist.p + (size_t)k * C->nrec * L;
-      launch_record_impl(C->nrec, C->rec_idx.empty() ? nullptr : C->d_rec_idx.p, L, M, C->d_z.p, hist_k, C->d_act.p,
-                         C->stream);
-      // events at step k: activation then deactivation (src/dynamics.jl:599-629)
-      bool changed = false;
-      std::fill(res.begin(), res.end(), 0);
-      auto ion = on_at.find(k);
-      if (ion != on_at.end())
-        for (const gp_event *e : ion->second) {
-          if (!act[e->scenario]) continue;
-          active_ev[e->scenario].push_back(e);
-          res[e->scenario] = 1;
-          changed = true;
-        }
-      auto ioff = off_at.find(k);
-      if (ioff != off_at.end())
-        for (const gp_event *e : ioff->second) {
-          if (!act[e->scenario]) continue;
-          auto &v = active_ev[e->scenario];
-          auto it = std::find(v.begin(), v.end(), e);
-          if (it != v.end()) v.erase(it);
-          res[e->scenario] = 1;
-          changed = true;
-        }
-      if (changed) {
-        for (int m = 0; m < M; m++) {
-          if (!res[m]) continue;
-          if ((int)active_ev[m].size() > MAXFAULT)
-            throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults in one scenario"};
-          for (int q = 0; q < MAXFAULT; q++) {
-            if (q < (int)active_ev[m].size()) {
-              fbus[(size_t)q * L + m] = active_ev[m][q]->bus;
-              fg[(size_t)q * L + m] = 1.0 / active_ev[m][q]->r_fault;
-            } else {
-              fbus[(size_t)q * L + m] = -1;
-              fg[(size_t)q * L + m] = 0.0;
-            }
-          }
-        }
-        HIPCHK(hipMemcpyAsync(C->d_fbus.p, fbus.data(), fbus.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
-        HIPCHK(hipMemcpyAsync(C->d_fg.p, fg.data(), fg.size() * sizeof(double), hipMemcpyHostToDevice, C->stream));
-        int nres = 0;
-        for (int m = 0; m < M; m++) nres += res[m];
-        S.event_resolves += nres;
-        // re-solve at dt = 0: differential states frozen, algebraic states re-solved
-        set_mask(C, C->d_sel, res);
-        launch_copy_active(C->n, L, C->d_z.p, C->d_zold.p, C->d_sel.p, C->stream);
-        NewtonOut nr = newton(C, method, 0.0, tol, itermax, res, &res);
-        for (int m = 0; m < M; m++)
-          if (res[m]) stale[m] = 1;   // factorised at dt = 0: the next step refactorises
-        S.newton_iters += nr.corrections;
-        S.refactorizations += nr.refac;
-        S.newton_rounds += nr.rounds;
-        S.factor_launches += nr.factor_launches;
-        if (!nr.failed.empty())
-          for (int m = 0; m < M; m++)
-            if (nr.failed[m]) {
-              everfail[m] = 1;
-              dead[m] = 1;
-              C->scen_status[m] = GP_ERR_NEWTON_NOT_CONVERGED;
-              C->scen_steps[m] = k;
-            }
-      }
-      // non-finite states stop a scenario
-      if (!no.failed.empty()) {
-        std::vector<double> fn(L);
-        HIPCHK(hipMemcpy(fn.data(), C->d_fnorm.p, L * sizeof(double), hipMemcpyDeviceToHost));
-        for (int m = 0; m < M; m++)
-          if (no.failed[m] && !std::isfinite(fn[m])) {
-            dead[m] = 1;
-            S.n_nan++;
-            C->scen_status[m] = GP_ERR_NAN;
-          }
-      }
-    }
+    DevTimer tm(C->stream);
+    record_initial(C);
+    for (int k = 1; k <= maxsteps; k++)
+      if (!sp.step(k, C->d_hist.p + (size_t)k * C->nrec * L, nullptr)) break;
     land_batch(C, true);   // no factorisation outlives the call
-    HIPCHK(hipEventRecord(e1, C->stream));
-    HIPCHK(hipEventSynchronize(e1));
-    float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    S.elapsed_ms = ms;
-    S.solve_ms = ms;
+    S = sp.S;
+    S.elapsed_ms = S.solve_ms = tm.ms();
     if (C->opt.profile) {
       HIPCHK(hipStreamSynchronize(C->fstream));
       S.solve_kernel_ms = prof_collect(C->prof_solve, S.solve_kernel_launches, S.solve_kernel_scenarios);
@@ -1281,22 +1461,10 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
       S.solve_bytes_per_scenario = bf * A.nnz_lu + (mono ? 24.0 : 32.0) * A.n;
       S.factor_bytes_per_scenario = 8.0 * A.nnz_a + bf * A.nnz_lu;
     }
-    for (int m = 0; m < M; m++) S.n_not_converged += everfail[m];
-    std::vector<int> status(L);
-    HIPCHK(hipMemcpy(status.data(), C->d_status.p, L * sizeof(int), hipMemcpyDeviceToHost));
-    int nsing = 0;
-    for (int m = 0; m < M; m++)
-      if (status[m] != 0) {
-        nsing++;
-        C->scen_status[m] = GP_ERR_SINGULAR_PIVOT;
-      }
-    HIPCHK(hipMemset(C->d_status.p, 0, L * sizeof(int)));
+    for (int m = 0; m < M; m++) S.n_not_converged += sp.st.everfail[m];
     if (st) *st = S;
-    if (nsing) return fail(GP_ERR_SINGULAR_PIVOT, std::to_string(nsing) + " scenario(s) hit a zero or non-finite pivot");
-    if (S.n_nan) return fail(GP_ERR_NAN, std::to_string(S.n_nan) + " scenario(s) produced a non-finite residual");
-    if (S.n_not_converged)
-      return fail(GP_ERR_NEWTON_NOT_CONVERGED,
-                  std::to_string(S.n_not_converged) + " scenario(s) had a Newton solve that did not converge");
+    const int rc = finish_status(C, S);
+    if (rc != GP_OK) return rc;
   } catch (const GpError &e) {
     try {
       land_batch(C, true);
@@ -1304,6 +1472,200 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
       C->inflight = false;
     }
     if (st) *st = S;
+    return fail(e.code, e.msg);
+  }
+  return GP_OK;
+}
+
+int gp_adjoint(gp_ctx *C, int32_t method, double dt, double tol, int32_t itermax, int32_t segment,
+               const gp_objective *obj, int32_t n_params, const int32_t *params, double *value, double *grad,
+               double *grad_scen, double *dz0, gp_adjoint_stats *ast) {
+  if (!C) return fail(GP_ERR_BADARG, "null context");
+  if (method < 0 || method > 2) return fail(GP_ERR_BADARG, "unknown method");
+  if (!(dt > 0.0) || !(tol > 0.0) || itermax <= 0) return fail(GP_ERR_BADARG, "dt, tol and itermax must be > 0");
+  if (!C->state_set) return fail(GP_ERR_BADARG, "gp_set_state was not called");
+  if (segment < 0 || n_params < 0 || (n_params > 0 && (!params || !grad)))
+    return fail(GP_ERR_BADARG, "gp_adjoint: bad segment, parameter list or gradient output");
+  // objective rows (default: the GENROU speed deviations)
+  std::vector<int> rows;
+  if (obj && obj->rows) {
+    if (obj->n_rows < 0) return fail(GP_ERR_BADARG, "gp_adjoint: negative objective row count");
+    rows.assign(obj->rows, obj->rows + obj->n_rows);
+    for (int r : rows)
+      if (r < 0 || r >= C->n) return fail(GP_ERR_BADARG, "gp_adjoint: objective row out of range");
+  } else {
+    const auto &g = C->dev[GP_GENROU];
+    for (size_t k = 0; k < g.size() / GP_COLS_GENROU; k++) rows.push_back(g[k * GP_COLS_GENROU] + 4);
+  }
+  const double scale = (obj && obj->scale != 0.0) ? obj->scale : 1.0;
+  const double off = obj ? obj->offset : 0.0;
+  std::vector<int2> ent(n_params);
+  for (int j = 0; j < n_params; j++)
+    if (params[j] < 0 || params[j] >= C->npar || !adj_entry(C, params[j], ent[j]))
+      return fail(GP_ERR_UNSUPPORTED, "gp_adjoint: parameter " + std::to_string(params[j]) +
+                                          " has no pullback (GENROU H, D; SEXS K; ESDC1A KA; TGOV1 R; "
+                                          "IEESGO K1; IEEEST KS are supported)");
+  const int SEG = segment > 0 ? segment : 30;
+  gp_adjoint_stats A{};
+  // factorisations start afresh in every Newton solve, so that a recomputed
+  // segment repeats the forward sweep's iterates exactly
+  PolicyGuard pg(C, GP_REFACTOR_EVERY_ITER);
+  try {
+    const int L = C->ldM, M = C->M, n = C->n, ne = n_params;
+    const size_t nL = (size_t)n * L;
+    int maxsteps = 0;
+    for (int m = 0; m < M; m++) maxsteps = std::max(maxsteps, C->nsteps[m]);
+    HIPCHK(hipMemcpyAsync(C->d_z.p, C->d_z0.p, nL * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
+    Stepper sp(C, method, dt, tol, itermax, maxsteps);   // no fault active at t = 0
+    prepare_method(C, method, dt);
+    analyse_transposed(C, dt);
+    C->d_hist.alloc((size_t)(maxsteps + 1) * C->nrec * L);
+    C->hist_steps = maxsteps + 1;
+    C->traj_len.assign(M, 0);
+    for (int m = 0; m < M; m++) C->traj_len[m] = C->nsteps[m] + 1;
+    C->scen_status.assign(M, GP_OK);
+    C->scen_steps.assign(C->nsteps.begin(), C->nsteps.begin() + M);
+    const int nseg = (maxsteps + SEG - 1) / SEG;
+    C->d_ckpt.ensure((size_t)(nseg + 1) * nL);   // checkpoints, then the final state
+    C->d_seg.ensure((size_t)std::min(SEG, std::max(1, maxsteps)) * nL);
+    C->d_lam.ensure(nL);
+    C->d_gacc.ensure((size_t)std::max(1, ne) * L);
+    C->d_val.ensure(L);
+    C->d_hz.ensure(L);
+    C->d_bact.ensure(L);
+    C->d_objrows.upload(rows);
+    C->d_ent.upload(ent);
+    std::vector<Stepper::State> snaps(std::max(1, nseg));
+    DevTimer total(C->stream);
+    // ---- forward sweep: the state after step s*SEG (and its events) is
+    // checkpoint s; the last segment's states are kept for the backward sweep
+    record_initial(C);
+    const int klast = (nseg - 1) * SEG;
+    int kdone = 0;   // last step any scenario ran
+    for (int k = 1; k <= maxsteps; k++) {
+      if ((k - 1) % SEG == 0) {
+        const int s = (k - 1) / SEG;
+        snaps[s] = sp.st;
+        launch_copy_plain(C->d_z.p, C->d_ckpt.p + (size_t)s * nL, (long long)nL, C->stream);
+      }
+      double *zs = k > klast ? C->d_seg.p + (size_t)(k - klast - 1) * nL : nullptr;
+      if (!sp.step(k, C->d_hist.p + (size_t)k * C->nrec * L, zs)) break;
+      kdone = k;
+    }
+    land_batch(C, true);
+    launch_copy_plain(C->d_z.p, C->d_ckpt.p + (size_t)nseg * nL, (long long)nL, C->stream);
+    const gp_stats fwd = sp.S;
+    A.scenario_steps = fwd.scenario_steps;
+    A.forward_ms = total.ms();
+    // scenarios in the backward sweep: those whose forward sweep finished
+    std::vector<int> status(L);
+    HIPCHK(hipMemcpy(status.data(), C->d_status.p, L * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemset(C->d_status.p, 0, L * sizeof(int)));
+    std::vector<int> hz(L, -1);
+    int nsing = 0;
+    for (int m = 0; m < M; m++) {
+      if (status[m] != 0) {
+        nsing++;
+        C->scen_status[m] = GP_ERR_SINGULAR_PIVOT;
+      }
+      if (C->scen_status[m] == GP_OK) hz[m] = C->nsteps[m];
+      else A.n_excluded++;
+    }
+    HIPCHK(hipMemcpy(C->d_hz.p, hz.data(), L * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(C->d_lam.p, 0, nL * sizeof(double), C->stream));
+    HIPCHK(hipMemsetAsync(C->d_gacc.p, 0, C->d_gacc.n * sizeof(double), C->stream));
+    HIPCHK(hipMemsetAsync(C->d_val.p, 0, L * sizeof(double), C->stream));
+    DevTimer back(C->stream);
+    const SpluDev &JT = C->luJT.dev;
+    for (int s = nseg - 1; s >= 0; s--) {
+      // steps after kdone ran no scenario, so none takes part in their
+      // backward steps either
+      const int k0 = s * SEG, k1 = std::min(k0 + SEG, kdone);
+      if (k1 <= k0) continue;
+      if (s != nseg - 1) {   // recompute the segment from its checkpoint
+        sp.restore(snaps[s]);
+        launch_copy_plain(C->d_ckpt.p + (size_t)s * nL, C->d_z.p, (long long)nL, C->stream);
+        for (int k = k0 + 1; k <= k1; k++)
+          if (!sp.step(k, nullptr, C->d_seg.p + (size_t)(k - k0 - 1) * nL)) break;
+      }
+      int tab = -1;
+      for (int k = k1; k > k0; k--) {
+        // fault tables of forward step k (the "+2" alignment of
+        // test/test_adjoint_event_offset.jl:3-24)
+        if (sp.step_tab[k] != tab) {
+          tab = sp.step_tab[k];
+          HIPCHK(hipMemcpyAsync(C->d_fbus.p, sp.tab_bus[tab].data(), sp.tab_bus[tab].size() * sizeof(int),
+                                hipMemcpyHostToDevice, C->stream));
+          HIPCHK(hipMemcpyAsync(C->d_fg.p, sp.tab_g[tab].data(), sp.tab_g[tab].size() * sizeof(double),
+                                hipMemcpyHostToDevice, C->stream));
+        }
+        const double *zk = C->d_seg.p + (size_t)(k - k0 - 1) * nL;
+        uint8_t *act = C->d_bact.p;
+        launch_adj_mask(L, k, C->d_hz.p, act, C->stream);
+        launch_adj_carry(n, L, C->d_is_diff.p, C->d_lam.p, C->d_f.p, act, C->stream);
+        launch_adj_objective((int)rows.size(), C->d_objrows.p, L, scale * dt, off, zk, C->d_f.p, C->d_val.p, act,
+                             C->stream);
+        launch_jac_mono(C->T, C->d_p.p, zk, dt, faults(C), C->d_jv.p, act, C->stream);
+        launch_splu_factor_impl<double>(JT, C->d_jv.p, nullptr, L, C->d_luJT.p, C->d_svJT.p, C->d_dnJT.p, act,
+                                        nullptr, M, C->d_status.p, C->stream);
+        launch_splu_solve<double>(JT, C->d_svJT.p, C->d_dnJT.p, C->d_f.p, L, 1.0, nullptr, nullptr, C->d_lam.p, act,
+                                  nullptr, M, C->stream);
+        launch_adj_param(C->T, C->d_p.p, ne, C->d_ent.p, zk, C->d_lam.p, dt, C->d_gacc.p, act, C->stream);
+        HIPCHK(hipGetLastError());
+      }
+      for (int k = k0 + 1; k <= k1; k++)
+        for (int m = 0; m < M; m++) A.adjoint_steps += (k <= hz[m]) ? 1 : 0;
+      // the next segment's forward recomputation uploads its own tables
+    }
+    launch_copy_plain(C->d_ckpt.p + (size_t)nseg * nL, C->d_z.p, (long long)nL, C->stream);
+    A.backward_ms = back.ms();
+    A.elapsed_ms = total.ms();
+    A.n_segments = nseg;
+    A.newton_iters = sp.S.newton_iters;
+    // outputs
+    std::vector<double> gacc((size_t)std::max(1, ne) * L), val(L), lam;
+    HIPCHK(hipMemcpy(gacc.data(), C->d_gacc.p, gacc.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(val.data(), C->d_val.p, L * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(status.data(), C->d_status.p, L * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemset(C->d_status.p, 0, L * sizeof(int)));
+    for (int m = 0; m < M; m++)
+      if (status[m] != 0 && hz[m] >= 0) {   // a zero pivot of J^T
+        nsing++;
+        C->scen_status[m] = GP_ERR_SINGULAR_PIVOT;
+        hz[m] = -1;
+        A.n_excluded++;
+      }
+    for (int j = 0; j < ne; j++) {
+      double acc = 0.0;
+      for (int m = 0; m < M; m++) acc += hz[m] >= 0 ? gacc[(size_t)j * L + m] : 0.0;
+      grad[j] = acc;
+    }
+    if (grad_scen)
+      for (int m = 0; m < M; m++)
+        for (int j = 0; j < ne; j++) grad_scen[(size_t)m * ne + j] = hz[m] >= 0 ? gacc[(size_t)j * L + m] : 0.0;
+    if (value)
+      for (int m = 0; m < M; m++) value[m] = hz[m] >= 0 ? val[m] : 0.0;
+    if (dz0) {
+      lam.resize(nL);
+      HIPCHK(hipMemcpy(lam.data(), C->d_lam.p, nL * sizeof(double), hipMemcpyDeviceToHost));
+      for (int m = 0; m < M; m++)
+        for (int i = 0; i < n; i++)
+          dz0[(size_t)m * n + i] = (hz[m] >= 0 && C->is_diff[i]) ? lam[(size_t)i * L + m] : 0.0;
+    }
+    if (ast) *ast = A;
+    if (nsing) return fail(GP_ERR_SINGULAR_PIVOT, std::to_string(nsing) + " scenario(s) hit a zero pivot");
+    if (fwd.n_nan) return fail(GP_ERR_NAN, std::to_string(fwd.n_nan) + " scenario(s) produced a non-finite residual");
+    if (A.n_excluded)
+      return fail(GP_ERR_NEWTON_NOT_CONVERGED, std::to_string(A.n_excluded) +
+                                                   " scenario(s) had a Newton solve that did not converge and "
+                                                   "were left out of the gradient");
+  } catch (const GpError &e) {
+    try {
+      land_batch(C, true);
+    } catch (const GpError &) {
+      C->inflight = false;
+    }
+    if (ast) *ast = A;
     return fail(e.code, e.msg);
   }
   return GP_OK;

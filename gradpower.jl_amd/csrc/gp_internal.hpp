@@ -89,6 +89,12 @@ constexpr int kResCB = 24, kResCX = 8;
 // staging of the triangular solve, per thread: blob dwords, values, diagonals
 constexpr int kSolCB = 8, kSolCV = 8, kSolCD = 4;
 
+// Parameters with a hand-written pullback in the discrete adjoint.
+enum AdjKind {
+  GP_ADJ_GENROU_H = 0, GP_ADJ_GENROU_D, GP_ADJ_SEXS_K, GP_ADJ_ESDC1A_KA, GP_ADJ_TGOV1_R, GP_ADJ_IEESGO_K1,
+  GP_ADJ_IEEEST_KS
+};
+
 // ---- kernel launchers (gp_kernels.hip) ----------------------------------
 void launch_cluster_residual(const DevTables &T, const double *p, const double *z, const double *zold,
                              double dt, double *f, double *inj, const uint8_t *work, hipStream_t s);
@@ -122,6 +128,15 @@ void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const doubl
                        const int *scatter, double *zupd, double *xout, const uint8_t *work, const uint8_t *cur, int M,
                        hipStream_t s);
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s);
+// discrete adjoint: backward-step mask, carried multipliers, objective rows,
+// parameter pullbacks
+void launch_adj_mask(int ldM, int k, const int *hz, uint8_t *act, hipStream_t s);
+void launch_adj_carry(int n, int ldM, const uint8_t *is_diff, const double *lam, double *rhs, const uint8_t *act,
+                      hipStream_t s);
+void launch_adj_objective(int nr, const int *rows, int ldM, double w, double off, const double *z, double *rhs,
+                          double *val, const uint8_t *act, hipStream_t s);
+void launch_adj_param(const DevTables &T, const double *p, int ne, const int2 *ent, const double *z,
+                      const double *lam, double dt, double *g, const uint8_t *act, hipStream_t s);
 void launch_gather_traj(int nrec, int len, int ldM, int m, const double *hist, double *out, hipStream_t s);
 void launch_copy_plain(const double *src, double *dst, long long n, hipStream_t s);
 void launch_record_impl(int nrec, const int *idx, int ldM, int M, const double *z, double *hist,

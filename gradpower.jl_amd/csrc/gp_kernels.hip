@@ -2135,6 +2135,126 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   }
 }
 
+// ---------------------------------------------------------------- discrete adjoint
+// Backward step k of the adjoint of the backward-Euler sweep
+//   F_k(z_k, z_{k-1}, p) = 0:  z - z_{k-1} - dt f(z, p) on differential rows,
+//                              g(z, p) on algebraic rows,
+// objective  J = sum_k w sum_r (z_k[r] - off)^2  (w = scale * dt):
+//   (dF_k/dz_k)^T lam_k = dJ/dz_k + lam_{k+1} restricted to differential rows
+//   dJ/dp -= lam_k^T dF_k/dp.
+// Scenario m takes part in backward step k when k <= hz[m] (its horizon; -1
+// for a scenario left out).
+__global__ void k_adj_mask(int ldM, int k, const int *__restrict__ hz, uint8_t *__restrict__ act) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < ldM) act[m] = k <= hz[m] ? 1 : 0;
+}
+
+// rhs = lam_{k+1} on the differential rows, 0 on the algebraic rows
+__global__ void k_adj_carry(int n, int ldM, const uint8_t *__restrict__ is_diff, const double *__restrict__ lam,
+                            double *__restrict__ rhs, const uint8_t *__restrict__ act) {
+  const int m = blockIdx.y * WAVE + threadIdx.x;
+  if (!act[m]) return;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const size_t o = (size_t)i * ldM + m;
+    rhs[o] = is_diff[i] ? lam[o] : 0.0;
+  }
+}
+
+// Objective rows at z_k: rhs += 2 w (z - off), val += w sum (z - off)^2,
+// rows in list order (one thread per scenario, a fixed summation order).
+__global__ void k_adj_objective(int nr, const int *__restrict__ rows, int ldM, double w, double off,
+                                const double *__restrict__ z, double *__restrict__ rhs, double *__restrict__ val,
+                                const uint8_t *__restrict__ act) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= ldM || !act[m]) return;
+  double s = 0.0;
+  for (int r = 0; r < nr; r++) {
+    const size_t o = (size_t)rows[r] * ldM + m;
+    const double e = z[o] - off;
+    s += e * e;
+    rhs[o] += 2.0 * w * e;
+  }
+  val[m] += w * s;
+}
+
+// Parameter pullbacks.  Entry e = (kind, device row); g[e][m] -= lam . dF/dp
+// at z_k, from the device equations above:
+//   GENROU H  (P[6]):  F_w = w - w_old - dt N/(2H)            dF_w/dH  =  dt f_w / H
+//   GENROU D  (P[7]):                                        dF_w/dD  =  dt w / (2H)
+//   SEXS K    (P[2]):  f_1 = (-x1 + K(x0 + TA/TB err))/TE    dF_1/dK  = -dt (x0 + TA/TB err) / TE
+//   ESDC1A KA (P[0]):  f_0 = (KA u - x0)/TA                  dF_0/dKA = -dt u / TA
+//   TGOV1 R   (P[0]):  f_1 = ((pref - w)/R - x1)/T1          dF_1/dR  =  dt (pref - w) / (R^2 T1)
+//   IEESGO K1 (P[6]):  f_0 = (K1 w - PF0)/T1                 dF_0/dK1 = -dt w / T1
+//   IEEEST KS (P[12]): f_6 = (KS y4 - s6)/T6, g = vs - T5/T6 (KS y4 - s6)
+//                      dF_6/dKS = -dt y4 / T6,  dg/dKS = -(T5/T6) y4
+__global__ void __launch_bounds__(64) k_adj_param(DevTables T, const double *__restrict__ p, const int2 *__restrict__ ent,
+                                                  const double *__restrict__ z, const double *__restrict__ lam,
+                                                  double dt, double *__restrict__ g, const uint8_t *__restrict__ act) {
+  const int e = blockIdx.x;
+  const int m = lane_scenario();
+  if (m >= T.M || !act[m]) return;
+  const int ldM = T.ldM;
+  const int kind = ent[e].x, row = ent[e].y;
+  auto Z = [&](int i) { return z[(size_t)i * ldM + m]; };
+  auto LAM = [&](int i) { return lam[(size_t)i * ldM + m]; };
+  double acc = 0.0;
+  if (kind == GP_ADJ_GENROU_H || kind == GP_ADJ_GENROU_D) {
+    const int *gr = T.genrou + row * GP_COLS_GENROU;
+    const int dp = gr[0], ap = gr[1], bus = gr[3], pm = gr[4], efd = gr[5];
+    const double *P = p + gr[2];
+    const double H = P[6], w = Z(dp + 4);
+    if (kind == GP_ADJ_GENROU_D) {
+      acc = LAM(dp + 4) * dt * w / (2.0 * H);
+    } else {
+      double zs[10], fr[10], ir, ii;
+#pragma unroll
+      for (int i = 0; i < 6; i++) zs[i] = Z(dp + i);
+#pragma unroll
+      for (int i = 0; i < 4; i++) zs[6 + i] = Z(ap + i);
+      const int vrz = T.net_ptr + 2 * bus;
+      const double e_fd = efd >= 0 ? Z(efd) : T.u0[2 * row];
+      const double p_m = pm >= 0 ? Z(pm) : T.u0[2 * row + 1];
+      genrou_residual(P, zs, e_fd, p_m, Z(vrz), Z(vrz + 1), fr, ir, ii);
+      acc = LAM(dp + 4) * dt * fr[4] / H;
+    }
+  } else if (kind == GP_ADJ_SEXS_K || kind == GP_ADJ_ESDC1A_KA) {
+    const int *d = kind == GP_ADJ_SEXS_K ? T.sexs + row * GP_COLS_SEXS : T.esdc1a + row * GP_COLS_ESDC1A;
+    const double *P = p + d[1];
+    const double vr = Z(d[2]), vi = Z(d[2] + 1), vs = d[3] >= 0 ? Z(d[3]) : 0.0;
+    double vm = sqrt(vr * vr + vi * vi);
+    if (vm == 0.0) vm = 1e-12;
+    if (kind == GP_ADJ_SEXS_K) {
+      const double err = P[6] - vm + vs;
+      acc = -LAM(d[0] + 1) * dt * (Z(d[0]) + P[0] * err) / P[3];
+    } else {
+      const double u = P[9] - vm + vs - Z(d[0] + 1) - (P[2] / P[3]) * Z(d[0] + 2);
+      acc = -LAM(d[0]) * dt * u / P[1];
+    }
+  } else if (kind == GP_ADJ_TGOV1_R) {
+    const int *d = T.tgov1 + row * GP_COLS_TGOV1;
+    const double *P = p + d[2];
+    const double w = d[3] >= 0 ? Z(d[3]) : 0.0, R = P[0];
+    acc = LAM(d[0] + 1) * dt * (P[7] - w) / (R * R * P[1]);
+  } else if (kind == GP_ADJ_IEESGO_K1) {
+    const int *d = T.ieesgo + row * GP_COLS_IEESGO;
+    const double *P = p + d[2];
+    const double w = d[3] >= 0 ? Z(d[3]) : 0.0;
+    acc = -LAM(d[0]) * dt * w / P[0];
+  } else if (kind == GP_ADJ_IEEEST_KS) {
+    const int *d = T.ieeest + row * GP_COLS_IEEEST;
+    const double *P = p + d[2];
+    const double A3 = P[2], A4 = P[3], A5 = P[4], A6 = P[5];
+    const double T1 = P[6], T2 = P[7], T3 = P[8], T4 = P[9], T5 = P[10], T6 = P[11];
+    const double s1 = Z(d[0] + 1), s2 = Z(d[0] + 2), s3 = Z(d[0] + 3), s4 = Z(d[0] + 4), s5 = Z(d[0] + 5);
+    const double qq = s1 - s3 - A3 * s2;
+    const double y2 = s3 + A5 * s2 + (A6 / A4) * qq;
+    const double y3 = s4 + (T1 / T2) * (y2 - s4);
+    const double y4 = s5 + (T3 / T4) * (y3 - s5);
+    acc = -LAM(d[0] + 6) * dt * y4 / T6 - LAM(d[1]) * (T5 / T6) * y4;
+  }
+  g[(size_t)e * ldM + m] -= acc;
+}
+
 // ---------------------------------------------------------------- small utilities
 __global__ void k_copy_active(int n, int ldM, const double *__restrict__ src, double *__restrict__ dst,
                               const uint8_t *__restrict__ act) {
@@ -2240,6 +2360,23 @@ void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const doubl
                               hipFuncAttributeMaxDynamicSharedMemorySize, S.lds_solve);
   hipLaunchKernelGGL(k_splu_solve<TF>, dim3(M), dim3(S.threads), S.lds_solve, s, S, sv, dn, rhs, ldM, sign, scatter,
                      zupd, xout, work, cur);
+}
+void launch_adj_mask(int ldM, int k, const int *hz, uint8_t *act, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_mask, dim3((ldM + 255) / 256), dim3(256), 0, s, ldM, k, hz, act);
+}
+void launch_adj_carry(int n, int ldM, const uint8_t *is_diff, const double *lam, double *rhs, const uint8_t *act,
+                      hipStream_t s) {
+  const int gx = n < 1024 ? n : 1024;
+  hipLaunchKernelGGL(k_adj_carry, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, n, ldM, is_diff, lam, rhs, act);
+}
+void launch_adj_objective(int nr, const int *rows, int ldM, double w, double off, const double *z, double *rhs,
+                          double *val, const uint8_t *act, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_objective, dim3((ldM + 63) / 64), dim3(64), 0, s, nr, rows, ldM, w, off, z, rhs, val, act);
+}
+void launch_adj_param(const DevTables &T, const double *p, int ne, const int2 *ent, const double *z,
+                      const double *lam, double dt, double *g, const uint8_t *act, hipStream_t s) {
+  if (ne > 0)
+    hipLaunchKernelGGL(k_adj_param, lane_grid(ne, T.ldM), dim3(WAVE), 0, s, T, p, ent, z, lam, dt, g, act);
 }
 void launch_copy_plain(const double *src, double *dst, long long n, hipStream_t s) {
   if (n <= 0) return;

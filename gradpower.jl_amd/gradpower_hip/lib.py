@@ -59,6 +59,17 @@ class CSymInfo(C.Structure):
                 ("w_total_nt", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
+class CObjective(C.Structure):
+    _fields_ = [("n_rows", C.c_int32), ("rows", C.POINTER(C.c_int32)), ("scale", C.c_double),
+                ("offset", C.c_double)]
+
+
+class CAdjStats(C.Structure):
+    _fields_ = [("elapsed_ms", C.c_double), ("forward_ms", C.c_double), ("backward_ms", C.c_double),
+                ("scenario_steps", C.c_int64), ("adjoint_steps", C.c_int64), ("n_segments", C.c_int32),
+                ("n_excluded", C.c_int32), ("newton_iters", C.c_int64), ("reserved", C.c_int64 * 4)]
+
+
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 _up = C.POINTER(C.c_uint8)
@@ -67,7 +78,7 @@ _LIB = None
 EXPORTS = ["gp_create", "gp_destroy", "gp_set_state", "gp_set_events", "gp_integrate",
            "gp_get_trajectory", "gp_get_state", "gp_eval_residual", "gp_jacobian_pattern",
            "gp_eval_jacobian", "gp_newton_step", "gp_get_symbolic_info", "gp_get_scenario_status",
-           "gp_copy_kernel", "gp_last_error", "gp_version"]
+           "gp_copy_kernel", "gp_adjoint", "gp_last_error", "gp_version"]
 
 
 def load():
@@ -94,6 +105,8 @@ def load():
     lib.gp_get_scenario_status.argtypes = [C.c_void_p, _ip, _ip]
     lib.gp_copy_kernel.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
     lib.gp_newton_step.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, C.c_double, _up, _dp]
+    lib.gp_adjoint.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                               C.c_void_p, C.c_int32, _ip, _dp, _dp, _dp, _dp, C.c_void_p]
     lib.gp_last_error.restype = C.c_char_p
     lib.gp_version.restype = C.c_char_p
     _LIB = lib
@@ -240,3 +253,31 @@ class HipContext:
         info = CSymInfo()
         self._check(self.lib.gp_get_symbolic_info(self.h, C.byref(info)))
         return {f: getattr(info, f) for f, _ in CSymInfo._fields_ if f != "reserved"}
+
+    def adjoint(self, method, dt, params, tol=1e-10, itermax=50, segment=30, rows=None, scale=1.0,
+                offset=0.0, want_dz0=False, raise_on_fail=True):
+        """Objective and gradient by the discrete adjoint (gp_adjoint).
+
+        Returns a dict: value (total), value_scen [M], grad [len(params)],
+        grad_scen [M x len(params)], dz0 [M x n] (when want_dz0), stats, status."""
+        par = np.ascontiguousarray(params, np.int32)
+        obj = CObjective()
+        if rows is not None:
+            self._objrows = np.ascontiguousarray(rows, np.int32)
+            obj.n_rows = len(self._objrows)
+            obj.rows = _i(self._objrows)
+        obj.scale = float(scale)
+        obj.offset = float(offset)
+        val = np.zeros(self.M)
+        grad = np.zeros(max(1, len(par)))
+        gs = np.zeros((self.M, max(1, len(par))))
+        dz0 = np.zeros((self.M, self.n)) if want_dz0 else None
+        st = CAdjStats()
+        rc = self.lib.gp_adjoint(self.h, int(method), float(dt), float(tol), int(itermax), int(segment),
+                                 C.byref(obj), len(par), _i(par) if len(par) else None, _d(val), _d(grad),
+                                 _d(gs), _d(dz0) if want_dz0 else None, C.byref(st))
+        stats = {f: getattr(st, f) for f, _ in CAdjStats._fields_ if f != "reserved"}
+        if rc != GP_OK and (raise_on_fail or rc in (GP_ERR_BADARG, GP_ERR_HIP, GP_ERR_OOM, GP_ERR_UNSUPPORTED)):
+            raise GpError(rc, self.lib.gp_last_error().decode(), stats)
+        return {"value": float(val.sum()), "value_scen": val, "grad": grad[:len(par)],
+                "grad_scen": gs[:, :len(par)], "dz0": dz0, "stats": stats, "status": rc}

@@ -238,6 +238,57 @@ int gp_eval_jacobian(gp_ctx *ctx, const double *z, double dt,
 int gp_newton_step(gp_ctx *ctx, int32_t method, const double *z, const double *zold,
                    double dt, const uint8_t *fault_on, double *dz);
 
+/* ---- discrete adjoint ------------------------------------------------------
+ * Trajectory objective: the reference's `functional` (src/dynamics.jl:330-337,
+ * sum of squared GENROU speeds) summed over the steps with weight dt as in
+ * test/test_adjoint_event_offset.jl:58-69, and over the scenarios:
+ *   J = scale * sum_m sum_{k=1..nsteps[m]} dt * sum_r (z_k[rows[r]] - offset)^2
+ * where z_k is the converged state of step k (before the step's event
+ * re-solve).  rows == NULL: the GENROU speed deviations (src/utils.jl:46-59);
+ * scale == 0: 1 (the reference's sum); 1/n_rows gives the mean. */
+typedef struct gp_objective {
+  int32_t n_rows;
+  const int32_t *rows;
+  double scale;
+  double offset;
+} gp_objective;
+
+typedef struct gp_adjoint_stats {
+  double elapsed_ms;        /* device time of the whole call                      */
+  double forward_ms;        /* the checkpointed forward sweep                      */
+  double backward_ms;       /* segment recomputation and the backward sweep       */
+  int64_t scenario_steps;   /* forward scenario-steps                              */
+  int64_t adjoint_steps;    /* backward scenario-steps (transposed solves)         */
+  int32_t n_segments;       /* checkpoint segments                                 */
+  int32_t n_excluded;       /* scenarios left out (their forward sweep failed)    */
+  int64_t newton_iters;     /* Newton corrections, recomputation included         */
+  int64_t reserved[4];
+} gp_adjoint_stats;
+
+/* Gradient of J w.r.t. the parameters p[params[j]] and the initial state by
+ * the discrete adjoint of the backward-Euler sweep, the initial state held
+ * fixed when p moves.  Replaces GradPower.adjoint(λ0, dprob, sys, traj, tvec;
+ * functional=true) -> (λ, μ, λ_traj) of examples/adjoint.jl:34-51,106, a
+ * function the reference snapshot calls but no longer contains.
+ * The forward sweep (method, dt, tol, itermax as in gp_integrate) starts from
+ * the state of the last gp_set_state with the events of gp_set_events, keeps
+ * a checkpoint every `segment` steps (0: 30) and recomputes each segment
+ * before its backward pass.  Backward step k solves
+ *   (dF_k/dz_k)^T lam_k = dJ/dz_k + lam_{k+1} on the differential rows
+ * with the Jacobian of forward step k (its fault state: the alignment of
+ * test/test_adjoint_event_offset.jl:3-24) and accumulates
+ * dJ/dp -= lam_k^T dF_k/dp.  Parameters with a pullback: GENROU H, D; SEXS K;
+ * ESDC1A KA; TGOV1 R; IEESGO K1; IEEEST KS (others: GP_ERR_UNSUPPORTED).
+ * Outputs (all but grad may be NULL): value[m] objective of scenario m,
+ * grad[j] = dJ/dp[params[j]], grad_scen[m*n_params + j] scenario m's share,
+ * dz0[m*n + i] = dJ/dz0 of scenario m.  A scenario whose forward sweep fails
+ * is left out (zero outputs) and the call returns that failure's status
+ * after filling the outputs.  Afterwards gp_get_trajectory and gp_get_state
+ * give the forward sweep's trajectory and final state. */
+int gp_adjoint(gp_ctx *ctx, int32_t method, double dt, double tol, int32_t itermax, int32_t segment,
+               const gp_objective *obj, int32_t n_params, const int32_t *params, double *value,
+               double *grad, double *grad_scen, double *dz0, gp_adjoint_stats *stats);
+
 /* Sizes of the symbolic analyses (for the roofline byte counts). */
 typedef struct gp_symbolic_info {
   int32_t n, nnz_j, nnz_lu_j, levels_j;

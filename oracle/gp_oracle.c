@@ -14,6 +14,10 @@
  *   backward Euler            src/dynamics.jl:339-500
  *   Newton                    src/numerics.jl:55-176
  *   time / event loop         src/dynamics.jl:502-655
+ *   discrete adjoint          examples/adjoint.jl:34-51,106 and
+ *                             test/test_adjoint_event_offset.jl:3-24 (the
+ *                             adjoint itself is absent from the reference
+ *                             snapshot: pinned by central differences)
  *   Schur Newton step         src/schur.jl:195-493
  * The sparse LU plays the role of KLU (an external dependency of the
  * reference): fill-reducing minimum-degree column order on A+A^T, then a
@@ -807,11 +811,12 @@ static int newton(const orc_sys *s, double *z, const double *zold, const double 
 /* integrate! (src/dynamics.jl:502-655) for one scenario.
  * events: n_ev rows of (bus, r_fault, step_on, step_off).
  * traj: (nsteps+1) x n, row k = state after step k (row 0 = z0), or NULL.
+ * fgs: (nsteps+1) x nbus, row k = fault conductances in force during step k
+ * (row 0 unused), or NULL.
  * stats[0] = Newton iterations, stats[1] = non-converged solves, stats[2] = max accepted |F|. */
-int orc_integrate(void *h, const double *z0, const double *p, double dt, int nsteps,
-                  const int *ev_bus, const double *ev_r, const int *ev_on, const int *ev_off, int n_ev,
-                  double tol, int itermax, double *traj, double *zfinal, double *stats) {
-  orc_sys *s = h;
+static int sweep(orc_sys *s, const double *z0, const double *p, double dt, int nsteps,
+                 const int *ev_bus, const double *ev_r, const int *ev_on, const int *ev_off, int n_ev,
+                 double tol, int itermax, double *traj, double *fgs, double *zfinal, double *stats) {
   int n = s->n;
   double *z = xmalloc(n * sizeof(double)), *zold = xmalloc(n * sizeof(double));
   double *f = xmalloc(n * sizeof(double)), *dx = xmalloc(n * sizeof(double)), *w = xmalloc(n * sizeof(double));
@@ -826,6 +831,7 @@ int orc_integrate(void *h, const double *z0, const double *p, double dt, int nst
     memcpy(zold, z, n * sizeof(double));
     for (int b = 0; b < s->nbus; b++) fg[b] = 0.0;
     for (int e = 0; e < n_ev; e++) if (active[e]) fg[ev_bus[e]] += 1.0 / ev_r[e];
+    if (fgs) memcpy(fgs + (size_t)k * s->nbus, fg, s->nbus * sizeof(double));
     int it = newton(s, z, zold, p, fg, dt, tol, itermax, f, J, dx, w, &F, &fn);
     if (it < 0) { fails++; if (it <= -2) { rc = it; } } else iters += it;
     if (fn > maxf) maxf = fn;
@@ -849,6 +855,107 @@ int orc_integrate(void *h, const double *z0, const double *p, double dt, int nst
   if (stats) { stats[0] = iters; stats[1] = fails; stats[2] = maxf; }
   lu_free(&F);
   free(z); free(zold); free(f); free(dx); free(w); free(J); free(fg); free(active);
+  return rc;
+}
+
+int orc_integrate(void *h, const double *z0, const double *p, double dt, int nsteps,
+                  const int *ev_bus, const double *ev_r, const int *ev_on, const int *ev_off, int n_ev,
+                  double tol, int itermax, double *traj, double *zfinal, double *stats) {
+  return sweep((orc_sys *)h, z0, p, dt, nsteps, ev_bus, ev_r, ev_on, ev_off, n_ev, tol, itermax, traj, NULL,
+               zfinal, stats);
+}
+
+/* y = A^{-T} c with the factorisation of A (P A Q = L U): Q U^T L^T P y = c,
+ * so v = Q^T c, U^T u = v (forward), L^T t = u (backward), y = P^T t. */
+static void lu_solve_t(const orc_lu *F, const int *q, const double *c, double *y, double *w) {
+  int n = F->n;
+  for (int k = 0; k < n; k++) w[k] = c[q[k]];
+  for (int k = 0; k < n; k++) {           /* U^T: row k of U^T is column k of U */
+    int pd = F->up[k + 1] - 1;
+    double t = w[k];
+    for (int p = F->up[k]; p < pd; p++) t -= F->ux[p] * w[F->ui[p]];
+    w[k] = t / F->ux[pd];
+  }
+  for (int k = n - 1; k >= 0; k--) {      /* L^T, unit diagonal */
+    double t = w[k];
+    for (int p = F->lp[k]; p < F->lp[k + 1]; p++) t -= F->lx[p] * w[F->li[p]];
+    w[k] = t;
+  }
+  for (int i = 0; i < n; i++) y[i] = w[F->pinv[i]];
+}
+
+/* Solve J^T y = c with the oracle's sparse LU (test hook). Returns 0 on success. */
+int orc_linsolve_t(void *h, const double *J, const double *c, double *y) {
+  orc_sys *s = h;
+  orc_lu F = {0};
+  if (lu_factor(s->n, s->cp, s->ri, J, s->q, &F)) return -1;
+  double *w = xmalloc(s->n * sizeof(double));
+  lu_solve_t(&F, s->q, c, y, w);
+  free(w); lu_free(&F);
+  return 0;
+}
+
+/* Discrete adjoint of the backward-Euler sweep of one scenario: the
+ * GradPower.adjoint of examples/adjoint.jl:34-51,106 (called there, absent
+ * from the reference snapshot), z0 held fixed when p moves.
+ * Objective  J = scale sum_{k=1..nsteps} dt sum_r (z_k[rows[r]] - off)^2
+ * over the converged states z_k of the forward sweep (traj rows 1..nsteps,
+ * test/test_adjoint_event_offset.jl:58-69).  For k = nsteps..1, with the
+ * Jacobian and fault state of forward step k:
+ *   J_k^T lam_k = dJ/dz_k + lam_{k+1} on the differential rows
+ *   grad_j     -= lam_k . dF_k/dp_j
+ * where dF_k/dp_j is a central difference of the residual in p_j (step
+ * 1e-6 |p_j|, or 1e-6 when p_j = 0; the zold terms cancel).  dz0 = lam_1 on the
+ * differential rows.  fault_shift (0 for the adjoint proper) takes the fault
+ * state of step k + fault_shift instead: a planted misalignment for tests.
+ * Returns the forward sweep's status. */
+int orc_adjoint(void *h, const double *z0, const double *p, double dt, int nsteps,
+                const int *ev_bus, const double *ev_r, const int *ev_on, const int *ev_off, int n_ev,
+                double tol, int itermax, int n_rows, const int *rows, double scale, double off,
+                int n_par, const int *par, int fault_shift, double *value, double *grad, double *dz0) {
+  orc_sys *s = h;
+  int n = s->n, nb = s->nbus;
+  double *traj = xmalloc((size_t)(nsteps + 1) * n * sizeof(double));
+  double *fgs = xmalloc((size_t)(nsteps + 1) * nb * sizeof(double));
+  double st[3];
+  int rc = sweep(s, z0, p, dt, nsteps, ev_bus, ev_r, ev_on, ev_off, n_ev, tol, itermax, traj, fgs, NULL, st);
+  if (rc) { free(traj); free(fgs); return rc; }
+  double *lam = xmalloc(n * sizeof(double)), *a = xmalloc(n * sizeof(double)), *w = xmalloc(n * sizeof(double));
+  double *J = xmalloc(s->nnz * sizeof(double)), *fp = xmalloc(n * sizeof(double)), *fm = xmalloc(n * sizeof(double));
+  double *pp = xmalloc((s->npar + 1) * sizeof(double));
+  memcpy(pp, p, s->npar * sizeof(double));
+  orc_lu F = {0};
+  double val = 0.0;
+  for (int k = 1; k <= nsteps; k++)
+    for (int r = 0; r < n_rows; r++) { double e = traj[(size_t)k * n + rows[r]] - off; val += scale * dt * e * e; }
+  for (int j = 0; j < n_par; j++) grad[j] = 0.0;
+  for (int i = 0; i < n; i++) lam[i] = 0.0;
+  for (int k = nsteps; k >= 1; k--) {
+    int kf = k + fault_shift;   /* fault_shift != 0 only plants a misalignment in tests */
+    kf = kf < 1 ? 1 : (kf > nsteps ? nsteps : kf);
+    const double *zk = traj + (size_t)k * n, *zkm = traj + (size_t)(k - 1) * n, *fg = fgs + (size_t)kf * nb;
+    for (int i = 0; i < n; i++) a[i] = s->is_diff[i] ? lam[i] : 0.0;
+    for (int r = 0; r < n_rows; r++) a[rows[r]] += 2.0 * scale * dt * (zk[rows[r]] - off);
+    beuler_jac(s, zk, p, fg, dt, J);
+    if (lu_factor(n, s->cp, s->ri, J, s->q, &F)) { rc = -2; break; }
+    lu_solve_t(&F, s->q, a, lam, w);
+    for (int j = 0; j < n_par; j++) {
+      int q = par[j];
+      double hq = p[q] != 0.0 ? 1e-6 * fabs(p[q]) : 1e-6;
+      pp[q] = p[q] + hq;
+      beuler(s, zk, zkm, pp, fg, dt, fp);
+      pp[q] = p[q] - hq;
+      beuler(s, zk, zkm, pp, fg, dt, fm);
+      pp[q] = p[q];
+      double g = 0.0;
+      for (int i = 0; i < n; i++) g += lam[i] * (fp[i] - fm[i]) / (2.0 * hq);
+      grad[j] -= g;
+    }
+  }
+  if (value) *value = val;
+  if (dz0) for (int i = 0; i < n; i++) dz0[i] = s->is_diff[i] ? lam[i] : 0.0;
+  lu_free(&F);
+  free(traj); free(fgs); free(lam); free(a); free(w); free(J); free(fp); free(fm); free(pp);
   return rc;
 }
 

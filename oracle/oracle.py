@@ -38,6 +38,10 @@ def _load():
     lib.orc_integrate.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, _ip, _dp, _ip, _ip,
                                   C.c_int, C.c_double, C.c_int, _dp, _dp, _dp]
     lib.orc_schur_step.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    lib.orc_linsolve_t.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    lib.orc_adjoint.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, _ip, _dp, _ip, _ip, C.c_int,
+                                C.c_double, C.c_int, C.c_int, _ip, C.c_double, C.c_double, C.c_int, _ip,
+                                C.c_int, _dp, _dp, _dp]
     return lib
 
 
@@ -135,3 +139,48 @@ class Oracle:
         rc = lib().orc_integrate(self.h, _d(z0), _d(p), dt, nsteps, _i(eb), _d(er), _i(eon), _i(eoff),
                                  len(ev), tol, itermax, _d(tr) if traj else None, _d(zf), _d(st))
         return tr, zf, {"rc": rc, "newton_iters": st[0], "not_converged": st[1], "max_residual": st[2]}
+
+    def linsolve_t(self, Jvals, c):
+        """y with J^T y = c."""
+        y = np.zeros(self.n)
+        rc = lib().orc_linsolve_t(self.h, _d(np.ascontiguousarray(Jvals)), _d(np.ascontiguousarray(c)), _d(y))
+        if rc:
+            raise RuntimeError("singular matrix")
+        return y
+
+    def adjoint(self, dt, nsteps, params, events=(), rows=None, scale=1.0, offset=0.0, z0=None, tol=1e-10,
+                itermax=50, p=None, fault_shift=0):
+        """Objective scale * sum_k dt sum_r (z_k[r] - offset)^2 and its gradient
+        w.r.t. p[params] by the oracle's discrete adjoint.  rows=None: the
+        GENROU speed deviations.  fault_shift != 0 plants a misaligned fault
+        state (tests only).  Returns (value, grad, dz0)."""
+        from gradpower_hip.desc import gen_speed_indices
+        z0 = np.ascontiguousarray(self.desc.z0 if z0 is None else z0, np.float64)
+        p = self.p if p is None else np.ascontiguousarray(p, np.float64)
+        rows = np.ascontiguousarray(gen_speed_indices(self.desc) if rows is None else rows, np.int32)
+        par = np.ascontiguousarray(params, np.int32)
+        ev = list(events)
+        eb = np.array([e[0] for e in ev] or [0], np.int32)
+        er = np.array([e[1] for e in ev] or [1.0], np.float64)
+        eon = np.array([e[2] for e in ev] or [0], np.int32)
+        eoff = np.array([e[3] for e in ev] or [0], np.int32)
+        val = C.c_double()
+        grad = np.zeros(max(1, len(par)))
+        dz0 = np.zeros(self.n)
+        rc = lib().orc_adjoint(self.h, _d(z0), _d(p), dt, nsteps, _i(eb), _d(er), _i(eon), _i(eoff), len(ev),
+                               tol, itermax, len(rows), _i(rows), scale, offset, len(par), _i(par),
+                               int(fault_shift), C.byref(val), _d(grad), _d(dz0))
+        if rc:
+            raise RuntimeError(f"oracle forward sweep failed ({rc})")
+        return val.value, grad[:len(par)], dz0
+
+    def objective(self, dt, nsteps, events=(), rows=None, scale=1.0, offset=0.0, z0=None, tol=1e-10,
+                  itermax=50, p=None):
+        """scale * sum_{k>=1} dt sum_r (z_k[r] - offset)^2 along the oracle's trajectory."""
+        from gradpower_hip.desc import gen_speed_indices
+        rows = gen_speed_indices(self.desc) if rows is None else np.asarray(rows)
+        tr, _, st = self.integrate(dt, nsteps, events, z0=z0, tol=tol, itermax=itermax, p=p)
+        if st["rc"]:
+            raise RuntimeError(f"oracle forward sweep failed ({st['rc']})")
+        e = tr[1:, rows] - offset
+        return scale * dt * float((e * e).sum())
