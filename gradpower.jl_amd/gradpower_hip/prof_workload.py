@@ -1,7 +1,8 @@
 """Workload driver for hardware-counter and kernel-trace runs (rocprofv3):
 a plain copy kernel of a known size (to calibrate the byte counters) and
 one pass of the 2,000-bus M=256 contingency workload, optionally with the
-scenario horizons capped so that a counter run stays short.
+scenario horizons capped so that a counter run stays short; or (--adjoint)
+one discrete-adjoint pass of the 200-bus M=128 3 s workload.
 
     rocprofv3 --kernel-trace --stats --output-format csv -d OUT -o run -- \\
         python3 -m gradpower_hip.prof_workload --horizon-cap 0.5
@@ -23,7 +24,11 @@ def main():
     ap.add_argument("--horizon-cap", type=float, default=0.5)
     ap.add_argument("--method", default="schur32", choices=["schur32", "schur64", "mono"])
     ap.add_argument("--passes", type=int, default=1)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="one gp_adjoint pass of the 200-bus, M=128, 3 s workload instead")
     args = ap.parse_args()
+    if args.adjoint:
+        return adjoint(args.passes)
     import ctypes as C
     from gradpower_hip import synth
     from gradpower_hip.lib import (HipContext, METHOD_MONO_FP64, METHOD_SCHUR_FP32, METHOD_SCHUR_FP64,
@@ -55,6 +60,26 @@ def main():
         st = ctx.integrate(meth, case.dt, 1e-10, 30, raise_on_fail=False)
         print(f"pass {it}: {st['scenario_steps']} steps, {st['newton_rounds']} rounds, "
               f"{st['elapsed_ms']:.1f} ms", flush=True)
+    ctx.close()
+
+
+def adjoint(passes):
+    from gradpower_hip import synth
+    from gradpower_hip.lib import HipContext, METHOD_MONO_FP64
+    case = synth.case3_200bus()
+    d = case.desc.dev
+    params = [int(r[2] + 6) for r in d[0]] + [int(r[2] + 7) for r in d[0]] + [int(r[1] + 2) for r in d[3]]
+    ev = [(m, e.bus, e.rfault, int(round(e.ton / case.dt)), int(round(e.toff / case.dt)))
+          for m in range(case.M) for e in case.events[m]]
+    ctx = HipContext(case.desc, case.M)
+    for it in range(passes):
+        ctx.set_state(case.desc.z0, case.desc.p)
+        ctx.set_events(ev, case.nsteps)
+        r = ctx.adjoint(METHOD_MONO_FP64, case.dt, params, tol=1e-12, itermax=50, segment=30,
+                        scale=1.0 / len(d[0]), raise_on_fail=False)
+        a = r["stats"]
+        print(f"adjoint pass {it}: {a['elapsed_ms']:.1f} ms (forward {a['forward_ms']:.1f}, "
+              f"recompute+backward {a['backward_ms']:.1f}), adjoint steps {a['adjoint_steps']}", flush=True)
     ctx.close()
 
 
