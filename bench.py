@@ -173,7 +173,7 @@ def run_adjoint(steps, warmup, n_fd, cpu_budget, no_cpu):
         worst, p0 = 0.0, case.desc.p.copy()
         for j in rng.choice(len(params), size=n_fd, replace=False):
             q = params[j]
-            h = 1e-6 * max(abs(p0[q]), 1.0)
+            h = max(1e-6 * abs(p0[q]), 1e-5)
             pp, pm = p0.copy(), p0.copy()
             pp[q] += h
             pm[q] -= h
@@ -181,7 +181,7 @@ def run_adjoint(steps, warmup, n_fd, cpu_budget, no_cpu):
                   adjoint_objective(case, pm, rows, scale, tol, ok)) / (2 * h)
             worst = max(worst, abs(r["grad"][j] - fd) / abs(fd))
             log(f"  fd check p[{q}]: adjoint {r['grad'][j]:.10e} fd {fd:.10e}")
-        out["fd_check"] = {"params": int(n_fd), "h": "1e-6 max(|p|, 1)", "max_rel_err": worst}
+        out["fd_check"] = {"params": int(n_fd), "h": "max(1e-6 |p|, 1e-5)", "max_rel_err": worst}
     if not no_cpu:
         from oracle.oracle import Oracle
         o = Oracle(case.desc)
@@ -242,7 +242,7 @@ def main():
     ap.add_argument("--methods", default="schur32,schur64,mono")
     ap.add_argument("--policy", default="async", choices=["async", "adaptive"],
                     help="refactorisation policy of the Schur methods")
-    ap.add_argument("--theta", type=float, default=100.0,
+    ap.add_argument("--theta", type=float, default=30.0,
                     help="refactorise when |F| falls by less than this factor in an iteration")
     ap.add_argument("--async-below", type=float, default=0.0,
                     help="async policy: refactorisations asked for below this |F| run on the second stream")

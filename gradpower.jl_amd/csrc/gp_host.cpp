@@ -43,6 +43,14 @@ template <typename T>
 struct DBuf {
   T *p = nullptr;
   size_t n = 0;
+  bool owned = true;
+  // a window into memory owned elsewhere (never freed here)
+  void view(void *base, size_t count) {
+    release();
+    p = static_cast<T *>(base);
+    n = count;
+    owned = false;
+  }
   void alloc(size_t count) {
     release();
     n = count;
@@ -62,11 +70,58 @@ struct DBuf {
     if (v.size()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     n = 0;
+    owned = true;
   }
   ~DBuf() { release(); }
+};
+
+// Pinned host memory (page-locked: copies to and from it are plain DMA in
+// stream order).
+struct HBuf {
+  uint8_t *p = nullptr;
+  void alloc(size_t bytes) {
+    release();
+    if (hipHostMalloc(reinterpret_cast<void **>(&p), bytes, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      throw GpError{GP_ERR_OOM, "hipHostMalloc of " + std::to_string(bytes) + " bytes failed"};
+    }
+    std::memset(p, 0, bytes);
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+  }
+  ~HBuf() { release(); }
+};
+
+// Per-round exchange of the scenario-asynchronous time loop: one block in
+// device memory and its pinned host mirror, laid out so that one copy
+// uploads what a round's evaluation reads and one copy brings back what it
+// decided (L = padded scenario count, a multiple of 64):
+//   [first L][refac0 L][dt 8L][work L][counts 16][failed L][areq L]
+//   upload   [0, 11L+16)      download [10L, 13L+16)
+// and a second block for the accepted steps: [acc L][adv L][col 4L].
+struct RoundLayout {
+  size_t L;
+  explicit RoundLayout(size_t l) : L(l) {}
+  size_t first() const { return 0; }
+  size_t refac0() const { return L; }
+  size_t dt() const { return 2 * L; }
+  size_t work() const { return 10 * L; }
+  size_t cnt() const { return 11 * L; }
+  size_t failed() const { return 11 * L + 16; }
+  size_t areq() const { return 12 * L + 16; }
+  size_t bytes() const { return 13 * L + 16; }
+  size_t up_bytes() const { return 11 * L + 16; }
+  size_t down_off() const { return 10 * L; }
+  size_t down_bytes() const { return 3 * L + 16; }
+  size_t acc() const { return 0; }
+  size_t adv() const { return L; }
+  size_t col() const { return 2 * L; }
+  size_t post_bytes() const { return 6 * L; }
 };
 
 struct LUPlan {
@@ -237,6 +292,8 @@ struct gp_ctx {
   DBuf<float> d_luS32, d_svS32, d_dnS32, d_X32;
   DBuf<double> d_X64;
   DBuf<uint8_t> d_work, d_act, d_refac, d_refac0, d_sel;
+  DBuf<uint8_t> d_round, d_post;   // per-round exchange blocks (RoundLayout); views point into them
+  HBuf h_round, h_post;
   DBuf<int> d_fbus, d_nunconv, d_status, d_nstall, d_iters;
   DBuf<double> d_fg, d_fbest, d_svar_snap, d_traj_tmp;
   DBuf<uint8_t> d_failed, d_upd, d_cur, d_tgt, d_pend, d_req, d_areq, d_sch;
@@ -263,6 +320,11 @@ struct gp_ctx {
   DBuf<int> d_hz, d_objrows;
   DBuf<int2> d_ent;
   DBuf<uint8_t> d_bact;
+  // scenario-asynchronous time loop: per-scenario step, first-round flags,
+  // record columns, accepted / advancing masks
+  DBuf<double> d_dtv;
+  DBuf<uint8_t> d_first, d_acc, d_adv;
+  DBuf<int> d_col;
   bool state_set = false;
   int hist_steps = 0;
   std::vector<int> traj_len;
@@ -615,11 +677,28 @@ void alloc_state(gp_ctx *C) {
   C->d_inj.alloc((size_t)std::max(1, 2 * C->ncl) * L);
   C->d_Dg.alloc((size_t)std::max(1, 6 * C->ncl) * L);
   C->d_fnorm.alloc(L);
-  C->d_work.alloc(L); C->d_act.alloc(L); C->d_refac.alloc(L); C->d_refac0.alloc(L); C->d_sel.alloc(L);
+  C->d_act.alloc(L); C->d_refac.alloc(L); C->d_sel.alloc(L);
+  {
+    const RoundLayout R(L);
+    C->d_round.alloc(R.bytes());
+    C->d_post.alloc(R.post_bytes());
+    HIPCHK(hipMemset(C->d_round.p, 0, R.bytes()));
+    C->h_round.alloc(R.bytes());
+    C->h_post.alloc(R.post_bytes());
+    uint8_t *b = C->d_round.p, *q = C->d_post.p;
+    C->d_first.view(b + R.first(), L);
+    C->d_refac0.view(b + R.refac0(), L);
+    C->d_dtv.view(b + R.dt(), L);
+    C->d_work.view(b + R.work(), L);
+    C->d_nunconv.view(b + R.cnt(), 4);
+    C->d_failed.view(b + R.failed(), L);
+    C->d_areq.view(b + R.areq(), L);
+    C->d_acc.view(q + R.acc(), L);
+    C->d_adv.view(q + R.adv(), L);
+    C->d_col.view(q + R.col(), L);
+  }
   C->d_fbus.alloc((size_t)MAXFAULT * L);
   C->d_fg.alloc((size_t)MAXFAULT * L);
-  C->d_nunconv.alloc(4);
-  C->d_areq.alloc(L);
   C->d_sch.alloc(L);
   C->d_nstall.alloc(L);
   C->d_iters.alloc(L);
@@ -636,7 +715,6 @@ void alloc_state(gp_ctx *C) {
   C->batch.assign(L, 0);
   C->queue.assign(L, 0);
   C->d_fbest.alloc(L);
-  C->d_failed.alloc(L);
   C->d_status.alloc(L);
   C->d_nb.alloc(L);
   HIPCHK(hipMemset(C->d_nb.p, 0, L * sizeof(unsigned long long)));
@@ -840,71 +918,88 @@ bool land_batch(gp_ctx *C, bool wait) {
   return true;
 }
 
-// newton_step! (src/numerics.jl:55-176) for every scenario in `mask`, in lock
-// step: each round evaluates F, drops the scenarios with |F|inf < tol (and
-// those that failed), and applies one correction to the rest.
-NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, const std::vector<uint8_t> &mask,
-                 const std::vector<uint8_t> *refac0 = nullptr) {
+// One Newton round engine (newton_step!, src/numerics.jl:55-176, batched):
+// eval() evaluates F, its max-norm and the per-scenario decisions of
+// k_norm_finish (converged scenarios leave `work`); correct() applies one
+// correction to the scenarios that asked for one and runs the
+// factorisations the refactorisation policy asks for.  Used in lock step by
+// newton() and by the scenario-asynchronous time loop.
+struct Newton {
+  gp_ctx *C;
+  int method, itermax, stall;
+  double dt, tol, theta, async_below;
+  bool schur, reuse, async, fp32;
+  const double *dtv = nullptr;   // per-scenario step on the device (NULL: dt for all)
+  std::vector<uint8_t> req;
+  const uint8_t *reqp;           // host copy of this round's asynchronous requests
   NewtonOut out;
-  const int L = C->ldM;
-  set_mask(C, C->d_work, mask);
-  const int pol = C->opt.refactor_policy;
-  const bool schur = method != GP_METHOD_MONO_FP64;
-  const bool reuse = refac0 && schur && (pol == GP_REFACTOR_ADAPTIVE || pol == GP_REFACTOR_ASYNC);
-  const bool async = schur && pol == GP_REFACTOR_ASYNC;
-  if (reuse) set_mask(C, C->d_refac0, *refac0);
-  const int stall = C->opt.stall_window;
-  HIPCHK(hipMemsetAsync(C->d_failed.p, 0, L, C->stream));
-  // refactor policy: theta <= 0 refactors every iteration; otherwise after
-  // the first iteration only when |F| fell by less than theta
-  const double theta = (!schur || pol == GP_REFACTOR_EVERY_ITER)
-                           ? 0.0
-                           : ((pol == GP_REFACTOR_ADAPTIVE || pol == GP_REFACTOR_ASYNC)
-                                  ? (C->opt.refactor_theta > 0.0 ? C->opt.refactor_theta : 100.0)
-                                  : 1e300);
-  const bool fp32 = method == GP_METHOD_SCHUR_FP32;
-  const double async_below = C->opt.async_below > 0.0 ? C->opt.async_below : 1e300;
-  const double *p = C->d_p.p;
-  double *z = C->d_z.p;
-  std::vector<uint8_t> req(L, 0);
-  // a scenario may sit out rounds while its factorisation is in flight, so
-  // rounds are bounded separately from each scenario's itermax corrections
-  const int max_rounds = 4 * itermax + 16;
-  bool done = false;
-  for (int it = 0; it < max_rounds; it++) {
-    launch_cluster_residual(C->T, p, z, C->d_zold.p, dt, C->d_f.p, C->d_inj.p, C->d_work.p, C->stream);
-    launch_bus_residual(C->T, p, z, C->d_inj.p, faults(C), C->d_f.p, C->d_work.p, C->stream);
-    HIPCHK(hipMemsetAsync(C->d_nunconv.p, 0, 4 * sizeof(int), C->stream));
+
+  Newton(gp_ctx *c, int meth, double dt_, double tol_, int itmax, bool has_refac0)
+      : C(c), method(meth), itermax(itmax), dt(dt_), tol(tol_) {
+    const int pol = C->opt.refactor_policy;
+    schur = method != GP_METHOD_MONO_FP64;
+    reuse = has_refac0 && schur && (pol == GP_REFACTOR_ADAPTIVE || pol == GP_REFACTOR_ASYNC);
+    async = schur && pol == GP_REFACTOR_ASYNC;
+    stall = C->opt.stall_window;
+    // refactor policy: theta <= 0 refactors every iteration; otherwise after
+    // the first iteration only when |F| fell by less than theta
+    theta = (!schur || pol == GP_REFACTOR_EVERY_ITER)
+                ? 0.0
+                : ((pol == GP_REFACTOR_ADAPTIVE || pol == GP_REFACTOR_ASYNC)
+                       ? (C->opt.refactor_theta > 0.0 ? C->opt.refactor_theta : 30.0)
+                       : 1e300);
+    fp32 = method == GP_METHOD_SCHUR_FP32;
+    async_below = C->opt.async_below > 0.0 ? C->opt.async_below : 1e300;
+    req.assign(C->ldM, 0);
+    reqp = req.data();
+  }
+
+  // first: every working scenario starts its solve (lock step); firstv: the
+  // scenarios that do (device flags).  cnt[] as k_norm_finish's n_unconv,
+  // read back here unless cnt is NULL (the caller uploaded zero counts and
+  // reads them back with the rest of the round's exchange block).
+  void eval(int first, const uint8_t *firstv, const uint8_t *refac0, int *cnt, bool read_req) {
+    const int L = C->ldM;
+    double *z = C->d_z.p;
+    launch_cluster_residual(C->T, C->d_p.p, z, C->d_zold.p, dt, C->d_f.p, C->d_inj.p, C->d_work.p, C->stream, dtv);
+    launch_bus_residual(C->T, C->d_p.p, z, C->d_inj.p, faults(C), C->d_f.p, C->d_work.p, C->stream);
+    if (cnt) HIPCHK(hipMemsetAsync(C->d_nunconv.p, 0, 4 * sizeof(int), C->stream));
     launch_rownorm_impl(C->n, C->M, L, C->d_f.p, C->d_work.p, tol, C->d_fnorm.p, C->d_nb.p, C->d_nunconv.p,
-                        C->d_refac.p, reuse ? C->d_refac0.p : nullptr, it == 0, theta, stall, itermax, C->d_fbest.p,
+                        C->d_refac.p, reuse ? refac0 : nullptr, first, theta, stall, itermax, C->d_fbest.p,
                         C->d_nstall.p, C->d_iters.p, C->d_failed.p, async ? C->d_pend.p : nullptr, C->d_upd.p,
-                        async ? async_below : 0.0, C->d_areq.p, C->d_sch.p, C->stream);
-    int cnt[4] = {0, 0, 0, 0};
+                        async ? async_below : 0.0, C->d_areq.p, C->d_sch.p, C->stream, firstv);
+    if (!cnt) return;
     HIPCHK(hipMemcpyAsync(cnt, C->d_nunconv.p, 4 * sizeof(int), hipMemcpyDeviceToHost, C->stream));
-    if (async && it > 0) HIPCHK(hipMemcpyAsync(req.data(), C->d_areq.p, L, hipMemcpyDeviceToHost, C->stream));
-    HIPCHK(hipStreamSynchronize(C->stream));
-    const int nun = cnt[0];
-    if (nun == 0) {
-      done = true;
-      break;
-    }
+    if (async && read_req) HIPCHK(hipMemcpyAsync(req.data(), C->d_areq.p, L, hipMemcpyDeviceToHost, C->stream));
+  }
+
+  bool queued() const {
+    for (int m = 0; m < C->ldM; m++)
+      if (C->queue[m]) return true;
+    return false;
+  }
+  void launch_queue() {
+    int nb = 0;
+    for (int m = 0; m < C->ldM; m++) nb += C->queue[m];
+    out.refac += nb;
+    out.factor_launches++;
+    if (fp32)
+      launch_batch<float>(C, C->d_luS32.p, C->d_svS32.p, C->d_dnS32.p);
+    else
+      launch_batch<double>(C, C->d_luS64.p, C->d_svS64.p, C->d_dnS64.p);
+  }
+
+  // One round's corrections after eval() (cnt read back).  land_first: a
+  // scenario starts a solve that refactorises (an event or a change of dt):
+  // a batch in flight lands first and the queued requests of the scenarios
+  // in `fresh` (made for the old system) are dropped.  Returns false when
+  // nothing can change any more.
+  bool correct(const int cnt[4], bool land_first, const std::vector<uint8_t> *fresh) {
+    const int L = C->ldM;
+    const double *p = C->d_p.p;
+    double *z = C->d_z.p;
     out.corrections += cnt[2];
     out.rounds++;
-    auto queued = [&]() {
-      for (int m = 0; m < L; m++)
-        if (C->queue[m]) return true;
-      return false;
-    };
-    auto launch_queue = [&]() {
-      int nb = 0;
-      for (int m = 0; m < L; m++) nb += C->queue[m];
-      out.refac += nb;
-      out.factor_launches++;
-      if (fp32)
-        launch_batch<float>(C, C->d_luS32.p, C->d_svS32.p, C->d_dnS32.p);
-      else
-        launch_batch<double>(C, C->d_luS64.p, C->d_svS64.p, C->d_dnS64.p);
-    };
     const bool landed = async && land_batch(C, false);
     if (cnt[2] == 0 && cnt[3] == 0) {
       // every remaining scenario waits for its factorisation
@@ -912,13 +1007,13 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
       if (C->inflight)
         land_batch(C, true);
       else if (!landed)
-        break;   // nothing will change (cannot happen)
-      continue;
+        return false;   // nothing will change (cannot happen)
+      return true;
     }
     if (method == GP_METHOD_MONO_FP64) {
       out.refac += cnt[1];
       if (cnt[1] > 0) out.factor_launches++;
-      launch_jac_mono(C->T, p, z, dt, faults(C), C->d_jv.p, C->d_upd.p, C->stream);
+      launch_jac_mono(C->T, p, z, dt, faults(C), C->d_jv.p, C->d_upd.p, C->stream, dtv);
       int pi = prof_begin(C, C->prof_factor, C->stream);
       launch_splu_factor_impl<double>(C->luJ.dev, C->d_jv.p, nullptr, L, C->d_luJ.p, C->d_svJ.p, C->d_dnJ.p,
                                       C->d_refac.p, nullptr, C->M, C->d_status.p, C->stream);
@@ -928,25 +1023,24 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
                                 C->d_upd.p, nullptr, C->M, C->stream);
       prof_end(C->prof_solve, pi, C->stream, cnt[2]);
       HIPCHK(hipGetLastError());
-      continue;
+      return true;
     }
     if (fp32)
-      launch_cluster_schur<float>(C->T, p, z, C->d_f.p, dt, C->d_X32.p, C->d_Dg.p, C->d_sch.p, C->stream);
+      launch_cluster_schur<float>(C->T, p, z, C->d_f.p, dt, C->d_X32.p, C->d_Dg.p, C->d_sch.p, C->stream, dtv);
     else
-      launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, C->d_sch.p, C->stream);
+      launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, C->d_sch.p, C->stream, dtv);
     launch_bus_schur(C->T, p, z, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_sch.p, C->stream);
-    const bool sync_fac = cnt[1] > 0;
-    if (sync_fac) {
+    if (cnt[1] > 0) {
       // synchronous factorisation into the buffer the solves read.  Later
       // iterations ask only for scenarios with nothing pending; the first one
-      // (a solve after an event or a change of dt) may include scenarios in
+      // of a solve after an event or a change of dt may include scenarios in
       // the batch in flight, which lands first, and their queued requests
       // (made for the old system) are dropped
-      if (async && it == 0) {
+      if (async && land_first) {
         land_batch(C, true);
-        if (refac0) {
+        if (fresh) {
           for (int m = 0; m < L; m++)
-            if ((*refac0)[m]) C->queue[m] = C->pend[m] = 0;
+            if ((*fresh)[m]) C->queue[m] = C->pend[m] = 0;
           HIPCHK(hipMemcpyAsync(C->d_pend.p, C->pend.data(), L, hipMemcpyHostToDevice, C->stream));
           HIPCHK(hipStreamSynchronize(C->stream));   // the host copy stays valid until then
         }
@@ -966,7 +1060,7 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
       // asynchronous requests: queued (and pending: the scenario waits when
       // its |F| stops decreasing), launched when no batch is in flight
       for (int m = 0; m < L; m++)
-        if (req[m]) C->queue[m] = C->pend[m] = 1;
+        if (reqp[m]) C->queue[m] = C->pend[m] = 1;
       HIPCHK(hipMemcpyAsync(C->d_pend.p, C->pend.data(), L, hipMemcpyHostToDevice, C->stream));
       HIPCHK(hipStreamSynchronize(C->stream));   // the host copy stays valid until then
     }
@@ -984,9 +1078,37 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
     else
       launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, C->d_upd.p, C->stream);
     HIPCHK(hipGetLastError());
+    return true;
+  }
+};
+
+// newton_step! (src/numerics.jl:55-176) for every scenario in `mask`, in lock
+// step: each round evaluates F, drops the scenarios with |F|inf < tol (and
+// those that failed), and applies one correction to the rest.
+NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, const std::vector<uint8_t> &mask,
+                 const std::vector<uint8_t> *refac0 = nullptr) {
+  const int L = C->ldM;
+  Newton N(C, method, dt, tol, itermax, refac0 != nullptr);
+  set_mask(C, C->d_work, mask);
+  if (N.reuse) set_mask(C, C->d_refac0, *refac0);
+  HIPCHK(hipMemsetAsync(C->d_failed.p, 0, L, C->stream));
+  // a scenario may sit out rounds while its factorisation is in flight, so
+  // rounds are bounded separately from each scenario's itermax corrections
+  const int max_rounds = 4 * itermax + 16;
+  bool done = false;
+  for (int it = 0; it < max_rounds; it++) {
+    int cnt[4] = {0, 0, 0, 0};
+    N.eval(it == 0, nullptr, C->d_refac0.p, cnt, it > 0);
+    HIPCHK(hipStreamSynchronize(C->stream));
+    if (cnt[0] == 0) {
+      done = true;
+      break;
+    }
+    if (!N.correct(cnt, it == 0, refac0)) break;
   }
   // scenarios stopped by the itermax / stall rules, plus (round cap) any
   // still working
+  NewtonOut out = N.out;
   std::vector<uint8_t> w(L, 0), fl(L, 0);
   if (!done) HIPCHK(hipMemcpyAsync(w.data(), C->d_work.p, L, hipMemcpyDeviceToHost, C->stream));
   HIPCHK(hipMemcpyAsync(fl.data(), C->d_failed.p, L, hipMemcpyDeviceToHost, C->stream));
@@ -1227,6 +1349,180 @@ struct Stepper {
 };
 
 
+// The time loop with every scenario advancing on its own: each scenario
+// follows integrate! (src/dynamics.jl:548-640) step by step, but a round
+// evaluates F for every running scenario at its own step k_m, and the
+// scenarios that converged accept their step at once (record z_k, fire
+// their events of step k with the dt = 0 re-solve, or start step k+1) while
+// the others receive a correction.  No scenario waits for a slower one, so
+// the rounds of a call are those of its slowest scenario, not the sum over
+// steps of the slowest scenario of each step.
+gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int itermax) {
+  const int L = C->ldM, M = C->M, n = C->n;
+  gp_stats S{};
+  std::vector<std::map<int, std::vector<const gp_event *>>> on(M), off(M);
+  for (auto &e : C->events) {
+    on[e.scenario][e.step_on].push_back(&e);
+    off[e.scenario][e.step_off].push_back(&e);
+  }
+  std::vector<std::vector<const gp_event *>> active_ev(M);
+  std::vector<int> k(L, 1);
+  std::vector<uint8_t> mode(L, 0), run(L, 0), first(L, 0), stale(L, 1), fresh(L, 0), wk(L, 0), fl(L, 0),
+      everfail(L, 0);
+  const RoundLayout R(L);
+  uint8_t *hr = C->h_round.p, *hp = C->h_post.p;   // pinned staging (RoundLayout)
+  uint8_t *acc = hp + R.acc(), *adv = hp + R.adv();
+  int *col = reinterpret_cast<int *>(hp + R.col());
+  std::vector<double> dtv(L, dt);
+  std::vector<int> fbus((size_t)MAXFAULT * L, -1);
+  std::vector<double> fg((size_t)MAXFAULT * L, 0.0);
+  int maxsteps = 0;
+  for (int m = 0; m < M; m++) {
+    run[m] = C->nsteps[m] >= 1 ? 1 : 0;
+    first[m] = 1;
+    maxsteps = std::max(maxsteps, C->nsteps[m]);
+  }
+  std::memcpy(adv, run.data(), L);
+  HIPCHK(hipMemcpyAsync(C->d_post.p, hp, R.post_bytes(), hipMemcpyHostToDevice, C->stream));
+  launch_copy_active(n, L, C->d_z.p, C->d_zold.p, C->d_adv.p, C->stream);
+  HIPCHK(hipMemsetAsync(C->d_failed.p, 0, L, C->stream));
+  HIPCHK(hipStreamSynchronize(C->stream));   // the staging block stays valid until then
+  Newton N(C, method, dt, tol, itermax, true);
+  N.dtv = C->d_dtv.p;
+  bool fault_dirty = true;
+  const long long max_rounds = (long long)(maxsteps + 1) * 2 * (4 * itermax + 16);
+  for (long long round = 0;; round++) {
+    int nrun = 0;
+    bool land_first = false;
+    double *hdt = reinterpret_cast<double *>(hr + R.dt());
+    for (int m = 0; m < L; m++) {
+      nrun += run[m];
+      fresh[m] = (run[m] && first[m] && stale[m]) ? 1 : 0;
+      land_first = land_first || fresh[m];
+      hr[R.first() + m] = first[m];
+      hr[R.refac0() + m] = fresh[m];
+      hr[R.work() + m] = run[m];
+      hdt[m] = dtv[m];
+    }
+    if (nrun == 0) break;
+    if (round >= max_rounds) throw GpError{GP_ERR_HIP, "internal: scenario-asynchronous loop did not finish"};
+    S.newton_rounds++;
+    std::memset(hr + R.cnt(), 0, 16);
+    HIPCHK(hipMemcpyAsync(C->d_round.p, hr, R.up_bytes(), hipMemcpyHostToDevice, C->stream));
+    if (fault_dirty) {
+      HIPCHK(hipMemcpyAsync(C->d_fbus.p, fbus.data(), fbus.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
+      HIPCHK(hipMemcpyAsync(C->d_fg.p, fg.data(), fg.size() * sizeof(double), hipMemcpyHostToDevice, C->stream));
+      fault_dirty = false;
+    }
+    N.eval(0, C->d_first.p, C->d_refac0.p, nullptr, true);
+    HIPCHK(hipMemcpyAsync(hr + R.down_off(), C->d_round.p + R.down_off(), R.down_bytes(), hipMemcpyDeviceToHost,
+                          C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    int cnt[4];
+    std::memcpy(cnt, hr + R.cnt(), sizeof(cnt));
+    std::memcpy(wk.data(), hr + R.work(), L);
+    std::memcpy(fl.data(), hr + R.failed(), L);
+    N.reqp = hr + R.areq();
+    // corrections of the scenarios still iterating run while the host
+    // accepts the converged ones
+    if (cnt[0] > 0 && !N.correct(cnt, land_first, &fresh))
+      throw GpError{GP_ERR_HIP, "internal: no scenario can make progress"};
+    std::memset(acc, 0, L);
+    std::memset(adv, 0, L);
+    bool any_acc = false, any_adv = false, any_fail = false;
+    for (int m = 0; m < M; m++) {
+      if (!run[m]) continue;
+      first[m] = 0;
+      if (fl[m]) {   // itermax or stall rule: the scenario stops and is reported
+        run[m] = 0;
+        everfail[m] = 1;
+        any_fail = true;
+        C->scen_status[m] = GP_ERR_NEWTON_NOT_CONVERGED;
+        C->scen_steps[m] = mode[m] ? k[m] : k[m] - 1;
+        continue;
+      }
+      if (wk[m]) continue;   // still iterating
+      bool next = true;
+      if (mode[m] == 0) {    // step k accepted: record z_k, then its events
+        acc[m] = 1;
+        col[m] = k[m];
+        any_acc = true;
+        stale[m] = 0;
+        S.scenario_steps++;
+        bool changed = false;
+        auto ion = on[m].find(k[m]);
+        if (ion != on[m].end())
+          for (const gp_event *e : ion->second) {
+            active_ev[m].push_back(e);
+            changed = true;
+          }
+        auto ioff = off[m].find(k[m]);
+        if (ioff != off[m].end())
+          for (const gp_event *e : ioff->second) {
+            auto &v = active_ev[m];
+            auto it = std::find(v.begin(), v.end(), e);
+            if (it != v.end()) v.erase(it);
+            changed = true;
+          }
+        if (changed) {       // re-solve at dt = 0 under the new fault state
+          if ((int)active_ev[m].size() > MAXFAULT)
+            throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults in one scenario"};
+          for (int q = 0; q < MAXFAULT; q++) {
+            const bool f_on = q < (int)active_ev[m].size();
+            fbus[(size_t)q * L + m] = f_on ? active_ev[m][q]->bus : -1;
+            fg[(size_t)q * L + m] = f_on ? 1.0 / active_ev[m][q]->r_fault : 0.0;
+          }
+          fault_dirty = true;
+          S.event_resolves++;
+          mode[m] = 1;
+          dtv[m] = 0.0;
+          first[m] = 1;
+          stale[m] = 1;
+          adv[m] = 1;
+          any_adv = true;
+          next = false;
+        }
+      } else {               // re-solve done: factorised at dt = 0, the next step refactorises
+        mode[m] = 0;
+        stale[m] = 1;
+      }
+      if (next) {
+        k[m]++;
+        if (k[m] > C->nsteps[m]) {
+          run[m] = 0;
+        } else {
+          dtv[m] = dt;
+          first[m] = 1;
+          adv[m] = 1;
+          any_adv = true;
+        }
+      }
+    }
+    if (any_acc || any_adv)
+      HIPCHK(hipMemcpyAsync(C->d_post.p, hp, R.post_bytes(), hipMemcpyHostToDevice, C->stream));
+    if (any_acc)
+      launch_record_at(C->nrec, C->rec_idx.empty() ? nullptr : C->d_rec_idx.p, L, M, C->d_z.p, C->d_hist.p,
+                       C->d_col.p, C->d_acc.p, C->stream);
+    if (any_adv) launch_copy_active(n, L, C->d_z.p, C->d_zold.p, C->d_adv.p, C->stream);
+    if (any_fail) {          // non-finite states
+      std::vector<double> fn(L);
+      HIPCHK(hipMemcpy(fn.data(), C->d_fnorm.p, L * sizeof(double), hipMemcpyDeviceToHost));
+      for (int m = 0; m < M; m++)
+        if (fl[m] && everfail[m] && C->scen_status[m] == GP_ERR_NEWTON_NOT_CONVERGED && !std::isfinite(fn[m])) {
+          S.n_nan++;
+          C->scen_status[m] = GP_ERR_NAN;
+        }
+      HIPCHK(hipMemsetAsync(C->d_failed.p, 0, L, C->stream));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(C->stream));
+  S.newton_iters = N.out.corrections;
+  S.refactorizations = N.out.refac;
+  S.factor_launches = N.out.factor_launches;
+  for (int m = 0; m < M; m++) S.n_not_converged += everfail[m];
+  return S;
+}
+
 // LU plan of J^T for the adjoint's transposed solves: the CSC pattern of J^T
 // (the CSR pattern of J) whose values the factorisation reads from the
 // Jacobian value array in place (a_src), pivots chosen on scenario 0's matrix
@@ -1446,10 +1742,15 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
     C->scen_steps.assign(C->nsteps.begin(), C->nsteps.begin() + M);
     DevTimer tm(C->stream);
     record_initial(C);
-    for (int k = 1; k <= maxsteps; k++)
-      if (!sp.step(k, C->d_hist.p + (size_t)k * C->nrec * L, nullptr)) break;
+    if (C->opt.lock_step) {
+      for (int k = 1; k <= maxsteps; k++)
+        if (!sp.step(k, C->d_hist.p + (size_t)k * C->nrec * L, nullptr)) break;
+      S = sp.S;
+      for (int m = 0; m < M; m++) S.n_not_converged += sp.st.everfail[m];
+    } else {
+      S = integrate_free(C, method, dt, tol, itermax);
+    }
     land_batch(C, true);   // no factorisation outlives the call
-    S = sp.S;
     S.elapsed_ms = S.solve_ms = tm.ms();
     if (C->opt.profile) {
       HIPCHK(hipStreamSynchronize(C->fstream));
@@ -1461,7 +1762,6 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
       S.solve_bytes_per_scenario = bf * A.nnz_lu + (mono ? 24.0 : 32.0) * A.n;
       S.factor_bytes_per_scenario = 8.0 * A.nnz_a + bf * A.nnz_lu;
     }
-    for (int m = 0; m < M; m++) S.n_not_converged += sp.st.everfail[m];
     if (st) *st = S;
     const int rc = finish_status(C, S);
     if (rc != GP_OK) return rc;

@@ -96,20 +96,23 @@ enum AdjKind {
 };
 
 // ---- kernel launchers (gp_kernels.hip) ----------------------------------
+// dtv (optional): per-scenario step, replacing dt (0 for an event re-solve)
 void launch_cluster_residual(const DevTables &T, const double *p, const double *z, const double *zold,
-                             double dt, double *f, double *inj, const uint8_t *work, hipStream_t s);
+                             double dt, double *f, double *inj, const uint8_t *work, hipStream_t s,
+                             const double *dtv = nullptr);
 void launch_bus_residual(const DevTables &T, const double *p, const double *z, const double *inj,
                          FaultState fs, double *f, const uint8_t *work, hipStream_t s);
 void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, double tol, double *fnorm,
                          unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
                          double theta, int stall, int itermax, double *fbest, int *nstall, int *iters,
                          uint8_t *failed, const uint8_t *pend, uint8_t *upd, double async_below, uint8_t *areq,
-                         uint8_t *sch, hipStream_t s);
+                         uint8_t *sch, hipStream_t s, const uint8_t *firstv = nullptr);
 void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs,
-                     double *jv, const uint8_t *work, hipStream_t s);
+                     double *jv, const uint8_t *work, hipStream_t s, const double *dtv = nullptr);
 template <typename TF>
 void launch_cluster_schur(const DevTables &T, const double *p, const double *z, const double *f,
-                          double dt, TF *X, double *Dg, const uint8_t *work, hipStream_t s);
+                          double dt, TF *X, double *Dg, const uint8_t *work, hipStream_t s,
+                          const double *dtv = nullptr);
 void launch_bus_schur(const DevTables &T, const double *p, const double *z, const double *f,
                       const double *Dg, FaultState fs, double *svar, double *rhs, const uint8_t *work,
                       hipStream_t s);
@@ -141,5 +144,7 @@ void launch_gather_traj(int nrec, int len, int ldM, int m, const double *hist, d
 void launch_copy_plain(const double *src, double *dst, long long n, hipStream_t s);
 void launch_record_impl(int nrec, const int *idx, int ldM, int M, const double *z, double *hist,
                         const uint8_t *act, hipStream_t s);
+void launch_record_at(int nrec, const int *idx, int ldM, int M, const double *z, double *hist, const int *col,
+                      const uint8_t *act, hipStream_t s);
 
 }  // namespace gp

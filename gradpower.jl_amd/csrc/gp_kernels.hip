@@ -209,11 +209,13 @@ __device__ __forceinline__ void ieeest_residual(const double *P, const double *x
 // of every cluster row and the generator's current injection.
 __global__ void __launch_bounds__(64) k_cluster_residual(DevTables T, const double *__restrict__ p,
                                                          const double *__restrict__ z, const double *__restrict__ zold,
-                                                         double dt, double *__restrict__ f, double *__restrict__ inj,
+                                                         double dt0, const double *__restrict__ dtv,
+                                                         double *__restrict__ f, double *__restrict__ inj,
                                                          const uint8_t *__restrict__ work) {
   const int c = T.cl_nt[blockIdx.x];
   const int m = lane_scenario();
   if (m >= T.M || !work[m]) return;
+  const double dt = dtv ? dtv[m] : dt0;   // per-scenario step (0: event re-solve)
   const int ldM = T.ldM;
   const int *cl = T.cl + c * GP_CLUSTER_COLS;
   const int ws = cl[GP_CL_WSTART], gk = cl[GP_CL_GENROU];
@@ -413,18 +415,20 @@ __global__ void __launch_bounds__(256) k_rownorm(int n, int ldM, int M, const do
 // or that asks for an asynchronous one, waits for it instead, so that its
 // iterates are those of the synchronous policy.  sch[m] = 1 where the Schur
 // kernels run (corrections and new asynchronous requests).  iters[m] counts
-// corrections; first resets the per-solve history.  n_unconv:
+// corrections; first (per scenario with firstv) resets the per-solve history.  n_unconv:
 // [0] working scenarios, [1] synchronous requests, [2] corrections, [3]
 // asynchronous requests.
 __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict__ nb, double *__restrict__ fnorm,
                               uint8_t *__restrict__ work, int *__restrict__ n_unconv, uint8_t *__restrict__ refac,
-                              const uint8_t *__restrict__ refac0, int first, double theta, int stall, int itermax,
+                              const uint8_t *__restrict__ refac0, int first0, const uint8_t *__restrict__ firstv,
+                              double theta, int stall, int itermax,
                               double *__restrict__ fbest, int *__restrict__ nstall, int *__restrict__ iters,
                               uint8_t *__restrict__ failed, const uint8_t *__restrict__ pend,
                               uint8_t *__restrict__ upd, double async_below, uint8_t *__restrict__ areq,
                               uint8_t *__restrict__ sch) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
+  const bool first = firstv ? firstv[m] != 0 : first0 != 0;   // first round of this scenario's solve
   const bool was_upd = first ? true : upd[m] != 0;
   upd[m] = 0;
   refac[m] = 0;
@@ -465,7 +469,10 @@ __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict_
   // its new factorisation has just arrived, so it asks for none
   const int rf = first ? (refac0 ? refac0[m] : 1)
                        : ((pending || !was_upd) ? 0 : ((theta <= 0.0 || !(v * theta < prev)) ? 1 : 0));
-  bool wait = pending;
+  // a solve that starts with a factorisation of its own (after an event or
+  // a change of dt) does not wait for one asked for under the old system:
+  // the host lands and drops that request before this round's factorisation
+  bool wait = pending && !(first && rf);
   if (rf && !first && async_below > 0.0 && v < async_below) {
     areq[m] = 1;
     atomicAdd(n_unconv + 3, 1);
@@ -491,11 +498,13 @@ __device__ __forceinline__ double be(double v, bool diff, bool diag, double dt) 
 }
 
 __global__ void __launch_bounds__(64) k_cluster_jac_mono(DevTables T, const double *__restrict__ p,
-                                                         const double *__restrict__ z, double dt,
-                                                         double *__restrict__ jv, const uint8_t *__restrict__ work) {
+                                                         const double *__restrict__ z, double dt0,
+                                                         const double *__restrict__ dtv, double *__restrict__ jv,
+                                                         const uint8_t *__restrict__ work) {
   const int c = T.cl_nt[blockIdx.x];
   const int m = lane_scenario();
   if (m >= T.M || !work[m]) return;
+  const double dt = dtv ? dtv[m] : dt0;
   const int ldM = T.ldM;
   const int *cl = T.cl + c * GP_CLUSTER_COLS;
   const int gk = cl[GP_CL_GENROU];
@@ -874,11 +883,13 @@ __device__ __forceinline__ void exciter_block(int et, const double *P, double dt
 template <typename TF>
 __global__ void __launch_bounds__(64) k_cluster_schur(DevTables T, const double *__restrict__ p,
                                                       const double *__restrict__ z, const double *__restrict__ f,
-                                                      double dt, TF *__restrict__ X, double *__restrict__ Dg,
+                                                      double dt0, const double *__restrict__ dtv,
+                                                      TF *__restrict__ X, double *__restrict__ Dg,
                                                       const uint8_t *__restrict__ work) {
   const int c = T.cl_nt[blockIdx.x];
   const int m = lane_scenario();
   if (m >= T.M || !work[m]) return;
+  const double dt = dtv ? dtv[m] : dt0;
   const int ldM = T.ldM;
   const int *cl = T.cl + c * GP_CLUSTER_COLS;
   const int gk = cl[GP_CL_GENROU];
@@ -2273,6 +2284,19 @@ __global__ void k_record(int nrec, const int *__restrict__ idx, int ldM, int M, 
   }
 }
 
+// Recorded channels of z into each scenario's own step column col[m]
+// (scenarios advancing independently): hist[(col[m]*nrec + j)*ldM + m].
+__global__ void k_record_at(int nrec, const int *__restrict__ idx, int ldM, int M, const double *__restrict__ z,
+                            double *__restrict__ hist, const int *__restrict__ col, const uint8_t *__restrict__ act) {
+  const int m = blockIdx.y * WAVE + threadIdx.x;
+  if (m >= M || !act[m]) return;
+  const size_t base = (size_t)col[m] * nrec;
+  for (int j = blockIdx.x; j < nrec; j += gridDim.x) {
+    const int r = idx ? idx[j] : j;
+    hist[(base + j) * ldM + m] = z[(size_t)r * ldM + m];
+  }
+}
+
 // One scenario's recorded history, transposed: out[j*len + k] = hist[(k*nrec + j)*ldM + m].
 __global__ void k_gather_traj(int nrec, int len, int ldM, int m, const double *__restrict__ hist,
                               double *__restrict__ out) {
@@ -2292,9 +2316,10 @@ __global__ void k_copy_plain(const double *__restrict__ src, double *__restrict_
 static inline dim3 lane_grid(int items, int ldM) { return dim3(items, ldM / WAVE); }
 
 void launch_cluster_residual(const DevTables &T, const double *p, const double *z, const double *zold, double dt,
-                             double *f, double *inj, const uint8_t *work, hipStream_t s) {
+                             double *f, double *inj, const uint8_t *work, hipStream_t s, const double *dtv) {
   if (T.n_cl_nt == 0) return;
-  hipLaunchKernelGGL(k_cluster_residual, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, zold, dt, f, inj, work);
+  hipLaunchKernelGGL(k_cluster_residual, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, zold, dt, dtv, f, inj,
+                     work);
 }
 void launch_bus_residual(const DevTables &T, const double *p, const double *z, const double *inj, FaultState fs,
                          double *f, const uint8_t *work, hipStream_t s) {
@@ -2304,24 +2329,25 @@ void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, 
                          unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
                          double theta, int stall, int itermax, double *fbest, int *nstall, int *iters,
                          uint8_t *failed, const uint8_t *pend, uint8_t *upd, double async_below, uint8_t *areq,
-                         uint8_t *sch, hipStream_t s) {
+                         uint8_t *sch, hipStream_t s, const uint8_t *firstv) {
   const int chunks = std::max(1, std::min((n + 15) / 16, 1024 / (ldM / WAVE)));
   hipLaunchKernelGGL(k_rownorm, dim3(ldM / WAVE, chunks), dim3(256), 0, s, n, ldM, M, f, work, nb);
   hipLaunchKernelGGL(k_norm_finish, dim3((M + 255) / 256), dim3(256), 0, s, M, tol, nb, fnorm, work, n_unconv, refac,
-                     refac0, first, theta, stall, itermax, fbest, nstall, iters, failed, pend, upd, async_below,
-                     areq, sch);
+                     refac0, first, firstv, theta, stall, itermax, fbest, nstall, iters, failed, pend, upd,
+                     async_below, areq, sch);
 }
 void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs, double *jv,
-                     const uint8_t *work, hipStream_t s) {
+                     const uint8_t *work, hipStream_t s, const double *dtv) {
   if (T.n_cl_nt)
-    hipLaunchKernelGGL(k_cluster_jac_mono, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, dt, jv, work);
+    hipLaunchKernelGGL(k_cluster_jac_mono, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, dt, dtv, jv, work);
   hipLaunchKernelGGL(k_bus_jac_mono, lane_grid(T.nbus, T.ldM), dim3(WAVE), 0, s, T, p, z, fs, jv, work);
 }
 template <typename TF>
 void launch_cluster_schur(const DevTables &T, const double *p, const double *z, const double *f, double dt, TF *X,
-                          double *Dg, const uint8_t *work, hipStream_t s) {
+                          double *Dg, const uint8_t *work, hipStream_t s, const double *dtv) {
   if (T.n_cl_nt == 0) return;
-  hipLaunchKernelGGL(k_cluster_schur<TF>, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, f, dt, X, Dg, work);
+  hipLaunchKernelGGL(k_cluster_schur<TF>, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, f, dt, dtv, X, Dg,
+                     work);
 }
 void launch_bus_schur(const DevTables &T, const double *p, const double *z, const double *f, const double *Dg,
                       FaultState fs, double *svar, double *rhs, const uint8_t *work, hipStream_t s) {
@@ -2392,6 +2418,11 @@ void launch_copy_active(int n, int ldM, const double *src, double *dst, const ui
   const int gx = n < 1024 ? n : 1024;
   hipLaunchKernelGGL(k_copy_active, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, n, ldM, src, dst, act);
 }
+void launch_record_at(int nrec, const int *idx, int ldM, int M, const double *z, double *hist, const int *col,
+                      const uint8_t *act, hipStream_t s) {
+  const int gx = nrec < 1024 ? (nrec > 0 ? nrec : 1) : 1024;
+  hipLaunchKernelGGL(k_record_at, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, nrec, idx, ldM, M, z, hist, col, act);
+}
 void launch_record_impl(int nrec, const int *idx, int ldM, int M, const double *z, double *hist, const uint8_t *act,
                         hipStream_t s) {
   const int gx = nrec < 1024 ? (nrec > 0 ? nrec : 1) : 1024;
@@ -2399,9 +2430,9 @@ void launch_record_impl(int nrec, const int *idx, int ldM, int M, const double *
 }
 
 template void launch_cluster_schur<double>(const DevTables &, const double *, const double *, const double *, double,
-                                           double *, double *, const uint8_t *, hipStream_t);
+                                           double *, double *, const uint8_t *, hipStream_t, const double *);
 template void launch_cluster_schur<float>(const DevTables &, const double *, const double *, const double *, double,
-                                          float *, double *, const uint8_t *, hipStream_t);
+                                          float *, double *, const uint8_t *, hipStream_t, const double *);
 template void launch_cluster_backsub<double>(const DevTables &, const double *, const double *, const double *,
                                              double *, const uint8_t *, hipStream_t);
 template void launch_cluster_backsub<float>(const DevTables &, const float *, const double *, const double *,

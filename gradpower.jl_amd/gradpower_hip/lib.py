@@ -35,7 +35,8 @@ class GpError(RuntimeError):
 class COptions(C.Structure):
     _fields_ = [("record_all", C.c_int32), ("n_record", C.c_int32),
                 ("record_idx", C.POINTER(C.c_int32)), ("refactor_policy", C.c_int32),
-                ("stall_window", C.c_int32), ("profile", C.c_int32), ("reserved", C.c_int32 * 4),
+                ("stall_window", C.c_int32), ("profile", C.c_int32), ("lock_step", C.c_int32),
+                ("reserved", C.c_int32 * 3),
                 ("refactor_theta", C.c_double),
                 ("async_below", C.c_double)]
 
@@ -130,7 +131,7 @@ class HipContext:
 
     def __init__(self, desc: SystemDesc, M: int, record_idx=None,
                  refactor_policy=REFACTOR_EVERY_ITER, stall_window=0, refactor_theta=0.0, async_below=0.0,
-                 profile=False):
+                 profile=False, lock_step=False):
         self.lib = load()
         self.desc = desc
         self.M = int(M)
@@ -151,6 +152,7 @@ class HipContext:
         opt.refactor_theta = float(refactor_theta)
         opt.async_below = float(async_below)
         opt.profile = 1 if profile else 0
+        opt.lock_step = 1 if lock_step else 0
         self._opt = opt
         h = C.c_void_p()
         self._check(self.lib.gp_create(C.byref(self._cs), self.M, C.byref(opt), C.byref(h)))

@@ -123,7 +123,7 @@ enum gp_refactor_policy {
   GP_REFACTOR_ADAPTIVE = 2,    /* keep the factorisation across time steps; factorise
                                   again after a fault event or a change of dt, and
                                   whenever an iteration fails to reduce |F|inf by
-                                  refactor_theta (default 100x)                        */
+                                  refactor_theta (default 30x)                         */
   GP_REFACTOR_ASYNC = 3        /* the iterates of ADAPTIVE, but a refactorisation asked
                                   for after the first iteration of a solve runs on a
                                   second stream: the scenario waits for it while the
@@ -145,9 +145,15 @@ typedef struct gp_options {
   int32_t profile;         /* 1: time every launch of the batched triangular solve
                               and of the numeric factorisation with HIP events on
                               the stream it runs on (gp_stats kernel fields)    */
-  int32_t reserved[4];
+  int32_t lock_step;       /* 0 (default): every scenario advances on its own --
+                              a scenario whose step converged moves on while the
+                              others iterate; 1: all scenarios take step k
+                              together (a step lasts as long as its slowest
+                              scenario).  Per-scenario iterates are the same
+                              either way for EVERY_ITER and FIRST_ITER          */
+  int32_t reserved[3];
   double refactor_theta;   /* ADAPTIVE / ASYNC: refactorise when an iteration
-                              reduces |F|inf by less than this factor (0: 100)  */
+                              reduces |F|inf by less than this factor (0: 30)   */
   double async_below;      /* ASYNC: a refactorisation asked for at |F|inf below
                               this runs on the second stream, above it in the
                               lock step (0: every one runs on the second stream) */

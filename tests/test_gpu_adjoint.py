@@ -10,10 +10,11 @@ Bars, all FP64:
   - the 2-bus event-offset test of test/test_adjoint_event_offset.jl:26-84
     within 1e-6 of the centred difference;
   - 20 random H / D / K of the 200-bus, 128-scenario, 3 s workload within
-    1e-5 of centred differences with h = 1e-6 max(|p|, 1) (the floor of
-    the reference's own h = 1e-5 max(|p|, 1): a purely relative step on a
-    damping D of 0.2 sits at the FP64 noise floor of the difference
-    quotient, whose error grows as 1/h there);
+    1e-5 of centred differences with h = max(1e-6 |p|, 1e-5): a relative
+    step, floored where the quotient meets the FP64 noise of the objective
+    (a damping D of 0.2 at h = 2e-7 is off by up to 3e-5 while the adjoint
+    value does not move; a gain K = 50 at h = 5e-4 crosses kinks of the
+    constant-power load clamp and is off by 3e-5 the other way);
   - checkpoint segments of any length give bit-identical gradients.
 """
 import numpy as np
@@ -168,7 +169,7 @@ def test_adjoint_200bus_vs_central_differences():
     """The 200-bus workload: 40 GENROU+SEXS+TGOV1, 128 fault scenarios, 3 s at
     dt = 1/120, objective sum_m sum_k dt mean_g w_g^2, 30-step segments;
     20 random parameters among all H, D and SEXS K against centred
-    differences with h = 1e-6 max(|p|, 1), relative error <= 1e-5.  Scenarios whose
+    differences with h = max(1e-6 |p|, 1e-5), relative error <= 1e-5.  Scenarios whose
     forward sweep fails (bolted faults on this weak grid) are left out of
     both sides."""
     case = get_case("case3_200bus")
@@ -196,7 +197,7 @@ def test_adjoint_200bus_vs_central_differences():
     worst = 0.0
     for j in pick:
         q = allp[j]
-        h = 1e-6 * max(abs(p0[q]), 1.0)
+        h = max(1e-6 * abs(p0[q]), 1e-5)
         pp, pm = p0.copy(), p0.copy()
         pp[q] += h
         pm[q] -= h
