@@ -294,6 +294,14 @@ struct gp_ctx {
   DBuf<uint8_t> d_work, d_act, d_refac, d_refac0, d_sel;
   DBuf<uint8_t> d_round, d_post;   // per-round exchange blocks (RoundLayout); views point into them
   HBuf h_round, h_post;
+  // a batch's requests and target buffers [req L][tgt L] (d_req, d_tgt view
+  // it), uploaded from a ring of pinned slots: each round synchronises the
+  // stream at least once and launches at most one batch, so a slot is
+  // reused only after its copy has completed
+  DBuf<uint8_t> d_batch;
+  HBuf h_batch;
+  int batch_slot = 0;
+  static constexpr int kBatchSlots = 4;
   DBuf<int> d_fbus, d_nunconv, d_status, d_nstall, d_iters;
   DBuf<double> d_fg, d_fbest, d_svar_snap, d_traj_tmp;
   DBuf<uint8_t> d_failed, d_upd, d_cur, d_tgt, d_pend, d_req, d_areq, d_sch;
@@ -704,9 +712,11 @@ void alloc_state(gp_ctx *C) {
   C->d_iters.alloc(L);
   C->d_upd.alloc(L);
   C->d_cur.alloc(L);
-  C->d_tgt.alloc(L);
   C->d_pend.alloc(L);
-  C->d_req.alloc(L);
+  C->d_batch.alloc(2 * L);
+  C->h_batch.alloc((size_t)gp_ctx::kBatchSlots * 2 * L);
+  C->d_req.view(C->d_batch.p, L);
+  C->d_tgt.view(C->d_batch.p + L, L);
   HIPCHK(hipMemset(C->d_cur.p, 0, L));
   HIPCHK(hipMemset(C->d_pend.p, 0, L));
   HIPCHK(hipMemset(C->d_upd.p, 0, L));
@@ -868,16 +878,17 @@ struct NewtonOut {
 template <typename TF>
 void launch_batch(gp_ctx *C, TF *luv, TF *sv, TF *dn) {
   const int L = C->ldM;
-  std::vector<uint8_t> tgt(L, 0);
+  uint8_t *slot = C->h_batch.p + (size_t)C->batch_slot * 2 * L;
+  C->batch_slot = (C->batch_slot + 1) % gp_ctx::kBatchSlots;
   for (int m = 0; m < L; m++) {
     C->batch[m] = C->queue[m];
-    tgt[m] = (uint8_t)(1 - C->cur[m]);
+    slot[m] = C->batch[m];
+    slot[L + m] = (uint8_t)(1 - C->cur[m]);
     if (C->batch[m]) C->pend[m] = 1;
     C->queue[m] = 0;
   }
-  HIPCHK(hipMemcpyAsync(C->d_req.p, C->batch.data(), L, hipMemcpyHostToDevice, C->stream));
-  HIPCHK(hipMemcpyAsync(C->d_tgt.p, tgt.data(), L, hipMemcpyHostToDevice, C->stream));
-  HIPCHK(hipMemcpyAsync(C->d_pend.p, C->pend.data(), L, hipMemcpyHostToDevice, C->stream));
+  HIPCHK(hipMemcpyAsync(C->d_batch.p, slot, 2 * L, hipMemcpyHostToDevice, C->stream));
+  launch_async_flags(L, 1, C->d_req.p, C->d_cur.p, C->d_pend.p, C->stream);
   launch_copy_active(std::max(1, C->n_svar), L, C->d_svar.p, C->d_svar_snap.p, C->d_req.p, C->stream);
   HIPCHK(hipEventRecord(C->ev_snap, C->stream));
   HIPCHK(hipStreamWaitEvent(C->fstream, C->ev_snap, 0));
@@ -889,8 +900,6 @@ void launch_batch(gp_ctx *C, TF *luv, TF *sv, TF *dn) {
   prof_end(C->prof_factor, pi, C->fstream, nb);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(C->ev_done, C->fstream));
-  // the host copies of batch/tgt/pend must outlive the copies above
-  HIPCHK(hipStreamSynchronize(C->stream));
   C->inflight = true;
 }
 
@@ -911,9 +920,9 @@ bool land_batch(gp_ctx *C, bool wait) {
       C->pend[m] = 0;
       C->batch[m] = 0;
     }
-  HIPCHK(hipMemcpyAsync(C->d_cur.p, C->cur.data(), L, hipMemcpyHostToDevice, C->stream));
-  HIPCHK(hipMemcpyAsync(C->d_pend.p, C->pend.data(), L, hipMemcpyHostToDevice, C->stream));
-  HIPCHK(hipStreamSynchronize(C->stream));
+  // the same update on the device, from the batch's request mask (no longer
+  // read by the finished factorisation)
+  launch_async_flags(L, 0, C->d_req.p, C->d_cur.p, C->d_pend.p, C->stream);
   C->inflight = false;
   return true;
 }
@@ -1038,11 +1047,10 @@ struct Newton {
       // (made for the old system) are dropped
       if (async && land_first) {
         land_batch(C, true);
-        if (fresh) {
+        if (fresh) {   // fresh is on the device as this round's refac0 mask
           for (int m = 0; m < L; m++)
             if ((*fresh)[m]) C->queue[m] = C->pend[m] = 0;
-          HIPCHK(hipMemcpyAsync(C->d_pend.p, C->pend.data(), L, hipMemcpyHostToDevice, C->stream));
-          HIPCHK(hipStreamSynchronize(C->stream));   // the host copy stays valid until then
+          launch_async_flags(L, 2, C->d_refac0.p, C->d_cur.p, C->d_pend.p, C->stream);
         }
       }
       out.refac += cnt[1];
@@ -1061,8 +1069,7 @@ struct Newton {
       // its |F| stops decreasing), launched when no batch is in flight
       for (int m = 0; m < L; m++)
         if (reqp[m]) C->queue[m] = C->pend[m] = 1;
-      HIPCHK(hipMemcpyAsync(C->d_pend.p, C->pend.data(), L, hipMemcpyHostToDevice, C->stream));
-      HIPCHK(hipStreamSynchronize(C->stream));   // the host copy stays valid until then
+      launch_async_flags(L, 1, C->d_areq.p, C->d_cur.p, C->d_pend.p, C->stream);
     }
     if (async && !C->inflight && queued()) launch_queue();
     const int ps = cnt[2] > 0 ? prof_begin(C, C->prof_solve, C->stream) : -1;

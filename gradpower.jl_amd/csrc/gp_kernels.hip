@@ -2267,6 +2267,21 @@ __global__ void __launch_bounds__(64) k_adj_param(DevTables T, const double *__r
 }
 
 // ---------------------------------------------------------------- small utilities
+// Flags of the asynchronous refactorisation for the scenarios in a:
+// mode 0 (a batch landed): cur ^= 1, pend = 0; mode 1 (requested): pend = 1;
+// mode 2 (request dropped): pend = 0.
+__global__ void k_async_flags(int L, int mode, const uint8_t *__restrict__ a, uint8_t *__restrict__ cur,
+                              uint8_t *__restrict__ pend) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= L || !a[m]) return;
+  if (mode == 0) {
+    cur[m] = (uint8_t)(1 - cur[m]);
+    pend[m] = 0;
+  } else {
+    pend[m] = mode == 1 ? 1 : 0;
+  }
+}
+
 __global__ void k_copy_active(int n, int ldM, const double *__restrict__ src, double *__restrict__ dst,
                               const uint8_t *__restrict__ act) {
   const int m = blockIdx.y * WAVE + threadIdx.x;
@@ -2413,6 +2428,9 @@ void launch_gather_traj(int nrec, int len, int ldM, int m, const double *hist, d
   const size_t tot = (size_t)nrec * len;
   if (tot == 0) return;
   hipLaunchKernelGGL(k_gather_traj, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nrec, len, ldM, m, hist, out);
+}
+void launch_async_flags(int L, int mode, const uint8_t *a, uint8_t *cur, uint8_t *pend, hipStream_t s) {
+  hipLaunchKernelGGL(k_async_flags, dim3((L + 255) / 256), dim3(256), 0, s, L, mode, a, cur, pend);
 }
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s) {
   const int gx = n < 1024 ? n : 1024;
