@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -143,13 +144,18 @@ struct LUPlan {
   // LDS bytes of the solve's x (16-byte aligned)
   int x_bytes() const { return (int)(((size_t)A.n * 8 + 15) / 16 * 16); }
 
-  // Schedules and uploads the analysis.  The solve keeps x (8 bytes per row),
-  // its level tables and the dense block in LDS, which bounds the block.
-  void upload(const std::vector<int> &asrc) {
+  // Largest dense trailing block the solve's LDS allows (x, 8 bytes per row,
+  // stays resident beside it).
+  int nd_bound() const {
     const long long room = std::min<long long>(kLdsBudget, (long long)kLdsMax - 8LL * A.n - 16384);
     int nd_max = 0;
     while ((long long)(nd_max + 16) * (nd_max + 16) * 8 <= room) nd_max += 16;
-    splu_schedule(A, kThreads, nd_max);
+    return nd_max;
+  }
+  // Schedules and uploads the analysis, the dense block at most nd_cap rows
+  // (and what the solve's LDS allows).
+  void upload(const std::vector<int> &asrc, int nd_cap = 1 << 30) {
+    splu_schedule(A, kThreads, std::min(nd_bound(), nd_cap));
     if (x_bytes() + (long long)A.nd * A.nd * 8 > kLdsMax)
       throw GpError{GP_ERR_UNSUPPORTED, "system too large for the in-LDS triangular solve (" +
                                             std::to_string(A.n) + " rows)"};
@@ -245,6 +251,95 @@ struct LUPlan {
         dev.lds_factor = (int)need;
       }
     }
+  }
+
+  // Dense-block autotuner.  The schedule's cost model picks the dense
+  // trailing block within a bound; the block's size also decides whether the
+  // multipliers fit in LDS (the resident factorisation) and how long the
+  // solves' two dense substitutions take, which the model does not see.  So
+  // each bound in 16-row steps is planned and timed on the device -- one
+  // factorisation and one solve of one scenario's matrix (var0: the value
+  // array's column of scenario 0), median of 3 launches each -- and the bound
+  // with the least factorisation + w_solve * solve time is kept.
+  double tuned_us = 0.0;
+  int tuned_cap = -1;
+  template <typename TF>
+  double time_one(const double *var, const double *cst, const double *rhs, double *x, const uint8_t *one, int *st,
+                  double w_solve, hipStream_t s) {
+    DBuf<TF> lu, sv, dn;
+    lu.alloc(lu_elems());
+    sv.alloc(dev.ns);
+    HIPCHK(hipMemsetAsync(sv.p, 0, sv.n * sizeof(TF), s));   // padding slots stay zero
+    dn.alloc((size_t)std::max(1, A.nd * A.nd));
+    auto fac = [&]() { launch_splu_factor_impl<TF>(dev, var, cst, 1, lu.p, sv.p, dn.p, one, nullptr, 1, st, s); };
+    auto sol = [&]() {
+      launch_splu_solve<TF>(dev, sv.p, dn.p, rhs, 1, 1.0, nullptr, nullptr, x, one, nullptr, 1, s);
+    };
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto med = [&](const std::function<void()> &f) {
+      float t[3];
+      for (int r = 0; r < 3; r++) {
+        HIPCHK(hipEventRecord(e0, s));
+        f();
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&t[r], e0, e1));
+      }
+      std::sort(t, t + 3);
+      return 1e3 * (double)t[1];
+    };
+    fac();
+    sol();
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    const double tf = med(fac), ts = med(sol);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return tf + w_solve * ts;
+  }
+  // mode (gp_options.dense_block): 0 tune, < 0 the cost model alone, > 0 a
+  // fixed bound of that many rows
+  void tune(const std::vector<int> &asrc, const std::vector<double> &var0, const double *cst, size_t elem,
+            double w_solve, hipStream_t s, int mode) {
+    if (mode != 0) {
+      upload(asrc, mode > 0 ? mode : 1 << 30);
+      shape(elem);
+      return;
+    }
+    DBuf<double> v1, r1, x1;
+    DBuf<uint8_t> one;
+    DBuf<int> st;
+    v1.upload(var0);
+    r1.upload(std::vector<double>(A.n, 1.0));
+    x1.alloc(A.n);
+    one.upload(std::vector<uint8_t>(1, 1));
+    st.upload(std::vector<int>(1, 0));
+    double best = 1e300;
+    int best_cap = -1;
+    std::vector<int> seen;
+    for (int cap = nd_bound(); cap >= 0; cap -= 16) {
+      try {
+        upload(asrc, cap);
+        if (std::find(seen.begin(), seen.end(), A.nd) != seen.end()) continue;
+        seen.push_back(A.nd);
+        shape(elem);
+      } catch (const GpError &) {
+        continue;
+      }
+      const double t = elem == 4 ? time_one<float>(v1.p, cst, r1.p, x1.p, one.p, st.p, w_solve, s)
+                                 : time_one<double>(v1.p, cst, r1.p, x1.p, one.p, st.p, w_solve, s);
+      if (t < best) {
+        best = t;
+        best_cap = cap;
+      }
+    }
+    if (best_cap < 0) throw GpError{GP_ERR_UNSUPPORTED, "no dense-block size fits the LU kernels"};
+    upload(asrc, best_cap);
+    shape(elem);
+    tuned_us = best;
+    tuned_cap = best_cap;
   }
 };
 
@@ -763,8 +858,7 @@ void analyse(gp_ctx *C, int method, double dt) {
     if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "Jacobian analysis: " + e};
     std::vector<int> asrc(nnz);
     for (int i = 0; i < nnz; i++) asrc[i] = i;
-    C->luJ.upload(asrc);
-    C->luJ.shape(sizeof(double));
+    C->luJ.tune(asrc, jv, nullptr, sizeof(double), 1.0, C->stream, C->opt.dense_block);   // a solve per factorisation
     C->luJ.ready = true;
     C->luJ.dt = dt;
     C->d_luJ.alloc(C->luJ.lu_elems() * C->M);
@@ -797,7 +891,10 @@ void analyse(gp_ctx *C, int method, double dt) {
     }
     std::string e = splu_analyse(C->nred, C->scp.data(), C->sri.data(), sx.data(), C->luS.A);
     if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "reduced-network analysis: " + e};
-    C->luS.upload(C->s_src);
+    // a factorisation serves several solves (refactorisation policies), and
+    // the solves are on every round's critical path
+    C->luS.tune(C->s_src, sv, C->d_s_cst.p, method == GP_METHOD_SCHUR_FP32 ? sizeof(float) : sizeof(double), 2.0,
+                C->stream, C->opt.dense_block);
     C->luS.ready = true;
     C->luS.dt = dt;
   }
@@ -1562,8 +1659,7 @@ void analyse_transposed(gp_ctx *C, double dt) {
   for (int t = 0; t < nnz; t++) tv[t] = jv[tsrc[t]];
   std::string e = splu_analyse(n, tcp.data(), tri.data(), tv.data(), C->luJT.A);
   if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "transposed Jacobian analysis: " + e};
-  C->luJT.upload(tsrc);
-  C->luJT.shape(sizeof(double));
+  C->luJT.tune(tsrc, jv, nullptr, sizeof(double), 1.0, C->stream, C->opt.dense_block);
   C->luJT.ready = true;
   C->luJT.dt = dt;
   C->d_luJT.alloc(C->luJT.lu_elems() * C->M);

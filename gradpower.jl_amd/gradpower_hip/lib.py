@@ -36,7 +36,7 @@ class COptions(C.Structure):
     _fields_ = [("record_all", C.c_int32), ("n_record", C.c_int32),
                 ("record_idx", C.POINTER(C.c_int32)), ("refactor_policy", C.c_int32),
                 ("stall_window", C.c_int32), ("profile", C.c_int32), ("lock_step", C.c_int32),
-                ("reserved", C.c_int32 * 3),
+                ("dense_block", C.c_int32), ("reserved", C.c_int32 * 2),
                 ("refactor_theta", C.c_double),
                 ("async_below", C.c_double)]
 
@@ -131,7 +131,7 @@ class HipContext:
 
     def __init__(self, desc: SystemDesc, M: int, record_idx=None,
                  refactor_policy=REFACTOR_EVERY_ITER, stall_window=0, refactor_theta=0.0, async_below=0.0,
-                 profile=False, lock_step=False):
+                 profile=False, lock_step=False, dense_block=0):
         self.lib = load()
         self.desc = desc
         self.M = int(M)
@@ -153,6 +153,7 @@ class HipContext:
         opt.async_below = float(async_below)
         opt.profile = 1 if profile else 0
         opt.lock_step = 1 if lock_step else 0
+        opt.dense_block = int(dense_block)
         self._opt = opt
         h = C.c_void_p()
         self._check(self.lib.gp_create(C.byref(self._cs), self.M, C.byref(opt), C.byref(h)))

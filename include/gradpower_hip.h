@@ -151,7 +151,14 @@ typedef struct gp_options {
                               together (a step lasts as long as its slowest
                               scenario).  Per-scenario iterates are the same
                               either way for EVERY_ITER and FIRST_ITER          */
-  int32_t reserved[3];
+  int32_t dense_block;     /* size of the sparse LU's dense trailing block, chosen
+                              once per analysis: 0 (default) plans every bound in
+                              16-row steps and keeps the one whose factorisation
+                              and solve of scenario 0 are fastest on the device
+                              (a timing: contexts may choose differently and then
+                              round differently); -1 the schedule's cost model
+                              alone (reproducible); k > 0 at most k rows       */
+  int32_t reserved[2];
   double refactor_theta;   /* ADAPTIVE / ASYNC: refactorise when an iteration
                               reduces |F|inf by less than this factor (0: 30)   */
   double async_below;      /* ASYNC: a refactorisation asked for at |F|inf below

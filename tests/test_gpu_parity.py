@@ -158,7 +158,7 @@ def test_scenarios_advancing_alone_match_lock_step(name, method, policy):
           for m in range(M) for e in case.events[m]]
     out = []
     for lock in (False, True):
-        ctx = _ctx(case, M, refactor_policy=policy, lock_step=lock)
+        ctx = _ctx(case, M, refactor_policy=policy, lock_step=lock, dense_block=-1)   # same plan in both
         ctx.set_events(ev, ns)
         st = ctx.integrate(METHODS[method], case.dt, 1e-12, 50, raise_on_fail=False)
         status, steps = ctx.scenario_status()
