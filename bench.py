@@ -220,7 +220,11 @@ def roofline(run, method):
     tfile = os.path.join(ROOT, "profiles", "traffic_" + name.replace("<", "_").replace(">", "") + ".json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
-            traffic = json.load(fh).get("traffic_bytes_per_launch")
+            tj = json.load(fh)
+        # the counter runs' launches serve another mix of scenarios: scale
+        # their measured bytes per scenario to this run's launches
+        per_scen = tj.get("traffic_bytes_per_scenario")
+        traffic = per_scen * k[which + "_sc"] / n if per_scen else tj.get("traffic_bytes_per_launch")
     return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_us": avg_s * 1e6,
             "launches": n, "algorithmic_bytes_per_launch": bytes_per_launch,

@@ -3,10 +3,13 @@
     python3 -m gradpower_hip.prof_summary stats DIR OUT.md --cmd "..."
         per-kernel table (calls, total ms, average us, share) from *kernel_stats.csv
 
-    python3 -m gradpower_hip.prof_summary traffic FETCH_DIR WRITE_DIR KERNEL COPY_BYTES OUT.json --cmd "..."
+    python3 -m gradpower_hip.prof_summary traffic FETCH_DIR WRITE_DIR KERNEL COPY_BYTES OUT.json --cmd "..." \
+            [--scenarios SCENARIOS_PER_LAUNCH] [--last N]
         HBM bytes per launch of KERNEL from a FETCH_SIZE run and a WRITE_SIZE run
         (KiB per dispatch), each scaled by what the counter reads for the plain
-        copy kernel k_copy_plain, which reads and writes COPY_BYTES
+        copy kernel k_copy_plain, which reads and writes COPY_BYTES; with the
+        mean number of scenarios a launch served in those runs, also per scenario;
+        with --last N over the last N dispatches only (the workload's own)
 """
 import csv
 import glob
@@ -55,21 +58,24 @@ def stats(d, out, cmd):
     print(open(out).read())
 
 
-def _counter(d, kernel, counter):
+def _counter(d, kernel, counter, last=None):
     vals = []
     for r in _rows(d, "*counter_collection.csv"):
         if kernel in _col(r, "Kernel_Name", "KernelName", "Name") and \
                 _col(r, "Counter_Name", "CounterName") == counter:
-            vals.append(float(_col(r, "Counter_Value", "CounterValue")))
+            vals.append((int(_col(r, "Dispatch_Id", "DispatchId")), float(_col(r, "Counter_Value", "CounterValue"))))
     if not vals:
         raise SystemExit(f"no {counter} samples for {kernel} under {d}")
-    return sum(vals) / len(vals), len(vals)
+    vals.sort()
+    if last:
+        vals = vals[-int(last):]   # the workload's own launches (earlier ones: the plan autotuner's)
+    return sum(v for _, v in vals) / len(vals), len(vals)
 
 
-def traffic(fdir, wdir, kernel, copy_bytes, out, cmd):
+def traffic(fdir, wdir, kernel, copy_bytes, out, cmd, scen=None, last=None):
     copy_bytes = float(copy_bytes)
-    kf, nkf = _counter(fdir, kernel, "FETCH_SIZE")
-    kw, nkw = _counter(wdir, kernel, "WRITE_SIZE")
+    kf, nkf = _counter(fdir, kernel, "FETCH_SIZE", last)
+    kw, nkw = _counter(wdir, kernel, "WRITE_SIZE", last)
     cf, _ = _counter(fdir, "k_copy_plain", "FETCH_SIZE")
     cw, _ = _counter(wdir, "k_copy_plain", "WRITE_SIZE")
     sf, sw = copy_bytes / cf, copy_bytes / cw   # bytes per counter unit
@@ -78,6 +84,9 @@ def traffic(fdir, wdir, kernel, copy_bytes, out, cmd):
            "fetch_scale_bytes_per_unit": sf, "write_scale_bytes_per_unit": sw,
            "fetch_bytes_per_launch": kf * sf, "write_bytes_per_launch": kw * sw,
            "traffic_bytes_per_launch": kf * sf + kw * sw, "command": cmd}
+    if scen:
+        res["scenarios_per_launch"] = float(scen)
+        res["traffic_bytes_per_scenario"] = (kf * sf + kw * sw) / float(scen)
     with open(out, "w") as fh:
         json.dump(res, fh, indent=1)
     print(json.dumps(res, indent=1))
@@ -85,14 +94,23 @@ def traffic(fdir, wdir, kernel, copy_bytes, out, cmd):
 
 if __name__ == "__main__":
     a = sys.argv[1:]
-    cmd = ""
+    cmd, scen = "", None
     if "--cmd" in a:
         i = a.index("--cmd")
         cmd = a[i + 1]
         a = a[:i] + a[i + 2:]
+    if "--scenarios" in a:
+        i = a.index("--scenarios")
+        scen = a[i + 1]
+        a = a[:i] + a[i + 2:]
+    last = None
+    if "--last" in a:
+        i = a.index("--last")
+        last = a[i + 1]
+        a = a[:i] + a[i + 2:]
     if a[0] == "stats":
         stats(a[1], a[2], cmd)
     elif a[0] == "traffic":
-        traffic(a[1], a[2], a[3], a[4], a[5], cmd)
+        traffic(a[1], a[2], a[3], a[4], a[5], cmd, scen, last)
     else:
         raise SystemExit(__doc__)

@@ -24,6 +24,8 @@ def main():
     ap.add_argument("--horizon-cap", type=float, default=0.5)
     ap.add_argument("--method", default="schur32", choices=["schur32", "schur64", "mono"])
     ap.add_argument("--passes", type=int, default=1)
+    ap.add_argument("--stats-out", default="",
+                    help="write the batched LU launches' scenario counts of the last pass here (JSON)")
     ap.add_argument("--adjoint", action="store_true",
                     help="one gp_adjoint pass of the 200-bus, M=128, 3 s workload instead")
     args = ap.parse_args()
@@ -39,7 +41,8 @@ def main():
     print(f"setup {time.perf_counter() - t0:.1f} s", flush=True)
     meth = {"schur32": METHOD_SCHUR_FP32, "schur64": METHOD_SCHUR_FP64, "mono": METHOD_MONO_FP64}[args.method]
     pol = REFACTOR_EVERY_ITER if args.method == "mono" else REFACTOR_ASYNC
-    ctx = HipContext(case.desc, case.M, record_idx=np.zeros(1, np.int32), refactor_policy=pol, stall_window=8)
+    ctx = HipContext(case.desc, case.M, record_idx=np.zeros(1, np.int32), refactor_policy=pol, stall_window=8,
+                     profile=True)
     # calibration copy: n doubles device to device through the library's copy kernel
     n = int(args.copy_mb * 1e6 / 8)
     hip = C.CDLL("libamdhip64.so")
@@ -60,6 +63,12 @@ def main():
         st = ctx.integrate(meth, case.dt, 1e-10, 30, raise_on_fail=False)
         print(f"pass {it}: {st['scenario_steps']} steps, {st['newton_rounds']} rounds, "
               f"{st['elapsed_ms']:.1f} ms", flush=True)
+    if args.stats_out:
+        import json
+        with open(args.stats_out, "w") as fh:
+            json.dump({"solve_launches": st["solve_kernel_launches"], "solve_scenarios": st["solve_kernel_scenarios"],
+                       "factor_launches": st["factor_kernel_launches"],
+                       "factor_scenarios": st["factor_kernel_scenarios"]}, fh)
     ctx.close()
 
 
