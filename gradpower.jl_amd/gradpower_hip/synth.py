@@ -22,7 +22,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .system import (Bus, Gen, Load, Branch, Shunt, PowerSystem, PowerSystemDynamics,
-                     Genrou, IEESGO, TGOV1, SEXS, ESDC1A, IEEEST, ContingencyEvent,
+                     Genrou, Gensal, IEESGO, TGOV1, SEXS, ESDC1A, IEEEST, ContingencyEvent,
                      esdc1a_sat_coefficients, set_dynamics, DynamicProblem, PQ, PV, SLACK)
 from .network import build_network, runpf_inplace
 from .init import initialize_dynamics
@@ -207,7 +207,9 @@ def case0_2bus(tf=10.0):
     return _finish("c0_2bus", ps, psd, 1, events, [tf], meta)
 
 
-def case1_9bus(M=64, tf=1.0):
+def case1_9bus(M=64, tf=1.0, gensal=False):
+    """gensal: the machine at bus 4 is a GENSAL (the spec's machine data with
+    the GENSAL substitutions) instead of a GENROU."""
     rng = np.random.default_rng(1001)
     meta = {"seed": 1001, "zip_alpha": 0.4,
             "zip_note": "Z/I/P 0.4/0.3/0.3 mapped to alpha=0.4 constant-Z, 0.6 constant-P"}
@@ -242,14 +244,21 @@ def case1_9bus(M=64, tf=1.0):
     ps = PowerSystem(100.0, buses, gens, loads, branches, [])
     psd = PowerSystemDynamics()
     for b in gen_buses:
-        psd.add_device(_genrou(b + 1))
+        if gensal and b == 3:
+            g = C0_GENROU
+            psd.add_device(Gensal(b + 1, "1", g["x_d"], g["x_q"], g["x_dp"], g["x_ddp"], g["xl"], g["H"], g["D"],
+                                  g["T_d0p"], g["T_d0dp"], g["T_q0dp"], g["S1"], g["S2"]))
+        else:
+            psd.add_device(_genrou(b + 1))
         psd.add_device(_sexs(b + 1))
         psd.add_device(_tgov1(b + 1))
     events = []
     for m in range(M):
         events.append([ContingencyEvent(m % nb, float(rng.uniform(0.01, 0.2)), 0.02,
                                         float(rng.uniform(0.1, 0.2)))])
-    return _finish("c1_9bus", ps, psd, M, events, [tf] * M, meta, zip_alpha=0.4)
+    if gensal:
+        meta["gensal_bus"] = 4
+    return _finish("c1_9bus_gensal" if gensal else "c1_9bus", ps, psd, M, events, [tf] * M, meta, zip_alpha=0.4)
 
 
 def _rgg_edges(rng, n, mean_degree):

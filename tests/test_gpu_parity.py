@@ -171,3 +171,21 @@ def test_scenarios_advancing_alone_match_lock_step(name, method, policy):
             assert np.array_equal(ta[m], tb[m]), (name, method, m)
     assert st_free["newton_iters"] == st_lock["newton_iters"]
     assert st_free["scenario_steps"] == st_lock["scenario_steps"]
+
+
+@pytest.mark.parametrize("method,policy", [("mono", 0), ("schur32", 3)])
+def test_gensal_trajectory_matches_oracle(method, policy):
+    """GENSAL enters as a GENROU with x_qp = x_dp and T_q0p = T_d0p, as the
+    reference's parser builds it (src/generators.jl:41-66): the 9-bus case
+    with its bus-4 machine a GENSAL, trajectories within 1e-9 of the oracle."""
+    case = get_case("case1_9bus", gensal=True)
+    M = 8
+    ctx = _ctx(case, M, refactor_policy=policy)
+    ev = [(m, e.bus, e.rfault, int(round(e.ton / case.dt)), int(round(e.toff / case.dt)))
+          for m in range(M) for e in case.events[m]]
+    ctx.set_events(ev, case.nsteps[:M])
+    st = ctx.integrate(METHODS[method], case.dt, 1e-12, 50)
+    ref = _oracle_trajs(case, range(M), 1e-12)
+    for m in range(M):
+        dev = np.max(np.abs(ctx.trajectory(m).T - ref[m]))
+        assert dev < 1e-9, (method, m, dev, st)

@@ -68,3 +68,20 @@ def test_zip_alpha_and_static_generators_in_grid_case():
     assert len(zp) > 0
     alphas = case.desc.p[zp[:, 1] + 2]
     assert np.allclose(alphas, 0.4)
+
+
+def test_gensal_is_genrou_with_substitutions():
+    """src/generators.jl:41-66: GENSAL = GENROU with x_qp = x_dp, T_q0p = T_d0p;
+    the case initialises to an equilibrium."""
+    from conftest import get_case
+    from oracle.oracle import Oracle
+    case = get_case("case1_9bus", gensal=True)
+    p = case.desc.p
+    rows = case.desc.dev[0]
+    g = rows[1]                     # the bus-4 machine
+    P = p[g[2]:g[2] + 14]
+    assert P[3] == P[2] and P[9] == P[8]
+    other = p[rows[0][2]:rows[0][2] + 14]
+    assert other[3] != other[2] and other[9] != other[8]
+    f = Oracle(case.desc).residual(case.desc.z0, case.desc.z0, case.dt)
+    assert np.max(np.abs(f)) < 1e-9
