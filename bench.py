@@ -157,6 +157,7 @@ def run_adjoint(steps, warmup, n_fd, cpu_budget, no_cpu):
             f"{a['forward_ms']:.1f}, recompute+backward {a['backward_ms']:.1f}), adjoint steps "
             f"{a['adjoint_steps']}, excluded {a['n_excluded']}")
     st, _ = ctx.scenario_status()
+    info = ctx.symbolic_info()
     ctx.close()
     ok = [m for m in range(case.M) if st[m] == 0]
     t = float(np.median(times))
@@ -167,7 +168,8 @@ def run_adjoint(steps, warmup, n_fd, cpu_budget, no_cpu):
            "scenario_grads_per_s": len(ok) / t, "adjoint_steps_per_s": r["stats"]["adjoint_steps"] / t,
            "forward_ms": r["stats"]["forward_ms"], "backward_ms": r["stats"]["backward_ms"],
            "excluded_scenarios": case.M - len(ok), "n_params": len(params), "newton_tol": tol,
-           "dtype": "f64"}
+           "dtype": "f64", "lu_dense_block_j": info["nd_j"], "lu_dense_block_jt": info["nd_jt"],
+           "lu_factor_lds_resident_j": bool(info["res_j"])}
     if n_fd > 0:
         rng = np.random.default_rng(2024)
         worst, p0 = 0.0, case.desc.p.copy()
@@ -223,8 +225,8 @@ def roofline(run, method):
             tj = json.load(fh)
         # the counter runs' launches serve another mix of scenarios: scale
         # their measured bytes per scenario to this run's launches
-        per_scen = tj.get("traffic_bytes_per_scenario")
-        traffic = per_scen * k[which + "_sc"] / n if per_scen else tj.get("traffic_bytes_per_launch")
+        t_scen = tj.get("traffic_bytes_per_scenario")
+        traffic = t_scen * k[which + "_sc"] / n if t_scen else tj.get("traffic_bytes_per_launch")
     return {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_us": avg_s * 1e6,
             "launches": n, "algorithmic_bytes_per_launch": bytes_per_launch,
@@ -324,7 +326,9 @@ def main():
                   "factor_launches": st["factor_launches"], "newton_corrections": st["newton_iters"],
                   "corrections_per_step": st["newton_iters"] / max(1, st["scenario_steps"]),
                   "refactorizations": st["refactorizations"], "status": st["status"],
-                  "failed_scenarios": int(np.sum(r["status"] != 0))}
+                  "failed_scenarios": int(np.sum(r["status"] != 0)),
+                  "lu_dense_block": r["info"]["nd_j" if k == "mono" else "nd_s"],
+                  "lu_factor_lds_resident": bool(r["info"]["res_j" if k == "mono" else "res_s"])}
         if not args.no_profile and k != head_name:
             out[k]["roofline"] = roofline(r, mcode[k])
     if "schur32" in runs and "schur64" in runs:

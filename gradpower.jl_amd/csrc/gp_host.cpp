@@ -2269,6 +2269,11 @@ int gp_get_symbolic_info(gp_ctx *C, gp_symbolic_info *info) {
   I.levels_s = C->luS.ready ? (int)C->luS.A.fac_lp.size() - 1 : 0;
   I.n_clusters_nt = (int)C->cl_nt.size();
   I.w_total_nt = C->xrows;
+  I.nd_j = C->luJ.ready ? C->luJ.A.nd : -1;
+  I.nd_s = C->luS.ready ? C->luS.A.nd : -1;
+  I.nd_jt = C->luJT.ready ? C->luJT.A.nd : -1;
+  I.res_j = C->luJ.ready && C->luJ.dev.res_nsteps > 0;
+  I.res_s = C->luS.ready && C->luS.dev.res_nsteps > 0;
   *info = I;
   return GP_OK;
 }

@@ -57,7 +57,8 @@ class CSymInfo(C.Structure):
     _fields_ = [("n", C.c_int32), ("nnz_j", C.c_int32), ("nnz_lu_j", C.c_int32),
                 ("levels_j", C.c_int32), ("n_red", C.c_int32), ("nnz_s", C.c_int32),
                 ("nnz_lu_s", C.c_int32), ("levels_s", C.c_int32), ("n_clusters_nt", C.c_int32),
-                ("w_total_nt", C.c_int32), ("reserved", C.c_int32 * 6)]
+                ("w_total_nt", C.c_int32), ("nd_j", C.c_int32), ("nd_s", C.c_int32), ("nd_jt", C.c_int32),
+                ("res_j", C.c_int32), ("res_s", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class CObjective(C.Structure):

@@ -307,7 +307,10 @@ typedef struct gp_symbolic_info {
   int32_t n, nnz_j, nnz_lu_j, levels_j;
   int32_t n_red, nnz_s, nnz_lu_s, levels_s;
   int32_t n_clusters_nt, w_total_nt;
-  int32_t reserved[6];
+  int32_t nd_j, nd_s, nd_jt;   /* dense trailing block of each LU plan (autotuned) */
+  int32_t res_j, res_s;        /* 1: the plan's factorisation keeps its multipliers in LDS
+                                  (at the element size it was last shaped for)       */
+  int32_t reserved[1];
 } gp_symbolic_info;
 int gp_get_symbolic_info(gp_ctx *ctx, gp_symbolic_info *info);
 
