@@ -104,7 +104,8 @@ struct HBuf {
 // decided (L = padded scenario count, a multiple of 64):
 //   [first L][refac0 L][dt 8L][work L][counts 16][failed L][areq L]
 //   upload   [0, 11L+16)      download [10L, 13L+16)
-// and a second block for the accepted steps: [acc L][adv L][col 4L].
+// and a second block for the accepted steps:
+//   [acc L][adv L][col 4L][sv L][svslot 4L][ck L][ckslot 4L].
 struct RoundLayout {
   size_t L;
   explicit RoundLayout(size_t l) : L(l) {}
@@ -122,7 +123,11 @@ struct RoundLayout {
   size_t acc() const { return 0; }
   size_t adv() const { return L; }
   size_t col() const { return 2 * L; }
-  size_t post_bytes() const { return 6 * L; }
+  size_t sv() const { return 6 * L; }
+  size_t svslot() const { return 7 * L; }
+  size_t ck() const { return 11 * L; }
+  size_t ckslot() const { return 12 * L; }
+  size_t post_bytes() const { return 16 * L; }
 };
 
 struct LUPlan {
@@ -426,8 +431,8 @@ struct gp_ctx {
   // scenario-asynchronous time loop: per-scenario step, first-round flags,
   // record columns, accepted / advancing masks
   DBuf<double> d_dtv;
-  DBuf<uint8_t> d_first, d_acc, d_adv;
-  DBuf<int> d_col;
+  DBuf<uint8_t> d_first, d_acc, d_adv, d_sv, d_ck;
+  DBuf<int> d_col, d_svslot, d_ckslot;
   bool state_set = false;
   int hist_steps = 0;
   std::vector<int> traj_len;
@@ -799,6 +804,10 @@ void alloc_state(gp_ctx *C) {
     C->d_acc.view(q + R.acc(), L);
     C->d_adv.view(q + R.adv(), L);
     C->d_col.view(q + R.col(), L);
+    C->d_sv.view(q + R.sv(), L);
+    C->d_svslot.view(q + R.svslot(), L);
+    C->d_ck.view(q + R.ck(), L);
+    C->d_ckslot.view(q + R.ckslot(), L);
   }
   C->d_fbus.alloc((size_t)MAXFAULT * L);
   C->d_fg.alloc((size_t)MAXFAULT * L);
@@ -1453,6 +1462,17 @@ struct Stepper {
 };
 
 
+// What a scenario-asynchronous sweep runs and keeps.
+struct FreeSpec {
+  std::vector<int> k_first, k_last;   // per scenario: steps k_first..k_last (none when k_last < k_first)
+  std::vector<std::vector<const gp_event *>> active0;   // faults in force at k_first (empty: none)
+  bool record = true;                 // recorded channels of z_k into trajectory column k
+  double *seg = nullptr;              // all of z_k, seg_k0 < k <= seg_k1, into slot k - seg_k0 - 1
+  int seg_k0 = 0, seg_k1 = -1;
+  double *ckpt = nullptr;             // the state after step s*every (its events included) into slot s,
+  int every = 0, nck = 0;             // 1 <= s < nck
+};
+
 // The time loop with every scenario advancing on its own: each scenario
 // follows integrate! (src/dynamics.jl:548-640) step by step, but a round
 // evaluates F for every running scenario at its own step k_m, and the
@@ -1461,8 +1481,9 @@ struct Stepper {
 // the others receive a correction.  No scenario waits for a slower one, so
 // the rounds of a call are those of its slowest scenario, not the sum over
 // steps of the slowest scenario of each step.
-gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int itermax) {
+gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int itermax, const FreeSpec &F) {
   const int L = C->ldM, M = C->M, n = C->n;
+  const size_t nL = (size_t)n * L;
   gp_stats S{};
   std::vector<std::map<int, std::vector<const gp_event *>>> on(M), off(M);
   for (auto &e : C->events) {
@@ -1475,16 +1496,29 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
       everfail(L, 0);
   const RoundLayout R(L);
   uint8_t *hr = C->h_round.p, *hp = C->h_post.p;   // pinned staging (RoundLayout)
-  uint8_t *acc = hp + R.acc(), *adv = hp + R.adv();
-  int *col = reinterpret_cast<int *>(hp + R.col());
+  uint8_t *acc = hp + R.acc(), *adv = hp + R.adv(), *sv = hp + R.sv(), *ck = hp + R.ck();
+  int *col = reinterpret_cast<int *>(hp + R.col()), *svslot = reinterpret_cast<int *>(hp + R.svslot()),
+      *ckslot = reinterpret_cast<int *>(hp + R.ckslot());
   std::vector<double> dtv(L, dt);
   std::vector<int> fbus((size_t)MAXFAULT * L, -1);
   std::vector<double> fg((size_t)MAXFAULT * L, 0.0);
-  int maxsteps = 0;
+  auto set_faults = [&](int m) {
+    if ((int)active_ev[m].size() > MAXFAULT)
+      throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults in one scenario"};
+    for (int q = 0; q < MAXFAULT; q++) {
+      const bool f_on = q < (int)active_ev[m].size();
+      fbus[(size_t)q * L + m] = f_on ? active_ev[m][q]->bus : -1;
+      fg[(size_t)q * L + m] = f_on ? 1.0 / active_ev[m][q]->r_fault : 0.0;
+    }
+  };
+  int kmax = 0;
   for (int m = 0; m < M; m++) {
-    run[m] = C->nsteps[m] >= 1 ? 1 : 0;
+    k[m] = F.k_first[m];
+    run[m] = F.k_first[m] <= F.k_last[m] ? 1 : 0;
     first[m] = 1;
-    maxsteps = std::max(maxsteps, C->nsteps[m]);
+    kmax = std::max(kmax, F.k_last[m]);
+    if (!F.active0.empty()) active_ev[m] = F.active0[m];
+    set_faults(m);
   }
   std::memcpy(adv, run.data(), L);
   HIPCHK(hipMemcpyAsync(C->d_post.p, hp, R.post_bytes(), hipMemcpyHostToDevice, C->stream));
@@ -1494,7 +1528,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
   Newton N(C, method, dt, tol, itermax, true);
   N.dtv = C->d_dtv.p;
   bool fault_dirty = true;
-  const long long max_rounds = (long long)(maxsteps + 1) * 2 * (4 * itermax + 16);
+  const long long max_rounds = (long long)(kmax + 1) * 2 * (4 * itermax + 16);
   for (long long round = 0;; round++) {
     int nrun = 0;
     bool land_first = false;
@@ -1533,7 +1567,9 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
       throw GpError{GP_ERR_HIP, "internal: no scenario can make progress"};
     std::memset(acc, 0, L);
     std::memset(adv, 0, L);
-    bool any_acc = false, any_adv = false, any_fail = false;
+    std::memset(sv, 0, L);
+    std::memset(ck, 0, L);
+    bool any_acc = false, any_adv = false, any_fail = false, any_sv = false, any_ck = false;
     for (int m = 0; m < M; m++) {
       if (!run[m]) continue;
       first[m] = 0;
@@ -1548,9 +1584,16 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
       if (wk[m]) continue;   // still iterating
       bool next = true;
       if (mode[m] == 0) {    // step k accepted: record z_k, then its events
-        acc[m] = 1;
-        col[m] = k[m];
-        any_acc = true;
+        if (F.record) {
+          acc[m] = 1;
+          col[m] = k[m];
+          any_acc = true;
+        }
+        if (F.seg && k[m] > F.seg_k0 && k[m] <= F.seg_k1) {
+          sv[m] = 1;
+          svslot[m] = k[m] - F.seg_k0 - 1;
+          any_sv = true;
+        }
         stale[m] = 0;
         S.scenario_steps++;
         bool changed = false;
@@ -1569,13 +1612,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
             changed = true;
           }
         if (changed) {       // re-solve at dt = 0 under the new fault state
-          if ((int)active_ev[m].size() > MAXFAULT)
-            throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults in one scenario"};
-          for (int q = 0; q < MAXFAULT; q++) {
-            const bool f_on = q < (int)active_ev[m].size();
-            fbus[(size_t)q * L + m] = f_on ? active_ev[m][q]->bus : -1;
-            fg[(size_t)q * L + m] = f_on ? 1.0 / active_ev[m][q]->r_fault : 0.0;
-          }
+          set_faults(m);
           fault_dirty = true;
           S.event_resolves++;
           mode[m] = 1;
@@ -1590,9 +1627,14 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
         mode[m] = 0;
         stale[m] = 1;
       }
-      if (next) {
+      if (next) {            // step k complete, events included
+        if (F.ckpt && F.every > 0 && k[m] % F.every == 0 && k[m] / F.every < F.nck) {
+          ck[m] = 1;
+          ckslot[m] = k[m] / F.every;
+          any_ck = true;
+        }
         k[m]++;
-        if (k[m] > C->nsteps[m]) {
+        if (k[m] > F.k_last[m]) {
           run[m] = 0;
         } else {
           dtv[m] = dt;
@@ -1602,11 +1644,13 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
         }
       }
     }
-    if (any_acc || any_adv)
+    if (any_acc || any_adv || any_sv || any_ck)
       HIPCHK(hipMemcpyAsync(C->d_post.p, hp, R.post_bytes(), hipMemcpyHostToDevice, C->stream));
     if (any_acc)
       launch_record_at(C->nrec, C->rec_idx.empty() ? nullptr : C->d_rec_idx.p, L, M, C->d_z.p, C->d_hist.p,
                        C->d_col.p, C->d_acc.p, C->stream);
+    if (any_sv) launch_copy_cols(n, L, C->d_z.p, F.seg, nL, C->d_svslot.p, C->d_sv.p, C->stream);
+    if (any_ck) launch_copy_cols(n, L, C->d_z.p, F.ckpt, nL, C->d_ckslot.p, C->d_ck.p, C->stream);
     if (any_adv) launch_copy_active(n, L, C->d_z.p, C->d_zold.p, C->d_adv.p, C->stream);
     if (any_fail) {          // non-finite states
       std::vector<double> fn(L);
@@ -1626,6 +1670,63 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
   for (int m = 0; m < M; m++) S.n_not_converged += everfail[m];
   return S;
 }
+
+// The fault state in force during each step of every scenario, from the
+// event lists (activation then deactivation at the end of a step, as the
+// time loop applies them): one table version per change, the version of
+// every step, and the active lists at the requested steps.
+struct EventTimeline {
+  std::vector<int> ver;                           // [k] -> version in force during step k
+  std::vector<std::vector<int>> bus;              // per version: [MAXFAULT][ldM]
+  std::vector<std::vector<double>> g;
+  std::map<int, std::vector<std::vector<const gp_event *>>> active_at;   // k -> per scenario
+  EventTimeline(const gp_ctx *C, int maxsteps, const std::vector<int> &want) {
+    const int L = C->ldM, M = C->M;
+    std::vector<std::map<int, std::vector<const gp_event *>>> on(M), off(M);
+    for (auto &e : C->events) {
+      on[e.scenario][e.step_on].push_back(&e);
+      off[e.scenario][e.step_off].push_back(&e);
+    }
+    std::vector<std::vector<const gp_event *>> act(M);
+    std::vector<int> fb((size_t)MAXFAULT * L, -1);
+    std::vector<double> fgv((size_t)MAXFAULT * L, 0.0);
+    bus.push_back(fb);
+    g.push_back(fgv);
+    ver.assign(maxsteps + 2, 0);
+    for (int k = 1; k <= maxsteps + 1; k++) {
+      ver[k] = (int)bus.size() - 1;
+      if (std::find(want.begin(), want.end(), k) != want.end()) active_at[k] = act;
+      bool changed = false;
+      for (int m = 0; m < M; m++) {
+        bool ch = false;
+        auto ion = on[m].find(k);
+        if (ion != on[m].end())
+          for (const gp_event *e : ion->second) {
+            act[m].push_back(e);
+            ch = true;
+          }
+        auto ioff = off[m].find(k);
+        if (ioff != off[m].end())
+          for (const gp_event *e : ioff->second) {
+            auto it = std::find(act[m].begin(), act[m].end(), e);
+            if (it != act[m].end()) act[m].erase(it);
+            ch = true;
+          }
+        if (!ch) continue;
+        changed = true;
+        for (int q = 0; q < MAXFAULT; q++) {
+          const bool f_on = q < (int)act[m].size();
+          fb[(size_t)q * L + m] = f_on ? act[m][q]->bus : -1;
+          fgv[(size_t)q * L + m] = f_on ? 1.0 / act[m][q]->r_fault : 0.0;
+        }
+      }
+      if (changed) {
+        bus.push_back(fb);
+        g.push_back(fgv);
+      }
+    }
+  }
+};
 
 // LU plan of J^T for the adjoint's transposed solves: the CSC pattern of J^T
 // (the CSR pattern of J) whose values the factorisation reads from the
@@ -1851,7 +1952,10 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
       S = sp.S;
       for (int m = 0; m < M; m++) S.n_not_converged += sp.st.everfail[m];
     } else {
-      S = integrate_free(C, method, dt, tol, itermax);
+      FreeSpec F;
+      F.k_first.assign(M, 1);
+      F.k_last.assign(C->nsteps.begin(), C->nsteps.begin() + M);
+      S = integrate_free(C, method, dt, tol, itermax, F);
     }
     land_batch(C, true);   // no factorisation outlives the call
     S.elapsed_ms = S.solve_ms = tm.ms();
@@ -1919,7 +2023,13 @@ int gp_adjoint(gp_ctx *C, int32_t method, double dt, double tol, int32_t itermax
     int maxsteps = 0;
     for (int m = 0; m < M; m++) maxsteps = std::max(maxsteps, C->nsteps[m]);
     HIPCHK(hipMemcpyAsync(C->d_z.p, C->d_z0.p, nL * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
-    Stepper sp(C, method, dt, tol, itermax, maxsteps);   // no fault active at t = 0
+    {   // no fault active at t = 0 (the analyses evaluate scenario 0 there)
+      std::vector<int> fb((size_t)MAXFAULT * L, -1);
+      std::vector<double> fgz((size_t)MAXFAULT * L, 0.0);
+      HIPCHK(hipMemcpyAsync(C->d_fbus.p, fb.data(), fb.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
+      HIPCHK(hipMemcpyAsync(C->d_fg.p, fgz.data(), fgz.size() * sizeof(double), hipMemcpyHostToDevice, C->stream));
+      HIPCHK(hipStreamSynchronize(C->stream));
+    }
     prepare_method(C, method, dt);
     analyse_transposed(C, dt);
     C->d_hist.alloc((size_t)(maxsteps + 1) * C->nrec * L);
@@ -1938,26 +2048,31 @@ int gp_adjoint(gp_ctx *C, int32_t method, double dt, double tol, int32_t itermax
     C->d_bact.ensure(L);
     C->d_objrows.upload(rows);
     C->d_ent.upload(ent);
-    std::vector<Stepper::State> snaps(std::max(1, nseg));
+    // fault tables of every step, and the faults in force where each
+    // segment's recomputation starts
+    std::vector<int> starts;
+    for (int sg = 0; sg < nseg; sg++) starts.push_back(sg * SEG + 1);
+    const EventTimeline TL(C, maxsteps, starts);
     DevTimer total(C->stream);
-    // ---- forward sweep: the state after step s*SEG (and its events) is
-    // checkpoint s; the last segment's states are kept for the backward sweep
+    // ---- forward sweep (every scenario advancing on its own): the state
+    // after step s*SEG (and its events) is checkpoint s; the last segment's
+    // states are kept for the backward sweep
     record_initial(C);
     const int klast = (nseg - 1) * SEG;
-    int kdone = 0;   // last step any scenario ran
-    for (int k = 1; k <= maxsteps; k++) {
-      if ((k - 1) % SEG == 0) {
-        const int s = (k - 1) / SEG;
-        snaps[s] = sp.st;
-        launch_copy_plain(C->d_z.p, C->d_ckpt.p + (size_t)s * nL, (long long)nL, C->stream);
-      }
-      double *zs = k > klast ? C->d_seg.p + (size_t)(k - klast - 1) * nL : nullptr;
-      if (!sp.step(k, C->d_hist.p + (size_t)k * C->nrec * L, zs)) break;
-      kdone = k;
-    }
+    launch_copy_plain(C->d_z.p, C->d_ckpt.p, (long long)nL, C->stream);
+    FreeSpec F;
+    F.k_first.assign(M, 1);
+    F.k_last.assign(C->nsteps.begin(), C->nsteps.begin() + M);
+    F.seg = C->d_seg.p;
+    F.seg_k0 = klast;
+    F.seg_k1 = maxsteps;
+    F.ckpt = C->d_ckpt.p;
+    F.every = SEG;
+    F.nck = nseg;
+    const gp_stats fwd = integrate_free(C, method, dt, tol, itermax, F);
     land_batch(C, true);
     launch_copy_plain(C->d_z.p, C->d_ckpt.p + (size_t)nseg * nL, (long long)nL, C->stream);
-    const gp_stats fwd = sp.S;
+    int64_t newton_total = fwd.newton_iters;
     A.scenario_steps = fwd.scenario_steps;
     A.forward_ms = total.ms();
     // scenarios in the backward sweep: those whose forward sweep finished
@@ -1980,26 +2095,35 @@ int gp_adjoint(gp_ctx *C, int32_t method, double dt, double tol, int32_t itermax
     HIPCHK(hipMemsetAsync(C->d_val.p, 0, L * sizeof(double), C->stream));
     DevTimer back(C->stream);
     const SpluDev &JT = C->luJT.dev;
+    int hmax = 0;   // no scenario of the backward sweep reaches past hmax
+    for (int m = 0; m < M; m++) hmax = std::max(hmax, hz[m]);
     for (int s = nseg - 1; s >= 0; s--) {
-      // steps after kdone ran no scenario, so none takes part in their
-      // backward steps either
-      const int k0 = s * SEG, k1 = std::min(k0 + SEG, kdone);
+      const int k0 = s * SEG, k1 = std::min(k0 + SEG, hmax);
       if (k1 <= k0) continue;
       if (s != nseg - 1) {   // recompute the segment from its checkpoint
-        sp.restore(snaps[s]);
+        FreeSpec Rs;
+        Rs.k_first.assign(M, k0 + 1);
+        Rs.k_last.assign(M, 0);
+        for (int m = 0; m < M; m++)
+          if (hz[m] > k0) Rs.k_last[m] = std::min(k1, hz[m]);
+        Rs.active0 = TL.active_at.at(k0 + 1);
+        Rs.record = false;
+        Rs.seg = C->d_seg.p;
+        Rs.seg_k0 = k0;
+        Rs.seg_k1 = k1;
         launch_copy_plain(C->d_ckpt.p + (size_t)s * nL, C->d_z.p, (long long)nL, C->stream);
-        for (int k = k0 + 1; k <= k1; k++)
-          if (!sp.step(k, nullptr, C->d_seg.p + (size_t)(k - k0 - 1) * nL)) break;
+        newton_total += integrate_free(C, method, dt, tol, itermax, Rs).newton_iters;
+        land_batch(C, true);
       }
       int tab = -1;
       for (int k = k1; k > k0; k--) {
         // fault tables of forward step k (the "+2" alignment of
         // test/test_adjoint_event_offset.jl:3-24)
-        if (sp.step_tab[k] != tab) {
-          tab = sp.step_tab[k];
-          HIPCHK(hipMemcpyAsync(C->d_fbus.p, sp.tab_bus[tab].data(), sp.tab_bus[tab].size() * sizeof(int),
+        if (TL.ver[k] != tab) {
+          tab = TL.ver[k];
+          HIPCHK(hipMemcpyAsync(C->d_fbus.p, TL.bus[tab].data(), TL.bus[tab].size() * sizeof(int),
                                 hipMemcpyHostToDevice, C->stream));
-          HIPCHK(hipMemcpyAsync(C->d_fg.p, sp.tab_g[tab].data(), sp.tab_g[tab].size() * sizeof(double),
+          HIPCHK(hipMemcpyAsync(C->d_fg.p, TL.g[tab].data(), TL.g[tab].size() * sizeof(double),
                                 hipMemcpyHostToDevice, C->stream));
         }
         const double *zk = C->d_seg.p + (size_t)(k - k0 - 1) * nL;
@@ -2024,7 +2148,7 @@ int gp_adjoint(gp_ctx *C, int32_t method, double dt, double tol, int32_t itermax
     A.backward_ms = back.ms();
     A.elapsed_ms = total.ms();
     A.n_segments = nseg;
-    A.newton_iters = sp.S.newton_iters;
+    A.newton_iters = newton_total;
     // outputs
     std::vector<double> gacc((size_t)std::max(1, ne) * L), val(L), lam;
     HIPCHK(hipMemcpy(gacc.data(), C->d_gacc.p, gacc.size() * sizeof(double), hipMemcpyDeviceToHost));

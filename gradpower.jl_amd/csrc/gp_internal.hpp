@@ -131,6 +131,9 @@ void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const doubl
                        const int *scatter, double *zupd, double *xout, const uint8_t *work, const uint8_t *cur, int M,
                        hipStream_t s);
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s);
+// columns of the scenarios in mask into slot[m] of a stack of scenario-major arrays
+void launch_copy_cols(int n, int ldM, const double *src, double *dst, size_t stride, const int *slot,
+                      const uint8_t *mask, hipStream_t s);
 // asynchronous-refactorisation flags: mode 0 landed (cur ^= 1, pend = 0),
 // 1 requested (pend = 1), 2 dropped (pend = 0), for the scenarios in a
 void launch_async_flags(int L, int mode, const uint8_t *a, uint8_t *cur, uint8_t *pend, hipStream_t s);

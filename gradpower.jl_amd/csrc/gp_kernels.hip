@@ -2282,6 +2282,16 @@ __global__ void k_async_flags(int L, int mode, const uint8_t *__restrict__ a, ui
   }
 }
 
+// Columns of the scenarios in mask into their own slot of a stack of
+// scenario-major arrays: dst[slot[m]*stride + i*ldM + m] = src[i*ldM + m].
+__global__ void k_copy_cols(int n, int ldM, const double *__restrict__ src, double *__restrict__ dst, size_t stride,
+                            const int *__restrict__ slot, const uint8_t *__restrict__ mask) {
+  const int m = blockIdx.y * WAVE + threadIdx.x;
+  if (!mask[m]) return;
+  double *d = dst + (size_t)slot[m] * stride;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) d[(size_t)i * ldM + m] = src[(size_t)i * ldM + m];
+}
+
 __global__ void k_copy_active(int n, int ldM, const double *__restrict__ src, double *__restrict__ dst,
                               const uint8_t *__restrict__ act) {
   const int m = blockIdx.y * WAVE + threadIdx.x;
@@ -2428,6 +2438,11 @@ void launch_gather_traj(int nrec, int len, int ldM, int m, const double *hist, d
   const size_t tot = (size_t)nrec * len;
   if (tot == 0) return;
   hipLaunchKernelGGL(k_gather_traj, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nrec, len, ldM, m, hist, out);
+}
+void launch_copy_cols(int n, int ldM, const double *src, double *dst, size_t stride, const int *slot,
+                      const uint8_t *mask, hipStream_t s) {
+  const int gx = n < 1024 ? n : 1024;
+  hipLaunchKernelGGL(k_copy_cols, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, n, ldM, src, dst, stride, slot, mask);
 }
 void launch_async_flags(int L, int mode, const uint8_t *a, uint8_t *cur, uint8_t *pend, hipStream_t s) {
   hipLaunchKernelGGL(k_async_flags, dim3((L + 255) / 256), dim3(256), 0, s, L, mode, a, cur, pend);
