@@ -16,6 +16,8 @@
 // faults src/dynamics.jl:265-327,823-894; backward Euler src/dynamics.jl:339-500.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "../../include/gradpower_hip.h"
 #include "gp_internal.hpp"
 #include "gp_splu.hpp"
@@ -2385,20 +2387,32 @@ void launch_cluster_backsub(const DevTables &T, const TF *X, const double *f, co
   if (T.n_cl_nt == 0) return;
   hipLaunchKernelGGL(k_cluster_backsub<TF>, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, X, dv, z, work);
 }
+// Dynamic LDS above 64 KB needs the kernel's attribute raised once per
+// device; raised here to the whole 160 KB of a CU the first time a launch
+// needs more than 64 KB (an attribute call on every launch costs host time
+// on the Newton rounds' critical path).
+static void allow_lds(const void *f, std::atomic<unsigned> &done, int bytes) {
+  if (bytes <= 65536) return;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const unsigned bit = 1u << (dev & 31);
+  if (done.load(std::memory_order_acquire) & bit) return;
+  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  done.fetch_or(bit, std::memory_order_acq_rel);
+}
+
 template <typename TF>
 void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv, TF *sv, TF *dn,
                              const uint8_t *refac, const uint8_t *tgt, int M, int *status, hipStream_t s) {
   if (S.res_nsteps > 0) {
-    if (S.lds_factor > 65536)
-      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_splu_factor_res<TF>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, S.lds_factor);
+    static std::atomic<unsigned> done_res{0};
+    allow_lds(reinterpret_cast<const void *>(k_splu_factor_res<TF>), done_res, S.lds_factor);
     hipLaunchKernelGGL(k_splu_factor_res<TF>, dim3(M), dim3(S.threads), S.lds_factor, s, S, var, cst, ldM, luv, sv,
                        dn, refac, tgt, status);
     return;
   }
-  if (S.lds_factor > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_splu_factor<TF>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, S.lds_factor);
+  static std::atomic<unsigned> done_fac{0};
+  allow_lds(reinterpret_cast<const void *>(k_splu_factor<TF>), done_fac, S.lds_factor);
   hipLaunchKernelGGL(k_splu_factor<TF>, dim3(M), dim3(S.threads), S.lds_factor, s, S, var, cst, ldM, luv, sv, dn,
                      refac, tgt, status);
 }
@@ -2406,9 +2420,8 @@ template <typename TF>
 void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const double *rhs, int ldM, double sign,
                        const int *scatter, double *zupd, double *xout, const uint8_t *work, const uint8_t *cur, int M,
                        hipStream_t s) {
-  if (S.lds_solve > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_splu_solve<TF>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, S.lds_solve);
+  static std::atomic<unsigned> done_sol{0};
+  allow_lds(reinterpret_cast<const void *>(k_splu_solve<TF>), done_sol, S.lds_solve);
   hipLaunchKernelGGL(k_splu_solve<TF>, dim3(M), dim3(S.threads), S.lds_solve, s, S, sv, dn, rhs, ldM, sign, scatter,
                      zupd, xout, work, cur);
 }
