@@ -394,23 +394,32 @@ struct gp_ctx {
   DBuf<uint8_t> d_work, d_act, d_refac, d_refac0, d_sel;
   DBuf<uint8_t> d_round, d_post;   // per-round exchange blocks (RoundLayout); views point into them
   HBuf h_round, h_post;
-  // a batch's requests and target buffers [req L][tgt L] (d_req, d_tgt view
-  // it), uploaded from a ring of pinned slots: each round synchronises the
-  // stream at least once and launches at most one batch, so a slot is
-  // reused only after its copy has completed
+  // a batch's requests and target buffers [req L][tgt L] per lane, uploaded
+  // from a ring of pinned slots: each round synchronises the stream at least
+  // once and launches at most kLanes batches, so a slot is reused only after
+  // its copy has completed
   DBuf<uint8_t> d_batch;
   HBuf h_batch;
   int batch_slot = 0;
-  static constexpr int kBatchSlots = 4;
+  static constexpr int kLanes = 1;   // measured at C2: a second concurrent batch saves rounds but slows them as much
+  static constexpr int kBatchSlots = 4 * kLanes;
+  // asynchronous refactorisation (GP_REFACTOR_ASYNC): up to kLanes batches in
+  // flight, each on a stream of its own, factorising into the buffers their
+  // scenarios' solves do not read; a scenario is in at most one batch
+  struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_snap = nullptr, ev_done = nullptr;
+    bool inflight = false;
+    std::vector<uint8_t> batch;
+    uint8_t *req = nullptr, *tgt = nullptr;   // in d_batch
+    DBuf<double> snap;                        // the batch's matrix values
+  } lanes[kLanes];
   DBuf<int> d_fbus, d_nunconv, d_status, d_nstall, d_iters;
-  DBuf<double> d_fg, d_fbest, d_svar_snap, d_traj_tmp;
-  DBuf<uint8_t> d_failed, d_upd, d_cur, d_tgt, d_pend, d_req, d_areq, d_sch;
+  DBuf<double> d_fg, d_fbest, d_traj_tmp;
+  DBuf<uint8_t> d_failed, d_upd, d_cur, d_pend, d_areq, d_sch;
   // asynchronous refactorisation (GP_REFACTOR_ASYNC): one batch at a time on
   // a second stream, into the buffer the solves do not read
-  hipStream_t fstream = nullptr;
-  hipEvent_t ev_snap = nullptr, ev_done = nullptr;
-  bool inflight = false;
-  std::vector<uint8_t> cur, pend, batch, queue;
+  std::vector<uint8_t> cur, pend, queue;
   // kernel timing (gp_options.profile): event pairs around each launch
   struct KProf {
     std::vector<hipEvent_t> ev;
@@ -817,16 +826,18 @@ void alloc_state(gp_ctx *C) {
   C->d_upd.alloc(L);
   C->d_cur.alloc(L);
   C->d_pend.alloc(L);
-  C->d_batch.alloc(2 * L);
+  C->d_batch.alloc((size_t)gp_ctx::kLanes * 2 * L);
   C->h_batch.alloc((size_t)gp_ctx::kBatchSlots * 2 * L);
-  C->d_req.view(C->d_batch.p, L);
-  C->d_tgt.view(C->d_batch.p + L, L);
+  for (int l = 0; l < gp_ctx::kLanes; l++) {
+    C->lanes[l].req = C->d_batch.p + (size_t)l * 2 * L;
+    C->lanes[l].tgt = C->lanes[l].req + L;
+    C->lanes[l].batch.assign(L, 0);
+  }
   HIPCHK(hipMemset(C->d_cur.p, 0, L));
   HIPCHK(hipMemset(C->d_pend.p, 0, L));
   HIPCHK(hipMemset(C->d_upd.p, 0, L));
   C->cur.assign(L, 0);
   C->pend.assign(L, 0);
-  C->batch.assign(L, 0);
   C->queue.assign(L, 0);
   C->d_fbest.alloc(L);
   C->d_status.alloc(L);
@@ -878,7 +889,7 @@ void analyse(gp_ctx *C, int method, double dt) {
     if (C->luS.ready && C->luS.dt == dt) return;
     const int nnzS = C->scp[C->nred];
     C->d_svar.alloc((size_t)std::max(1, C->n_svar) * L);
-    C->d_svar_snap.alloc((size_t)std::max(1, C->n_svar) * L);
+    for (auto &ln : C->lanes) ln.snap.alloc((size_t)std::max(1, C->n_svar) * L);
     C->d_rhs.alloc((size_t)C->nred * L);
     C->d_dv.alloc((size_t)C->nred * L);
     C->d_X64.alloc((size_t)std::max(1, C->xrows) * 3 * L);
@@ -981,56 +992,75 @@ struct NewtonOut {
 // switch buffers.  The copy is taken on the main stream and the factorisation
 // waits for it, so it also waits for every solve that was still reading the
 // buffer it overwrites.
+bool any_inflight(const gp_ctx *C) {
+  for (const auto &ln : C->lanes)
+    if (ln.inflight) return true;
+  return false;
+}
+int free_lane(const gp_ctx *C) {
+  for (int l = 0; l < gp_ctx::kLanes; l++)
+    if (!C->lanes[l].inflight) return l;
+  return -1;
+}
+
 template <typename TF>
 void launch_batch(gp_ctx *C, TF *luv, TF *sv, TF *dn) {
   const int L = C->ldM;
+  const int l = free_lane(C);
+  if (l < 0) return;
+  gp_ctx::Lane &ln = C->lanes[l];
   uint8_t *slot = C->h_batch.p + (size_t)C->batch_slot * 2 * L;
   C->batch_slot = (C->batch_slot + 1) % gp_ctx::kBatchSlots;
+  int nb = 0;
   for (int m = 0; m < L; m++) {
-    C->batch[m] = C->queue[m];
-    slot[m] = C->batch[m];
+    ln.batch[m] = C->queue[m];
+    slot[m] = ln.batch[m];
     slot[L + m] = (uint8_t)(1 - C->cur[m]);
-    if (C->batch[m]) C->pend[m] = 1;
+    if (ln.batch[m]) C->pend[m] = 1;
+    nb += ln.batch[m];
     C->queue[m] = 0;
   }
-  HIPCHK(hipMemcpyAsync(C->d_batch.p, slot, 2 * L, hipMemcpyHostToDevice, C->stream));
-  launch_async_flags(L, 1, C->d_req.p, C->d_cur.p, C->d_pend.p, C->stream);
-  launch_copy_active(std::max(1, C->n_svar), L, C->d_svar.p, C->d_svar_snap.p, C->d_req.p, C->stream);
-  HIPCHK(hipEventRecord(C->ev_snap, C->stream));
-  HIPCHK(hipStreamWaitEvent(C->fstream, C->ev_snap, 0));
-  int nb = 0;
-  for (int m = 0; m < L; m++) nb += C->batch[m];
-  const int pi = prof_begin(C, C->prof_factor, C->fstream);
-  launch_splu_factor_impl<TF>(C->luS.dev, C->d_svar_snap.p, C->d_s_cst.p, L, luv, sv, dn, C->d_req.p, C->d_tgt.p,
-                              C->M, C->d_status.p, C->fstream);
-  prof_end(C->prof_factor, pi, C->fstream, nb);
+  HIPCHK(hipMemcpyAsync(ln.req, slot, 2 * L, hipMemcpyHostToDevice, C->stream));
+  launch_async_flags(L, 1, ln.req, C->d_cur.p, C->d_pend.p, C->stream);
+  launch_copy_active(std::max(1, C->n_svar), L, C->d_svar.p, ln.snap.p, ln.req, C->stream);
+  HIPCHK(hipEventRecord(ln.ev_snap, C->stream));
+  HIPCHK(hipStreamWaitEvent(ln.stream, ln.ev_snap, 0));
+  const int pi = prof_begin(C, C->prof_factor, ln.stream);
+  launch_splu_factor_impl<TF>(C->luS.dev, ln.snap.p, C->d_s_cst.p, L, luv, sv, dn, ln.req, ln.tgt, C->M,
+                              C->d_status.p, ln.stream);
+  prof_end(C->prof_factor, pi, ln.stream, nb);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(C->ev_done, C->fstream));
-  C->inflight = true;
+  HIPCHK(hipEventRecord(ln.ev_done, ln.stream));
+  ln.inflight = true;
 }
 
-// The batch in flight has finished (or `wait`): its scenarios switch buffers.
+// The batches in flight that have finished (all of them with `wait`): their
+// scenarios switch buffers.  Returns whether any landed.
 bool land_batch(gp_ctx *C, bool wait) {
-  if (!C->inflight) return false;
-  if (wait) {
-    HIPCHK(hipEventSynchronize(C->ev_done));
-  } else {
-    const hipError_t q = hipEventQuery(C->ev_done);
-    if (q == hipErrorNotReady) return false;
-    HIPCHK(q);
-  }
   const int L = C->ldM;
-  for (int m = 0; m < L; m++)
-    if (C->batch[m]) {
-      C->cur[m] = (uint8_t)(1 - C->cur[m]);
-      C->pend[m] = 0;
-      C->batch[m] = 0;
+  bool any = false;
+  for (auto &ln : C->lanes) {
+    if (!ln.inflight) continue;
+    if (wait) {
+      HIPCHK(hipEventSynchronize(ln.ev_done));
+    } else {
+      const hipError_t q = hipEventQuery(ln.ev_done);
+      if (q == hipErrorNotReady) continue;
+      HIPCHK(q);
     }
-  // the same update on the device, from the batch's request mask (no longer
-  // read by the finished factorisation)
-  launch_async_flags(L, 0, C->d_req.p, C->d_cur.p, C->d_pend.p, C->stream);
-  C->inflight = false;
-  return true;
+    for (int m = 0; m < L; m++)
+      if (ln.batch[m]) {
+        C->cur[m] = (uint8_t)(1 - C->cur[m]);
+        C->pend[m] = 0;
+        ln.batch[m] = 0;
+      }
+    // the same update on the device, from the batch's request mask (no
+    // longer read by the finished factorisation)
+    launch_async_flags(L, 0, ln.req, C->d_cur.p, C->d_pend.p, C->stream);
+    ln.inflight = false;
+    any = true;
+  }
+  return any;
 }
 
 // One Newton round engine (newton_step!, src/numerics.jl:55-176, batched):
@@ -1118,8 +1148,8 @@ struct Newton {
     const bool landed = async && land_batch(C, false);
     if (cnt[2] == 0 && cnt[3] == 0) {
       // every remaining scenario waits for its factorisation
-      if (async && !C->inflight && queued()) launch_queue();
-      if (C->inflight)
+      if (async && free_lane(C) >= 0 && queued()) launch_queue();
+      if (any_inflight(C))
         land_batch(C, true);
       else if (!landed)
         return false;   // nothing will change (cannot happen)
@@ -1177,7 +1207,7 @@ struct Newton {
         if (reqp[m]) C->queue[m] = C->pend[m] = 1;
       launch_async_flags(L, 1, C->d_areq.p, C->d_cur.p, C->d_pend.p, C->stream);
     }
-    if (async && !C->inflight && queued()) launch_queue();
+    if (async && free_lane(C) >= 0 && queued()) launch_queue();
     const int ps = cnt[2] > 0 ? prof_begin(C, C->prof_solve, C->stream) : -1;
     if (fp32)
       launch_splu_solve<float>(C->luS.dev, C->d_svS32.p, C->d_dnS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
@@ -1816,9 +1846,11 @@ int gp_create(const gp_system_desc *d, int32_t M, const gp_options *opt, gp_ctx 
   try {
     HIPCHK(hipGetDevice(&C->device));
     HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
-    HIPCHK(hipStreamCreateWithFlags(&C->fstream, hipStreamNonBlocking));
-    HIPCHK(hipEventCreateWithFlags(&C->ev_snap, hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&C->ev_done, hipEventDisableTiming));
+    for (auto &ln : C->lanes) {
+      HIPCHK(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&ln.ev_snap, hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&ln.ev_done, hipEventDisableTiming));
+    }
     C->n = d->n; C->nbus = d->nbus; C->net_ptr = d->net_ptr; C->npar = d->n_par; C->M = M;
     C->ldM = (M + WAVE - 1) / WAVE * WAVE;
     if (opt) C->opt = *opt;
@@ -1876,16 +1908,24 @@ int gp_create(const gp_system_desc *d, int32_t M, const gp_options *opt, gp_ctx 
 
 void gp_destroy(gp_ctx *C) {
   if (!C) return;
-  if (C->fstream) (void)hipStreamSynchronize(C->fstream);
+  for (auto &ln : C->lanes)
+    if (ln.stream) (void)hipStreamSynchronize(ln.stream);
   if (C->stream) (void)hipStreamSynchronize(C->stream);
-  hipStream_t s = C->stream, fs = C->fstream;
-  hipEvent_t e1 = C->ev_snap, e2 = C->ev_done;
+  hipStream_t s = C->stream;
+  std::vector<hipStream_t> ls;
+  std::vector<hipEvent_t> le;
+  for (auto &ln : C->lanes) {
+    ls.push_back(ln.stream);
+    le.push_back(ln.ev_snap);
+    le.push_back(ln.ev_done);
+  }
   for (hipEvent_t e : C->prof_solve.ev) (void)hipEventDestroy(e);
   for (hipEvent_t e : C->prof_factor.ev) (void)hipEventDestroy(e);
   delete C;
-  if (e1) (void)hipEventDestroy(e1);
-  if (e2) (void)hipEventDestroy(e2);
-  if (fs) (void)hipStreamDestroy(fs);
+  for (hipEvent_t e : le)
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t x : ls)
+    if (x) (void)hipStreamDestroy(x);
   if (s) (void)hipStreamDestroy(s);
 }
 
@@ -1960,7 +2000,7 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
     land_batch(C, true);   // no factorisation outlives the call
     S.elapsed_ms = S.solve_ms = tm.ms();
     if (C->opt.profile) {
-      HIPCHK(hipStreamSynchronize(C->fstream));
+      for (auto &ln : C->lanes) HIPCHK(hipStreamSynchronize(ln.stream));
       S.solve_kernel_ms = prof_collect(C->prof_solve, S.solve_kernel_launches, S.solve_kernel_scenarios);
       S.factor_kernel_ms = prof_collect(C->prof_factor, S.factor_kernel_launches, S.factor_kernel_scenarios);
       const bool mono = method == GP_METHOD_MONO_FP64;
@@ -1976,7 +2016,7 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
     try {
       land_batch(C, true);
     } catch (const GpError &) {
-      C->inflight = false;
+      for (auto &ln : C->lanes) ln.inflight = false;
     }
     if (st) *st = S;
     return fail(e.code, e.msg);
@@ -2190,7 +2230,7 @@ int gp_adjoint(gp_ctx *C, int32_t method, double dt, double tol, int32_t itermax
     try {
       land_batch(C, true);
     } catch (const GpError &) {
-      C->inflight = false;
+      for (auto &ln : C->lanes) ln.inflight = false;
     }
     if (ast) *ast = A;
     return fail(e.code, e.msg);
