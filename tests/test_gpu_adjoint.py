@@ -207,3 +207,26 @@ def test_adjoint_200bus_vs_central_differences():
         print(f"param {q}: adjoint {r['grad'][j]:.12e} fd {fd:.12e} rel {rel:.2e}", flush=True)
         worst = max(worst, rel)
     assert worst <= 1e-5
+
+
+def test_adjoint_ragged_horizons_and_event_tables():
+    """The adjoint with ragged horizons (0 and 1 step included) and awkward
+    event tables (on and off in one step, overlapping faults, a fault past
+    the horizon or never cleared), short segments, against the oracle."""
+    from test_gpu_parity import _edge_events
+    case = get_case("case1_9bus")
+    nsteps, ev = _edge_events(case)
+    M = len(nsteps)
+    ctx = _ctx(case, M)
+    ctx.set_events(ev, nsteps)
+    params = _gains(case)
+    r = ctx.adjoint(MONO, case.dt, params, tol=1e-12, segment=7, scale=0.5, want_dz0=True)
+    o = _oracle(case)
+    for m in range(M):
+        val, g, dz0 = o.adjoint(case.dt, nsteps[m], params, [e[1:] for e in ev if e[0] == m], tol=1e-12, scale=0.5)
+        if nsteps[m] == 0:
+            assert r["value_scen"][m] == 0.0 and np.all(r["grad_scen"][m] == 0.0) and np.all(r["dz0"][m] == 0.0)
+            continue
+        assert r["value_scen"][m] == pytest.approx(val, rel=1e-9, abs=1e-18), m
+        assert np.max(np.abs(r["grad_scen"][m] - g)) <= 1e-7 * np.max(np.abs(g)) + 1e-15, m
+        assert np.max(np.abs(r["dz0"][m] - dz0)) <= 1e-7 * np.max(np.abs(dz0)) + 1e-15, m

@@ -189,3 +189,39 @@ def test_gensal_trajectory_matches_oracle(method, policy):
     for m in range(M):
         dev = np.max(np.abs(ctx.trajectory(m).T - ref[m]))
         assert dev < 1e-9, (method, m, dev, st)
+
+
+def _edge_events(case):
+    """Ragged horizons (0, 1, 37, 120, 60 steps) and awkward fault tables:
+    on and off in the same step, two overlapping faults, a fault past the
+    horizon, a fault that never clears."""
+    b = case.desc.nbus
+    nsteps = [0, 1, 37, 120, 60]
+    ev = [(0, 1, 0.05, 1, 2),
+          (1, 2, 0.05, 1, 1),
+          (2, 3, 0.08, 3, 20), (2, 5, 0.1, 10, 30),
+          (3, 4, 0.05, 200, 210), (3, 6, 0.07, 2, 6),
+          (4, b - 1, 0.05, 5, 500)]
+    return nsteps, ev
+
+
+@pytest.mark.parametrize("method,policy,lock", [("mono", 0, False), ("mono", 0, True), ("schur64", 2, False),
+                                                ("schur32", 3, False), ("schur32", 3, True)])
+def test_ragged_horizons_and_event_tables_match_oracle(method, policy, lock):
+    """Five scenarios (M not a multiple of 64) with ragged horizons, including
+    0 and 1 step, and the awkward event tables above, against the oracle."""
+    case = get_case("case1_9bus")
+    nsteps, ev = _edge_events(case)
+    M = len(nsteps)
+    ctx = _ctx(case, M, refactor_policy=policy, lock_step=lock)
+    ctx.set_events(ev, nsteps)
+    st = ctx.integrate(METHODS[method], case.dt, 1e-12, 50)
+    assert st["scenario_steps"] == sum(nsteps)
+    o = _oracle(case)
+    for m in range(M):
+        tr = ctx.trajectory(m)
+        assert tr.shape == (case.desc.n, nsteps[m] + 1)
+        ref, _, rs = o.integrate(case.dt, nsteps[m], [e[1:] for e in ev if e[0] == m], tol=1e-12, itermax=50)
+        assert rs["not_converged"] == 0
+        dev = np.max(np.abs(tr.T - ref))
+        assert dev < 1e-9, (method, lock, m, dev)
