@@ -411,6 +411,7 @@ struct gp_ctx {
     hipEvent_t ev_snap = nullptr, ev_done = nullptr;
     bool inflight = false;
     std::vector<uint8_t> batch;
+    int nb = 0;                               // scenarios in the batch
     uint8_t *req = nullptr, *tgt = nullptr;   // in d_batch
     DBuf<double> snap;                        // the batch's matrix values
   } lanes[kLanes];
@@ -1003,11 +1004,14 @@ int free_lane(const gp_ctx *C) {
   return -1;
 }
 
-template <typename TF>
-void launch_batch(gp_ctx *C, TF *luv, TF *sv, TF *dn) {
+// A batch is staged on the main stream (its masks, the pending flags, the
+// snapshot of its matrix values) and then started on its lane: the caller
+// launches the round's solve between the two, so that the host calls that
+// start the factorisation overlap the solve instead of delaying it.
+int stage_batch(gp_ctx *C) {
   const int L = C->ldM;
   const int l = free_lane(C);
-  if (l < 0) return;
+  if (l < 0) return -1;
   gp_ctx::Lane &ln = C->lanes[l];
   uint8_t *slot = C->h_batch.p + (size_t)C->batch_slot * 2 * L;
   C->batch_slot = (C->batch_slot + 1) % gp_ctx::kBatchSlots;
@@ -1024,6 +1028,14 @@ void launch_batch(gp_ctx *C, TF *luv, TF *sv, TF *dn) {
   launch_async_flags(L, 1, ln.req, C->d_cur.p, C->d_pend.p, C->stream);
   launch_copy_active(std::max(1, C->n_svar), L, C->d_svar.p, ln.snap.p, ln.req, C->stream);
   HIPCHK(hipEventRecord(ln.ev_snap, C->stream));
+  ln.nb = nb;
+  return l;
+}
+
+template <typename TF>
+void start_batch(gp_ctx *C, int l, TF *luv, TF *sv, TF *dn) {
+  gp_ctx::Lane &ln = C->lanes[l];
+  const int L = C->ldM, nb = ln.nb;
   HIPCHK(hipStreamWaitEvent(ln.stream, ln.ev_snap, 0));
   const int pi = prof_begin(C, C->prof_factor, ln.stream);
   launch_splu_factor_impl<TF>(C->luS.dev, ln.snap.p, C->d_s_cst.p, L, luv, sv, dn, ln.req, ln.tgt, C->M,
@@ -1123,16 +1135,25 @@ struct Newton {
       if (C->queue[m]) return true;
     return false;
   }
-  void launch_queue() {
+  // stage the queued requests as a batch (-1: no lane free), then start it
+  int stage_queue() {
     int nb = 0;
     for (int m = 0; m < C->ldM; m++) nb += C->queue[m];
-    out.refac += nb;
-    out.factor_launches++;
-    if (fp32)
-      launch_batch<float>(C, C->d_luS32.p, C->d_svS32.p, C->d_dnS32.p);
-    else
-      launch_batch<double>(C, C->d_luS64.p, C->d_svS64.p, C->d_dnS64.p);
+    const int l = stage_batch(C);
+    if (l >= 0) {
+      out.refac += nb;
+      out.factor_launches++;
+    }
+    return l;
   }
+  void start_queue(int l) {
+    if (l < 0) return;
+    if (fp32)
+      start_batch<float>(C, l, C->d_luS32.p, C->d_svS32.p, C->d_dnS32.p);
+    else
+      start_batch<double>(C, l, C->d_luS64.p, C->d_svS64.p, C->d_dnS64.p);
+  }
+  void launch_queue() { start_queue(stage_queue()); }
 
   // One round's corrections after eval() (cnt read back).  land_first: a
   // scenario starts a solve that refactorises (an event or a change of dt):
@@ -1207,7 +1228,7 @@ struct Newton {
         if (reqp[m]) C->queue[m] = C->pend[m] = 1;
       launch_async_flags(L, 1, C->d_areq.p, C->d_cur.p, C->d_pend.p, C->stream);
     }
-    if (async && free_lane(C) >= 0 && queued()) launch_queue();
+    const int lane = (async && free_lane(C) >= 0 && queued()) ? stage_queue() : -1;
     const int ps = cnt[2] > 0 ? prof_begin(C, C->prof_solve, C->stream) : -1;
     if (fp32)
       launch_splu_solve<float>(C->luS.dev, C->d_svS32.p, C->d_dnS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
@@ -1220,6 +1241,7 @@ struct Newton {
       launch_cluster_backsub<float>(C->T, C->d_X32.p, C->d_f.p, C->d_dv.p, z, C->d_upd.p, C->stream);
     else
       launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, C->d_upd.p, C->stream);
+    start_queue(lane);
     HIPCHK(hipGetLastError());
     return true;
   }
@@ -1325,33 +1347,33 @@ int finish_status(gp_ctx *C, const gp_stats &S) {
   return GP_OK;
 }
 
-// The time loop of integrate! (src/dynamics.jl:548-640) one step at a time,
-// for every scenario still running: the Newton solve of step k, the record
-// of z_k, then the fault events at step k (activation, then deactivation)
-// and the dt = 0 re-solve of the scenarios they touch.
+// The time loop of integrate! (src/dynamics.jl:548-640) in lock step
+// (gp_options.lock_step), one step at a time for every scenario still
+// running: the Newton solve of step k, the record of z_k, then the fault
+// events at step k (activation, then deactivation) and the dt = 0 re-solve
+// of the scenarios they touch.
 struct Stepper {
   gp_ctx *C;
   int method, itermax;
   double dt, tol;
   std::map<int, std::vector<const gp_event *>> on_at, off_at;
-  // per-scenario state that a checkpoint restores
+  // per-scenario state
   struct State {
     std::vector<std::vector<const gp_event *>> active_ev;   // active faults, in event order
     std::vector<uint8_t> dead, everfail;
     std::vector<uint8_t> stale;   // factorisation made with another dt or before an event
     std::vector<int> fbus;        // fault tables [MAXFAULT][ldM]
     std::vector<double> fg;
-    int ftab = 0;                 // index of the fault tables in `tabs`
+    int ftab = 0;                 // index of the fault tables in tab_bus / tab_g
   } st;
   std::vector<uint8_t> act, res, need0;
   gp_stats S{};
-  // every fault table the sweep used (version 0: no fault) and, per step k,
-  // the version in force during step k
+  // every fault table the sweep used (version 0: no fault), kept so that
+  // the asynchronous uploads read memory that stays valid
   std::vector<std::vector<int>> tab_bus;
   std::vector<std::vector<double>> tab_g;
-  std::vector<int> step_tab;
 
-  Stepper(gp_ctx *c, int meth, double dt_, double tol_, int itmax, int maxsteps)
+  Stepper(gp_ctx *c, int meth, double dt_, double tol_, int itmax)
       : C(c), method(meth), itermax(itmax), dt(dt_), tol(tol_) {
     const int L = C->ldM, M = C->M;
     for (auto &e : C->events) {
@@ -1369,7 +1391,6 @@ struct Stepper {
     need0.assign(L, 0);
     tab_bus.push_back(st.fbus);
     tab_g.push_back(st.fg);
-    step_tab.assign(maxsteps + 2, 0);
     upload_faults();
   }
   void upload_faults() {
@@ -1377,10 +1398,6 @@ struct Stepper {
                           C->stream));
     HIPCHK(hipMemcpyAsync(C->d_fg.p, tab_g[st.ftab].data(), st.fg.size() * sizeof(double), hipMemcpyHostToDevice,
                           C->stream));
-  }
-  void restore(const State &s) {
-    st = s;
-    upload_faults();
   }
   void add_stats(const NewtonOut &no) {
     S.newton_iters += no.corrections;
@@ -1397,7 +1414,6 @@ struct Stepper {
     int nact = 0;
     for (int m = 0; m < M; m++) nact += act[m];
     if (nact == 0) return false;
-    step_tab[k] = st.ftab;
     set_mask(C, C->d_act, act);
     launch_copy_active(C->n, L, C->d_z.p, C->d_zold.p, C->d_act.p, C->stream);
     for (int m = 0; m < L; m++) need0[m] = act[m] && st.stale[m];
@@ -1975,7 +1991,7 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
     const int L = C->ldM, M = C->M;
     int maxsteps = 0;
     for (int m = 0; m < M; m++) maxsteps = std::max(maxsteps, C->nsteps[m]);
-    Stepper sp(C, method, dt, tol, itermax, maxsteps);   // no fault active at t = 0
+    Stepper sp(C, method, dt, tol, itermax);   // no fault active at t = 0
     prepare_method(C, method, dt);
     // trajectory storage [step][channel][ldM]
     C->d_hist.alloc((size_t)(maxsteps + 1) * C->nrec * L);
