@@ -140,7 +140,7 @@ struct LUPlan {
   // LU values per scenario the factorisation kernels use
   size_t lu_elems() const { return rp_ok ? std::max<size_t>(A.nnz_lu, rp.xtot) : (size_t)A.nnz_lu; }
   bool ready = false;
-  double dt = -1.0;
+  double dt = -1.0;                               // J^T plan: dt of its pivot matrix
   static constexpr int kThreads = 512;
   static constexpr int kLdsBudget = 120 * 1024;   // staging / dense block
   static constexpr int kLdsMax = 160 * 1024;      // LDS of one CU
@@ -858,14 +858,23 @@ void set_mask(gp_ctx *C, DBuf<uint8_t> &d, const std::vector<uint8_t> &h) {
 }
 
 // Representative matrix of scenario 0 for the pivot analysis, evaluated by
-// the product's own kernels.
-void analyse(gp_ctx *C, int method, double dt) {
+// the product's own kernels at the current state with dt = 0 (the matrix of
+// the event re-solves, whose differential rows are the identity).  On that
+// matrix no entry that scales with dt (a differential row's -dt df/dz) can
+// be chosen as a pivot, so one set of static pivots serves every dt.  A
+// pivot chosen on a dt > 0 matrix can be such an entry -- in the column of a
+// bus voltage whose own diagonal is zero, as on a generator bus fed only by
+// lossless transformers (G = 0, and the machine's subtransient admittance is
+// imaginary) -- and vanishes at dt = 0, which ruins the re-solve's factor.
+// The plan does not depend on dt and is made once.
+void analyse(gp_ctx *C, int method) {
+  const double dt = 0.0;
   const int L = C->ldM;
   std::vector<uint8_t> sel(L, 0);
   sel[0] = 1;
   set_mask(C, C->d_sel, sel);
   if (method == GP_METHOD_MONO_FP64) {
-    if (C->luJ.ready && C->luJ.dt == dt) return;
+    if (C->luJ.ready) return;
     const int nnz = C->jcp[C->n];
     C->d_jv.alloc((size_t)nnz * L);
     HIPCHK(hipMemsetAsync(C->d_jv.p, 0, (size_t)nnz * L * sizeof(double), C->stream));
@@ -881,13 +890,12 @@ void analyse(gp_ctx *C, int method, double dt) {
     for (int i = 0; i < nnz; i++) asrc[i] = i;
     C->luJ.tune(asrc, jv, nullptr, sizeof(double), 1.0, C->stream, C->opt.dense_block);   // a solve per factorisation
     C->luJ.ready = true;
-    C->luJ.dt = dt;
     C->d_luJ.alloc(C->luJ.lu_elems() * C->M);
     C->d_svJ.alloc((size_t)C->luJ.dev.ns * C->M);
     HIPCHK(hipMemset(C->d_svJ.p, 0, C->d_svJ.n * sizeof(double)));   // padding slots stay zero
     C->d_dnJ.alloc((size_t)std::max(1, C->luJ.A.nd * C->luJ.A.nd) * C->M);
   } else {
-    if (C->luS.ready && C->luS.dt == dt) return;
+    if (C->luS.ready) return;
     const int nnzS = C->scp[C->nred];
     C->d_svar.alloc((size_t)std::max(1, C->n_svar) * L);
     for (auto &ln : C->lanes) ln.snap.alloc((size_t)std::max(1, C->n_svar) * L);
@@ -917,7 +925,6 @@ void analyse(gp_ctx *C, int method, double dt) {
     C->luS.tune(C->s_src, sv, C->d_s_cst.p, method == GP_METHOD_SCHUR_FP32 ? sizeof(float) : sizeof(double), 2.0,
                 C->stream, C->opt.dense_block);
     C->luS.ready = true;
-    C->luS.dt = dt;
   }
 }
 
@@ -1287,9 +1294,9 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
   return out;
 }
 
-// Symbolic analysis and buffers of `method` at step dt.
-void prepare_method(gp_ctx *C, int method, double dt) {
-  analyse(C, method, dt);
+// Symbolic analysis and buffers of `method`.
+void prepare_method(gp_ctx *C, int method) {
+  analyse(C, method);
   if (method != GP_METHOD_MONO_FP64) ensure_schur_buffers(C, method == GP_METHOD_SCHUR_FP32);
   else C->luJ.shape(sizeof(double));
 }
@@ -1779,7 +1786,7 @@ struct EventTimeline {
 // Jacobian value array in place (a_src), pivots chosen on scenario 0's matrix
 // at the current state.
 void analyse_transposed(gp_ctx *C, double dt) {
-  analyse(C, GP_METHOD_MONO_FP64, dt);
+  analyse(C, GP_METHOD_MONO_FP64);
   C->luJ.shape(sizeof(double));
   if (C->luJT.ready && C->luJT.dt == dt) return;
   const int n = C->n, nnz = C->jcp[n], L = C->ldM;
@@ -1992,7 +1999,7 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
     int maxsteps = 0;
     for (int m = 0; m < M; m++) maxsteps = std::max(maxsteps, C->nsteps[m]);
     Stepper sp(C, method, dt, tol, itermax);   // no fault active at t = 0
-    prepare_method(C, method, dt);
+    prepare_method(C, method);
     // trajectory storage [step][channel][ldM]
     C->d_hist.alloc((size_t)(maxsteps + 1) * C->nrec * L);
     C->hist_steps = maxsteps + 1;
@@ -2086,7 +2093,7 @@ int gp_adjoint(gp_ctx *C, int32_t method, double dt, double tol, int32_t itermax
       HIPCHK(hipMemcpyAsync(C->d_fg.p, fgz.data(), fgz.size() * sizeof(double), hipMemcpyHostToDevice, C->stream));
       HIPCHK(hipStreamSynchronize(C->stream));
     }
-    prepare_method(C, method, dt);
+    prepare_method(C, method);
     analyse_transposed(C, dt);
     C->d_hist.alloc((size_t)(maxsteps + 1) * C->nrec * L);
     C->hist_steps = maxsteps + 1;
@@ -2417,7 +2424,7 @@ int gp_newton_step(gp_ctx *C, int32_t method, const double *z, const double *zol
     // the symbolic analysis evaluates its representative matrix with
     // zold = z, so zold is loaded after it
     load_eval_state(C, z, nullptr);
-    analyse(C, method, dt > 0.0 ? dt : 1.0 / 120.0);
+    analyse(C, method);
     if (method != GP_METHOD_MONO_FP64) ensure_schur_buffers(C, method == GP_METHOD_SCHUR_FP32);
     std::vector<double> zt((size_t)C->n * L, 0.0);
     for (int m = 0; m < C->M; m++)
