@@ -12,5 +12,7 @@ from .system import (Bus, Gen, Load, Branch, Shunt, ContingencyEvent, PowerSyste
 from .network import build_network, runpf, runpf_inplace
 from .init import initialize_dynamics
 from .desc import build_desc, SystemDesc, gen_speed_indices
+from .psse import (from_psse, read_psse_raw, raw_to_grad, read_psse_dyr, create_device_vector,
+                   dynamics_from_dyr)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

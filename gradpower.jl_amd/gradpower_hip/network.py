@@ -154,20 +154,24 @@ def runpf(ps: PowerSystem, tol=1e-11, maxit=100, dc_start=False):
     return vm, va, P, Q
 
 
-def runpf_inplace(ps: PowerSystem, **kw):
+def runpf_inplace(ps: PowerSystem, exact_slack=False, **kw):
     """Write the solution back like src/pflow.jl:375-418: bus voltages, PV-bus
-    reactive power and slack P/Q split evenly over each bus's generators."""
+    reactive power and slack P/Q split evenly over each bus's generators.
+
+    exact_slack=False repeats the reference's arithmetic (src/pflow.jl:393-397
+    subtracts pd and qd from the net injection).  For Q this recovers the
+    generator output; for P it is exact only where the slack bus has no load
+    (the synthetic cases keep it load-free): a slack-bus load pd leaves the
+    slack machines 2 pd short and the initial state off balance on that bus.
+    exact_slack=True adds pd instead, so the machines carry the solved
+    output."""
     vm, va, P, Q = runpf(ps, **kw)
     for k, b in enumerate(ps.buses):
         b.v0m = float(vm[k])
         b.v0a = float(va[k])
     sP, sQ = P.copy(), Q.copy()
     for ld in ps.loads:
-        # Same arithmetic as src/pflow.jl:393-397 (subtracts pd and qd from the
-        # net injection).  For Q this recovers the generator output; for P it
-        # is only exact on buses without load, so synthetic cases keep the
-        # slack bus load-free.
-        sP[ld.bus] -= ld.pd
+        sP[ld.bus] += ld.pd if exact_slack else -ld.pd
         sQ[ld.bus] -= ld.qd
     bus_gens = [[] for _ in range(ps.nbus)]
     for gi, g in enumerate(ps.gens):

@@ -450,3 +450,32 @@ def case3_200bus(M=128, tf=3.0):
     events = [[ContingencyEvent(int(rng.integers(nbus)), float(rng.uniform(0.01, 0.2)), 0.02,
                                 float(rng.uniform(0.05, 0.25)))] for _ in range(M)]
     return _finish("c3_200bus", ps, psd, M, events, [tf] * M, meta, zip_alpha=0.4)
+
+
+def case_psse(raw, dyr, M=16, tf=1.0, exact_slack=True, zip_alpha=None, seed=1004):
+    """A system read from PSS/E files (raw + dyr) with M fault scenarios:
+    scenario m puts a shunt fault on bus m mod nbus (internal order) from
+    0.05 s to a clearing time drawn in [0.1, 0.2] s, fault resistance drawn
+    in [0.005, 0.05] pu.  exact_slack: see network.runpf_inplace (the
+    bundled IEEE 39-bus case has a load on its slack bus).  zip_alpha, when
+    given, is the constant-impedance fraction of every load."""
+    import os
+    from .psse import from_psse
+    rng = np.random.default_rng(seed)
+    ps = from_psse(raw, dyr)
+    if zip_alpha is not None:
+        for dev in ps.dynamic.devices:
+            if dev.dtype.kind == "zipload":
+                dev.dtype.alpha = zip_alpha
+    build_network(ps)
+    runpf_inplace(ps, exact_slack=exact_slack)
+    dp = DynamicProblem(ps)
+    initialize_dynamics(dp, ps)
+    desc = build_desc(ps, dp)
+    nb = ps.nbus
+    events = [[ContingencyEvent(m % nb, float(rng.uniform(0.005, 0.05)), 0.05, float(rng.uniform(0.1, 0.2)))]
+              for m in range(M)]
+    name = os.path.splitext(os.path.basename(dyr))[0]
+    meta = {"seed": seed, "raw": os.path.basename(raw), "dyr": os.path.basename(dyr),
+            "exact_slack": exact_slack, "zip_alpha": zip_alpha}
+    return Case(name, ps, dp, desc, M, DT, events, np.full(M, float(tf)), meta)
