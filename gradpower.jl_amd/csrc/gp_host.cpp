@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <map>
 #include <memory>
 #include <string>
@@ -140,6 +141,7 @@ struct LUPlan {
   // LU values per scenario the factorisation kernels use
   size_t lu_elems() const { return rp_ok ? std::max<size_t>(A.nnz_lu, rp.xtot) : (size_t)A.nnz_lu; }
   bool ready = false;
+  int trip_gen = -1;                              // trip set its pivots were chosen for
   double dt = -1.0;                               // J^T plan: dt of its pivot matrix
   static constexpr int kThreads = 512;
   static constexpr int kLdsBudget = 120 * 1024;   // staging / dense block
@@ -376,8 +378,16 @@ struct gp_ctx {
   int n_svar = 0;
   // per-bus attachments
   std::vector<int> bus_cl_p, bus_cl, bus_zip_p, bus_zip, bus_sg;
-  // events / horizons
-  std::vector<gp_event> events;
+  // events / horizons: faults of gp_set_events, then one record per line
+  // trip (bus = -2 - t for trip record t, see FaultState)
+  std::vector<gp_event> events, faults_user;
+  std::vector<gp_line_trip> trips;
+  // line trips: records (from, to; stamp) and the off-diagonal Y entries
+  // that trips change, kept once made variable (see DevTables)
+  std::vector<int> tr_bus, tv_p, tv_i;
+  std::vector<double> tr_y, tv_base;
+  std::map<std::pair<int, int>, int> tv_of;   // (row z, column z) -> entry
+  int trip_gen = 0;                             // bumped when the set of tripped lines changes
   std::vector<int> nsteps;
   std::vector<int> rec_idx;
   int nrec = 0;
@@ -387,6 +397,8 @@ struct gp_ctx {
   DBuf<int> d_jp[6], d_jp_bus, d_jp_yoff_p, d_jp_yoff_pos, d_cl_nt, d_cl_xoff, d_red_of_z, d_z_of_red, d_sv_bus;
   DBuf<int> d_rec_idx;
   DBuf<double> d_yv, d_u0, d_jp_yoff_val, d_ydiag, d_s_cst;
+  DBuf<int> d_tr_bus, d_tv_p, d_tv_i;
+  DBuf<double> d_tr_y, d_tv_base;
   DBuf<double> d_p, d_z, d_zold, d_f, d_inj, d_jv, d_svar, d_rhs, d_dv, d_Dg, d_fnorm, d_hist, d_luJ, d_luS64;
   DBuf<double> d_svJ, d_svS64, d_dnJ, d_dnS64;
   DBuf<float> d_luS32, d_svS32, d_dnS32, d_X32;
@@ -744,6 +756,17 @@ void build_structures(gp_ctx *C) {
 template <typename T>
 void up(DBuf<T> &d, const std::vector<T> &v) { d.upload(v); }
 
+void upload_trip_tables(gp_ctx *C) {
+  up(C->d_tr_bus, C->tr_bus.empty() ? std::vector<int>{0, 0} : C->tr_bus);
+  up(C->d_tr_y, C->tr_y.empty() ? std::vector<double>(8, 0.0) : C->tr_y);
+  up(C->d_tv_p, C->tv_p);
+  up(C->d_tv_i, C->tv_i.empty() ? std::vector<int>(4, 0) : C->tv_i);
+  up(C->d_tv_base, C->tv_base.empty() ? std::vector<double>{0.0} : C->tv_base);
+  DevTables &T = C->T;
+  T.tr_bus = C->d_tr_bus.p; T.tr_y = C->d_tr_y.p;
+  T.tv_p = C->d_tv_p.p; T.tv_i = C->d_tv_i.p; T.tv_base = C->d_tv_base.p;
+}
+
 void upload_tables(gp_ctx *C) {
   up(C->d_is_diff, C->is_diff);
   for (int t = 0; t < GP_NDEVTYPES; t++) {
@@ -783,6 +806,8 @@ void upload_tables(gp_ctx *C) {
   T.ydiag = C->d_ydiag.p;
   T.cl_nt = C->d_cl_nt.p; T.n_cl_nt = (int)C->cl_nt.size(); T.cl_xoff = C->d_cl_xoff.p;
   T.red_of_z = C->d_red_of_z.p; T.sv_bus = C->d_sv_bus.p;
+  if (C->tv_p.empty()) C->tv_p.assign(C->nbus + 1, 0);
+  upload_trip_tables(C);
 }
 
 void alloc_state(gp_ctx *C) {
@@ -857,6 +882,42 @@ void set_mask(gp_ctx *C, DBuf<uint8_t> &d, const std::vector<uint8_t> &h) {
   HIPCHK(hipMemcpyAsync(d.p, h.data(), h.size(), hipMemcpyHostToDevice, C->stream));
 }
 
+// The pivot analysis's second matrix (see splu_analyse): the representative
+// values `a` (J in CSC order, or S when reduced) with every distinct tripped
+// line taken out at once -- the diagonal blocks of its buses gain +yff / +ytt
+// and the off-diagonal blocks +yft / +ytf, realified [g -b; b g].  Empty when
+// no line is tripped.
+std::vector<double> trips_out(const gp_ctx *C, const std::vector<double> &a, bool reduced) {
+  std::vector<double> out;
+  if (C->trips.empty()) return out;
+  out = a;
+  const int npt = C->net_ptr;
+  auto pos = [&](int zr, int zc) {
+    return reduced ? find_pos(C->scp, C->sri, C->red_of_z[zr], C->red_of_z[zc]) : find_pos(C->jcp, C->jri, zr, zc);
+  };
+  auto add = [&](int rb, int cb, const double *y) {
+    const double v[4] = {y[0], -y[1], y[1], y[0]};
+    for (int rr = 0; rr < 2; rr++)
+      for (int cc = 0; cc < 2; cc++) {
+        const int p = pos(npt + 2 * rb + rr, npt + 2 * cb + cc);
+        if (p < 0) throw GpError{GP_ERR_HIP, "internal: tripped line's entry missing from a pattern"};
+        out[p] += v[2 * rr + cc];
+      }
+  };
+  std::vector<std::vector<double>> seen;
+  for (const gp_line_trip &t : C->trips) {
+    std::vector<double> key{(double)t.from_bus, (double)t.to_bus};
+    key.insert(key.end(), t.y, t.y + 8);
+    if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+    seen.push_back(key);
+    add(t.from_bus, t.from_bus, t.y);
+    add(t.from_bus, t.to_bus, t.y + 2);
+    add(t.to_bus, t.from_bus, t.y + 4);
+    add(t.to_bus, t.to_bus, t.y + 6);
+  }
+  return out;
+}
+
 // Representative matrix of scenario 0 for the pivot analysis, evaluated by
 // the product's own kernels at the current state with dt = 0 (the matrix of
 // the event re-solves, whose differential rows are the identity).  On that
@@ -874,7 +935,7 @@ void analyse(gp_ctx *C, int method) {
   sel[0] = 1;
   set_mask(C, C->d_sel, sel);
   if (method == GP_METHOD_MONO_FP64) {
-    if (C->luJ.ready) return;
+    if (C->luJ.ready && C->luJ.trip_gen == C->trip_gen) return;
     const int nnz = C->jcp[C->n];
     C->d_jv.alloc((size_t)nnz * L);
     HIPCHK(hipMemsetAsync(C->d_jv.p, 0, (size_t)nnz * L * sizeof(double), C->stream));
@@ -884,18 +945,21 @@ void analyse(gp_ctx *C, int method) {
     HIPCHK(hipMemcpy2DAsync(jv.data(), sizeof(double), C->d_jv.p, (size_t)L * sizeof(double), sizeof(double), nnz,
                             hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
-    std::string e = splu_analyse(C->n, C->jcp.data(), C->jri.data(), jv.data(), C->luJ.A);
+    std::vector<double> jv2 = trips_out(C, jv, false);
+    std::string e = splu_analyse(C->n, C->jcp.data(), C->jri.data(), jv.data(), C->luJ.A, 1e-3,
+                                 jv2.empty() ? nullptr : jv2.data());
     if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "Jacobian analysis: " + e};
     std::vector<int> asrc(nnz);
     for (int i = 0; i < nnz; i++) asrc[i] = i;
     C->luJ.tune(asrc, jv, nullptr, sizeof(double), 1.0, C->stream, C->opt.dense_block);   // a solve per factorisation
     C->luJ.ready = true;
+    C->luJ.trip_gen = C->trip_gen;
     C->d_luJ.alloc(C->luJ.lu_elems() * C->M);
     C->d_svJ.alloc((size_t)C->luJ.dev.ns * C->M);
     HIPCHK(hipMemset(C->d_svJ.p, 0, C->d_svJ.n * sizeof(double)));   // padding slots stay zero
     C->d_dnJ.alloc((size_t)std::max(1, C->luJ.A.nd * C->luJ.A.nd) * C->M);
   } else {
-    if (C->luS.ready) return;
+    if (C->luS.ready && C->luS.trip_gen == C->trip_gen) return;
     const int nnzS = C->scp[C->nred];
     C->d_svar.alloc((size_t)std::max(1, C->n_svar) * L);
     for (auto &ln : C->lanes) ln.snap.alloc((size_t)std::max(1, C->n_svar) * L);
@@ -918,13 +982,16 @@ void analyse(gp_ctx *C, int method) {
       int s = C->s_src[e];
       sx[e] = s >= 0 ? sv[s] : C->s_cst[-s - 1];
     }
-    std::string e = splu_analyse(C->nred, C->scp.data(), C->sri.data(), sx.data(), C->luS.A);
+    std::vector<double> sx2 = trips_out(C, sx, true);
+    std::string e = splu_analyse(C->nred, C->scp.data(), C->sri.data(), sx.data(), C->luS.A, 1e-3,
+                                 sx2.empty() ? nullptr : sx2.data());
     if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "reduced-network analysis: " + e};
     // a factorisation serves several solves (refactorisation policies), and
     // the solves are on every round's critical path
     C->luS.tune(C->s_src, sv, C->d_s_cst.p, method == GP_METHOD_SCHUR_FP32 ? sizeof(float) : sizeof(double), 2.0,
                 C->stream, C->opt.dense_block);
     C->luS.ready = true;
+    C->luS.trip_gen = C->trip_gen;
   }
 }
 
@@ -1467,7 +1534,7 @@ struct Stepper {
       for (int m = 0; m < M; m++) {
         if (!res[m]) continue;
         if ((int)st.active_ev[m].size() > MAXFAULT)
-          throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults in one scenario"};
+          throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults or line trips in one scenario"};
         for (int q = 0; q < MAXFAULT; q++) {
           const bool on = q < (int)st.active_ev[m].size();
           st.fbus[(size_t)q * L + m] = on ? st.active_ev[m][q]->bus : -1;
@@ -1557,7 +1624,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
   std::vector<double> fg((size_t)MAXFAULT * L, 0.0);
   auto set_faults = [&](int m) {
     if ((int)active_ev[m].size() > MAXFAULT)
-      throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults in one scenario"};
+      throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults or line trips in one scenario"};
     for (int q = 0; q < MAXFAULT; q++) {
       const bool f_on = q < (int)active_ev[m].size();
       fbus[(size_t)q * L + m] = f_on ? active_ev[m][q]->bus : -1;
@@ -1788,7 +1855,7 @@ struct EventTimeline {
 void analyse_transposed(gp_ctx *C, double dt) {
   analyse(C, GP_METHOD_MONO_FP64);
   C->luJ.shape(sizeof(double));
-  if (C->luJT.ready && C->luJT.dt == dt) return;
+  if (C->luJT.ready && C->luJT.dt == dt && C->luJT.trip_gen == C->trip_gen) return;
   const int n = C->n, nnz = C->jcp[n], L = C->ldM;
   std::vector<uint8_t> sel(L, 0);
   sel[0] = 1;
@@ -1811,11 +1878,17 @@ void analyse_transposed(gp_ctx *C, double dt) {
     }
   std::vector<double> tv(nnz);
   for (int t = 0; t < nnz; t++) tv[t] = jv[tsrc[t]];
-  std::string e = splu_analyse(n, tcp.data(), tri.data(), tv.data(), C->luJT.A);
+  std::vector<double> jv2 = trips_out(C, jv, false), tv2;
+  if (!jv2.empty()) {
+    tv2.resize(nnz);
+    for (int t = 0; t < nnz; t++) tv2[t] = jv2[tsrc[t]];
+  }
+  std::string e = splu_analyse(n, tcp.data(), tri.data(), tv.data(), C->luJT.A, 1e-3, tv2.empty() ? nullptr : tv2.data());
   if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "transposed Jacobian analysis: " + e};
   C->luJT.tune(tsrc, jv, nullptr, sizeof(double), 1.0, C->stream, C->opt.dense_block);
   C->luJT.ready = true;
   C->luJT.dt = dt;
+  C->luJT.trip_gen = C->trip_gen;
   C->d_luJT.alloc(C->luJT.lu_elems() * C->M);
   C->d_svJT.alloc((size_t)C->luJT.dev.ns * C->M);
   HIPCHK(hipMemset(C->d_svJT.p, 0, C->d_svJT.n * sizeof(double)));   // padding slots stay zero
@@ -1983,8 +2056,115 @@ int gp_set_events(gp_ctx *C, const gp_event *ev, int32_t n_ev, const int32_t *ns
   }
   for (int m = 0; m < C->M; m++)
     if (nsteps[m] < 0) return fail(GP_ERR_BADARG, "gp_set_events: negative nsteps");
-  C->events.assign(ev, ev + n_ev);
+  C->faults_user.assign(ev, ev + n_ev);
+  C->events = C->faults_user;
+  if (!C->trips.empty()) C->trip_gen++;   // the LU plans' pivots were chosen with the trips
+  C->trips.clear();
+  C->tr_bus.clear();
+  C->tr_y.clear();
   C->nsteps.assign(nsteps, nsteps + C->M);
+  try {
+    upload_trip_tables(C);
+  } catch (const GpError &e) {
+    return fail(e.code, e.msg);
+  }
+  return GP_OK;
+}
+
+int gp_set_line_trips(gp_ctx *C, const gp_line_trip *tr, int32_t n) {
+  if (!C || (n > 0 && !tr) || n < 0) return fail(GP_ERR_BADARG, "gp_set_line_trips: bad argument");
+  const int npt = C->net_ptr;
+  // Y entry (row, column) of the realified Y, or none
+  auto yval = [&](int r, int c, double &v) {
+    for (int q = C->yrp[r]; q < C->yrp[r + 1]; q++)
+      if (C->yci[q] == c) { v = C->yv[q]; return true; }
+    return false;
+  };
+  for (int i = 0; i < n; i++) {
+    const gp_line_trip &t = tr[i];
+    if (t.scenario < 0 || t.scenario >= C->M || t.from_bus < 0 || t.from_bus >= C->nbus || t.to_bus < 0 ||
+        t.to_bus >= C->nbus || t.from_bus == t.to_bus)
+      return fail(GP_ERR_BADARG, "gp_set_line_trips: scenario or bus out of range");
+    if (t.step_trip < 1 || t.step_reclose < 0)
+      return fail(GP_ERR_BADARG, "gp_set_line_trips: step_trip must be >= 1 and step_reclose >= 0");
+    for (int k = 0; k < 8; k++)
+      if (!std::isfinite(t.y[k])) return fail(GP_ERR_BADARG, "gp_set_line_trips: non-finite admittance");
+    for (int d = 0; d < 2; d++)
+      for (int rr = 0; rr < 2; rr++)
+        for (int cc = 0; cc < 2; cc++) {
+          const int rb = d ? t.to_bus : t.from_bus, cb = d ? t.from_bus : t.to_bus;
+          double v;
+          if (!yval(2 * rb + rr, 2 * cb + cc, v) ||
+              find_pos(C->jcp, C->jri, npt + 2 * rb + rr, npt + 2 * cb + cc) < 0)
+            return fail(GP_ERR_BADARG, "gp_set_line_trips: no Y entry between buses " + std::to_string(t.from_bus) +
+                                           " and " + std::to_string(t.to_bus));
+        }
+  }
+  try {
+    // the off-diagonal entries these trips change become per-scenario values:
+    // J positions, and S slots taken out of the constants (the reduced
+    // system's LU plan is then made again)
+    bool s_changed = false;
+    std::vector<std::vector<int>> add_i(C->nbus);
+    std::vector<std::vector<double>> add_b(C->nbus);
+    for (int i = 0; i < n; i++)
+      for (int d = 0; d < 2; d++) {
+        const int rb = d ? tr[i].to_bus : tr[i].from_bus, cb = d ? tr[i].from_bus : tr[i].to_bus;
+        for (int rr = 0; rr < 2; rr++)
+          for (int cc = 0; cc < 2; cc++) {
+            const int zr = npt + 2 * rb + rr, zc = npt + 2 * cb + cc;
+            if (C->tv_of.count({zr, zc})) continue;
+            double v = 0.0;
+            yval(2 * rb + rr, 2 * cb + cc, v);
+            const int jp = find_pos(C->jcp, C->jri, zr, zc);
+            const int ps = find_pos(C->scp, C->sri, C->red_of_z[zr], C->red_of_z[zc]);
+            if (ps < 0) throw GpError{GP_ERR_HIP, "internal: Y entry missing from the reduced pattern"};
+            if (C->s_src[ps] < 0) {
+              C->s_src[ps] = C->n_svar++;
+              s_changed = true;
+            }
+            C->tv_of[{zr, zc}] = 1;
+            add_i[rb].insert(add_i[rb].end(), {cb, 2 * rr + cc, jp, C->s_src[ps]});
+            add_b[rb].push_back(-v);
+          }
+      }
+    // per-bus lists: the entries kept so far, then the new ones
+    std::vector<int> tv_p(C->nbus + 1, 0), tv_i;
+    std::vector<double> tv_base;
+    for (int b = 0; b < C->nbus; b++) {
+      for (int q = C->tv_p[b]; q < C->tv_p[b + 1]; q++) {
+        tv_i.insert(tv_i.end(), C->tv_i.begin() + 4 * q, C->tv_i.begin() + 4 * q + 4);
+        tv_base.push_back(C->tv_base[q]);
+      }
+      tv_i.insert(tv_i.end(), add_i[b].begin(), add_i[b].end());
+      tv_base.insert(tv_base.end(), add_b[b].begin(), add_b[b].end());
+      tv_p[b + 1] = (int)tv_base.size();
+    }
+    C->tv_p = tv_p;
+    C->tv_i = tv_i;
+    C->tv_base = tv_base;
+    if (s_changed) C->luS.ready = false;
+    if (n > 0 || !C->trips.empty()) C->trip_gen++;   // pivots are chosen with the tripped lines
+    C->trips.assign(tr, tr + n);
+    C->tr_bus.clear();
+    C->tr_y.clear();
+    C->events = C->faults_user;
+    for (int i = 0; i < n; i++) {
+      C->tr_bus.push_back(tr[i].from_bus);
+      C->tr_bus.push_back(tr[i].to_bus);
+      C->tr_y.insert(C->tr_y.end(), tr[i].y, tr[i].y + 8);
+      gp_event e{};
+      e.scenario = tr[i].scenario;
+      e.bus = -2 - i;
+      e.r_fault = 1.0;
+      e.step_on = tr[i].step_trip;
+      e.step_off = tr[i].step_reclose > 0 ? tr[i].step_reclose : std::numeric_limits<int32_t>::max();
+      C->events.push_back(e);
+    }
+    upload_trip_tables(C);
+  } catch (const GpError &e) {
+    return fail(e.code, e.msg);
+  }
   return GP_OK;
 }
 

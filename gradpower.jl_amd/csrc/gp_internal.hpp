@@ -48,9 +48,19 @@ struct DevTables {
   const int *cl_xoff;     // per cluster: offset (in 3-column rows) into X
   const int *red_of_z;    // z index -> reduced index (-1 if eliminated)
   const int *sv_bus;      // per bus: 4 var slots of S diag + 6 static-gen var slots
+  // line trips: per trip record t, its buses (from, to) and its admittance
+  // stamp (yff, yft, ytf, ytt as re, im); per bus, the off-diagonal Y entries
+  // of its rows that some trip changes: 4 ints each (column bus, component
+  // 2*rr + cc of the 2x2 block, J position, S slot) and the entry of -Y
+  const int *tr_bus;
+  const double *tr_y;
+  const int *tv_p, *tv_i;
+  const double *tv_base;
 };
 
-// Per-scenario fault state: fault_bus[k*ldM+m], fault_g[k*ldM+m].
+// Per-scenario event slots: bus[k*ldM+m] >= 0 is a fault on that bus with
+// conductance g[k*ldM+m]; bus = -2 - t is the trip of line record t; -1 is
+// an empty slot.
 struct FaultState {
   int *bus;
   double *g;

@@ -309,6 +309,52 @@ __global__ void __launch_bounds__(64) k_cluster_residual(DevTables T, const doub
   (void)ws;
 }
 
+// Line trips active in scenario m (event slots holding -2 - t): a tripped
+// line's stamp leaves Y.  The network rows hold -Y v, so the rows of the
+// line's buses gain +y v: at the from bus yff v_from + yft v_to, at the to
+// bus ytf v_from + ytt v_to (src/GradPower.jl:556-568 takes the stamp out of
+// ybus_real instead).
+__device__ __forceinline__ void trip_residual(const DevTables &T, const FaultState &fs, const double *__restrict__ z,
+                                              int b, int m, double vr, double vi, double &fr, double &fi) {
+  const int ldM = T.ldM;
+#pragma unroll
+  for (int k = 0; k < MAXFAULT; k++) {
+    const int code = fs.bus[k * ldM + m];
+    if (code > -2) continue;
+    const int t = -2 - code;
+    const int a = T.tr_bus[2 * t], c = T.tr_bus[2 * t + 1];
+    if (a != b && c != b) continue;
+    const double *y = T.tr_y + 8 * t;
+    const double *ys = y + (a == b ? 0 : 6);   // own bus: yff or ytt
+    const double *yo = y + (a == b ? 2 : 4);   // other bus: yft or ytf
+    const int oz = T.net_ptr + 2 * (a == b ? c : a);
+    const double ur = z[(size_t)oz * ldM + m], ui = z[(size_t)(oz + 1) * ldM + m];
+    fr += ys[0] * vr - ys[1] * vi + yo[0] * ur - yo[1] * ui;
+    fi += ys[1] * vr + ys[0] * vi + yo[1] * ur + yo[0] * ui;
+  }
+}
+
+// Entry (row bus b, column bus col, component comp = 2 rr + cc of the 2x2
+// block) of -Y with the lines tripped in scenario m taken out: -Y + y_line,
+// y_line realified as [g -b; b g].
+__device__ __forceinline__ double trip_offdiag(const DevTables &T, const FaultState &fs, int b, int m, int col,
+                                               int comp, double base) {
+  const int ldM = T.ldM;
+  double v = base;
+#pragma unroll
+  for (int k = 0; k < MAXFAULT; k++) {
+    const int code = fs.bus[k * ldM + m];
+    if (code > -2) continue;
+    const int t = -2 - code;
+    const int a = T.tr_bus[2 * t], c = T.tr_bus[2 * t + 1];
+    const double *y = T.tr_y + 8 * t;
+    const double *yy = (a == b && c == col) ? y + 2 : ((c == b && a == col) ? y + 4 : nullptr);
+    if (!yy) continue;
+    v += (comp == 0 || comp == 3) ? yy[0] : (comp == 1 ? -yy[1] : yy[1]);
+  }
+  return v;
+}
+
 // One wave = one bus x 64 scenarios: network rows (-Y v + injections, faults)
 // and the static generator's own rows.  Summation order per bus follows the
 // reference: -Y v, machines, loads, static generator, faults.
@@ -368,6 +414,7 @@ __global__ void __launch_bounds__(64) k_bus_residual(DevTables T, const double *
       fi -= gf * vi;
     }
   }
+  trip_residual(T, fs, z, b, m, vr, vi, fr, fi);
   f[(size_t)vrz * ldM + m] = fr;
   f[(size_t)(vrz + 1) * ldM + m] = fi;
 }
@@ -674,12 +721,24 @@ __device__ __forceinline__ void bus_diag(const DevTables &T, const double *p, co
     dg[0] += sgo[0]; dg[1] += sgo[1]; dg[2] += sgo[2]; dg[3] += sgo[3];
   }
 #pragma unroll
-  for (int k = 0; k < MAXFAULT; k++)
-    if (fs.bus[k * ldM + m] == b) {
+  for (int k = 0; k < MAXFAULT; k++) {
+    const int code = fs.bus[k * ldM + m];
+    if (code == b) {
       const double gf = fs.g[k * ldM + m];
       dg[0] -= gf;
       dg[3] -= gf;
+    } else if (code <= -2) {   // a tripped line at this bus: -Y_bb + yff (or ytt)
+      const int t = -2 - code;
+      const int a = T.tr_bus[2 * t], c = T.tr_bus[2 * t + 1];
+      if (a == b || c == b) {
+        const double *yy = T.tr_y + 8 * t + (a == b ? 0 : 6);
+        dg[0] += yy[0];
+        dg[1] -= yy[1];
+        dg[2] += yy[1];
+        dg[3] += yy[0];
+      }
     }
+  }
 }
 
 __global__ void __launch_bounds__(64) k_bus_jac_mono(DevTables T, const double *__restrict__ p,
@@ -700,6 +759,11 @@ __global__ void __launch_bounds__(64) k_bus_jac_mono(DevTables T, const double *
   for (int i = 4; i < 11; i++)
     if (jb[i] >= 0) jv[(size_t)jb[i] * ldM + m] = sgo[i];
   for (int q = T.jp_yoff_p[b]; q < T.jp_yoff_p[b + 1]; q++) jv[(size_t)T.jp_yoff_pos[q] * ldM + m] = -T.jp_yoff_val[q];
+  // off-diagonal entries that line trips change (written again, after the constants)
+  for (int q = T.tv_p[b]; q < T.tv_p[b + 1]; q++) {
+    const int *e = T.tv_i + 4 * q;
+    jv[(size_t)e[2] * ldM + m] = trip_offdiag(T, fs, b, m, e[0], e[1], T.tv_base[q]);
+  }
 }
 
 // ---------------------------------------------------------------- small dense solves
@@ -1158,6 +1222,11 @@ __global__ void __launch_bounds__(64) k_bus_schur(DevTables T, const double *__r
 #pragma unroll
   for (int i = 4; i < 11; i++)
     if (sv[i] >= 0) svar[(size_t)sv[i] * ldM + m] = sgo[i];
+  // off-diagonal entries that line trips change (variable slots instead of constants)
+  for (int q = T.tv_p[b]; q < T.tv_p[b + 1]; q++) {
+    const int *e = T.tv_i + 4 * q;
+    svar[(size_t)e[3] * ldM + m] = trip_offdiag(T, fs, b, m, e[0], e[1], T.tv_base[q]);
+  }
   rhs[(size_t)T.red_of_z[vrz] * ldM + m] = -rr;
   rhs[(size_t)T.red_of_z[vrz + 1] * ldM + m] = -ri;
   const int sgk = T.bus_sg[b];

@@ -57,7 +57,7 @@ std::vector<int> min_degree_order(int n, const int *cp, const int *ri) {
 }
 
 std::string splu_analyse(int n, const int *cp, const int *ri, const double *ax, SpluAnalysis &A,
-                         double thr) {
+                         double thr, const double *ax2) {
   A = SpluAnalysis();
   A.n = n;
   A.nnz_a = cp[n];
@@ -69,7 +69,9 @@ std::string splu_analyse(int n, const int *cp, const int *ri, const double *ax, 
   // ---- numeric left-looking LU with threshold partial pivoting ----------
   std::vector<int> pinv(n, -1), prow(n, -1);
   std::vector<std::vector<std::pair<int, double>>> Lcol(n), Ucol(n);  // L: original rows; U: pivot rows
-  std::vector<double> x(n, 0.0);
+  std::vector<double> x(n, 0.0), x2(ax2 ? n : 0, 0.0);
+  std::vector<std::vector<double>> L2(ax2 ? n : 0);   // second matrix's L values, parallel to Lcol
+  bool dual = ax2 != nullptr;
   std::vector<int> xi(n), mark(n, -1), stk(n), pst(n);
   for (int k = 0; k < n; k++) {
     int col = A.q[k];
@@ -112,26 +114,64 @@ std::string splu_analyse(int n, const int *cp, const int *ri, const double *ax, 
       double xj = x[j];
       for (auto &e : Lcol[jp]) x[e.first] -= e.second * xj;
     }
+    if (dual) {   // the same column of the second matrix, same pivots so far
+      for (int p = top; p < n; p++) x2[xi[p]] = 0.0;
+      for (int p = cp[col]; p < cp[col + 1]; p++) x2[ri[p]] = ax2[p];
+      for (int p = top; p < n; p++) {
+        int j = xi[p], jp = pinv[j];
+        if (jp < 0) continue;
+        double xj = x2[j];
+        const auto &lc = Lcol[jp];
+        const auto &lv = L2[jp];
+        for (size_t e = 0; e < lc.size(); e++) x2[lc[e].first] -= lv[e] * xj;
+      }
+    }
     int ipiv = -1;
-    double amax = -1.0;
+    double amax = -1.0, amax2 = 0.0;
     for (int p = top; p < n; p++) {
       int i = xi[p];
       if (pinv[i] < 0) {
         double t = std::fabs(x[i]);
         if (t > amax) { amax = t; ipiv = i; }
+        if (dual) amax2 = std::max(amax2, std::fabs(x2[i]));
       } else {
         Ucol[k].push_back({pinv[i], x[i]});
       }
     }
     if (ipiv < 0 || !(amax > 0.0)) return "structurally or numerically singular matrix at pivot " + std::to_string(k);
-    if (pinv[col] < 0 && std::fabs(x[col]) >= thr * amax) ipiv = col;
+    bool both = dual && amax2 > 0.0;
+    if (both) {
+      // a pivot that is large in both matrices: the largest min(|x|/amax, |x2|/amax2)
+      double best = 0.0;
+      int ib = -1;
+      for (int p = top; p < n; p++) {
+        int i = xi[p];
+        if (pinv[i] >= 0) continue;
+        double sc = std::min(std::fabs(x[i]) / amax, std::fabs(x2[i]) / amax2);
+        if (sc > best) { best = sc; ib = i; }
+      }
+      if (ib < 0) both = false;   // no row is nonzero in both: the first matrix decides from here on
+      else {
+        ipiv = ib;
+        if (pinv[col] < 0 && std::min(std::fabs(x[col]) / amax, std::fabs(x2[col]) / amax2) >= thr) ipiv = col;
+      }
+    }
+    if (!both) {
+      dual = false;
+      if (pinv[col] < 0 && std::fabs(x[col]) >= thr * amax) ipiv = col;
+    }
     double piv = x[ipiv];
+    double piv2 = dual ? x2[ipiv] : 1.0;
     pinv[ipiv] = k;
     prow[k] = ipiv;
     for (int p = top; p < n; p++) {
       int i = xi[p];
-      if (pinv[i] < 0) Lcol[k].push_back({i, x[i] / piv});
+      if (pinv[i] < 0) {
+        Lcol[k].push_back({i, x[i] / piv});
+        if (dual) L2[k].push_back(x2[i] / piv2);
+      }
       x[i] = 0.0;
+      if (dual) x2[i] = 0.0;
     }
   }
   A.pinv = pinv;

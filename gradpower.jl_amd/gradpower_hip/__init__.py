@@ -6,7 +6,8 @@ integrator itself is the C-ABI library libgradpower_hip.so (HIP kernels for
 gfx950), loaded through `gradpower_hip.lib`.
 """
 from .system import (Bus, Gen, Load, Branch, Shunt, ContingencyEvent, PowerSystem,
-                     PowerSystemDynamics, DynamicProblem, Genrou, Gensal, IEESGO, TGOV1,
+                     PowerSystemDynamics, DynamicProblem, TripLineEvent, create_trip_line_event,
+                     Genrou, Gensal, IEESGO, TGOV1,
                      SEXS, ESDC1A, IEEEST, ZIPLoad, StaticGenerator, add_device,
                      add_event, set_dynamics, esdc1a_sat_coefficients, PQ, PV, SLACK)
 from .network import build_network, runpf, runpf_inplace

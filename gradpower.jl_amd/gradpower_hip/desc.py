@@ -38,6 +38,12 @@ class CEvent(C.Structure):
                 ("step_on", C.c_int32), ("step_off", C.c_int32)]
 
 
+class CLineTrip(C.Structure):
+    _fields_ = [("scenario", C.c_int32), ("from_bus", C.c_int32), ("to_bus", C.c_int32),
+                ("step_trip", C.c_int32), ("step_reclose", C.c_int32), ("reserved", C.c_int32),
+                ("y", C.c_double * 8)]
+
+
 def _p(a, ct):
     return a.ctypes.data_as(C.POINTER(ct))
 

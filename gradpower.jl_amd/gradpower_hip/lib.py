@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-from .desc import SystemDesc, CEvent
+from .desc import SystemDesc, CEvent, CLineTrip
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libgradpower_hip.so")
@@ -77,7 +77,7 @@ _ip = C.POINTER(C.c_int32)
 _up = C.POINTER(C.c_uint8)
 _LIB = None
 
-EXPORTS = ["gp_create", "gp_destroy", "gp_set_state", "gp_set_events", "gp_integrate",
+EXPORTS = ["gp_create", "gp_destroy", "gp_set_state", "gp_set_events", "gp_set_line_trips", "gp_integrate",
            "gp_get_trajectory", "gp_get_state", "gp_eval_residual", "gp_jacobian_pattern",
            "gp_eval_jacobian", "gp_newton_step", "gp_get_symbolic_info", "gp_get_scenario_status",
            "gp_copy_kernel", "gp_adjoint", "gp_last_error", "gp_version"]
@@ -97,6 +97,7 @@ def load():
     lib.gp_destroy.restype = None
     lib.gp_set_state.argtypes = [C.c_void_p, _dp, C.c_int32, _dp]
     lib.gp_set_events.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, _ip]
+    lib.gp_set_line_trips.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
     lib.gp_integrate.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_void_p]
     lib.gp_get_trajectory.argtypes = [C.c_void_p, C.c_int32, _dp, C.c_int64]
     lib.gp_get_state.argtypes = [C.c_void_p, _dp]
@@ -190,6 +191,26 @@ class HipContext:
             arr[i].step_on, arr[i].step_off = int(on), int(off)
         self.nsteps = np.ascontiguousarray(nsteps, np.int32)
         self._check(self.lib.gp_set_events(self.h, C.byref(arr), len(ev), _i(self.nsteps)))
+
+    def set_line_trips(self, trips):
+        """trips: iterable of (scenario, from_bus, to_bus, step_trip,
+        step_reclose (0: never), y) with y the line's stamp (yff, yft, ytf,
+        ytt as 4 complex or 8 floats, see system.TripLineEvent.stamp).
+        Call after set_events, which clears them."""
+        tr = list(trips)
+        arr = (CLineTrip * max(1, len(tr)))()
+        for i, (m, a, b, on, off, y) in enumerate(tr):
+            yv = np.asarray(y)
+            if np.iscomplexobj(yv):
+                yv = np.column_stack([yv.real, yv.imag]).reshape(-1)
+            yv = np.asarray(yv, np.float64).reshape(-1)
+            if yv.size != 8:
+                raise ValueError("a line trip's stamp has 4 complex (8 real) entries")
+            arr[i].scenario, arr[i].from_bus, arr[i].to_bus = int(m), int(a), int(b)
+            arr[i].step_trip, arr[i].step_reclose = int(on), int(off)
+            for k in range(8):
+                arr[i].y[k] = float(yv[k])
+        self._check(self.lib.gp_set_line_trips(self.h, C.byref(arr), len(tr)))
 
     def integrate(self, method, dt, tol=1e-10, itermax=50, raise_on_fail=True):
         st = CStats()

@@ -530,9 +530,11 @@ def dynamics_from_dyr(filename, active_gen_keys=None) -> PowerSystemDynamics:
 
 
 def from_psse(raw_file, dyr_file=None, add_static_gen_stubs=True) -> PowerSystem:
-    """Network from `raw_file`; with `dyr_file`, its device models for the
-    generators in service, wired by set_dynamics (src/parse.jl:81-98)."""
-    ps = raw_to_grad(read_psse_raw(raw_file))
+    """Network from `raw_file` (a path, or a PsystemRaw already read); with
+    `dyr_file`, its device models for the generators in service, wired by
+    set_dynamics (src/parse.jl:81-98)."""
+    raw = raw_file if isinstance(raw_file, PsystemRaw) else read_psse_raw(raw_file)
+    ps = raw_to_grad(raw)
     if dyr_file is not None:
         active = {(ps.buses[g.bus].i, _norm_id(g.id)) for g in ps.gens}
         psd = dynamics_from_dyr(dyr_file, active)

@@ -452,17 +452,24 @@ def case3_200bus(M=128, tf=3.0):
     return _finish("c3_200bus", ps, psd, M, events, [tf] * M, meta, zip_alpha=0.4)
 
 
-def case_psse(raw, dyr, M=16, tf=1.0, exact_slack=True, zip_alpha=None, seed=1004):
+def case_psse(raw, dyr, M=16, tf=1.0, exact_slack=True, zip_alpha=None, seed=1004, xf_tap=None):
     """A system read from PSS/E files (raw + dyr) with M fault scenarios:
     scenario m puts a shunt fault on bus m mod nbus (internal order) from
     0.05 s to a clearing time drawn in [0.1, 0.2] s, fault resistance drawn
     in [0.005, 0.05] pu.  exact_slack: see network.runpf_inplace (the
     bundled IEEE 39-bus case has a load on its slack bus).  zip_alpha, when
-    given, is the constant-impedance fraction of every load."""
+    given, is the constant-impedance fraction of every load.  xf_tap =
+    (k, WINDV1, ANG1) sets the winding-1 ratio and phase shift (degrees) of
+    the k-th two-winding transformer before the conversion."""
     import os
-    from .psse import from_psse
+    from .psse import from_psse, read_psse_raw
     rng = np.random.default_rng(seed)
-    ps = from_psse(raw, dyr)
+    rd = read_psse_raw(raw)
+    if xf_tap is not None:
+        k, w1, a1 = xf_tap
+        rd.transformers[k].WINDV1 = float(w1)
+        rd.transformers[k].ANG1 = float(a1)
+    ps = from_psse(rd, dyr)
     if zip_alpha is not None:
         for dev in ps.dynamic.devices:
             if dev.dtype.kind == "zipload":
@@ -477,5 +484,5 @@ def case_psse(raw, dyr, M=16, tf=1.0, exact_slack=True, zip_alpha=None, seed=100
               for m in range(M)]
     name = os.path.splitext(os.path.basename(dyr))[0]
     meta = {"seed": seed, "raw": os.path.basename(raw), "dyr": os.path.basename(dyr),
-            "exact_slack": exact_slack, "zip_alpha": zip_alpha}
+            "exact_slack": exact_slack, "zip_alpha": zip_alpha, "xf_tap": xf_tap}
     return Case(name, ps, dp, desc, M, DT, events, np.full(M, float(tf)), meta)

@@ -17,6 +17,7 @@ the reference's cluster-contiguous order z = [w_1 | ... | w_Nc | v]
 """
 from __future__ import annotations
 
+import cmath
 import math
 from dataclasses import dataclass, field
 from typing import List, Optional
@@ -89,6 +90,50 @@ class ContingencyEvent:
     ton: float
     toff: float
     status: bool = False
+
+
+@dataclass
+class TripLineEvent:
+    """Trip of one line at ton, reclosed at toff (inf: never), with its
+    admittance stamp (src/GradPower.jl:140-167).  Buses are 0-based."""
+    from_bus: int
+    to_bus: int
+    ton: float
+    toff: float
+    yff: complex
+    yft: complex
+    ytf: complex
+    ytt: complex
+
+    def stamp(self):
+        """The 8 reals of the C ABI: yff, yft, ytf, ytt as (re, im)."""
+        return [v for y in (self.yff, self.yft, self.ytf, self.ytt) for v in (y.real, y.imag)]
+
+    def steps(self, dt):
+        """(step_trip, step_reclose) at round(t/dt); 0 = never reclosed."""
+        off = 0 if math.isinf(self.toff) else int(round(self.toff / dt))
+        return int(round(self.ton / dt)), off
+
+
+def create_trip_line_event(ps, from_bus_ext, to_bus_ext, ton, toff=math.inf):
+    """The first branch between the two buses (either orientation) and the
+    admittance it adds to Y, by the same formulas as the Y builder
+    (src/GradPower.jl:509-542)."""
+    a, b = ps.busmap[from_bus_ext], ps.busmap[to_bus_ext]
+    br = next((x for x in ps.branches if (x.fr, x.to) in ((a, b), (b, a))), None)
+    if br is None:
+        raise ValueError(f"No branch found between buses {from_bus_ext} and {to_bus_ext}")
+    tap, shift = br.tap, br.shift
+    if tap > 0.0:
+        tpsh = tap * cmath.exp(1j * math.pi / 180.0 * shift)
+    else:
+        tpsh, tap = 1.0 + 0j, 1.0
+    y = 1.0 / complex(br.r, br.x)
+    return TripLineEvent(br.fr, br.to, ton, toff,
+                         y / (tap * tap) + 1j * 0.5 * br.sh,
+                         -y / tpsh.conjugate(),
+                         -y / tpsh,
+                         y + (1j * 0.5 * br.sh) / (tap * tap))
 
 
 # ----------------------------------------------------------------------------

@@ -8,6 +8,8 @@
  *   integrate_gpu_schur_cudss!(gbl, ps, tf; ..) ext:3008-3080    -> gp_integrate(GP_METHOD_SCHUR_*)
  *   integrate_gpu_shared_multi!(..., fault_buses, rfaults, ton, toff)
  *                                               ext:2640-2708    -> gp_set_events (per-scenario faults)
+ *   TripLineEvent / create_trip_line_event     src/GradPower.jl:140-167,509-570
+ *   (CPU integrate! only, src/dynamics.jl:536-540,616-624)        -> gp_set_line_trips (per scenario)
  *   trajs[m] = z_hist[:, m, :]                  ext:3075-3079    -> gp_get_trajectory
  *   _residual_all_scenarios_gpu! / _jacobian_all_scenarios_gpu!
  *                                               ext:1318-1516    -> gp_eval_residual / gp_eval_jacobian
@@ -108,6 +110,24 @@ typedef struct gp_event {
   int32_t step_off;
 } gp_event;
 
+/* One line trip of one scenario (TripLineEvent, src/GradPower.jl:140-167):
+ * the line's admittance stamp y -- what the line adds to Y at (from, from),
+ * (from, to), (to, from), (to, to), each as re, im (create_trip_line_event,
+ * src/GradPower.jl:509-542) -- is taken out of the network at the end of step
+ * step_trip and put back at the end of step step_reclose (0: never), with a
+ * dt = 0 re-solve after each, as a fault's activation and clearing
+ * (src/dynamics.jl:536-540,599-645).  Both off-diagonal blocks must be in
+ * the pattern of Y. */
+typedef struct gp_line_trip {
+  int32_t scenario;
+  int32_t from_bus;
+  int32_t to_bus;
+  int32_t step_trip;
+  int32_t step_reclose;
+  int32_t reserved;
+  double y[8];                 /* yff, yft, ytf, ytt as (re, im) pairs */
+} gp_line_trip;
+
 enum gp_method {
   GP_METHOD_MONO_FP64 = 0,    /* monolithic Newton, sparse LU of J refactored every iteration */
   GP_METHOD_SCHUR_FP64 = 1,   /* cluster Schur reduction, FP64 factors                        */
@@ -205,9 +225,17 @@ void gp_destroy(gp_ctx *ctx);
 int gp_set_state(gp_ctx *ctx, const double *z0, int32_t per_scenario,
                  const double *p);
 
-/* Per-scenario faults and horizons (nsteps[m] = round(tf_m/dt)). */
+/* Per-scenario faults and horizons (nsteps[m] = round(tf_m/dt)).  Clears
+ * the line trips of gp_set_line_trips. */
 int gp_set_events(gp_ctx *ctx, const gp_event *ev, int32_t n_ev,
                   const int32_t *nsteps);
+
+/* Per-scenario line trips and reclosures, beside the faults of the last
+ * gp_set_events (call it after that).  Faults and tripped lines share the
+ * limit of 4 simultaneously active events per scenario.  The reference runs
+ * these events in its CPU integrator only (src/dynamics.jl:502-655); here
+ * every method takes them, in the forward sweep and in gp_adjoint. */
+int gp_set_line_trips(gp_ctx *ctx, const gp_line_trip *trips, int32_t n_trips);
 
 /* Integrate all scenarios from the current state to their horizons.
  * tol: Newton tolerance on |F|inf per scenario; itermax per Newton solve. */

@@ -14,6 +14,8 @@
  *   backward Euler            src/dynamics.jl:339-500
  *   Newton                    src/numerics.jl:55-176
  *   time / event loop         src/dynamics.jl:502-655
+ *   line trips / reclosures   src/GradPower.jl:140-167,509-623 (TripLineEvent,
+ *                             _apply_trip_line!, _apply_reclose_line!)
  *   discrete adjoint          examples/adjoint.jl:34-51,106 and
  *                             test/test_adjoint_event_offset.jl:3-24 (the
  *                             adjoint itself is absent from the reference
@@ -49,6 +51,10 @@ typedef struct {
   /* sparse LU */
   int *q;          /* column order: k-th pivot column is q[k] */
   int lu_ready;
+  /* line trips of the next sweeps (orc_set_trips) */
+  int ntr, ynnz;
+  int *tr_fr, *tr_to, *tr_on, *tr_off;
+  double *tr_y;
 } orc_sys;
 
 /* ---------------------------------------------------------------- utils */
@@ -737,6 +743,7 @@ void *orc_create(const gp_system_desc *d) {
   s->yrp = idup(d->y_rowptr, 2 * d->nbus + 1);
   s->yci = idup(d->y_colind, d->y_nnz);
   s->yv = xmalloc(d->y_nnz * sizeof(double)); memcpy(s->yv, d->y_val, d->y_nnz * sizeof(double));
+  s->ynnz = d->y_nnz;
   static const int cols[GP_NDEVTYPES] = {GP_COLS_GENROU, GP_COLS_IEESGO, GP_COLS_TGOV1, GP_COLS_SEXS,
                                          GP_COLS_ESDC1A, GP_COLS_IEEEST, GP_COLS_ZIPLOAD, GP_COLS_STATICGEN};
   for (int t = 0; t < GP_NDEVTYPES; t++) { s->ndev[t] = d->n_dev[t]; s->dev[t] = idup(d->dev[t], (size_t)d->n_dev[t] * cols[t]); }
@@ -754,7 +761,46 @@ void orc_destroy(void *h) {
   if (!s) return;
   free(s->is_diff); free(s->yrp); free(s->yci); free(s->yv);
   for (int t = 0; t < GP_NDEVTYPES; t++) free(s->dev[t]);
-  free(s->u0); free(s->cl); free(s->cp); free(s->ri); free(s->q); free(s);
+  free(s->u0); free(s->cl); free(s->cp); free(s->ri); free(s->q);
+  free(s->tr_fr); free(s->tr_to); free(s->tr_on); free(s->tr_off); free(s->tr_y); free(s);
+}
+
+/* Line trips of the following sweeps (TripLineEvent, src/GradPower.jl:140-167):
+ * n rows of (from, to, step_trip, step_reclose; 0 = never) with the line's
+ * stamp y[8*i..] = yff, yft, ytf, ytt as (re, im) (create_trip_line_event,
+ * src/GradPower.jl:509-542).  n = 0 clears them. */
+void orc_set_trips(void *h, int n, const int *fr, const int *to, const int *on, const int *off, const double *y) {
+  orc_sys *s = h;
+  free(s->tr_fr); free(s->tr_to); free(s->tr_on); free(s->tr_off); free(s->tr_y);
+  s->ntr = n;
+  s->tr_fr = idup(fr, n); s->tr_to = idup(to, n); s->tr_on = idup(on, n); s->tr_off = idup(off, n);
+  s->tr_y = xmalloc(8 * (size_t)n * sizeof(double) + 8);
+  if (n) memcpy(s->tr_y, y, 8 * (size_t)n * sizeof(double));
+}
+
+/* Add sign * y (complex) to the 2x2 block (bus rb, bus cb) of the realified
+ * Y, [re -im; im re] (_add_complex_to_realified! / _subtract_complex_from_
+ * realified!, src/GradPower.jl:580-623). */
+static void y_block_add(orc_sys *s, int rb, int cb, const double *yy, double sign) {
+  for (int rr = 0; rr < 2; rr++)
+    for (int cc = 0; cc < 2; cc++) {
+      int r = 2 * rb + rr, c = 2 * cb + cc, q;
+      double v = rr == cc ? yy[0] : (rr == 0 ? -yy[1] : yy[1]);
+      for (q = s->yrp[r]; q < s->yrp[r + 1]; q++) if (s->yci[q] == c) break;
+      if (q == s->yrp[r + 1]) abort();
+      s->yv[q] += sign * v;
+    }
+}
+
+/* Take trip t out of Y (sign -1, _apply_trip_line!, src/GradPower.jl:556-568)
+ * or put it back (+1, _apply_reclose_line!, :570-578), in place. */
+static void trip_apply(orc_sys *s, int t, double sign) {
+  const double *y = s->tr_y + 8 * t;
+  int a = s->tr_fr[t], b = s->tr_to[t];
+  y_block_add(s, a, a, y, sign);
+  y_block_add(s, a, b, y + 2, sign);
+  y_block_add(s, b, a, y + 4, sign);
+  y_block_add(s, b, b, y + 6, sign);
 }
 
 int orc_nnz(void *h) { return ((orc_sys *)h)->nnz; }
@@ -816,7 +862,11 @@ static int newton(const orc_sys *s, double *z, const double *zold, const double 
  * stats[0] = Newton iterations, stats[1] = non-converged solves, stats[2] = max accepted |F|. */
 static int sweep(orc_sys *s, const double *z0, const double *p, double dt, int nsteps,
                  const int *ev_bus, const double *ev_r, const int *ev_on, const int *ev_off, int n_ev,
-                 double tol, int itermax, double *traj, double *fgs, double *zfinal, double *stats) {
+                 double tol, int itermax, double *traj, double *fgs, double *ys, double *zfinal, double *stats) {
+  /* the line trips change Y in place during the sweep, as integrate! does
+   * (src/dynamics.jl:616-624); Y is restored when the sweep ends */
+  double *y_saved = xmalloc((size_t)s->ynnz * sizeof(double) + 8);
+  memcpy(y_saved, s->yv, (size_t)s->ynnz * sizeof(double));
   int n = s->n;
   double *z = xmalloc(n * sizeof(double)), *zold = xmalloc(n * sizeof(double));
   double *f = xmalloc(n * sizeof(double)), *dx = xmalloc(n * sizeof(double)), *w = xmalloc(n * sizeof(double));
@@ -832,6 +882,7 @@ static int sweep(orc_sys *s, const double *z0, const double *p, double dt, int n
     for (int b = 0; b < s->nbus; b++) fg[b] = 0.0;
     for (int e = 0; e < n_ev; e++) if (active[e]) fg[ev_bus[e]] += 1.0 / ev_r[e];
     if (fgs) memcpy(fgs + (size_t)k * s->nbus, fg, s->nbus * sizeof(double));
+    if (ys) memcpy(ys + (size_t)k * s->ynnz, s->yv, (size_t)s->ynnz * sizeof(double));
     int it = newton(s, z, zold, p, fg, dt, tol, itermax, f, J, dx, w, &F, &fn);
     if (it < 0) { fails++; if (it <= -2) { rc = it; } } else iters += it;
     if (fn > maxf) maxf = fn;
@@ -840,6 +891,10 @@ static int sweep(orc_sys *s, const double *z0, const double *p, double dt, int n
     for (int e = 0; e < n_ev; e++) {
       if (ev_on[e] == k) { active[e] = 1; any = 1; }
       if (ev_off[e] == k) { active[e] = 0; any = 1; }
+    }
+    for (int t = 0; t < s->ntr; t++) {   /* trip, then reclose (src/dynamics.jl:536-540) */
+      if (s->tr_on[t] == k) { trip_apply(s, t, -1.0); any = 1; }
+      if (s->tr_off[t] == k) { trip_apply(s, t, 1.0); any = 1; }
     }
     if (any) {
       memcpy(zold, z, n * sizeof(double));
@@ -853,15 +908,16 @@ static int sweep(orc_sys *s, const double *z0, const double *p, double dt, int n
   }
   if (zfinal) memcpy(zfinal, z, n * sizeof(double));
   if (stats) { stats[0] = iters; stats[1] = fails; stats[2] = maxf; }
+  memcpy(s->yv, y_saved, (size_t)s->ynnz * sizeof(double));
   lu_free(&F);
-  free(z); free(zold); free(f); free(dx); free(w); free(J); free(fg); free(active);
+  free(z); free(zold); free(f); free(dx); free(w); free(J); free(fg); free(active); free(y_saved);
   return rc;
 }
 
 int orc_integrate(void *h, const double *z0, const double *p, double dt, int nsteps,
                   const int *ev_bus, const double *ev_r, const int *ev_on, const int *ev_off, int n_ev,
                   double tol, int itermax, double *traj, double *zfinal, double *stats) {
-  return sweep((orc_sys *)h, z0, p, dt, nsteps, ev_bus, ev_r, ev_on, ev_off, n_ev, tol, itermax, traj, NULL,
+  return sweep((orc_sys *)h, z0, p, dt, nsteps, ev_bus, ev_r, ev_on, ev_off, n_ev, tol, itermax, traj, NULL, NULL,
                zfinal, stats);
 }
 
@@ -901,7 +957,7 @@ int orc_linsolve_t(void *h, const double *J, const double *c, double *y) {
  * Objective  J = scale sum_{k=1..nsteps} dt sum_r (z_k[rows[r]] - off)^2
  * over the converged states z_k of the forward sweep (traj rows 1..nsteps,
  * test/test_adjoint_event_offset.jl:58-69).  For k = nsteps..1, with the
- * Jacobian and fault state of forward step k:
+ * Jacobian, fault state and Y (line trips) of forward step k:
  *   J_k^T lam_k = dJ/dz_k + lam_{k+1} on the differential rows
  *   grad_j     -= lam_k . dF_k/dp_j
  * where dF_k/dp_j is a central difference of the residual in p_j (step
@@ -917,9 +973,12 @@ int orc_adjoint(void *h, const double *z0, const double *p, double dt, int nstep
   int n = s->n, nb = s->nbus;
   double *traj = xmalloc((size_t)(nsteps + 1) * n * sizeof(double));
   double *fgs = xmalloc((size_t)(nsteps + 1) * nb * sizeof(double));
+  double *ys = xmalloc((size_t)(nsteps + 1) * s->ynnz * sizeof(double) + 8);
+  double *y_saved = xmalloc((size_t)s->ynnz * sizeof(double) + 8);
+  memcpy(y_saved, s->yv, (size_t)s->ynnz * sizeof(double));
   double st[3];
-  int rc = sweep(s, z0, p, dt, nsteps, ev_bus, ev_r, ev_on, ev_off, n_ev, tol, itermax, traj, fgs, NULL, st);
-  if (rc) { free(traj); free(fgs); return rc; }
+  int rc = sweep(s, z0, p, dt, nsteps, ev_bus, ev_r, ev_on, ev_off, n_ev, tol, itermax, traj, fgs, ys, NULL, st);
+  if (rc) { free(traj); free(fgs); free(ys); free(y_saved); return rc; }
   double *lam = xmalloc(n * sizeof(double)), *a = xmalloc(n * sizeof(double)), *w = xmalloc(n * sizeof(double));
   double *J = xmalloc(s->nnz * sizeof(double)), *fp = xmalloc(n * sizeof(double)), *fm = xmalloc(n * sizeof(double));
   double *pp = xmalloc((s->npar + 1) * sizeof(double));
@@ -934,6 +993,7 @@ int orc_adjoint(void *h, const double *z0, const double *p, double dt, int nstep
     int kf = k + fault_shift;   /* fault_shift != 0 only plants a misalignment in tests */
     kf = kf < 1 ? 1 : (kf > nsteps ? nsteps : kf);
     const double *zk = traj + (size_t)k * n, *zkm = traj + (size_t)(k - 1) * n, *fg = fgs + (size_t)kf * nb;
+    memcpy(s->yv, ys + (size_t)kf * s->ynnz, (size_t)s->ynnz * sizeof(double));   /* Y of step kf */
     for (int i = 0; i < n; i++) a[i] = s->is_diff[i] ? lam[i] : 0.0;
     for (int r = 0; r < n_rows; r++) a[rows[r]] += 2.0 * scale * dt * (zk[rows[r]] - off);
     beuler_jac(s, zk, p, fg, dt, J);
@@ -954,8 +1014,9 @@ int orc_adjoint(void *h, const double *z0, const double *p, double dt, int nstep
   }
   if (value) *value = val;
   if (dz0) for (int i = 0; i < n; i++) dz0[i] = s->is_diff[i] ? lam[i] : 0.0;
+  memcpy(s->yv, y_saved, (size_t)s->ynnz * sizeof(double));
   lu_free(&F);
-  free(traj); free(fgs); free(lam); free(a); free(w); free(J); free(fp); free(fm); free(pp);
+  free(traj); free(fgs); free(ys); free(y_saved); free(lam); free(a); free(w); free(J); free(fp); free(fm); free(pp);
   return rc;
 }
 

@@ -38,6 +38,7 @@ def _load():
     lib.orc_integrate.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, _ip, _dp, _ip, _ip,
                                   C.c_int, C.c_double, C.c_int, _dp, _dp, _dp]
     lib.orc_schur_step.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    lib.orc_set_trips.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _dp]
     lib.orc_linsolve_t.argtypes = [C.c_void_p, _dp, _dp, _dp]
     lib.orc_adjoint.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, _ip, _dp, _ip, _ip, C.c_int,
                                 C.c_double, C.c_int, C.c_int, _ip, C.c_double, C.c_double, C.c_int, _ip,
@@ -123,9 +124,22 @@ class Oracle:
         lib().orc_schur_step(self.h, _d(np.ascontiguousarray(Jvals)), _d(np.ascontiguousarray(f)), _d(dz))
         return dz
 
-    def integrate(self, dt, nsteps, events=(), z0=None, tol=1e-10, itermax=50, traj=True, p=None):
-        """events: iterable of (bus, r_fault, step_on, step_off).
+    def _set_trips(self, trips):
+        """trips: iterable of (from_bus, to_bus, step_trip, step_reclose (0:
+        never), y[8] = yff, yft, ytf, ytt as re, im)."""
+        tr = list(trips)
+        fr = np.array([t[0] for t in tr] or [0], np.int32)
+        to = np.array([t[1] for t in tr] or [0], np.int32)
+        on = np.array([t[2] for t in tr] or [0], np.int32)
+        off = np.array([t[3] for t in tr] or [0], np.int32)
+        y = np.ascontiguousarray(np.array([list(t[4]) for t in tr], np.float64).reshape(-1) if tr else np.zeros(8))
+        lib().orc_set_trips(self.h, len(tr), _i(fr), _i(to), _i(on), _i(off), _d(y))
+
+    def integrate(self, dt, nsteps, events=(), z0=None, tol=1e-10, itermax=50, traj=True, p=None, trips=()):
+        """events: iterable of (bus, r_fault, step_on, step_off); trips: line
+        trips, see _set_trips.
         Returns (traj [(nsteps+1) x n] or None, zfinal, stats dict)."""
+        self._set_trips(trips)
         z0 = np.ascontiguousarray(self.desc.z0 if z0 is None else z0, np.float64)
         p = self.p if p is None else np.ascontiguousarray(p, np.float64)
         ev = list(events)
@@ -149,12 +163,13 @@ class Oracle:
         return y
 
     def adjoint(self, dt, nsteps, params, events=(), rows=None, scale=1.0, offset=0.0, z0=None, tol=1e-10,
-                itermax=50, p=None, fault_shift=0):
+                itermax=50, p=None, fault_shift=0, trips=()):
         """Objective scale * sum_k dt sum_r (z_k[r] - offset)^2 and its gradient
         w.r.t. p[params] by the oracle's discrete adjoint.  rows=None: the
         GENROU speed deviations.  fault_shift != 0 plants a misaligned fault
         state (tests only).  Returns (value, grad, dz0)."""
         from gradpower_hip.desc import gen_speed_indices
+        self._set_trips(trips)
         z0 = np.ascontiguousarray(self.desc.z0 if z0 is None else z0, np.float64)
         p = self.p if p is None else np.ascontiguousarray(p, np.float64)
         rows = np.ascontiguousarray(gen_speed_indices(self.desc) if rows is None else rows, np.int32)
@@ -175,11 +190,11 @@ class Oracle:
         return val.value, grad[:len(par)], dz0
 
     def objective(self, dt, nsteps, events=(), rows=None, scale=1.0, offset=0.0, z0=None, tol=1e-10,
-                  itermax=50, p=None):
+                  itermax=50, p=None, trips=()):
         """scale * sum_{k>=1} dt sum_r (z_k[r] - offset)^2 along the oracle's trajectory."""
         from gradpower_hip.desc import gen_speed_indices
         rows = gen_speed_indices(self.desc) if rows is None else np.asarray(rows)
-        tr, _, st = self.integrate(dt, nsteps, events, z0=z0, tol=tol, itermax=itermax, p=p)
+        tr, _, st = self.integrate(dt, nsteps, events, z0=z0, tol=tol, itermax=itermax, p=p, trips=trips)
         if st["rc"]:
             raise RuntimeError(f"oracle forward sweep failed ({st['rc']})")
         e = tr[1:, rows] - offset
