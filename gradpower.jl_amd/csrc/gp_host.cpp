@@ -382,6 +382,7 @@ struct gp_ctx {
   // trip (bus = -2 - t for trip record t, see FaultState)
   std::vector<gp_event> events, faults_user;
   std::vector<gp_line_trip> trips;
+  std::vector<gp_load_drop> drops;
   // line trips: records (from, to; stamp) and the off-diagonal Y entries
   // that trips change, kept once made variable (see DevTables)
   std::vector<int> tr_bus, tv_p, tv_i;
@@ -1534,7 +1535,7 @@ struct Stepper {
       for (int m = 0; m < M; m++) {
         if (!res[m]) continue;
         if ((int)st.active_ev[m].size() > MAXFAULT)
-          throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults or line trips in one scenario"};
+          throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults, line trips or load drops in one scenario"};
         for (int q = 0; q < MAXFAULT; q++) {
           const bool on = q < (int)st.active_ev[m].size();
           st.fbus[(size_t)q * L + m] = on ? st.active_ev[m][q]->bus : -1;
@@ -1624,7 +1625,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
   std::vector<double> fg((size_t)MAXFAULT * L, 0.0);
   auto set_faults = [&](int m) {
     if ((int)active_ev[m].size() > MAXFAULT)
-      throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults or line trips in one scenario"};
+      throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults, line trips or load drops in one scenario"};
     for (int q = 0; q < MAXFAULT; q++) {
       const bool f_on = q < (int)active_ev[m].size();
       fbus[(size_t)q * L + m] = f_on ? active_ev[m][q]->bus : -1;
@@ -2060,6 +2061,7 @@ int gp_set_events(gp_ctx *C, const gp_event *ev, int32_t n_ev, const int32_t *ns
   C->events = C->faults_user;
   if (!C->trips.empty()) C->trip_gen++;   // the LU plans' pivots were chosen with the trips
   C->trips.clear();
+  C->drops.clear();
   C->tr_bus.clear();
   C->tr_y.clear();
   C->nsteps.assign(nsteps, nsteps + C->M);
@@ -2068,6 +2070,43 @@ int gp_set_events(gp_ctx *C, const gp_event *ev, int32_t n_ev, const int32_t *ns
   } catch (const GpError &e) {
     return fail(e.code, e.msg);
   }
+  return GP_OK;
+}
+
+// The event list the time loops read: the faults, then one record per line
+// trip (slot code -2 - t) and per ZIP load disconnect (kDropCode0 - load).
+static void rebuild_events(gp_ctx *C) {
+  C->events = C->faults_user;
+  for (size_t i = 0; i < C->trips.size(); i++) {
+    gp_event e{};
+    e.scenario = C->trips[i].scenario;
+    e.bus = -2 - (int)i;
+    e.r_fault = 1.0;
+    e.step_on = C->trips[i].step_trip;
+    e.step_off = C->trips[i].step_reclose > 0 ? C->trips[i].step_reclose : std::numeric_limits<int32_t>::max();
+    C->events.push_back(e);
+  }
+  for (const gp_load_drop &d : C->drops) {
+    gp_event e{};
+    e.scenario = d.scenario;
+    e.bus = kDropCode0 - d.load;
+    e.r_fault = 1.0;
+    e.step_on = d.step;
+    e.step_off = std::numeric_limits<int32_t>::max();
+    C->events.push_back(e);
+  }
+}
+
+int gp_set_load_drops(gp_ctx *C, const gp_load_drop *d, int32_t n) {
+  if (!C || (n > 0 && !d) || n < 0) return fail(GP_ERR_BADARG, "gp_set_load_drops: bad argument");
+  const int nzip = (int)(C->dev[GP_ZIPLOAD].size() / GP_COLS_ZIPLOAD);
+  for (int i = 0; i < n; i++) {
+    if (d[i].scenario < 0 || d[i].scenario >= C->M || d[i].load < 0 || d[i].load >= nzip)
+      return fail(GP_ERR_BADARG, "gp_set_load_drops: scenario or load out of range");
+    if (d[i].step < 1) return fail(GP_ERR_BADARG, "gp_set_load_drops: step must be >= 1");
+  }
+  C->drops.assign(d, d + n);
+  rebuild_events(C);
   return GP_OK;
 }
 
@@ -2148,19 +2187,12 @@ int gp_set_line_trips(gp_ctx *C, const gp_line_trip *tr, int32_t n) {
     C->trips.assign(tr, tr + n);
     C->tr_bus.clear();
     C->tr_y.clear();
-    C->events = C->faults_user;
     for (int i = 0; i < n; i++) {
       C->tr_bus.push_back(tr[i].from_bus);
       C->tr_bus.push_back(tr[i].to_bus);
       C->tr_y.insert(C->tr_y.end(), tr[i].y, tr[i].y + 8);
-      gp_event e{};
-      e.scenario = tr[i].scenario;
-      e.bus = -2 - i;
-      e.r_fault = 1.0;
-      e.step_on = tr[i].step_trip;
-      e.step_off = tr[i].step_reclose > 0 ? tr[i].step_reclose : std::numeric_limits<int32_t>::max();
-      C->events.push_back(e);
     }
+    rebuild_events(C);
     upload_trip_tables(C);
   } catch (const GpError &e) {
     return fail(e.code, e.msg);

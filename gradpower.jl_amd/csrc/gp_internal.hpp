@@ -11,6 +11,7 @@ namespace gp {
 // scenarios) touch 512 contiguous bytes.
 constexpr int WAVE = 64;
 constexpr int MAXFAULT = 4;   // simultaneously active faults per scenario
+constexpr int kDropCode0 = -(1 << 24);   // event slot code of ZIP load 0 disconnected
 
 // Jacobian slot counts per device (order of the slot lists built on the host).
 constexpr int NS_GENROU = 47;   // incl. optional pm, efd slots (-1 when unwired) and the delta diagonal
@@ -59,8 +60,9 @@ struct DevTables {
 };
 
 // Per-scenario event slots: bus[k*ldM+m] >= 0 is a fault on that bus with
-// conductance g[k*ldM+m]; bus = -2 - t is the trip of line record t; -1 is
-// an empty slot.
+// conductance g[k*ldM+m]; kDropCode0 < bus <= -2 is the trip of line record
+// t = -2 - bus; bus <= kDropCode0 is ZIP load kDropCode0 - bus disconnected;
+// -1 is an empty slot.
 struct FaultState {
   int *bus;
   double *g;

@@ -309,6 +309,15 @@ __global__ void __launch_bounds__(64) k_cluster_residual(DevTables T, const doub
   (void)ws;
 }
 
+// ZIP load k disconnected in scenario m (DisconnectDeviceEvent,
+// src/GradPower.jl:130-143,681-698: its residual and Jacobian leave the bus).
+__device__ __forceinline__ bool zip_dropped(const FaultState &fs, int ldM, int m, int k) {
+  bool off = false;
+#pragma unroll
+  for (int s = 0; s < MAXFAULT; s++) off = off || fs.bus[s * ldM + m] == kDropCode0 - k;
+  return off;
+}
+
 // Line trips active in scenario m (event slots holding -2 - t): a tripped
 // line's stamp leaves Y.  The network rows hold -Y v, so the rows of the
 // line's buses gain +y v: at the from bus yff v_from + yft v_to, at the to
@@ -320,7 +329,7 @@ __device__ __forceinline__ void trip_residual(const DevTables &T, const FaultSta
 #pragma unroll
   for (int k = 0; k < MAXFAULT; k++) {
     const int code = fs.bus[k * ldM + m];
-    if (code > -2) continue;
+    if (code > -2 || code <= kDropCode0) continue;
     const int t = -2 - code;
     const int a = T.tr_bus[2 * t], c = T.tr_bus[2 * t + 1];
     if (a != b && c != b) continue;
@@ -344,7 +353,7 @@ __device__ __forceinline__ double trip_offdiag(const DevTables &T, const FaultSt
 #pragma unroll
   for (int k = 0; k < MAXFAULT; k++) {
     const int code = fs.bus[k * ldM + m];
-    if (code > -2) continue;
+    if (code > -2 || code <= kDropCode0) continue;
     const int t = -2 - code;
     const int a = T.tr_bus[2 * t], c = T.tr_bus[2 * t + 1];
     const double *y = T.tr_y + 8 * t;
@@ -380,6 +389,7 @@ __global__ void __launch_bounds__(64) k_bus_residual(DevTables T, const double *
   }
   const double vm2r = vr * vr + vi * vi;
   for (int q = T.bus_zip_p[b]; q < T.bus_zip_p[b + 1]; q++) {
+    if (zip_dropped(fs, ldM, m, T.bus_zip[q])) continue;
     const double *P = p + T.zip[T.bus_zip[q] * GP_COLS_ZIPLOAD + 1];
     const double pl = P[0], ql = P[1], al = P[2], yre = al * P[7], yim = al * P[8];
     const double den = vm2r > 0.2 ? vm2r : 0.2;
@@ -709,7 +719,7 @@ __device__ __forceinline__ void bus_diag(const DevTables &T, const double *p, co
   const int ldM = T.ldM;
   dg[0] = -T.ydiag[4 * b]; dg[1] = -T.ydiag[4 * b + 1]; dg[2] = -T.ydiag[4 * b + 2]; dg[3] = -T.ydiag[4 * b + 3];
   for (int q = T.bus_zip_p[b]; q < T.bus_zip_p[b + 1]; q++)
-    zip_jac(p + T.zip[T.bus_zip[q] * GP_COLS_ZIPLOAD + 1], vr, vi, dg);
+    if (!zip_dropped(fs, ldM, m, T.bus_zip[q])) zip_jac(p + T.zip[T.bus_zip[q] * GP_COLS_ZIPLOAD + 1], vr, vi, dg);
   const int sgk = T.bus_sg[b];
   for (int i = 0; i < 11; i++) sgo[i] = 0.0;
   if (sgk >= 0) {
@@ -727,7 +737,7 @@ __device__ __forceinline__ void bus_diag(const DevTables &T, const double *p, co
       const double gf = fs.g[k * ldM + m];
       dg[0] -= gf;
       dg[3] -= gf;
-    } else if (code <= -2) {   // a tripped line at this bus: -Y_bb + yff (or ytt)
+    } else if (code <= -2 && code > kDropCode0) {   // a tripped line at this bus: -Y_bb + yff (or ytt)
       const int t = -2 - code;
       const int a = T.tr_bus[2 * t], c = T.tr_bus[2 * t + 1];
       if (a == b || c == b) {

@@ -44,6 +44,10 @@ class CLineTrip(C.Structure):
                 ("y", C.c_double * 8)]
 
 
+class CLoadDrop(C.Structure):
+    _fields_ = [("scenario", C.c_int32), ("load", C.c_int32), ("step", C.c_int32), ("reserved", C.c_int32)]
+
+
 def _p(a, ct):
     return a.ctypes.data_as(C.POINTER(ct))
 

@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-from .desc import SystemDesc, CEvent, CLineTrip
+from .desc import SystemDesc, CEvent, CLineTrip, CLoadDrop
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libgradpower_hip.so")
@@ -77,7 +77,8 @@ _ip = C.POINTER(C.c_int32)
 _up = C.POINTER(C.c_uint8)
 _LIB = None
 
-EXPORTS = ["gp_create", "gp_destroy", "gp_set_state", "gp_set_events", "gp_set_line_trips", "gp_integrate",
+EXPORTS = ["gp_create", "gp_destroy", "gp_set_state", "gp_set_events", "gp_set_line_trips", "gp_set_load_drops",
+           "gp_integrate",
            "gp_get_trajectory", "gp_get_state", "gp_eval_residual", "gp_jacobian_pattern",
            "gp_eval_jacobian", "gp_newton_step", "gp_get_symbolic_info", "gp_get_scenario_status",
            "gp_copy_kernel", "gp_adjoint", "gp_last_error", "gp_version"]
@@ -98,6 +99,7 @@ def load():
     lib.gp_set_state.argtypes = [C.c_void_p, _dp, C.c_int32, _dp]
     lib.gp_set_events.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, _ip]
     lib.gp_set_line_trips.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
+    lib.gp_set_load_drops.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
     lib.gp_integrate.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_void_p]
     lib.gp_get_trajectory.argtypes = [C.c_void_p, C.c_int32, _dp, C.c_int64]
     lib.gp_get_state.argtypes = [C.c_void_p, _dp]
@@ -211,6 +213,16 @@ class HipContext:
             for k in range(8):
                 arr[i].y[k] = float(yv[k])
         self._check(self.lib.gp_set_line_trips(self.h, C.byref(arr), len(tr)))
+
+    def set_load_drops(self, drops):
+        """drops: iterable of (scenario, ZIP table row, step): the load goes
+        offline for good at the end of that step (system.DisconnectDeviceEvent).
+        Call after set_events, which clears them."""
+        d = list(drops)
+        arr = (CLoadDrop * max(1, len(d)))()
+        for i, (m, k, step) in enumerate(d):
+            arr[i].scenario, arr[i].load, arr[i].step = int(m), int(k), int(step)
+        self._check(self.lib.gp_set_load_drops(self.h, C.byref(arr), len(d)))
 
     def integrate(self, method, dt, tol=1e-10, itermax=50, raise_on_fail=True):
         st = CStats()

@@ -115,6 +115,22 @@ class TripLineEvent:
         return int(round(self.ton / dt)), off
 
 
+@dataclass
+class DisconnectDeviceEvent:
+    """Device `device_idx` (0-based index into ps.dynamic.devices) taken
+    offline for good at ton (src/GradPower.jl:130-143).  The HIP library
+    takes ZIP loads only (zip_row gives the row of its ZIP table)."""
+    device_idx: int
+    ton: float
+
+    def zip_row(self, ps):
+        devs = ps.dynamic.devices
+        if devs[self.device_idx].dtype.kind != "zipload":
+            raise NotImplementedError("only ZIP loads can be disconnected: a device with algebraic states "
+                                      "left out of F and J leaves them undetermined")
+        return sum(1 for d in devs[:self.device_idx] if d.dtype.kind == "zipload")
+
+
 def create_trip_line_event(ps, from_bus_ext, to_bus_ext, ton, toff=math.inf):
     """The first branch between the two buses (either orientation) and the
     admittance it adds to Y, by the same formulas as the Y builder

@@ -10,6 +10,8 @@
  *                                               ext:2640-2708    -> gp_set_events (per-scenario faults)
  *   TripLineEvent / create_trip_line_event     src/GradPower.jl:140-167,509-570
  *   (CPU integrate! only, src/dynamics.jl:536-540,616-624)        -> gp_set_line_trips (per scenario)
+ *   DisconnectDeviceEvent (ZIP loads)          src/GradPower.jl:130-143,681-698
+ *   (CPU integrate! only, src/dynamics.jl:533-535,612-615)        -> gp_set_load_drops (per scenario)
  *   trajs[m] = z_hist[:, m, :]                  ext:3075-3079    -> gp_get_trajectory
  *   _residual_all_scenarios_gpu! / _jacobian_all_scenarios_gpu!
  *                                               ext:1318-1516    -> gp_eval_residual / gp_eval_jacobian
@@ -128,6 +130,19 @@ typedef struct gp_line_trip {
   double y[8];                 /* yff, yft, ytf, ytt as (re, im) pairs */
 } gp_line_trip;
 
+/* One ZIP load of one scenario disconnected (DisconnectDeviceEvent,
+ * src/GradPower.jl:130-143,681-698) at the end of step `step`, for good, with
+ * a dt = 0 re-solve after it: its current leaves the bus's rows (the
+ * reference skips offline devices in its kernels).  `load` is the row of the
+ * ZIP load in desc->dev[GP_ZIPLOAD].  Only ZIP loads: a device with algebraic
+ * states left out of F and J leaves them undetermined. */
+typedef struct gp_load_drop {
+  int32_t scenario;
+  int32_t load;
+  int32_t step;
+  int32_t reserved;
+} gp_load_drop;
+
 enum gp_method {
   GP_METHOD_MONO_FP64 = 0,    /* monolithic Newton, sparse LU of J refactored every iteration */
   GP_METHOD_SCHUR_FP64 = 1,   /* cluster Schur reduction, FP64 factors                        */
@@ -236,6 +251,11 @@ int gp_set_events(gp_ctx *ctx, const gp_event *ev, int32_t n_ev,
  * these events in its CPU integrator only (src/dynamics.jl:502-655); here
  * every method takes them, in the forward sweep and in gp_adjoint. */
 int gp_set_line_trips(gp_ctx *ctx, const gp_line_trip *trips, int32_t n_trips);
+
+/* Per-scenario ZIP load disconnects (call after gp_set_events, which clears
+ * them).  They share the 4 event slots of a scenario with its active faults
+ * and tripped lines, and hold a slot from their step on. */
+int gp_set_load_drops(gp_ctx *ctx, const gp_load_drop *drops, int32_t n_drops);
 
 /* Integrate all scenarios from the current state to their horizons.
  * tol: Newton tolerance on |F|inf per scenario; itermax per Newton solve. */

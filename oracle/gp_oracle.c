@@ -16,6 +16,7 @@
  *   time / event loop         src/dynamics.jl:502-655
  *   line trips / reclosures   src/GradPower.jl:140-167,509-623 (TripLineEvent,
  *                             _apply_trip_line!, _apply_reclose_line!)
+ *   ZIP load disconnects      src/GradPower.jl:130-143,681-698 (DisconnectDeviceEvent)
  *   discrete adjoint          examples/adjoint.jl:34-51,106 and
  *                             test/test_adjoint_event_offset.jl:3-24 (the
  *                             adjoint itself is absent from the reference
@@ -55,6 +56,11 @@ typedef struct {
   int ntr, ynnz;
   int *tr_fr, *tr_to, *tr_on, *tr_off;
   double *tr_y;
+  /* ZIP load disconnects of the next sweeps (orc_set_drops) and, during a
+   * sweep, the online flag of every ZIP load (NULL: all online) */
+  int ndrop;
+  int *drop_k, *drop_step;
+  uint8_t *zon;
 } orc_sys;
 
 /* ---------------------------------------------------------------- utils */
@@ -306,6 +312,7 @@ static void rhs(const orc_sys *s, const double *z, const double *p, const double
     f[dp + 2] = (vr1 - Ke * e_fd - sat) / Te;
   }
   for (int k = 0; k < s->ndev[GP_ZIPLOAD]; k++) {   /* src/kernels/zipload.jl:33-85 */
+    if (s->zon && !s->zon[k]) continue;
     int bus = D(6, k, 0);
     const double *P = p + D(6, k, 1);
     int vri = npt + 2 * bus;
@@ -520,6 +527,7 @@ static void rhs_jac(const orc_sys *s, const double *z, const double *p, const do
     jset(s, J, dp + 2, dp + 2, -(Ke + dsat) / Te);
   }
   for (int k = 0; k < s->ndev[GP_ZIPLOAD]; k++) {   /* src/kernels/zipload.jl:133-179 */
+    if (s->zon && !s->zon[k]) continue;
     int bus = D(6, k, 0), vri = npt + 2 * bus;
     const double *P = p + D(6, k, 1);
     double vr = z[vri], vi = z[vri + 1], pl = P[0], ql = P[1], al = P[2], yre = P[7], yim = P[8];
@@ -762,7 +770,20 @@ void orc_destroy(void *h) {
   free(s->is_diff); free(s->yrp); free(s->yci); free(s->yv);
   for (int t = 0; t < GP_NDEVTYPES; t++) free(s->dev[t]);
   free(s->u0); free(s->cl); free(s->cp); free(s->ri); free(s->q);
-  free(s->tr_fr); free(s->tr_to); free(s->tr_on); free(s->tr_off); free(s->tr_y); free(s);
+  free(s->tr_fr); free(s->tr_to); free(s->tr_on); free(s->tr_off); free(s->tr_y);
+  free(s->drop_k); free(s->drop_step); free(s);
+}
+
+/* ZIP load disconnects of the following sweeps (DisconnectDeviceEvent,
+ * src/GradPower.jl:130-143,681-698): ZIP load k[i] (row of the ZIP table)
+ * goes offline for good at the end of step step[i]; an offline device is
+ * skipped by the residual and Jacobian (src/kernels/zipload.jl, `online`). */
+void orc_set_drops(void *h, int n, const int *k, const int *step) {
+  orc_sys *s = h;
+  free(s->drop_k); free(s->drop_step);
+  s->ndrop = n;
+  s->drop_k = idup(k, n);
+  s->drop_step = idup(step, n);
 }
 
 /* Line trips of the following sweeps (TripLineEvent, src/GradPower.jl:140-167):
@@ -862,11 +883,16 @@ static int newton(const orc_sys *s, double *z, const double *zold, const double 
  * stats[0] = Newton iterations, stats[1] = non-converged solves, stats[2] = max accepted |F|. */
 static int sweep(orc_sys *s, const double *z0, const double *p, double dt, int nsteps,
                  const int *ev_bus, const double *ev_r, const int *ev_on, const int *ev_off, int n_ev,
-                 double tol, int itermax, double *traj, double *fgs, double *ys, double *zfinal, double *stats) {
+                 double tol, int itermax, double *traj, double *fgs, double *ys, uint8_t *zons, double *zfinal,
+                 double *stats) {
   /* the line trips change Y in place during the sweep, as integrate! does
    * (src/dynamics.jl:616-624); Y is restored when the sweep ends */
   double *y_saved = xmalloc((size_t)s->ynnz * sizeof(double) + 8);
   memcpy(y_saved, s->yv, (size_t)s->ynnz * sizeof(double));
+  int nz = s->ndev[GP_ZIPLOAD];
+  uint8_t *zon = xmalloc((size_t)nz + 1);
+  memset(zon, 1, (size_t)nz + 1);
+  s->zon = zon;
   int n = s->n;
   double *z = xmalloc(n * sizeof(double)), *zold = xmalloc(n * sizeof(double));
   double *f = xmalloc(n * sizeof(double)), *dx = xmalloc(n * sizeof(double)), *w = xmalloc(n * sizeof(double));
@@ -883,6 +909,7 @@ static int sweep(orc_sys *s, const double *z0, const double *p, double dt, int n
     for (int e = 0; e < n_ev; e++) if (active[e]) fg[ev_bus[e]] += 1.0 / ev_r[e];
     if (fgs) memcpy(fgs + (size_t)k * s->nbus, fg, s->nbus * sizeof(double));
     if (ys) memcpy(ys + (size_t)k * s->ynnz, s->yv, (size_t)s->ynnz * sizeof(double));
+    if (zons) memcpy(zons + (size_t)k * (nz + 1), zon, (size_t)nz + 1);
     int it = newton(s, z, zold, p, fg, dt, tol, itermax, f, J, dx, w, &F, &fn);
     if (it < 0) { fails++; if (it <= -2) { rc = it; } } else iters += it;
     if (fn > maxf) maxf = fn;
@@ -892,6 +919,8 @@ static int sweep(orc_sys *s, const double *z0, const double *p, double dt, int n
       if (ev_on[e] == k) { active[e] = 1; any = 1; }
       if (ev_off[e] == k) { active[e] = 0; any = 1; }
     }
+    for (int d = 0; d < s->ndrop; d++)   /* disconnects (src/dynamics.jl:533-535,612-615) */
+      if (s->drop_step[d] == k) { zon[s->drop_k[d]] = 0; any = 1; }
     for (int t = 0; t < s->ntr; t++) {   /* trip, then reclose (src/dynamics.jl:536-540) */
       if (s->tr_on[t] == k) { trip_apply(s, t, -1.0); any = 1; }
       if (s->tr_off[t] == k) { trip_apply(s, t, 1.0); any = 1; }
@@ -909,8 +938,9 @@ static int sweep(orc_sys *s, const double *z0, const double *p, double dt, int n
   if (zfinal) memcpy(zfinal, z, n * sizeof(double));
   if (stats) { stats[0] = iters; stats[1] = fails; stats[2] = maxf; }
   memcpy(s->yv, y_saved, (size_t)s->ynnz * sizeof(double));
+  s->zon = NULL;
   lu_free(&F);
-  free(z); free(zold); free(f); free(dx); free(w); free(J); free(fg); free(active); free(y_saved);
+  free(z); free(zold); free(f); free(dx); free(w); free(J); free(fg); free(active); free(y_saved); free(zon);
   return rc;
 }
 
@@ -918,7 +948,7 @@ int orc_integrate(void *h, const double *z0, const double *p, double dt, int nst
                   const int *ev_bus, const double *ev_r, const int *ev_on, const int *ev_off, int n_ev,
                   double tol, int itermax, double *traj, double *zfinal, double *stats) {
   return sweep((orc_sys *)h, z0, p, dt, nsteps, ev_bus, ev_r, ev_on, ev_off, n_ev, tol, itermax, traj, NULL, NULL,
-               zfinal, stats);
+               NULL, zfinal, stats);
 }
 
 /* y = A^{-T} c with the factorisation of A (P A Q = L U): Q U^T L^T P y = c,
@@ -957,7 +987,7 @@ int orc_linsolve_t(void *h, const double *J, const double *c, double *y) {
  * Objective  J = scale sum_{k=1..nsteps} dt sum_r (z_k[rows[r]] - off)^2
  * over the converged states z_k of the forward sweep (traj rows 1..nsteps,
  * test/test_adjoint_event_offset.jl:58-69).  For k = nsteps..1, with the
- * Jacobian, fault state and Y (line trips) of forward step k:
+ * Jacobian, fault state, Y (line trips) and online loads of forward step k:
  *   J_k^T lam_k = dJ/dz_k + lam_{k+1} on the differential rows
  *   grad_j     -= lam_k . dF_k/dp_j
  * where dF_k/dp_j is a central difference of the residual in p_j (step
@@ -976,9 +1006,11 @@ int orc_adjoint(void *h, const double *z0, const double *p, double dt, int nstep
   double *ys = xmalloc((size_t)(nsteps + 1) * s->ynnz * sizeof(double) + 8);
   double *y_saved = xmalloc((size_t)s->ynnz * sizeof(double) + 8);
   memcpy(y_saved, s->yv, (size_t)s->ynnz * sizeof(double));
+  int nz = s->ndev[GP_ZIPLOAD];
+  uint8_t *zons = xmalloc((size_t)(nsteps + 1) * (nz + 1));
   double st[3];
-  int rc = sweep(s, z0, p, dt, nsteps, ev_bus, ev_r, ev_on, ev_off, n_ev, tol, itermax, traj, fgs, ys, NULL, st);
-  if (rc) { free(traj); free(fgs); free(ys); free(y_saved); return rc; }
+  int rc = sweep(s, z0, p, dt, nsteps, ev_bus, ev_r, ev_on, ev_off, n_ev, tol, itermax, traj, fgs, ys, zons, NULL, st);
+  if (rc) { free(traj); free(fgs); free(ys); free(y_saved); free(zons); return rc; }
   double *lam = xmalloc(n * sizeof(double)), *a = xmalloc(n * sizeof(double)), *w = xmalloc(n * sizeof(double));
   double *J = xmalloc(s->nnz * sizeof(double)), *fp = xmalloc(n * sizeof(double)), *fm = xmalloc(n * sizeof(double));
   double *pp = xmalloc((s->npar + 1) * sizeof(double));
@@ -994,6 +1026,7 @@ int orc_adjoint(void *h, const double *z0, const double *p, double dt, int nstep
     kf = kf < 1 ? 1 : (kf > nsteps ? nsteps : kf);
     const double *zk = traj + (size_t)k * n, *zkm = traj + (size_t)(k - 1) * n, *fg = fgs + (size_t)kf * nb;
     memcpy(s->yv, ys + (size_t)kf * s->ynnz, (size_t)s->ynnz * sizeof(double));   /* Y of step kf */
+    s->zon = zons + (size_t)kf * (nz + 1);                                         /* its loads */
     for (int i = 0; i < n; i++) a[i] = s->is_diff[i] ? lam[i] : 0.0;
     for (int r = 0; r < n_rows; r++) a[rows[r]] += 2.0 * scale * dt * (zk[rows[r]] - off);
     beuler_jac(s, zk, p, fg, dt, J);
@@ -1015,8 +1048,9 @@ int orc_adjoint(void *h, const double *z0, const double *p, double dt, int nstep
   if (value) *value = val;
   if (dz0) for (int i = 0; i < n; i++) dz0[i] = s->is_diff[i] ? lam[i] : 0.0;
   memcpy(s->yv, y_saved, (size_t)s->ynnz * sizeof(double));
+  s->zon = NULL;
   lu_free(&F);
-  free(traj); free(fgs); free(ys); free(y_saved); free(lam); free(a); free(w); free(J); free(fp); free(fm); free(pp);
+  free(traj); free(fgs); free(ys); free(y_saved); free(zons); free(lam); free(a); free(w); free(J); free(fp); free(fm); free(pp);
   return rc;
 }
 

@@ -39,6 +39,7 @@ def _load():
                                   C.c_int, C.c_double, C.c_int, _dp, _dp, _dp]
     lib.orc_schur_step.argtypes = [C.c_void_p, _dp, _dp, _dp]
     lib.orc_set_trips.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _dp]
+    lib.orc_set_drops.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
     lib.orc_linsolve_t.argtypes = [C.c_void_p, _dp, _dp, _dp]
     lib.orc_adjoint.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, _ip, _dp, _ip, _ip, C.c_int,
                                 C.c_double, C.c_int, C.c_int, _ip, C.c_double, C.c_double, C.c_int, _ip,
@@ -108,9 +109,9 @@ class Oracle:
         lib().orc_jacobian(self.h, _d(z), _d(p), _d(g), dt, _d(J))
         return J
 
-    def jacobian_csc(self, z, dt, fault_g=None):
+    def jacobian_csc(self, z, dt, fault_g=None, p=None):
         import scipy.sparse as sp
-        return sp.csc_matrix((self.jacobian(z, dt, fault_g), self.ri, self.cp), shape=(self.n, self.n))
+        return sp.csc_matrix((self.jacobian(z, dt, fault_g, p=p), self.ri, self.cp), shape=(self.n, self.n))
 
     def linsolve(self, Jvals, b):
         x = np.zeros(self.n)
@@ -135,11 +136,20 @@ class Oracle:
         y = np.ascontiguousarray(np.array([list(t[4]) for t in tr], np.float64).reshape(-1) if tr else np.zeros(8))
         lib().orc_set_trips(self.h, len(tr), _i(fr), _i(to), _i(on), _i(off), _d(y))
 
-    def integrate(self, dt, nsteps, events=(), z0=None, tol=1e-10, itermax=50, traj=True, p=None, trips=()):
+    def _set_drops(self, drops):
+        """drops: iterable of (ZIP table row, step)."""
+        d = list(drops)
+        k = np.array([x[0] for x in d] or [0], np.int32)
+        st = np.array([x[1] for x in d] or [0], np.int32)
+        lib().orc_set_drops(self.h, len(d), _i(k), _i(st))
+
+    def integrate(self, dt, nsteps, events=(), z0=None, tol=1e-10, itermax=50, traj=True, p=None, trips=(),
+                  drops=()):
         """events: iterable of (bus, r_fault, step_on, step_off); trips: line
-        trips, see _set_trips.
+        trips, see _set_trips; drops: ZIP load disconnects, see _set_drops.
         Returns (traj [(nsteps+1) x n] or None, zfinal, stats dict)."""
         self._set_trips(trips)
+        self._set_drops(drops)
         z0 = np.ascontiguousarray(self.desc.z0 if z0 is None else z0, np.float64)
         p = self.p if p is None else np.ascontiguousarray(p, np.float64)
         ev = list(events)
@@ -163,13 +173,14 @@ class Oracle:
         return y
 
     def adjoint(self, dt, nsteps, params, events=(), rows=None, scale=1.0, offset=0.0, z0=None, tol=1e-10,
-                itermax=50, p=None, fault_shift=0, trips=()):
+                itermax=50, p=None, fault_shift=0, trips=(), drops=()):
         """Objective scale * sum_k dt sum_r (z_k[r] - offset)^2 and its gradient
         w.r.t. p[params] by the oracle's discrete adjoint.  rows=None: the
         GENROU speed deviations.  fault_shift != 0 plants a misaligned fault
         state (tests only).  Returns (value, grad, dz0)."""
         from gradpower_hip.desc import gen_speed_indices
         self._set_trips(trips)
+        self._set_drops(drops)
         z0 = np.ascontiguousarray(self.desc.z0 if z0 is None else z0, np.float64)
         p = self.p if p is None else np.ascontiguousarray(p, np.float64)
         rows = np.ascontiguousarray(gen_speed_indices(self.desc) if rows is None else rows, np.int32)
@@ -190,11 +201,12 @@ class Oracle:
         return val.value, grad[:len(par)], dz0
 
     def objective(self, dt, nsteps, events=(), rows=None, scale=1.0, offset=0.0, z0=None, tol=1e-10,
-                  itermax=50, p=None, trips=()):
+                  itermax=50, p=None, trips=(), drops=()):
         """scale * sum_{k>=1} dt sum_r (z_k[r] - offset)^2 along the oracle's trajectory."""
         from gradpower_hip.desc import gen_speed_indices
         rows = gen_speed_indices(self.desc) if rows is None else np.asarray(rows)
-        tr, _, st = self.integrate(dt, nsteps, events, z0=z0, tol=tol, itermax=itermax, p=p, trips=trips)
+        tr, _, st = self.integrate(dt, nsteps, events, z0=z0, tol=tol, itermax=itermax, p=p, trips=trips,
+                                   drops=drops)
         if st["rc"]:
             raise RuntimeError(f"oracle forward sweep failed ({st['rc']})")
         e = tr[1:, rows] - offset

@@ -6,7 +6,9 @@ dt > 0 and dt = 0 for every method equals the oracle's; trajectories of
 scenarios mixing faults, trips and reclosures (a tapped, phase-shifting
 transformer included, several scenarios tripping the same line) stay within
 1e-9 pu of the oracle for every method, free-running and lock step; the
-discrete adjoint with trips matches the oracle's within 1e-7 relative.
+discrete adjoint with trips matches the oracle's within 1e-7 relative.  ZIP
+load disconnects (gp_set_load_drops) ride along in the same scenarios, and a
+disconnected load's residual and Jacobian equal a zero load's.
 
 The system: the bundled IEEE 39-bus case with governors, its transformer
 12-11 set to ratio 1.03 and a 2.5 degree shift; no trip islands a bus (a
@@ -95,29 +97,31 @@ def test_trip_residual_jacobian_newton_match_network_without_line(line):
 
 
 def _mixed(case):
-    """Per scenario: faults (bus, r, on, off) and trips (a_ext, b_ext, on, off)."""
-    return [([(15, 0.05, 6, 12)], [(16, 17, 30, 0)]),
-            ([], [(12, 11, 10, 40)]),
-            ([(3, 0.02, 6, 15)], [(16, 17, 15, 0), (26, 27, 20, 60)]),
-            ([(5, 0.1, 24, 24)], []),
-            ([], [(4, 14, 1, 1)]),                  # trip and reclose in one step
-            ([(8, 0.03, 3, 9)], [(12, 11, 9, 0)]),  # trip as the fault clears
-            ([], [(21, 22, 50, 0), (2, 3, 50, 0)]),
-            ([(3, 0.04, 5, 10)], [(26, 27, 70, 90)])]
+    """Per scenario: faults (bus, r, on, off), trips (a_ext, b_ext, on, off) and
+    ZIP load disconnects (ZIP row, step)."""
+    return [([(15, 0.05, 6, 12)], [(16, 17, 30, 0)], []),
+            ([], [(12, 11, 10, 40)], [(3, 20)]),
+            ([(3, 0.02, 6, 15)], [(16, 17, 15, 0), (26, 27, 20, 60)], []),
+            ([(5, 0.1, 24, 24)], [], [(0, 24), (7, 30)]),
+            ([], [(4, 14, 1, 1)], []),                  # trip and reclose in one step
+            ([(8, 0.03, 3, 9)], [(12, 11, 9, 0)], [(5, 9)]),   # trip and drop as the fault clears
+            ([], [(21, 22, 50, 0), (2, 3, 50, 0)], []),
+            ([(3, 0.04, 5, 10)], [(26, 27, 70, 90)], [(12, 60)])]
 
 
 def _tables(case, M):
     from gradpower_hip.system import create_trip_line_event
-    ev, tr, per = [], [], []
-    for m, (faults, trips) in enumerate(_mixed(case)[:M]):
+    ev, tr, dr, per = [], [], [], []
+    for m, (faults, trips, drops) in enumerate(_mixed(case)[:M]):
         ev += [(m,) + f for f in faults]
+        dr += [(m,) + d for d in drops]
         rows = []
         for (a, b, on, off) in trips:
             t = create_trip_line_event(case.ps, a, b, 0.0)
             tr.append((m, t.from_bus, t.to_bus, on, off, t.stamp()))
             rows.append((t.from_bus, t.to_bus, on, off, t.stamp()))
-        per.append((faults, rows))
-    return ev, tr, per
+        per.append((faults, rows, drops))
+    return ev, tr, dr, per
 
 
 @pytest.mark.parametrize("method,policy,lock", [("mono", 0, False), ("schur64", 0, False), ("schur64", 2, True),
@@ -127,15 +131,16 @@ def test_trajectories_with_faults_trips_reclosures(method, policy, lock):
     case = _case()
     M = 8
     nsteps = np.full(M, 120, np.int32)
-    ev, tr, per = _tables(case, M)
+    ev, tr, dr, per = _tables(case, M)
     ctx = _ctx(case, M, refactor_policy=policy, lock_step=lock)
     ctx.set_events(ev, nsteps)
     ctx.set_line_trips(tr)
+    ctx.set_load_drops(dr)
     st = ctx.integrate(METHODS[method], case.dt, TOL, 50)
     o = Oracle(case.desc)
     for m in range(M):
-        faults, rows = per[m]
-        ref, _, ost = o.integrate(case.dt, int(nsteps[m]), faults, tol=TOL, trips=rows)
+        faults, rows, drops = per[m]
+        ref, _, ost = o.integrate(case.dt, int(nsteps[m]), faults, tol=TOL, trips=rows, drops=drops)
         assert ost["not_converged"] == 0
         dev = np.max(np.abs(ctx.trajectory(m).T - ref))
         assert dev < 1e-9, (method, policy, lock, m, dev, st)
@@ -147,20 +152,55 @@ def test_adjoint_with_trips_matches_oracle():
     case = _case()
     M = 8
     nsteps = np.array([90, 120, 100, 60, 80, 120, 110, 100], np.int32)
-    ev, tr, per = _tables(case, M)
+    ev, tr, dr, per = _tables(case, M)
     d = case.desc.dev
     params = [int(r[2] + 6) for r in d[0]] + [int(r[2] + 7) for r in d[0]] + [int(r[2] + 6) for r in d[1]]
     ctx = _ctx(case, M)
     ctx.set_events(ev, nsteps)
     ctx.set_line_trips(tr)
+    ctx.set_load_drops(dr)
     r = ctx.adjoint(0, case.dt, params, tol=TOL, segment=25, scale=0.5, want_dz0=True)
     o = Oracle(case.desc)
     for m in range(M):
-        faults, rows = per[m]
-        val, g, dz0 = o.adjoint(case.dt, int(nsteps[m]), params, faults, tol=TOL, scale=0.5, trips=rows)
+        faults, rows, drops = per[m]
+        val, g, dz0 = o.adjoint(case.dt, int(nsteps[m]), params, faults, tol=TOL, scale=0.5, trips=rows,
+                                drops=drops)
         assert r["value_scen"][m] == pytest.approx(val, rel=1e-9, abs=1e-18), m
         assert np.max(np.abs(r["grad_scen"][m] - g)) <= 1e-7 * np.max(np.abs(g)) + 1e-15, m
         assert np.max(np.abs(r["dz0"][m] - dz0)) <= 1e-7 * np.max(np.abs(dz0)) + 1e-15, m
+    ctx.close()
+
+
+def test_load_drop_residual_jacobian_match_zero_load():
+    """A disconnected ZIP load leaves the bus rows as a load of zero power and
+    admittance does (residual and Jacobian, FP64)."""
+    import scipy.sparse as sp
+    from oracle.oracle import Oracle
+    case = _case()
+    M = 2
+    row = 4
+    ctx = _ctx(case, M)
+    ctx.set_events([], case.nsteps[:M])
+    ctx.set_load_drops([(m, row, 1) for m in range(M)])
+    rng = np.random.default_rng(5)
+    z = case.desc.z0[None, :] * (1 + 1e-3 * rng.standard_normal((M, case.desc.n)))
+    zold = np.tile(case.desc.z0, (M, 1))
+    on = np.array([1, 0], np.uint8)
+    o = Oracle(case.desc)
+    p0 = case.desc.p.copy()
+    q = int(case.desc.dev[6][row][1])
+    p0[[q, q + 1, q + 7, q + 8]] = 0.0
+    n = case.desc.n
+    f = ctx.eval_residual(z, zold, case.dt, on)
+    rp, ci, vals = ctx.eval_jacobian(z, case.dt, on)
+    for m in range(M):
+        pm = p0 if on[m] else case.desc.p
+        fo = o.residual(z[m], zold[m], case.dt, p=pm)
+        assert np.max(np.abs(f[m] - fo) / (1 + np.abs(fo))) < 1e-12, m
+        Jo = o.jacobian_csc(z[m], case.dt, p=pm).toarray()
+        Jg = sp.csr_matrix((vals[m], ci, rp), shape=(n, n)).toarray()
+        assert np.max(np.abs(Jg - Jo) / (1 + np.abs(Jo))) < 1e-12, m
+    assert np.max(np.abs(f[0] - f[1])) > 1e-3
     ctx.close()
 
 

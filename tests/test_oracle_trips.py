@@ -1,7 +1,7 @@
-"""CPU checks of line trips in the oracle (TripLineEvent, src/GradPower.jl:
-140-167,509-623; integrate!'s trip/reclose handling, src/dynamics.jl:536-540,
-616-624).  No reference test covers trips, so they are pinned by what the
-network algebra fixes:
+"""CPU checks of line trips and load disconnects in the oracle (TripLineEvent,
+src/GradPower.jl:140-167,509-623; integrate!'s trip/reclose handling,
+src/dynamics.jl:536-540,616-624).  No reference test covers them, so they are
+pinned by what the network algebra fixes:
 
   - the stamp of create_trip_line_event equals Y(all branches) - Y(without
     the branch) from the independent Y builder, for a line with charging and
@@ -12,7 +12,10 @@ network algebra fixes:
     the full network the same way;
   - the adjoint with trips (and reclosures) agrees with central differences
     of the objective, and a planted one-step misalignment of the event state
-    breaks that agreement.
+    breaks that agreement;
+  - a ZIP load disconnected at step k0 (DisconnectDeviceEvent, src/GradPower.jl:
+    130-143,681-698) continues as the sweep whose load has zero power and
+    admittance; the adjoint with disconnects agrees with central differences.
 
 The system: the bundled IEEE 9-bus case (ieee9_v33.raw + ieee9bus_SEXS.dyr)
 with its first transformer at ratio 1.05 and a 2.5 degree phase shift.
@@ -138,4 +141,60 @@ def test_adjoint_with_trips_vs_central_differences():
         assert abs(g[j] - fd) <= 1e-5 * abs(fd), (q, g[j], fd)
     # the event state one step off (faults and trips alike) breaks it
     _, gs, _ = o.adjoint(dt, n, params, ev, rows=rows, tol=1e-12, trips=trips, fault_shift=1)
+    assert np.max(np.abs(gs - g) / np.abs(g)) > 1e-3
+
+
+def _zip_rows(case):
+    """Parameter offset of each ZIP table row."""
+    return [int(r[1]) for r in case.desc.dev[6]]
+
+
+def test_load_drop_continues_as_a_zero_load():
+    """A ZIP load disconnected at step k0 (DisconnectDeviceEvent) continues as
+    the sweep whose load has zero power and admittance, from the state after
+    the dt = 0 re-solve; the helper maps a device index to its ZIP row."""
+    from gradpower_hip.system import DisconnectDeviceEvent
+    case = _case()
+    dt, k0, n = case.dt, 25, 80
+    devs = case.ps.dynamic.devices
+    di = [i for i, d in enumerate(devs) if d.dtype.kind == "zipload"][1]
+    row = DisconnectDeviceEvent(di, k0 * dt).zip_row(case.ps)
+    assert row == 1
+    with pytest.raises(NotImplementedError):
+        DisconnectDeviceEvent(0, 0.1).zip_row(case.ps)
+    o = _oracle(case.desc)
+    ev = [(7, 0.05, 6, 12)]
+    tr, _, st = o.integrate(dt, n, ev, tol=1e-12, drops=[(row, k0)])
+    assert st["not_converged"] == 0
+    _, z_k0, _ = o.integrate(dt, k0, ev, tol=1e-12, drops=[(row, k0)], traj=False)
+    p0 = case.desc.p.copy()
+    q = _zip_rows(case)[row]
+    p0[[q, q + 1, q + 7, q + 8]] = 0.0          # pinj, qinj, yreal, yimag
+    trb, _, _ = o.integrate(dt, n - k0, (), z0=z_k0, tol=1e-12, p=p0)
+    assert np.max(np.abs(tr[k0 + 1:] - trb[1:])) < 1e-10
+    tr0, _, _ = o.integrate(dt, n, ev, tol=1e-12)
+    assert np.array_equal(tr[:k0 + 1], tr0[:k0 + 1])
+    assert np.max(np.abs(tr[-1] - tr0[-1])) > 1e-3
+
+
+def test_adjoint_with_load_drop_vs_central_differences():
+    case = _case()
+    dt, n = case.dt, 70
+    ev = [(6, 0.08, 5, 11)]
+    _, r1 = _trip(case, 4, 5, 30)
+    drops = [(0, 18), (2, 40)]
+    o = _oracle(case.desc)
+    d = case.desc.dev
+    params = [int(d[0][0][2] + 6), int(d[0][2][2] + 7), int(d[3][1][1] + 2)]
+    rows = _speed_rows(case)
+    kw = dict(rows=rows, tol=1e-12, trips=[r1], drops=drops)
+    val, g, _ = o.adjoint(dt, n, params, ev, **kw)
+    for j, q in enumerate(params):
+        h = max(1e-6 * abs(o.p[q]), 1e-6)
+        pp, pm = o.p.copy(), o.p.copy()
+        pp[q] += h
+        pm[q] -= h
+        fd = (o.objective(dt, n, ev, p=pp, **kw) - o.objective(dt, n, ev, p=pm, **kw)) / (2 * h)
+        assert abs(g[j] - fd) <= 1e-5 * abs(fd), (q, g[j], fd)
+    _, gs, _ = o.adjoint(dt, n, params, ev, fault_shift=1, **kw)
     assert np.max(np.abs(gs - g) / np.abs(g)) > 1e-3
