@@ -186,3 +186,57 @@ def test_esdc1a_step_response_and_saturation_points():
     den0 = np.polyadd(_poly([Ta, 1.0], [Tf, 1.0], [Te, Ke]), [Ka * Kf, 0.0])
     with pytest.raises(AssertionError):   # the saturation slope is in the response
         _check(y, _tf_response(num, den0), rel=1e-6)
+
+
+def _ieee9_slack_stub(tmp_path):
+    """The IEEE 9-bus case with machine models on buses 2 and 3 only: bus 1
+    (the slack) gets a static generator stub."""
+    import os
+    g = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "psse")
+    recs = open(os.path.join(g, "ieee9bus.dyr")).read().split("/")
+    keep = [r for r in recs if r.strip() and not r.strip().startswith("1 ")]
+    dyr = tmp_path / "ieee9_slack_stub.dyr"
+    dyr.write_text("/\n".join(keep) + "/\n")
+    return get_case("case_psse", raw=os.path.join(g, "ieee9_v33.raw"), dyr=str(dyr), M=2)
+
+
+@pytest.mark.parametrize("which", ["grid_pv", "ieee9_slack"])
+def test_static_generator_injects_its_power(which, tmp_path):
+    """The static generator stub (src/kernels/static_gen.jl): its current in
+    the bus rows satisfies V conj(I) = P + jQ (P, Q its parameters or its
+    states: Q a state on a PV bus, P and Q on the slack bus), and its own rows
+    hold |V| = Vset (PV) or V = Vset e^{j angle} (slack)."""
+    from oracle.oracle import Oracle
+    case = _case() if which == "grid_pv" else _ieee9_slack_stub(tmp_path)
+    o = Oracle(case.desc)
+    npt = case.desc.n - 2 * case.desc.nbus
+    rng = np.random.default_rng(9)
+    z = case.desc.z0 * (1 + 1e-2 * rng.standard_normal(case.desc.n))
+    kinds = set()
+    for b, ap, pp, bt in case.desc.dev[7]:
+        p = case.desc.p
+        vri = npt + 2 * b
+        V = complex(z[vri], z[vri + 1])
+        f1 = o.residual(z, z, DT)
+        z0, p0 = z.copy(), p.copy()
+        if bt == 2:
+            P, Q = p[pp], z[ap]
+            p0[pp] = 0.0
+            z0[ap] = 0.0
+        elif bt == 3:
+            P, Q = z[ap], z[ap + 1]
+            z0[ap] = z0[ap + 1] = 0.0
+        else:
+            P, Q = p[pp], p[pp + 1]
+            p0[pp] = p0[pp + 1] = 0.0
+        f0 = o.residual(z0, z0, DT, p=p0)
+        I = complex(f1[vri] - f0[vri], f1[vri + 1] - f0[vri + 1])
+        S = V * I.conjugate()
+        assert abs(S - complex(P, Q)) <= 1e-13 * max(1.0, abs(complex(P, Q))), (b, bt)
+        if bt == 2:
+            assert f1[ap] == pytest.approx(abs(V) ** 2 - p[pp + 2] ** 2, abs=1e-15)
+        elif bt == 3:
+            assert f1[ap] == pytest.approx(V.real - p[pp + 2] * np.cos(p[pp + 3]), abs=1e-15)
+            assert f1[ap + 1] == pytest.approx(V.imag - p[pp + 2] * np.sin(p[pp + 3]), abs=1e-15)
+        kinds.add(int(bt))
+    assert (2 if which == "grid_pv" else 3) in kinds
