@@ -98,3 +98,28 @@ def test_psse_ieee39_adjoint_matches_oracle():
         assert np.max(np.abs(r["grad_scen"][m] - g)) <= 1e-7 * np.max(np.abs(g)), m
         assert np.max(np.abs(r["dz0"][m] - dz0)) <= 1e-7 * np.max(np.abs(dz0)), m
     ctx.close()
+
+
+@pytest.mark.parametrize("method,policy", [("mono", 0), ("schur64", 0), ("schur32", 3)])
+def test_slack_static_generator_trajectories_match_oracle(method, policy, tmp_path):
+    """The IEEE 9-bus case with machine models on buses 2 and 3 only: the
+    slack bus is held by a static generator stub (its P and Q are states)."""
+    from gradpower_hip.lib import HipContext
+    from oracle.oracle import Oracle
+    recs = open(os.path.join(G, "ieee9bus_SEXS.dyr")).read().split("/")
+    keep = [r for r in recs if r.strip() and not r.strip().startswith("1 ")]
+    dyr = tmp_path / "ieee9_slack_stub.dyr"
+    dyr.write_text("/\n".join(keep) + "/\n")
+    case = get_case("case_psse", raw=os.path.join(G, "ieee9_v33.raw"), dyr=str(dyr), M=8)
+    assert [int(r[3]) for r in case.desc.dev[7]] == [3]
+    ctx = HipContext(case.desc, case.M, refactor_policy=policy)
+    ctx.set_state(case.desc.z0, case.desc.p)
+    ctx.set_events([(m,) + e for m in range(case.M) for e in _events(case, m)], case.nsteps)
+    st = ctx.integrate(METHODS[method], case.dt, TOL, 50)
+    o = Oracle(case.desc)
+    for m in range(case.M):
+        ref, _, ost = o.integrate(case.dt, int(case.nsteps[m]), _events(case, m), tol=TOL, itermax=50)
+        assert ost["not_converged"] == 0
+        dev = np.max(np.abs(ctx.trajectory(m).T - ref))
+        assert dev < 1e-9, (method, m, dev, st)
+    ctx.close()
