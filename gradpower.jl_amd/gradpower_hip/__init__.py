@@ -7,7 +7,7 @@ gfx950), loaded through `gradpower_hip.lib`.
 """
 from .system import (Bus, Gen, Load, Branch, Shunt, ContingencyEvent, PowerSystem,
                      PowerSystemDynamics, DynamicProblem, TripLineEvent, create_trip_line_event,
-                     DisconnectDeviceEvent,
+                     DisconnectDeviceEvent, add_trip_event, add_disconnect_event,
                      Genrou, Gensal, IEESGO, TGOV1,
                      SEXS, ESDC1A, IEEEST, ZIPLoad, StaticGenerator, add_device,
                      add_event, set_dynamics, esdc1a_sat_coefficients, PQ, PV, SLACK)

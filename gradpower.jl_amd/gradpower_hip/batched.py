@@ -44,8 +44,18 @@ def _step(t, dt):
     return int(round(t / dt))
 
 
-def _run(gbl, method, events, nsteps, dt, newton_tol, itermax, raise_on_fail):
+def _run(gbl, method, events, nsteps, dt, newton_tol, itermax, raise_on_fail, ps=None):
     gbl.ctx.set_events(events, nsteps)
+    if ps is not None:
+        # line trips and load disconnects of ps.dynamic, in every scenario (the
+        # reference runs them in its CPU integrate!, src/dynamics.jl:533-540)
+        trips = [(m, t.from_bus, t.to_bus) + t.steps(dt) + (t.stamp(),)
+                 for m in range(gbl.M) for t in ps.dynamic.trip_events]
+        if trips:
+            gbl.ctx.set_line_trips(trips)
+        drops = [(m, d.zip_row(ps), _step(d.ton, dt)) for m in range(gbl.M) for d in ps.dynamic.disconnect_events]
+        if drops:
+            gbl.ctx.set_load_drops(drops)
     gbl.last_stats = gbl.ctx.integrate(method, dt, newton_tol, itermax, raise_on_fail=raise_on_fail)
     trajs = [gbl.ctx.trajectory(m) for m in range(gbl.M)]
     return trajs
@@ -64,7 +74,7 @@ def integrate_hip(gbl, ps, tf, dt=1.0 / 120.0, newton_tol=1e-10, itermax=30, rai
     nsteps = _step(tf, dt)
     tvec = np.arange(nsteps + 1) * dt
     trajs = _run(gbl, METHOD_MONO_FP64, _shared_events(ps, gbl.M, dt), [nsteps] * gbl.M, dt,
-                 newton_tol, itermax, raise_on_fail)
+                 newton_tol, itermax, raise_on_fail, ps)
     return tvec, trajs
 
 
@@ -75,7 +85,7 @@ def integrate_hip_schur(gbl, ps, tf, dt=1.0 / 120.0, newton_tol=1e-10, itermax=3
     nsteps = _step(tf, dt)
     tvec = np.arange(nsteps + 1) * dt
     trajs = _run(gbl, METHOD_SCHUR_FP32 if fp32 else METHOD_SCHUR_FP64, _shared_events(ps, gbl.M, dt),
-                 [nsteps] * gbl.M, dt, newton_tol, itermax, raise_on_fail)
+                 [nsteps] * gbl.M, dt, newton_tol, itermax, raise_on_fail, ps)
     return tvec, trajs
 
 
@@ -90,6 +100,6 @@ def integrate_hip_multi(gbl, ps, tf, fault_buses, rfaults, ton, toff, dt=1.0 / 1
     ev = [(m, int(fault_buses[m]), float(rfaults[m]), _step(tonv[m], dt), _step(toffv[m], dt))
           for m in range(M)]
     nsteps = [_step(t, dt) for t in tfv]
-    trajs = _run(gbl, method, ev, nsteps, dt, newton_tol, itermax, raise_on_fail)
+    trajs = _run(gbl, method, ev, nsteps, dt, newton_tol, itermax, raise_on_fail, ps)
     tvecs = [np.arange(ns + 1) * dt for ns in nsteps]
     return tvecs, trajs

@@ -446,6 +446,8 @@ class PowerSystemDynamics:
         self.par_dim = 0
         self.uvec_idx: Optional[np.ndarray] = None
         self.events: List[ContingencyEvent] = []
+        self.trip_events: list = []          # TripLineEvent
+        self.disconnect_events: list = []    # DisconnectDeviceEvent
         self.clusters: List[Cluster] = []
         self.is_diff: Optional[np.ndarray] = None
         self.dev_bus: List[int] = []     # internal bus of each device
@@ -489,6 +491,18 @@ class PowerSystem:
 
 def add_event(ps: PowerSystem, event: ContingencyEvent):
     ps.dynamic.events.append(event)
+    return event
+
+
+def add_trip_event(ps: PowerSystem, event):
+    """add_trip_event! (src/GradPower.jl:489-496)."""
+    ps.dynamic.trip_events.append(event)
+    return event
+
+
+def add_disconnect_event(ps: PowerSystem, event):
+    """add_disconnect_event! (src/GradPower.jl:477-484)."""
+    ps.dynamic.disconnect_events.append(event)
     return event
 
 

@@ -213,3 +213,29 @@ def test_trip_without_a_line_is_refused():
         ctx.set_line_trips([(0, 0, 5, 5, 0, [0.0, 1.0] * 4)])   # buses 1 and 6 share no branch
     assert e.value.code == GP_ERR_BADARG
     ctx.close()
+
+
+def test_front_end_takes_trips_and_disconnects_from_the_system():
+    """integrate_hip_schur (the HipBatchedLayout front end) passes the
+    system's line trips and load disconnects (add_trip_event,
+    add_disconnect_event, src/GradPower.jl:477-496) to every scenario."""
+    from gradpower_hip import synth
+    from gradpower_hip.batched import HipBatchedLayout, integrate_hip_schur
+    from gradpower_hip.system import (ContingencyEvent, DisconnectDeviceEvent, add_disconnect_event,
+                                      add_event, add_trip_event, create_trip_line_event)
+    from oracle.oracle import Oracle
+    case = synth.case_psse(os.path.join(G, "IEEE39.raw"), os.path.join(G, "IEEE39_gov.dyr"), M=2)
+    ps = case.ps
+    dt = case.dt
+    add_event(ps, ContingencyEvent(20, 0.05, 0.05, 0.15))
+    t = add_trip_event(ps, create_trip_line_event(ps, 16, 17, 0.25))
+    zi = [i for i, d in enumerate(ps.dynamic.devices) if d.dtype.kind == "zipload"][2]
+    dd = add_disconnect_event(ps, DisconnectDeviceEvent(zi, 0.4))
+    hbl = HipBatchedLayout(case.dp, ps, 2)
+    tvec, trajs = integrate_hip_schur(hbl, ps, 0.6, dt=dt, newton_tol=TOL)
+    ref, _, st = Oracle(case.desc).integrate(dt, 72, [(20, 0.05, 6, 18)], tol=TOL,
+                                             trips=[(t.from_bus, t.to_bus) + t.steps(dt) + (t.stamp(),)],
+                                             drops=[(dd.zip_row(ps), 48)])
+    assert st["not_converged"] == 0
+    for m in range(2):
+        assert np.max(np.abs(trajs[m].T - ref)) < 1e-9
