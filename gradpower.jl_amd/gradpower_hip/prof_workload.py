@@ -73,8 +73,9 @@ def main():
 
 
 def adjoint(passes):
+    """The bench's adjoint: Schur forward with FP32 factors, FP64 transposed solves."""
     from gradpower_hip import synth
-    from gradpower_hip.lib import HipContext, METHOD_MONO_FP64
+    from gradpower_hip.lib import HipContext, METHOD_SCHUR_FP32
     case = synth.case3_200bus()
     d = case.desc.dev
     params = [int(r[2] + 6) for r in d[0]] + [int(r[2] + 7) for r in d[0]] + [int(r[1] + 2) for r in d[3]]
@@ -84,7 +85,7 @@ def adjoint(passes):
     for it in range(passes):
         ctx.set_state(case.desc.z0, case.desc.p)
         ctx.set_events(ev, case.nsteps)
-        r = ctx.adjoint(METHOD_MONO_FP64, case.dt, params, tol=1e-12, itermax=50, segment=30,
+        r = ctx.adjoint(METHOD_SCHUR_FP32, case.dt, params, tol=1e-12, itermax=50, segment=30,
                         scale=1.0 / len(d[0]), raise_on_fail=False)
         a = r["stats"]
         print(f"adjoint pass {it}: {a['elapsed_ms']:.1f} ms (forward {a['forward_ms']:.1f}, "
