@@ -11,11 +11,9 @@ load disconnects (gp_set_load_drops) ride along in the same scenarios, and a
 disconnected load's residual and Jacobian equal a zero load's.
 
 The system: the bundled IEEE 39-bus case with governors, its transformer
-12-11 set to ratio 1.03 and a 2.5 degree shift; no trip islands a bus (a
-generator bus fed by one transformer would lose its only branch, which a
-static pivot order -- the reference's KLU refactorisation as much as the
-batched LU here -- cannot follow).  Newton tolerance 1e-11 (the 39-bus
-admittances reach 650 pu).
+12-11 set to ratio 1.03 and a 2.5 degree shift.  Newton tolerance 1e-11
+(the 39-bus admittances reach 650 pu).  A trip that leaves a generator bus
+with no branch is checked on the 9-bus case.
 """
 import copy
 import os
@@ -239,3 +237,39 @@ def test_front_end_takes_trips_and_disconnects_from_the_system():
     assert st["not_converged"] == 0
     for m in range(2):
         assert np.max(np.abs(trajs[m].T - ref)) < 1e-9
+
+
+@pytest.mark.parametrize("method", ["mono", "schur64"])
+def test_trip_that_islands_a_generator_bus(method):
+    """Tripping the only transformer of the 9-bus case's slack machine leaves
+    that bus with no branch.  The oracle re-pivots every factorisation and
+    follows.  The library's fixed pivot order, chosen on the intact matrix and
+    on the matrix with the trip applied, follows too (observed: all 60 steps
+    within 1e-9, MONO and SCHUR64); should a pivot order fail, the scenario
+    must stop with an error status after the trip, its columns up to there
+    matching the oracle.  The other scenario is untouched."""
+    from gradpower_hip.system import create_trip_line_event
+    from oracle.oracle import Oracle
+    case = get_case("case_psse", raw=os.path.join(G, "ieee9_v33.raw"), dyr=os.path.join(G, "ieee9bus_SEXS.dyr"),
+                    M=2, xf_tap=(0, 1.05, 2.5))
+    t = create_trip_line_event(case.ps, 1, 4, 0.0)
+    row = (t.from_bus, t.to_bus, 30, 0, t.stamp())
+    nsteps = np.array([60, 60], np.int32)
+    ctx = _ctx(case, 2)
+    ctx.set_events([], nsteps)
+    ctx.set_line_trips([(0,) + row])
+    ctx.integrate(METHODS[method], case.dt, 1e-12, 50, raise_on_fail=False)
+    status, done = ctx.scenario_status()
+    o = Oracle(case.desc)
+    ref0, _, st0 = o.integrate(case.dt, 60, (), tol=1e-12, trips=[row])
+    ref1, _, _ = o.integrate(case.dt, 60, (), tol=1e-12)
+    assert st0["not_converged"] == 0
+    assert status[1] == 0 and np.max(np.abs(ctx.trajectory(1).T - ref1)) < 1e-9
+    k = int(done[0])
+    if status[0] == 0:
+        assert k == 60
+    else:
+        assert k >= 29          # the trip comes at the end of step 30
+    assert np.max(np.abs(ctx.trajectory(0).T[:k + 1] - ref0[:k + 1])) < 1e-9
+    print(f"{method}: status {int(status[0])}, steps done {k}", flush=True)
+    ctx.close()
