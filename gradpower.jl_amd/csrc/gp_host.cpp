@@ -103,8 +103,8 @@ struct HBuf {
 // device memory and its pinned host mirror, laid out so that one copy
 // uploads what a round's evaluation reads and one copy brings back what it
 // decided (L = padded scenario count, a multiple of 64):
-//   [first L][refac0 L][dt 8L][work L][counts 16][failed L][areq L]
-//   upload   [0, 11L+16)      download [10L, 13L+16)
+//   [first L][refac0 L][exact L][dt 8L][work L][counts 16][failed L][areq L]
+//   upload   [0, 12L+16)      download [11L, 14L+16)
 // and a second block for the accepted steps:
 //   [acc L][adv L][col 4L][sv L][svslot 4L][ck L][ckslot 4L].
 struct RoundLayout {
@@ -112,14 +112,15 @@ struct RoundLayout {
   explicit RoundLayout(size_t l) : L(l) {}
   size_t first() const { return 0; }
   size_t refac0() const { return L; }
-  size_t dt() const { return 2 * L; }
-  size_t work() const { return 10 * L; }
-  size_t cnt() const { return 11 * L; }
-  size_t failed() const { return 11 * L + 16; }
-  size_t areq() const { return 12 * L + 16; }
-  size_t bytes() const { return 13 * L + 16; }
-  size_t up_bytes() const { return 11 * L + 16; }
-  size_t down_off() const { return 10 * L; }
+  size_t exact() const { return 2 * L; }
+  size_t dt() const { return 3 * L; }
+  size_t work() const { return 11 * L; }
+  size_t cnt() const { return 12 * L; }
+  size_t failed() const { return 12 * L + 16; }
+  size_t areq() const { return 13 * L + 16; }
+  size_t bytes() const { return 14 * L + 16; }
+  size_t up_bytes() const { return 12 * L + 16; }
+  size_t down_off() const { return 11 * L; }
   size_t down_bytes() const { return 3 * L + 16; }
   size_t acc() const { return 0; }
   size_t adv() const { return L; }
@@ -443,6 +444,9 @@ struct gp_ctx {
   DBuf<unsigned long long> d_nb;
   DevTables T{};
   LUPlan luJ, luS;
+  SpluDev luS64{};   // luS shaped for FP64 factors (the FP64 methods' and the FP32 method's exact retries)
+  bool has64 = false;   // luS64 fits the LU kernels' LDS
+  DBuf<uint8_t> d_split;   // a round's masks split by factor precision [6][L]
   // discrete adjoint: LU plan of J^T (values read from d_jv), its factors,
   // the initial state of the last gp_set_state, checkpoints, the states of
   // one segment, multipliers, parameter pullbacks and objective per scenario
@@ -454,7 +458,7 @@ struct gp_ctx {
   // scenario-asynchronous time loop: per-scenario step, first-round flags,
   // record columns, accepted / advancing masks
   DBuf<double> d_dtv;
-  DBuf<uint8_t> d_first, d_acc, d_adv, d_sv, d_ck;
+  DBuf<uint8_t> d_first, d_acc, d_adv, d_sv, d_ck, d_exact;
   DBuf<int> d_col, d_svslot, d_ckslot;
   bool state_set = false;
   int hist_steps = 0;
@@ -822,6 +826,7 @@ void alloc_state(gp_ctx *C) {
   C->d_Dg.alloc((size_t)std::max(1, 6 * C->ncl) * L);
   C->d_fnorm.alloc(L);
   C->d_act.alloc(L); C->d_refac.alloc(L); C->d_sel.alloc(L);
+  C->d_split.alloc(6 * L);
   {
     const RoundLayout R(L);
     C->d_round.alloc(R.bytes());
@@ -832,6 +837,7 @@ void alloc_state(gp_ctx *C) {
     uint8_t *b = C->d_round.p, *q = C->d_post.p;
     C->d_first.view(b + R.first(), L);
     C->d_refac0.view(b + R.refac0(), L);
+    C->d_exact.view(b + R.exact(), L);
     C->d_dtv.view(b + R.dt(), L);
     C->d_work.view(b + R.work(), L);
     C->d_nunconv.view(b + R.cnt(), 4);
@@ -996,8 +1002,24 @@ void analyse(gp_ctx *C, int method) {
   }
 }
 
+// Factor buffers of the Schur methods.  The FP64 ones exist for both: the
+// FP32 method runs a solve that failed again with FP64 factors.
 void ensure_schur_buffers(gp_ctx *C, bool fp32) {
   const size_t L = C->ldM;
+  C->d_luS64.ensure(C->luS.lu_elems() * C->M);
+  if (C->d_svS64.n != (size_t)2 * C->luS.dev.ns * C->M) {   // two buffers per scenario
+    C->d_svS64.alloc((size_t)2 * C->luS.dev.ns * C->M);
+    HIPCHK(hipMemset(C->d_svS64.p, 0, C->d_svS64.n * sizeof(double)));   // padding slots stay zero
+  }
+  C->d_dnS64.ensure((size_t)2 * std::max(1, C->luS.A.nd * C->luS.A.nd) * C->M);
+  try {
+    C->luS.shape(sizeof(double));
+    C->luS64 = C->luS.dev;
+    C->has64 = true;
+  } catch (const GpError &) {   // the FP64 solve's LDS does not fit: FP32 only, no FP64 retries
+    C->has64 = false;
+    if (!fp32) throw;
+  }
   if (fp32) {
     C->d_X32.ensure((size_t)std::max(1, C->xrows) * 3 * L);
     C->d_luS32.ensure(C->luS.lu_elems() * C->M);
@@ -1007,14 +1029,6 @@ void ensure_schur_buffers(gp_ctx *C, bool fp32) {
     }
     C->d_dnS32.ensure((size_t)2 * std::max(1, C->luS.A.nd * C->luS.A.nd) * C->M);
     C->luS.shape(sizeof(float));
-  } else {
-    C->d_luS64.ensure(C->luS.lu_elems() * C->M);
-    if (C->d_svS64.n != (size_t)2 * C->luS.dev.ns * C->M) {   // two buffers per scenario
-      C->d_svS64.alloc((size_t)2 * C->luS.dev.ns * C->M);
-      HIPCHK(hipMemset(C->d_svS64.p, 0, C->d_svS64.n * sizeof(double)));   // padding slots stay zero
-    }
-    C->d_dnS64.ensure((size_t)2 * std::max(1, C->luS.A.nd * C->luS.A.nd) * C->M);
-    C->luS.shape(sizeof(double));
   }
 }
 
@@ -1058,6 +1072,7 @@ struct NewtonOut {
   int64_t corrections = 0;
   int64_t refac = 0;
   int64_t rounds = 0, factor_launches = 0;
+  int64_t exact_retries = 0;     // solves run again as exact Newton
   std::vector<uint8_t> failed;   // scenarios that did not converge (empty: none)
 };
 
@@ -1113,7 +1128,7 @@ void start_batch(gp_ctx *C, int l, TF *luv, TF *sv, TF *dn) {
   const int L = C->ldM, nb = ln.nb;
   HIPCHK(hipStreamWaitEvent(ln.stream, ln.ev_snap, 0));
   const int pi = prof_begin(C, C->prof_factor, ln.stream);
-  launch_splu_factor_impl<TF>(C->luS.dev, ln.snap.p, C->d_s_cst.p, L, luv, sv, dn, ln.req, ln.tgt, C->M,
+  launch_splu_factor_impl<TF>(sizeof(TF) == 8 ? C->luS64 : C->luS.dev, ln.snap.p, C->d_s_cst.p, L, luv, sv, dn, ln.req, ln.tgt, C->M,
                               C->d_status.p, ln.stream);
   prof_end(C->prof_factor, pi, ln.stream, nb);
   HIPCHK(hipGetLastError());
@@ -1161,7 +1176,11 @@ struct Newton {
   int method, itermax, stall;
   double dt, tol, theta, async_below;
   bool schur, reuse, async, fp32;
+  bool lagged;                   // a factorisation may serve several iterations
+  bool all64 = false;            // FP32 method running exact Newton: every factor in FP64
   const double *dtv = nullptr;   // per-scenario step on the device (NULL: dt for all)
+  const uint8_t *exactv = nullptr;   // per-scenario exact-Newton flags on the device (NULL: none)
+  const uint8_t *exacth = nullptr;   // the same flags on the host
   std::vector<uint8_t> req;
   const uint8_t *reqp;           // host copy of this round's asynchronous requests
   NewtonOut out;
@@ -1174,17 +1193,29 @@ struct Newton {
     async = schur && pol == GP_REFACTOR_ASYNC;
     stall = C->opt.stall_window;
     // refactor policy: theta <= 0 refactors every iteration; otherwise after
-    // the first iteration only when |F| fell by less than theta
+    // the first iteration only when |F| fell by less than theta (FIRST_ITER:
+    // a factor 1e-300, which every iteration reaches, so never)
     theta = (!schur || pol == GP_REFACTOR_EVERY_ITER)
                 ? 0.0
                 : ((pol == GP_REFACTOR_ADAPTIVE || pol == GP_REFACTOR_ASYNC)
                        ? (C->opt.refactor_theta > 0.0 ? C->opt.refactor_theta : 30.0)
-                       : 1e300);
+                       : 1e-300);
+    lagged = theta > 0.0;
     fp32 = method == GP_METHOD_SCHUR_FP32;
     async_below = C->opt.async_below > 0.0 ? C->opt.async_below : 1e300;
     req.assign(C->ldM, 0);
     reqp = req.data();
   }
+  // exact Newton for every scenario of this engine: a synchronous FP64
+  // factorisation at every iteration
+  void make_exact() {
+    theta = 0.0;
+    reuse = async = false;
+    all64 = fp32 && C->has64;
+  }
+  // A failed solve runs again as exact Newton: when factorisations are
+  // reused, or are FP32 (the reference factors in FP64 at every iteration).
+  bool retryable() const { return lagged || (fp32 && C->has64); }
 
   // first: every working scenario starts its solve (lock step); firstv: the
   // scenarios that do (device flags).  cnt[] as k_norm_finish's n_unconv,
@@ -1199,7 +1230,7 @@ struct Newton {
     launch_rownorm_impl(C->n, C->M, L, C->d_f.p, C->d_work.p, tol, C->d_fnorm.p, C->d_nb.p, C->d_nunconv.p,
                         C->d_refac.p, reuse ? refac0 : nullptr, first, theta, stall, itermax, C->d_fbest.p,
                         C->d_nstall.p, C->d_iters.p, C->d_failed.p, async ? C->d_pend.p : nullptr, C->d_upd.p,
-                        async ? async_below : 0.0, C->d_areq.p, C->d_sch.p, C->stream, firstv);
+                        async ? async_below : 0.0, C->d_areq.p, C->d_sch.p, C->stream, firstv, exactv);
     if (!cnt) return;
     HIPCHK(hipMemcpyAsync(cnt, C->d_nunconv.p, 4 * sizeof(int), hipMemcpyDeviceToHost, C->stream));
     if (async && read_req) HIPCHK(hipMemcpyAsync(req.data(), C->d_areq.p, L, hipMemcpyDeviceToHost, C->stream));
@@ -1266,10 +1297,25 @@ struct Newton {
       HIPCHK(hipGetLastError());
       return true;
     }
-    if (fp32)
-      launch_cluster_schur<float>(C->T, p, z, C->d_f.p, dt, C->d_X32.p, C->d_Dg.p, C->d_sch.p, C->stream, dtv);
-    else
-      launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, C->d_sch.p, C->stream, dtv);
+    // factor precision per scenario: FP32 for the FP32 method, except the
+    // scenarios running exact Newton (exacth), whose factors are FP64
+    const bool use32 = fp32 && !all64;
+    bool mixed = false;
+    if (use32 && exacth && C->has64)
+      for (int m = 0; m < L && !mixed; m++) mixed = exacth[m] != 0;
+    const uint8_t *sch32 = C->d_sch.p, *ref32 = C->d_refac.p, *upd32 = C->d_upd.p;
+    const uint8_t *sch64 = sch32, *ref64 = ref32, *upd64 = upd32;
+    if (mixed) {
+      launch_split_precision(L, exactv, C->d_sch.p, C->d_refac.p, C->d_upd.p, C->d_split.p, C->stream);
+      const uint8_t *b = C->d_split.p;
+      sch32 = b; ref32 = b + L; upd32 = b + 2 * L;
+      sch64 = b + 3 * L; ref64 = b + 4 * L; upd64 = b + 5 * L;
+    }
+    const bool run32 = use32, run64 = !use32 || mixed;
+    if (run32)
+      launch_cluster_schur<float>(C->T, p, z, C->d_f.p, dt, C->d_X32.p, C->d_Dg.p, sch32, C->stream, dtv);
+    if (run64)
+      launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, sch64, C->stream, dtv);
     launch_bus_schur(C->T, p, z, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_sch.p, C->stream);
     if (cnt[1] > 0) {
       // synchronous factorisation into the buffer the solves read.  Later
@@ -1288,12 +1334,12 @@ struct Newton {
       out.refac += cnt[1];
       out.factor_launches++;
       const int pi = prof_begin(C, C->prof_factor, C->stream);
-      if (fp32)
+      if (run32)
         launch_splu_factor_impl<float>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS32.p, C->d_svS32.p,
-                                       C->d_dnS32.p, C->d_refac.p, C->d_cur.p, C->M, C->d_status.p, C->stream);
-      else
-        launch_splu_factor_impl<double>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
-                                        C->d_dnS64.p, C->d_refac.p, C->d_cur.p, C->M, C->d_status.p, C->stream);
+                                       C->d_dnS32.p, ref32, C->d_cur.p, C->M, C->d_status.p, C->stream);
+      if (run64)
+        launch_splu_factor_impl<double>(C->luS64, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
+                                        C->d_dnS64.p, ref64, C->d_cur.p, C->M, C->d_status.p, C->stream);
       prof_end(C->prof_factor, pi, C->stream, cnt[1]);
     }
     if (async && cnt[3] > 0) {
@@ -1305,30 +1351,47 @@ struct Newton {
     }
     const int lane = (async && free_lane(C) >= 0 && queued()) ? stage_queue() : -1;
     const int ps = cnt[2] > 0 ? prof_begin(C, C->prof_solve, C->stream) : -1;
-    if (fp32)
+    if (run32)
       launch_splu_solve<float>(C->luS.dev, C->d_svS32.p, C->d_dnS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
-                               C->d_dv.p, C->d_upd.p, C->d_cur.p, C->M, C->stream);
-    else
-      launch_splu_solve<double>(C->luS.dev, C->d_svS64.p, C->d_dnS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
-                                C->d_dv.p, C->d_upd.p, C->d_cur.p, C->M, C->stream);
+                               C->d_dv.p, upd32, C->d_cur.p, C->M, C->stream);
+    if (run64)
+      launch_splu_solve<double>(C->luS64, C->d_svS64.p, C->d_dnS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
+                                C->d_dv.p, upd64, C->d_cur.p, C->M, C->stream);
     prof_end(C->prof_solve, ps, C->stream, cnt[2]);
-    if (fp32)
-      launch_cluster_backsub<float>(C->T, C->d_X32.p, C->d_f.p, C->d_dv.p, z, C->d_upd.p, C->stream);
-    else
-      launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, C->d_upd.p, C->stream);
+    if (run32)
+      launch_cluster_backsub<float>(C->T, C->d_X32.p, C->d_f.p, C->d_dv.p, z, upd32, C->stream);
+    if (run64)
+      launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, upd64, C->stream);
     start_queue(lane);
     HIPCHK(hipGetLastError());
     return true;
   }
 };
 
+// Drops the asynchronous refactorisations asked for by the scenarios in
+// `sel` (a batch in flight lands first) and restarts their Newton solves
+// from the step's initial iterate (z = zold).  dsel: `sel` on the device.
+void restart_solves(gp_ctx *C, const std::vector<uint8_t> &sel, const uint8_t *dsel) {
+  const int L = C->ldM;
+  land_batch(C, true);
+  for (int m = 0; m < L; m++)
+    if (sel[m]) C->queue[m] = C->pend[m] = 0;
+  launch_async_flags(L, 2, dsel, C->d_cur.p, C->d_pend.p, C->stream);
+  launch_copy_active(C->n, L, C->d_zold.p, C->d_z.p, dsel, C->stream);
+}
+
 // newton_step! (src/numerics.jl:55-176) for every scenario in `mask`, in lock
 // step: each round evaluates F, drops the scenarios with |F|inf < tol (and
-// those that failed), and applies one correction to the rest.
+// those that failed), and applies one correction to the rest.  With
+// `retry_exact`, a scenario whose solve failed under a policy that reuses
+// factorisations solves again from the step's initial iterate as exact
+// Newton (a factorisation at every iteration, the reference's method) before
+// it counts as failed.
 NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, const std::vector<uint8_t> &mask,
-                 const std::vector<uint8_t> *refac0 = nullptr) {
+                 const std::vector<uint8_t> *refac0 = nullptr, bool retry_exact = false, bool exact = false) {
   const int L = C->ldM;
   Newton N(C, method, dt, tol, itermax, refac0 != nullptr);
+  if (exact) N.make_exact();
   set_mask(C, C->d_work, mask);
   if (N.reuse) set_mask(C, C->d_refac0, *refac0);
   HIPCHK(hipMemsetAsync(C->d_failed.p, 0, L, C->stream));
@@ -1359,6 +1422,18 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
     any = any || w[m];
   }
   if (any) out.failed = w;
+  if (any && retry_exact && N.retryable() && !exact) {
+    set_mask(C, C->d_sel, w);
+    restart_solves(C, w, C->d_sel.p);
+    const NewtonOut r = newton(C, method, dt, tol, itermax, w, nullptr, false, true);
+    out.corrections += r.corrections;
+    out.refac += r.refac;
+    out.rounds += r.rounds;
+    out.factor_launches += r.factor_launches;
+    out.exact_retries += r.exact_retries;
+    for (int m = 0; m < L; m++) out.exact_retries += w[m];
+    out.failed = r.failed;
+  }
   return out;
 }
 
@@ -1479,6 +1554,7 @@ struct Stepper {
     S.refactorizations += no.refac;
     S.newton_rounds += no.rounds;
     S.factor_launches += no.factor_launches;
+    S.exact_retries += no.exact_retries;
   }
   // Step k for the scenarios with k <= nsteps[m] that have not stopped.
   // hist_k: recorded channels of z_k (NULL: none); zsave: all of z_k,
@@ -1492,7 +1568,7 @@ struct Stepper {
     set_mask(C, C->d_act, act);
     launch_copy_active(C->n, L, C->d_z.p, C->d_zold.p, C->d_act.p, C->stream);
     for (int m = 0; m < L; m++) need0[m] = act[m] && st.stale[m];
-    NewtonOut no = newton(C, method, dt, tol, itermax, act, &need0);
+    NewtonOut no = newton(C, method, dt, tol, itermax, act, &need0, true);
     for (int m = 0; m < M; m++)
       if (act[m]) st.stale[m] = 0;
     add_stats(no);
@@ -1554,7 +1630,7 @@ struct Stepper {
       // re-solve at dt = 0: differential states frozen, algebraic states re-solved
       set_mask(C, C->d_sel, res);
       launch_copy_active(C->n, L, C->d_z.p, C->d_zold.p, C->d_sel.p, C->stream);
-      NewtonOut nr = newton(C, method, 0.0, tol, itermax, res, &res);
+      NewtonOut nr = newton(C, method, 0.0, tol, itermax, res, &res, true);
       for (int m = 0; m < M; m++)
         if (res[m]) st.stale[m] = 1;   // factorised at dt = 0: the next step refactorises
       add_stats(nr);
@@ -1614,7 +1690,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
   std::vector<std::vector<const gp_event *>> active_ev(M);
   std::vector<int> k(L, 1);
   std::vector<uint8_t> mode(L, 0), run(L, 0), first(L, 0), stale(L, 1), fresh(L, 0), wk(L, 0), fl(L, 0),
-      everfail(L, 0);
+      everfail(L, 0), exact(L, 0), restart(L, 0);
   const RoundLayout R(L);
   uint8_t *hr = C->h_round.p, *hp = C->h_post.p;   // pinned staging (RoundLayout)
   uint8_t *acc = hp + R.acc(), *adv = hp + R.adv(), *sv = hp + R.sv(), *ck = hp + R.ck();
@@ -1648,6 +1724,8 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
   HIPCHK(hipStreamSynchronize(C->stream));   // the staging block stays valid until then
   Newton N(C, method, dt, tol, itermax, true);
   N.dtv = C->d_dtv.p;
+  N.exactv = C->d_exact.p;
+  N.exacth = exact.data();
   bool fault_dirty = true;
   const long long max_rounds = (long long)(kmax + 1) * 2 * (4 * itermax + 16);
   for (long long round = 0;; round++) {
@@ -1660,6 +1738,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
       land_first = land_first || fresh[m];
       hr[R.first() + m] = first[m];
       hr[R.refac0() + m] = fresh[m];
+      hr[R.exact() + m] = exact[m];
       hr[R.work() + m] = run[m];
       hdt[m] = dtv[m];
     }
@@ -1690,10 +1769,23 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
     std::memset(adv, 0, L);
     std::memset(sv, 0, L);
     std::memset(ck, 0, L);
-    bool any_acc = false, any_adv = false, any_fail = false, any_sv = false, any_ck = false;
+    bool any_acc = false, any_adv = false, any_fail = false, any_sv = false, any_ck = false, any_restart = false;
+    std::fill(restart.begin(), restart.end(), 0);
     for (int m = 0; m < M; m++) {
       if (!run[m]) continue;
       first[m] = 0;
+      if (fl[m] && N.retryable() && !exact[m]) {
+        // a solve on reused or FP32 factorisations failed: it runs again
+        // from the step's initial iterate as exact Newton with FP64 factors
+        // (the reference's method)
+        exact[m] = 1;
+        first[m] = 1;
+        stale[m] = 1;
+        restart[m] = 1;
+        any_restart = true;
+        S.exact_retries++;
+        continue;
+      }
       if (fl[m]) {   // itermax or stall rule: the scenario stops and is reported
         run[m] = 0;
         everfail[m] = 1;
@@ -1703,6 +1795,8 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
         continue;
       }
       if (wk[m]) continue;   // still iterating
+      const bool was_exact = exact[m] != 0;
+      exact[m] = 0;          // solve accepted
       bool next = true;
       if (mode[m] == 0) {    // step k accepted: record z_k, then its events
         if (F.record) {
@@ -1715,7 +1809,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
           svslot[m] = k[m] - F.seg_k0 - 1;
           any_sv = true;
         }
-        stale[m] = 0;
+        stale[m] = was_exact ? 1 : 0;   // after an exact solve the next step factorises afresh
         S.scenario_steps++;
         bool changed = false;
         auto ion = on[m].find(k[m]);
@@ -1773,6 +1867,11 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
     if (any_sv) launch_copy_cols(n, L, C->d_z.p, F.seg, nL, C->d_svslot.p, C->d_sv.p, C->stream);
     if (any_ck) launch_copy_cols(n, L, C->d_z.p, F.ckpt, nL, C->d_ckslot.p, C->d_ck.p, C->stream);
     if (any_adv) launch_copy_active(n, L, C->d_z.p, C->d_zold.p, C->d_adv.p, C->stream);
+    if (any_restart) {
+      set_mask(C, C->d_sel, restart);
+      restart_solves(C, restart, C->d_sel.p);
+      if (!any_fail) HIPCHK(hipMemsetAsync(C->d_failed.p, 0, L, C->stream));
+    }
     if (any_fail) {          // non-finite states
       std::vector<double> fn(L);
       HIPCHK(hipMemcpy(fn.data(), C->d_fnorm.p, L * sizeof(double), hipMemcpyDeviceToHost));

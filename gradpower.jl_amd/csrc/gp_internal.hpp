@@ -118,7 +118,8 @@ void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, 
                          unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
                          double theta, int stall, int itermax, double *fbest, int *nstall, int *iters,
                          uint8_t *failed, const uint8_t *pend, uint8_t *upd, double async_below, uint8_t *areq,
-                         uint8_t *sch, hipStream_t s, const uint8_t *firstv = nullptr);
+                         uint8_t *sch, hipStream_t s, const uint8_t *firstv = nullptr,
+                         const uint8_t *exact = nullptr);
 void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs,
                      double *jv, const uint8_t *work, hipStream_t s, const double *dtv = nullptr);
 template <typename TF>
@@ -149,6 +150,10 @@ void launch_copy_cols(int n, int ldM, const double *src, double *dst, size_t str
 // asynchronous-refactorisation flags: mode 0 landed (cur ^= 1, pend = 0),
 // 1 requested (pend = 1), 2 dropped (pend = 0), for the scenarios in a
 void launch_async_flags(int L, int mode, const uint8_t *a, uint8_t *cur, uint8_t *pend, hipStream_t s);
+// the round's (sch, refac, upd) masks split by factor precision: out [6][L],
+// FP32 scenarios' masks then those of the scenarios with x64[m] != 0
+void launch_split_precision(int L, const uint8_t *x64, const uint8_t *sch, const uint8_t *refac, const uint8_t *upd,
+                            uint8_t *out, hipStream_t s);
 // discrete adjoint: backward-step mask, carried multipliers, objective rows,
 // parameter pullbacks
 void launch_adj_mask(int ldM, int k, const int *hz, uint8_t *act, hipStream_t s);

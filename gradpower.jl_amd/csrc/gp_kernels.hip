@@ -474,9 +474,11 @@ __global__ void __launch_bounds__(256) k_rownorm(int n, int ldM, int M, const do
 // or that asks for an asynchronous one, waits for it instead, so that its
 // iterates are those of the synchronous policy.  sch[m] = 1 where the Schur
 // kernels run (corrections and new asynchronous requests).  iters[m] counts
-// corrections; first (per scenario with firstv) resets the per-solve history.  n_unconv:
-// [0] working scenarios, [1] synchronous requests, [2] corrections, [3]
-// asynchronous requests.
+// corrections; first (per scenario with firstv) resets the per-solve history.
+// exact[m] != 0 (optional) runs scenario m's solve as exact Newton: a
+// synchronous factorisation at every iteration, whatever theta says.
+// n_unconv: [0] working scenarios, [1] synchronous requests, [2]
+// corrections, [3] asynchronous requests.
 __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict__ nb, double *__restrict__ fnorm,
                               uint8_t *__restrict__ work, int *__restrict__ n_unconv, uint8_t *__restrict__ refac,
                               const uint8_t *__restrict__ refac0, int first0, const uint8_t *__restrict__ firstv,
@@ -484,9 +486,11 @@ __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict_
                               double *__restrict__ fbest, int *__restrict__ nstall, int *__restrict__ iters,
                               uint8_t *__restrict__ failed, const uint8_t *__restrict__ pend,
                               uint8_t *__restrict__ upd, double async_below, uint8_t *__restrict__ areq,
-                              uint8_t *__restrict__ sch) {
+                              uint8_t *__restrict__ sch, const uint8_t *__restrict__ exact) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
+  const bool ex = exact && exact[m];
+  if (ex) theta = 0.0;
   const bool first = firstv ? firstv[m] != 0 : first0 != 0;   // first round of this scenario's solve
   const bool was_upd = first ? true : upd[m] != 0;
   upd[m] = 0;
@@ -532,7 +536,7 @@ __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict_
   // a change of dt) does not wait for one asked for under the old system:
   // the host lands and drops that request before this round's factorisation
   bool wait = pending && !(first && rf);
-  if (rf && !first && async_below > 0.0 && v < async_below) {
+  if (rf && !first && !ex && async_below > 0.0 && v < async_below) {
     areq[m] = 1;
     atomicAdd(n_unconv + 3, 1);
     wait = true;
@@ -2363,6 +2367,23 @@ __global__ void k_async_flags(int L, int mode, const uint8_t *__restrict__ a, ui
   }
 }
 
+// Splits the round's Schur, refactorisation and correction masks by the
+// precision of each scenario's factors: out[0..2] = (sch, refac, upd) of the
+// scenarios with FP32 factors, out[3..5] those with FP64 factors (x64[m]).
+__global__ void k_split_precision(int L, const uint8_t *__restrict__ x64, const uint8_t *__restrict__ sch,
+                                  const uint8_t *__restrict__ refac, const uint8_t *__restrict__ upd,
+                                  uint8_t *__restrict__ out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= L) return;
+  const uint8_t x = x64[m] ? 1 : 0, k = (uint8_t)(1 - x);
+  out[m] = sch[m] & k;
+  out[L + m] = refac[m] & k;
+  out[2 * L + m] = upd[m] & k;
+  out[3 * L + m] = sch[m] & x;
+  out[4 * L + m] = refac[m] & x;
+  out[5 * L + m] = upd[m] & x;
+}
+
 // Columns of the scenarios in mask into their own slot of a stack of
 // scenario-major arrays: dst[slot[m]*stride + i*ldM + m] = src[i*ldM + m].
 __global__ void k_copy_cols(int n, int ldM, const double *__restrict__ src, double *__restrict__ dst, size_t stride,
@@ -2435,12 +2456,12 @@ void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, 
                          unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
                          double theta, int stall, int itermax, double *fbest, int *nstall, int *iters,
                          uint8_t *failed, const uint8_t *pend, uint8_t *upd, double async_below, uint8_t *areq,
-                         uint8_t *sch, hipStream_t s, const uint8_t *firstv) {
+                         uint8_t *sch, hipStream_t s, const uint8_t *firstv, const uint8_t *exact) {
   const int chunks = std::max(1, std::min((n + 15) / 16, 1024 / (ldM / WAVE)));
   hipLaunchKernelGGL(k_rownorm, dim3(ldM / WAVE, chunks), dim3(256), 0, s, n, ldM, M, f, work, nb);
   hipLaunchKernelGGL(k_norm_finish, dim3((M + 255) / 256), dim3(256), 0, s, M, tol, nb, fnorm, work, n_unconv, refac,
                      refac0, first, firstv, theta, stall, itermax, fbest, nstall, iters, failed, pend, upd,
-                     async_below, areq, sch);
+                     async_below, areq, sch, exact);
 }
 void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs, double *jv,
                      const uint8_t *work, hipStream_t s, const double *dtv) {
@@ -2538,6 +2559,10 @@ void launch_copy_cols(int n, int ldM, const double *src, double *dst, size_t str
 }
 void launch_async_flags(int L, int mode, const uint8_t *a, uint8_t *cur, uint8_t *pend, hipStream_t s) {
   hipLaunchKernelGGL(k_async_flags, dim3((L + 255) / 256), dim3(256), 0, s, L, mode, a, cur, pend);
+}
+void launch_split_precision(int L, const uint8_t *x64, const uint8_t *sch, const uint8_t *refac, const uint8_t *upd,
+                            uint8_t *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_split_precision, dim3((L + 255) / 256), dim3(256), 0, s, L, x64, sch, refac, upd, out);
 }
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s) {
   const int gx = n < 1024 ? n : 1024;

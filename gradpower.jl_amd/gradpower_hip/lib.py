@@ -50,7 +50,8 @@ class CStats(C.Structure):
                 ("solve_kernel_ms", C.c_double), ("solve_kernel_launches", C.c_int64),
                 ("solve_kernel_scenarios", C.c_int64), ("factor_kernel_ms", C.c_double),
                 ("factor_kernel_launches", C.c_int64), ("factor_kernel_scenarios", C.c_int64),
-                ("solve_bytes_per_scenario", C.c_double), ("factor_bytes_per_scenario", C.c_double)]
+                ("solve_bytes_per_scenario", C.c_double), ("factor_bytes_per_scenario", C.c_double),
+                ("exact_retries", C.c_int64)]
 
 
 class CSymInfo(C.Structure):

@@ -224,6 +224,12 @@ typedef struct gp_stats {
   double factor_kernel_ms;
   int64_t factor_kernel_launches, factor_kernel_scenarios;
   double solve_bytes_per_scenario, factor_bytes_per_scenario;
+  /* Newton solves run again as exact Newton: under a refactorisation policy
+     that reuses factorisations (FIRST_ITER, ADAPTIVE, ASYNC) a solve that
+     fails restarts from the step's initial iterate with a factorisation at
+     every iteration (the reference's method) before the scenario counts as
+     failed                                                                 */
+  int64_t exact_retries;
 } gp_stats;
 
 typedef struct gp_ctx gp_ctx;

@@ -59,6 +59,9 @@ typedef struct {
   /* ZIP load disconnects of the next sweeps (orc_set_drops) and, during a
    * sweep, the online flag of every ZIP load (NULL: all online) */
   int ndrop;
+  /* 1: a sweep stops at its first Newton solve that does not converge (the
+   * HIP library's rule; the reference itself keeps integrating) */
+  int stop_on_fail;
   int *drop_k, *drop_step;
   uint8_t *zon;
 } orc_sys;
@@ -778,6 +781,8 @@ void orc_destroy(void *h) {
  * src/GradPower.jl:130-143,681-698): ZIP load k[i] (row of the ZIP table)
  * goes offline for good at the end of step step[i]; an offline device is
  * skipped by the residual and Jacobian (src/kernels/zipload.jl, `online`). */
+void orc_set_stop_on_fail(void *h, int flag) { ((orc_sys *)h)->stop_on_fail = flag != 0; }
+
 void orc_set_drops(void *h, int n, const int *k, const int *step) {
   orc_sys *s = h;
   free(s->drop_k); free(s->drop_step);
@@ -880,7 +885,9 @@ static int newton(const orc_sys *s, double *z, const double *zold, const double 
  * traj: (nsteps+1) x n, row k = state after step k (row 0 = z0), or NULL.
  * fgs: (nsteps+1) x nbus, row k = fault conductances in force during step k
  * (row 0 unused), or NULL.
- * stats[0] = Newton iterations, stats[1] = non-converged solves, stats[2] = max accepted |F|. */
+ * stats[0] = Newton iterations, stats[1] = non-converged solves, stats[2] = max accepted |F|,
+ * stats[3] = steps completed before the first Newton solve that did not
+ * converge (k-1 for the solve of step k, k for its event re-solve; -1: none). */
 static int sweep(orc_sys *s, const double *z0, const double *p, double dt, int nsteps,
                  const int *ev_bus, const double *ev_r, const int *ev_on, const int *ev_off, int n_ev,
                  double tol, int itermax, double *traj, double *fgs, double *ys, uint8_t *zons, double *zfinal,
@@ -901,7 +908,7 @@ static int sweep(orc_sys *s, const double *z0, const double *p, double dt, int n
   orc_lu F = {0};
   memcpy(z, z0, n * sizeof(double));
   if (traj) memcpy(traj, z, n * sizeof(double));
-  double iters = 0, fails = 0, maxf = 0.0, fn;
+  double iters = 0, fails = 0, maxf = 0.0, fn, first_fail = -1.0;
   int rc = 0;
   for (int k = 1; k <= nsteps; k++) {
     memcpy(zold, z, n * sizeof(double));
@@ -911,7 +918,8 @@ static int sweep(orc_sys *s, const double *z0, const double *p, double dt, int n
     if (ys) memcpy(ys + (size_t)k * s->ynnz, s->yv, (size_t)s->ynnz * sizeof(double));
     if (zons) memcpy(zons + (size_t)k * (nz + 1), zon, (size_t)nz + 1);
     int it = newton(s, z, zold, p, fg, dt, tol, itermax, f, J, dx, w, &F, &fn);
-    if (it < 0) { fails++; if (it <= -2) { rc = it; } } else iters += it;
+    if (it < 0) { fails++; if (first_fail < 0) first_fail = k - 1; if (it <= -2) { rc = it; } } else iters += it;
+    if (it < 0 && s->stop_on_fail) break;
     if (fn > maxf) maxf = fn;
     if (traj) memcpy(traj + (size_t)k * n, z, n * sizeof(double));
     int any = 0;
@@ -930,13 +938,14 @@ static int sweep(orc_sys *s, const double *z0, const double *p, double dt, int n
       for (int b = 0; b < s->nbus; b++) fg[b] = 0.0;
       for (int e = 0; e < n_ev; e++) if (active[e]) fg[ev_bus[e]] += 1.0 / ev_r[e];
       it = newton(s, z, zold, p, fg, 0.0, tol, itermax, f, J, dx, w, &F, &fn);
-      if (it < 0) { fails++; if (it <= -2) rc = it; } else iters += it;
+      if (it < 0) { fails++; if (first_fail < 0) first_fail = k; if (it <= -2) rc = it; } else iters += it;
+      if (it < 0 && s->stop_on_fail) break;
       if (fn > maxf) maxf = fn;
     }
     if (rc) break;
   }
   if (zfinal) memcpy(zfinal, z, n * sizeof(double));
-  if (stats) { stats[0] = iters; stats[1] = fails; stats[2] = maxf; }
+  if (stats) { stats[0] = iters; stats[1] = fails; stats[2] = maxf; stats[3] = first_fail; }
   memcpy(s->yv, y_saved, (size_t)s->ynnz * sizeof(double));
   s->zon = NULL;
   lu_free(&F);
@@ -1008,7 +1017,7 @@ int orc_adjoint(void *h, const double *z0, const double *p, double dt, int nstep
   memcpy(y_saved, s->yv, (size_t)s->ynnz * sizeof(double));
   int nz = s->ndev[GP_ZIPLOAD];
   uint8_t *zons = xmalloc((size_t)(nsteps + 1) * (nz + 1));
-  double st[3];
+  double st[4];
   int rc = sweep(s, z0, p, dt, nsteps, ev_bus, ev_r, ev_on, ev_off, n_ev, tol, itermax, traj, fgs, ys, zons, NULL, st);
   if (rc) { free(traj); free(fgs); free(ys); free(y_saved); free(zons); return rc; }
   double *lam = xmalloc(n * sizeof(double)), *a = xmalloc(n * sizeof(double)), *w = xmalloc(n * sizeof(double));

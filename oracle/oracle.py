@@ -40,6 +40,7 @@ def _load():
     lib.orc_schur_step.argtypes = [C.c_void_p, _dp, _dp, _dp]
     lib.orc_set_trips.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _dp]
     lib.orc_set_drops.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
+    lib.orc_set_stop_on_fail.argtypes = [C.c_void_p, C.c_int]
     lib.orc_linsolve_t.argtypes = [C.c_void_p, _dp, _dp, _dp]
     lib.orc_adjoint.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, _ip, _dp, _ip, _ip, C.c_int,
                                 C.c_double, C.c_int, C.c_int, _ip, C.c_double, C.c_double, C.c_int, _ip,
@@ -144,12 +145,16 @@ class Oracle:
         lib().orc_set_drops(self.h, len(d), _i(k), _i(st))
 
     def integrate(self, dt, nsteps, events=(), z0=None, tol=1e-10, itermax=50, traj=True, p=None, trips=(),
-                  drops=()):
+                  drops=(), stop_on_fail=False):
         """events: iterable of (bus, r_fault, step_on, step_off); trips: line
         trips, see _set_trips; drops: ZIP load disconnects, see _set_drops.
-        Returns (traj [(nsteps+1) x n] or None, zfinal, stats dict)."""
+        stop_on_fail: stop at the first Newton solve that does not converge
+        (the reference keeps integrating).  Returns (traj [(nsteps+1) x n]
+        or None, zfinal, stats dict); stats["first_fail_step"] is the number
+        of steps completed before the first such solve (-1: none)."""
         self._set_trips(trips)
         self._set_drops(drops)
+        lib().orc_set_stop_on_fail(self.h, 1 if stop_on_fail else 0)
         z0 = np.ascontiguousarray(self.desc.z0 if z0 is None else z0, np.float64)
         p = self.p if p is None else np.ascontiguousarray(p, np.float64)
         ev = list(events)
@@ -159,10 +164,11 @@ class Oracle:
         eoff = np.array([e[3] for e in ev] or [0], np.int32)
         tr = np.zeros((nsteps + 1, self.n)) if traj else None
         zf = np.zeros(self.n)
-        st = np.zeros(3)
+        st = np.zeros(4)
         rc = lib().orc_integrate(self.h, _d(z0), _d(p), dt, nsteps, _i(eb), _d(er), _i(eon), _i(eoff),
                                  len(ev), tol, itermax, _d(tr) if traj else None, _d(zf), _d(st))
-        return tr, zf, {"rc": rc, "newton_iters": st[0], "not_converged": st[1], "max_residual": st[2]}
+        return tr, zf, {"rc": rc, "newton_iters": st[0], "not_converged": st[1], "max_residual": st[2],
+                        "first_fail_step": int(st[3])}
 
     def linsolve_t(self, Jvals, c):
         """y with J^T y = c."""

@@ -1,9 +1,9 @@
 """Summaries of rocprofv3 CSV output for the profiles/ folder.
 
-    python3 -m gradpower_hip.prof_summary stats DIR OUT.md --cmd "..."
+    python3 prof_summary.py stats DIR OUT.md --cmd "..."
         per-kernel table (calls, total ms, average us, share) from *kernel_stats.csv
 
-    python3 -m gradpower_hip.prof_summary traffic FETCH_DIR WRITE_DIR KERNEL COPY_BYTES OUT.json --cmd "..." \
+    python3 prof_summary.py traffic FETCH_DIR WRITE_DIR KERNEL COPY_BYTES OUT.json --cmd "..." \
             [--scenarios SCENARIOS_PER_LAUNCH] [--last N]
         HBM bytes per launch of KERNEL from a FETCH_SIZE run and a WRITE_SIZE run
         (KiB per dispatch), each scaled by what the counter reads for the plain

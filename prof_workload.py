@@ -5,7 +5,7 @@ scenario horizons capped so that a counter run stays short; or (--adjoint)
 one discrete-adjoint pass of the 200-bus M=128 3 s workload.
 
     rocprofv3 --kernel-trace --stats --output-format csv -d OUT -o run -- \\
-        python3 -m gradpower_hip.prof_workload --horizon-cap 0.5
+        python3 prof_workload.py --horizon-cap 0.5
 """
 import argparse
 import os
@@ -13,7 +13,7 @@ import sys
 import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(HERE, "gradpower.jl_amd"))
 
 import numpy as np  # noqa: E402
 

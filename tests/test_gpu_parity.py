@@ -31,7 +31,7 @@ def _perturbed(case, M, seed, scale=0.02):
     return z0[None, :] * (1 + scale * rng.standard_normal((M, len(z0)))) + scale * rng.standard_normal((M, len(z0)))
 
 
-@pytest.mark.parametrize("name", ["case_2bus_testdata", "case0_2bus", "case1_9bus", "case3_200bus"])
+@pytest.mark.parametrize("name", ["case_2bus_testdata", "case0_2bus", "case1_9bus", "case3_200bus", "case2_60bus"])
 def test_residual_matches_oracle(name):
     case = get_case(name)
     M = min(case.M, 8)
@@ -53,7 +53,7 @@ def test_residual_matches_oracle(name):
             assert np.max(np.abs(f[m] - fo) / scale) < 1e-12, (name, m, dt)
 
 
-@pytest.mark.parametrize("name", ["case0_2bus", "case1_9bus", "case3_200bus"])
+@pytest.mark.parametrize("name", ["case0_2bus", "case1_9bus", "case3_200bus", "case2_60bus"])
 def test_jacobian_matches_oracle(name):
     import scipy.sparse as sp
     case = get_case(name)
@@ -91,7 +91,7 @@ METHODS = {"mono": 0, "schur64": 1, "schur32": 2}
 
 
 @pytest.mark.parametrize("method", ["mono", "schur64", "schur32"])
-@pytest.mark.parametrize("name", ["case0_2bus", "case1_9bus", "case3_200bus"])
+@pytest.mark.parametrize("name", ["case0_2bus", "case1_9bus", "case3_200bus", "case2_60bus"])
 def test_newton_step_matches_oracle(name, method):
     """One Newton correction of every method equals the oracle's -J^{-1}F
     (the Schur elimination is exact algebra; FP32 factors give the step to
@@ -120,7 +120,7 @@ def test_newton_step_matches_oracle(name, method):
 @pytest.mark.parametrize("method,policy", [("mono", 0), ("schur64", 0), ("schur64", 2), ("schur32", 2),
                                            ("schur64", 3), ("schur32", 3)])
 @pytest.mark.parametrize("name,nsteps", [("case_2bus_testdata", 120), ("case0_2bus", 240), ("case1_9bus", None),
-                                         ("case3_200bus", 60)])
+                                         ("case3_200bus", 60), ("case2_60bus", 120)])
 def test_trajectory_matches_oracle(name, nsteps, method, policy):
     case = get_case(name)
     M = min(case.M, 8)
@@ -148,10 +148,7 @@ def test_scenarios_advancing_alone_match_lock_step(name, method, policy):
     starts with its own factorisation (EVERY_ITER, FIRST_ITER): each
     scenario's arithmetic does not depend on the others.  Uneven horizons and
     per-scenario fault times included."""
-    if name == "case2_grid_small":
-        case = get_case("case2_grid", M=8, nbus=60, n_genrou=12, n_pss=4, n_static=6, seed=77)
-    else:
-        case = get_case(name)
+    case = get_case("case2_60bus" if name == "case2_grid_small" else name)
     M = min(case.M, 8)
     ns = [int(case.nsteps[m]) for m in range(M)]
     ev = [(m, e.bus, e.rfault, int(round(e.ton / case.dt)), int(round(e.toff / case.dt)))

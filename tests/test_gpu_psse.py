@@ -62,7 +62,9 @@ def test_psse_newton_step_matches_oracle(dyr, method):
 
 @pytest.mark.parametrize("method,policy", [("mono", 0), ("schur64", 0), ("schur32", 3)])
 @pytest.mark.parametrize("raw,dyr", [("ieee9_v33.raw", "ieee9bus_SEXS.dyr"), ("ieee9_v33.raw", "ieee9bus_gov.dyr"),
-                                     ("IEEE39.raw", "IEEE39_gov.dyr")])
+                                     ("IEEE39.raw", "IEEE39_gov.dyr"), ("2bus.raw", "2bus_ESDC1A.dyr"),
+                                     ("2bus.raw", "2bus_IEESGO.dyr"), ("2bus.raw", "2bus_SEXS.dyr"),
+                                     ("2bus.raw", "2bus_TGOV1.dyr")])
 def test_psse_case_trajectories_match_oracle(raw, dyr, method, policy):
     from gradpower_hip.lib import HipContext
     from oracle.oracle import Oracle
@@ -122,4 +124,32 @@ def test_slack_static_generator_trajectories_match_oracle(method, policy, tmp_pa
         assert ost["not_converged"] == 0
         dev = np.max(np.abs(ctx.trajectory(m).T - ref))
         assert dev < 1e-9, (method, m, dev, st)
+    ctx.close()
+
+
+@pytest.mark.parametrize("method,policy", [("mono", 0), ("schur64", 0), ("schur32", 3)])
+def test_2bus_esdc1a_stored_trajectory_on_gpu(method, policy):
+    """The reference's 2bus.raw + 2bus_ESDC1A.dyr under the fault and load
+    model of its stored trajectory (examples/refs/2bus_esdc1a.npz: bus 1,
+    r = 0.02, 0.2-0.3 s, 5 s): the GPU follows the oracle within 1e-9 pu per
+    step, and the stored rotor speed of the independent simulator within
+    2e-4 pu (its ESDC1A keeps two more states)."""
+    from gradpower_hip.lib import HipContext
+    from oracle.oracle import Oracle
+    gold = np.load(os.path.join(G, "2bus_esdc1a.npz"))
+    case = _case("2bus.raw", "2bus_ESDC1A.dyr", M=2, zip_alpha=float(gold["zipload_alpha"]))
+    h = float(gold["tvec"][1])   # the stored run's step (5.008 s / 600)
+    n = gold["history"].shape[1] - 1
+    ev = (int(gold["fault_bus"]), float(gold["rfault"]), int(round(float(gold["ton"]) / h)),
+          int(round(float(gold["toff"]) / h)))
+    ctx = HipContext(case.desc, 2, refactor_policy=policy)
+    ctx.set_state(case.desc.z0, case.desc.p)
+    ctx.set_events([(0,) + ev, (1,) + ev], [n, n // 2])
+    ctx.integrate(METHODS[method], h, 1e-12, 50)
+    ref, _, st = Oracle(case.desc).integrate(h, n, [ev], tol=1e-12, itermax=50)
+    assert st["not_converged"] == 0
+    tr = ctx.trajectory(0)
+    assert np.max(np.abs(tr.T - ref)) < 1e-9, method
+    assert np.max(np.abs(ctx.trajectory(1).T - ref[:n // 2 + 1])) < 1e-9, method
+    assert np.max(np.abs(tr[4] - gold["history"][4])) < 2e-4
     ctx.close()
