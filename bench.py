@@ -74,7 +74,8 @@ def run_method(case, method, steps, warmup, record_idx, tol, itermax, opts):
             kern["factor_n"] += stats["factor_kernel_launches"]
             kern["factor_sc"] += stats["factor_kernel_scenarios"]
         log(f"  method {method} pass {it}: {t1 - t0:.3f} s, device {stats['elapsed_ms']:.1f} ms, "
-            f"steps {stats['scenario_steps']}, rounds {stats['newton_rounds']} "
+            f"steps {stats['scenario_steps']}, rounds {stats['newton_rounds']}, exact retries "
+            f"{stats['exact_retries']} "
             f"(factor {stats['factor_launches']}), corrections {stats['newton_iters']}, "
             f"refactorizations {stats['refactorizations']}, not converged {stats['n_not_converged']}, "
             f"solve kernel {stats['solve_kernel_ms']:.1f} ms / {stats['solve_kernel_launches']}, "
@@ -316,6 +317,9 @@ def main():
     ap.add_argument("--stall-window", type=int, default=0,
                     help="stop a scenario whose |F| has not halved in this many Newton iterations "
                          "(0, the reference's rule: run to itermax)")
+    ap.add_argument("--retry-window", type=int, default=0,
+                    help="corrections without halving |F| after which an inexact solve restarts as exact FP64 "
+                         "Newton (0: the library's default, 8; -1: at itermax only)")
     ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete-adjoint measurement")
     ap.add_argument("--adjoint-fd", type=int, default=4,
@@ -346,7 +350,7 @@ def main():
 
     spol = {"async": REFACTOR_ASYNC, "adaptive": REFACTOR_ADAPTIVE}[args.policy]
     base = {"stall_window": args.stall_window, "refactor_theta": args.theta, "async_below": args.async_below,
-            "profile": not args.no_profile}
+            "profile": not args.no_profile, "retry_window": args.retry_window}
     methods = args.methods.split(",")
     runs, mcode = {}, {}
     for name, meth, pol in (("schur32", METHOD_SCHUR_FP32, spol), ("schur64", METHOD_SCHUR_FP64, spol),
@@ -381,6 +385,7 @@ def main():
                    "contingency_steps_completed": int(head["stats"]["scenario_steps"]),
                    "scenarios_stopped": f"{int(np.sum(head['status'] != 0))} of {case.M}",
                    "newton_tol": tol, "itermax": itermax, "stall_window": args.stall_window,
+                   "retry_window": args.retry_window or 8,
                    "refactor_policy": args.policy, "refactor_theta": args.theta,
                    "horizon_cap": args.horizon_cap or None,
                    "deviations": {

@@ -427,8 +427,11 @@ struct gp_ctx {
     std::vector<uint8_t> batch;
     int nb = 0;                               // scenarios in the batch
     uint8_t *req = nullptr, *tgt = nullptr;   // in d_batch
+    uint8_t *split = nullptr;                 // req split by factor precision, [6][L] (FP32 method)
+    bool any32 = false, any64 = false;        // the batch holds FP32 / FP64 factorisations
     DBuf<double> snap;                        // the batch's matrix values
   } lanes[kLanes];
+  DBuf<uint8_t> d_lsplit;
   DBuf<int> d_fbus, d_nunconv, d_status, d_nstall, d_iters;
   DBuf<double> d_fg, d_fbest, d_traj_tmp;
   DBuf<uint8_t> d_failed, d_upd, d_cur, d_pend, d_areq, d_sch;
@@ -860,10 +863,12 @@ void alloc_state(gp_ctx *C) {
   C->d_cur.alloc(L);
   C->d_pend.alloc(L);
   C->d_batch.alloc((size_t)gp_ctx::kLanes * 2 * L);
+  C->d_lsplit.alloc((size_t)gp_ctx::kLanes * 6 * L);
   C->h_batch.alloc((size_t)gp_ctx::kBatchSlots * 2 * L);
   for (int l = 0; l < gp_ctx::kLanes; l++) {
     C->lanes[l].req = C->d_batch.p + (size_t)l * 2 * L;
     C->lanes[l].tgt = C->lanes[l].req + L;
+    C->lanes[l].split = C->d_lsplit.p + (size_t)l * 6 * L;
     C->lanes[l].batch.assign(L, 0);
   }
   HIPCHK(hipMemset(C->d_cur.p, 0, L));
@@ -889,40 +894,70 @@ void set_mask(gp_ctx *C, DBuf<uint8_t> &d, const std::vector<uint8_t> &h) {
   HIPCHK(hipMemcpyAsync(d.p, h.data(), h.size(), hipMemcpyHostToDevice, C->stream));
 }
 
-// The pivot analysis's second matrix (see splu_analyse): the representative
-// values `a` (J in CSC order, or S when reduced) with every distinct tripped
-// line taken out at once -- the diagonal blocks of its buses gain +yff / +ytt
-// and the off-diagonal blocks +yft / +ytf, realified [g -b; b g].  Empty when
-// no line is tripped.
-std::vector<double> trips_out(const gp_ctx *C, const std::vector<double> &a, bool reduced) {
-  std::vector<double> out;
+// The pivot analysis's extra matrices (see splu_analyse): the
+// representative values `a` (J in CSC order, or S when reduced) with a set
+// of tripped lines taken out at once -- the diagonal blocks of its buses gain
+// +yff / +ytt and the off-diagonal blocks +yft / +ytf, realified [g -b; b g].
+// One matrix per distinct set: every line a scenario trips (at once), and
+// every tripped line alone.  Empty when no line is tripped.
+std::vector<std::vector<double>> trip_matrices(const gp_ctx *C, const std::vector<double> &a, bool reduced) {
+  std::vector<std::vector<double>> out;
   if (C->trips.empty()) return out;
-  out = a;
   const int npt = C->net_ptr;
   auto pos = [&](int zr, int zc) {
     return reduced ? find_pos(C->scp, C->sri, C->red_of_z[zr], C->red_of_z[zc]) : find_pos(C->jcp, C->jri, zr, zc);
   };
-  auto add = [&](int rb, int cb, const double *y) {
+  auto add = [&](std::vector<double> &m, int rb, int cb, const double *y) {
     const double v[4] = {y[0], -y[1], y[1], y[0]};
     for (int rr = 0; rr < 2; rr++)
       for (int cc = 0; cc < 2; cc++) {
         const int p = pos(npt + 2 * rb + rr, npt + 2 * cb + cc);
         if (p < 0) throw GpError{GP_ERR_HIP, "internal: tripped line's entry missing from a pattern"};
-        out[p] += v[2 * rr + cc];
+        m[p] += v[2 * rr + cc];
       }
   };
-  std::vector<std::vector<double>> seen;
-  for (const gp_line_trip &t : C->trips) {
-    std::vector<double> key{(double)t.from_bus, (double)t.to_bus};
-    key.insert(key.end(), t.y, t.y + 8);
-    if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
-    seen.push_back(key);
-    add(t.from_bus, t.from_bus, t.y);
-    add(t.from_bus, t.to_bus, t.y + 2);
-    add(t.to_bus, t.from_bus, t.y + 4);
-    add(t.to_bus, t.to_bus, t.y + 6);
+  // distinct lines (a line tripped in several scenarios, or tripped again
+  // after a reclosure, is one line)
+  std::vector<std::vector<double>> lines;
+  std::vector<int> line_of(C->trips.size());
+  for (size_t t = 0; t < C->trips.size(); t++) {
+    const gp_line_trip &tr = C->trips[t];
+    std::vector<double> key{(double)tr.from_bus, (double)tr.to_bus};
+    key.insert(key.end(), tr.y, tr.y + 8);
+    auto it = std::find(lines.begin(), lines.end(), key);
+    line_of[t] = (int)(it - lines.begin());
+    if (it == lines.end()) lines.push_back(key);
+  }
+  std::vector<std::vector<int>> sets;
+  for (int l = 0; l < (int)lines.size(); l++) sets.push_back({l});
+  std::map<int, std::vector<int>> per_scen;
+  for (size_t t = 0; t < C->trips.size(); t++) per_scen[C->trips[t].scenario].push_back(line_of[t]);
+  for (auto &kv : per_scen) {
+    std::vector<int> st = kv.second;
+    std::sort(st.begin(), st.end());
+    st.erase(std::unique(st.begin(), st.end()), st.end());
+    if (std::find(sets.begin(), sets.end(), st) == sets.end()) sets.push_back(st);
+  }
+  for (const auto &st : sets) {
+    std::vector<double> m = a;
+    for (int l : st) {
+      const std::vector<double> &k = lines[l];
+      const int fb = (int)k[0], tb = (int)k[1];
+      const double *y = k.data() + 2;
+      add(m, fb, fb, y);
+      add(m, fb, tb, y + 2);
+      add(m, tb, fb, y + 4);
+      add(m, tb, tb, y + 6);
+    }
+    out.push_back(std::move(m));
   }
   return out;
+}
+
+static std::vector<const double *> ptrs(const std::vector<std::vector<double>> &v) {
+  std::vector<const double *> p;
+  for (const auto &x : v) p.push_back(x.data());
+  return p;
 }
 
 // Representative matrix of scenario 0 for the pivot analysis, evaluated by
@@ -952,9 +987,8 @@ void analyse(gp_ctx *C, int method) {
     HIPCHK(hipMemcpy2DAsync(jv.data(), sizeof(double), C->d_jv.p, (size_t)L * sizeof(double), sizeof(double), nnz,
                             hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
-    std::vector<double> jv2 = trips_out(C, jv, false);
-    std::string e = splu_analyse(C->n, C->jcp.data(), C->jri.data(), jv.data(), C->luJ.A, 1e-3,
-                                 jv2.empty() ? nullptr : jv2.data());
+    const std::vector<std::vector<double>> jx = trip_matrices(C, jv, false);
+    std::string e = splu_analyse(C->n, C->jcp.data(), C->jri.data(), jv.data(), C->luJ.A, 1e-3, ptrs(jx));
     if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "Jacobian analysis: " + e};
     std::vector<int> asrc(nnz);
     for (int i = 0; i < nnz; i++) asrc[i] = i;
@@ -989,9 +1023,8 @@ void analyse(gp_ctx *C, int method) {
       int s = C->s_src[e];
       sx[e] = s >= 0 ? sv[s] : C->s_cst[-s - 1];
     }
-    std::vector<double> sx2 = trips_out(C, sx, true);
-    std::string e = splu_analyse(C->nred, C->scp.data(), C->sri.data(), sx.data(), C->luS.A, 1e-3,
-                                 sx2.empty() ? nullptr : sx2.data());
+    const std::vector<std::vector<double>> sxx = trip_matrices(C, sx, true);
+    std::string e = splu_analyse(C->nred, C->scp.data(), C->sri.data(), sx.data(), C->luS.A, 1e-3, ptrs(sxx));
     if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "reduced-network analysis: " + e};
     // a factorisation serves several solves (refactorisation policies), and
     // the solves are on every round's critical path
@@ -1097,8 +1130,10 @@ int free_lane(const gp_ctx *C) {
 // A batch is staged on the main stream (its masks, the pending flags, the
 // snapshot of its matrix values) and then started on its lane: the caller
 // launches the round's solve between the two, so that the host calls that
-// start the factorisation overlap the solve instead of delaying it.
-int stage_batch(gp_ctx *C) {
+// start the factorisation overlap the solve instead of delaying it.  With
+// FP32 factors, the scenarios running exact Newton (exacth / exactv, host
+// and device flags; NULL: none) are factorised in FP64.
+int stage_batch(gp_ctx *C, bool fp32, const uint8_t *exacth, const uint8_t *exactv) {
   const int L = C->ldM;
   const int l = free_lane(C);
   if (l < 0) return -1;
@@ -1115,6 +1150,14 @@ int stage_batch(gp_ctx *C) {
     C->queue[m] = 0;
   }
   HIPCHK(hipMemcpyAsync(ln.req, slot, 2 * L, hipMemcpyHostToDevice, C->stream));
+  ln.any32 = ln.any64 = false;
+  for (int m = 0; m < L; m++)
+    if (ln.batch[m]) {
+      if (fp32 && C->has64 && exacth && exacth[m]) ln.any64 = true;
+      else ln.any32 = true;
+    }
+  if (ln.any64)   // the precision of every request, taken now (the flags change with the rounds)
+    launch_split_precision(L, exactv, ln.req, ln.req, ln.req, ln.split, C->stream);
   launch_async_flags(L, 1, ln.req, C->d_cur.p, C->d_pend.p, C->stream);
   launch_copy_active(std::max(1, C->n_svar), L, C->d_svar.p, ln.snap.p, ln.req, C->stream);
   HIPCHK(hipEventRecord(ln.ev_snap, C->stream));
@@ -1128,8 +1171,16 @@ void start_batch(gp_ctx *C, int l, TF *luv, TF *sv, TF *dn) {
   const int L = C->ldM, nb = ln.nb;
   HIPCHK(hipStreamWaitEvent(ln.stream, ln.ev_snap, 0));
   const int pi = prof_begin(C, C->prof_factor, ln.stream);
-  launch_splu_factor_impl<TF>(sizeof(TF) == 8 ? C->luS64 : C->luS.dev, ln.snap.p, C->d_s_cst.p, L, luv, sv, dn, ln.req, ln.tgt, C->M,
-                              C->d_status.p, ln.stream);
+  if (!ln.any64) {
+    launch_splu_factor_impl<TF>(sizeof(TF) == 8 ? C->luS64 : C->luS.dev, ln.snap.p, C->d_s_cst.p, L, luv, sv, dn,
+                                ln.req, ln.tgt, C->M, C->d_status.p, ln.stream);
+  } else {   // FP32 method with exact-Newton scenarios: their factors in FP64
+    if (ln.any32)
+      launch_splu_factor_impl<TF>(C->luS.dev, ln.snap.p, C->d_s_cst.p, L, luv, sv, dn, ln.split + L, ln.tgt, C->M,
+                                  C->d_status.p, ln.stream);
+    launch_splu_factor_impl<double>(C->luS64, ln.snap.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p, C->d_dnS64.p,
+                                    ln.split + 4 * L, ln.tgt, C->M, C->d_status.p, ln.stream);
+  }
   prof_end(C->prof_factor, pi, ln.stream, nb);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ln.ev_done, ln.stream));
@@ -1202,6 +1253,8 @@ struct Newton {
                        : 1e-300);
     lagged = theta > 0.0;
     fp32 = method == GP_METHOD_SCHUR_FP32;
+    switch_window = C->opt.retry_window < 0 ? 0 : (C->opt.retry_window > 0 ? C->opt.retry_window : 8);
+    if (!retryable()) switch_window = 0;
     async_below = C->opt.async_below > 0.0 ? C->opt.async_below : 1e300;
     req.assign(C->ldM, 0);
     reqp = req.data();
@@ -1212,10 +1265,14 @@ struct Newton {
     theta = 0.0;
     reuse = async = false;
     all64 = fp32 && C->has64;
+    switch_window = 0;
   }
   // A failed solve runs again as exact Newton: when factorisations are
   // reused, or are FP32 (the reference factors in FP64 at every iteration).
   bool retryable() const { return lagged || (fp32 && C->has64); }
+  // corrections without halving |F| after which an inexact solve is given
+  // up for exact Newton (0: only at itermax)
+  int switch_window = 0;
 
   // first: every working scenario starts its solve (lock step); firstv: the
   // scenarios that do (device flags).  cnt[] as k_norm_finish's n_unconv,
@@ -1230,7 +1287,8 @@ struct Newton {
     launch_rownorm_impl(C->n, C->M, L, C->d_f.p, C->d_work.p, tol, C->d_fnorm.p, C->d_nb.p, C->d_nunconv.p,
                         C->d_refac.p, reuse ? refac0 : nullptr, first, theta, stall, itermax, C->d_fbest.p,
                         C->d_nstall.p, C->d_iters.p, C->d_failed.p, async ? C->d_pend.p : nullptr, C->d_upd.p,
-                        async ? async_below : 0.0, C->d_areq.p, C->d_sch.p, C->stream, firstv, exactv);
+                        async ? async_below : 0.0, C->d_areq.p, C->d_sch.p, C->stream, firstv, exactv,
+                        switch_window);
     if (!cnt) return;
     HIPCHK(hipMemcpyAsync(cnt, C->d_nunconv.p, 4 * sizeof(int), hipMemcpyDeviceToHost, C->stream));
     if (async && read_req) HIPCHK(hipMemcpyAsync(req.data(), C->d_areq.p, L, hipMemcpyDeviceToHost, C->stream));
@@ -1245,7 +1303,7 @@ struct Newton {
   int stage_queue() {
     int nb = 0;
     for (int m = 0; m < C->ldM; m++) nb += C->queue[m];
-    const int l = stage_batch(C);
+    const int l = stage_batch(C, fp32, exacth, exactv);
     if (l >= 0) {
       out.refac += nb;
       out.factor_launches++;
@@ -1392,6 +1450,7 @@ NewtonOut newton(gp_ctx *C, int method, double dt, double tol, int itermax, cons
   const int L = C->ldM;
   Newton N(C, method, dt, tol, itermax, refac0 != nullptr);
   if (exact) N.make_exact();
+  if (!retry_exact) N.switch_window = 0;
   set_mask(C, C->d_work, mask);
   if (N.reuse) set_mask(C, C->d_refac0, *refac0);
   HIPCHK(hipMemsetAsync(C->d_failed.p, 0, L, C->stream));
@@ -1611,7 +1670,7 @@ struct Stepper {
       for (int m = 0; m < M; m++) {
         if (!res[m]) continue;
         if ((int)st.active_ev[m].size() > MAXFAULT)
-          throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults, line trips or load drops in one scenario"};
+          throw GpError{GP_ERR_UNSUPPORTED, "more than " + std::to_string(MAXFAULT) + " simultaneous faults, line trips or load drops in one scenario"};
         for (int q = 0; q < MAXFAULT; q++) {
           const bool on = q < (int)st.active_ev[m].size();
           st.fbus[(size_t)q * L + m] = on ? st.active_ev[m][q]->bus : -1;
@@ -1701,7 +1760,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
   std::vector<double> fg((size_t)MAXFAULT * L, 0.0);
   auto set_faults = [&](int m) {
     if ((int)active_ev[m].size() > MAXFAULT)
-      throw GpError{GP_ERR_UNSUPPORTED, "more than 4 simultaneous faults, line trips or load drops in one scenario"};
+      throw GpError{GP_ERR_UNSUPPORTED, "more than " + std::to_string(MAXFAULT) + " simultaneous faults, line trips or load drops in one scenario"};
     for (int q = 0; q < MAXFAULT; q++) {
       const bool f_on = q < (int)active_ev[m].size();
       fbus[(size_t)q * L + m] = f_on ? active_ev[m][q]->bus : -1;
@@ -1978,12 +2037,11 @@ void analyse_transposed(gp_ctx *C, double dt) {
     }
   std::vector<double> tv(nnz);
   for (int t = 0; t < nnz; t++) tv[t] = jv[tsrc[t]];
-  std::vector<double> jv2 = trips_out(C, jv, false), tv2;
-  if (!jv2.empty()) {
-    tv2.resize(nnz);
-    for (int t = 0; t < nnz; t++) tv2[t] = jv2[tsrc[t]];
-  }
-  std::string e = splu_analyse(n, tcp.data(), tri.data(), tv.data(), C->luJT.A, 1e-3, tv2.empty() ? nullptr : tv2.data());
+  const std::vector<std::vector<double>> jx = trip_matrices(C, jv, false);
+  std::vector<std::vector<double>> tx(jx.size(), std::vector<double>(nnz));
+  for (size_t k = 0; k < jx.size(); k++)
+    for (int t = 0; t < nnz; t++) tx[k][t] = jx[k][tsrc[t]];
+  std::string e = splu_analyse(n, tcp.data(), tri.data(), tv.data(), C->luJT.A, 1e-3, ptrs(tx));
   if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "transposed Jacobian analysis: " + e};
   C->luJT.tune(tsrc, jv, nullptr, sizeof(double), 1.0, C->stream, C->opt.dense_block);
   C->luJT.ready = true;
@@ -2144,6 +2202,55 @@ int gp_set_state(gp_ctx *C, const double *z0, int32_t per_scenario, const double
   return GP_OK;
 }
 
+// The event list the time loops read: the faults, then one record per line
+// trip (slot code -2 - t) and per ZIP load disconnect (kDropCode0 - load).
+static std::vector<gp_event> event_list(const std::vector<gp_event> &faults, const std::vector<gp_line_trip> &trips,
+                                        const std::vector<gp_load_drop> &drops) {
+  std::vector<gp_event> ev = faults;
+  for (size_t i = 0; i < trips.size(); i++) {
+    gp_event e{};
+    e.scenario = trips[i].scenario;
+    e.bus = -2 - (int)i;
+    e.r_fault = 1.0;
+    e.step_on = trips[i].step_trip;
+    e.step_off = trips[i].step_reclose > 0 ? trips[i].step_reclose : std::numeric_limits<int32_t>::max();
+    ev.push_back(e);
+  }
+  for (const gp_load_drop &d : drops) {
+    gp_event e{};
+    e.scenario = d.scenario;
+    e.bus = kDropCode0 - d.load;
+    e.r_fault = 1.0;
+    e.step_on = d.step;
+    e.step_off = std::numeric_limits<int32_t>::max();
+    ev.push_back(e);
+  }
+  return ev;
+}
+
+// The time loops apply a step's activations, then its deactivations, so an
+// event is in force after step k when step_on <= k and it was not switched
+// off at or after its activation by step k.  The most events in force at
+// once in any scenario (checked when the events are set, so that a sweep
+// never meets more than its MAXFAULT slots).
+static int max_in_force(const std::vector<gp_event> &ev, int M) {
+  std::vector<std::vector<const gp_event *>> by(M);
+  for (const gp_event &e : ev) by[e.scenario].push_back(&e);
+  int best = 0;
+  for (const auto &v : by)
+    for (const gp_event *a : v) {
+      const int k = a->step_on;   // the count only rises at an activation
+      int cnt = 0;
+      for (const gp_event *e : v) cnt += (e->step_on <= k && !(e->step_on <= e->step_off && e->step_off <= k)) ? 1 : 0;
+      best = std::max(best, cnt);
+    }
+  return best;
+}
+
+static bool events_fit(const std::vector<gp_event> &ev, int M) { return max_in_force(ev, M) <= MAXFAULT; }
+
+static void rebuild_events(gp_ctx *C) { C->events = event_list(C->faults_user, C->trips, C->drops); }
+
 int gp_set_events(gp_ctx *C, const gp_event *ev, int32_t n_ev, const int32_t *nsteps) {
   if (!C || (n_ev > 0 && !ev) || !nsteps) return fail(GP_ERR_BADARG, "gp_set_events: null argument");
   for (int i = 0; i < n_ev; i++) {
@@ -2156,6 +2263,9 @@ int gp_set_events(gp_ctx *C, const gp_event *ev, int32_t n_ev, const int32_t *ns
   }
   for (int m = 0; m < C->M; m++)
     if (nsteps[m] < 0) return fail(GP_ERR_BADARG, "gp_set_events: negative nsteps");
+  if (!events_fit(std::vector<gp_event>(ev, ev + n_ev), C->M))
+    return fail(GP_ERR_UNSUPPORTED, "gp_set_events: more than " + std::to_string(MAXFAULT) +
+                                        " faults in force at once in a scenario");
   C->faults_user.assign(ev, ev + n_ev);
   C->events = C->faults_user;
   if (!C->trips.empty()) C->trip_gen++;   // the LU plans' pivots were chosen with the trips
@@ -2172,29 +2282,6 @@ int gp_set_events(gp_ctx *C, const gp_event *ev, int32_t n_ev, const int32_t *ns
   return GP_OK;
 }
 
-// The event list the time loops read: the faults, then one record per line
-// trip (slot code -2 - t) and per ZIP load disconnect (kDropCode0 - load).
-static void rebuild_events(gp_ctx *C) {
-  C->events = C->faults_user;
-  for (size_t i = 0; i < C->trips.size(); i++) {
-    gp_event e{};
-    e.scenario = C->trips[i].scenario;
-    e.bus = -2 - (int)i;
-    e.r_fault = 1.0;
-    e.step_on = C->trips[i].step_trip;
-    e.step_off = C->trips[i].step_reclose > 0 ? C->trips[i].step_reclose : std::numeric_limits<int32_t>::max();
-    C->events.push_back(e);
-  }
-  for (const gp_load_drop &d : C->drops) {
-    gp_event e{};
-    e.scenario = d.scenario;
-    e.bus = kDropCode0 - d.load;
-    e.r_fault = 1.0;
-    e.step_on = d.step;
-    e.step_off = std::numeric_limits<int32_t>::max();
-    C->events.push_back(e);
-  }
-}
 
 int gp_set_load_drops(gp_ctx *C, const gp_load_drop *d, int32_t n) {
   if (!C || (n > 0 && !d) || n < 0) return fail(GP_ERR_BADARG, "gp_set_load_drops: bad argument");
@@ -2204,6 +2291,9 @@ int gp_set_load_drops(gp_ctx *C, const gp_load_drop *d, int32_t n) {
       return fail(GP_ERR_BADARG, "gp_set_load_drops: scenario or load out of range");
     if (d[i].step < 1) return fail(GP_ERR_BADARG, "gp_set_load_drops: step must be >= 1");
   }
+  if (!events_fit(event_list(C->faults_user, C->trips, std::vector<gp_load_drop>(d, d + n)), C->M))
+    return fail(GP_ERR_UNSUPPORTED, "gp_set_load_drops: more than " + std::to_string(MAXFAULT) +
+                                        " faults, tripped lines and dropped loads in force at once in a scenario");
   C->drops.assign(d, d + n);
   rebuild_events(C);
   return GP_OK;
@@ -2238,6 +2328,9 @@ int gp_set_line_trips(gp_ctx *C, const gp_line_trip *tr, int32_t n) {
                                            " and " + std::to_string(t.to_bus));
         }
   }
+  if (!events_fit(event_list(C->faults_user, std::vector<gp_line_trip>(tr, tr + n), C->drops), C->M))
+    return fail(GP_ERR_UNSUPPORTED, "gp_set_line_trips: more than " + std::to_string(MAXFAULT) +
+                                        " faults, tripped lines and dropped loads in force at once in a scenario");
   try {
     // the off-diagonal entries these trips change become per-scenario values:
     // J positions, and S slots taken out of the constants (the reduced
@@ -2590,6 +2683,29 @@ int gp_get_trajectory(gp_ctx *C, int32_t m, double *out, int64_t ld) {
       for (int k = 0; k < len; k++) out[(size_t)j * ld + k] = tmp[(size_t)j * len + k];
   } catch (const GpError &e) {
     return fail(e.code, e.msg);
+  }
+  return GP_OK;
+}
+
+int gp_analyse_pivots(int32_t n, const int32_t *colptr, const int32_t *rowind, const double *val, int32_t n_extra,
+                      const double *const *extra, double threshold, int32_t *col_order, int32_t *row_of_pivot) {
+  if (n <= 0 || !colptr || !rowind || !val || !col_order || !row_of_pivot || n_extra < 0 || (n_extra > 0 && !extra))
+    return fail(GP_ERR_BADARG, "gp_analyse_pivots: bad argument");
+  for (int c = 0; c < n; c++)
+    if (colptr[c + 1] < colptr[c]) return fail(GP_ERR_BADARG, "gp_analyse_pivots: colptr not ascending");
+  for (int p = 0; p < colptr[n]; p++)
+    if (rowind[p] < 0 || rowind[p] >= n) return fail(GP_ERR_BADARG, "gp_analyse_pivots: row index out of range");
+  try {
+    std::vector<const double *> ex(extra, extra + n_extra);
+    SpluAnalysis A;
+    const std::string e = splu_analyse(n, colptr, rowind, val, A, threshold > 0.0 ? threshold : 1e-3, ex);
+    if (!e.empty()) return fail(GP_ERR_SINGULAR_PIVOT, "gp_analyse_pivots: " + e);
+    for (int k = 0; k < n; k++) {
+      col_order[k] = A.q[k];
+      row_of_pivot[k] = A.prow[k];
+    }
+  } catch (const std::exception &e) {
+    return fail(GP_ERR_OOM, e.what());
   }
   return GP_OK;
 }

@@ -10,7 +10,7 @@ namespace gp {
 // lives at i*ldM + m, so the 64 lanes of a wavefront (64 consecutive
 // scenarios) touch 512 contiguous bytes.
 constexpr int WAVE = 64;
-constexpr int MAXFAULT = 4;   // simultaneously active faults per scenario
+constexpr int MAXFAULT = 8;   // simultaneously active events (faults, tripped lines, dropped loads) per scenario
 constexpr int kDropCode0 = -(1 << 24);   // event slot code of ZIP load 0 disconnected
 
 // Jacobian slot counts per device (order of the slot lists built on the host).
@@ -119,7 +119,7 @@ void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, 
                          double theta, int stall, int itermax, double *fbest, int *nstall, int *iters,
                          uint8_t *failed, const uint8_t *pend, uint8_t *upd, double async_below, uint8_t *areq,
                          uint8_t *sch, hipStream_t s, const uint8_t *firstv = nullptr,
-                         const uint8_t *exact = nullptr);
+                         const uint8_t *exact = nullptr, int switch_window = 0);
 void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs,
                      double *jv, const uint8_t *work, hipStream_t s, const double *dtv = nullptr);
 template <typename TF>

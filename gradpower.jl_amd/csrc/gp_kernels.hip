@@ -476,7 +476,10 @@ __global__ void __launch_bounds__(256) k_rownorm(int n, int ldM, int M, const do
 // kernels run (corrections and new asynchronous requests).  iters[m] counts
 // corrections; first (per scenario with firstv) resets the per-solve history.
 // exact[m] != 0 (optional) runs scenario m's solve as exact Newton: a
-// synchronous factorisation at every iteration, whatever theta says.
+// factorisation at every iteration, whatever theta says (asynchronous ones
+// are waited for, as above).  switch_window > 0: a scenario not in exact
+// mode whose |F| has not halved within that many corrections stops as failed
+// too (the host then runs its solve again as exact Newton).
 // n_unconv: [0] working scenarios, [1] synchronous requests, [2]
 // corrections, [3] asynchronous requests.
 __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict__ nb, double *__restrict__ fnorm,
@@ -486,7 +489,7 @@ __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict_
                               double *__restrict__ fbest, int *__restrict__ nstall, int *__restrict__ iters,
                               uint8_t *__restrict__ failed, const uint8_t *__restrict__ pend,
                               uint8_t *__restrict__ upd, double async_below, uint8_t *__restrict__ areq,
-                              uint8_t *__restrict__ sch, const uint8_t *__restrict__ exact) {
+                              uint8_t *__restrict__ sch, const uint8_t *__restrict__ exact, int switch_window) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
   const bool ex = exact && exact[m];
@@ -508,6 +511,7 @@ __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict_
   }
   const int it = first ? 0 : iters[m];
   bool fail = it >= itermax;
+  if (!ex && switch_window > 0) stall = stall > 0 ? min(stall, switch_window) : switch_window;
   if (stall > 0) {
     double best = first ? 1e300 : fbest[m];
     int ns = first ? 0 : nstall[m];
@@ -536,7 +540,7 @@ __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict_
   // a change of dt) does not wait for one asked for under the old system:
   // the host lands and drops that request before this round's factorisation
   bool wait = pending && !(first && rf);
-  if (rf && !first && !ex && async_below > 0.0 && v < async_below) {
+  if (rf && !first && async_below > 0.0 && v < async_below) {
     areq[m] = 1;
     atomicAdd(n_unconv + 3, 1);
     wait = true;
@@ -2456,12 +2460,13 @@ void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, 
                          unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
                          double theta, int stall, int itermax, double *fbest, int *nstall, int *iters,
                          uint8_t *failed, const uint8_t *pend, uint8_t *upd, double async_below, uint8_t *areq,
-                         uint8_t *sch, hipStream_t s, const uint8_t *firstv, const uint8_t *exact) {
+                         uint8_t *sch, hipStream_t s, const uint8_t *firstv, const uint8_t *exact,
+                         int switch_window) {
   const int chunks = std::max(1, std::min((n + 15) / 16, 1024 / (ldM / WAVE)));
   hipLaunchKernelGGL(k_rownorm, dim3(ldM / WAVE, chunks), dim3(256), 0, s, n, ldM, M, f, work, nb);
   hipLaunchKernelGGL(k_norm_finish, dim3((M + 255) / 256), dim3(256), 0, s, M, tol, nb, fnorm, work, n_unconv, refac,
                      refac0, first, firstv, theta, stall, itermax, fbest, nstall, iters, failed, pend, upd,
-                     async_below, areq, sch, exact);
+                     async_below, areq, sch, exact, switch_window);
 }
 void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs, double *jv,
                      const uint8_t *work, hipStream_t s, const double *dtv) {

@@ -57,7 +57,7 @@ std::vector<int> min_degree_order(int n, const int *cp, const int *ri) {
 }
 
 std::string splu_analyse(int n, const int *cp, const int *ri, const double *ax, SpluAnalysis &A,
-                         double thr, const double *ax2) {
+                         double thr, const std::vector<const double *> &extra) {
   A = SpluAnalysis();
   A.n = n;
   A.nnz_a = cp[n];
@@ -67,11 +67,15 @@ std::string splu_analyse(int n, const int *cp, const int *ri, const double *ax, 
   for (int k = 0; k < n; k++) A.qinv[A.q[k]] = k;
 
   // ---- numeric left-looking LU with threshold partial pivoting ----------
+  const int ne = (int)extra.size();
   std::vector<int> pinv(n, -1), prow(n, -1);
   std::vector<std::vector<std::pair<int, double>>> Lcol(n), Ucol(n);  // L: original rows; U: pivot rows
-  std::vector<double> x(n, 0.0), x2(ax2 ? n : 0, 0.0);
-  std::vector<std::vector<double>> L2(ax2 ? n : 0);   // second matrix's L values, parallel to Lcol
-  bool dual = ax2 != nullptr;
+  std::vector<double> x(n, 0.0);
+  // the extra matrices: column work vectors and L values parallel to Lcol
+  std::vector<std::vector<double>> xe(ne, std::vector<double>(n, 0.0));
+  std::vector<std::vector<std::vector<double>>> Le(ne, std::vector<std::vector<double>>(n));
+  std::vector<uint8_t> alive(ne, 1), inrule(ne, 0);
+  std::vector<double> amaxe(ne, 0.0);
   std::vector<int> xi(n), mark(n, -1), stk(n), pst(n);
   for (int k = 0; k < n; k++) {
     int col = A.q[k];
@@ -114,65 +118,96 @@ std::string splu_analyse(int n, const int *cp, const int *ri, const double *ax, 
       double xj = x[j];
       for (auto &e : Lcol[jp]) x[e.first] -= e.second * xj;
     }
-    if (dual) {   // the same column of the second matrix, same pivots so far
-      for (int p = top; p < n; p++) x2[xi[p]] = 0.0;
-      for (int p = cp[col]; p < cp[col + 1]; p++) x2[ri[p]] = ax2[p];
+    for (int e = 0; e < ne; e++) {   // the same column of each extra matrix, same pivots so far
+      if (!alive[e]) continue;
+      std::vector<double> &y = xe[e];
+      for (int p = top; p < n; p++) y[xi[p]] = 0.0;
+      for (int p = cp[col]; p < cp[col + 1]; p++) y[ri[p]] = extra[e][p];
       for (int p = top; p < n; p++) {
         int j = xi[p], jp = pinv[j];
         if (jp < 0) continue;
-        double xj = x2[j];
+        double yj = y[j];
         const auto &lc = Lcol[jp];
-        const auto &lv = L2[jp];
-        for (size_t e = 0; e < lc.size(); e++) x2[lc[e].first] -= lv[e] * xj;
+        const auto &lv = Le[e][jp];
+        for (size_t q = 0; q < lc.size(); q++) y[lc[q].first] -= lv[q] * yj;
       }
     }
     int ipiv = -1;
-    double amax = -1.0, amax2 = 0.0;
+    double amax = -1.0;
+    for (int e = 0; e < ne; e++) amaxe[e] = 0.0;
     for (int p = top; p < n; p++) {
       int i = xi[p];
       if (pinv[i] < 0) {
         double t = std::fabs(x[i]);
         if (t > amax) { amax = t; ipiv = i; }
-        if (dual) amax2 = std::max(amax2, std::fabs(x2[i]));
+        for (int e = 0; e < ne; e++)
+          if (alive[e]) amaxe[e] = std::max(amaxe[e], std::fabs(xe[e][i]));
       } else {
         Ucol[k].push_back({pinv[i], x[i]});
       }
     }
     if (ipiv < 0 || !(amax > 0.0)) return "structurally or numerically singular matrix at pivot " + std::to_string(k);
-    bool both = dual && amax2 > 0.0;
-    if (both) {
-      // a pivot that is large in both matrices: the largest min(|x|/amax, |x2|/amax2)
-      double best = 0.0;
-      int ib = -1;
+    int nin = 0;
+    for (int e = 0; e < ne; e++) {
+      inrule[e] = alive[e] && amaxe[e] > 0.0;
+      nin += inrule[e];
+    }
+    if (nin == 0) {
+      if (pinv[col] < 0 && std::fabs(x[col]) >= thr * amax) ipiv = col;
+    } else {
+      // the measure of a candidate: min over the matrices in the rule of
+      // |x|/max|x|, and (ties, all zero) how many of them it is nonzero in
+      auto score = [&](int i, int &cnt) {
+        double sc = std::fabs(x[i]) / amax;
+        cnt = 0;
+        for (int e = 0; e < ne; e++) {
+          if (!inrule[e]) continue;
+          const double v = std::fabs(xe[e][i]) / amaxe[e];
+          cnt += v > 0.0;
+          sc = std::min(sc, v);
+        }
+        return sc;
+      };
+      double best = -1.0;
+      int bcnt = -1, ib = -1;
       for (int p = top; p < n; p++) {
         int i = xi[p];
-        if (pinv[i] >= 0) continue;
-        double sc = std::min(std::fabs(x[i]) / amax, std::fabs(x2[i]) / amax2);
-        if (sc > best) { best = sc; ib = i; }
+        if (pinv[i] >= 0 || x[i] == 0.0) continue;
+        int cnt;
+        const double sc = score(i, cnt);
+        if (cnt > bcnt || (cnt == bcnt && sc > best)) {
+          best = sc;
+          bcnt = cnt;
+          ib = i;
+        }
       }
-      if (ib < 0) both = false;   // no row is nonzero in both: the first matrix decides from here on
-      else {
-        ipiv = ib;
-        if (pinv[col] < 0 && std::min(std::fabs(x[col]) / amax, std::fabs(x2[col]) / amax2) >= thr) ipiv = col;
+      if (ib >= 0) ipiv = ib;
+      if (pinv[col] < 0 && x[col] != 0.0) {
+        int cnt;
+        const double sc = score(col, cnt);
+        if (cnt == bcnt && sc >= thr) ipiv = col;
       }
-    }
-    if (!both) {
-      dual = false;
-      if (pinv[col] < 0 && std::fabs(x[col]) >= thr * amax) ipiv = col;
     }
     double piv = x[ipiv];
-    double piv2 = dual ? x2[ipiv] : 1.0;
     pinv[ipiv] = k;
     prow[k] = ipiv;
+    for (int e = 0; e < ne; e++)
+      if (alive[e] && xe[e][ipiv] == 0.0) alive[e] = 0;   // its elimination cannot go on
     for (int p = top; p < n; p++) {
       int i = xi[p];
       if (pinv[i] < 0) {
         Lcol[k].push_back({i, x[i] / piv});
-        if (dual) L2[k].push_back(x2[i] / piv2);
+        for (int e = 0; e < ne; e++)
+          if (alive[e]) Le[e][k].push_back(xe[e][i] / xe[e][ipiv]);
       }
       x[i] = 0.0;
-      if (dual) x2[i] = 0.0;
     }
+    for (int e = 0; e < ne; e++)
+      if (alive[e])
+        for (int p = top; p < n; p++) xe[e][xi[p]] = 0.0;
+    // Le[e][k] of a matrix that left the rule for good stays empty: never read
+    for (int e = 0; e < ne; e++)
+      if (!alive[e]) Le[e][k].clear();
   }
   A.pinv = pinv;
   A.prow = prow;

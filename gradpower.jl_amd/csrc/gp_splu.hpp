@@ -95,18 +95,21 @@ struct SpluAnalysis {
 // dense trailing block of at most nd_max pivots.
 void splu_schedule(SpluAnalysis &A, int threads, int nd_max = 112);
 
-// Analyse the CSC matrix (cp, ri, representative values ax).  With ax2 (a
-// second representative matrix of the same pattern, or NULL), each pivot is
-// chosen large in both: the largest min(|x|/max|x|, |x2|/max|x2|) over the
-// candidates, the diagonal when that measure reaches the threshold -- the
-// line trips make the second matrix (every tripped line out at once), so
-// no pivot relies on an entry that a trip, or the fill it feeds, makes zero.
-// Where no candidate is nonzero in both (the trips island a part of the
-// network), the first matrix alone decides from that column on.  Returns an
-// empty string on success, otherwise an error message.
+// Analyse the CSC matrix (cp, ri, representative values ax).  With extra
+// representative matrices of the same pattern (the line trips make one per
+// distinct set of lines out at once), each pivot is chosen large in all of
+// them: the largest min over the matrices of |x|/max|x| among the
+// candidates, the diagonal when that measure reaches the threshold -- so no
+// pivot relies on an entry that a trip, or the fill it feeds, makes zero.
+// A matrix in which no candidate of a column is nonzero (a trip set that
+// islands part of the network) leaves the rule for that column; one whose
+// chosen pivot is zero leaves it for good (its elimination cannot go on).
+// Where no candidate is nonzero in every matrix still in the rule, the
+// candidate nonzero in the most of them is taken.  Returns an empty string
+// on success, otherwise an error message.
 std::string splu_analyse(int n, const int *cp, const int *ri, const double *ax,
                          SpluAnalysis &out, double pivot_threshold = 1e-3,
-                         const double *ax2 = nullptr);
+                         const std::vector<const double *> &extra = {});
 
 // Plan of the LDS-resident factorisation.  Every L entry of the columns
 // below the dense block (B = n - nd) that an update reads is kept in LDS

@@ -36,7 +36,7 @@ class COptions(C.Structure):
     _fields_ = [("record_all", C.c_int32), ("n_record", C.c_int32),
                 ("record_idx", C.POINTER(C.c_int32)), ("refactor_policy", C.c_int32),
                 ("stall_window", C.c_int32), ("profile", C.c_int32), ("lock_step", C.c_int32),
-                ("dense_block", C.c_int32), ("reserved", C.c_int32 * 2),
+                ("dense_block", C.c_int32), ("retry_window", C.c_int32), ("reserved", C.c_int32 * 1),
                 ("refactor_theta", C.c_double),
                 ("async_below", C.c_double)]
 
@@ -82,7 +82,7 @@ EXPORTS = ["gp_create", "gp_destroy", "gp_set_state", "gp_set_events", "gp_set_l
            "gp_integrate",
            "gp_get_trajectory", "gp_get_state", "gp_eval_residual", "gp_jacobian_pattern",
            "gp_eval_jacobian", "gp_newton_step", "gp_get_symbolic_info", "gp_get_scenario_status",
-           "gp_copy_kernel", "gp_adjoint", "gp_last_error", "gp_version"]
+           "gp_copy_kernel", "gp_adjoint", "gp_analyse_pivots", "gp_last_error", "gp_version"]
 
 
 def load():
@@ -113,6 +113,7 @@ def load():
     lib.gp_newton_step.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, C.c_double, _up, _dp]
     lib.gp_adjoint.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                C.c_void_p, C.c_int32, _ip, _dp, _dp, _dp, _dp, C.c_void_p]
+    lib.gp_analyse_pivots.argtypes = [C.c_int32, _ip, _ip, _dp, C.c_int32, C.c_void_p, C.c_double, _ip, _ip]
     lib.gp_last_error.restype = C.c_char_p
     lib.gp_version.restype = C.c_char_p
     _LIB = lib
@@ -136,7 +137,7 @@ class HipContext:
 
     def __init__(self, desc: SystemDesc, M: int, record_idx=None,
                  refactor_policy=REFACTOR_EVERY_ITER, stall_window=0, refactor_theta=0.0, async_below=0.0,
-                 profile=False, lock_step=False, dense_block=0):
+                 profile=False, lock_step=False, dense_block=0, retry_window=0):
         self.lib = load()
         self.desc = desc
         self.M = int(M)
@@ -159,6 +160,7 @@ class HipContext:
         opt.profile = 1 if profile else 0
         opt.lock_step = 1 if lock_step else 0
         opt.dense_block = int(dense_block)
+        opt.retry_window = int(retry_window)
         self._opt = opt
         h = C.c_void_p()
         self._check(self.lib.gp_create(C.byref(self._cs), self.M, C.byref(opt), C.byref(h)))
@@ -319,3 +321,22 @@ class HipContext:
             raise GpError(rc, self.lib.gp_last_error().decode(), stats)
         return {"value": float(val.sum()), "value_scen": val, "grad": grad[:len(par)],
                 "grad_scen": gs[:, :len(par)], "dz0": dz0, "stats": stats, "status": rc}
+
+
+def analyse_pivots(cp, ri, val, extra=(), threshold=1e-3):
+    """Column order and static pivot rows of the library's LU analysis
+    (gp_analyse_pivots; host only, no GPU): returns (q, prow) with column
+    q[k] and row prow[k] at pivot k."""
+    lib = load()
+    cp = np.ascontiguousarray(cp, np.int32)
+    ri = np.ascontiguousarray(ri, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    n = len(cp) - 1
+    ex = [np.ascontiguousarray(e, np.float64) for e in extra]
+    arr = (C.c_void_p * max(1, len(ex)))(*[e.ctypes.data for e in ex])
+    q = np.zeros(n, np.int32)
+    prow = np.zeros(n, np.int32)
+    rc = lib.gp_analyse_pivots(n, _i(cp), _i(ri), _d(val), len(ex), arr, float(threshold), _i(q), _i(prow))
+    if rc != GP_OK:
+        raise GpError(rc, lib.gp_last_error().decode())
+    return q, prow

@@ -193,7 +193,15 @@ typedef struct gp_options {
                               (a timing: contexts may choose differently and then
                               round differently); -1 the schedule's cost model
                               alone (reproducible); k > 0 at most k rows       */
-  int32_t reserved[2];
+  int32_t retry_window;    /* an inexact Newton solve (factorisations reused, or
+                              FP32 factors) whose |F|inf has not halved within
+                              this many corrections, or that reaches itermax,
+                              restarts from the step's initial iterate as
+                              exact Newton with FP64 factors (the reference's
+                              method, run to itermax) before its scenario
+                              counts as failed: 0 (default) 8, -1 at itermax
+                              only                                              */
+  int32_t reserved[1];
   double refactor_theta;   /* ADAPTIVE / ASYNC: refactorise when an iteration
                               reduces |F|inf by less than this factor (0: 30)   */
   double async_below;      /* ASYNC: a refactorisation asked for at |F|inf below
@@ -247,20 +255,27 @@ int gp_set_state(gp_ctx *ctx, const double *z0, int32_t per_scenario,
                  const double *p);
 
 /* Per-scenario faults and horizons (nsteps[m] = round(tf_m/dt)).  Clears
- * the line trips of gp_set_line_trips. */
+ * the line trips of gp_set_line_trips.  At most 8 events in force at once
+ * per scenario (GP_ERR_UNSUPPORTED otherwise). */
 int gp_set_events(gp_ctx *ctx, const gp_event *ev, int32_t n_ev,
                   const int32_t *nsteps);
 
 /* Per-scenario line trips and reclosures, beside the faults of the last
- * gp_set_events (call it after that).  Faults and tripped lines share the
- * limit of 4 simultaneously active events per scenario.  The reference runs
- * these events in its CPU integrator only (src/dynamics.jl:502-655); here
- * every method takes them, in the forward sweep and in gp_adjoint. */
+ * gp_set_events (call it after that).  Faults, tripped lines and dropped
+ * loads share 8 event slots per scenario: a set of events that would have
+ * more than 8 in force at once in a scenario is refused here
+ * (GP_ERR_UNSUPPORTED), before any integration.  The LU plans choose pivots
+ * large in the intact matrix and in the matrix of every distinct set of
+ * lines out (each tripped line alone, and all of a scenario's at once).
+ * The reference runs these events in its CPU integrator only
+ * (src/dynamics.jl:502-655); here every method takes them, in the forward
+ * sweep and in gp_adjoint. */
 int gp_set_line_trips(gp_ctx *ctx, const gp_line_trip *trips, int32_t n_trips);
 
 /* Per-scenario ZIP load disconnects (call after gp_set_events, which clears
- * them).  They share the 4 event slots of a scenario with its active faults
- * and tripped lines, and hold a slot from their step on. */
+ * them).  They share the 8 event slots of a scenario with its active faults
+ * and tripped lines, hold a slot from their step on, and are refused here
+ * when a scenario would have more than 8 events in force at once. */
 int gp_set_load_drops(gp_ctx *ctx, const gp_load_drop *drops, int32_t n_drops);
 
 /* Integrate all scenarios from the current state to their horizons.
@@ -367,6 +382,19 @@ typedef struct gp_symbolic_info {
   int32_t reserved[1];
 } gp_symbolic_info;
 int gp_get_symbolic_info(gp_ctx *ctx, gp_symbolic_info *info);
+
+/* Pivot analysis of the batched LU plans, on the host (no GPU): the column
+ * order (minimum degree on A+A^T) and the static pivot row of every step,
+ * chosen by threshold partial pivoting (KLU's rule, diagonal preferred
+ * within `threshold`, the reference's klu(J), src/dynamics.jl:576-580) on
+ * the CSC matrix (colptr, rowind, val) and, with n_extra > 0, large in every
+ * extra representative matrix of the same pattern as well (the library
+ * passes one per distinct set of tripped lines).  col_order[k] is the column
+ * and row_of_pivot[k] the row of pivot k, 0-based.  Returns
+ * GP_ERR_SINGULAR_PIVOT when the matrix has no nonzero candidate at a step. */
+int gp_analyse_pivots(int32_t n, const int32_t *colptr, const int32_t *rowind, const double *val,
+                      int32_t n_extra, const double *const *extra, double threshold, int32_t *col_order,
+                      int32_t *row_of_pivot);
 
 /* Measurement helper with no counterpart in the reference: a plain device
  * copy kernel dst[i] = src[i], i < n (device pointers), on the context's
