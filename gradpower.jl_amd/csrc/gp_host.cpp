@@ -401,6 +401,12 @@ struct gp_ctx {
   DBuf<double> d_yv, d_u0, d_jp_yoff_val, d_ydiag, d_s_cst;
   DBuf<int> d_tr_bus, d_tv_p, d_tv_i;
   DBuf<double> d_tr_y, d_tv_base;
+  // every scenario's events in slot form ([MAXFAULT][ldM], the first
+  // MAXFAULT events of each scenario, as the parity hooks apply them) and
+  // the GPU's allocation-free scratch array (gp_eval_residual_noalloc)
+  DBuf<int> d_evbus;
+  DBuf<double> d_evg;
+  void *scratch = nullptr;
   DBuf<double> d_p, d_z, d_zold, d_f, d_inj, d_jv, d_svar, d_rhs, d_dv, d_Dg, d_fnorm, d_hist, d_luJ, d_luS64;
   DBuf<double> d_svJ, d_svS64, d_dnJ, d_dnS64;
   DBuf<float> d_luS32, d_svS32, d_dnS32, d_X32;
@@ -2086,6 +2092,35 @@ struct PolicyGuard {
 
 }  // namespace
 
+// The parity hooks' fault tables: scenario m's first MAXFAULT events where
+// fault_on[m] != 0 (all scenarios' with all_on).
+static void eval_fault_tables(const gp_ctx *C, const uint8_t *fault_on, bool all_on, std::vector<int> &fbus,
+                              std::vector<double> &fg) {
+  const int L = C->ldM;
+  fbus.assign((size_t)MAXFAULT * L, -1);
+  fg.assign((size_t)MAXFAULT * L, 0.0);
+  for (int m = 0; m < C->M; m++) {
+    if (!all_on && !(fault_on && fault_on[m])) continue;
+    int q = 0;
+    for (auto &e : C->events)
+      if (e.scenario == m && q < MAXFAULT) {
+        fbus[(size_t)q * L + m] = e.bus;
+        fg[(size_t)q * L + m] = 1.0 / e.r_fault;
+        q++;
+      }
+  }
+}
+
+// Every scenario's events in slot form on the device (after any change of
+// the event tables), for gp_eval_residual_noalloc.
+static void upload_event_slots(gp_ctx *C) {
+  std::vector<int> fbus;
+  std::vector<double> fg;
+  eval_fault_tables(C, nullptr, true, fbus, fg);
+  C->d_evbus.upload(fbus);
+  C->d_evg.upload(fg);
+}
+
 // ================================================================= C ABI
 extern "C" {
 
@@ -2151,6 +2186,8 @@ int gp_create(const gp_system_desc *d, int32_t M, const gp_options *opt, gp_ctx 
       up(C->d_rec_idx, C->rec_idx);
     }
     C->nsteps.assign(M, 0);
+    upload_event_slots(C.get());
+    if (scratch_setup(C->device, &C->scratch) != 0) C->scratch = nullptr;   // the noalloc entry point refuses
   } catch (const GpError &e) {
     return fail(e.code, e.msg);
   } catch (const std::exception &e) {
@@ -2276,6 +2313,7 @@ int gp_set_events(gp_ctx *C, const gp_event *ev, int32_t n_ev, const int32_t *ns
   C->nsteps.assign(nsteps, nsteps + C->M);
   try {
     upload_trip_tables(C);
+    upload_event_slots(C);
   } catch (const GpError &e) {
     return fail(e.code, e.msg);
   }
@@ -2296,6 +2334,11 @@ int gp_set_load_drops(gp_ctx *C, const gp_load_drop *d, int32_t n) {
                                         " faults, tripped lines and dropped loads in force at once in a scenario");
   C->drops.assign(d, d + n);
   rebuild_events(C);
+  try {
+    upload_event_slots(C);
+  } catch (const GpError &e) {
+    return fail(e.code, e.msg);
+  }
   return GP_OK;
 }
 
@@ -2386,6 +2429,7 @@ int gp_set_line_trips(gp_ctx *C, const gp_line_trip *tr, int32_t n) {
     }
     rebuild_events(C);
     upload_trip_tables(C);
+    upload_event_slots(C);
   } catch (const GpError &e) {
     return fail(e.code, e.msg);
   }
@@ -2752,21 +2796,40 @@ static void load_eval_state(gp_ctx *C, const double *z, const uint8_t *fault_on)
   for (int m = 0; m < C->M; m++)
     for (int i = 0; i < C->n; i++) zt[(size_t)i * L + m] = z[(size_t)m * C->n + i];
   HIPCHK(hipMemcpy(C->d_z.p, zt.data(), zt.size() * sizeof(double), hipMemcpyHostToDevice));
-  std::vector<int> fbus((size_t)MAXFAULT * L, -1);
-  std::vector<double> fg((size_t)MAXFAULT * L, 0.0);
-  if (fault_on)
-    for (int m = 0; m < C->M; m++) {
-      if (!fault_on[m]) continue;
-      int q = 0;
-      for (auto &e : C->events)
-        if (e.scenario == m && q < MAXFAULT) {
-          fbus[(size_t)q * L + m] = e.bus;
-          fg[(size_t)q * L + m] = 1.0 / e.r_fault;
-          q++;
-        }
-    }
+  std::vector<int> fbus;
+  std::vector<double> fg;
+  eval_fault_tables(C, fault_on, false, fbus, fg);
   HIPCHK(hipMemcpy(C->d_fbus.p, fbus.data(), fbus.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(C->d_fg.p, fg.data(), fg.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+
+int gp_eval_residual_noalloc(gp_ctx *C, const double *z, const double *zold, double dt, const uint8_t *fault_on,
+                             double *f, void *stream) {
+  if (!C || !z || !zold || !f) return fail(GP_ERR_BADARG, "gp_eval_residual_noalloc: null argument");
+  if (!C->state_set) return fail(GP_ERR_BADARG, "gp_eval_residual_noalloc: gp_set_state was not called");
+  if (!C->scratch) return fail(GP_ERR_HIP, "gp_eval_residual_noalloc: no scratch array on this device");
+  const size_t L = (size_t)C->ldM;
+  auto up256 = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t o_fb = up256(L), o_fg = o_fb + up256((size_t)MAXFAULT * L * sizeof(int)),
+               o_inj = o_fg + up256((size_t)MAXFAULT * L * sizeof(double)),
+               need = o_inj + up256((size_t)std::max(1, 2 * C->ncl) * L * sizeof(double));
+  if (need > (size_t)GP_NOALLOC_SCRATCH_BYTES)
+    return fail(GP_ERR_UNSUPPORTED, "gp_eval_residual_noalloc: this context needs " + std::to_string(need) +
+                                        " bytes of scratch, the maximum is " +
+                                        std::to_string((size_t)GP_NOALLOC_SCRATCH_BYTES));
+  uint8_t *base = static_cast<uint8_t *>(C->scratch);
+  uint8_t *mask = base;
+  int *fbus = reinterpret_cast<int *>(base + o_fb);
+  double *fg = reinterpret_cast<double *>(base + o_fg);
+  double *inj = reinterpret_cast<double *>(base + o_inj);
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : C->stream;
+  launch_prefix_mask((int)L, C->M, mask, s);
+  launch_select_faults((int)L, C->d_evbus.p, C->d_evg.p, fault_on, fbus, fg, s);
+  launch_cluster_residual(C->T, C->d_p.p, z, zold, dt, f, inj, mask, s);
+  launch_bus_residual(C->T, C->d_p.p, z, inj, FaultState{fbus, fg}, f, mask, s);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(GP_ERR_HIP, std::string("gp_eval_residual_noalloc: ") + hipGetErrorString(e));
+  return GP_OK;
 }
 
 int gp_eval_residual(gp_ctx *C, const double *z, const double *zold, double dt, const uint8_t *fault_on, double *f) {
@@ -2888,6 +2951,7 @@ int gp_get_symbolic_info(gp_ctx *C, gp_symbolic_info *info) {
   I.nd_jt = C->luJT.ready ? C->luJT.A.nd : -1;
   I.res_j = C->luJ.ready && C->luJ.dev.res_nsteps > 0;
   I.res_s = C->luS.ready && C->luS.dev.res_nsteps > 0;
+  I.ld_scen = C->ldM;
   *info = I;
   return GP_OK;
 }

@@ -165,6 +165,13 @@ void launch_adj_param(const DevTables &T, const double *p, int ne, const int2 *e
                       const double *lam, double dt, double *g, const uint8_t *act, hipStream_t s);
 void launch_gather_traj(int nrec, int len, int ldM, int m, const double *hist, double *out, hipStream_t s);
 void launch_copy_plain(const double *src, double *dst, long long n, hipStream_t s);
+// allocation-free residual: the address of this GPU's scratch array (taken
+// once per device, that device current; 0 or an error code), the all-M
+// scenario mask and the selected fault tables
+int scratch_setup(int device, void **out);
+void launch_prefix_mask(int ldM, int M, uint8_t *mask, hipStream_t s);
+void launch_select_faults(int ldM, const int *ev_bus, const double *ev_g, const uint8_t *fault_on, int *bus,
+                          double *g, hipStream_t s);
 void launch_record_impl(int nrec, const int *idx, int ldM, int M, const double *z, double *hist,
                         const uint8_t *act, hipStream_t s);
 void launch_record_at(int nrec, const int *idx, int ldM, int M, const double *z, double *hist, const int *col,

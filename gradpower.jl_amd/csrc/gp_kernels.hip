@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <mutex>
 
 #include "../../include/gradpower_hip.h"
 #include "gp_internal.hpp"
@@ -2582,6 +2583,61 @@ void launch_record_impl(int nrec, const int *idx, int ldM, int M, const double *
                         hipStream_t s) {
   const int gx = nrec < 1024 ? (nrec > 0 ? nrec : 1) : 1024;
   hipLaunchKernelGGL(k_record, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, nrec, idx, ldM, M, z, hist, act);
+}
+
+// ---------------------------------------------------------------- allocation-free residual
+// Device memory of gp_eval_residual_noalloc: the scenario mask, the fault
+// tables and the generators' current injections of one evaluation.  One copy
+// per GPU, held until the process ends; every call on a GPU shares it (calls
+// that overlap on one GPU race).  Host code reaches it only through the
+// address scratch_setup() takes once per device.
+static __device__ __align__(256) unsigned char scratch[GP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int scratch_setup(int device, void **out) {
+  if (device < 0 || device >= kMaxDevices) return -1;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return e != hipSuccess ? (int)e : -1;
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return 0;
+}
+
+// mask[m] = (m < M), m < ldM
+__global__ void k_prefix_mask(int ldM, int M, uint8_t *__restrict__ mask) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < ldM) mask[m] = m < M ? 1 : 0;
+}
+
+// The fault tables of an evaluation: scenario m's events (ev_bus / ev_g,
+// [MAXFAULT][ldM]) where fault_on[m] != 0 (fault_on NULL: none), empty slots
+// elsewhere.
+__global__ void k_select_faults(int ldM, const int *__restrict__ ev_bus, const double *__restrict__ ev_g,
+                                const uint8_t *__restrict__ fault_on, int *__restrict__ bus, double *__restrict__ g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= MAXFAULT * ldM) return;
+  const int m = i % ldM;
+  const bool on = fault_on && fault_on[m];
+  bus[i] = on ? ev_bus[i] : -1;
+  g[i] = on ? ev_g[i] : 0.0;
+}
+
+void launch_prefix_mask(int ldM, int M, uint8_t *mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_prefix_mask, dim3((ldM + 255) / 256), dim3(256), 0, s, ldM, M, mask);
+}
+void launch_select_faults(int ldM, const int *ev_bus, const double *ev_g, const uint8_t *fault_on, int *bus,
+                          double *g, hipStream_t s) {
+  hipLaunchKernelGGL(k_select_faults, dim3((MAXFAULT * ldM + 255) / 256), dim3(256), 0, s, ldM, ev_bus, ev_g,
+                     fault_on, bus, g);
 }
 
 template void launch_cluster_schur<double>(const DevTables &, const double *, const double *, const double *, double,

@@ -59,7 +59,7 @@ class CSymInfo(C.Structure):
                 ("levels_j", C.c_int32), ("n_red", C.c_int32), ("nnz_s", C.c_int32),
                 ("nnz_lu_s", C.c_int32), ("levels_s", C.c_int32), ("n_clusters_nt", C.c_int32),
                 ("w_total_nt", C.c_int32), ("nd_j", C.c_int32), ("nd_s", C.c_int32), ("nd_jt", C.c_int32),
-                ("res_j", C.c_int32), ("res_s", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("res_j", C.c_int32), ("res_s", C.c_int32), ("ld_scen", C.c_int32)]
 
 
 class CObjective(C.Structure):
@@ -82,7 +82,8 @@ EXPORTS = ["gp_create", "gp_destroy", "gp_set_state", "gp_set_events", "gp_set_l
            "gp_integrate",
            "gp_get_trajectory", "gp_get_state", "gp_eval_residual", "gp_jacobian_pattern",
            "gp_eval_jacobian", "gp_newton_step", "gp_get_symbolic_info", "gp_get_scenario_status",
-           "gp_copy_kernel", "gp_adjoint", "gp_analyse_pivots", "gp_last_error", "gp_version"]
+           "gp_copy_kernel", "gp_adjoint", "gp_analyse_pivots", "gp_eval_residual_noalloc", "gp_last_error",
+           "gp_version"]
 
 
 def load():
@@ -114,6 +115,8 @@ def load():
     lib.gp_adjoint.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                C.c_void_p, C.c_int32, _ip, _dp, _dp, _dp, _dp, C.c_void_p]
     lib.gp_analyse_pivots.argtypes = [C.c_int32, _ip, _ip, _dp, C.c_int32, C.c_void_p, C.c_double, _ip, _ip]
+    lib.gp_eval_residual_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                             C.c_void_p, C.c_void_p]
     lib.gp_last_error.restype = C.c_char_p
     lib.gp_version.restype = C.c_char_p
     _LIB = lib

@@ -306,6 +306,24 @@ int gp_get_state(gp_ctx *ctx, double *out);
 int gp_eval_residual(gp_ctx *ctx, const double *z, const double *zold,
                      double dt, const uint8_t *fault_on, double *f);
 
+/* Allocation-free residual.  No counterpart in the reference: for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops).
+ * The same residual as gp_eval_residual, but every array is in device
+ * memory in the library's layout -- row i of scenario m at i*ld + m, with
+ * ld = gp_symbolic_info.ld_scen (M rounded up to 64) -- and the call only
+ * launches kernels on `stream` (NULL: the context's stream) and returns: no
+ * allocation, copy or synchronisation.  z, zold: [n*ld] inputs; f: [n*ld]
+ * output (rows of scenarios m >= M untouched); fault_on: [ld] device flags
+ * or NULL, with gp_eval_residual's meaning.  The library's device memory
+ * for the call is a scratch array of GP_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends and shared by every call on that GPU: calls that
+ * overlap on one GPU race.  A context whose evaluation needs more
+ * (ld + 12*MAXFAULT*ld + 16*n_clusters*ld bytes, 256-byte aligned parts) is
+ * refused with GP_ERR_UNSUPPORTED.  Needs gp_set_state (the parameters). */
+#define GP_NOALLOC_SCRATCH_BYTES (4u << 20)
+int gp_eval_residual_noalloc(gp_ctx *ctx, const double *z, const double *zold, double dt,
+                             const uint8_t *fault_on, double *f, void *stream);
+
 /* Jacobian values of the backward-Euler system at z, in the library's CSR
  * pattern (gp_jacobian_pattern), out [M*nnz]. */
 int gp_jacobian_pattern(gp_ctx *ctx, int32_t *nnz, int32_t *rowptr,
@@ -379,7 +397,7 @@ typedef struct gp_symbolic_info {
   int32_t nd_j, nd_s, nd_jt;   /* dense trailing block of each LU plan (autotuned) */
   int32_t res_j, res_s;        /* 1: the plan's factorisation keeps its multipliers in LDS
                                   (at the element size it was last shaped for)       */
-  int32_t reserved[1];
+  int32_t ld_scen;             /* row stride of the device arrays: M rounded up to 64 */
 } gp_symbolic_info;
 int gp_get_symbolic_info(gp_ctx *ctx, gp_symbolic_info *info);
 
