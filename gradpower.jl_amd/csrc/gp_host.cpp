@@ -421,7 +421,11 @@ struct gp_ctx {
   DBuf<uint8_t> d_batch;
   HBuf h_batch;
   int batch_slot = 0;
-  static constexpr int kLanes = 1;   // measured at C2: a second concurrent batch saves rounds but slows them as much
+  // lane 0 factorises in the method's precision (measured at C2: a second
+  // concurrent lane of those batches saves rounds but slows them as much);
+  // lane 1 takes the FP32 method's FP64 factorisations of the scenarios
+  // running exact Newton, so that they do not queue behind lane 0's batches
+  static constexpr int kLanes = 2;
   static constexpr int kBatchSlots = 4 * kLanes;
   // asynchronous refactorisation (GP_REFACTOR_ASYNC): up to kLanes batches in
   // flight, each on a stream of its own, factorising into the buffers their
@@ -433,11 +437,8 @@ struct gp_ctx {
     std::vector<uint8_t> batch;
     int nb = 0;                               // scenarios in the batch
     uint8_t *req = nullptr, *tgt = nullptr;   // in d_batch
-    uint8_t *split = nullptr;                 // req split by factor precision, [6][L] (FP32 method)
-    bool any32 = false, any64 = false;        // the batch holds FP32 / FP64 factorisations
     DBuf<double> snap;                        // the batch's matrix values
   } lanes[kLanes];
-  DBuf<uint8_t> d_lsplit;
   DBuf<int> d_fbus, d_nunconv, d_status, d_nstall, d_iters;
   DBuf<double> d_fg, d_fbest, d_traj_tmp;
   DBuf<uint8_t> d_failed, d_upd, d_cur, d_pend, d_areq, d_sch;
@@ -454,6 +455,11 @@ struct gp_ctx {
   DevTables T{};
   LUPlan luJ, luS;
   SpluDev luS64{};   // luS shaped for FP64 factors (the FP64 methods' and the FP32 method's exact retries)
+  // the FP32 method's FP64 exact-Newton kernels run beside the FP32 ones on
+  // a stream of their own: forked after the round's precision split and
+  // after its Schur assembly, joined before the next use of their outputs
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_side = nullptr, ev_fork2 = nullptr, ev_join = nullptr;
   bool has64 = false;   // luS64 fits the LU kernels' LDS
   DBuf<uint8_t> d_split;   // a round's masks split by factor precision [6][L]
   // discrete adjoint: LU plan of J^T (values read from d_jv), its factors,
@@ -869,12 +875,10 @@ void alloc_state(gp_ctx *C) {
   C->d_cur.alloc(L);
   C->d_pend.alloc(L);
   C->d_batch.alloc((size_t)gp_ctx::kLanes * 2 * L);
-  C->d_lsplit.alloc((size_t)gp_ctx::kLanes * 6 * L);
   C->h_batch.alloc((size_t)gp_ctx::kBatchSlots * 2 * L);
   for (int l = 0; l < gp_ctx::kLanes; l++) {
     C->lanes[l].req = C->d_batch.p + (size_t)l * 2 * L;
     C->lanes[l].tgt = C->lanes[l].req + L;
-    C->lanes[l].split = C->d_lsplit.p + (size_t)l * 6 * L;
     C->lanes[l].batch.assign(L, 0);
   }
   HIPCHK(hipMemset(C->d_cur.p, 0, L));
@@ -1136,34 +1140,32 @@ int free_lane(const gp_ctx *C) {
 // A batch is staged on the main stream (its masks, the pending flags, the
 // snapshot of its matrix values) and then started on its lane: the caller
 // launches the round's solve between the two, so that the host calls that
-// start the factorisation overlap the solve instead of delaying it.  With
-// FP32 factors, the scenarios running exact Newton (exacth / exactv, host
-// and device flags; NULL: none) are factorised in FP64.
-int stage_batch(gp_ctx *C, bool fp32, const uint8_t *exacth, const uint8_t *exactv) {
+// start the factorisation overlap the solve instead of delaying it.  Lane l
+// takes the queued requests of its class: with FP32 factors, those of the
+// scenarios running exact Newton (exacth[m]) go to lane 1 in FP64, the
+// others to lane 0.  Returns l, or -1 (lane busy or nothing queued for it).
+int lane_class(const gp_ctx *C, bool fp32, const uint8_t *exacth, int m) {
+  return (fp32 && C->has64 && exacth && exacth[m]) ? 1 : 0;
+}
+int stage_batch(gp_ctx *C, int l, bool fp32, const uint8_t *exacth) {
   const int L = C->ldM;
-  const int l = free_lane(C);
-  if (l < 0) return -1;
   gp_ctx::Lane &ln = C->lanes[l];
+  if (ln.inflight) return -1;
+  int nb = 0;
+  for (int m = 0; m < L; m++) nb += (C->queue[m] && lane_class(C, fp32, exacth, m) == l) ? 1 : 0;
+  if (nb == 0) return -1;
   uint8_t *slot = C->h_batch.p + (size_t)C->batch_slot * 2 * L;
   C->batch_slot = (C->batch_slot + 1) % gp_ctx::kBatchSlots;
-  int nb = 0;
   for (int m = 0; m < L; m++) {
-    ln.batch[m] = C->queue[m];
+    ln.batch[m] = (C->queue[m] && lane_class(C, fp32, exacth, m) == l) ? 1 : 0;
     slot[m] = ln.batch[m];
     slot[L + m] = (uint8_t)(1 - C->cur[m]);
-    if (ln.batch[m]) C->pend[m] = 1;
-    nb += ln.batch[m];
-    C->queue[m] = 0;
+    if (ln.batch[m]) {
+      C->pend[m] = 1;
+      C->queue[m] = 0;
+    }
   }
   HIPCHK(hipMemcpyAsync(ln.req, slot, 2 * L, hipMemcpyHostToDevice, C->stream));
-  ln.any32 = ln.any64 = false;
-  for (int m = 0; m < L; m++)
-    if (ln.batch[m]) {
-      if (fp32 && C->has64 && exacth && exacth[m]) ln.any64 = true;
-      else ln.any32 = true;
-    }
-  if (ln.any64)   // the precision of every request, taken now (the flags change with the rounds)
-    launch_split_precision(L, exactv, ln.req, ln.req, ln.req, ln.split, C->stream);
   launch_async_flags(L, 1, ln.req, C->d_cur.p, C->d_pend.p, C->stream);
   launch_copy_active(std::max(1, C->n_svar), L, C->d_svar.p, ln.snap.p, ln.req, C->stream);
   HIPCHK(hipEventRecord(ln.ev_snap, C->stream));
@@ -1177,16 +1179,8 @@ void start_batch(gp_ctx *C, int l, TF *luv, TF *sv, TF *dn) {
   const int L = C->ldM, nb = ln.nb;
   HIPCHK(hipStreamWaitEvent(ln.stream, ln.ev_snap, 0));
   const int pi = prof_begin(C, C->prof_factor, ln.stream);
-  if (!ln.any64) {
-    launch_splu_factor_impl<TF>(sizeof(TF) == 8 ? C->luS64 : C->luS.dev, ln.snap.p, C->d_s_cst.p, L, luv, sv, dn,
-                                ln.req, ln.tgt, C->M, C->d_status.p, ln.stream);
-  } else {   // FP32 method with exact-Newton scenarios: their factors in FP64
-    if (ln.any32)
-      launch_splu_factor_impl<TF>(C->luS.dev, ln.snap.p, C->d_s_cst.p, L, luv, sv, dn, ln.split + L, ln.tgt, C->M,
-                                  C->d_status.p, ln.stream);
-    launch_splu_factor_impl<double>(C->luS64, ln.snap.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p, C->d_dnS64.p,
-                                    ln.split + 4 * L, ln.tgt, C->M, C->d_status.p, ln.stream);
-  }
+  launch_splu_factor_impl<TF>(sizeof(TF) == 8 ? C->luS64 : C->luS.dev, ln.snap.p, C->d_s_cst.p, L, luv, sv, dn,
+                              ln.req, ln.tgt, C->M, C->d_status.p, ln.stream);
   prof_end(C->prof_factor, pi, ln.stream, nb);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ln.ev_done, ln.stream));
@@ -1238,6 +1232,7 @@ struct Newton {
   const double *dtv = nullptr;   // per-scenario step on the device (NULL: dt for all)
   const uint8_t *exactv = nullptr;   // per-scenario exact-Newton flags on the device (NULL: none)
   const uint8_t *exacth = nullptr;   // the same flags on the host
+  const uint8_t *workh = nullptr;    // host copy of this round's working flags (NULL: unknown)
   std::vector<uint8_t> req;
   const uint8_t *reqp;           // host copy of this round's asynchronous requests
   NewtonOut out;
@@ -1305,23 +1300,26 @@ struct Newton {
       if (C->queue[m]) return true;
     return false;
   }
-  // stage the queued requests as a batch (-1: no lane free), then start it
+  // stage the queued requests as batches on the free lanes of their class
+  // (bit l of the result: lane l staged), then start them
   int stage_queue() {
-    int nb = 0;
-    for (int m = 0; m < C->ldM; m++) nb += C->queue[m];
-    const int l = stage_batch(C, fp32, exacth, exactv);
-    if (l >= 0) {
-      out.refac += nb;
-      out.factor_launches++;
-    }
-    return l;
+    int staged = 0;
+    for (int l = 0; l < gp_ctx::kLanes; l++)
+      if (stage_batch(C, l, fp32, exacth) >= 0) {
+        staged |= 1 << l;
+        out.refac += C->lanes[l].nb;
+        out.factor_launches++;
+      }
+    return staged;
   }
-  void start_queue(int l) {
-    if (l < 0) return;
-    if (fp32)
-      start_batch<float>(C, l, C->d_luS32.p, C->d_svS32.p, C->d_dnS32.p);
-    else
-      start_batch<double>(C, l, C->d_luS64.p, C->d_svS64.p, C->d_dnS64.p);
+  void start_queue(int staged) {
+    for (int l = 0; l < gp_ctx::kLanes; l++) {
+      if (!(staged >> l & 1)) continue;
+      if (fp32 && l == 0)
+        start_batch<float>(C, l, C->d_luS32.p, C->d_svS32.p, C->d_dnS32.p);
+      else
+        start_batch<double>(C, l, C->d_luS64.p, C->d_svS64.p, C->d_dnS64.p);
+    }
   }
   void launch_queue() { start_queue(stage_queue()); }
 
@@ -1365,8 +1363,10 @@ struct Newton {
     // scenarios running exact Newton (exacth), whose factors are FP64
     const bool use32 = fp32 && !all64;
     bool mixed = false;
+    // (a scenario waiting for a factorisation in flight applies nothing this
+    // round: no FP64 launches for it)
     if (use32 && exacth && C->has64)
-      for (int m = 0; m < L && !mixed; m++) mixed = exacth[m] != 0;
+      for (int m = 0; m < L && !mixed; m++) mixed = exacth[m] && (!workh || workh[m]) && !C->pend[m];
     const uint8_t *sch32 = C->d_sch.p, *ref32 = C->d_refac.p, *upd32 = C->d_upd.p;
     const uint8_t *sch64 = sch32, *ref64 = ref32, *upd64 = upd32;
     if (mixed) {
@@ -1376,10 +1376,20 @@ struct Newton {
       sch64 = b + 3 * L; ref64 = b + 4 * L; upd64 = b + 5 * L;
     }
     const bool run32 = use32, run64 = !use32 || mixed;
+    // the FP64 kernels' stream: the side stream beside the FP32 kernels
+    // (they touch other scenarios' columns only), else the main stream
+    hipStream_t s64 = mixed ? C->side : C->stream;
+    if (mixed) {
+      HIPCHK(hipEventRecord(C->ev_fork, C->stream));
+      HIPCHK(hipStreamWaitEvent(C->side, C->ev_fork, 0));
+    }
+    if (run64) launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, sch64, s64, dtv);
     if (run32)
       launch_cluster_schur<float>(C->T, p, z, C->d_f.p, dt, C->d_X32.p, C->d_Dg.p, sch32, C->stream, dtv);
-    if (run64)
-      launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, sch64, C->stream, dtv);
+    if (mixed) {
+      HIPCHK(hipEventRecord(C->ev_side, C->side));
+      HIPCHK(hipStreamWaitEvent(C->stream, C->ev_side, 0));
+    }
     launch_bus_schur(C->T, p, z, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_sch.p, C->stream);
     if (cnt[1] > 0) {
       // synchronous factorisation into the buffer the solves read.  Later
@@ -1397,11 +1407,24 @@ struct Newton {
       }
       out.refac += cnt[1];
       out.factor_launches++;
+    }
+    if (mixed) {   // the FP64 scenarios' factorisation, solve and back-substitution on the side stream
+      HIPCHK(hipEventRecord(C->ev_fork2, C->stream));
+      HIPCHK(hipStreamWaitEvent(C->side, C->ev_fork2, 0));
+      if (cnt[1] > 0)
+        launch_splu_factor_impl<double>(C->luS64, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
+                                        C->d_dnS64.p, ref64, C->d_cur.p, C->M, C->d_status.p, s64);
+      launch_splu_solve<double>(C->luS64, C->d_svS64.p, C->d_dnS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
+                                C->d_dv.p, upd64, C->d_cur.p, C->M, s64);
+      launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, upd64, s64);
+      HIPCHK(hipEventRecord(C->ev_join, C->side));
+    }
+    if (cnt[1] > 0) {
       const int pi = prof_begin(C, C->prof_factor, C->stream);
       if (run32)
         launch_splu_factor_impl<float>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS32.p, C->d_svS32.p,
                                        C->d_dnS32.p, ref32, C->d_cur.p, C->M, C->d_status.p, C->stream);
-      if (run64)
+      if (run64 && !mixed)
         launch_splu_factor_impl<double>(C->luS64, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
                                         C->d_dnS64.p, ref64, C->d_cur.p, C->M, C->d_status.p, C->stream);
       prof_end(C->prof_factor, pi, C->stream, cnt[1]);
@@ -1413,20 +1436,21 @@ struct Newton {
         if (reqp[m]) C->queue[m] = C->pend[m] = 1;
       launch_async_flags(L, 1, C->d_areq.p, C->d_cur.p, C->d_pend.p, C->stream);
     }
-    const int lane = (async && free_lane(C) >= 0 && queued()) ? stage_queue() : -1;
+    const int lane = (async && free_lane(C) >= 0 && queued()) ? stage_queue() : 0;
     const int ps = cnt[2] > 0 ? prof_begin(C, C->prof_solve, C->stream) : -1;
     if (run32)
       launch_splu_solve<float>(C->luS.dev, C->d_svS32.p, C->d_dnS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
                                C->d_dv.p, upd32, C->d_cur.p, C->M, C->stream);
-    if (run64)
+    if (run64 && !mixed)
       launch_splu_solve<double>(C->luS64, C->d_svS64.p, C->d_dnS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
                                 C->d_dv.p, upd64, C->d_cur.p, C->M, C->stream);
     prof_end(C->prof_solve, ps, C->stream, cnt[2]);
     if (run32)
       launch_cluster_backsub<float>(C->T, C->d_X32.p, C->d_f.p, C->d_dv.p, z, upd32, C->stream);
-    if (run64)
+    if (run64 && !mixed)
       launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, upd64, C->stream);
     start_queue(lane);
+    if (mixed) HIPCHK(hipStreamWaitEvent(C->stream, C->ev_join, 0));   // the side stream's work joins the round
     HIPCHK(hipGetLastError());
     return true;
   }
@@ -1826,6 +1850,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
     std::memcpy(wk.data(), hr + R.work(), L);
     std::memcpy(fl.data(), hr + R.failed(), L);
     N.reqp = hr + R.areq();
+    N.workh = wk.data();
     // corrections of the scenarios still iterating run while the host
     // accepts the converged ones
     if (cnt[0] > 0 && !N.correct(cnt, land_first, &fresh))
@@ -2135,6 +2160,9 @@ int gp_create(const gp_system_desc *d, int32_t M, const gp_options *opt, gp_ctx 
   try {
     HIPCHK(hipGetDevice(&C->device));
     HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&C->side, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&C->ev_fork, &C->ev_side, &C->ev_fork2, &C->ev_join})
+      HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
     for (auto &ln : C->lanes) {
       HIPCHK(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
       HIPCHK(hipEventCreateWithFlags(&ln.ev_snap, hipEventDisableTiming));
@@ -2210,6 +2238,10 @@ void gp_destroy(gp_ctx *C) {
     le.push_back(ln.ev_snap);
     le.push_back(ln.ev_done);
   }
+  if (C->side) (void)hipStreamSynchronize(C->side);
+  for (hipEvent_t e : {C->ev_fork, C->ev_side, C->ev_fork2, C->ev_join})
+    if (e) (void)hipEventDestroy(e);
+  if (C->side) (void)hipStreamDestroy(C->side);
   for (hipEvent_t e : C->prof_solve.ev) (void)hipEventDestroy(e);
   for (hipEvent_t e : C->prof_factor.ev) (void)hipEventDestroy(e);
   delete C;

@@ -862,9 +862,15 @@ bool solve_plan(const SpluAnalysis::SolveSide &S, bool diag, int dg_base, int bc
     for (size_t k = i; k < j; k++) {
       const Sub &sb = subs[k];
       const int l = sb.l, T = S.lvl[4 * l], g = S.lvl[4 * l + 1], cb = S.lvl[4 * l + 3];
+      // a sub-level of one chunk runs on wavefront 0 alone: when the next
+      // sub-level of the step is one chunk too, wavefront 0 goes straight on
+      // (a wavefront's LDS accesses complete in order), and the barrier
+      // comes after the last of the run
+      const bool single = sb.c1 - sb.c0 == 1;
+      const bool chain = single && k + 1 < j && subs[k + 1].c1 - subs[k + 1].c0 == 1;
       // passes of 8 chunks (one per wavefront); a barrier after the last
       for (int c = sb.c0; c < sb.c1; c += 8, nlp++) {
-        lv.insert(lv.end(), {g, c + 8 >= sb.c1 ? 1 : 0, 0, 0});
+        lv.insert(lv.end(), {g, (c + 8 >= sb.c1 && !chain) ? 1 : 0, 0, 0});
         for (int w = 0; w < 8; w++) {
           const int cw = c + w;
           if (cw < sb.c1) {
