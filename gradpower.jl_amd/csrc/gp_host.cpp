@@ -151,11 +151,19 @@ struct LUPlan {
   static std::vector<int> nonempty(const std::vector<int> &v) { return v.empty() ? std::vector<int>{0} : v; }
   // LDS bytes of the solve's x (16-byte aligned)
   int x_bytes() const { return (int)(((size_t)A.n * 8 + 15) / 16 * 16); }
+  // The solve keeps x in global memory when x and the staging area of FP64
+  // values do not fit in LDS together (above about 12k rows)
+  bool force_gx = false;   // gp_options.global_x
+  bool gx_needed() const {
+    return force_gx || (long long)x_bytes() + (long long)kSolCB * kThreads * 4 +
+                               (long long)(kSolCV + kSolCD) * kThreads * 8 >
+                           kLdsMax;
+  }
 
   // Largest dense trailing block the solve's LDS allows (x, 8 bytes per row,
-  // stays resident beside it).
+  // stays resident beside it unless it lives in global memory).
   int nd_bound() const {
-    const long long room = std::min<long long>(kLdsBudget, (long long)kLdsMax - 8LL * A.n - 16384);
+    const long long room = std::min<long long>(kLdsBudget, (long long)kLdsMax - (gx_needed() ? 0LL : 8LL * A.n) - 16384);
     int nd_max = 0;
     while ((long long)(nd_max + 16) * (nd_max + 16) * 8 <= room) nd_max += 16;
     return nd_max;
@@ -164,7 +172,7 @@ struct LUPlan {
   // (and what the solve's LDS allows).
   void upload(const std::vector<int> &asrc, int nd_cap = 1 << 30) {
     splu_schedule(A, kThreads, std::min(nd_bound(), nd_cap));
-    if (x_bytes() + (long long)A.nd * A.nd * 8 > kLdsMax)
+    if (!gx_needed() && x_bytes() + (long long)A.nd * A.nd * 8 > kLdsMax)
       throw GpError{GP_ERR_UNSUPPORTED, "system too large for the in-LDS triangular solve (" +
                                             std::to_string(A.n) + " rows)"};
     a2lu.upload(A.a2lu); a_src.upload(asrc);
@@ -215,6 +223,8 @@ struct LUPlan {
     d.bw_lvl = bw_lvl.p; d.bw_chunk = bw_chunk.p; d.bw_cols = bw_cols.p; d.bw_rows = bw_rows.p;
     d.q = q.p; d.pinv = pinv.p;
     d.threads = kThreads;
+    d.gx = gx_needed() ? 1 : 0;
+    d.gx_ws = nullptr;   // the context's [n][ldM] workspace, set after the plan is made
   }
   // LDS staging sizes for element size `elem`; columns (or op lists) larger
   // than the staging buffers run in place in global memory.
@@ -233,10 +243,12 @@ struct LUPlan {
     dev.w_bytes = wbytes(wc, wo);
     dev.lds_factor = std::max<int>(std::max<int>((int)((size_t)kThreads * tc * elem), W * dev.w_bytes),
                                    (int)((((size_t)A.nd * A.nd + 3) / 4 * 4 + 512) * elem));   // dense block, buffers
-    // solve: x, then the staging area, which the dense block reuses
-    dev.lds_solve = x_bytes() + (int)std::max<size_t>((size_t)kSolCB * kThreads * 4 +
-                                                          (size_t)(kSolCV + kSolCD) * kThreads * elem,
-                                                      (size_t)A.nd * A.nd * elem);
+    // solve: x, then the staging area, which the dense block reuses; with x
+    // in global memory only the dense block
+    dev.lds_solve = dev.gx ? (int)std::max<size_t>(16, (size_t)A.nd * A.nd * elem)
+                           : x_bytes() + (int)std::max<size_t>((size_t)kSolCB * kThreads * 4 +
+                                                                   (size_t)(kSolCV + kSolCD) * kThreads * elem,
+                                                               (size_t)A.nd * A.nd * elem);
     if (dev.lds_solve > kLdsMax)
       throw GpError{GP_ERR_UNSUPPORTED, "system too large for the in-LDS triangular solve (" +
                                             std::to_string(A.n) + " rows)"};
@@ -275,6 +287,11 @@ struct LUPlan {
   double time_one(const double *var, const double *cst, const double *rhs, double *x, const uint8_t *one, int *st,
                   double w_solve, hipStream_t s) {
     DBuf<TF> lu, sv, dn;
+    DBuf<double> ws;
+    if (dev.gx) {   // one scenario: ldM 1
+      ws.alloc(A.n);
+      dev.gx_ws = ws.p;
+    }
     lu.alloc(lu_elems());
     sv.alloc(dev.ns);
     HIPCHK(hipMemsetAsync(sv.p, 0, sv.n * sizeof(TF), s));   // padding slots stay zero
@@ -305,6 +322,7 @@ struct LUPlan {
     const double tf = med(fac), ts = med(sol);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    dev.gx_ws = nullptr;
     return tf + w_solve * ts;
   }
   // mode (gp_options.dense_block): 0 tune, < 0 the cost model alone, > 0 a
@@ -455,6 +473,7 @@ struct gp_ctx {
   DevTables T{};
   LUPlan luJ, luS;
   SpluDev luS64{};   // luS shaped for FP64 factors (the FP64 methods' and the FP32 method's exact retries)
+  DBuf<double> d_gxJ, d_gxS, d_gxJT;   // solves' x in global memory [n][ldM] (large systems)
   // the FP32 method's FP64 exact-Newton kernels run beside the FP32 ones on
   // a stream of their own: forked after the round's precision split and
   // after its Schur assembly, joined before the next use of their outputs
@@ -1003,6 +1022,10 @@ void analyse(gp_ctx *C, int method) {
     std::vector<int> asrc(nnz);
     for (int i = 0; i < nnz; i++) asrc[i] = i;
     C->luJ.tune(asrc, jv, nullptr, sizeof(double), 1.0, C->stream, C->opt.dense_block);   // a solve per factorisation
+    if (C->luJ.dev.gx) {
+      C->d_gxJ.ensure((size_t)C->n * L);
+      C->luJ.dev.gx_ws = C->d_gxJ.p;
+    }
     C->luJ.ready = true;
     C->luJ.trip_gen = C->trip_gen;
     C->d_luJ.alloc(C->luJ.lu_elems() * C->M);
@@ -1040,6 +1063,10 @@ void analyse(gp_ctx *C, int method) {
     // the solves are on every round's critical path
     C->luS.tune(C->s_src, sv, C->d_s_cst.p, method == GP_METHOD_SCHUR_FP32 ? sizeof(float) : sizeof(double), 2.0,
                 C->stream, C->opt.dense_block);
+    if (C->luS.dev.gx) {
+      C->d_gxS.ensure((size_t)C->nred * L);
+      C->luS.dev.gx_ws = C->d_gxS.p;
+    }
     C->luS.ready = true;
     C->luS.trip_gen = C->trip_gen;
   }
@@ -2075,6 +2102,10 @@ void analyse_transposed(gp_ctx *C, double dt) {
   std::string e = splu_analyse(n, tcp.data(), tri.data(), tv.data(), C->luJT.A, 1e-3, ptrs(tx));
   if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "transposed Jacobian analysis: " + e};
   C->luJT.tune(tsrc, jv, nullptr, sizeof(double), 1.0, C->stream, C->opt.dense_block);
+  if (C->luJT.dev.gx) {
+    C->d_gxJT.ensure((size_t)n * L);
+    C->luJT.dev.gx_ws = C->d_gxJT.p;
+  }
   C->luJT.ready = true;
   C->luJT.dt = dt;
   C->luJT.trip_gen = C->trip_gen;
@@ -2172,6 +2203,7 @@ int gp_create(const gp_system_desc *d, int32_t M, const gp_options *opt, gp_ctx 
     C->ldM = (M + WAVE - 1) / WAVE * WAVE;
     if (opt) C->opt = *opt;
     else { C->opt.record_all = 1; C->opt.refactor_policy = GP_REFACTOR_EVERY_ITER; }
+    C->luJ.force_gx = C->luS.force_gx = C->luJT.force_gx = C->opt.global_x != 0;
     C->is_diff.assign(d->is_diff, d->is_diff + d->n);
     C->yrp.assign(d->y_rowptr, d->y_rowptr + 2 * d->nbus + 1);
     C->yci.assign(d->y_colind, d->y_colind + d->y_nnz);

@@ -92,6 +92,10 @@ struct SpluDev {
   int res_nsteps, res_lv, res_nl, res_xtot;
   // staged triangular solve: steps per side, headers and blobs
   int sol_nfw, sol_nbw;
+  // systems whose solution does not fit in LDS: the solve keeps x in global
+  // memory, gx_ws [n][ldM] (scenario m's x at j*ldM + m, pivot order)
+  int gx;
+  double *gx_ws;
   const int *sol_fw_hdr, *sol_fw_blob, *sol_bw_hdr, *sol_bw_blob;
   const int *res_hdr, *res_blob, *res_a2x, *res_omap;
 };

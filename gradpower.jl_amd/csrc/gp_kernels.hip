@@ -2236,6 +2236,100 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   }
 }
 
+// Systems whose solution does not fit in LDS (tens of thousands of rows and
+// more): the same two triangular solves, one workgroup per scenario, with x
+// in global memory (S.gx_ws, scenario m's x at j*ldM + m, pivot order) and
+// the level tables read where they lie (sliced ELL: lane l of a level holds
+// the entries E + u + 64 j of its chunk, u its lane in the chunk, the g lanes
+// of a row add up, the row's first lane updates x[row]); a workgroup barrier,
+// which orders the global memory of the workgroup too, closes each level.
+// The dense block's substitutions run on one wavefront from LDS.
+template <typename TF, bool BW>
+__device__ __forceinline__ void gx_levels(int nlev, const int *__restrict__ lvl, const int *__restrict__ chunk,
+                                          const int *__restrict__ cols, const int *__restrict__ rows,
+                                          const TF *__restrict__ vals, const TF *__restrict__ diag, double *x,
+                                          int ldM, int tid, int nt) {
+  for (int lv = 0; lv < nlev; lv++) {
+    const int T = lvl[4 * lv], g = lvl[4 * lv + 1], R = lvl[4 * lv + 2], c0 = lvl[4 * lv + 3];
+    const int Tp = (T + 63) & ~63;
+    for (int l = tid; l < Tp; l += nt) {
+      const int lc = min(l, T - 1);   // spare lanes of the last chunk read lane T - 1's entries
+      const int c = c0 + (lc >> 6), uc = lc & 63;
+      const int E = chunk[2 * c], pf = chunk[2 * c + 1];
+      double acc = 0.0;
+      for (int j = 0; j < pf; j++) {
+        const int e = E + uc + WAVE * j;
+        acc += (double)vals[e] * x[(size_t)cols[e] * ldM];
+      }
+      acc = group_sum(acc, g);
+      if (l < T && (uc & (g - 1)) == 0) {
+        const int row = rows[R + lc];
+        const double xr = x[(size_t)row * ldM];
+        x[(size_t)row * ldM] = BW ? (xr - acc) / (double)diag[R + lc] : xr - acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename TF>
+__global__ void __launch_bounds__(512) k_splu_solve_gx(SpluDev S, const TF *__restrict__ svv,
+                                                       const TF *__restrict__ dnv, const double *__restrict__ rhs,
+                                                       int ldM, double sign, const int *__restrict__ scatter,
+                                                       double *__restrict__ zupd, double *__restrict__ xout,
+                                                       const uint8_t *__restrict__ work,
+                                                       const uint8_t *__restrict__ cur) {
+  const int m = scenario_of_block(blockIdx.x, gridDim.x);
+  if (!work[m]) return;
+  extern __shared__ __align__(16) char smem_raw[];
+  const int n = S.n, nd = S.nd, B = n - nd;
+  const int Mb = gridDim.x;
+  const int bsel = cur ? cur[m] : 0;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  double *x = S.gx_ws + m;
+  const TF *fv = svv + ((size_t)bsel * Mb + m) * S.ns;
+  const TF *bv = fv + S.fw_nvals;
+  const TF *bd = bv + S.bw_nvals;
+  for (int r = tid; r < n; r += nt) x[(size_t)S.pinv[r] * ldM] = rhs[(size_t)r * ldM + m];
+  __syncthreads();
+  gx_levels<TF, false>(S.n_fwlev, S.fw_lvl, S.fw_chunk, S.fw_cols, S.fw_rows, fv, nullptr, x, ldM, tid, nt);
+  if (nd) {
+    TF *Dl = reinterpret_cast<TF *>(smem_raw);
+    const TF *dn = dnv + ((size_t)bsel * Mb + m) * nd * nd;
+    for (int i = tid; i < nd * nd; i += nt) Dl[i] = dn[i];
+    __syncthreads();
+    if (tid < WAVE) {   // lane i holds x[B + i] and x[B + 64 + i]
+      const int i0 = tid, i1 = tid + WAVE;
+      double x0 = i0 < nd ? x[(size_t)(B + i0) * ldM] : 0.0, x1 = i1 < nd ? x[(size_t)(B + i1) * ldM] : 0.0;
+      for (int p = 0; p < nd; p++) {
+        const double xp = p < WAVE ? readlane_d(x0, p) : readlane_d(x1, p - WAVE);
+        if (i0 > p && i0 < nd) x0 -= (double)Dl[min(i0, nd - 1) + p * nd] * xp;
+        if (i1 > p && i1 < nd) x1 -= (double)Dl[min(i1, nd - 1) + p * nd] * xp;
+      }
+      for (int p = nd - 1; p >= 0; p--) {
+        const double xp = (p < WAVE ? readlane_d(x0, p) : readlane_d(x1, p - WAVE)) / (double)Dl[p + p * nd];
+        if (i0 == p) x0 = xp;
+        if (i1 == p) x1 = xp;
+        if (i0 < p) x0 -= (double)Dl[min(i0, nd - 1) + p * nd] * xp;
+        if (i1 < p && i1 < nd) x1 -= (double)Dl[min(i1, nd - 1) + p * nd] * xp;
+      }
+      if (i0 < nd) x[(size_t)(B + i0) * ldM] = x0;
+      if (i1 < nd) x[(size_t)(B + i1) * ldM] = x1;
+    }
+    __syncthreads();
+  }
+  gx_levels<TF, true>(S.n_bwlev, S.bw_lvl, S.bw_chunk, S.bw_cols, S.bw_rows, bv, bd, x, ldM, tid, nt);
+  for (int k = tid; k < n; k += nt) {
+    const int j = S.q[k];
+    const double v = x[(size_t)k * ldM];
+    if (xout) xout[(size_t)j * ldM + m] = v;
+    if (zupd) {
+      const int zr = scatter ? scatter[j] : j;
+      zupd[(size_t)zr * ldM + m] += sign * v;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- discrete adjoint
 // Backward step k of the adjoint of the backward-Euler sweep
 //   F_k(z_k, z_{k-1}, p) = 0:  z - z_{k-1} - dt f(z, p) on differential rows,
@@ -2526,6 +2620,13 @@ template <typename TF>
 void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const double *rhs, int ldM, double sign,
                        const int *scatter, double *zupd, double *xout, const uint8_t *work, const uint8_t *cur, int M,
                        hipStream_t s) {
+  if (S.gx) {
+    static std::atomic<unsigned> done_gx{0};
+    allow_lds(reinterpret_cast<const void *>(k_splu_solve_gx<TF>), done_gx, S.lds_solve);
+    hipLaunchKernelGGL(k_splu_solve_gx<TF>, dim3(M), dim3(S.threads), S.lds_solve, s, S, sv, dn, rhs, ldM, sign,
+                       scatter, zupd, xout, work, cur);
+    return;
+  }
   static std::atomic<unsigned> done_sol{0};
   allow_lds(reinterpret_cast<const void *>(k_splu_solve<TF>), done_sol, S.lds_solve);
   hipLaunchKernelGGL(k_splu_solve<TF>, dim3(M), dim3(S.threads), S.lds_solve, s, S, sv, dn, rhs, ldM, sign, scatter,

@@ -290,17 +290,33 @@ def _spread_buses(rng, edges, n, k):
     distance from a random start, so every load bus has generation a few hops
     away (random placement leaves long generator-free corridors on sparse
     random geometric graphs, where the power flow has no solution)."""
-    from scipy.sparse import coo_matrix
-    from scipy.sparse.csgraph import shortest_path
-    e = np.array(edges)
-    A = coo_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(n, n)).tocsr()
+    from collections import deque
+    adj = [[] for _ in range(n)]
+    for a, b in edges:
+        adj[a].append(b)
+        adj[b].append(a)
+    # hop distance to the nearest chosen bus, updated by a breadth-first
+    # search from each new choice that stops where the distance does not
+    # fall (the same distances as a full search from every chosen bus)
+    dist = np.full(n, np.inf)
+
+    def lower(src):
+        dist[src] = 0.0
+        dq = deque([src])
+        while dq:
+            u = dq.popleft()
+            du = dist[u] + 1.0
+            for v in adj[u]:
+                if du < dist[v]:
+                    dist[v] = du
+                    dq.append(v)
     chosen = [int(rng.integers(n))]
-    dist = shortest_path(A, directed=False, unweighted=True, indices=chosen[0])
+    lower(chosen[0])
     while len(chosen) < k:
         far = np.flatnonzero(dist == dist.max())
         nxt = int(rng.choice(far))
         chosen.append(nxt)
-        dist = np.minimum(dist, shortest_path(A, directed=False, unweighted=True, indices=nxt))
+        lower(nxt)
     return np.array(chosen)
 
 

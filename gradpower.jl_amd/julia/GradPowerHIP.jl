@@ -82,7 +82,7 @@ struct GpOptions
     lock_step::Int32
     dense_block::Int32
     retry_window::Int32
-    reserved::NTuple{1,Int32}
+    global_x::Int32
     refactor_theta::Float64
     async_below::Float64
 end
@@ -299,7 +299,7 @@ function HipBatchedLayout(dp::GradPower.DynamicProblem, ps::GradPower.PowerSyste
                             pointer(t.yrp), pointer(t.yci), pointer(t.yv), t.n_dev,
                             ntuple(i -> pointer(t.dev[i]), 8), pointer(t.u0), t.n_clusters, pointer(t.clusters))
         opt = GpOptions(record_idx === nothing ? 1 : 0, length(rec), isempty(rec) ? C_NULL : pointer(rec),
-                        Int32(refactor_policy), 0, 0, 0, 0, Int32(retry_window), (Int32(0),), 0.0, 0.0)
+                        Int32(refactor_policy), 0, 0, 0, 0, Int32(retry_window), Int32(0), 0.0, 0.0)
         check(ccall((:gp_create, LIB), Cint, (Ref{GpSystemDesc}, Int32, Ref{GpOptions}, Ref{Ptr{Cvoid}}),
                     desc, Int32(M), opt, h))
     end

@@ -201,7 +201,9 @@ typedef struct gp_options {
                               method, run to itermax) before its scenario
                               counts as failed: 0 (default) 8, -1 at itermax
                               only                                              */
-  int32_t reserved[1];
+  int32_t global_x;        /* 1: the batched triangular solves keep the solution in
+                              global memory even where it fits in LDS (the form
+                              they take by themselves above about 12k rows)    */
   double refactor_theta;   /* ADAPTIVE / ASYNC: refactorise when an iteration
                               reduces |F|inf by less than this factor (0: 30)   */
   double async_below;      /* ASYNC: a refactorisation asked for at |F|inf below

@@ -247,3 +247,21 @@ def test_adjoint_ragged_horizons_and_event_tables():
         assert r["value_scen"][m] == pytest.approx(val, rel=1e-9, abs=1e-18), m
         assert np.max(np.abs(r["grad_scen"][m] - g)) <= 1e-7 * np.max(np.abs(g)) + 1e-15, m
         assert np.max(np.abs(r["dz0"][m] - dz0)) <= 1e-7 * np.max(np.abs(dz0)) + 1e-15, m
+
+
+def test_adjoint_with_global_memory_solves_matches_oracle():
+    """The discrete adjoint with the triangular solves (forward J, backward
+    J^T) keeping x in global memory: within 1e-7 of the oracle's adjoint."""
+    case = get_case("case1_9bus")
+    M = 4
+    from gradpower_hip.lib import HipContext
+    ctx = HipContext(case.desc, M, global_x=True)
+    ctx.set_state(case.desc.z0, case.desc.p)
+    ctx.set_events(_all_events(case, M), case.nsteps[:M])
+    params = _gains(case)
+    r = ctx.adjoint(MONO, case.dt, params, tol=1e-12, segment=30)
+    o = _oracle(case)
+    for m in range(M):
+        _, g, _ = o.adjoint(case.dt, int(case.nsteps[m]), params, _events(case, m), tol=1e-12)
+        assert np.max(np.abs(r["grad_scen"][m] - g)) <= 1e-7 * np.max(np.abs(g)), m
+    ctx.close()

@@ -222,3 +222,24 @@ def test_ragged_horizons_and_event_tables_match_oracle(method, policy, lock):
         assert rs["not_converged"] == 0
         dev = np.max(np.abs(tr.T - ref))
         assert dev < 1e-9, (method, lock, m, dev)
+
+
+@pytest.mark.parametrize("method,policy", [("mono", 0), ("schur64", 0), ("schur32", 3)])
+@pytest.mark.parametrize("name,nsteps", [("case3_200bus", 60), ("case2_60bus", 120)])
+def test_global_memory_solve_matches_oracle(name, nsteps, method, policy):
+    """The triangular solves that keep x in global memory (the form they take
+    above about 12k rows, where x does not fit in LDS; forced here with
+    gp_options.global_x) give trajectories within 1e-9 pu of the oracle."""
+    case = get_case(name)
+    M = min(case.M, 8)
+    tol = 1e-12
+    ctx = _ctx(case, M, refactor_policy=policy, global_x=True)
+    ev = [(m, e.bus, e.rfault, int(round(e.ton / case.dt)), int(round(e.toff / case.dt)))
+          for m in range(M) for e in case.events[m]]
+    ctx.set_events(ev, [nsteps] * M)
+    st = ctx.integrate(METHODS[method], case.dt, tol, 50)
+    ref = _oracle_trajs(case, range(M), tol, nsteps)
+    for m in range(M):
+        dev = np.max(np.abs(ctx.trajectory(m).T - ref[m]))
+        assert dev < 1e-9, (name, method, m, dev, st)
+    ctx.close()
