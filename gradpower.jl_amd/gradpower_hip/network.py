@@ -101,10 +101,14 @@ def _power(ybus, vm, va):
     return s.real, s.imag
 
 
-def runpf(ps: PowerSystem, tol=1e-11, maxit=100, dc_start=False):
+def runpf(ps: PowerSystem, tol=1e-11, maxit=100, dc_start=False, max_step=None):
     """Polar Newton power flow on |V| (PQ buses) and angle (PQ+PV buses),
     the unknown set of src/pflow.jl:286-371.  Returns (vm, va, P, Q) with the
-    net injections at the solution."""
+    net injections at the solution.  max_step (angle rad, magnitude pu), when
+    given, shortens a Newton step whose largest change exceeds it: on grids
+    of tens of thousands of buses with one slack, the angle block of the
+    Jacobian is nearly singular and a full first step from a flat start
+    moves every angle by hundreds of radians."""
     y = ps.network.ybus.tocsr()
     btype = np.array([b.type for b in ps.buses])
     vm = np.array([b.v0m for b in ps.buses], dtype=float)
@@ -139,6 +143,10 @@ def runpf(ps: PowerSystem, tol=1e-11, maxit=100, dc_start=False):
         # weak meshed grids; the accepted point is still a Newton solution.
         f0n = np.max(np.abs(f))
         lam = 1.0
+        if max_step is not None:
+            da = np.max(np.abs(dx[npq:]), initial=0.0) / max_step[0]
+            dm = np.max(np.abs(dx[:npq]), initial=0.0) / max_step[1]
+            lam = min(1.0, 1.0 / max(da, dm, 1e-300))
         while True:
             vm_t, va_t = vm.copy(), va.copy()
             vm_t[pq] += lam * dx[:npq]

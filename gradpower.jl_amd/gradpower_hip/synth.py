@@ -13,6 +13,9 @@ the drawn values that the case definitions leave open are recorded in
                           with SEXS/ESDC1A + TGOV1/IEESGO, 100 IEEEST, 300 static
                           generators, M=256 uneven faults and horizons
   case3_200bus         -- 200 buses, 40 x GENROU+SEXS+TGOV1, M=128, 3 s
+  case4_grid           -- 70,000 buses (35 tiles of the 2,000-bus grid), 12,005
+                          machines (a quarter GENSAL) with exciters, governors
+                          and stabilisers, 9,100 static generators, M=8, 2 s
 """
 from __future__ import annotations
 
@@ -89,7 +92,7 @@ def _ieeest(bus, gid="1"):
                   2.0, 2.0, 5.0, 0.1, -0.1, 0.0, 0.0)
 
 
-def _solve_pf_scaling(ps, meta):
+def _solve_pf_scaling(ps, meta, max_step=None, maxit=100, first=1.0, min_step=1.0 / 256):
     """Power flow by continuation in the loading: solve with every load and
     scheduled generation scaled by s, starting at s = 0 and raising s towards
     1 from the previous solution (halving the increment after a failure).  If
@@ -107,11 +110,13 @@ def _solve_pf_scaling(ps, meta):
             g.psch = p0 * sc
 
     s_ok, step, sol = None, 1.0, None
-    s_try = 1.0
+    s_try = first
+    if first < 1.0:
+        step = 0.25
     while True:
         apply(s_try)
         try:
-            vm, va, _, _ = runpf(ps)
+            vm, va, _, _ = runpf(ps, max_step=max_step, maxit=maxit)
             s_ok, sol = s_try, (vm, va)
         except RuntimeError:
             if s_try == 0.0:
@@ -120,7 +125,7 @@ def _solve_pf_scaling(ps, meta):
                 s_try, step = 0.0, 0.25
                 continue
             step *= 0.5
-            if step < 1.0 / 256:
+            if step < min_step:
                 break
             s_try = s_ok + step
             continue
@@ -134,11 +139,12 @@ def _solve_pf_scaling(ps, meta):
     for b, m, a, vs in zip(ps.buses, sol[0], sol[1], vset):
         b.v0a = float(a)
         b.v0m = float(m) if b.type == PQ else vs
-    runpf_inplace(ps)
+    runpf_inplace(ps, max_step=max_step)
     meta["load_scale"] = s_ok
 
 
-def _finish(name, ps, psd, M, events, tf, meta, zip_alpha=None, pq_after_pf=None, size_machines=False):
+def _finish(name, ps, psd, M, events, tf, meta, zip_alpha=None, pq_after_pf=None, size_machines=False,
+            pf_max_step=None, pf_kw=None):
     pf_done = False
     if pq_after_pf is not None or size_machines:
         # power flow first: optionally freeze the given voltage-controlled
@@ -147,7 +153,7 @@ def _finish(name, ps, psd, M, events, tf, meta, zip_alpha=None, pq_after_pf=None
         # which the machine models scale with (the slack machine in particular
         # picks up the losses and the dispatch mismatch)
         build_network(ps)
-        _solve_pf_scaling(ps, meta)
+        _solve_pf_scaling(ps, meta, pf_max_step, **(pf_kw or {}))
         pf_done = True
         for b in pq_after_pf or []:
             ps.buses[int(b)].type = PQ
@@ -161,9 +167,9 @@ def _finish(name, ps, psd, M, events, tf, meta, zip_alpha=None, pq_after_pf=None
                 dev.dtype.alpha = zip_alpha
     build_network(ps)
     if not pf_done:
-        _solve_pf_scaling(ps, meta)
+        _solve_pf_scaling(ps, meta, pf_max_step, **(pf_kw or {}))
     else:
-        runpf_inplace(ps)
+        runpf_inplace(ps, max_step=pf_max_step)
     # ZIP loads were created from ps.loads in order by set_dynamics; carry a
     # load scaling applied by the power-flow fallback over to them.
     zips = [dev.dtype for dev in psd.devices if dev.dtype.kind == "zipload"]
@@ -351,7 +357,8 @@ def _regional_dispatch(edges, n, gen_buses, loads, floor=0.1):
 
 def case2_grid(M=256, nbus=2000, n_genrou=400, n_pss=100, n_static=300, seed=1002,
                load_fraction=0.2, tf_range=(1.0, 5.0), name="c2_grid", charging_scale=1.0,
-               zip_alpha=1.0, static_gen_type=PV, static_gen_p=1.0, fault_sg_hops=3, size_machines=True):
+               zip_alpha=1.0, static_gen_type=PV, static_gen_p=1.0, fault_sg_hops=3, size_machines=True,
+               pf_max_step=None):
     rng = np.random.default_rng(seed)
     meta = {"seed": seed, "zip_alpha": zip_alpha, "nbus": nbus,
             "zip_note": "constant-impedance loads (alpha=1, the reference default of "
@@ -431,7 +438,122 @@ def case2_grid(M=256, nbus=2000, n_genrou=400, n_pss=100, n_static=300, seed=100
         tf.append(float(rng.uniform(*tf_range)))
     return _finish(name, ps, psd, M, events, tf, meta, zip_alpha=zip_alpha,
                    pq_after_pf=[int(b) for b in sbus] if static_gen_type == PQ else None,
-                   size_machines=size_machines)
+                   size_machines=size_machines, pf_max_step=pf_max_step)
+
+
+def case4_grid(M=8, tiles=(7, 5), tile_bus=2000, tile_genrou=343, tile_static=260, tile_pss=86,
+               gensal_frac=0.25, ties=4, tf=2.0, seed=1004, name="c4_grid"):
+    """The 70,000-bus workload: 35 tiles of the 2,000-bus generator's grid
+    (random geometric graph, mean degree 2.6, the same line, load, machine and
+    controller draws) on a 7 x 5 lattice, neighbouring tiles joined by `ties`
+    lines between their closest buses.  12,005 machines, a quarter GENSAL,
+    with SEXS/ESDC1A exciters, TGOV1/IEESGO governors and IEEEST stabilisers
+    on a quarter; 9,100 static generator stubs.  One slack machine per tile
+    (a single slack for 70,000 buses leaves the angle block of the power-flow
+    Jacobian nearly singular, and flat-start Newton diverges even at zero
+    load).  Faults as in the 2,000-bus case: bus uniform over the buses at
+    least 3 hops from every static generator, r ~ U(0.01, 0.2), on at 0.02 s,
+    cleared at U(0.05, 0.25) s; M scenarios over `tf` seconds."""
+    rng = np.random.default_rng(seed)
+    tx, ty = tiles
+    nt = tx * ty
+    nbus = nt * tile_bus
+    meta = {"seed": seed, "zip_alpha": 1.0, "nbus": nbus, "tiles": [tx, ty], "ties_per_neighbour": ties}
+    buses = [Bus(i + 1, f"B{i+1}", PQ, 230.0, 1.0, 0.0) for i in range(nbus)]
+    branches, loads, gens, edges_all = [], [], [], []
+    psd = PowerSystemDynamics()
+    gbus_all, sbus_all, pts_all = [], [], np.zeros((nbus, 2))
+    for t in range(nt):
+        o = t * tile_bus
+        edges, pts = _rgg_edges(rng, tile_bus, 2.6)
+        pts_all[o:o + tile_bus] = pts + np.array([t % tx, t // tx], float)
+        for (a, b) in edges:
+            x = rng.uniform(0.05, 0.2)
+            branches.append(Branch(o + a, o + b, "1", x / rng.uniform(5.0, 15.0), x, rng.uniform(0.0, 0.2)))
+            edges_all.append((o + a, o + b))
+        spread = _spread_buses(rng, edges, tile_bus, tile_genrou + tile_static)
+        spread = spread[rng.permutation(len(spread))]
+        gb, sb = spread[:tile_genrou], spread[tile_genrou:]
+        rest = np.setdiff1d(np.arange(tile_bus), spread)
+        lb = rest[rng.random(len(rest)) < 0.2 / max(1e-9, len(rest) / tile_bus)]
+        tl = []
+        for b in sorted(int(x) for x in lb):
+            P = rng.uniform(0.5, 1.5)
+            tl.append(Load(o + b, "1", P, -0.3 * P))
+        loads += tl
+        allg = [int(b) for b in gb] + [int(b) for b in sb]
+        pg = _regional_dispatch(edges, tile_bus, allg, [Load(ld.bus - o, ld.id, ld.pd, ld.qd) for ld in tl])
+        for k, b in enumerate(allg):
+            bb = o + b
+            if k < tile_genrou:
+                buses[bb].type = SLACK if k == 0 else PV
+                buses[bb].v0m = 1.02
+            else:
+                buses[bb].type = PV
+                buses[bb].v0m = 1.01
+            gens.append(Gen(bb, "1", float(pg[k]), 0.0, max(100.0, 100.0 * pg[k] / 0.7)))
+        gbus_all += [o + int(b) for b in gb]
+        sbus_all += [o + int(b) for b in sb]
+        n_m = tile_genrou
+        exc_kind = rng.random(n_m) < 0.5
+        gov_kind = rng.random(n_m) < 0.5
+        sal = rng.random(n_m) < gensal_frac
+        pss_on = np.zeros(n_m, bool)
+        pss_on[rng.choice(n_m, tile_pss, replace=False)] = True
+        H = rng.uniform(2.0, 8.0, n_m)
+        D = rng.uniform(0.0, 2.0, n_m)
+        for k, b in enumerate(gb):
+            eb = o + int(b) + 1
+            if sal[k]:
+                g = C0_GENROU
+                psd.add_device(Gensal(eb, "1", g["x_d"], g["x_q"], g["x_dp"], g["x_ddp"], g["xl"], float(H[k]),
+                                      float(D[k]), g["T_d0p"], g["T_d0dp"], g["T_q0dp"], g["S1"], g["S2"]))
+            else:
+                psd.add_device(_genrou(eb, H=float(H[k]), D=float(D[k])))
+            psd.add_device(_ieesgo(eb, "1", rng, meta) if gov_kind[k] else _tgov1(eb))
+            psd.add_device(_sexs(eb) if exc_kind[k] else _esdc1a(eb))
+            if pss_on[k]:
+                psd.add_device(_ieeest(eb))
+    meta.pop("ieesgo_T", None)
+    # tie lines between neighbouring tiles: the closest bus pairs across each boundary
+    from scipy.spatial import cKDTree
+    for t in range(nt):
+        i, j = t % tx, t // tx
+        for u in ([t + 1] if i + 1 < tx else []) + ([t + tx] if j + 1 < ty else []):
+            A = pts_all[t * tile_bus:(t + 1) * tile_bus]
+            B = pts_all[u * tile_bus:(u + 1) * tile_bus]
+            d, k = cKDTree(B).query(A)
+            used = set()
+            for a in np.argsort(d):
+                if len(used) >= ties:
+                    break
+                if int(k[a]) in used:
+                    continue
+                used.add(int(k[a]))
+                x = rng.uniform(0.05, 0.2)
+                fa, fb = t * tile_bus + int(a), u * tile_bus + int(k[a])
+                branches.append(Branch(fa, fb, "1", x / rng.uniform(5.0, 15.0), x, rng.uniform(0.0, 0.2)))
+                edges_all.append((fa, fb))
+    meta["mean_degree"] = 2 * len(edges_all) / nbus
+    meta["n_machines"] = nt * tile_genrou
+    meta["n_static"] = nt * tile_static
+    ps = PowerSystem(100.0, buses, gens, loads, branches, [])
+    adj = [[] for _ in range(nbus)]
+    for (a, b) in edges_all:
+        adj[a].append(b)
+        adj[b].append(a)
+    excl = set(sbus_all)
+    for _ in range(2):
+        for b in list(excl):
+            excl.update(adj[b])
+    fault_buses = np.array([b for b in range(nbus) if b not in excl])
+    meta["fault_bus_pool"] = int(len(fault_buses))
+    events = [[ContingencyEvent(int(fault_buses[rng.integers(len(fault_buses))]), float(rng.uniform(0.01, 0.2)),
+                                0.02, float(rng.uniform(0.05, 0.25)))] for _ in range(M)]
+    # power-flow continuation from zero load in steps of 1/4 (halved on
+    # failure, down to 1/32), at most 25 Newton iterations per attempt
+    return _finish(name, ps, psd, M, events, [tf] * M, meta, zip_alpha=1.0, size_machines=True,
+                   pf_kw={"maxit": 25, "first": 0.0, "min_step": 1.0 / 32})
 
 
 def case3_200bus(M=128, tf=3.0):
