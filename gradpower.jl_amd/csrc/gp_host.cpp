@@ -479,6 +479,7 @@ struct gp_ctx {
   // after its Schur assembly, joined before the next use of their outputs
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_side = nullptr, ev_fork2 = nullptr, ev_join = nullptr;
+  hipEvent_t ev_round = nullptr;   // end of a round's exchange, polled by the host
   bool has64 = false;   // luS64 fits the LU kernels' LDS
   DBuf<uint8_t> d_split;   // a round's masks split by factor precision [6][L]
   // discrete adjoint: LU plan of J^T (values read from d_jv), its factors,
@@ -1138,6 +1139,18 @@ double prof_collect(gp_ctx::KProf &P, int64_t &launches, int64_t &scen) {
   return tot;
 }
 
+// Waits for everything enqueued on s by polling an event recorded behind it:
+// a round's exchange is short, and polling wakes the host sooner than a
+// blocking synchronisation.
+void spin_sync(hipStream_t s, hipEvent_t ev) {
+  HIPCHK(hipEventRecord(ev, s));
+  for (;;) {
+    const hipError_t q = hipEventQuery(ev);
+    if (q == hipSuccess) return;
+    if (q != hipErrorNotReady) HIPCHK(q);
+  }
+}
+
 struct NewtonOut {
   int64_t corrections = 0;
   int64_t refac = 0;
@@ -1181,19 +1194,28 @@ int stage_batch(gp_ctx *C, int l, bool fp32, const uint8_t *exacth) {
   int nb = 0;
   for (int m = 0; m < L; m++) nb += (C->queue[m] && lane_class(C, fp32, exacth, m) == l) ? 1 : 0;
   if (nb == 0) return -1;
-  uint8_t *slot = C->h_batch.p + (size_t)C->batch_slot * 2 * L;
-  C->batch_slot = (C->batch_slot + 1) % gp_ctx::kBatchSlots;
   for (int m = 0; m < L; m++) {
     ln.batch[m] = (C->queue[m] && lane_class(C, fp32, exacth, m) == l) ? 1 : 0;
-    slot[m] = ln.batch[m];
-    slot[L + m] = (uint8_t)(1 - C->cur[m]);
     if (ln.batch[m]) {
       C->pend[m] = 1;
       C->queue[m] = 0;
     }
   }
-  HIPCHK(hipMemcpyAsync(ln.req, slot, 2 * L, hipMemcpyHostToDevice, C->stream));
-  launch_async_flags(L, 1, ln.req, C->d_cur.p, C->d_pend.p, C->stream);
+  if (L <= kStageBits) {   // the mask travels as kernel arguments
+    ScenarioBits bits{};
+    for (int m = 0; m < L; m++)
+      if (ln.batch[m]) bits.w[m >> 5] |= 1u << (m & 31);
+    launch_stage_batch(L, bits, ln.req, ln.tgt, C->d_cur.p, C->d_pend.p, C->stream);
+  } else {
+    uint8_t *slot = C->h_batch.p + (size_t)C->batch_slot * 2 * L;
+    C->batch_slot = (C->batch_slot + 1) % gp_ctx::kBatchSlots;
+    for (int m = 0; m < L; m++) {
+      slot[m] = ln.batch[m];
+      slot[L + m] = (uint8_t)(1 - C->cur[m]);
+    }
+    HIPCHK(hipMemcpyAsync(ln.req, slot, 2 * L, hipMemcpyHostToDevice, C->stream));
+    launch_async_flags(L, 1, ln.req, C->d_cur.p, C->d_pend.p, C->stream);
+  }
   launch_copy_active(std::max(1, C->n_svar), L, C->d_svar.p, ln.snap.p, ln.req, C->stream);
   HIPCHK(hipEventRecord(ln.ev_snap, C->stream));
   ln.nb = nb;
@@ -1215,31 +1237,35 @@ void start_batch(gp_ctx *C, int l, TF *luv, TF *sv, TF *dn) {
 }
 
 // The batches in flight that have finished (all of them with `wait`): their
-// scenarios switch buffers.  Returns whether any landed.
-bool land_batch(gp_ctx *C, bool wait) {
+// scenarios switch buffers.  With `wait` the context's stream waits for the
+// batches (the host does not): what it runs next sees them landed.  Returns
+// whether any landed.
+bool land_lane(gp_ctx *C, int l, bool wait) {
   const int L = C->ldM;
-  bool any = false;
-  for (auto &ln : C->lanes) {
-    if (!ln.inflight) continue;
-    if (wait) {
-      HIPCHK(hipEventSynchronize(ln.ev_done));
-    } else {
-      const hipError_t q = hipEventQuery(ln.ev_done);
-      if (q == hipErrorNotReady) continue;
-      HIPCHK(q);
-    }
-    for (int m = 0; m < L; m++)
-      if (ln.batch[m]) {
-        C->cur[m] = (uint8_t)(1 - C->cur[m]);
-        C->pend[m] = 0;
-        ln.batch[m] = 0;
-      }
-    // the same update on the device, from the batch's request mask (no
-    // longer read by the finished factorisation)
-    launch_async_flags(L, 0, ln.req, C->d_cur.p, C->d_pend.p, C->stream);
-    ln.inflight = false;
-    any = true;
+  gp_ctx::Lane &ln = C->lanes[l];
+  if (!ln.inflight) return false;
+  if (wait) {
+    HIPCHK(hipStreamWaitEvent(C->stream, ln.ev_done, 0));
+  } else {
+    const hipError_t q = hipEventQuery(ln.ev_done);
+    if (q == hipErrorNotReady) return false;
+    HIPCHK(q);
   }
+  for (int m = 0; m < L; m++)
+    if (ln.batch[m]) {
+      C->cur[m] = (uint8_t)(1 - C->cur[m]);
+      C->pend[m] = 0;
+      ln.batch[m] = 0;
+    }
+  // the same update on the device, from the batch's request mask (no
+  // longer read by the finished factorisation)
+  launch_async_flags(L, 0, ln.req, C->d_cur.p, C->d_pend.p, C->stream);
+  ln.inflight = false;
+  return true;
+}
+bool land_batch(gp_ctx *C, bool wait) {
+  bool any = false;
+  for (int l = 0; l < gp_ctx::kLanes; l++) any = land_lane(C, l, wait) || any;
   return any;
 }
 
@@ -1425,7 +1451,15 @@ struct Newton {
       // the batch in flight, which lands first, and their queued requests
       // (made for the old system) are dropped
       if (async && land_first) {
-        land_batch(C, true);
+        // a batch in flight that holds a scenario starting a solve of its
+        // own lands first (the factorisation below writes the buffer that
+        // landing would switch that scenario to); the others stay in flight
+        for (int l = 0; l < gp_ctx::kLanes; l++) {
+          const gp_ctx::Lane &ln = C->lanes[l];
+          bool hit = ln.inflight && !fresh;
+          for (int m = 0; m < L && ln.inflight && !hit; m++) hit = ln.batch[m] && (*fresh)[m];
+          if (hit) land_lane(C, l, true);
+        }
         if (fresh) {   // fresh is on the device as this round's refac0 mask
           for (int m = 0; m < L; m++)
             if ((*fresh)[m]) C->queue[m] = C->pend[m] = 0;
@@ -1871,7 +1905,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
     N.eval(0, C->d_first.p, C->d_refac0.p, nullptr, true);
     HIPCHK(hipMemcpyAsync(hr + R.down_off(), C->d_round.p + R.down_off(), R.down_bytes(), hipMemcpyDeviceToHost,
                           C->stream));
-    HIPCHK(hipStreamSynchronize(C->stream));
+    spin_sync(C->stream, C->ev_round);
     int cnt[4];
     std::memcpy(cnt, hr + R.cnt(), sizeof(cnt));
     std::memcpy(wk.data(), hr + R.work(), L);
@@ -2192,7 +2226,7 @@ int gp_create(const gp_system_desc *d, int32_t M, const gp_options *opt, gp_ctx 
     HIPCHK(hipGetDevice(&C->device));
     HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&C->side, hipStreamNonBlocking));
-    for (hipEvent_t *e : {&C->ev_fork, &C->ev_side, &C->ev_fork2, &C->ev_join})
+    for (hipEvent_t *e : {&C->ev_fork, &C->ev_side, &C->ev_fork2, &C->ev_join, &C->ev_round})
       HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
     for (auto &ln : C->lanes) {
       HIPCHK(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
@@ -2271,7 +2305,7 @@ void gp_destroy(gp_ctx *C) {
     le.push_back(ln.ev_done);
   }
   if (C->side) (void)hipStreamSynchronize(C->side);
-  for (hipEvent_t e : {C->ev_fork, C->ev_side, C->ev_fork2, C->ev_join})
+  for (hipEvent_t e : {C->ev_fork, C->ev_side, C->ev_fork2, C->ev_join, C->ev_round})
     if (e) (void)hipEventDestroy(e);
   if (C->side) (void)hipStreamDestroy(C->side);
   for (hipEvent_t e : C->prof_solve.ev) (void)hipEventDestroy(e);

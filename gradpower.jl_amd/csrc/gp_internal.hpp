@@ -154,6 +154,14 @@ void launch_copy_cols(int n, int ldM, const double *src, double *dst, size_t str
 // asynchronous-refactorisation flags: mode 0 landed (cur ^= 1, pend = 0),
 // 1 requested (pend = 1), 2 dropped (pend = 0), for the scenarios in a
 void launch_async_flags(int L, int mode, const uint8_t *a, uint8_t *cur, uint8_t *pend, hipStream_t s);
+// a batch's request mask as kernel arguments: one bit per scenario, up to
+// kStageBits scenarios (larger batches upload their mask instead)
+constexpr int kStageBits = 2048;
+struct ScenarioBits {
+  uint32_t w[kStageBits / 32];
+};
+void launch_stage_batch(int L, const ScenarioBits &bits, uint8_t *req, uint8_t *tgt, const uint8_t *cur,
+                        uint8_t *pend, hipStream_t s);
 // the round's (sch, refac, upd) masks split by factor precision: out [6][L],
 // FP32 scenarios' masks then those of the scenarios with x64[m] != 0
 void launch_split_precision(int L, const uint8_t *x64, const uint8_t *sch, const uint8_t *refac, const uint8_t *upd,

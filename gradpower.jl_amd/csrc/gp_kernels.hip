@@ -2466,6 +2466,23 @@ __global__ void k_async_flags(int L, int mode, const uint8_t *__restrict__ a, ui
   }
 }
 
+// A batch's masks from a bit set passed by value (no copy from the host):
+// req[m] = bit m, tgt[m] = the buffer the solves of m do not read, and the
+// requested scenarios become pending.
+__global__ void k_stage_batch(int L, ScenarioBits bits, uint8_t *__restrict__ req, uint8_t *__restrict__ tgt,
+                              const uint8_t *__restrict__ cur, uint8_t *__restrict__ pend) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= L) return;
+  const uint8_t r = (uint8_t)((bits.w[m >> 5] >> (m & 31)) & 1u);
+  req[m] = r;
+  tgt[m] = (uint8_t)(1 - cur[m]);
+  if (r) pend[m] = 1;
+}
+void launch_stage_batch(int L, const ScenarioBits &bits, uint8_t *req, uint8_t *tgt, const uint8_t *cur,
+                        uint8_t *pend, hipStream_t s) {
+  hipLaunchKernelGGL(k_stage_batch, dim3((L + 255) / 256), dim3(256), 0, s, L, bits, req, tgt, cur, pend);
+}
+
 // Splits the round's Schur, refactorisation and correction masks by the
 // precision of each scenario's factors: out[0..2] = (sch, refac, upd) of the
 // scenarios with FP32 factors, out[3..5] those with FP64 factors (x64[m]).
