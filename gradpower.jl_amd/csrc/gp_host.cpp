@@ -466,8 +466,8 @@ struct gp_ctx {
   // kernel timing (gp_options.profile): event pairs around each launch
   struct KProf {
     std::vector<hipEvent_t> ev;
+    std::vector<int64_t> scen;   // per pair: the scenarios its launch worked on (0: none, not counted)
     size_t used = 0;
-    int64_t launches = 0, scen = 0;
   } prof_solve, prof_factor;
   DBuf<unsigned long long> d_nb;
   DevTables T{};
@@ -1103,7 +1103,9 @@ void ensure_schur_buffers(gp_ctx *C, bool fp32) {
   }
 }
 
-// Events around one launch of a timed kernel (gp_options.profile).
+// Events around one launch of a timed kernel (gp_options.profile).  A
+// launch whose scenario count is not known when it goes out gets it later
+// (prof_count); launches that worked on no scenario are left out.
 int prof_begin(gp_ctx *C, gp_ctx::KProf &P, hipStream_t s) {
   if (!C->opt.profile) return -1;
   if (P.used + 2 > P.ev.size()) {
@@ -1112,38 +1114,42 @@ int prof_begin(gp_ctx *C, gp_ctx::KProf &P, hipStream_t s) {
       HIPCHK(hipEventCreate(&e));
       P.ev.push_back(e);
     }
+    P.scen.resize(P.ev.size() / 2, 0);
   }
   const int i = (int)P.used;
   P.used += 2;
+  P.scen[i / 2] = 0;
   HIPCHK(hipEventRecord(P.ev[i], s));
   return i;
 }
 void prof_end(gp_ctx::KProf &P, int i, hipStream_t s, int64_t scenarios) {
   if (i < 0) return;
   HIPCHK(hipEventRecord(P.ev[i + 1], s));
-  P.launches++;
-  P.scen += scenarios;
+  P.scen[i / 2] = scenarios;
+}
+void prof_count(gp_ctx::KProf &P, int i, int64_t scenarios) {
+  if (i >= 0) P.scen[i / 2] = scenarios;
 }
 // Total milliseconds of the recorded pairs (all complete), then reset.
 double prof_collect(gp_ctx::KProf &P, int64_t &launches, int64_t &scen) {
   double tot = 0.0;
+  launches = scen = 0;
   for (size_t i = 0; i + 1 < P.used; i += 2) {
+    if (P.scen[i / 2] <= 0) continue;
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, P.ev[i], P.ev[i + 1]));
     tot += ms;
+    launches++;
+    scen += P.scen[i / 2];
   }
-  launches = P.launches;
-  scen = P.scen;
   P.used = 0;
-  P.launches = P.scen = 0;
   return tot;
 }
 
-// Waits for everything enqueued on s by polling an event recorded behind it:
+// Waits for an event recorded on a stream by polling it:
 // a round's exchange is short, and polling wakes the host sooner than a
 // blocking synchronisation.
-void spin_sync(hipStream_t s, hipEvent_t ev) {
-  HIPCHK(hipEventRecord(ev, s));
+void spin_wait(hipEvent_t ev) {
   for (;;) {
     const hipError_t q = hipEventQuery(ev);
     if (q == hipSuccess) return;
@@ -1376,59 +1382,61 @@ struct Newton {
   }
   void launch_queue() { start_queue(stage_queue()); }
 
-  // One round's corrections after eval() (cnt read back).  land_first: a
+  // One round's corrections after eval(), in two halves: launch_pre() puts
+  // the Schur kernels and the synchronous factorisation on the streams,
+  // launch_post() the staging of asynchronous requests, the solve and the
+  // back-substitution.  The scenario-asynchronous loop calls launch_pre()
+  // before the round's decisions come back (cnt NULL: every kernel goes out
+  // and works from the device masks alone, a launch with an empty mask
+  // returning at once), so the device runs it while the host waits for the
+  // decisions, and launch_post() with k_norm_finish's counts (cnt[] as
+  // n_unconv), which stages this round's requests at once.  absorb() then
+  // takes the counts into the statistics and handles the rounds in which
+  // every remaining scenario waits for a factorisation.  land_first: a
   // scenario starts a solve that refactorises (an event or a change of dt):
-  // a batch in flight lands first and the queued requests of the scenarios
-  // in `fresh` (made for the old system) are dropped.  Returns false when
-  // nothing can change any more.
-  bool correct(const int cnt[4], bool land_first, const std::vector<uint8_t> *fresh) {
+  // a batch in flight that holds one of the scenarios in `fresh` lands first
+  // and their queued requests (made for the old system) are dropped.
+  int prof_f = -1;
+  bool landed = false, skip = false;
+  bool mixed = false, run32 = false, run64 = false;
+  const uint8_t *upd32 = nullptr, *upd64 = nullptr;
+  void launch_pre(const int *cnt, bool land_first, const std::vector<uint8_t> *fresh) {
     const int L = C->ldM;
     const double *p = C->d_p.p;
     double *z = C->d_z.p;
-    out.corrections += cnt[2];
-    out.rounds++;
-    const bool landed = async && land_batch(C, false);
-    if (cnt[2] == 0 && cnt[3] == 0) {
-      // every remaining scenario waits for its factorisation
-      if (async && free_lane(C) >= 0 && queued()) launch_queue();
-      if (any_inflight(C))
-        land_batch(C, true);
-      else if (!landed)
-        return false;   // nothing will change (cannot happen)
-      return true;
-    }
+    const bool known = cnt != nullptr;
+    prof_f = -1;
+    mixed = false;
+    skip = known && cnt[2] == 0 && cnt[3] == 0;   // every remaining scenario waits (absorb)
+    if (skip) return;
+    const bool fac = !known || cnt[1] > 0;
     if (method == GP_METHOD_MONO_FP64) {
-      out.refac += cnt[1];
-      if (cnt[1] > 0) out.factor_launches++;
       launch_jac_mono(C->T, p, z, dt, faults(C), C->d_jv.p, C->d_upd.p, C->stream, dtv);
-      int pi = prof_begin(C, C->prof_factor, C->stream);
+      prof_f = prof_begin(C, C->prof_factor, C->stream);
       launch_splu_factor_impl<double>(C->luJ.dev, C->d_jv.p, nullptr, L, C->d_luJ.p, C->d_svJ.p, C->d_dnJ.p,
                                       C->d_refac.p, nullptr, C->M, C->d_status.p, C->stream);
-      prof_end(C->prof_factor, pi, C->stream, cnt[1]);
-      pi = prof_begin(C, C->prof_solve, C->stream);
-      launch_splu_solve<double>(C->luJ.dev, C->d_svJ.p, C->d_dnJ.p, C->d_f.p, L, -1.0, nullptr, z, nullptr,
-                                C->d_upd.p, nullptr, C->M, C->stream);
-      prof_end(C->prof_solve, pi, C->stream, cnt[2]);
-      HIPCHK(hipGetLastError());
-      return true;
+      prof_end(C->prof_factor, prof_f, C->stream, known ? cnt[1] : 0);
+      return;
     }
     // factor precision per scenario: FP32 for the FP32 method, except the
     // scenarios running exact Newton (exacth), whose factors are FP64
     const bool use32 = fp32 && !all64;
-    bool mixed = false;
     // (a scenario waiting for a factorisation in flight applies nothing this
     // round: no FP64 launches for it)
     if (use32 && exacth && C->has64)
       for (int m = 0; m < L && !mixed; m++) mixed = exacth[m] && (!workh || workh[m]) && !C->pend[m];
-    const uint8_t *sch32 = C->d_sch.p, *ref32 = C->d_refac.p, *upd32 = C->d_upd.p;
-    const uint8_t *sch64 = sch32, *ref64 = ref32, *upd64 = upd32;
+    const uint8_t *sch32 = C->d_sch.p, *ref32 = C->d_refac.p;
+    upd32 = C->d_upd.p;
+    const uint8_t *sch64 = sch32, *ref64 = ref32;
+    upd64 = upd32;
     if (mixed) {
       launch_split_precision(L, exactv, C->d_sch.p, C->d_refac.p, C->d_upd.p, C->d_split.p, C->stream);
       const uint8_t *b = C->d_split.p;
       sch32 = b; ref32 = b + L; upd32 = b + 2 * L;
       sch64 = b + 3 * L; ref64 = b + 4 * L; upd64 = b + 5 * L;
     }
-    const bool run32 = use32, run64 = !use32 || mixed;
+    run32 = use32;
+    run64 = !use32 || mixed;
     // the FP64 kernels' stream: the side stream beside the FP32 kernels
     // (they touch other scenarios' columns only), else the main stream
     hipStream_t s64 = mixed ? C->side : C->stream;
@@ -1444,35 +1452,31 @@ struct Newton {
       HIPCHK(hipStreamWaitEvent(C->stream, C->ev_side, 0));
     }
     launch_bus_schur(C->T, p, z, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_sch.p, C->stream);
-    if (cnt[1] > 0) {
-      // synchronous factorisation into the buffer the solves read.  Later
-      // iterations ask only for scenarios with nothing pending; the first one
-      // of a solve after an event or a change of dt may include scenarios in
-      // the batch in flight, which lands first, and their queued requests
-      // (made for the old system) are dropped
-      if (async && land_first) {
-        // a batch in flight that holds a scenario starting a solve of its
-        // own lands first (the factorisation below writes the buffer that
-        // landing would switch that scenario to); the others stay in flight
-        for (int l = 0; l < gp_ctx::kLanes; l++) {
-          const gp_ctx::Lane &ln = C->lanes[l];
-          bool hit = ln.inflight && !fresh;
-          for (int m = 0; m < L && ln.inflight && !hit; m++) hit = ln.batch[m] && (*fresh)[m];
-          if (hit) land_lane(C, l, true);
-        }
-        if (fresh) {   // fresh is on the device as this round's refac0 mask
-          for (int m = 0; m < L; m++)
-            if ((*fresh)[m]) C->queue[m] = C->pend[m] = 0;
-          launch_async_flags(L, 2, C->d_refac0.p, C->d_cur.p, C->d_pend.p, C->stream);
-        }
+    // synchronous factorisation into the buffer the solves read.  Later
+    // iterations ask only for scenarios with nothing pending; the first one
+    // of a solve after an event or a change of dt may include scenarios in
+    // the batch in flight, which lands first, and their queued requests
+    // (made for the old system) are dropped
+    if (fac && async && land_first) {
+      // a batch in flight that holds a scenario starting a solve of its
+      // own lands first (the factorisation below writes the buffer that
+      // landing would switch that scenario to); the others stay in flight
+      for (int l = 0; l < gp_ctx::kLanes; l++) {
+        const gp_ctx::Lane &ln = C->lanes[l];
+        bool hit = ln.inflight && !fresh;
+        for (int m = 0; m < L && ln.inflight && !hit; m++) hit = ln.batch[m] && (*fresh)[m];
+        if (hit) land_lane(C, l, true);
       }
-      out.refac += cnt[1];
-      out.factor_launches++;
+      if (fresh) {   // fresh is on the device as this round's refac0 mask
+        for (int m = 0; m < L; m++)
+          if ((*fresh)[m]) C->queue[m] = C->pend[m] = 0;
+        launch_async_flags(L, 2, C->d_refac0.p, C->d_cur.p, C->d_pend.p, C->stream);
+      }
     }
     if (mixed) {   // the FP64 scenarios' factorisation, solve and back-substitution on the side stream
       HIPCHK(hipEventRecord(C->ev_fork2, C->stream));
       HIPCHK(hipStreamWaitEvent(C->side, C->ev_fork2, 0));
-      if (cnt[1] > 0)
+      if (fac)
         launch_splu_factor_impl<double>(C->luS64, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
                                         C->d_dnS64.p, ref64, C->d_cur.p, C->M, C->d_status.p, s64);
       launch_splu_solve<double>(C->luS64, C->d_svS64.p, C->d_dnS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
@@ -1480,40 +1484,88 @@ struct Newton {
       launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, upd64, s64);
       HIPCHK(hipEventRecord(C->ev_join, C->side));
     }
-    if (cnt[1] > 0) {
-      const int pi = prof_begin(C, C->prof_factor, C->stream);
+    if (fac) {
+      prof_f = prof_begin(C, C->prof_factor, C->stream);
       if (run32)
         launch_splu_factor_impl<float>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS32.p, C->d_svS32.p,
                                        C->d_dnS32.p, ref32, C->d_cur.p, C->M, C->d_status.p, C->stream);
       if (run64 && !mixed)
         launch_splu_factor_impl<double>(C->luS64, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
                                         C->d_dnS64.p, ref64, C->d_cur.p, C->M, C->d_status.p, C->stream);
-      prof_end(C->prof_factor, pi, C->stream, cnt[1]);
+      prof_end(C->prof_factor, prof_f, C->stream, known ? cnt[1] : 0);
     }
-    if (async && cnt[3] > 0) {
-      // asynchronous requests: queued (and pending: the scenario waits when
-      // its |F| stops decreasing), launched when no batch is in flight
-      for (int m = 0; m < L; m++)
-        if (reqp[m]) C->queue[m] = C->pend[m] = 1;
-      launch_async_flags(L, 1, C->d_areq.p, C->d_cur.p, C->d_pend.p, C->stream);
+    HIPCHK(hipGetLastError());
+  }
+  void launch_post(const int cnt[4]) {
+    const int L = C->ldM;
+    double *z = C->d_z.p;
+    // the batches that have finished by now switch their scenarios' buffers
+    // before the solve (none of those scenarios refactorised this round:
+    // pending scenarios ask for nothing, and a fresh one's batch landed in
+    // launch_pre)
+    landed = async && land_batch(C, false);
+    if (skip) return;
+    if (method == GP_METHOD_MONO_FP64) {
+      if (cnt[2] > 0) {
+        const int ps = prof_begin(C, C->prof_solve, C->stream);
+        launch_splu_solve<double>(C->luJ.dev, C->d_svJ.p, C->d_dnJ.p, C->d_f.p, L, -1.0, nullptr, z, nullptr,
+                                  C->d_upd.p, nullptr, C->M, C->stream);
+        prof_end(C->prof_solve, ps, C->stream, cnt[2]);
+      }
+      HIPCHK(hipGetLastError());
+      return;
     }
+    // asynchronous requests (pending on the device from k_norm_finish on:
+    // the scenario waits when its |F| stops decreasing) join the host queue
+    // and are staged when a lane is free
+    if (async && cnt[3] > 0) take_requests();
     const int lane = (async && free_lane(C) >= 0 && queued()) ? stage_queue() : 0;
-    const int ps = cnt[2] > 0 ? prof_begin(C, C->prof_solve, C->stream) : -1;
-    if (run32)
-      launch_splu_solve<float>(C->luS.dev, C->d_svS32.p, C->d_dnS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
-                               C->d_dv.p, upd32, C->d_cur.p, C->M, C->stream);
-    if (run64 && !mixed)
-      launch_splu_solve<double>(C->luS64, C->d_svS64.p, C->d_dnS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
-                                C->d_dv.p, upd64, C->d_cur.p, C->M, C->stream);
-    prof_end(C->prof_solve, ps, C->stream, cnt[2]);
-    if (run32)
-      launch_cluster_backsub<float>(C->T, C->d_X32.p, C->d_f.p, C->d_dv.p, z, upd32, C->stream);
-    if (run64 && !mixed)
-      launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, upd64, C->stream);
+    if (cnt[2] > 0) {
+      const int ps = prof_begin(C, C->prof_solve, C->stream);
+      if (run32)
+        launch_splu_solve<float>(C->luS.dev, C->d_svS32.p, C->d_dnS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
+                                 C->d_dv.p, upd32, C->d_cur.p, C->M, C->stream);
+      if (run64 && !mixed)
+        launch_splu_solve<double>(C->luS64, C->d_svS64.p, C->d_dnS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
+                                  C->d_dv.p, upd64, C->d_cur.p, C->M, C->stream);
+      prof_end(C->prof_solve, ps, C->stream, cnt[2]);
+      if (run32)
+        launch_cluster_backsub<float>(C->T, C->d_X32.p, C->d_f.p, C->d_dv.p, z, upd32, C->stream);
+      if (run64 && !mixed)
+        launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, upd64, C->stream);
+    }
     start_queue(lane);
     if (mixed) HIPCHK(hipStreamWaitEvent(C->stream, C->ev_join, 0));   // the side stream's work joins the round
     HIPCHK(hipGetLastError());
+  }
+  // the round's asynchronous requests (host copy reqp) into the queue
+  void take_requests() {
+    for (int m = 0; m < C->ldM; m++)
+      if (reqp[m]) C->queue[m] = C->pend[m] = 1;
+  }
+  // After launch() and the counts: false when nothing can change any more.
+  bool absorb(const int cnt[4]) {
+    out.corrections += cnt[2];
+    out.rounds++;
+    if (cnt[1] > 0) {
+      out.refac += cnt[1];
+      out.factor_launches++;
+    }
+    prof_count(C->prof_factor, prof_f, cnt[1]);
+    if (cnt[2] == 0 && cnt[3] == 0) {
+      // every remaining scenario waits for its factorisation
+      if (async && free_lane(C) >= 0 && queued()) launch_queue();
+      if (any_inflight(C))
+        land_batch(C, true);
+      else if (!landed)
+        return false;   // nothing will change (cannot happen)
+    }
     return true;
+  }
+  bool correct(const int cnt[4], bool land_first, const std::vector<uint8_t> *fresh) {
+    launch_pre(cnt, land_first, fresh);
+    launch_post(cnt);
+    return absorb(cnt);
   }
 };
 
@@ -1905,17 +1957,20 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
     N.eval(0, C->d_first.p, C->d_refac0.p, nullptr, true);
     HIPCHK(hipMemcpyAsync(hr + R.down_off(), C->d_round.p + R.down_off(), R.down_bytes(), hipMemcpyDeviceToHost,
                           C->stream));
-    spin_sync(C->stream, C->ev_round);
+    HIPCHK(hipEventRecord(C->ev_round, C->stream));
+    // the round's corrections go out before its decisions come back (the
+    // device masks select their scenarios), so the device runs them while
+    // the host reads the decisions and accepts the converged scenarios
+    N.workh = run.data();
+    N.launch_pre(nullptr, land_first, &fresh);
+    spin_wait(C->ev_round);
     int cnt[4];
     std::memcpy(cnt, hr + R.cnt(), sizeof(cnt));
     std::memcpy(wk.data(), hr + R.work(), L);
     std::memcpy(fl.data(), hr + R.failed(), L);
     N.reqp = hr + R.areq();
-    N.workh = wk.data();
-    // corrections of the scenarios still iterating run while the host
-    // accepts the converged ones
-    if (cnt[0] > 0 && !N.correct(cnt, land_first, &fresh))
-      throw GpError{GP_ERR_HIP, "internal: no scenario can make progress"};
+    N.launch_post(cnt);
+    if (cnt[0] > 0 && !N.absorb(cnt)) throw GpError{GP_ERR_HIP, "internal: no scenario can make progress"};
     std::memset(acc, 0, L);
     std::memset(adv, 0, L);
     std::memset(sv, 0, L);

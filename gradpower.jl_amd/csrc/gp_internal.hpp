@@ -121,7 +121,7 @@ void launch_bus_residual(const DevTables &T, const double *p, const double *z, c
 void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, double tol, double *fnorm,
                          unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
                          double theta, int stall, int itermax, double *fbest, int *nstall, int *iters,
-                         uint8_t *failed, const uint8_t *pend, uint8_t *upd, double async_below, uint8_t *areq,
+                         uint8_t *failed, uint8_t *pend, uint8_t *upd, double async_below, uint8_t *areq,
                          uint8_t *sch, hipStream_t s, const uint8_t *firstv = nullptr,
                          const uint8_t *exact = nullptr, int switch_window = 0);
 void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs,

@@ -488,7 +488,7 @@ __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict_
                               const uint8_t *__restrict__ refac0, int first0, const uint8_t *__restrict__ firstv,
                               double theta, int stall, int itermax,
                               double *__restrict__ fbest, int *__restrict__ nstall, int *__restrict__ iters,
-                              uint8_t *__restrict__ failed, const uint8_t *__restrict__ pend,
+                              uint8_t *__restrict__ failed, uint8_t *__restrict__ pend,
                               uint8_t *__restrict__ upd, double async_below, uint8_t *__restrict__ areq,
                               uint8_t *__restrict__ sch, const uint8_t *__restrict__ exact, int switch_window) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -542,7 +542,8 @@ __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict_
   // the host lands and drops that request before this round's factorisation
   bool wait = pending && !(first && rf);
   if (rf && !first && async_below > 0.0 && v < async_below) {
-    areq[m] = 1;
+    areq[m] = 1;   // requested: pending until its batch lands
+    pend[m] = 1;
     atomicAdd(n_unconv + 3, 1);
     wait = true;
   } else if (rf) {
@@ -2571,7 +2572,7 @@ void launch_bus_residual(const DevTables &T, const double *p, const double *z, c
 void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, double tol, double *fnorm,
                          unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
                          double theta, int stall, int itermax, double *fbest, int *nstall, int *iters,
-                         uint8_t *failed, const uint8_t *pend, uint8_t *upd, double async_below, uint8_t *areq,
+                         uint8_t *failed, uint8_t *pend, uint8_t *upd, double async_below, uint8_t *areq,
                          uint8_t *sch, hipStream_t s, const uint8_t *firstv, const uint8_t *exact,
                          int switch_window) {
   const int chunks = std::max(1, std::min((n + 15) / 16, 1024 / (ldM / WAVE)));
