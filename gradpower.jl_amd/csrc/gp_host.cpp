@@ -136,7 +136,8 @@ struct LUPlan {
   SpluAnalysis A;
   SpluDev dev{};
   DBuf<int> a2lu, a_src, op_loc, op_sl, st_o, flp, fac_mode, crec, lu2s, dmap, fw_lvl, fw_chunk, fw_cols, fw_rows,
-      bw_lvl, bw_chunk, bw_cols, bw_rows, q, pinv, r_hdr, r_blob, r_a2x, r_omap, sf_hdr, sf_blob, sb_hdr, sb_blob;
+      bw_lvl, bw_chunk, bw_cols, bw_rows, q, pinv, r_hdr, r_blob, r_a2x, r_omap, r_xsrc, sf_hdr, sf_blob, sb_hdr,
+      sb_blob;
   ResPlan rp;                  // LDS-resident factorisation plan
   bool rp_ok = false;
   // LU values per scenario the factorisation kernels use
@@ -209,6 +210,11 @@ struct LUPlan {
       r_blob.upload(rp.blob);
       r_a2x.upload(rp.a2x);
       r_omap.upload(rp.omap);
+      // initial value source of each step-ordered slot (four spare slots:
+      // the staged loads read whole groups of four)
+      std::vector<int> xs((size_t)rp.xtot + 4, kResNoSrc);
+      for (int e = 0; e < A.nnz_a; e++) xs[rp.a2x[e]] = asrc[e];
+      r_xsrc.upload(xs);
     }
     SpluDev &d = dev;
     d.n = A.n; d.nnz_a = A.nnz_a; d.nnz_lu = A.nnz_lu; d.ns = std::max(1, A.ns);
@@ -222,6 +228,7 @@ struct LUPlan {
     d.fw_lvl = fw_lvl.p; d.fw_chunk = fw_chunk.p; d.fw_cols = fw_cols.p; d.fw_rows = fw_rows.p;
     d.bw_lvl = bw_lvl.p; d.bw_chunk = bw_chunk.p; d.bw_cols = bw_cols.p; d.bw_rows = bw_rows.p;
     d.q = q.p; d.pinv = pinv.p;
+    d.res_xsrc = rp_ok ? r_xsrc.p : nullptr;
     d.threads = kThreads;
     d.gx = gx_needed() ? 1 : 0;
     d.gx_ws = nullptr;   // the context's [n][ldM] workspace, set after the plan is made

@@ -98,7 +98,12 @@ struct SpluDev {
   double *gx_ws;
   const int *sol_fw_hdr, *sol_fw_blob, *sol_bw_hdr, *sol_bw_blob;
   const int *res_hdr, *res_blob, *res_a2x, *res_omap;
+  // per step-ordered value slot of the LDS-resident factorisation: where its
+  // initial value comes from (a_src code: var row >= 0, constant -1 - i, or
+  // kResNoSrc for a fill entry, which starts at zero)
+  const int *res_xsrc;
 };
+constexpr int kResNoSrc = -2147483647 - 1;
 // staging of the LDS-resident factorisation, per thread of a 512-thread
 // workgroup: blob dwords and initial column values
 constexpr int kResCB = 24, kResCX = 8;

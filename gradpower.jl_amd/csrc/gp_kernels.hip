@@ -1283,19 +1283,6 @@ __global__ void __launch_bounds__(64) k_cluster_backsub(DevTables T, const TF *_
 // One workgroup per scenario.  Workgroups are dispatched to the 8 XCDs round
 // robin; the map below gives the workgroups of one XCD consecutive scenarios,
 // so the scenario-major vectors they gather from share that XCD's L2 lines.
-#ifdef GP_LU_TRACE
-// Level timestamps of workgroup 0 (microbenchmark builds only).
-__device__ long long gp_lu_trace[2][4096];
-#define GP_TRACE(which, idx)                                             \
-  do {                                                                   \
-    if (threadIdx.x == 0 && blockIdx.x == 0 && (idx) < 4096)             \
-      gp_lu_trace[which][idx] = (long long)__builtin_readcyclecounter(); \
-  } while (0)
-#else
-#define GP_TRACE(which, idx) \
-  do {                       \
-  } while (0)
-#endif
 
 __device__ __forceinline__ int scenario_of_block(int b, int nb) {
   if (nb & 7) return b;
@@ -1690,6 +1677,58 @@ __device__ __forceinline__ int dense_lu(TF *D, int nd, int tid, TF *lbuf, TF *ub
   return bad;
 }
 
+// The same LU for nd <= 64 on the first wavefront alone, with no workgroup
+// barrier per pivot: lane i holds row i of the block in registers (NDC >= nd
+// columns, the rest zero); the pivot row is read lane by lane (readlane),
+// each lane below the pivot scales its entry in the pivot column and updates
+// the columns right of it.  Same arithmetic as dense_lu (l = a * (1/piv),
+// a -= l * u, rows on and above the pivot with l = 0).  Ends with a
+// workgroup barrier.
+template <typename TF>
+__device__ __forceinline__ TF readlane_t(TF v, int lane) {
+  if constexpr (sizeof(TF) == 8) {
+    return readlane_d(v, lane);
+  } else {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+  }
+}
+template <typename TF, int NDC>
+__device__ int dense_lu_wave(TF *D, int nd, int tid) {
+  int bad = 0;
+  if (tid < WAVE) {
+    const int i = tid;
+    TF a[NDC];
+#pragma unroll
+    for (int j = 0; j < NDC; j++) a[j] = (i < nd && j < nd) ? D[i + j * nd] : TF(0);
+#pragma unroll
+    for (int p = 0; p < NDC; p++) {
+      if (p < nd) {
+        const TF piv = readlane_t(a[p], p);
+        if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
+        const TF inv = TF(1) / piv;
+        const TF l = i > p ? a[p] * inv : TF(0);
+        if (i > p) a[p] = l;
+#pragma unroll
+        for (int j = p + 1; j < NDC; j++) a[j] = a[j] - l * readlane_t(a[j], p);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NDC; j++)
+      if (i < nd && j < nd) D[i + j * nd] = a[j];
+  }
+  __syncthreads();
+  return bad;
+}
+template <typename TF>
+__device__ int dense_lu_any(TF *D, int nd, int tid, TF *lbuf) {
+  if (nd <= 16) return dense_lu_wave<TF, 16>(D, nd, tid);
+  if (nd <= 32) return dense_lu_wave<TF, 32>(D, nd, tid);
+  if (nd <= 48) return dense_lu_wave<TF, 48>(D, nd, tid);
+  // (64 FP64 values per lane would spill)
+  if (sizeof(TF) == 4 && nd <= 64) return dense_lu_wave<TF, 64>(D, nd, tid);
+  return dense_lu<TF>(D, nd, tid, lbuf, lbuf + 256);
+}
+
 // svv/dnv hold two buffers per scenario ([2][M][...]); tgt[m] selects the
 // one written (NULL: buffer 0).
 template <typename TF>
@@ -1710,7 +1749,6 @@ __global__ void __launch_bounds__(512) k_splu_factor(SpluDev S, const double *__
   __syncthreads();
   scatter_a<TF>(S.nnz_a, S.a_src, S.a2lu, var, cst, ldM, m, lu, tid, nt);
   __syncthreads();
-  GP_TRACE(0, 0);
   int bad = 0;
   for (int l = 0; l < S.n_flev; l++) {
     const int rb = S.flp[l], re = S.flp[l + 1], mode = S.fac_mode[l];
@@ -1726,7 +1764,6 @@ __global__ void __launch_bounds__(512) k_splu_factor(SpluDev S, const double *__
         bad |= wave_column<TF>(S, lu, so, load_rec(S.crec, idx), mode == 2, col, lvs, locs, lane);
     }
     __syncthreads();
-    GP_TRACE(0, l + 1);
   }
   if (S.nd) {
     // dense trailing block: right-looking LU without pivoting, in LDS
@@ -1749,10 +1786,9 @@ __global__ void __launch_bounds__(512) k_splu_factor(SpluDev S, const double *__
     }
     __syncthreads();
     TF *lbuf = D + (((size_t)nd * nd + 3) & ~(size_t)3);
-    bad |= dense_lu<TF>(D, nd, tid, lbuf, lbuf + 256);
+    bad |= dense_lu_any<TF>(D, nd, tid, lbuf);
     TF *dn = dnv + ((size_t)bsel * Mb + m) * nd * nd;
     for (int i = tid; i < nd * nd; i += nt) dn[i] = D[i];
-    GP_TRACE(0, S.n_flev + 1);
   }
   if (bad) atomicOr(&status[m], 1);
 }
@@ -1803,7 +1839,16 @@ __device__ __forceinline__ int res_group_column(const int *rec, const int2 *ops,
   return (!(piv != TF(0)) || !isfinite((double)piv)) ? 1 : 0;
 }
 
-// Same interface as k_splu_factor; luv holds the step-ordered LU values.
+// Initial value of a matrix slot: var row s of scenario m (s >= 0), constant
+// -1 - s, or zero (kResNoSrc: fill).
+template <typename TF>
+__device__ __forceinline__ TF init_value(int s, const double *__restrict__ var, const double *__restrict__ cst,
+                                         int ldM, int m) {
+  if (s >= 0) return (TF)var[(size_t)s * ldM + m];
+  return s == kResNoSrc ? TF(0) : (TF)cst[-1 - s];
+}
+
+// Same interface as k_splu_factor; luv holds the step-ordered values.
 // Step s: its column values sit in xb[s & 1]; while it computes (LDS only),
 // the blob and initial values of step s + 1 and the output map of step s
 // are loaded into registers; after a barrier they go to the staging area and
@@ -1820,7 +1865,6 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
   extern __shared__ __align__(16) char smem_raw[];
   const int Mb = gridDim.x;
   const int bsel = tgt ? tgt[m] : 0;
-  TF *lu = luv + (size_t)m * S.res_xtot;
   TF *so = svv + ((size_t)bsel * Mb + m) * S.ns;
   TF *dn = dnv + ((size_t)bsel * Mb + m) * (size_t)S.nd * S.nd;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -1830,13 +1874,35 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
   int *stg = reinterpret_cast<int *>(smem_raw + lvb);
   TF *xbuf = reinterpret_cast<TF *>(smem_raw + lvb + (size_t)kResCB * nt * 4);
   const int xcap = kResCX * nt;
-  for (int p = tid; p < S.res_xtot; p += nt) lu[p] = TF(0);
+  TF *lu = luv + (size_t)m * S.res_xtot;
   // the dense copy starts at zero: the external updates write only the
   // block's pattern entries, and the block is read back whole
   for (int i = tid; i < S.nd * S.nd; i += nt) dn[i] = TF(0);
   if (tid == 0) Lv[S.res_nl] = TF(0);   // multiplier of the no-op updates
-  __syncthreads();
-  scatter_a<TF>(S.nnz_a, S.a_src, S.res_a2x, var, cst, ldM, m, lu, tid, nt);
+  // the step-ordered initial values in one pass (res_xsrc: the source of
+  // each slot, zero for fill), groups of four slots per thread, four groups
+  // in flight
+  for (int t0 = 4 * tid; t0 < S.res_xtot; t0 += 16 * nt) {
+    int4 sv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      sv[u] = *reinterpret_cast<const int4 *>(S.res_xsrc + min(t0 + 4 * u * nt, (S.res_xtot - 1) & ~3));
+    TF v[16];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      v[4 * u] = init_value<TF>(sv[u].x, var, cst, ldM, m);
+      v[4 * u + 1] = init_value<TF>(sv[u].y, var, cst, ldM, m);
+      v[4 * u + 2] = init_value<TF>(sv[u].z, var, cst, ldM, m);
+      v[4 * u + 3] = init_value<TF>(sv[u].w, var, cst, ldM, m);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int t = t0 + 4 * u * nt;
+      if (t < S.res_xtot)
+#pragma unroll
+        for (int k = 0; k < 4; k++) lu[t + k] = v[4 * u + k];
+    }
+  }
   __syncthreads();
   // per thread: blob dwords 4 (tid + j nt) .. + 3 (j < kResCB / 4), value
   // and output-map slots 4 (tid + j nt) .. + 3 (j < kResCX / 4), at clamped
@@ -1912,7 +1978,6 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
   fetch_blob(0);
   store(0);
   __syncthreads();
-  GP_TRACE(0, 0);
   int bad = 0;
   for (int st = 0; st < nst; st++) {
     const int *h = S.res_hdr + kResHdr * st;
@@ -1932,7 +1997,6 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
     if (st + 1 < nst) store(st + 1);
     flush(st);
     lds_barrier();
-    GP_TRACE(0, st + 1);
   }
   if (S.nd) {
     // dense trailing block (over the multipliers, no longer needed): the
@@ -1943,9 +2007,8 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
     copy16(D, dn, (size_t)nd * nd * sizeof(TF), tid, nt);
     __syncthreads();
     TF *lbuf = reinterpret_cast<TF *>(stg);   // the staging area is free now
-    bad |= dense_lu<TF>(D, nd, tid, lbuf, lbuf + 256);
+    bad |= dense_lu_any<TF>(D, nd, tid, lbuf);
     for (int i = tid; i < nd * nd; i += nt) dn[i] = D[i];
-    GP_TRACE(0, nst + 1);
   }
   if (bad) atomicOr(&status[m], 1);
 }
@@ -2134,7 +2197,6 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   }
   if (nfs) sol_store<TF>(sb, svl, sdg, tid, nt, P, pv, pd);
   __syncthreads();
-  GP_TRACE(1, 0);
   for (int st = 0; st < nfs; st++) {
     if (st + 1 < nfs) sol_fetch<TF>(S.sol_fw_hdr, S.sol_fw_blob, st + 1, fv, bd, tid, nt, P, pv, pd);
     sol_step<TF, false>(S.sol_fw_hdr + kSolHdr * st, sb, svl, sdg, x, tid);
@@ -2142,7 +2204,6 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
       sol_store<TF>(sb, svl, sdg, tid, nt, P, pv, pd);
       lds_barrier();
     }
-    GP_TRACE(1, st + 1);
   }
   if (nbs) sol_fetch<TF>(S.sol_bw_hdr, S.sol_bw_blob, 0, bv, bd, tid, nt, P, pv, pd);
   if (nd) {
@@ -2198,7 +2259,6 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
     if (i1 < nd) x[B + i1] = x1;
   }
   __syncthreads();
-  GP_TRACE(1, nfs + 1);
   if (nbs) {
     sol_store<TF>(sb, svl, sdg, tid, nt, P, pv, pd);
     __syncthreads();
@@ -2210,7 +2270,6 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
       sol_store<TF>(sb, svl, sdg, tid, nt, P, pv, pd);
       lds_barrier();
     }
-    GP_TRACE(1, nfs + 2 + st);
   }
   // outputs, eight rows per thread in flight
   for (int k0 = 0; k0 < n; k0 += 8 * nt) {
