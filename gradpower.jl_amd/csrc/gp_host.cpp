@@ -106,7 +106,7 @@ struct HBuf {
 //   [first L][refac0 L][exact L][dt 8L][work L][counts 16][failed L][areq L]
 //   upload   [0, 12L+16)      download [11L, 14L+16)
 // and a second block for the accepted steps:
-//   [acc L][adv L][col 4L][sv L][svslot 4L][ck L][ckslot 4L].
+//   [acc L][adv L][col 4L][sv L][svslot 4L][ck L][ckslot 4L][pred L].
 struct RoundLayout {
   size_t L;
   explicit RoundLayout(size_t l) : L(l) {}
@@ -129,7 +129,8 @@ struct RoundLayout {
   size_t svslot() const { return 7 * L; }
   size_t ck() const { return 11 * L; }
   size_t ckslot() const { return 12 * L; }
-  size_t post_bytes() const { return 16 * L; }
+  size_t pred() const { return 16 * L; }
+  size_t post_bytes() const { return 17 * L; }
 };
 
 struct LUPlan {
@@ -500,7 +501,7 @@ struct gp_ctx {
   // scenario-asynchronous time loop: per-scenario step, first-round flags,
   // record columns, accepted / advancing masks
   DBuf<double> d_dtv;
-  DBuf<uint8_t> d_first, d_acc, d_adv, d_sv, d_ck, d_exact;
+  DBuf<uint8_t> d_first, d_acc, d_adv, d_sv, d_ck, d_exact, d_pred;
   DBuf<int> d_col, d_svslot, d_ckslot;
   bool state_set = false;
   int hist_steps = 0;
@@ -892,6 +893,7 @@ void alloc_state(gp_ctx *C) {
     C->d_svslot.view(q + R.svslot(), L);
     C->d_ck.view(q + R.ck(), L);
     C->d_ckslot.view(q + R.ckslot(), L);
+    C->d_pred.view(q + R.pred(), L);
   }
   C->d_fbus.alloc((size_t)MAXFAULT * L);
   C->d_fg.alloc((size_t)MAXFAULT * L);
@@ -1902,7 +1904,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
       everfail(L, 0), exact(L, 0), restart(L, 0);
   const RoundLayout R(L);
   uint8_t *hr = C->h_round.p, *hp = C->h_post.p;   // pinned staging (RoundLayout)
-  uint8_t *acc = hp + R.acc(), *adv = hp + R.adv(), *sv = hp + R.sv(), *ck = hp + R.ck();
+  uint8_t *acc = hp + R.acc(), *adv = hp + R.adv(), *sv = hp + R.sv(), *ck = hp + R.ck(), *pred = hp + R.pred();
   int *col = reinterpret_cast<int *>(hp + R.col()), *svslot = reinterpret_cast<int *>(hp + R.svslot()),
       *ckslot = reinterpret_cast<int *>(hp + R.ckslot());
   std::vector<double> dtv(L, dt);
@@ -1927,6 +1929,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
     set_faults(m);
   }
   std::memcpy(adv, run.data(), L);
+  std::memset(pred, 0, L);
   HIPCHK(hipMemcpyAsync(C->d_post.p, hp, R.post_bytes(), hipMemcpyHostToDevice, C->stream));
   launch_copy_active(n, L, C->d_z.p, C->d_zold.p, C->d_adv.p, C->stream);
   HIPCHK(hipMemsetAsync(C->d_failed.p, 0, L, C->stream));
@@ -1935,6 +1938,9 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
   N.dtv = C->d_dtv.p;
   N.exactv = C->d_exact.p;
   N.exacth = exact.data();
+  // first iterate of a step by extrapolation (gp_options.predictor) under
+  // the policies that reuse factorisations
+  const bool extrapolate = C->opt.predictor >= 0 && N.reuse;
   bool fault_dirty = true;
   const long long max_rounds = (long long)(kmax + 1) * 2 * (4 * itermax + 16);
   for (long long round = 0;; round++) {
@@ -1982,6 +1988,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
     std::memset(adv, 0, L);
     std::memset(sv, 0, L);
     std::memset(ck, 0, L);
+    std::memset(pred, 0, L);
     bool any_acc = false, any_adv = false, any_fail = false, any_sv = false, any_ck = false, any_restart = false;
     std::fill(restart.begin(), restart.end(), 0);
     for (int m = 0; m < M; m++) {
@@ -2011,6 +2018,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
       const bool was_exact = exact[m] != 0;
       exact[m] = 0;          // solve accepted
       bool next = true;
+      const int mode_was = mode[m];
       if (mode[m] == 0) {    // step k accepted: record z_k, then its events
         if (F.record) {
           acc[m] = 1;
@@ -2061,6 +2069,9 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
           ckslot[m] = k[m] / F.every;
           any_ck = true;
         }
+        // z_k - zold is the increment of a step of length dt unless z_k
+        // came from an event's re-solve
+        const bool plain = mode_was == 0;
         k[m]++;
         if (k[m] > F.k_last[m]) {
           run[m] = 0;
@@ -2068,6 +2079,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
           dtv[m] = dt;
           first[m] = 1;
           adv[m] = 1;
+          pred[m] = (extrapolate && plain) ? 1 : 0;
           any_adv = true;
         }
       }
@@ -2079,7 +2091,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
                        C->d_col.p, C->d_acc.p, C->stream);
     if (any_sv) launch_copy_cols(n, L, C->d_z.p, F.seg, nL, C->d_svslot.p, C->d_sv.p, C->stream);
     if (any_ck) launch_copy_cols(n, L, C->d_z.p, F.ckpt, nL, C->d_ckslot.p, C->d_ck.p, C->stream);
-    if (any_adv) launch_copy_active(n, L, C->d_z.p, C->d_zold.p, C->d_adv.p, C->stream);
+    if (any_adv) launch_advance(n, L, C->d_z.p, C->d_zold.p, C->d_adv.p, C->d_pred.p, C->stream);
     if (any_restart) {
       set_mask(C, C->d_sel, restart);
       restart_solves(C, restart, C->d_sel.p);

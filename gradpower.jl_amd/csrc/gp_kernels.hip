@@ -2577,6 +2577,22 @@ __global__ void k_copy_active(int n, int ldM, const double *__restrict__ src, do
   for (int i = blockIdx.x; i < n; i += gridDim.x) dst[(size_t)i * ldM + m] = src[(size_t)i * ldM + m];
 }
 
+// The scenarios in act start a new step: zold = z, and where pred[m] the
+// first iterate is the linear extrapolation z = 2 z - zold (the previous
+// step's increment once more).
+__global__ void k_advance(int n, int ldM, double *__restrict__ z, double *__restrict__ zold,
+                          const uint8_t *__restrict__ act, const uint8_t *__restrict__ pred) {
+  const int m = blockIdx.y * WAVE + threadIdx.x;
+  if (!act[m]) return;
+  const bool p = pred[m] != 0;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const size_t o = (size_t)i * ldM + m;
+    const double v = z[o];
+    if (p) z[o] = 2.0 * v - zold[o];
+    zold[o] = v;
+  }
+}
+
 __global__ void k_record(int nrec, const int *__restrict__ idx, int ldM, int M, const double *__restrict__ z,
                          double *__restrict__ hist, const uint8_t *__restrict__ act) {
   const int m = blockIdx.y * WAVE + threadIdx.x;
@@ -2751,6 +2767,10 @@ void launch_split_precision(int L, const uint8_t *x64, const uint8_t *sch, const
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s) {
   const int gx = n < 1024 ? n : 1024;
   hipLaunchKernelGGL(k_copy_active, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, n, ldM, src, dst, act);
+}
+void launch_advance(int n, int ldM, double *z, double *zold, const uint8_t *act, const uint8_t *pred, hipStream_t s) {
+  const int gx = n < 1024 ? n : 1024;
+  hipLaunchKernelGGL(k_advance, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, n, ldM, z, zold, act, pred);
 }
 void launch_record_at(int nrec, const int *idx, int ldM, int M, const double *z, double *hist, const int *col,
                       const uint8_t *act, hipStream_t s) {

@@ -85,6 +85,7 @@ struct GpOptions
     global_x::Int32
     refactor_theta::Float64
     async_below::Float64
+    predictor::Int32
 end
 
 struct GpStats
@@ -290,7 +291,7 @@ limits the recorded trajectory channels; `nothing` records every state.
 """
 function HipBatchedLayout(dp::GradPower.DynamicProblem, ps::GradPower.PowerSystem, M::Int;
                           refactor_policy::Integer = REFACTOR_ASYNC, record_idx = nothing,
-                          retry_window::Integer = 0)
+                          retry_window::Integer = 0, predictor::Integer = 0)
     t = build_tables(dp, ps)
     rec = record_idx === nothing ? Int32[] : Int32.(collect(record_idx) .- 1)
     h = Ref{Ptr{Cvoid}}(C_NULL)
@@ -299,7 +300,8 @@ function HipBatchedLayout(dp::GradPower.DynamicProblem, ps::GradPower.PowerSyste
                             pointer(t.yrp), pointer(t.yci), pointer(t.yv), t.n_dev,
                             ntuple(i -> pointer(t.dev[i]), 8), pointer(t.u0), t.n_clusters, pointer(t.clusters))
         opt = GpOptions(record_idx === nothing ? 1 : 0, length(rec), isempty(rec) ? C_NULL : pointer(rec),
-                        Int32(refactor_policy), 0, 0, 0, 0, Int32(retry_window), Int32(0), 0.0, 0.0)
+                        Int32(refactor_policy), 0, 0, 0, 0, Int32(retry_window), Int32(0), 0.0, 0.0,
+                        Int32(predictor))
         check(ccall((:gp_create, LIB), Cint, (Ref{GpSystemDesc}, Int32, Ref{GpOptions}, Ref{Ptr{Cvoid}}),
                     desc, Int32(M), opt, h))
     end

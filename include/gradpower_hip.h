@@ -209,6 +209,14 @@ typedef struct gp_options {
   double async_below;      /* ASYNC: a refactorisation asked for at |F|inf below
                               this runs on the second stream, above it in the
                               lock step (0: every one runs on the second stream) */
+  int32_t predictor;       /* first Newton iterate of a step: 0 (default) the
+                              linear extrapolation 2 z_k - z_(k-1) under the
+                              policies that reuse factorisations (ADAPTIVE,
+                              ASYNC; not across an event, whose step starts
+                              from z_k), z_k under exact Newton (the
+                              reference's start, src/dynamics.jl:548-640);
+                              -1: z_k always.  A solve that fails restarts
+                              from z_k as exact Newton either way            */
 } gp_options;
 
 typedef struct gp_stats {
