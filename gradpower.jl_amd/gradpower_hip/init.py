@@ -221,11 +221,16 @@ def initialize_dynamics(dp: DynamicProblem, ps: PowerSystem, check_tol=1e-9):
         p[dev.par_ptr] = ps_
         p[dev.par_ptr + 1] = qs_
 
+    gen_of = {}   # (bus, id) -> the first GENROU there, in device order
+    for c in psd.devices:
+        if c.dtype.kind == "genrou":
+            gen_of.setdefault((c.dtype.bus, _norm_id(c.dtype.id)), c.dtype)
+
     def matched_gen_efd0(d):
-        for c in psd.devices:
-            if c.dtype.kind == "genrou" and c.dtype.bus == d.bus and _norm_id(c.dtype.id) == _norm_id(d.id):
-                return c.dtype.e_fd0
-        raise InitError("exciter without generator")
+        g = gen_of.get((d.bus, _norm_id(d.id)))
+        if g is None:
+            raise InitError("exciter without generator")
+        return g.e_fd0
 
     for i, dev in enumerate(psd.devices):
         d = dev.dtype

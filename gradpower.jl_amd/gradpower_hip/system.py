@@ -509,12 +509,22 @@ def add_disconnect_event(ps: PowerSystem, event):
 # ----------------------------------------------------------------------------
 # set_dynamics (src/GradPower.jl:297-447)
 # ----------------------------------------------------------------------------
-def _find_gen(ps, bus_internal, gen_id):
+def _find_gen(ps, bus_internal, gen_id, index=None):
+    """First generator at (internal bus, id); index: a prebuilt (bus, id) map."""
     gid = _norm_id(gen_id)
+    if index is not None:
+        return index.get((bus_internal, gid), -1)
     for i, g in enumerate(ps.gens):
         if g.bus == bus_internal and _norm_id(g.id) == gid:
             return i
     return -1
+
+
+def _gen_index(ps):
+    idx = {}
+    for i, g in enumerate(ps.gens):
+        idx.setdefault((g.bus, _norm_id(g.id)), i)
+    return idx
 
 
 def set_dynamics(ps: PowerSystem, psd: PowerSystemDynamics, add_loads=True,
@@ -523,13 +533,14 @@ def set_dynamics(ps: PowerSystem, psd: PowerSystemDynamics, add_loads=True,
     dev_bus = []
     dev_gen = []
     matched = []
+    gidx = _gen_index(ps)
     for dev in psd.devices:
         d = dev.dtype
         b = ps.busmap[d.bus]
         dev_bus.append(b)
         g = -1
         if d.kind == "genrou":
-            g = _find_gen(ps, b, d.id)
+            g = _find_gen(ps, b, d.id, gidx)
             if g >= 0:
                 d.set_ratio(ps.gens[g].mbase / mvbase)
                 matched.append(g)
@@ -561,6 +572,7 @@ def set_dynamics(ps: PowerSystem, psd: PowerSystemDynamics, add_loads=True,
     psd.dev_bus = dev_bus
     psd.dev_gen = dev_gen
     psd.uvec_idx = np.full(psd.ctrl_dim, -1, dtype=np.int64)
+    psd.target_index = _target_index(psd)
     _wire_controls(psd)
 
     for i, dev in enumerate(psd.devices):
@@ -573,9 +585,28 @@ def set_dynamics(ps: PowerSystem, psd: PowerSystemDynamics, add_loads=True,
     _reorder_state(psd)
 
 
+def _target_index(psd):
+    """(bus, id) -> [(device index, kind)] in device order, for the devices a
+    controller can attach to (loads and static generator stubs excluded)."""
+    idx = {}
+    for j, c in enumerate(psd.devices):
+        k = c.dtype.kind
+        if k in ("zipload", "static_gen") or getattr(c.dtype, "bus", None) is None:
+            continue
+        idx.setdefault((c.dtype.bus, _norm_id(c.dtype.id)), []).append((j, k))
+    return idx
+
+
 def _find_target(psd, dev, kinds):
+    """First device of a kind in `kinds` at the same (bus, id) as dev."""
     d = dev.dtype
     gid = _norm_id(d.id)
+    idx = getattr(psd, "target_index", None)
+    if idx is not None:
+        for j, k in idx.get((d.bus, gid), ()):
+            if k in kinds:
+                return j
+        return -1
     for j, c in enumerate(psd.devices):
         if c.dtype.kind in kinds and getattr(c.dtype, "bus", None) == d.bus \
                 and c.dtype.kind not in ("zipload", "static_gen") and _norm_id(c.dtype.id) == gid:
