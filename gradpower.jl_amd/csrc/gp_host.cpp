@@ -189,7 +189,10 @@ struct LUPlan {
     bw_lvl.upload(nonempty(A.sb.lvl)); bw_chunk.upload(nonempty(A.sb.chunk));
     bw_cols.upload(nonempty(A.sb.cols)); bw_rows.upload(nonempty(A.sb.rows));
     q.upload(A.q); pinv.upload(A.pinv);
-    {
+    if (gx_needed()) {   // the global-memory solve reads the level tables where they lie
+      dev.sol_nfw = dev.sol_nbw = 0;
+      dev.sol_fw_hdr = dev.sol_fw_blob = dev.sol_bw_hdr = dev.sol_bw_blob = nullptr;
+    } else {
       SolvePlan pf, pb;
       if (!solve_plan(A.sf, false, 0, kSolCB * kThreads, kSolCV * kThreads, pf) ||
           !solve_plan(A.sb, true, 0, kSolCB * kThreads, kSolCV * kThreads, pb))
@@ -233,6 +236,16 @@ struct LUPlan {
     d.threads = kThreads;
     d.gx = gx_needed() ? 1 : 0;
     d.gx_ws = nullptr;   // the context's [n][ldM] workspace, set after the plan is made
+    // levels of at least kWideLanes lanes take a launch of their own: one
+    // workgroup would need more than four passes of its 512 threads
+    d.gx_fw_wide = d.gx_bw_wide = 0;
+    d.h_fw_lvl = A.sf.lvl.data();
+    d.h_bw_lvl = A.sb.lvl.data();
+    if (d.gx) {
+      constexpr int kWideLanes = 2048;
+      while (d.gx_fw_wide < d.n_fwlev && A.sf.lvl[4 * d.gx_fw_wide] >= kWideLanes) d.gx_fw_wide++;
+      while (d.gx_bw_wide < d.n_bwlev && A.sb.lvl[4 * (d.n_bwlev - 1 - d.gx_bw_wide)] >= kWideLanes) d.gx_bw_wide++;
+    }
   }
   // LDS staging sizes for element size `elem`; columns (or op lists) larger
   // than the staging buffers run in place in global memory.
@@ -353,13 +366,15 @@ struct LUPlan {
     double best = 1e300;
     int best_cap = -1;
     std::vector<int> seen;
+    std::string why;
     for (int cap = nd_bound(); cap >= 0; cap -= 16) {
       try {
         upload(asrc, cap);
         if (std::find(seen.begin(), seen.end(), A.nd) != seen.end()) continue;
         seen.push_back(A.nd);
         shape(elem);
-      } catch (const GpError &) {
+      } catch (const GpError &e) {
+        why = e.msg;
         continue;
       }
       const double t = elem == 4 ? time_one<float>(v1.p, cst, r1.p, x1.p, one.p, st.p, w_solve, s)
@@ -369,7 +384,7 @@ struct LUPlan {
         best_cap = cap;
       }
     }
-    if (best_cap < 0) throw GpError{GP_ERR_UNSUPPORTED, "no dense-block size fits the LU kernels"};
+    if (best_cap < 0) throw GpError{GP_ERR_UNSUPPORTED, "no dense-block size fits the LU kernels (" + why + ")"};
     upload(asrc, best_cap);
     shape(elem);
     tuned_us = best;

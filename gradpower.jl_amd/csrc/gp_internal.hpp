@@ -96,6 +96,11 @@ struct SpluDev {
   // memory, gx_ws [n][ldM] (scenario m's x at j*ldM + m, pivot order)
   int gx;
   double *gx_ws;
+  // the leading forward / trailing backward levels wide enough to run as
+  // launches of their own across the device, and (host memory, read by the
+  // launcher only) the level tables for their lane counts
+  int gx_fw_wide, gx_bw_wide;
+  const int *h_fw_lvl, *h_bw_lvl;
   const int *sol_fw_hdr, *sol_fw_blob, *sol_bw_hdr, *sol_bw_blob;
   const int *res_hdr, *res_blob, *res_a2x, *res_omap;
   // per step-ordered value slot of the LDS-resident factorisation: where its

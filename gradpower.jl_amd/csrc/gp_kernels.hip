@@ -2297,31 +2297,68 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
 }
 
 // Systems whose solution does not fit in LDS (tens of thousands of rows and
-// more): the same two triangular solves, one workgroup per scenario, with x
-// in global memory (S.gx_ws, scenario m's x at j*ldM + m, pivot order) and
-// the level tables read where they lie (sliced ELL: lane l of a level holds
-// the entries E + u + 64 j of its chunk, u its lane in the chunk, the g lanes
-// of a row add up, the row's first lane updates x[row]); a workgroup barrier,
-// which orders the global memory of the workgroup too, closes each level.
-// The dense block's substitutions run on one wavefront from LDS.
+// more): the same two triangular solves with x in global memory (S.gx_ws,
+// scenario m's x at j*ldM + m, pivot order) and the level tables read where
+// they lie (sliced ELL: lane l of a level holds the entries E + u + 64 j of
+// its chunk, u its lane in the chunk, the g lanes of a row add up, the row's
+// first lane updates x[row]).  The wide levels near the leaves (the first
+// forward and the last backward levels, S.gx_fw_wide and S.gx_bw_wide of
+// them) run as launches of their own across the whole device; the narrow
+// levels near the root and the dense block run in one workgroup per
+// scenario, where a workgroup barrier (which orders the workgroup's global
+// memory too) closes each level.  The gather of b and the scatter of the
+// result are launches of their own, lane per scenario.
+__device__ __forceinline__ int gx_lane_row(int l, int T, int c0, const int *__restrict__ chunk, int &E, int &pf,
+                                           int &uc) {
+  const int lc = min(l, T - 1);   // spare lanes of the last chunk read lane T - 1's entries
+  const int c = c0 + (lc >> 6);
+  uc = lc & 63;
+  E = chunk[2 * c];
+  pf = chunk[2 * c + 1];
+  return lc;
+}
+
+// sum over j < pf of vals[e] x[cols[e]] (e = e0 + 64 j), U entries in
+// flight (their column indices, then their x values, are loaded together)
+template <typename TF, int U>
+__device__ __forceinline__ double gx_row_sum(int e0, int pf, const int *__restrict__ cols, const TF *__restrict__ vals,
+                                             const double *x, int ldM) {
+  double acc = 0.0;
+  int j = 0;
+  for (; j + U <= pf; j += U) {
+    int cc[U];
+    TF vv[U];
+    double xv[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const int e = e0 + WAVE * (j + q);
+      cc[q] = cols[e];
+      vv[q] = vals[e];
+    }
+#pragma unroll
+    for (int q = 0; q < U; q++) xv[q] = x[(size_t)cc[q] * ldM];
+#pragma unroll
+    for (int q = 0; q < U; q++) acc += (double)vv[q] * xv[q];
+  }
+  for (; j < pf; j++) {
+    const int e = e0 + WAVE * j;
+    acc += (double)vals[e] * x[(size_t)cols[e] * ldM];
+  }
+  return acc;
+}
+
 template <typename TF, bool BW>
-__device__ __forceinline__ void gx_levels(int nlev, const int *__restrict__ lvl, const int *__restrict__ chunk,
+__device__ __forceinline__ void gx_levels(int l0, int l1, const int *__restrict__ lvl, const int *__restrict__ chunk,
                                           const int *__restrict__ cols, const int *__restrict__ rows,
                                           const TF *__restrict__ vals, const TF *__restrict__ diag, double *x,
                                           int ldM, int tid, int nt) {
-  for (int lv = 0; lv < nlev; lv++) {
+  for (int lv = l0; lv < l1; lv++) {
     const int T = lvl[4 * lv], g = lvl[4 * lv + 1], R = lvl[4 * lv + 2], c0 = lvl[4 * lv + 3];
     const int Tp = (T + 63) & ~63;
     for (int l = tid; l < Tp; l += nt) {
-      const int lc = min(l, T - 1);   // spare lanes of the last chunk read lane T - 1's entries
-      const int c = c0 + (lc >> 6), uc = lc & 63;
-      const int E = chunk[2 * c], pf = chunk[2 * c + 1];
-      double acc = 0.0;
-      for (int j = 0; j < pf; j++) {
-        const int e = E + uc + WAVE * j;
-        acc += (double)vals[e] * x[(size_t)cols[e] * ldM];
-      }
-      acc = group_sum(acc, g);
+      int E, pf, uc;
+      const int lc = gx_lane_row(l, T, c0, chunk, E, pf, uc);
+      const double acc = group_sum(gx_row_sum<TF, 8>(E + uc, pf, cols, vals, x, ldM), g);
       if (l < T && (uc & (g - 1)) == 0) {
         const int row = rows[R + lc];
         const double xr = x[(size_t)row * ldM];
@@ -2332,13 +2369,63 @@ __device__ __forceinline__ void gx_levels(int nlev, const int *__restrict__ lvl,
   }
 }
 
+// One wide level lv of one side: block (b, m) takes lanes 256 b .. of the
+// level for scenario m (whole wavefronts: the level's lane count is padded
+// to 64).  vals / diag at off_vals / off_diag of scenario m's solve array.
+template <typename TF, bool BW>
+__global__ void __launch_bounds__(256) k_gx_level(const int *__restrict__ lvl, int lv, const int *__restrict__ chunk,
+                                                  const int *__restrict__ cols, const int *__restrict__ rows,
+                                                  const TF *__restrict__ svv, int ns, int off_vals, int off_diag,
+                                                  int Mb, double *__restrict__ xw, int ldM,
+                                                  const uint8_t *__restrict__ work, const uint8_t *__restrict__ cur) {
+  const int m = blockIdx.y;
+  if (!work[m]) return;
+  const int T = lvl[4 * lv], g = lvl[4 * lv + 1], R = lvl[4 * lv + 2], c0 = lvl[4 * lv + 3];
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= ((T + 63) & ~63)) return;   // whole wavefronts
+  const int bsel = cur ? cur[m] : 0;
+  const TF *base = svv + ((size_t)bsel * Mb + m) * ns;
+  const TF *vals = base + off_vals;
+  double *x = xw + m;
+  int E, pf, uc;
+  const int lc = gx_lane_row(l, T, c0, chunk, E, pf, uc);
+  const double acc = group_sum(gx_row_sum<TF, 4>(E + uc, pf, cols, vals, x, ldM), g);
+  if (l < T && (uc & (g - 1)) == 0) {
+    const int row = rows[R + lc];
+    const double xr = x[(size_t)row * ldM];
+    x[(size_t)row * ldM] = BW ? (xr - acc) / (double)base[off_diag + R + lc] : xr - acc;
+  }
+}
+
+// b into x (pivot order), and the result out of x, lane per scenario
+__global__ void k_gx_gather(int n, int ldM, const int *__restrict__ pinv, const double *__restrict__ rhs,
+                            double *__restrict__ x, const uint8_t *__restrict__ work) {
+  const int m = blockIdx.y * WAVE + threadIdx.x;
+  if (m >= ldM || !work[m]) return;
+  for (int r = blockIdx.x; r < n; r += gridDim.x) x[(size_t)pinv[r] * ldM + m] = rhs[(size_t)r * ldM + m];
+}
+__global__ void k_gx_output(int n, int ldM, const int *__restrict__ q, const int *__restrict__ scatter,
+                            const double *__restrict__ x, double sign, double *__restrict__ zupd,
+                            double *__restrict__ xout, const uint8_t *__restrict__ work) {
+  const int m = blockIdx.y * WAVE + threadIdx.x;
+  if (m >= ldM || !work[m]) return;
+  for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    const int j = q[k];
+    const double v = x[(size_t)k * ldM + m];
+    if (xout) xout[(size_t)j * ldM + m] = v;
+    if (zupd) {
+      const int zr = scatter ? scatter[j] : j;
+      zupd[(size_t)zr * ldM + m] += sign * v;
+    }
+  }
+}
+
+// The narrow levels (forward from S.gx_fw_wide on, backward up to
+// n_bwlev - S.gx_bw_wide) and the dense block, one workgroup per scenario.
 template <typename TF>
 __global__ void __launch_bounds__(512) k_splu_solve_gx(SpluDev S, const TF *__restrict__ svv,
-                                                       const TF *__restrict__ dnv, const double *__restrict__ rhs,
-                                                       int ldM, double sign, const int *__restrict__ scatter,
-                                                       double *__restrict__ zupd, double *__restrict__ xout,
-                                                       const uint8_t *__restrict__ work,
-                                                       const uint8_t *__restrict__ cur) {
+                                                       const TF *__restrict__ dnv, const uint8_t *__restrict__ work,
+                                                       const uint8_t *__restrict__ cur, int ldM) {
   const int m = scenario_of_block(blockIdx.x, gridDim.x);
   if (!work[m]) return;
   extern __shared__ __align__(16) char smem_raw[];
@@ -2350,9 +2437,8 @@ __global__ void __launch_bounds__(512) k_splu_solve_gx(SpluDev S, const TF *__re
   const TF *fv = svv + ((size_t)bsel * Mb + m) * S.ns;
   const TF *bv = fv + S.fw_nvals;
   const TF *bd = bv + S.bw_nvals;
-  for (int r = tid; r < n; r += nt) x[(size_t)S.pinv[r] * ldM] = rhs[(size_t)r * ldM + m];
-  __syncthreads();
-  gx_levels<TF, false>(S.n_fwlev, S.fw_lvl, S.fw_chunk, S.fw_cols, S.fw_rows, fv, nullptr, x, ldM, tid, nt);
+  gx_levels<TF, false>(S.gx_fw_wide, S.n_fwlev, S.fw_lvl, S.fw_chunk, S.fw_cols, S.fw_rows, fv, nullptr, x, ldM, tid,
+                       nt);
   if (nd) {
     TF *Dl = reinterpret_cast<TF *>(smem_raw);
     const TF *dn = dnv + ((size_t)bsel * Mb + m) * nd * nd;
@@ -2378,16 +2464,8 @@ __global__ void __launch_bounds__(512) k_splu_solve_gx(SpluDev S, const TF *__re
     }
     __syncthreads();
   }
-  gx_levels<TF, true>(S.n_bwlev, S.bw_lvl, S.bw_chunk, S.bw_cols, S.bw_rows, bv, bd, x, ldM, tid, nt);
-  for (int k = tid; k < n; k += nt) {
-    const int j = S.q[k];
-    const double v = x[(size_t)k * ldM];
-    if (xout) xout[(size_t)j * ldM + m] = v;
-    if (zupd) {
-      const int zr = scatter ? scatter[j] : j;
-      zupd[(size_t)zr * ldM + m] += sign * v;
-    }
-  }
+  gx_levels<TF, true>(0, S.n_bwlev - S.gx_bw_wide, S.bw_lvl, S.bw_chunk, S.bw_cols, S.bw_rows, bv, bd, x, ldM, tid,
+                      nt);
 }
 
 // ---------------------------------------------------------------- discrete adjoint
@@ -2716,8 +2794,17 @@ void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const doubl
   if (S.gx) {
     static std::atomic<unsigned> done_gx{0};
     allow_lds(reinterpret_cast<const void *>(k_splu_solve_gx<TF>), done_gx, S.lds_solve);
-    hipLaunchKernelGGL(k_splu_solve_gx<TF>, dim3(M), dim3(S.threads), S.lds_solve, s, S, sv, dn, rhs, ldM, sign,
-                       scatter, zupd, xout, work, cur);
+    const dim3 lanes(std::min(S.n, 4096), (ldM + WAVE - 1) / WAVE);
+    hipLaunchKernelGGL(k_gx_gather, lanes, dim3(WAVE), 0, s, S.n, ldM, S.pinv, rhs, S.gx_ws, work);
+    for (int l = 0; l < S.gx_fw_wide; l++)
+      hipLaunchKernelGGL((k_gx_level<TF, false>), dim3((S.h_fw_lvl[4 * l] + 255) / 256, M), dim3(256), 0, s,
+                         S.fw_lvl, l, S.fw_chunk, S.fw_cols, S.fw_rows, sv, S.ns, 0, 0, M, S.gx_ws, ldM, work, cur);
+    hipLaunchKernelGGL(k_splu_solve_gx<TF>, dim3(M), dim3(S.threads), S.lds_solve, s, S, sv, dn, work, cur, ldM);
+    for (int l = S.n_bwlev - S.gx_bw_wide; l < S.n_bwlev; l++)
+      hipLaunchKernelGGL((k_gx_level<TF, true>), dim3((S.h_bw_lvl[4 * l] + 255) / 256, M), dim3(256), 0, s,
+                         S.bw_lvl, l, S.bw_chunk, S.bw_cols, S.bw_rows, sv, S.ns, S.fw_nvals,
+                         S.fw_nvals + S.bw_nvals, M, S.gx_ws, ldM, work, cur);
+    hipLaunchKernelGGL(k_gx_output, lanes, dim3(WAVE), 0, s, S.n, ldM, S.q, scatter, S.gx_ws, sign, zupd, xout, work);
     return;
   }
   static std::atomic<unsigned> done_sol{0};
