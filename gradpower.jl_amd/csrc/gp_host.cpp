@@ -1257,7 +1257,9 @@ void start_batch(gp_ctx *C, int l, TF *luv, TF *sv, TF *dn) {
   gp_ctx::Lane &ln = C->lanes[l];
   const int L = C->ldM, nb = ln.nb;
   HIPCHK(hipStreamWaitEvent(ln.stream, ln.ev_snap, 0));
-  const int pi = prof_begin(C, C->prof_factor, ln.stream);
+  // timed: lane 0, the method's own precision (lane 1 holds the FP32
+  // method's FP64 exact-Newton factorisations)
+  const int pi = l == 0 ? prof_begin(C, C->prof_factor, ln.stream) : -1;
   launch_splu_factor_impl<TF>(sizeof(TF) == 8 ? C->luS64 : C->luS.dev, ln.snap.p, C->d_s_cst.p, L, luv, sv, dn,
                               ln.req, ln.tgt, C->M, C->d_status.p, ln.stream);
   prof_end(C->prof_factor, pi, ln.stream, nb);
@@ -1433,7 +1435,11 @@ struct Newton {
     mixed = false;
     skip = known && cnt[2] == 0 && cnt[3] == 0;   // every remaining scenario waits (absorb)
     if (skip) return;
-    const bool fac = !known || cnt[1] > 0;
+    // with the counts unknown: under ASYNC only a solve that starts afresh
+    // (an event or a change of dt: land_first) factorises in the round
+    // itself, every other request goes to a lane
+    const bool fac = known ? cnt[1] > 0 : (!async || land_first || C->opt.async_below > 0.0);
+    fac_done = fac;
     if (method == GP_METHOD_MONO_FP64) {
       launch_jac_mono(C->T, p, z, dt, faults(C), C->d_jv.p, C->d_upd.p, C->stream, dtv);
       prof_f = prof_begin(C, C->prof_factor, C->stream);
@@ -1500,25 +1506,33 @@ struct Newton {
     if (mixed) {   // the FP64 scenarios' factorisation, solve and back-substitution on the side stream
       HIPCHK(hipEventRecord(C->ev_fork2, C->stream));
       HIPCHK(hipStreamWaitEvent(C->side, C->ev_fork2, 0));
-      if (fac)
-        launch_splu_factor_impl<double>(C->luS64, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
-                                        C->d_dnS64.p, ref64, C->d_cur.p, C->M, C->d_status.p, s64);
+      launch_splu_factor_impl<double>(C->luS64, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
+                                      C->d_dnS64.p, ref64, C->d_cur.p, C->M, C->d_status.p, s64);
       launch_splu_solve<double>(C->luS64, C->d_svS64.p, C->d_dnS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
                                 C->d_dv.p, upd64, C->d_cur.p, C->M, s64);
       launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, upd64, s64);
       HIPCHK(hipEventRecord(C->ev_join, C->side));
     }
-    if (fac) {
-      prof_f = prof_begin(C, C->prof_factor, C->stream);
-      if (run32)
-        launch_splu_factor_impl<float>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS32.p, C->d_svS32.p,
-                                       C->d_dnS32.p, ref32, C->d_cur.p, C->M, C->d_status.p, C->stream);
-      if (run64 && !mixed)
-        launch_splu_factor_impl<double>(C->luS64, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
-                                        C->d_dnS64.p, ref64, C->d_cur.p, C->M, C->d_status.p, C->stream);
-      prof_end(C->prof_factor, prof_f, C->stream, known ? cnt[1] : 0);
-    }
+    ref32_ = ref32;
+    ref64_ = ref64;
+    if (fac) sync_factor(known ? cnt[1] : 0);
     HIPCHK(hipGetLastError());
+  }
+  // the synchronous factorisation of the main stream (the FP64 one of a
+  // mixed round runs on the side stream)
+  const uint8_t *ref32_ = nullptr, *ref64_ = nullptr;
+  bool fac_done = false;
+  void sync_factor(int64_t nscen) {
+    const int L = C->ldM;
+    // timed: the launches in the method's own precision
+    prof_f = (run32 || !fp32) ? prof_begin(C, C->prof_factor, C->stream) : -1;
+    if (run32)
+      launch_splu_factor_impl<float>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS32.p, C->d_svS32.p,
+                                     C->d_dnS32.p, ref32_, C->d_cur.p, C->M, C->d_status.p, C->stream);
+    if (run64 && !mixed)
+      launch_splu_factor_impl<double>(C->luS64, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
+                                      C->d_dnS64.p, ref64_, C->d_cur.p, C->M, C->d_status.p, C->stream);
+    prof_end(C->prof_factor, prof_f, C->stream, nscen);
   }
   void launch_post(const int cnt[4]) {
     const int L = C->ldM;
@@ -1538,6 +1552,13 @@ struct Newton {
       }
       HIPCHK(hipGetLastError());
       return;
+    }
+    // a synchronous request launch_pre did not expect (a residual that is
+    // not finite under exact Newton, whose request is not sent to a lane):
+    // its factorisation goes out now, still before the solve
+    if (cnt[1] > 0 && !fac_done) {
+      sync_factor(cnt[1]);
+      fac_done = true;
     }
     // asynchronous requests (pending on the device from k_norm_finish on:
     // the scenario waits when its |F| stops decreasing) join the host queue

@@ -1,6 +1,7 @@
 """Workload driver for hardware-counter and kernel-trace runs (rocprofv3):
-one warm-up pass of the 2,000-bus M=256 contingency workload (symbolic
-analysis, LU-plan autotuning), then a plain copy kernel of a known size
+one warm-up pass of the 2,000-bus M=256 contingency workload (or, with
+--case c4, of the 70,000-bus M=8 one) -- symbolic analysis, LU-plan
+autotuning -- then a plain copy kernel of a known size
 (which calibrates the byte counters and marks where the measured launches
 begin), then the measured pass(es), optionally with the scenario horizons
 capped so that a counter run stays short; or (--adjoint) one
@@ -25,6 +26,8 @@ def main():
     ap.add_argument("--copy-mb", type=float, default=256.0)
     ap.add_argument("--horizon-cap", type=float, default=0.5)
     ap.add_argument("--method", default="schur32", choices=["schur32", "schur64", "mono"])
+    ap.add_argument("--case", default="c2", choices=["c2", "c4"],
+                    help="c2: the 2,000-bus M=256 workload; c4: the 70,000-bus M=8 workload")
     ap.add_argument("--passes", type=int, default=1)
     ap.add_argument("--stats-out", default="",
                     help="write the batched LU launches' scenario counts of the last pass here (JSON)")
@@ -38,7 +41,7 @@ def main():
     from gradpower_hip.lib import (HipContext, METHOD_MONO_FP64, METHOD_SCHUR_FP32, METHOD_SCHUR_FP64,
                                    REFACTOR_ASYNC, REFACTOR_EVERY_ITER)
     t0 = time.perf_counter()
-    case = synth.case2_grid(M=256)
+    case = synth.case2_grid(M=256) if args.case == "c2" else synth.case4_grid()
     case.tf = np.minimum(case.tf, args.horizon_cap)
     print(f"setup {time.perf_counter() - t0:.1f} s", flush=True)
     meth = {"schur32": METHOD_SCHUR_FP32, "schur64": METHOD_SCHUR_FP64, "mono": METHOD_MONO_FP64}[args.method]
