@@ -36,6 +36,9 @@ for _p in (ROOT, os.path.join(ROOT, "gradpower.jl_amd")):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, published
+# the library times every launch of the factorisation and every k-th launch
+# of the triangular solve (an event pair costs the round's stream a few us)
+SOLVE_TIMING_STRIDE = 8
 KERNEL_NAMES = {("solve", 2): "k_splu_solve<float>", ("solve", 1): "k_splu_solve<double>",
                 ("solve", 0): "k_splu_solve<double>", ("factor", 2): "k_splu_factor_res<float>",
                 ("factor", 1): "k_splu_factor<double>", ("factor", 0): "k_splu_factor<double>"}
@@ -75,7 +78,7 @@ def run_method(case, method, steps, warmup, record_idx, tol, itermax, opts):
             kern["factor_sc"] += stats["factor_kernel_scenarios"]
         log(f"  method {method} pass {it}: {t1 - t0:.3f} s, device {stats['elapsed_ms']:.1f} ms, "
             f"steps {stats['scenario_steps']}, rounds {stats['newton_rounds']}, exact retries "
-            f"{stats['exact_retries']} "
+            f"{stats['exact_retries']}, repivots {stats['repivots']} "
             f"(factor {stats['factor_launches']}), corrections {stats['newton_iters']}, "
             f"refactorizations {stats['refactorizations']}, not converged {stats['n_not_converged']}, "
             f"solve kernel {stats['solve_kernel_ms']:.1f} ms / {stats['solve_kernel_launches']}, "
@@ -330,7 +333,8 @@ def roofline(run, method):
     """Dominant kernel of one method: HIP-event device time per launch and
     algorithmic bytes per launch (bytes per scenario x scenarios served)."""
     k, st = run["kern"], run["stats"]
-    which = "solve" if k["solve_ms"] >= k["factor_ms"] else "factor"
+    stride = {"solve": SOLVE_TIMING_STRIDE, "factor": 1}
+    which = "solve" if k["solve_ms"] * SOLVE_TIMING_STRIDE >= k["factor_ms"] else "factor"
     n = k[which + "_n"]
     if n == 0:
         return None
@@ -352,7 +356,8 @@ def roofline(run, method):
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_us": avg_s * 1e6,
             "launches": n, "algorithmic_bytes_per_launch": bytes_per_launch,
             "scenarios_per_launch": k[which + "_sc"] / n, "bytes_per_scenario": per_scen,
-            "share_of_wall": k[which + "_ms"] * 1e-3 / float(np.sum(run["times"]))}
+            "share_of_wall": k[which + "_ms"] * stride[which] * 1e-3 / float(np.sum(run["times"])),
+            "timed_launches": f"every {stride[which]}-th" if stride[which] > 1 else "every"}
 
 
 def main():
@@ -379,6 +384,9 @@ def main():
     ap.add_argument("--retry-window", type=int, default=0,
                     help="corrections without halving |F| after which an inexact solve restarts as exact FP64 "
                          "Newton (0: the library's default, 8; -1: at itermax only)")
+    ap.add_argument("--predictor", type=int, default=0,
+                    help="first iterate of a step under the Schur methods' policies: 0 linear extrapolation, "
+                         "2 quadratic, -1 none")
     ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete-adjoint measurement")
     ap.add_argument("--no-c4", action="store_true", help="skip the 70,000-bus M=8 measurement")
@@ -410,7 +418,8 @@ def main():
 
     spol = {"async": REFACTOR_ASYNC, "adaptive": REFACTOR_ADAPTIVE}[args.policy]
     base = {"stall_window": args.stall_window, "refactor_theta": args.theta, "async_below": args.async_below,
-            "profile": not args.no_profile, "retry_window": args.retry_window}
+            "profile": 0 if args.no_profile else SOLVE_TIMING_STRIDE, "retry_window": args.retry_window,
+            "predictor": args.predictor}
     methods = args.methods.split(",")
     runs, mcode = {}, {}
     for name, meth, pol in (("schur32", METHOD_SCHUR_FP32, spol), ("schur64", METHOD_SCHUR_FP64, spol),
@@ -446,7 +455,7 @@ def main():
                    "scenarios_stopped": f"{int(np.sum(head['status'] != 0))} of {case.M}",
                    "newton_tol": tol, "itermax": itermax, "stall_window": args.stall_window,
                    "retry_window": args.retry_window or 8,
-                   "refactor_policy": args.policy, "refactor_theta": args.theta,
+                   "refactor_policy": args.policy, "refactor_theta": args.theta, "predictor": args.predictor,
                    "horizon_cap": args.horizon_cap or None,
                    "deviations": {
                        "loads": f"constant impedance (ZIP alpha={case.meta['zip_alpha']}), not the 0.4/0.3/0.3 "
@@ -465,7 +474,8 @@ def main():
                   "completed_steps": st["scenario_steps"], "newton_rounds": st["newton_rounds"],
                   "factor_launches": st["factor_launches"], "newton_corrections": st["newton_iters"],
                   "corrections_per_step": st["newton_iters"] / max(1, st["scenario_steps"]),
-                  "refactorizations": st["refactorizations"], "status": st["status"],
+                  "refactorizations": st["refactorizations"], "exact_retries": st["exact_retries"],
+                  "repivots": st["repivots"], "status": st["status"],
                   "failed_scenarios": int(np.sum(r["status"] != 0)),
                   "lu_dense_block": r["info"]["nd_j" if k == "mono" else "nd_s"],
                   "lu_factor_lds_resident": bool(r["info"]["res_j" if k == "mono" else "res_s"])}

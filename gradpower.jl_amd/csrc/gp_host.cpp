@@ -100,28 +100,18 @@ struct HBuf {
 };
 
 // Per-round exchange of the scenario-asynchronous time loop: one block in
-// device memory and its pinned host mirror, laid out so that one copy
-// uploads what a round's evaluation reads and one copy brings back what it
-// decided (L = padded scenario count, a multiple of 64):
+// device memory and its pinned host mirror (L = padded scenario count, a
+// multiple of 64).  The accepted steps of round r (what the kernels that
+// record and advance them read) come first, then what round r + 1's
+// evaluation reads, then what it decides:
+//   [acc L][adv L][col 4L][sv L][svslot 4L][ck L][ckslot 4L][pred L]      (P = 17L)
 //   [first L][refac0 L][exact L][dt 8L][work L][counts 16][failed L][areq L]
-//   upload   [0, 12L+16)      download [11L, 14L+16)
-// and a second block for the accepted steps:
-//   [acc L][adv L][col 4L][sv L][svslot 4L][ck L][ckslot 4L][pred L].
+// so that one copy uploads [0, P + 13L + 16) (the failed flags, which only
+// the evaluation sets, go up cleared) and one copy brings back
+// [P + 11L, P + 14L + 16).
 struct RoundLayout {
   size_t L;
   explicit RoundLayout(size_t l) : L(l) {}
-  size_t first() const { return 0; }
-  size_t refac0() const { return L; }
-  size_t exact() const { return 2 * L; }
-  size_t dt() const { return 3 * L; }
-  size_t work() const { return 11 * L; }
-  size_t cnt() const { return 12 * L; }
-  size_t failed() const { return 12 * L + 16; }
-  size_t areq() const { return 13 * L + 16; }
-  size_t bytes() const { return 14 * L + 16; }
-  size_t up_bytes() const { return 12 * L + 16; }
-  size_t down_off() const { return 11 * L; }
-  size_t down_bytes() const { return 3 * L + 16; }
   size_t acc() const { return 0; }
   size_t adv() const { return L; }
   size_t col() const { return 2 * L; }
@@ -130,7 +120,19 @@ struct RoundLayout {
   size_t ck() const { return 11 * L; }
   size_t ckslot() const { return 12 * L; }
   size_t pred() const { return 16 * L; }
-  size_t post_bytes() const { return 17 * L; }
+  size_t P() const { return 17 * L; }
+  size_t first() const { return P(); }
+  size_t refac0() const { return P() + L; }
+  size_t exact() const { return P() + 2 * L; }
+  size_t dt() const { return P() + 3 * L; }
+  size_t work() const { return P() + 11 * L; }
+  size_t cnt() const { return P() + 12 * L; }
+  size_t failed() const { return P() + 12 * L + 16; }
+  size_t areq() const { return P() + 13 * L + 16; }
+  size_t bytes() const { return P() + 14 * L + 16; }
+  size_t up_bytes() const { return P() + 13 * L + 16; }
+  size_t down_off() const { return P() + 11 * L; }
+  size_t down_bytes() const { return 3 * L + 16; }
 };
 
 struct LUPlan {
@@ -448,13 +450,14 @@ struct gp_ctx {
   DBuf<int> d_evbus;
   DBuf<double> d_evg;
   void *scratch = nullptr;
+  DBuf<double> d_zold2;   // z_(k-2) for the quadratic predictor
   DBuf<double> d_p, d_z, d_zold, d_f, d_inj, d_jv, d_svar, d_rhs, d_dv, d_Dg, d_fnorm, d_hist, d_luJ, d_luS64;
   DBuf<double> d_svJ, d_svS64, d_dnJ, d_dnS64;
   DBuf<float> d_luS32, d_svS32, d_dnS32, d_X32;
   DBuf<double> d_X64;
   DBuf<uint8_t> d_work, d_act, d_refac, d_refac0, d_sel;
-  DBuf<uint8_t> d_round, d_post;   // per-round exchange blocks (RoundLayout); views point into them
-  HBuf h_round, h_post;
+  DBuf<uint8_t> d_round;   // per-round exchange block (RoundLayout); views point into it
+  HBuf h_round;
   // a batch's requests and target buffers [req L][tgt L] per lane, uploaded
   // from a ring of pinned slots: each round synchronises the stream at least
   // once and launches at most kLanes batches, so a slot is reused only after
@@ -491,10 +494,15 @@ struct gp_ctx {
     std::vector<hipEvent_t> ev;
     std::vector<int64_t> scen;   // per pair: the scenarios its launch worked on (0: none, not counted)
     size_t used = 0;
+    int64_t calls = 0;           // launches seen; every `every`-th one is timed
+    int every = 1;
   } prof_solve, prof_factor;
   DBuf<unsigned long long> d_nb;
   DevTables T{};
   LUPlan luJ, luS;
+  // the plans' representative matrices at analysis (scenario 0, dt = 0, no
+  // event in force), values in the J (CSC) and S patterns
+  std::vector<double> rep0_J, rep0_S;
   SpluDev luS64{};   // luS shaped for FP64 factors (the FP64 methods' and the FP32 method's exact retries)
   DBuf<double> d_gxJ, d_gxS, d_gxJT;   // solves' x in global memory [n][ldM] (large systems)
   // the FP32 method's FP64 exact-Newton kernels run beside the FP32 ones on
@@ -888,11 +896,9 @@ void alloc_state(gp_ctx *C) {
   {
     const RoundLayout R(L);
     C->d_round.alloc(R.bytes());
-    C->d_post.alloc(R.post_bytes());
     HIPCHK(hipMemset(C->d_round.p, 0, R.bytes()));
     C->h_round.alloc(R.bytes());
-    C->h_post.alloc(R.post_bytes());
-    uint8_t *b = C->d_round.p, *q = C->d_post.p;
+    uint8_t *b = C->d_round.p, *q = C->d_round.p;
     C->d_first.view(b + R.first(), L);
     C->d_refac0.view(b + R.refac0(), L);
     C->d_exact.view(b + R.exact(), L);
@@ -953,10 +959,15 @@ void set_mask(gp_ctx *C, DBuf<uint8_t> &d, const std::vector<uint8_t> &h) {
 // of tripped lines taken out at once -- the diagonal blocks of its buses gain
 // +yff / +ytt and the off-diagonal blocks +yft / +ytf, realified [g -b; b g].
 // One matrix per distinct set: every line a scenario trips (at once), and
-// every tripped line alone.  Empty when no line is tripped.
-std::vector<std::vector<double>> trip_matrices(const gp_ctx *C, const std::vector<double> &a, bool reduced) {
+// every tripped line alone.  Empty when no line is tripped.  only >= 0: the
+// lines of that scenario alone.
+std::vector<std::vector<double>> trip_matrices(const gp_ctx *C, const std::vector<double> &a, bool reduced,
+                                               int only = -1) {
   std::vector<std::vector<double>> out;
-  if (C->trips.empty()) return out;
+  std::vector<gp_line_trip> trips;
+  for (const gp_line_trip &t : C->trips)
+    if (only < 0 || t.scenario == only) trips.push_back(t);
+  if (trips.empty()) return out;
   const int npt = C->net_ptr;
   auto pos = [&](int zr, int zc) {
     return reduced ? find_pos(C->scp, C->sri, C->red_of_z[zr], C->red_of_z[zc]) : find_pos(C->jcp, C->jri, zr, zc);
@@ -973,9 +984,9 @@ std::vector<std::vector<double>> trip_matrices(const gp_ctx *C, const std::vecto
   // distinct lines (a line tripped in several scenarios, or tripped again
   // after a reclosure, is one line)
   std::vector<std::vector<double>> lines;
-  std::vector<int> line_of(C->trips.size());
-  for (size_t t = 0; t < C->trips.size(); t++) {
-    const gp_line_trip &tr = C->trips[t];
+  std::vector<int> line_of(trips.size());
+  for (size_t t = 0; t < trips.size(); t++) {
+    const gp_line_trip &tr = trips[t];
     std::vector<double> key{(double)tr.from_bus, (double)tr.to_bus};
     key.insert(key.end(), tr.y, tr.y + 8);
     auto it = std::find(lines.begin(), lines.end(), key);
@@ -985,7 +996,7 @@ std::vector<std::vector<double>> trip_matrices(const gp_ctx *C, const std::vecto
   std::vector<std::vector<int>> sets;
   for (int l = 0; l < (int)lines.size(); l++) sets.push_back({l});
   std::map<int, std::vector<int>> per_scen;
-  for (size_t t = 0; t < C->trips.size(); t++) per_scen[C->trips[t].scenario].push_back(line_of[t]);
+  for (size_t t = 0; t < trips.size(); t++) per_scen[trips[t].scenario].push_back(line_of[t]);
   for (auto &kv : per_scen) {
     std::vector<int> st = kv.second;
     std::sort(st.begin(), st.end());
@@ -1041,8 +1052,10 @@ void analyse(gp_ctx *C, int method) {
     HIPCHK(hipMemcpy2DAsync(jv.data(), sizeof(double), C->d_jv.p, (size_t)L * sizeof(double), sizeof(double), nnz,
                             hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
-    const std::vector<std::vector<double>> jx = trip_matrices(C, jv, false);
+    const std::vector<std::vector<double>> jx =
+        C->opt.pivot_rule == 1 ? std::vector<std::vector<double>>{} : trip_matrices(C, jv, false);
     std::string e = splu_analyse(C->n, C->jcp.data(), C->jri.data(), jv.data(), C->luJ.A, 1e-3, ptrs(jx));
+    C->rep0_J = jv;
     if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "Jacobian analysis: " + e};
     std::vector<int> asrc(nnz);
     for (int i = 0; i < nnz; i++) asrc[i] = i;
@@ -1081,8 +1094,10 @@ void analyse(gp_ctx *C, int method) {
       int s = C->s_src[e];
       sx[e] = s >= 0 ? sv[s] : C->s_cst[-s - 1];
     }
-    const std::vector<std::vector<double>> sxx = trip_matrices(C, sx, true);
+    const std::vector<std::vector<double>> sxx =
+        C->opt.pivot_rule == 1 ? std::vector<std::vector<double>>{} : trip_matrices(C, sx, true);
     std::string e = splu_analyse(C->nred, C->scp.data(), C->sri.data(), sx.data(), C->luS.A, 1e-3, ptrs(sxx));
+    C->rep0_S = sx;
     if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "reduced-network analysis: " + e};
     // a factorisation serves several solves (refactorisation policies), and
     // the solves are on every round's critical path
@@ -1132,6 +1147,7 @@ void ensure_schur_buffers(gp_ctx *C, bool fp32) {
 // (prof_count); launches that worked on no scenario are left out.
 int prof_begin(gp_ctx *C, gp_ctx::KProf &P, hipStream_t s) {
   if (!C->opt.profile) return -1;
+  if (P.calls++ % P.every) return -1;
   if (P.used + 2 > P.ev.size()) {
     for (int k = 0; k < 64; k++) {
       hipEvent_t e;
@@ -1907,9 +1923,21 @@ struct Stepper {
 
 
 // What a scenario-asynchronous sweep runs and keeps.
+// A scenario stopped by an exactly zero pivot: it continues from the
+// start of solve (k, mode) -- mode 1: step k's dt = 0 re-solve under the
+// events `active` -- on a plan of its own (repivot_runs).
+struct RepivotReq {
+  int m = 0, k = 0, mode = 0;
+  std::vector<const gp_event *> active;
+};
+
 struct FreeSpec {
   std::vector<int> k_first, k_last;   // per scenario: steps k_first..k_last (none when k_last < k_first)
   std::vector<std::vector<const gp_event *>> active0;   // faults in force at k_first (empty: none)
+  std::vector<uint8_t> resolve0;      // per scenario (empty: none): the first solve is step k_first's
+                                      // dt = 0 re-solve (its events already in active0)
+  std::vector<RepivotReq> *repivot = nullptr;   // scenarios stopped by an exactly zero pivot go here
+                                                // instead of failing
   bool record = true;                 // recorded channels of z_k into trajectory column k
   double *seg = nullptr;              // all of z_k, seg_k0 < k <= seg_k1, into slot k - seg_k0 - 1
   int seg_k0 = 0, seg_k1 = -1;
@@ -1939,7 +1967,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
   std::vector<uint8_t> mode(L, 0), run(L, 0), first(L, 0), stale(L, 1), fresh(L, 0), wk(L, 0), fl(L, 0),
       everfail(L, 0), exact(L, 0), restart(L, 0);
   const RoundLayout R(L);
-  uint8_t *hr = C->h_round.p, *hp = C->h_post.p;   // pinned staging (RoundLayout)
+  uint8_t *hr = C->h_round.p, *hp = C->h_round.p;   // pinned staging (RoundLayout)
   uint8_t *acc = hp + R.acc(), *adv = hp + R.adv(), *sv = hp + R.sv(), *ck = hp + R.ck(), *pred = hp + R.pred();
   int *col = reinterpret_cast<int *>(hp + R.col()), *svslot = reinterpret_cast<int *>(hp + R.svslot()),
       *ckslot = reinterpret_cast<int *>(hp + R.ckslot());
@@ -1960,16 +1988,21 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
     k[m] = F.k_first[m];
     run[m] = F.k_first[m] <= F.k_last[m] ? 1 : 0;
     first[m] = 1;
+    if (!F.resolve0.empty() && F.resolve0[m]) {
+      mode[m] = 1;
+      dtv[m] = 0.0;
+    }
     kmax = std::max(kmax, F.k_last[m]);
     if (!F.active0.empty()) active_ev[m] = F.active0[m];
     set_faults(m);
   }
   std::memcpy(adv, run.data(), L);
   std::memset(pred, 0, L);
-  HIPCHK(hipMemcpyAsync(C->d_post.p, hp, R.post_bytes(), hipMemcpyHostToDevice, C->stream));
+  HIPCHK(hipMemcpyAsync(C->d_round.p, hp, R.P(), hipMemcpyHostToDevice, C->stream));
   launch_copy_active(n, L, C->d_z.p, C->d_zold.p, C->d_adv.p, C->stream);
   HIPCHK(hipMemsetAsync(C->d_failed.p, 0, L, C->stream));
   HIPCHK(hipStreamSynchronize(C->stream));   // the staging block stays valid until then
+  std::memset(adv, 0, L);
   Newton N(C, method, dt, tol, itermax, true);
   N.dtv = C->d_dtv.p;
   N.exactv = C->d_exact.p;
@@ -1977,8 +2010,27 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
   // first iterate of a step by extrapolation (gp_options.predictor) under
   // the policies that reuse factorisations
   const bool extrapolate = C->opt.predictor >= 0 && N.reuse;
+  // quadratic extrapolation (predictor 2) after two smooth steps: z_(k-2)
+  // is kept in zold2; nplain counts the accepted steps since the start or
+  // since the last event's re-solve
+  const bool quad = extrapolate && C->opt.predictor == 2;
+  if (quad) C->d_zold2.ensure(nL);
+  std::vector<int> nplain(L, 0);
   bool fault_dirty = true;
   const long long max_rounds = (long long)(kmax + 1) * 2 * (4 * itermax + 16);
+  // the previous round's accepted steps (record, segment and checkpoint
+  // copies, advance), launched after the copy that uploads them together
+  // with this round's flags
+  bool any_acc = false, any_adv = false, any_sv = false, any_ck = false;
+  auto launch_accepted = [&]() {
+    if (any_acc)
+      launch_record_at(C->nrec, C->rec_idx.empty() ? nullptr : C->d_rec_idx.p, L, M, C->d_z.p, C->d_hist.p,
+                       C->d_col.p, C->d_acc.p, C->stream);
+    if (any_sv) launch_copy_cols(n, L, C->d_z.p, F.seg, nL, C->d_svslot.p, C->d_sv.p, C->stream);
+    if (any_ck) launch_copy_cols(n, L, C->d_z.p, F.ckpt, nL, C->d_ckslot.p, C->d_ck.p, C->stream);
+    if (any_adv) launch_advance(n, L, C->d_z.p, C->d_zold.p, quad ? C->d_zold2.p : nullptr, C->d_adv.p, C->d_pred.p,
+                                C->stream);
+  };
   for (long long round = 0;; round++) {
     int nrun = 0;
     bool land_first = false;
@@ -1993,11 +2045,19 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
       hr[R.work() + m] = run[m];
       hdt[m] = dtv[m];
     }
-    if (nrun == 0) break;
+    if (nrun == 0) {
+      if (any_acc || any_sv || any_ck) {
+        HIPCHK(hipMemcpyAsync(C->d_round.p, hp, R.P(), hipMemcpyHostToDevice, C->stream));
+        launch_accepted();
+      }
+      break;
+    }
     if (round >= max_rounds) throw GpError{GP_ERR_HIP, "internal: scenario-asynchronous loop did not finish"};
     S.newton_rounds++;
     std::memset(hr + R.cnt(), 0, 16);
+    std::memset(hr + R.failed(), 0, L);
     HIPCHK(hipMemcpyAsync(C->d_round.p, hr, R.up_bytes(), hipMemcpyHostToDevice, C->stream));
+    launch_accepted();
     if (fault_dirty) {
       HIPCHK(hipMemcpyAsync(C->d_fbus.p, fbus.data(), fbus.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
       HIPCHK(hipMemcpyAsync(C->d_fg.p, fg.data(), fg.size() * sizeof(double), hipMemcpyHostToDevice, C->stream));
@@ -2025,11 +2085,35 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
     std::memset(sv, 0, L);
     std::memset(ck, 0, L);
     std::memset(pred, 0, L);
-    bool any_acc = false, any_adv = false, any_fail = false, any_sv = false, any_ck = false, any_restart = false;
+    any_acc = any_adv = any_sv = any_ck = false;
+    bool any_fail = false, any_restart = false;
     std::fill(restart.begin(), restart.end(), 0);
+    // zero or non-finite pivots (the factorisations' status bits 1, 2), read
+    // when a solve failed
+    std::vector<int> pst;
+    if (F.repivot)
+      for (int m = 0; m < M && pst.empty(); m++)
+        if (run[m] && fl[m]) {
+          pst.resize(L);
+          HIPCHK(hipMemcpyAsync(pst.data(), C->d_status.p, L * sizeof(int), hipMemcpyDeviceToHost, C->stream));
+          HIPCHK(hipStreamSynchronize(C->stream));
+        }
     for (int m = 0; m < M; m++) {
       if (!run[m]) continue;
       first[m] = 0;
+      if (fl[m] && !pst.empty() && (pst[m] & 1)) {
+        // the static pivot sequence met an exactly zero pivot: the scenario
+        // leaves this run and continues on a plan of its own
+        run[m] = 0;
+        RepivotReq q;
+        q.m = m;
+        q.k = k[m];
+        q.mode = mode[m];
+        q.active = active_ev[m];
+        F.repivot->push_back(q);
+        C->queue[m] = 0;   // its queued refactorisation is not staged any more
+        continue;
+      }
       if (fl[m] && N.retryable() && !exact[m]) {
         // a solve on reused or FP32 factorisations failed: it runs again
         // from the step's initial iterate as exact Newton with FP64 factors
@@ -2083,6 +2167,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
             if (it != v.end()) v.erase(it);
             changed = true;
           }
+        nplain[m] = changed ? 0 : nplain[m] + 1;
         if (changed) {       // re-solve at dt = 0 under the new fault state
           set_faults(m);
           fault_dirty = true;
@@ -2115,34 +2200,22 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
           dtv[m] = dt;
           first[m] = 1;
           adv[m] = 1;
-          pred[m] = (extrapolate && plain) ? 1 : 0;
+          pred[m] = (extrapolate && plain) ? ((quad && nplain[m] >= 2) ? 2 : 1) : 0;
           any_adv = true;
         }
       }
     }
-    if (any_acc || any_adv || any_sv || any_ck)
-      HIPCHK(hipMemcpyAsync(C->d_post.p, hp, R.post_bytes(), hipMemcpyHostToDevice, C->stream));
-    if (any_acc)
-      launch_record_at(C->nrec, C->rec_idx.empty() ? nullptr : C->d_rec_idx.p, L, M, C->d_z.p, C->d_hist.p,
-                       C->d_col.p, C->d_acc.p, C->stream);
-    if (any_sv) launch_copy_cols(n, L, C->d_z.p, F.seg, nL, C->d_svslot.p, C->d_sv.p, C->stream);
-    if (any_ck) launch_copy_cols(n, L, C->d_z.p, F.ckpt, nL, C->d_ckslot.p, C->d_ck.p, C->stream);
-    if (any_adv) launch_advance(n, L, C->d_z.p, C->d_zold.p, C->d_adv.p, C->d_pred.p, C->stream);
     if (any_restart) {
       set_mask(C, C->d_sel, restart);
       restart_solves(C, restart, C->d_sel.p);
-      if (!any_fail) HIPCHK(hipMemsetAsync(C->d_failed.p, 0, L, C->stream));
     }
-    if (any_fail) {          // non-finite states
-      std::vector<double> fn(L);
-      HIPCHK(hipMemcpy(fn.data(), C->d_fnorm.p, L * sizeof(double), hipMemcpyDeviceToHost));
+    // scenarios that stopped: non-finite |F| (failed flag bit 1)
+    if (any_fail)
       for (int m = 0; m < M; m++)
-        if (fl[m] && everfail[m] && C->scen_status[m] == GP_ERR_NEWTON_NOT_CONVERGED && !std::isfinite(fn[m])) {
+        if (fl[m] && everfail[m] && C->scen_status[m] == GP_ERR_NEWTON_NOT_CONVERGED && !run[m] && (fl[m] & 2)) {
           S.n_nan++;
           C->scen_status[m] = GP_ERR_NAN;
         }
-      HIPCHK(hipMemsetAsync(C->d_failed.p, 0, L, C->stream));
-    }
   }
   HIPCHK(hipStreamSynchronize(C->stream));
   S.newton_iters = N.out.corrections;
@@ -2150,6 +2223,161 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
   S.factor_launches = N.out.factor_launches;
   for (int m = 0; m < M; m++) S.n_not_converged += everfail[m];
   return S;
+}
+
+// ---- pivot fallback -------------------------------------------------------
+// The reference's Newton refactors its KLU factorisation in place and, when
+// that meets a zero pivot (SingularException), factorises afresh with new
+// pivots (src/numerics.jl:144-167).  Here one static pivot sequence serves a
+// batch; a scenario whose factorisation meets an exactly zero pivot
+// leaves the batch (integrate_free) and continues from the start of the
+// failed solve, alone, on a plan analysed for its own matrix.
+
+// Values of scenario m's matrix (J in CSC order, or S) at its current state
+// with dt = 0, the matrix kind the plans' pivots are chosen on, evaluated by
+// the product's kernels under the scenario's event slots.
+std::vector<double> scenario_matrix(gp_ctx *C, int method, int m) {
+  const int L = C->ldM;
+  std::vector<uint8_t> sel(L, 0);
+  sel[m] = 1;
+  set_mask(C, C->d_sel, sel);
+  if (method == GP_METHOD_MONO_FP64) {
+    const int nnz = C->jcp[C->n];
+    HIPCHK(hipMemsetAsync(C->d_jv.p, 0, (size_t)nnz * L * sizeof(double), C->stream));
+    launch_jac_mono(C->T, C->d_p.p, C->d_z.p, 0.0, faults(C), C->d_jv.p, C->d_sel.p, C->stream);
+    HIPCHK(hipGetLastError());
+    std::vector<double> jv(nnz);
+    HIPCHK(hipMemcpy2DAsync(jv.data(), sizeof(double), C->d_jv.p + m, (size_t)L * sizeof(double), sizeof(double),
+                            nnz, hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return jv;
+  }
+  // at dt = 0 the matrix does not depend on zold (z stands in for it)
+  launch_cluster_residual(C->T, C->d_p.p, C->d_z.p, C->d_z.p, 0.0, C->d_f.p, C->d_inj.p, C->d_sel.p, C->stream);
+  launch_bus_residual(C->T, C->d_p.p, C->d_z.p, C->d_inj.p, faults(C), C->d_f.p, C->d_sel.p, C->stream);
+  launch_cluster_schur<double>(C->T, C->d_p.p, C->d_z.p, C->d_f.p, 0.0, C->d_X64.p, C->d_Dg.p, C->d_sel.p, C->stream);
+  launch_bus_schur(C->T, C->d_p.p, C->d_z.p, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_sel.p,
+                   C->stream);
+  HIPCHK(hipGetLastError());
+  const int nv = std::max(1, C->n_svar);
+  std::vector<double> sv(nv);
+  HIPCHK(hipMemcpy2DAsync(sv.data(), sizeof(double), C->d_svar.p + m, (size_t)L * sizeof(double), sizeof(double), nv,
+                          hipMemcpyDeviceToHost, C->stream));
+  HIPCHK(hipStreamSynchronize(C->stream));
+  const int nnzS = C->scp[C->nred];
+  std::vector<double> sx(nnzS);
+  for (int e = 0; e < nnzS; e++) {
+    const int q = C->s_src[e];
+    sx[e] = q >= 0 ? sv[q] : C->s_cst[-q - 1];
+  }
+  return sx;
+}
+
+// The method's plan made for scenario m: pivots large in its own matrices
+// (own[0] the latest, own[1..] those of its earlier pivot failures), in the
+// batch plan's representative matrix and in the matrix of every set of lines
+// m trips; the dense block within the batch plan's bound (no timing).  The
+// next gp_integrate makes the batch plan again.
+void analyse_own(gp_ctx *C, int method, int m, const std::vector<std::vector<double>> &own) {
+  const bool mono = method == GP_METHOD_MONO_FP64;
+  LUPlan &P = mono ? C->luJ : C->luS;
+  int cap = P.tuned_cap >= 0 ? P.tuned_cap : C->opt.dense_block;
+  if (cap == 0) cap = -1;
+  const std::vector<double> &r0 = mono ? C->rep0_J : C->rep0_S;
+  std::vector<std::vector<double>> extra(own.begin() + 1, own.end());
+  extra.push_back(r0);
+  for (auto &t : trip_matrices(C, r0, !mono, m)) extra.push_back(std::move(t));
+  if (mono) {
+    std::string e = splu_analyse(C->n, C->jcp.data(), C->jri.data(), own[0].data(), C->luJ.A, 1e-3, ptrs(extra));
+    if (!e.empty()) throw GpError{GP_ERR_SINGULAR_PIVOT, "Jacobian analysis of scenario " + std::to_string(m) + ": " + e};
+    const int nnz = C->jcp[C->n];
+    std::vector<int> asrc(nnz);
+    for (int i = 0; i < nnz; i++) asrc[i] = i;
+    C->luJ.tune(asrc, own[0], nullptr, sizeof(double), 1.0, C->stream, cap);
+    if (C->luJ.dev.gx) {
+      C->d_gxJ.ensure((size_t)C->n * C->ldM);
+      C->luJ.dev.gx_ws = C->d_gxJ.p;
+    }
+    C->d_luJ.alloc(C->luJ.lu_elems() * C->M);
+    C->d_svJ.alloc((size_t)C->luJ.dev.ns * C->M);
+    HIPCHK(hipMemset(C->d_svJ.p, 0, C->d_svJ.n * sizeof(double)));   // padding slots stay zero
+    C->d_dnJ.alloc((size_t)std::max(1, C->luJ.A.nd * C->luJ.A.nd) * C->M);
+    C->luJ.shape(sizeof(double));
+  } else {
+    std::string e = splu_analyse(C->nred, C->scp.data(), C->sri.data(), own[0].data(), C->luS.A, 1e-3, ptrs(extra));
+    if (!e.empty())
+      throw GpError{GP_ERR_SINGULAR_PIVOT, "reduced-network analysis of scenario " + std::to_string(m) + ": " + e};
+    const bool fp32 = method == GP_METHOD_SCHUR_FP32;
+    C->luS.tune(C->s_src, {}, C->d_s_cst.p, fp32 ? sizeof(float) : sizeof(double), 2.0, C->stream, cap);
+    if (C->luS.dev.gx) {
+      C->d_gxS.ensure((size_t)C->nred * C->ldM);
+      C->luS.dev.gx_ws = C->d_gxS.p;
+    }
+    ensure_schur_buffers(C, fp32);
+    // the new plan's padding slots of the solve arrays must read zero
+    HIPCHK(hipMemset(C->d_svS64.p, 0, C->d_svS64.n * sizeof(double)));
+    if (fp32) HIPCHK(hipMemset(C->d_svS32.p, 0, C->d_svS32.n * sizeof(float)));
+  }
+  P.ready = true;
+  P.trip_gen = -2;   // not the batch plan: the next call analyses again
+}
+
+gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int itermax, const FreeSpec &F);
+
+// Runs the scenarios of `reqs` one after another on plans of their own, each
+// from the start of its failed solve to its horizon; a scenario that meets a
+// zero pivot again is analysed once more with that matrix added (at most
+// kTries plans).  Their statistics join S.
+void repivot_runs(gp_ctx *C, int method, double dt, double tol, int itermax, std::vector<RepivotReq> reqs,
+                  gp_stats &S) {
+  constexpr int kTries = 4;
+  const int L = C->ldM, M = C->M;
+  land_batch(C, true);
+  HIPCHK(hipStreamSynchronize(C->stream));
+  std::map<int, std::vector<std::vector<double>>> own;
+  while (!reqs.empty()) {
+    const RepivotReq q = reqs.front();
+    reqs.erase(reqs.begin());
+    const int m = q.m;
+    // back to the start of the failed solve (zold: z_(k-1), or z_k for a
+    // re-solve); the scenario's event slots are those of that solve
+    std::vector<uint8_t> sel(L, 0);
+    sel[m] = 1;
+    set_mask(C, C->d_sel, sel);
+    launch_copy_active(C->n, L, C->d_zold.p, C->d_z.p, C->d_sel.p, C->stream);
+    if ((int)own[m].size() >= kTries) {   // its status bit stays: reported as a singular pivot
+      C->scen_steps[m] = q.mode ? q.k : q.k - 1;
+      continue;
+    }
+    HIPCHK(hipMemsetAsync(C->d_status.p + m, 0, sizeof(int), C->stream));   // every batch has landed
+    own[m].insert(own[m].begin(), scenario_matrix(C, method, m));
+    analyse_own(C, method, m, own[m]);
+    FreeSpec F;
+    F.k_first.assign(M, 1);
+    F.k_last.assign(M, 0);
+    F.k_first[m] = q.k;
+    F.k_last[m] = C->nsteps[m];
+    F.active0.assign(M, {});
+    F.active0[m] = q.active;
+    F.resolve0.assign(M, 0);
+    F.resolve0[m] = (uint8_t)q.mode;
+    std::vector<RepivotReq> again;
+    F.repivot = &again;
+    const gp_stats s2 = integrate_free(C, method, dt, tol, itermax, F);
+    land_batch(C, true);
+    S.newton_iters += s2.newton_iters;
+    S.scenario_steps += s2.scenario_steps;
+    S.event_resolves += s2.event_resolves;
+    S.refactorizations += s2.refactorizations;
+    S.n_not_converged += s2.n_not_converged;
+    S.n_nan += s2.n_nan;
+    S.newton_rounds += s2.newton_rounds;
+    S.factor_launches += s2.factor_launches;
+    S.exact_retries += s2.exact_retries;
+    S.repivots++;
+    for (const RepivotReq &r : again) reqs.insert(reqs.begin(), r);
+  }
+  C->luS.ready = C->luJ.ready = false;
 }
 
 // The fault state in force during each step of every scenario, from the
@@ -2348,6 +2576,7 @@ int gp_create(const gp_system_desc *d, int32_t M, const gp_options *opt, gp_ctx 
     if (opt) C->opt = *opt;
     else { C->opt.record_all = 1; C->opt.refactor_policy = GP_REFACTOR_EVERY_ITER; }
     C->luJ.force_gx = C->luS.force_gx = C->luJT.force_gx = C->opt.global_x != 0;
+    C->prof_solve.every = std::max(1, C->opt.profile);   // profile = k: every k-th solve launch timed
     C->is_diff.assign(d->is_diff, d->is_diff + d->n);
     C->yrp.assign(d->y_rowptr, d->y_rowptr + 2 * d->nbus + 1);
     C->yci.assign(d->y_colind, d->y_colind + d->y_nnz);
@@ -2674,7 +2903,10 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
       FreeSpec F;
       F.k_first.assign(M, 1);
       F.k_last.assign(C->nsteps.begin(), C->nsteps.begin() + M);
+      std::vector<RepivotReq> reqs;
+      F.repivot = &reqs;
       S = integrate_free(C, method, dt, tol, itermax, F);
+      if (!reqs.empty()) repivot_runs(C, method, dt, tol, itermax, reqs, S);
     }
     land_batch(C, true);   // no factorisation outlives the call
     S.elapsed_ms = S.solve_ms = tm.ms();

@@ -158,7 +158,8 @@ void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const doubl
                        const int *scatter, double *zupd, double *xout, const uint8_t *work, const uint8_t *cur, int M,
                        hipStream_t s);
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s);
-void launch_advance(int n, int ldM, double *z, double *zold, const uint8_t *act, const uint8_t *pred, hipStream_t s);
+void launch_advance(int n, int ldM, double *z, double *zold, double *zold2, const uint8_t *act, const uint8_t *pred,
+                    hipStream_t s);
 // columns of the scenarios in mask into slot[m] of a stack of scenario-major arrays
 void launch_copy_cols(int n, int ldM, const double *src, double *dst, size_t stride, const int *slot,
                       const uint8_t *mask, hipStream_t s);

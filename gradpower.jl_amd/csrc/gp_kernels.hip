@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(256) k_rownorm(int n, int ldM, int M, const do
 // does.  Converged scenarios (|F| < tol) leave `work`.  A scenario that has
 // used its itermax corrections, or (stall > 0) whose |F| is not finite,
 // exceeds 1e10 or has not halved within `stall` of its corrections, leaves
-// `work` with failed[m] = 1.  refac[m] asks for a numeric factorisation: at
+// `work` with failed[m] = 1 (3 when |F| is not finite).  refac[m] asks for a numeric factorisation: at
 // the first iteration refac0[m] (or 1 without refac0); later when every
 // iteration refactors (theta <= 0) or |F| fell by less than theta since the
 // previous iteration, except while one is pending (pend[m]).  With
@@ -528,7 +528,7 @@ __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict_
   }
   if (fail) {
     work[m] = 0;
-    failed[m] = 1;
+    failed[m] = isfinite(v) ? 1 : 3;   // bit 1: |F| not finite
     return;
   }
   atomicAdd(n_unconv, 1);
@@ -1284,6 +1284,13 @@ __global__ void __launch_bounds__(64) k_cluster_backsub(DevTables T, const TF *_
 // robin; the map below gives the workgroups of one XCD consecutive scenarios,
 // so the scenario-major vectors they gather from share that XCD's L2 lines.
 
+// Status bits of a pivot: 1 exactly zero (the static pivot sequence does not
+// fit this matrix), 2 not finite (a non-finite operand).
+template <typename TF>
+__device__ __forceinline__ int pivot_flag(TF piv) {
+  return piv == TF(0) ? 1 : (isfinite((double)piv) ? 0 : 2);
+}
+
 __device__ __forceinline__ int scenario_of_block(int b, int nb) {
   if (nb & 7) return b;
   return (b & 7) * (nb >> 3) + (b >> 3);
@@ -1442,7 +1449,7 @@ __device__ __forceinline__ int thread_column(const SpluDev &S, TF *__restrict__ 
       if (d >= 0) so[d] = v;
     }
   }
-  return (!(piv != TF(0)) || !isfinite((double)piv)) ? 1 : 0;
+  return pivot_flag(piv);
 }
 
 // Wavefront mode: the column, its op locations and every multiplier it needs
@@ -1577,7 +1584,7 @@ __device__ __forceinline__ int wave_column(const SpluDev &S, TF *__restrict__ lu
     }
   }
   wave_sync();
-  return (!(piv != TF(0)) || !isfinite((double)piv)) ? 1 : 0;
+  return pivot_flag(piv);
 }
 
 // Right-looking LU without pivoting of the nd x nd block D (column-major,
@@ -1599,7 +1606,7 @@ __device__ __forceinline__ int wave_column(const SpluDev &S, TF *__restrict__ lu
 template <typename TF, int K>
 __device__ __forceinline__ void dense_pivot(TF (&a)[2][16], int p, int nd, int i0, TF *lb, int &bad) {
   const TF piv = __shfl(p < 64 ? a[0][K] : a[1][K], p & 63);
-  if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
+  bad |= pivot_flag(piv);
   const TF inv = TF(1) / piv;
 #pragma unroll
   for (int r = 0; r < 2; r++) {
@@ -1704,7 +1711,7 @@ __device__ int dense_lu_wave(TF *D, int nd, int tid) {
     for (int p = 0; p < NDC; p++) {
       if (p < nd) {
         const TF piv = readlane_t(a[p], p);
-        if (!(piv != TF(0)) || !isfinite((double)piv)) bad = 1;
+        bad |= pivot_flag(piv);
         const TF inv = TF(1) / piv;
         const TF l = i > p ? a[p] * inv : TF(0);
         if (i > p) a[p] = l;
@@ -1836,7 +1843,7 @@ __device__ __forceinline__ int res_group_column(const int *rec, const int2 *ops,
     x[t] = v;
     Lv[lbase + t - dl - 1] = v;
   }
-  return (!(piv != TF(0)) || !isfinite((double)piv)) ? 1 : 0;
+  return pivot_flag(piv);
 }
 
 // Initial value of a matrix slot: var row s of scenario m (s >= 0), constant
@@ -2658,15 +2665,22 @@ __global__ void k_copy_active(int n, int ldM, const double *__restrict__ src, do
 // The scenarios in act start a new step: zold = z, and where pred[m] the
 // first iterate is the linear extrapolation z = 2 z - zold (the previous
 // step's increment once more).
+// Accepted z_k of the scenarios in act becomes zold; z starts the next step
+// at z_k (pred 0), at the linear extrapolation 2 z_k - z_(k-1) (pred 1) or at
+// the quadratic one 3 z_k - 3 z_(k-1) + z_(k-2) (pred 2).  zold2 (optional)
+// keeps z_(k-1) for the quadratic one.
 __global__ void k_advance(int n, int ldM, double *__restrict__ z, double *__restrict__ zold,
-                          const uint8_t *__restrict__ act, const uint8_t *__restrict__ pred) {
+                          double *__restrict__ zold2, const uint8_t *__restrict__ act,
+                          const uint8_t *__restrict__ pred) {
   const int m = blockIdx.y * WAVE + threadIdx.x;
   if (!act[m]) return;
-  const bool p = pred[m] != 0;
+  const int p = pred[m];
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const size_t o = (size_t)i * ldM + m;
-    const double v = z[o];
-    if (p) z[o] = 2.0 * v - zold[o];
+    const double v = z[o], w = zold[o];
+    if (p == 1) z[o] = 2.0 * v - w;
+    if (p == 2) z[o] = 3.0 * (v - w) + zold2[o];
+    if (zold2) zold2[o] = w;
     zold[o] = v;
   }
 }
@@ -2855,9 +2869,10 @@ void launch_copy_active(int n, int ldM, const double *src, double *dst, const ui
   const int gx = n < 1024 ? n : 1024;
   hipLaunchKernelGGL(k_copy_active, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, n, ldM, src, dst, act);
 }
-void launch_advance(int n, int ldM, double *z, double *zold, const uint8_t *act, const uint8_t *pred, hipStream_t s) {
+void launch_advance(int n, int ldM, double *z, double *zold, double *zold2, const uint8_t *act, const uint8_t *pred,
+                    hipStream_t s) {
   const int gx = n < 1024 ? n : 1024;
-  hipLaunchKernelGGL(k_advance, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, n, ldM, z, zold, act, pred);
+  hipLaunchKernelGGL(k_advance, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, n, ldM, z, zold, zold2, act, pred);
 }
 void launch_record_at(int nrec, const int *idx, int ldM, int M, const double *z, double *hist, const int *col,
                       const uint8_t *act, hipStream_t s) {

@@ -38,7 +38,7 @@ class COptions(C.Structure):
                 ("stall_window", C.c_int32), ("profile", C.c_int32), ("lock_step", C.c_int32),
                 ("dense_block", C.c_int32), ("retry_window", C.c_int32), ("global_x", C.c_int32),
                 ("refactor_theta", C.c_double),
-                ("async_below", C.c_double), ("predictor", C.c_int32)]
+                ("async_below", C.c_double), ("predictor", C.c_int32), ("pivot_rule", C.c_int32)]
 
 
 class CStats(C.Structure):
@@ -51,7 +51,7 @@ class CStats(C.Structure):
                 ("solve_kernel_scenarios", C.c_int64), ("factor_kernel_ms", C.c_double),
                 ("factor_kernel_launches", C.c_int64), ("factor_kernel_scenarios", C.c_int64),
                 ("solve_bytes_per_scenario", C.c_double), ("factor_bytes_per_scenario", C.c_double),
-                ("exact_retries", C.c_int64)]
+                ("exact_retries", C.c_int64), ("repivots", C.c_int64)]
 
 
 class CSymInfo(C.Structure):
@@ -141,7 +141,7 @@ class HipContext:
     def __init__(self, desc: SystemDesc, M: int, record_idx=None,
                  refactor_policy=REFACTOR_EVERY_ITER, stall_window=0, refactor_theta=0.0, async_below=0.0,
                  profile=False, lock_step=False, dense_block=0, retry_window=0, global_x=False,
-                 predictor=0):
+                 predictor=0, pivot_rule=0):
         self.lib = load()
         self.desc = desc
         self.M = int(M)
@@ -161,12 +161,13 @@ class HipContext:
         opt.stall_window = int(stall_window)
         opt.refactor_theta = float(refactor_theta)
         opt.async_below = float(async_below)
-        opt.profile = 1 if profile else 0
+        opt.profile = int(profile) if profile else 0   # k: every k-th solve launch timed
         opt.lock_step = 1 if lock_step else 0
         opt.dense_block = int(dense_block)
         opt.retry_window = int(retry_window)
         opt.global_x = 1 if global_x else 0
         opt.predictor = int(predictor)
+        opt.pivot_rule = int(pivot_rule)
         self._opt = opt
         h = C.c_void_p()
         self._check(self.lib.gp_create(C.byref(self._cs), self.M, C.byref(opt), C.byref(h)))

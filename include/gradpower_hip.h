@@ -177,9 +177,11 @@ typedef struct gp_options {
                               1e10, or is not finite, stops at once and is reported
                               as not converged (failed contingencies cost k rounds
                               of the lock step instead of itermax)               */
-  int32_t profile;         /* 1: time every launch of the batched triangular solve
-                              and of the numeric factorisation with HIP events on
-                              the stream it runs on (gp_stats kernel fields)    */
+  int32_t profile;         /* k >= 1: time every launch of the numeric
+                              factorisation and every k-th launch of the batched
+                              triangular solve with HIP events on the stream it
+                              runs on (gp_stats kernel fields: the solve's cover
+                              the timed launches); 0: no timing                 */
   int32_t lock_step;       /* 0 (default): every scenario advances on its own --
                               a scenario whose step converged moves on while the
                               others iterate; 1: all scenarios take step k
@@ -215,8 +217,21 @@ typedef struct gp_options {
                               ASYNC; not across an event, whose step starts
                               from z_k), z_k under exact Newton (the
                               reference's start, src/dynamics.jl:548-640);
+                              2: the quadratic one 3 z_k - 3 z_(k-1) + z_(k-2)
+                              once two steps have been accepted since the
+                              start or the last event (linear before);
                               -1: z_k always.  A solve that fails restarts
                               from z_k as exact Newton either way            */
+  int32_t pivot_rule;      /* static pivots of gp_integrate's batched LU plans:
+                              0 (default) large in the intact matrix and in the
+                              matrix of every distinct set of tripped lines; 1
+                              the intact matrix alone.  Either way, in the
+                              free-running loop (lock_step 0), a scenario whose
+                              factorisation meets an exactly zero pivot
+                              (klu!'s SingularException, src/numerics.jl:144-167)
+                              continues on a pivot sequence chosen for its own
+                              matrix (gp_stats.repivots); lock step and the
+                              adjoint report GP_ERR_SINGULAR_PIVOT instead    */
 } gp_options;
 
 typedef struct gp_stats {
@@ -248,6 +263,11 @@ typedef struct gp_stats {
      every iteration (the reference's method) before the scenario counts as
      failed                                                                 */
   int64_t exact_retries;
+  /* scenarios that met an exactly zero pivot and continued on a pivot
+     sequence analysed for their own matrix (the reference's fresh klu after
+     a SingularException, src/numerics.jl:144-167): each continues from the
+     start of the failed step, alone, on that plan                          */
+  int64_t repivots;
 } gp_stats;
 
 typedef struct gp_ctx gp_ctx;

@@ -42,11 +42,12 @@ def main():
                                    REFACTOR_ASYNC, REFACTOR_EVERY_ITER)
     t0 = time.perf_counter()
     case = synth.case2_grid(M=256) if args.case == "c2" else synth.case4_grid()
-    case.tf = np.minimum(case.tf, args.horizon_cap)
+    if args.horizon_cap > 0:   # 0: the full horizons
+        case.tf = np.minimum(case.tf, args.horizon_cap)
     print(f"setup {time.perf_counter() - t0:.1f} s", flush=True)
     meth = {"schur32": METHOD_SCHUR_FP32, "schur64": METHOD_SCHUR_FP64, "mono": METHOD_MONO_FP64}[args.method]
     pol = REFACTOR_EVERY_ITER if args.method == "mono" else REFACTOR_ASYNC
-    ctx = HipContext(case.desc, case.M, record_idx=np.zeros(1, np.int32), refactor_policy=pol, profile=True)
+    ctx = HipContext(case.desc, case.M, record_idx=np.zeros(1, np.int32), refactor_policy=pol, profile=8)
     ev = [(m, e.bus, e.rfault, int(round(e.ton / case.dt)), int(round(e.toff / case.dt)))
           for m in range(case.M) for e in case.events[m]]
     ctx.set_state(case.desc.z0, case.desc.p)

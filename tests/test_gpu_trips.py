@@ -298,6 +298,48 @@ def test_n1_screening_every_line_of_ieee39(method, policy):
     ctx.close()
 
 
+@pytest.mark.parametrize("method,policy", [("mono", 0), ("schur64", 0), ("schur32", 3)])
+def test_zero_pivot_fallback_n1_screening_ieee39(method, policy):
+    """The KLU fallback (src/numerics.jl:144-167: klu! meets a zero pivot,
+    klu factorises afresh).  N-1 screening of every line of the IEEE 39-bus
+    case on a plan whose pivots were chosen on the intact matrix alone
+    (pivot_rule 1).  For the monolithic J that plan meets exactly zero
+    pivots on 14 of the 46 single-trip matrices (the CPU emulation in
+    tests/test_pivots.py; 13 on the GPU, whose 14th is a rounding residue
+    instead of an exact zero): those scenarios leave the batch when their
+    solve fails and continue on plans analysed for their own matrices, from
+    the start of the failed step.  The reduced network's plan keeps nonzero
+    pivots on every trip here (its voltage columns keep their diagonals), so
+    the Schur methods run without a fallback.  Every scenario follows the
+    oracle within 1e-9 pu to its horizon."""
+    from gradpower_hip.lib import HipContext
+    from gradpower_hip.system import create_trip_line_event
+    from oracle.oracle import Oracle
+    case = get_case("case_psse", raw=os.path.join(G, "IEEE39.raw"), dyr=os.path.join(G, "IEEE39_gov.dyr"), M=8)
+    ps = case.ps
+    trips = [create_trip_line_event(ps, ps.buses[b.fr].i, ps.buses[b.to].i, 0.0) for b in ps.branches]
+    M, ns = len(trips), 60
+    print(f"zero-pivot fallback: {method}, {M} trips", flush=True)
+    ctx = HipContext(case.desc, M, refactor_policy=policy, pivot_rule=1)
+    ctx.set_state(case.desc.z0, case.desc.p)
+    ctx.set_events([], [ns] * M)
+    ctx.set_line_trips([(m, t.from_bus, t.to_bus, 6, 0, t.stamp()) for m, t in enumerate(trips)])
+    st = ctx.integrate(METHODS[method], case.dt, TOL, 50, raise_on_fail=False)
+    status, done = ctx.scenario_status()
+    print(f"  status {st['status']}, repivots {st['repivots']}, steps {st['scenario_steps']}, "
+          f"stopped {[m for m in range(M) if status[m] != 0]}", flush=True)
+    if method == "mono":
+        assert st["repivots"] >= 10
+    assert st["status"] == 0 and st["scenario_steps"] == M * ns
+    o = Oracle(case.desc)
+    for m, t in enumerate(trips):
+        ref, _, ost = o.integrate(case.dt, ns, [], tol=TOL, itermax=50, trips=[(t.from_bus, t.to_bus, 6, 0, t.stamp())])
+        assert ost["not_converged"] == 0
+        dev = np.max(np.abs(ctx.trajectory(m).T - ref))
+        assert dev < 1e-9, (method, m, dev)
+    ctx.close()
+
+
 def test_event_slots_checked_when_set():
     """Faults, tripped lines and dropped loads share 8 event slots per
     scenario.  Scenario 0 holds 8 at once (6 ZIP loads dropped for good plus
