@@ -441,6 +441,7 @@ struct gp_ctx {
   DBuf<int> d_dev[GP_NDEVTYPES], d_cl, d_yrp, d_yci, d_bus_cl_p, d_bus_cl, d_bus_zip_p, d_bus_zip, d_bus_sg;
   DBuf<int> d_jp[6], d_jp_bus, d_jp_yoff_p, d_jp_yoff_pos, d_cl_nt, d_cl_xoff, d_red_of_z, d_z_of_red, d_sv_bus;
   DBuf<int> d_rec_idx;
+  DBuf<int> d_rec_p, d_rec_ch;   // recorded channels of each row (CSR; empty when every row is recorded)
   DBuf<double> d_yv, d_u0, d_jp_yoff_val, d_ydiag, d_s_cst;
   DBuf<int> d_tr_bus, d_tv_p, d_tv_i;
   DBuf<double> d_tr_y, d_tv_base;
@@ -1262,7 +1263,7 @@ int stage_batch(gp_ctx *C, int l, bool fp32, const uint8_t *exacth) {
     HIPCHK(hipMemcpyAsync(ln.req, slot, 2 * L, hipMemcpyHostToDevice, C->stream));
     launch_async_flags(L, 1, ln.req, C->d_cur.p, C->d_pend.p, C->stream);
   }
-  launch_copy_active(std::max(1, C->n_svar), L, C->d_svar.p, ln.snap.p, ln.req, C->stream);
+  launch_copy_transpose(std::max(1, C->n_svar), L, C->d_svar.p, ln.snap.p, ln.req, C->stream);   // [m][n_svar]
   HIPCHK(hipEventRecord(ln.ev_snap, C->stream));
   ln.nb = nb;
   return l;
@@ -1276,7 +1277,8 @@ void start_batch(gp_ctx *C, int l, TF *luv, TF *sv, TF *dn) {
   // timed: lane 0, the method's own precision (lane 1 holds the FP32
   // method's FP64 exact-Newton factorisations)
   const int pi = l == 0 ? prof_begin(C, C->prof_factor, ln.stream) : -1;
-  launch_splu_factor_impl<TF>(sizeof(TF) == 8 ? C->luS64 : C->luS.dev, ln.snap.p, C->d_s_cst.p, L, luv, sv, dn,
+  launch_splu_factor_impl<TF>(sizeof(TF) == 8 ? C->luS64 : C->luS.dev, ln.snap.p, C->d_s_cst.p,
+                              -std::max(1, C->n_svar), luv, sv, dn,
                               ln.req, ln.tgt, C->M, C->d_status.p, ln.stream);
   prof_end(C->prof_factor, pi, ln.stream, nb);
   HIPCHK(hipGetLastError());
@@ -2023,13 +2025,12 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
   // with this round's flags
   bool any_acc = false, any_adv = false, any_sv = false, any_ck = false;
   auto launch_accepted = [&]() {
-    if (any_acc)
-      launch_record_at(C->nrec, C->rec_idx.empty() ? nullptr : C->d_rec_idx.p, L, M, C->d_z.p, C->d_hist.p,
-                       C->d_col.p, C->d_acc.p, C->stream);
     if (any_sv) launch_copy_cols(n, L, C->d_z.p, F.seg, nL, C->d_svslot.p, C->d_sv.p, C->stream);
     if (any_ck) launch_copy_cols(n, L, C->d_z.p, F.ckpt, nL, C->d_ckslot.p, C->d_ck.p, C->stream);
-    if (any_adv) launch_advance(n, L, C->d_z.p, C->d_zold.p, quad ? C->d_zold2.p : nullptr, C->d_adv.p, C->d_pred.p,
-                                C->stream);
+    if (any_acc || any_adv)
+      launch_advance(n, L, C->d_z.p, C->d_zold.p, quad ? C->d_zold2.p : nullptr, C->d_adv.p, C->d_pred.p,
+                     any_acc ? C->d_acc.p : nullptr, C->d_col.p, C->rec_idx.empty() ? nullptr : C->d_rec_p.p,
+                     C->d_rec_ch.p, C->nrec, C->d_hist.p, C->stream);
   };
   for (long long round = 0;; round++) {
     int nrun = 0;
@@ -2617,6 +2618,13 @@ int gp_create(const gp_system_desc *d, int32_t M, const gp_options *opt, gp_ctx 
       for (int r : C->rec_idx)
         if (r < 0 || r >= C->n) return fail(GP_ERR_BADARG, "record index out of range");
       up(C->d_rec_idx, C->rec_idx);
+      std::vector<int> rp(C->n + 1, 0), rc(C->nrec);
+      for (int r : C->rec_idx) rp[r + 1]++;
+      for (int i = 0; i < C->n; i++) rp[i + 1] += rp[i];
+      std::vector<int> nx(rp.begin(), rp.end() - 1);
+      for (int j = 0; j < C->nrec; j++) rc[nx[C->rec_idx[j]]++] = j;
+      up(C->d_rec_p, rp);
+      up(C->d_rec_ch, rc);
     }
     C->nsteps.assign(M, 0);
     upload_event_slots(C.get());

@@ -149,7 +149,8 @@ void launch_cluster_backsub(const DevTables &T, const TF *X, const double *f, co
 // luv: [M][nnz_lu] factor workspace; sv: [2][M][ns] solve values; dn:
 // [2][M][nd*nd] dense trailing block (column-major LU, unit lower part
 // implied).  The factorisation writes buffer tgt[m], the solve reads cur[m]
-// (NULL: buffer 0).
+// (NULL: buffer 0).  The matrix values var are rows [s][ldM] (ldM > 0) or
+// scenario-major rows [m][-ldM] (ldM < 0).
 template <typename TF>
 void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv, TF *sv,
                              TF *dn, const uint8_t *refac, const uint8_t *tgt, int M, int *status, hipStream_t s);
@@ -158,7 +159,10 @@ void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const doubl
                        const int *scatter, double *zupd, double *xout, const uint8_t *work, const uint8_t *cur, int M,
                        hipStream_t s);
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s);
-void launch_advance(int n, int ldM, double *z, double *zold, double *zold2, const uint8_t *act, const uint8_t *pred,
+// dst[m*n + r] = src[r*ldM + m] for the scenarios in act
+void launch_copy_transpose(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s);
+void launch_advance(int n, int ldM, double *z, double *zold, double *zold2, const uint8_t *adv, const uint8_t *pred,
+                    const uint8_t *acc, const int *col, const int *rec_p, const int *rec_ch, int nrec, double *hist,
                     hipStream_t s);
 // columns of the scenarios in mask into slot[m] of a stack of scenario-major arrays
 void launch_copy_cols(int n, int ldM, const double *src, double *dst, size_t stride, const int *slot,
@@ -198,7 +202,4 @@ void launch_select_faults(int ldM, const int *ev_bus, const double *ev_g, const 
                           double *g, hipStream_t s);
 void launch_record_impl(int nrec, const int *idx, int ldM, int M, const double *z, double *hist,
                         const uint8_t *act, hipStream_t s);
-void launch_record_at(int nrec, const int *idx, int ldM, int M, const double *z, double *hist, const int *col,
-                      const uint8_t *act, hipStream_t s);
-
 }  // namespace gp

@@ -1314,6 +1314,14 @@ __device__ __forceinline__ void lds_barrier() {
 
 // A's values into the LU storage, 8 entries per thread in flight (the
 // index loads, then the value loads, then the stores).
+// Matrix value s of scenario m: var in rows [s][ldM] (ldM > 0, the Schur
+// kernels' layout) or in scenario-major rows [m][-ldM] (ldM < 0: the
+// factorisation lanes' transposed snapshots, one contiguous range per
+// scenario instead of one cache line per value).
+__device__ __forceinline__ double var_at(const double *__restrict__ var, int s, int ldM, int m) {
+  return ldM > 0 ? var[(size_t)s * ldM + m] : var[(size_t)m * (size_t)(-ldM) + s];
+}
+
 template <typename TF>
 __device__ __forceinline__ void scatter_a(int nnz_a, const int *__restrict__ a_src, const int *__restrict__ a2,
                                           const double *__restrict__ var, const double *__restrict__ cst, int ldM,
@@ -1329,7 +1337,7 @@ __device__ __forceinline__ void scatter_a(int nnz_a, const int *__restrict__ a_s
       dst[u] = a2[e];
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) v[u] = src[u] >= 0 ? var[(size_t)src[u] * ldM + m] : cst[-src[u] - 1];
+    for (int u = 0; u < U; u++) v[u] = src[u] >= 0 ? var_at(var, src[u], ldM, m) : cst[-src[u] - 1];
 #pragma unroll
     for (int u = 0; u < U; u++)
       if (e0 + tid + u * nt < nnz_a) lu[dst[u]] = TF(v[u]);
@@ -1797,7 +1805,7 @@ __global__ void __launch_bounds__(512) k_splu_factor(SpluDev S, const double *__
     TF *dn = dnv + ((size_t)bsel * Mb + m) * nd * nd;
     for (int i = tid; i < nd * nd; i += nt) dn[i] = D[i];
   }
-  if (bad) atomicOr(&status[m], 1);
+  if (bad) atomicOr(&status[m], bad);   // bit 1 zero pivot, bit 2 not finite
 }
 
 // ---- LDS-resident factorisation -----------------------------------------
@@ -1851,7 +1859,7 @@ __device__ __forceinline__ int res_group_column(const int *rec, const int2 *ops,
 template <typename TF>
 __device__ __forceinline__ TF init_value(int s, const double *__restrict__ var, const double *__restrict__ cst,
                                          int ldM, int m) {
-  if (s >= 0) return (TF)var[(size_t)s * ldM + m];
+  if (s >= 0) return (TF)var_at(var, s, ldM, m);
   return s == kResNoSrc ? TF(0) : (TF)cst[-1 - s];
 }
 
@@ -2017,7 +2025,7 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
     bad |= dense_lu_any<TF>(D, nd, tid, lbuf);
     for (int i = tid; i < nd * nd; i += nt) dn[i] = D[i];
   }
-  if (bad) atomicOr(&status[m], 1);
+  if (bad) atomicOr(&status[m], bad);   // bit 1 zero pivot, bit 2 not finite
 }
 
 // DPP lane exchange of a double within a row of 16 lanes
@@ -2655,6 +2663,30 @@ __global__ void k_copy_cols(int n, int ldM, const double *__restrict__ src, doub
   for (int i = blockIdx.x; i < n; i += gridDim.x) d[(size_t)i * ldM + m] = src[(size_t)i * ldM + m];
 }
 
+// Scenario-major copy of the scenarios in act: dst[m*n + r] = src[r*ldM + m],
+// through an LDS tile of 64 rows x 64 scenarios (rows read for the active
+// lanes only, each active scenario's 64 rows written as one run).  A block
+// whose 64 scenarios are all inactive returns at once (the four waves see
+// the same mask).
+__global__ void __launch_bounds__(256) k_copy_transpose(int n, int ldM, const double *__restrict__ src,
+                                                        double *__restrict__ dst, const uint8_t *__restrict__ act) {
+  __shared__ double tile[64][65];
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
+  const int r0 = blockIdx.x * 64, s0 = blockIdx.y * WAVE;
+  const bool on = act[s0 + lane] != 0;
+  if (!__any(on)) return;
+  for (int i = w; i < 64; i += 4) {
+    const int r = r0 + i;
+    tile[i][lane] = (on && r < n) ? src[(size_t)r * ldM + s0 + lane] : 0.0;
+  }
+  __syncthreads();
+  for (int j = w; j < WAVE; j += 4) {
+    if (!act[s0 + j]) continue;
+    const int r = r0 + lane;
+    if (r < n) dst[(size_t)(s0 + j) * n + r] = tile[lane][j];
+  }
+}
+
 __global__ void k_copy_active(int n, int ldM, const double *__restrict__ src, double *__restrict__ dst,
                               const uint8_t *__restrict__ act) {
   const int m = blockIdx.y * WAVE + threadIdx.x;
@@ -2665,23 +2697,40 @@ __global__ void k_copy_active(int n, int ldM, const double *__restrict__ src, do
 // The scenarios in act start a new step: zold = z, and where pred[m] the
 // first iterate is the linear extrapolation z = 2 z - zold (the previous
 // step's increment once more).
-// Accepted z_k of the scenarios in act becomes zold; z starts the next step
-// at z_k (pred 0), at the linear extrapolation 2 z_k - z_(k-1) (pred 1) or at
-// the quadratic one 3 z_k - 3 z_(k-1) + z_(k-2) (pred 2).  zold2 (optional)
-// keeps z_(k-1) for the quadratic one.
+// The accepted steps of a round, one pass over the rows.  Scenarios in acc
+// record z_k into trajectory column col[m]: row r goes to the channels
+// rec_ch[rec_p[r] .. rec_p[r+1]) (rec_p NULL: channel r).  Scenarios in adv
+// then start their next step: z_k becomes zold, and z starts at z_k
+// (pred 0), at the linear extrapolation 2 z_k - z_(k-1) (pred 1) or at the
+// quadratic one 3 z_k - 3 z_(k-1) + z_(k-2) (pred 2); zold2 (optional) keeps
+// z_(k-1) for the quadratic one.
 __global__ void k_advance(int n, int ldM, double *__restrict__ z, double *__restrict__ zold,
-                          double *__restrict__ zold2, const uint8_t *__restrict__ act,
-                          const uint8_t *__restrict__ pred) {
+                          double *__restrict__ zold2, const uint8_t *__restrict__ adv,
+                          const uint8_t *__restrict__ pred, const uint8_t *__restrict__ acc,
+                          const int *__restrict__ col, const int *__restrict__ rec_p,
+                          const int *__restrict__ rec_ch, int nrec, double *__restrict__ hist) {
   const int m = blockIdx.y * WAVE + threadIdx.x;
-  if (!act[m]) return;
+  const bool a = adv[m] != 0, r = acc && acc[m] != 0;
+  if (!a && !r) return;
   const int p = pred[m];
+  const size_t base = r ? (size_t)col[m] * nrec : 0;
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const size_t o = (size_t)i * ldM + m;
-    const double v = z[o], w = zold[o];
-    if (p == 1) z[o] = 2.0 * v - w;
-    if (p == 2) z[o] = 3.0 * (v - w) + zold2[o];
-    if (zold2) zold2[o] = w;
-    zold[o] = v;
+    const double v = z[o];
+    if (r) {
+      if (rec_p) {
+        for (int c = rec_p[i]; c < rec_p[i + 1]; c++) hist[(base + rec_ch[c]) * ldM + m] = v;
+      } else {
+        hist[(base + i) * ldM + m] = v;
+      }
+    }
+    if (a) {
+      const double w = zold[o];
+      if (p == 1) z[o] = 2.0 * v - w;
+      if (p == 2) z[o] = 3.0 * (v - w) + zold2[o];
+      if (zold2) zold2[o] = w;
+      zold[o] = v;
+    }
   }
 }
 
@@ -2692,19 +2741,6 @@ __global__ void k_record(int nrec, const int *__restrict__ idx, int ldM, int M, 
   for (int j = blockIdx.x; j < nrec; j += gridDim.x) {
     const int r = idx ? idx[j] : j;
     hist[(size_t)j * ldM + m] = z[(size_t)r * ldM + m];
-  }
-}
-
-// Recorded channels of z into each scenario's own step column col[m]
-// (scenarios advancing independently): hist[(col[m]*nrec + j)*ldM + m].
-__global__ void k_record_at(int nrec, const int *__restrict__ idx, int ldM, int M, const double *__restrict__ z,
-                            double *__restrict__ hist, const int *__restrict__ col, const uint8_t *__restrict__ act) {
-  const int m = blockIdx.y * WAVE + threadIdx.x;
-  if (m >= M || !act[m]) return;
-  const size_t base = (size_t)col[m] * nrec;
-  for (int j = blockIdx.x; j < nrec; j += gridDim.x) {
-    const int r = idx ? idx[j] : j;
-    hist[(base + j) * ldM + m] = z[(size_t)r * ldM + m];
   }
 }
 
@@ -2865,19 +2901,19 @@ void launch_split_precision(int L, const uint8_t *x64, const uint8_t *sch, const
                             uint8_t *out, hipStream_t s) {
   hipLaunchKernelGGL(k_split_precision, dim3((L + 255) / 256), dim3(256), 0, s, L, x64, sch, refac, upd, out);
 }
+void launch_copy_transpose(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_transpose, dim3((n + 63) / 64, ldM / WAVE), dim3(256), 0, s, n, ldM, src, dst, act);
+}
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s) {
   const int gx = n < 1024 ? n : 1024;
   hipLaunchKernelGGL(k_copy_active, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, n, ldM, src, dst, act);
 }
-void launch_advance(int n, int ldM, double *z, double *zold, double *zold2, const uint8_t *act, const uint8_t *pred,
+void launch_advance(int n, int ldM, double *z, double *zold, double *zold2, const uint8_t *adv, const uint8_t *pred,
+                    const uint8_t *acc, const int *col, const int *rec_p, const int *rec_ch, int nrec, double *hist,
                     hipStream_t s) {
   const int gx = n < 1024 ? n : 1024;
-  hipLaunchKernelGGL(k_advance, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, n, ldM, z, zold, zold2, act, pred);
-}
-void launch_record_at(int nrec, const int *idx, int ldM, int M, const double *z, double *hist, const int *col,
-                      const uint8_t *act, hipStream_t s) {
-  const int gx = nrec < 1024 ? (nrec > 0 ? nrec : 1) : 1024;
-  hipLaunchKernelGGL(k_record_at, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, nrec, idx, ldM, M, z, hist, col, act);
+  hipLaunchKernelGGL(k_advance, dim3(gx, ldM / WAVE), dim3(WAVE), 0, s, n, ldM, z, zold, zold2, adv, pred, acc, col,
+                     rec_p, rec_ch, nrec, hist);
 }
 void launch_record_impl(int nrec, const int *idx, int ldM, int M, const double *z, double *hist, const uint8_t *act,
                         hipStream_t s) {
