@@ -1286,6 +1286,12 @@ __global__ void __launch_bounds__(64) k_cluster_backsub(DevTables T, const TF *_
 
 // Status bits of a pivot: 1 exactly zero (the static pivot sequence does not
 // fit this matrix), 2 not finite (a non-finite operand).
+// An entry of a read-only plan table at a wave-uniform address, through a
+// scalar load (constant address space).  The vector-memory counter does not
+// track scalar loads, so waiting for a step header never waits for the next
+// step's prefetches still in flight.
+__device__ __forceinline__ int ldu(const int *p) { return *(const __attribute__((address_space(4))) int *)p; }
+
 template <typename TF>
 __device__ __forceinline__ int pivot_flag(TF piv) {
   return piv == TF(0) ? 1 : (isfinite((double)piv) ? 0 : 2);
@@ -1319,7 +1325,10 @@ __device__ __forceinline__ void lds_barrier() {
 // factorisation lanes' transposed snapshots, one contiguous range per
 // scenario instead of one cache line per value).
 __device__ __forceinline__ double var_at(const double *__restrict__ var, int s, int ldM, int m) {
-  return ldM > 0 ? var[(size_t)s * ldM + m] : var[(size_t)m * (size_t)(-ldM) + s];
+  // one load at a selected index (two loads under a condition would be
+  // branches, each waited for)
+  const size_t i = ldM > 0 ? (size_t)s * ldM + m : (size_t)m * (size_t)(-ldM) + s;
+  return var[i];
 }
 
 template <typename TF>
@@ -1854,14 +1863,6 @@ __device__ __forceinline__ int res_group_column(const int *rec, const int2 *ops,
   return pivot_flag(piv);
 }
 
-// Initial value of a matrix slot: var row s of scenario m (s >= 0), constant
-// -1 - s, or zero (kResNoSrc: fill).
-template <typename TF>
-__device__ __forceinline__ TF init_value(int s, const double *__restrict__ var, const double *__restrict__ cst,
-                                         int ldM, int m) {
-  if (s >= 0) return (TF)var_at(var, s, ldM, m);
-  return s == kResNoSrc ? TF(0) : (TF)cst[-1 - s];
-}
 
 // Same interface as k_splu_factor; luv holds the step-ordered values.
 // Step s: its column values sit in xb[s & 1]; while it computes (LDS only),
@@ -1869,6 +1870,8 @@ __device__ __forceinline__ TF init_value(int s, const double *__restrict__ var, 
 // are loaded into registers; after a barrier they go to the staging area and
 // xb[(s + 1) & 1], and the finished values of step s are written out (solve
 // array, dense copy), so that no wait for those stores ever holds up a step.
+__device__ double g_dzero = 0.0;
+
 template <typename TF>
 __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double *__restrict__ var,
                                                          const double *__restrict__ cst, int ldM,
@@ -1896,19 +1899,39 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
   if (tid == 0) Lv[S.res_nl] = TF(0);   // multiplier of the no-op updates
   // the step-ordered initial values in one pass (res_xsrc: the source of
   // each slot, zero for fill), groups of four slots per thread, four groups
-  // in flight
+  // in flight.  Plans without constants (J, J^T) pass cst NULL: their loads
+  // of a constant read a zero of this file instead
+  const double *cbase = cst ? cst : &g_dzero;
   for (int t0 = 4 * tid; t0 < S.res_xtot; t0 += 16 * nt) {
     int4 sv[4];
 #pragma unroll
     for (int u = 0; u < 4; u++)
       sv[u] = *reinterpret_cast<const int4 *>(S.res_xsrc + min(t0 + 4 * u * nt, (S.res_xtot - 1) & ~3));
-    TF v[16];
+    // all 32 loads (matrix value and constant, at clamped indices) are
+    // issued before any is used, and both enter every value through 0/1
+    // weights: a value chosen by a branch would let the compiler sink each
+    // load into its branch and wait for it there (a scenario with a
+    // non-finite value is non-finite throughout anyway)
+    int sx[16];
+    double a[16], c[16];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      v[4 * u] = init_value<TF>(sv[u].x, var, cst, ldM, m);
-      v[4 * u + 1] = init_value<TF>(sv[u].y, var, cst, ldM, m);
-      v[4 * u + 2] = init_value<TF>(sv[u].z, var, cst, ldM, m);
-      v[4 * u + 3] = init_value<TF>(sv[u].w, var, cst, ldM, m);
+      sx[4 * u] = sv[u].x;
+      sx[4 * u + 1] = sv[u].y;
+      sx[4 * u + 2] = sv[u].z;
+      sx[4 * u + 3] = sv[u].w;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int s = sx[q];
+      a[q] = var_at(var, max(s, 0), ldM, m);
+      c[q] = cbase[(s < 0 && s != kResNoSrc) ? -1 - s : 0];
+    }
+    TF v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const double wa = sx[q] >= 0 ? 1.0 : 0.0, wc = (sx[q] < 0 && sx[q] != kResNoSrc) ? 1.0 : 0.0;
+      v[q] = (TF)fma(a[q], wa, c[q] * wc);
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -1931,7 +1954,7 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
   const int nst = S.res_nsteps;
   auto fetch_blob = [&](int st) {
     const int *h = S.res_hdr + kResHdr * st;
-    const int bo = h[2], bl = h[3], xo = h[4], xl = h[5];
+    const int bo = ldu(h + 2), bl = ldu(h + 3), xo = ldu(h + 4), xl = ldu(h + 5);
 #pragma unroll
     for (int j = 0; j < PB; j++) {
       const int4 v = *reinterpret_cast<const int4 *>(S.res_blob + bo + min(4 * (tid + j * nt), bl - 4));
@@ -1949,7 +1972,7 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
   };
   auto fetch_omap = [&](int st) {
     const int *h = S.res_hdr + kResHdr * st;
-    const int xo = h[4], xl = h[5];
+    const int xo = ldu(h + 4), xl = ldu(h + 5);
 #pragma unroll
     for (int j = 0; j < PX; j++) {
       const int4 v = *reinterpret_cast<const int4 *>(S.res_omap + xo + min(4 * (tid + j * nt), xl - 4));
@@ -1971,7 +1994,7 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
       for (int u = 0; u < 4; u++) xd[4 * (tid + j * nt) + u] = px[4 * j + u];
   };
   auto flush = [&](int st) {
-    const int xl = S.res_hdr[kResHdr * st + 5];
+    const int xl = ldu(S.res_hdr + kResHdr * st + 5);
     const TF *xs = xbuf + (st & 1) * xcap;
     // every load of this step has landed by now (the staging stores waited
     // for the younger ones); waiting here, before the first store, keeps
@@ -1996,13 +2019,13 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
   int bad = 0;
   for (int st = 0; st < nst; st++) {
     const int *h = S.res_hdr + kResHdr * st;
-    const int mode = h[0], nc = h[1];
-    const int2 *ops = reinterpret_cast<const int2 *>(stg + h[6]);
+    const int mode = ldu(h), nc = ldu(h + 1);
+    const int2 *ops = reinterpret_cast<const int2 *>(stg + ldu(h + 6));
     TF *xb = xbuf + (st & 1) * xcap;
     fetch_omap(st);
     if (st + 1 < nst) fetch_blob(st + 1);
     {
-      const int g = h[9], G = nt / g;
+      const int g = ldu(h + 9), G = nt / g;
       for (int c = tid / g; c < nc; c += G) {
         const int *rec = stg + 8 * c;
         bad |= res_group_column<TF>(rec, ops, xb + rec[0], Lv, mode == 2, tid & (g - 1), g);
@@ -2067,7 +2090,7 @@ __device__ __forceinline__ void sol_fetch(const int *__restrict__ hdr, const int
                                           const TF *__restrict__ vals, const TF *__restrict__ dg, int tid, int nt,
                                           SolStage &P, TF (&pv)[kSolCV], TF (&pd)[kSolCD]) {
   const int *h = hdr + kSolHdr * st;
-  const int bo = h[1], bl = h[2], vo = h[3], vl = h[4], dgo = h[5], dl = h[6];
+  const int bo = ldu(h + 1), bl = ldu(h + 2), vo = ldu(h + 3), vl = ldu(h + 4), dgo = ldu(h + 5), dl = ldu(h + 6);
 #pragma unroll
   for (int j = 0; j < kSolCB / 4; j++) {
     const int4 v = *reinterpret_cast<const int4 *>(blob + bo + min(4 * (tid + j * nt), bl - 4));
@@ -2076,12 +2099,19 @@ __device__ __forceinline__ void sol_fetch(const int *__restrict__ hdr, const int
     P.b[4 * j + 2] = v.z;
     P.b[4 * j + 3] = v.w;
   }
-  const int vend = h[10];   // values actually present (the rest of vl is padding)
+  const int vend = ldu(h + 10);   // values actually present (the rest of vl is padding)
 #pragma unroll
   for (int j = 0; j < kSolCV; j++) pv[j] = vals[vo + min(tid + j * nt, max(vend - 1, 0))];
-  const int dend = h[11];
+  const int dend = ldu(h + 11);
+  // loaded either way and used as loaded: a branch around the load, or a
+  // select on its value here, would wait for it before the step computes.
+  // A step without diagonals (dl = 0: the forward side, whose dg_off counts
+  // its lanes, not diagonals) loads its own values again instead (the
+  // addresses of the loads above) and never reads them.
+  const TF *dgp = dl ? dg + dgo : vals + vo;
+  const int dlast = dl ? max(dend - 1, 0) : max(vend - 1, 0);
 #pragma unroll
-  for (int j = 0; j < kSolCD; j++) pd[j] = dl ? dg[dgo + min(tid + j * nt, max(dend - 1, 0))] : TF(0);
+  for (int j = 0; j < kSolCD; j++) pd[j] = dgp[min(tid + j * nt, dlast)];
   (void)vl;
 }
 
@@ -2107,9 +2137,9 @@ __device__ __forceinline__ void sol_store(int *sb, TF *svl, TF *sdg, int tid, in
 template <typename TF, bool BW>
 __device__ __forceinline__ void sol_step(const int *h, const int *sb, const TF *svl,
                                          const TF *sdg, double *x, int tid) {
-  const int np = h[0];
-  const int *cols = sb + h[8];
-  const int *rows = sb + h[9];
+  const int np = ldu(h);
+  const int *cols = sb + ldu(h + 8);
+  const int *rows = sb + ldu(h + 9);
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   // pass records from the staged blob (LDS: in-order, unlike scalar loads,
   // so waiting for them never waits for anything else); the next pass's
