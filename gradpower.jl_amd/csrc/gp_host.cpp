@@ -139,12 +139,22 @@ struct LUPlan {
   SpluAnalysis A;
   SpluDev dev{};
   DBuf<int> a2lu, a_src, op_loc, op_sl, st_o, flp, fac_mode, crec, lu2s, dmap, fw_lvl, fw_chunk, fw_cols, fw_rows,
-      bw_lvl, bw_chunk, bw_cols, bw_rows, q, pinv, r_hdr, r_blob, r_a2x, r_omap, r_xsrc, sf_hdr, sf_blob, sb_hdr,
+      bw_lvl, bw_chunk, bw_cols, bw_rows, q, pinv, sf_hdr, sf_blob, sb_hdr,
       sb_blob;
-  ResPlan rp;                  // LDS-resident factorisation plan
-  bool rp_ok = false;
+  // LDS-resident factorisation plans, one per factor precision (their
+  // steps are cut at the precision's value staging capacity)
+  struct Res {
+    ResPlan rp;
+    bool ok = false;
+    DBuf<int> hdr, blob, a2x, omap, xsrc;
+  } res[2];   // [0] FP32, [1] FP64
   // LU values per scenario the factorisation kernels use
-  size_t lu_elems() const { return rp_ok ? std::max<size_t>(A.nnz_lu, rp.xtot) : (size_t)A.nnz_lu; }
+  size_t lu_elems() const {
+    size_t e = (size_t)A.nnz_lu;
+    for (const Res &r : res)
+      if (r.ok) e = std::max<size_t>(e, r.rp.xtot);
+    return e;
+  }
   bool ready = false;
   int trip_gen = -1;                              // trip set its pivots were chosen for
   double dt = -1.0;                               // J^T plan: dt of its pivot matrix
@@ -210,17 +220,20 @@ struct LUPlan {
       dev.sol_bw_hdr = sb_hdr.p;
       dev.sol_bw_blob = sb_blob.p;
     }
-    rp_ok = res_plan(A, kResCB * kThreads, kResCX * kThreads, rp);
-    if (rp_ok) {
-      r_hdr.upload(rp.hdr);
-      r_blob.upload(rp.blob);
-      r_a2x.upload(rp.a2x);
-      r_omap.upload(rp.omap);
+    for (int pr = 0; pr < 2; pr++) {
+      Res &R = res[pr];
+      R.ok = res_plan(A, kResCB * kThreads, res_cx(pr ? 8 : 4) * kThreads, R.rp);
+      if (!R.ok) continue;
+      const ResPlan &rp = R.rp;
+      R.hdr.upload(rp.hdr);
+      R.blob.upload(rp.blob);
+      R.a2x.upload(rp.a2x);
+      R.omap.upload(rp.omap);
       // initial value source of each step-ordered slot (four spare slots:
       // the staged loads read whole groups of four)
       std::vector<int> xs((size_t)rp.xtot + 4, kResNoSrc);
       for (int e = 0; e < A.nnz_a; e++) xs[rp.a2x[e]] = asrc[e];
-      r_xsrc.upload(xs);
+      R.xsrc.upload(xs);
     }
     SpluDev &d = dev;
     d.n = A.n; d.nnz_a = A.nnz_a; d.nnz_lu = A.nnz_lu; d.ns = std::max(1, A.ns);
@@ -234,7 +247,7 @@ struct LUPlan {
     d.fw_lvl = fw_lvl.p; d.fw_chunk = fw_chunk.p; d.fw_cols = fw_cols.p; d.fw_rows = fw_rows.p;
     d.bw_lvl = bw_lvl.p; d.bw_chunk = bw_chunk.p; d.bw_cols = bw_cols.p; d.bw_rows = bw_rows.p;
     d.q = q.p; d.pinv = pinv.p;
-    d.res_xsrc = rp_ok ? r_xsrc.p : nullptr;
+    d.res_xsrc = nullptr;   // set by shape() with the precision's plan
     d.threads = kThreads;
     d.gx = gx_needed() ? 1 : 0;
     d.gx_ws = nullptr;   // the context's [n][ldM] workspace, set after the plan is made
@@ -278,19 +291,23 @@ struct LUPlan {
     // LDS-resident factorisation when its multipliers (or the dense block),
     // staging blob and column values fit
     dev.res_nsteps = 0;
-    if (rp_ok) {
+    dev.res_xsrc = nullptr;
+    const Res &R = res[elem == 8 ? 1 : 0];
+    if (R.ok) {
+      const ResPlan &rp = R.rp;
       const int lv = std::max(rp.nL + 1, A.nd * A.nd);
       const size_t need = ((size_t)lv * elem + 15) / 16 * 16 + (size_t)kResCB * kThreads * 4 +
-                          2 * (size_t)kResCX * kThreads * elem;
+                          2 * (size_t)res_cx(elem) * kThreads * elem;
       if (need <= (size_t)kLdsMax) {
         dev.res_nsteps = rp.nsteps();
         dev.res_lv = lv;
         dev.res_nl = rp.nL;
         dev.res_xtot = rp.xtot;
-        dev.res_hdr = r_hdr.p;
-        dev.res_blob = r_blob.p;
-        dev.res_a2x = r_a2x.p;
-        dev.res_omap = r_omap.p;
+        dev.res_hdr = R.hdr.p;
+        dev.res_blob = R.blob.p;
+        dev.res_a2x = R.a2x.p;
+        dev.res_omap = R.omap.p;
+        dev.res_xsrc = R.xsrc.p;
         dev.lds_factor = (int)need;
       }
     }

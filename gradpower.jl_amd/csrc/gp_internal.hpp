@@ -112,6 +112,9 @@ constexpr int kResNoSrc = -2147483647 - 1;
 // staging of the LDS-resident factorisation, per thread of a 512-thread
 // workgroup: blob dwords and initial column values
 constexpr int kResCB = 24, kResCX = 8;
+// value slots per thread of the resident factorisation's step staging: FP32
+// kResCX, FP64 half as many (its multipliers and values take twice the LDS)
+constexpr int res_cx(size_t elem) { return elem == 8 ? kResCX / 2 : kResCX; }
 // staging of the triangular solve, per thread: blob dwords, values, diagonals
 constexpr int kSolCB = 8, kSolCV = 8, kSolCD = 4;
 

@@ -1890,8 +1890,9 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
   TF *Lv = reinterpret_cast<TF *>(smem_raw);
   const size_t lvb = ((size_t)S.res_lv * sizeof(TF) + 15) & ~(size_t)15;
   int *stg = reinterpret_cast<int *>(smem_raw + lvb);
+  constexpr int CX = res_cx(sizeof(TF));
   TF *xbuf = reinterpret_cast<TF *>(smem_raw + lvb + (size_t)kResCB * nt * 4);
-  const int xcap = kResCX * nt;
+  const int xcap = CX * nt;
   TF *lu = luv + (size_t)m * S.res_xtot;
   // the dense copy starts at zero: the external updates write only the
   // block's pattern entries, and the block is read back whole
@@ -1943,14 +1944,14 @@ __global__ void __launch_bounds__(512) k_splu_factor_res(SpluDev S, const double
   }
   __syncthreads();
   // per thread: blob dwords 4 (tid + j nt) .. + 3 (j < kResCB / 4), value
-  // and output-map slots 4 (tid + j nt) .. + 3 (j < kResCX / 4), at clamped
+  // and output-map slots 4 (tid + j nt) .. + 3 (j < CX / 4), at clamped
   // addresses (blobs and value ranges start at multiples of 4 and are
   // followed by at least 4 readable entries); the LDS stores are not masked
   // (slots past a step's length are never read)
-  constexpr int PB = kResCB / 4, PX = kResCX / 4;
+  constexpr int PB = kResCB / 4, PX = CX / 4;
   int pb[kResCB];
-  TF px[kResCX];
-  int po[kResCX];
+  TF px[CX];
+  int po[CX];
   const int nst = S.res_nsteps;
   auto fetch_blob = [&](int st) {
     const int *h = S.res_hdr + kResHdr * st;

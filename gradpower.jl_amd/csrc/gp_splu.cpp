@@ -641,12 +641,71 @@ bool res_plan(const SpluAnalysis &A, int bcap, int xcap, ResPlan &R) {
   R = ResPlan{};
   const int n = A.n, B = n - A.nd;
   const int qb = A.lu_cp[B];
-  // Lv slots: the L entries of the columns below the block; slot nL is zero
-  std::vector<int> lslot(A.nnz_lu, -1);
-  for (int k = 0; k < B; k++)
-    for (int p = A.lu_dp[k] + 1; p < A.lu_cp[k + 1]; p++) lslot[p] = R.nL++;
-  std::vector<int> xstart(n, -1);
   const int nlev = (int)A.flp.size() - 1;
+  // Lv slots: the L entries of the columns below the block, each column's in
+  // one contiguous range, live from the level that writes them to the last
+  // level whose updates read them; ranges are reused once dead (first fit),
+  // which keeps about a third of the L entries in LDS at once.  Slot nL is
+  // zero.  A range is released only for levels after its last reader, so a
+  // column written in level l never overwrites what another column of
+  // level l reads.
+  std::vector<int> lslot(A.nnz_lu, -1);
+  {
+    std::vector<int> lev_of(n, -1), colof(A.nnz_lu, -1), last(B, -1);
+    for (int l = 0; l < nlev; l++)
+      for (int i = A.flp[l]; i < A.flp[l + 1]; i++) lev_of[A.crec[8 * i]] = l;
+    for (int k = 0; k < n; k++)
+      for (int p = A.lu_cp[k]; p < A.lu_cp[k + 1]; p++) colof[p] = k;
+    for (int l = 0; l < nlev; l++)
+      for (int i = A.flp[l]; i < A.flp[l + 1]; i++) {
+        const int *r = &A.crec[8 * i];
+        for (int o = r[4]; o < r[4] + r[5]; o++) {
+          const int j = colof[A.op_sl[o]];
+          if (j >= 0 && j < B) last[j] = std::max(last[j], l);
+        }
+      }
+    std::vector<std::pair<int, int>> freel;   // free ranges (start, length), by start
+    std::vector<std::vector<std::pair<int, int>>> release(nlev + 1);
+    auto give_back = [&](int st, int ln) {
+      auto it = std::lower_bound(freel.begin(), freel.end(), std::make_pair(st, 0));
+      it = freel.insert(it, {st, ln});
+      if (it + 1 != freel.end() && it->first + it->second == (it + 1)->first) {
+        it->second += (it + 1)->second;
+        freel.erase(it + 1);
+      }
+      if (it != freel.begin() && (it - 1)->first + (it - 1)->second == it->first) {
+        (it - 1)->second += it->second;
+        freel.erase(it);
+      }
+    };
+    for (int l = 0; l < nlev; l++) {
+      for (const auto &rg : release[l]) give_back(rg.first, rg.second);
+      for (int i = A.flp[l]; i < A.flp[l + 1]; i++) {
+        const int k = A.crec[8 * i];
+        if (k >= B) continue;
+        const int ln = A.lu_cp[k + 1] - A.lu_dp[k] - 1;
+        if (ln <= 0) continue;
+        int base = -1;
+        for (auto &fr : freel)
+          if (fr.second >= ln) {
+            base = fr.first;
+            fr.first += ln;
+            fr.second -= ln;
+            break;
+          }
+        freel.erase(std::remove_if(freel.begin(), freel.end(), [](const std::pair<int, int> &q) { return q.second == 0; }),
+                    freel.end());
+        if (base < 0) {
+          base = R.nL;
+          R.nL += ln;
+        }
+        for (int p = A.lu_dp[k] + 1; p < A.lu_cp[k + 1]; p++) lslot[p] = base + (p - A.lu_dp[k] - 1);
+        const int e = std::max(last[k], l);   // written in level l even if never read
+        release[std::min(e + 1, nlev)].push_back({base, ln});
+      }
+    }
+  }
+  std::vector<int> xstart(n, -1);
   auto al4 = [](size_t v) { return (v + 3) & ~(size_t)3; };
   // Rows of a column with g lanes: its ops (in program order) packed into
   // rows of at most g, an op going to the first row after every op that
