@@ -343,6 +343,8 @@ def roofline(run, method):
     bytes_per_launch = per_scen * k[which + "_sc"] / n
     achieved = bytes_per_launch / avg_s / 1e9
     name = KERNEL_NAMES[(which, method)]
+    if which == "factor" and method != 2 and run["info"]["res_j" if method == 0 else "res_s"]:
+        name = "k_splu_factor_res<double>"   # FP64 factorisation with its multipliers in LDS
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_" + name.replace("<", "_").replace(">", "") + ".json")
     if os.path.exists(tfile):

@@ -2191,6 +2191,37 @@ __device__ __forceinline__ void sol_step(const int *h, const int *sb, const TF *
   }
 }
 
+// The solve's outputs from x (LDS): xout[q[k]] = x[k] and zupd[zr] +=
+// sign * x[k], zr = scatter[q[k]] (SC) or q[k].
+template <bool SC, bool ZU, bool XO>
+__device__ __forceinline__ void sol_output(const SpluDev &S, const double *x, const int *__restrict__ scatter,
+                                           double *__restrict__ zupd, double *__restrict__ xout, int ldM, int m,
+                                           double sign, int tid, int nt) {
+  const int n = S.n;
+  for (int k0 = 0; k0 < n; k0 += 16 * nt) {
+    int j[16], zr[16];
+    double v[16], z[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int k = min(k0 + tid + u * nt, n - 1);
+      j[u] = S.q[k];
+      v[u] = x[k];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) zr[u] = SC ? scatter[j[u]] : j[u];
+    if (ZU) {
+#pragma unroll
+      for (int u = 0; u < 16; u++) z[u] = zupd[(size_t)zr[u] * ldM + m];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++)
+      if (k0 + tid + u * nt < n) {
+        if (XO) xout[(size_t)j[u] * ldM + m] = v[u];
+        if (ZU) zupd[(size_t)zr[u] * ldM + m] = z[u] + sign * v[u];
+      }
+  }
+}
+
 // One workgroup per scenario: x = A^{-1} b with b gathered from rhs (row
 // space), the solution kept in LDS: forward steps over the sparse part and
 // the block rows' external entries, the dense block's forward and backward
@@ -2226,18 +2257,18 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   TF pv[kSolCV], pd[kSolCD];
   if (nfs) sol_fetch<TF>(S.sol_fw_hdr, S.sol_fw_blob, 0, fv, bd, tid, nt, P, pv, pd);
   {
-    // gather b, eight rows per thread in flight
-    for (int r0 = 0; r0 < n; r0 += 8 * nt) {
-      double b8[8];
-      int p8[8];
+    // gather b, sixteen rows per thread in flight (one round up to 8k rows)
+    for (int r0 = 0; r0 < n; r0 += 16 * nt) {
+      double b8[16];
+      int p8[16];
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
+      for (int u = 0; u < 16; u++) {
         const int r = min(r0 + tid + u * nt, n - 1);
         b8[u] = rhs[(size_t)r * ldM + m];
         p8[u] = S.pinv[r];
       }
 #pragma unroll
-      for (int u = 0; u < 8; u++)
+      for (int u = 0; u < 16; u++)
         if (r0 + tid + u * nt < n) x[p8[u]] = b8[u];
     }
   }
@@ -2317,28 +2348,17 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
       lds_barrier();
     }
   }
-  // outputs, eight rows per thread in flight
-  for (int k0 = 0; k0 < n; k0 += 8 * nt) {
-    int j[8], zr[8];
-    double v[8], z[8];
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int k = min(k0 + tid + u * nt, n - 1);
-      j[u] = S.q[k];
-      v[u] = x[k];
-    }
-#pragma unroll
-    for (int u = 0; u < 8; u++) zr[u] = scatter ? scatter[j[u]] : j[u];
-    if (zupd) {
-#pragma unroll
-      for (int u = 0; u < 8; u++) z[u] = zupd[(size_t)zr[u] * ldM + m];
-    }
-#pragma unroll
-    for (int u = 0; u < 8; u++)
-      if (k0 + tid + u * nt < n) {
-        if (xout) xout[(size_t)j[u] * ldM + m] = v[u];
-        if (zupd) zupd[(size_t)zr[u] * ldM + m] = z[u] + sign * v[u];
-      }
+  // outputs, sixteen rows per thread in flight (the variants without a
+  // scatter table or a state update carry no conditional loads, which
+  // would be waited for one by one)
+  switch ((scatter ? 4 : 0) | (zupd ? 2 : 0) | (xout ? 1 : 0)) {
+    case 7: sol_output<true, true, true>(S, x, scatter, zupd, xout, ldM, m, sign, tid, nt); break;
+    case 6: sol_output<true, true, false>(S, x, scatter, zupd, xout, ldM, m, sign, tid, nt); break;
+    case 5: sol_output<true, false, true>(S, x, scatter, zupd, xout, ldM, m, sign, tid, nt); break;
+    case 3: sol_output<false, true, true>(S, x, scatter, zupd, xout, ldM, m, sign, tid, nt); break;
+    case 2: sol_output<false, true, false>(S, x, scatter, zupd, xout, ldM, m, sign, tid, nt); break;
+    case 1: sol_output<false, false, true>(S, x, scatter, zupd, xout, ldM, m, sign, tid, nt); break;
+    default: break;   // nothing to write
   }
 }
 
