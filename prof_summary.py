@@ -153,8 +153,10 @@ def kernels(tdir, fdir, wdir, copy_bytes, out, cmd):
         fb = fetch.get(k, 0.0) * sf
         wb = write.get(k, 0.0) * sw
         gbs = (fb + wb) / avg if avg > 0 else 0.0   # bytes per ns = GB/s
-        lines.append(f"| `{k}` | {calls} | {tot / 1e6:.2f} | {100 * tot / ktot:.1f} | {avg / 1e3:.2f} | "
-                     f"{fb / 1e6:.3f} | {wb / 1e6:.3f} | {gbs:.1f} | {gbs / HBM_PEAK_GBS:.4f} |")
+        row = f"| `{k}` | {calls} | {tot / 1e6:.2f} | {100 * tot / ktot:.1f} | {avg / 1e3:.2f} |"
+        if fdir != "-":
+            row += f" {fb / 1e6:.3f} | {wb / 1e6:.3f} | {gbs:.1f} | {gbs / HBM_PEAK_GBS:.4f} |"
+        lines.append(row)
     qlines = []
     for q, iv in sorted(queues.items()):
         iv.sort()
@@ -169,14 +171,20 @@ def kernels(tdir, fdir, wdir, copy_bytes, out, cmd):
         busy += ce - cs
         qlines.append(f"| {q} | {len(iv)} | {busy / 1e6:.2f} | {100 * busy / (t1 - t0):.1f} |")
     with open(out, "w") as fh:
-        fh.write("# Per-kernel time and HBM traffic\n\nCommands (from the repository root; the measured launches are "
-                 "those after the marker copy kernel):\n\n```\n" + cmd + "\n```\n\n"
-                 f"Counter scale (copy kernel of {copy_bytes / 1e6:.0f} MB read and written): FETCH_SIZE "
-                 f"{sf:.1f} B per unit, WRITE_SIZE {sw:.1f} B per unit.  GB/s = (fetched + written bytes per launch) "
-                 f"/ average launch duration; peak {HBM_PEAK_GBS:.0f} GB/s.  Measured span "
-                 f"{(t1 - t0) / 1e6:.2f} ms.\n\n"
-                 "| kernel | calls | total ms | share % | avg us | fetched MB/launch | written MB/launch | GB/s | "
-                 "frac of peak |\n|---|---:|---:|---:|---:|---:|---:|---:|---:|\n" + "\n".join(lines) +
+        if fdir == "-":
+            head = (f"# Per-kernel time\n\nCommand (the measured launches are those after the marker copy kernel):"
+                    "\n\n```\n" + cmd + f"\n```\n\nMeasured span {(t1 - t0) / 1e6:.2f} ms.\n\n"
+                    "| kernel | calls | total ms | share % | avg us |\n|---|---:|---:|---:|---:|\n")
+        else:
+            head = ("# Per-kernel time and HBM traffic\n\nCommands (from the repository root; the measured launches "
+                    "are those after the marker copy kernel):\n\n```\n" + cmd + "\n```\n\n"
+                    f"Counter scale (copy kernel of {copy_bytes / 1e6:.0f} MB read and written): FETCH_SIZE "
+                    f"{sf:.1f} B per unit, WRITE_SIZE {sw:.1f} B per unit.  GB/s = (fetched + written bytes per "
+                    f"launch) / average launch duration; peak {HBM_PEAK_GBS:.0f} GB/s.  Measured span "
+                    f"{(t1 - t0) / 1e6:.2f} ms.\n\n"
+                    "| kernel | calls | total ms | share % | avg us | fetched MB/launch | written MB/launch | GB/s | "
+                    "frac of peak |\n|---|---:|---:|---:|---:|---:|---:|---:|---:|\n")
+        fh.write(head + "\n".join(lines) +
                  "\n\nKernel occupancy of each hardware queue over the measured span (the context's stream and the "
                  "asynchronous factorisation's stream):\n\n| queue | launches | busy ms | busy % of span |\n"
                  "|---|---:|---:|---:|\n" + "\n".join(qlines) + "\n")
