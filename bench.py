@@ -41,7 +41,8 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, published
 SOLVE_TIMING_STRIDE = 8
 KERNEL_NAMES = {("solve", 2): "k_splu_solve<float>", ("solve", 1): "k_splu_solve<double>",
                 ("solve", 0): "k_splu_solve<double>", ("factor", 2): "k_splu_factor_res<float>",
-                ("factor", 1): "k_splu_factor<double>", ("factor", 0): "k_splu_factor<double>"}
+                ("factor", 1): "k_splu_factor<double>", ("factor", 0): "k_splu_factor<double>",
+                ("solve", 3): "k_splu_solve<double>", ("factor", 3): "k_splu_factor_res<double>"}
 
 
 def log(*a):
@@ -373,7 +374,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--horizon-cap", type=float, default=0.0,
                     help="cap every scenario horizon (s); profiling runs only, not the benchmark workload")
-    ap.add_argument("--methods", default="schur32,schur64,mono")
+    ap.add_argument("--methods", default="schur32,schur64,mono,shared")
     ap.add_argument("--policy", default="async", choices=["async", "adaptive"],
                     help="refactorisation policy of the Schur methods")
     ap.add_argument("--theta", type=float, default=30.0,
@@ -387,8 +388,8 @@ def main():
                     help="corrections without halving |F| after which an inexact solve restarts as exact FP64 "
                          "Newton (0: the library's default, 8; -1: at itermax only)")
     ap.add_argument("--predictor", type=int, default=0,
-                    help="first iterate of a step under the Schur methods' policies: 0 linear extrapolation, "
-                         "2 quadratic, -1 none")
+                    help="first iterate of a step: 0 z_k (the reference's start), 1 linear extrapolation, "
+                         "2 quadratic (under the Schur methods' policies)")
     ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete-adjoint measurement")
     ap.add_argument("--no-c4", action="store_true", help="skip the 70,000-bus M=8 measurement")
@@ -401,8 +402,8 @@ def main():
         sys.exit(2)
     from gradpower_hip import synth
     from gradpower_hip.desc import gen_speed_indices
-    from gradpower_hip.lib import (METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32, REFACTOR_ADAPTIVE,
-                                   REFACTOR_ASYNC, REFACTOR_EVERY_ITER)
+    from gradpower_hip.lib import (METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32, METHOD_SCHUR_SHARED,
+                                   REFACTOR_ADAPTIVE, REFACTOR_ASYNC, REFACTOR_EVERY_ITER)
 
     t0 = time.perf_counter()
     case = synth.case2_grid(M=args.M, nbus=args.nbus)
@@ -425,7 +426,8 @@ def main():
     methods = args.methods.split(",")
     runs, mcode = {}, {}
     for name, meth, pol in (("schur32", METHOD_SCHUR_FP32, spol), ("schur64", METHOD_SCHUR_FP64, spol),
-                            ("mono", METHOD_MONO_FP64, REFACTOR_EVERY_ITER)):
+                            ("mono", METHOD_MONO_FP64, REFACTOR_EVERY_ITER),
+                            ("shared", METHOD_SCHUR_SHARED, spol)):
         if name not in methods:
             continue
         log(f"method {name}")
@@ -490,6 +492,12 @@ def main():
         out["schur32"]["rotor_speed_err_vs_fp64_pu"] = err
         out["schur32"]["rotor_speed_err_scenarios"] = len(ok)
         out["schur32"]["extra_corrections_vs_fp64"] = (a["stats"]["newton_iters"] - b["stats"]["newton_iters"])
+    if "shared" in runs and "schur64" in runs:
+        a, b = runs["shared"], runs["schur64"]
+        ok = [m for m in range(case.M) if a["status"][m] == 0 and b["status"][m] == 0]
+        err = max((float(np.max(np.abs(a["speeds"][m] - b["speeds"][m]))) for m in ok), default=None)
+        out["shared"]["rotor_speed_err_vs_fp64_pu"] = err
+        out["shared"]["rotor_speed_err_scenarios"] = len(ok)
     if "schur32" in v and "mono" in v:
         out["speedup_vs_mono_fp64"] = v["schur32"] / v["mono"]
     if cpu is not None:

@@ -530,6 +530,26 @@ struct gp_ctx {
   hipEvent_t ev_fork = nullptr, ev_side = nullptr, ev_fork2 = nullptr, ev_join = nullptr;
   hipEvent_t ev_round = nullptr;   // end of a round's exchange, polled by the host
   bool has64 = false;   // luS64 fits the LU kernels' LDS
+  // shared-factor method (GP_METHOD_SCHUR_SHARED): the one FP64 factorisation
+  // P of the reduced network (slot 0 of lu/sv/dn), the Woodbury columns
+  // W = P^{-1} U of every faulted bus (U: unit columns at the bus's vr and vi
+  // rows; W [column][nred] in the solve's pivot order), and per scenario the
+  // columns, rows and inverse capacitance matrix of its faults in force
+  struct Shared {
+    DBuf<double> lu, sv, dn, W, h;
+    DBuf<int> nc, col, pos;
+    std::map<int, int> col_of_bus;   // faulted bus -> its first column (vr; vi next)
+    std::vector<double> G;           // U^T W: G[i*ncol + j] = W[row of column i][j]
+    std::vector<int> upos;           // pivot-order position of each column's row
+    int ncol = 0, kw = 2;            // columns; per-scenario table width (2 x events in force)
+    std::vector<int> hnc, hcol, hpos;
+    std::vector<double> hh;
+    WbTab tab() const {
+      WbTab t;
+      t.nc = nc.p; t.col = col.p; t.pos = pos.p; t.h = h.p; t.W = W.p; t.kw = kw;
+      return t;
+    }
+  } sh;
   DBuf<uint8_t> d_split;   // a round's masks split by factor precision [6][L]
   // discrete adjoint: LU plan of J^T (values read from d_jv), its factors,
   // the initial state of the last gp_set_state, checkpoints, the states of
@@ -1160,6 +1180,77 @@ void ensure_schur_buffers(gp_ctx *C, bool fp32) {
   }
 }
 
+// Shared-factor method: scenario m's Woodbury table row from its event slots
+// (fbus/fg [MAXFAULT][ldM]: bus and 1/r_fault, bus < 0 free).  The faults
+// of a bus add -g_b to the bus's two diagonal entries of S, so that
+// S = P + U D U^T with D = diag(-g_b) and, by Woodbury,
+//   S^{-1} b = y - W H^{-1} U^T y,  y = P^{-1} b,  H = D^{-1} + U^T W
+// (_precompute_woodbury_multi_gpu!, ext:2441-2495: H = -r I + U^T W).
+void shared_set(gp_ctx *C, int m, const int *fbus, const double *fg) {
+  gp_ctx::Shared &W = C->sh;
+  const int L = C->ldM;
+  std::vector<std::pair<int, double>> gb;   // faulted bus, summed admittance
+  for (int q = 0; q < MAXFAULT; q++) {
+    const int b = fbus[(size_t)q * L + m];
+    if (b < 0) continue;
+    auto it = std::find_if(gb.begin(), gb.end(), [b](const std::pair<int, double> &x) { return x.first == b; });
+    if (it == gb.end())
+      gb.emplace_back(b, fg[(size_t)q * L + m]);
+    else
+      it->second += fg[(size_t)q * L + m];
+  }
+  const int nc = 2 * (int)gb.size();
+  if (nc > W.kw) throw GpError{GP_ERR_HIP, "internal: Woodbury table too narrow"};
+  int cols[2 * MAXFAULT];
+  double dinv[2 * MAXFAULT], H[2 * MAXFAULT][4 * MAXFAULT];
+  for (int i = 0; i < (int)gb.size(); i++) {
+    auto it = W.col_of_bus.find(gb[i].first);
+    if (it == W.col_of_bus.end()) throw GpError{GP_ERR_HIP, "internal: faulted bus without Woodbury columns"};
+    cols[2 * i] = it->second;
+    cols[2 * i + 1] = it->second + 1;
+    dinv[2 * i] = dinv[2 * i + 1] = -1.0 / gb[i].second;
+  }
+  // H | I, inverted by Gauss-Jordan elimination with partial pivoting
+  for (int i = 0; i < nc; i++)
+    for (int j = 0; j < 2 * nc; j++)
+      H[i][j] = j < nc ? W.G[(size_t)cols[i] * W.ncol + cols[j]] + (i == j ? dinv[i] : 0.0)
+                       : (j - nc == i ? 1.0 : 0.0);
+  for (int p = 0; p < nc; p++) {
+    int r = p;
+    for (int i = p + 1; i < nc; i++)
+      if (std::fabs(H[i][p]) > std::fabs(H[r][p])) r = i;
+    if (!(std::fabs(H[r][p]) > 0.0) || !std::isfinite(H[r][p]))
+      throw GpError{GP_ERR_SINGULAR_PIVOT, "shared factorisation: singular Woodbury capacitance matrix (scenario " +
+                                               std::to_string(m) + ")"};
+    if (r != p)
+      for (int j = 0; j < 2 * nc; j++) std::swap(H[p][j], H[r][j]);
+    const double inv = 1.0 / H[p][p];
+    for (int j = 0; j < 2 * nc; j++) H[p][j] *= inv;
+    for (int i = 0; i < nc; i++) {
+      if (i == p || H[i][p] == 0.0) continue;
+      const double f = H[i][p];
+      for (int j = 0; j < 2 * nc; j++) H[i][j] -= f * H[p][j];
+    }
+  }
+  const int kw = W.kw;
+  W.hnc[m] = nc;
+  for (int c = 0; c < kw; c++) {
+    W.hcol[(size_t)m * kw + c] = c < nc ? cols[c] : 0;
+    W.hpos[(size_t)m * kw + c] = c < nc ? W.upos[cols[c]] : 0;
+    for (int j = 0; j < kw; j++) W.hh[((size_t)m * kw + c) * kw + j] = (c < nc && j < nc) ? H[c][nc + j] : 0.0;
+  }
+}
+// the tables on the device, in stream order (the host rows stay valid until
+// the stream has passed the copies: callers change them only after a later
+// synchronisation)
+void shared_upload(gp_ctx *C) {
+  gp_ctx::Shared &W = C->sh;
+  HIPCHK(hipMemcpyAsync(W.nc.p, W.hnc.data(), W.hnc.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
+  HIPCHK(hipMemcpyAsync(W.col.p, W.hcol.data(), W.hcol.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
+  HIPCHK(hipMemcpyAsync(W.pos.p, W.hpos.data(), W.hpos.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
+  HIPCHK(hipMemcpyAsync(W.h.p, W.hh.data(), W.hh.size() * sizeof(double), hipMemcpyHostToDevice, C->stream));
+}
+
 // Events around one launch of a timed kernel (gp_options.profile).  A
 // launch whose scenario count is not known when it goes out gets it later
 // (prof_count); launches that worked on no scenario are left out.
@@ -1347,6 +1438,7 @@ struct Newton {
   int method, itermax, stall;
   double dt, tol, theta, async_below;
   bool schur, reuse, async, fp32;
+  bool shared;                   // one shared factorisation with Woodbury corrections: nothing factorises
   bool lagged;                   // a factorisation may serve several iterations
   bool all64 = false;            // FP32 method running exact Newton: every factor in FP64
   const double *dtv = nullptr;   // per-scenario step on the device (NULL: dt for all)
@@ -1361,8 +1453,9 @@ struct Newton {
       : C(c), method(meth), itermax(itmax), dt(dt_), tol(tol_) {
     const int pol = C->opt.refactor_policy;
     schur = method != GP_METHOD_MONO_FP64;
-    reuse = has_refac0 && schur && (pol == GP_REFACTOR_ADAPTIVE || pol == GP_REFACTOR_ASYNC);
-    async = schur && pol == GP_REFACTOR_ASYNC;
+    shared = method == GP_METHOD_SCHUR_SHARED;
+    reuse = has_refac0 && schur && !shared && (pol == GP_REFACTOR_ADAPTIVE || pol == GP_REFACTOR_ASYNC);
+    async = schur && !shared && pol == GP_REFACTOR_ASYNC;
     stall = C->opt.stall_window;
     // refactor policy: theta <= 0 refactors every iteration; otherwise after
     // the first iteration only when |F| fell by less than theta (FIRST_ITER:
@@ -1372,6 +1465,7 @@ struct Newton {
                 : ((pol == GP_REFACTOR_ADAPTIVE || pol == GP_REFACTOR_ASYNC)
                        ? (C->opt.refactor_theta > 0.0 ? C->opt.refactor_theta : 30.0)
                        : 1e-300);
+    if (shared) theta = -1.0;   // one factorisation, made before the loop: no requests
     lagged = theta > 0.0;
     fp32 = method == GP_METHOD_SCHUR_FP32;
     switch_window = C->opt.retry_window < 0 ? 0 : (C->opt.retry_window > 0 ? C->opt.retry_window : 8);
@@ -1384,13 +1478,14 @@ struct Newton {
   // factorisation at every iteration
   void make_exact() {
     theta = 0.0;
+    shared = false;   // (the shared method's exact Newton is the FP64 Schur method's)
     reuse = async = false;
     all64 = fp32 && C->has64;
     switch_window = 0;
   }
   // A failed solve runs again as exact Newton: when factorisations are
   // reused, or are FP32 (the reference factors in FP64 at every iteration).
-  bool retryable() const { return lagged || (fp32 && C->has64); }
+  bool retryable() const { return shared || lagged || (fp32 && C->has64); }
   // corrections without halving |F| after which an inexact solve is given
   // up for exact Newton (0: only at itermax)
   int switch_window = 0;
@@ -1460,6 +1555,7 @@ struct Newton {
   int prof_f = -1;
   bool landed = false, skip = false;
   bool mixed = false, run32 = false, run64 = false;
+  bool shmix = false;   // shared method: some scenarios run exact Newton on their own factors
   const uint8_t *upd32 = nullptr, *upd64 = nullptr;
   void launch_pre(const int *cnt, bool land_first, const std::vector<uint8_t> *fresh) {
     const int L = C->ldM;
@@ -1473,7 +1569,7 @@ struct Newton {
     // with the counts unknown: under ASYNC only a solve that starts afresh
     // (an event or a change of dt: land_first) factorises in the round
     // itself, every other request goes to a lane
-    const bool fac = known ? cnt[1] > 0 : (!async || land_first || C->opt.async_below > 0.0);
+    const bool fac = !shared && (known ? cnt[1] > 0 : (!async || land_first || C->opt.async_below > 0.0));
     fac_done = fac;
     if (method == GP_METHOD_MONO_FP64) {
       launch_jac_mono(C->T, p, z, dt, faults(C), C->d_jv.p, C->d_upd.p, C->stream, dtv);
@@ -1481,6 +1577,36 @@ struct Newton {
       launch_splu_factor_impl<double>(C->luJ.dev, C->d_jv.p, nullptr, L, C->d_luJ.p, C->d_svJ.p, C->d_dnJ.p,
                                       C->d_refac.p, nullptr, C->M, C->d_status.p, C->stream);
       prof_end(C->prof_factor, prof_f, C->stream, known ? cnt[1] : 0);
+      return;
+    }
+    if (shared) {
+      // the shared factorisation serves every scenario except those running
+      // exact Newton (exacth), which factorise in FP64 at every iteration
+      // (the synchronous factorisation, launched here whatever the counts:
+      // an empty mask returns at once)
+      fac_done = true;
+      shmix = false;
+      if (exacth)
+        for (int m = 0; m < L && !shmix; m++) shmix = exacth[m] && (!workh || workh[m]);
+      run32 = false;
+      run64 = true;
+      upd32 = upd64 = C->d_upd.p;
+      const uint8_t *ref64 = C->d_refac.p;
+      if (shmix) {
+        launch_split_precision(L, exactv, C->d_sch.p, C->d_refac.p, C->d_upd.p, C->d_split.p, C->stream);
+        upd32 = C->d_split.p + 2 * L;
+        ref64 = C->d_split.p + 4 * L;
+        upd64 = C->d_split.p + 5 * L;
+      }
+      launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, C->d_sch.p, C->stream, dtv);
+      launch_bus_schur(C->T, p, z, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_sch.p, C->stream);
+      if (shmix) {
+        prof_f = prof_begin(C, C->prof_factor, C->stream);
+        launch_splu_factor_impl<double>(C->luS64, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
+                                        C->d_dnS64.p, ref64, C->d_cur.p, C->M, C->d_status.p, C->stream);
+        prof_end(C->prof_factor, prof_f, C->stream, known ? cnt[1] : 0);
+      }
+      HIPCHK(hipGetLastError());
       return;
     }
     // factor precision per scenario: FP32 for the FP32 method, except the
@@ -1591,7 +1717,7 @@ struct Newton {
     // a synchronous request launch_pre did not expect (a residual that is
     // not finite under exact Newton, whose request is not sent to a lane):
     // its factorisation goes out now, still before the solve
-    if (cnt[1] > 0 && !fac_done) {
+    if (cnt[1] > 0 && !fac_done && !shared) {
       sync_factor(cnt[1]);
       fac_done = true;
     }
@@ -1605,13 +1731,23 @@ struct Newton {
       if (run32)
         launch_splu_solve<float>(C->luS.dev, C->d_svS32.p, C->d_dnS32.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
                                  C->d_dv.p, upd32, C->d_cur.p, C->M, C->stream);
-      if (run64 && !mixed)
+      if (shared) {
+        const WbTab wb = C->sh.tab();
+        launch_splu_solve<double>(C->luS64, C->sh.sv.p, C->sh.dn.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
+                                  C->d_dv.p, upd32, nullptr, C->M, C->stream, true, &wb);
+        if (shmix)
+          launch_splu_solve<double>(C->luS64, C->d_svS64.p, C->d_dnS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
+                                    C->d_dv.p, upd64, C->d_cur.p, C->M, C->stream);
+      } else if (run64 && !mixed) {
         launch_splu_solve<double>(C->luS64, C->d_svS64.p, C->d_dnS64.p, C->d_rhs.p, L, 1.0, C->d_z_of_red.p, z,
                                   C->d_dv.p, upd64, C->d_cur.p, C->M, C->stream);
+      }
       prof_end(C->prof_solve, ps, C->stream, cnt[2]);
+      if (shared)
+        launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, C->d_upd.p, C->stream);
       if (run32)
         launch_cluster_backsub<float>(C->T, C->d_X32.p, C->d_f.p, C->d_dv.p, z, upd32, C->stream);
-      if (run64 && !mixed)
+      if (run64 && !mixed && !shared)
         launch_cluster_backsub<double>(C->T, C->d_X64.p, C->d_f.p, C->d_dv.p, z, upd64, C->stream);
     }
     start_queue(lane);
@@ -1993,6 +2129,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
   std::vector<double> dtv(L, dt);
   std::vector<int> fbus((size_t)MAXFAULT * L, -1);
   std::vector<double> fg((size_t)MAXFAULT * L, 0.0);
+  const bool shared = method == GP_METHOD_SCHUR_SHARED;
   auto set_faults = [&](int m) {
     if ((int)active_ev[m].size() > MAXFAULT)
       throw GpError{GP_ERR_UNSUPPORTED, "more than " + std::to_string(MAXFAULT) + " simultaneous faults, line trips or load drops in one scenario"};
@@ -2001,6 +2138,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
       fbus[(size_t)q * L + m] = f_on ? active_ev[m][q]->bus : -1;
       fg[(size_t)q * L + m] = f_on ? 1.0 / active_ev[m][q]->r_fault : 0.0;
     }
+    if (shared) shared_set(C, m, fbus.data(), fg.data());
   };
   int kmax = 0;
   for (int m = 0; m < M; m++) {
@@ -2028,7 +2166,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
   N.exacth = exact.data();
   // first iterate of a step by extrapolation (gp_options.predictor) under
   // the policies that reuse factorisations
-  const bool extrapolate = C->opt.predictor >= 0 && N.reuse;
+  const bool extrapolate = (C->opt.predictor == 1 || C->opt.predictor == 2) && N.reuse;
   // quadratic extrapolation (predictor 2) after two smooth steps: z_(k-2)
   // is kept in zold2; nplain counts the accepted steps since the start or
   // since the last event's re-solve
@@ -2079,6 +2217,7 @@ gp_stats integrate_free(gp_ctx *C, int method, double dt, double tol, int iterma
     if (fault_dirty) {
       HIPCHK(hipMemcpyAsync(C->d_fbus.p, fbus.data(), fbus.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
       HIPCHK(hipMemcpyAsync(C->d_fg.p, fg.data(), fg.size() * sizeof(double), hipMemcpyHostToDevice, C->stream));
+      if (shared) shared_upload(C);
       fault_dirty = false;
     }
     N.eval(0, C->d_first.p, C->d_refac0.p, nullptr, true);
@@ -2748,6 +2887,108 @@ static int max_in_force(const std::vector<gp_event> &ev, int M) {
 
 static bool events_fit(const std::vector<gp_event> &ev, int M) { return max_in_force(ev, M) <= MAXFAULT; }
 
+// Shared-factor method, set-up of one call (_factor_shared_P_gpu! and
+// _precompute_woodbury_multi_gpu!, ext:2149-2173, 2441-2495): P = S of
+// scenario 0 at its current state, step dt, no event in force, factorised
+// once in FP64; W = P^{-1} U for the two unit columns of every bus that
+// one of the events faults, by one solve with the columns as its
+// scenarios; U^T W on the host for the per-scenario capacitance matrices.
+// Every scenario's table row starts with no fault in force.
+static void prepare_shared(gp_ctx *C, double dt, int in_force = -1) {
+  if (!C->trips.empty() || !C->drops.empty())
+    throw GpError{GP_ERR_UNSUPPORTED, "shared-factor method: line trips and load drops change P itself"};
+  if (C->opt.lock_step) throw GpError{GP_ERR_UNSUPPORTED, "shared-factor method: not in lock step"};
+  const SpluDev &D = C->luS64;
+  if (D.gx) throw GpError{GP_ERR_UNSUPPORTED, "shared-factor method: the reduced network's solution exceeds LDS"};
+  gp_ctx::Shared &W = C->sh;
+  const int L = C->ldM, nr = C->nred;
+  const std::vector<int> fb0((size_t)MAXFAULT * L, -1);
+  const std::vector<double> fg0((size_t)MAXFAULT * L, 0.0);
+  HIPCHK(hipMemcpyAsync(C->d_fbus.p, fb0.data(), fb0.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
+  HIPCHK(hipMemcpyAsync(C->d_fg.p, fg0.data(), fg0.size() * sizeof(double), hipMemcpyHostToDevice, C->stream));
+  std::vector<uint8_t> sel(L, 0);
+  sel[0] = 1;
+  set_mask(C, C->d_sel, sel);
+  launch_copy_active(C->n, L, C->d_z.p, C->d_zold.p, C->d_sel.p, C->stream);
+  launch_cluster_residual(C->T, C->d_p.p, C->d_z.p, C->d_zold.p, dt, C->d_f.p, C->d_inj.p, C->d_sel.p, C->stream);
+  launch_bus_residual(C->T, C->d_p.p, C->d_z.p, C->d_inj.p, faults(C), C->d_f.p, C->d_sel.p, C->stream);
+  launch_cluster_schur<double>(C->T, C->d_p.p, C->d_z.p, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, C->d_sel.p, C->stream);
+  launch_bus_schur(C->T, C->d_p.p, C->d_z.p, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_sel.p,
+                   C->stream);
+  W.lu.ensure(std::max<size_t>(1, C->luS.lu_elems()));
+  if (W.sv.n != (size_t)D.ns) {
+    W.sv.alloc(D.ns);
+    HIPCHK(hipMemset(W.sv.p, 0, W.sv.n * sizeof(double)));   // padding slots stay zero
+  }
+  W.dn.ensure((size_t)std::max(1, D.nd * D.nd));
+  HIPCHK(hipMemsetAsync(C->d_status.p, 0, sizeof(int), C->stream));
+  launch_splu_factor_impl<double>(D, C->d_svar.p, C->d_s_cst.p, L, W.lu.p, W.sv.p, W.dn.p, C->d_sel.p, nullptr, 1,
+                                  C->d_status.p, C->stream);
+  HIPCHK(hipGetLastError());
+  int st0 = 0;
+  HIPCHK(hipMemcpyAsync(&st0, C->d_status.p, sizeof(int), hipMemcpyDeviceToHost, C->stream));
+  HIPCHK(hipStreamSynchronize(C->stream));
+  if (st0) throw GpError{GP_ERR_SINGULAR_PIVOT, "shared factorisation: zero or non-finite pivot in P"};
+  // Woodbury columns of the faulted buses
+  std::vector<int> buses;
+  for (const gp_event &e : C->events)
+    if (e.bus >= 0) buses.push_back(e.bus);
+  std::sort(buses.begin(), buses.end());
+  buses.erase(std::unique(buses.begin(), buses.end()), buses.end());
+  const int nb = (int)buses.size();
+  W.ncol = 2 * nb;
+  W.kw = 2 * std::max(1, std::min(MAXFAULT, in_force >= 0 ? in_force : max_in_force(C->events, C->M)));
+  W.col_of_bus.clear();
+  W.G.clear();
+  W.upos.assign(W.ncol, 0);
+  if (nb > 0) {
+    const int ldw = (W.ncol + 63) / 64 * 64;
+    std::vector<int> urow(W.ncol);
+    for (int b = 0; b < nb; b++) {
+      W.col_of_bus[buses[b]] = 2 * b;
+      for (int t = 0; t < 2; t++) urow[2 * b + t] = C->red_of_z[C->net_ptr + 2 * buses[b] + t];
+    }
+    std::vector<double> E((size_t)nr * ldw, 0.0);
+    for (int c = 0; c < W.ncol; c++) E[(size_t)urow[c] * ldw + c] = 1.0;
+    std::vector<uint8_t> on(ldw, 0);
+    std::fill(on.begin(), on.begin() + W.ncol, 1);
+    DBuf<double> dE, dWn;
+    DBuf<uint8_t> dOn;
+    dE.upload(E);
+    dOn.upload(on);
+    dWn.alloc((size_t)nr * ldw);
+    launch_splu_solve<double>(D, W.sv.p, W.dn.p, dE.p, ldw, 1.0, nullptr, nullptr, dWn.p, dOn.p, nullptr, W.ncol,
+                              C->stream, true);
+    HIPCHK(hipGetLastError());
+    std::vector<double> Wn((size_t)nr * ldw);
+    HIPCHK(hipMemcpyAsync(Wn.data(), dWn.p, Wn.size() * sizeof(double), hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    const SpluAnalysis &A = C->luS.A;
+    std::vector<double> Wi((size_t)W.ncol * nr);
+    for (int c = 0; c < W.ncol; c++)
+      for (int k = 0; k < nr; k++) Wi[(size_t)c * nr + k] = Wn[(size_t)A.q[k] * ldw + c];
+    W.W.upload(Wi);
+    W.G.assign((size_t)W.ncol * W.ncol, 0.0);
+    for (int i = 0; i < W.ncol; i++) {
+      W.upos[i] = A.qinv[urow[i]];
+      for (int j = 0; j < W.ncol; j++) W.G[(size_t)i * W.ncol + j] = Wn[(size_t)urow[i] * ldw + j];
+    }
+  } else {
+    W.W.ensure(1);
+  }
+  const int kw = W.kw;
+  W.hnc.assign(L, 0);
+  W.hcol.assign((size_t)L * kw, 0);
+  W.hpos.assign((size_t)L * kw, 0);
+  W.hh.assign((size_t)L * kw * kw, 0.0);
+  W.nc.ensure(L);
+  W.col.ensure((size_t)L * kw);
+  W.pos.ensure((size_t)L * kw);
+  W.h.ensure((size_t)L * kw * kw);
+  shared_upload(C);
+  HIPCHK(hipStreamSynchronize(C->stream));
+}
+
 static void rebuild_events(gp_ctx *C) { C->events = event_list(C->faults_user, C->trips, C->drops); }
 
 int gp_set_events(gp_ctx *C, const gp_event *ev, int32_t n_ev, const int32_t *nsteps) {
@@ -2900,16 +3141,20 @@ int gp_set_line_trips(gp_ctx *C, const gp_line_trip *tr, int32_t n) {
 
 int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t itermax, gp_stats *st) {
   if (!C) return fail(GP_ERR_BADARG, "null context");
-  if (method < 0 || method > 2) return fail(GP_ERR_BADARG, "unknown method");
+  if (method < 0 || method > 3) return fail(GP_ERR_BADARG, "unknown method");
   if (!(dt > 0.0) || !(tol > 0.0) || itermax <= 0) return fail(GP_ERR_BADARG, "dt, tol and itermax must be > 0");
   if (!C->state_set) return fail(GP_ERR_BADARG, "gp_set_state was not called");
   gp_stats S{};
   try {
     const int L = C->ldM, M = C->M;
+    const bool shared = method == GP_METHOD_SCHUR_SHARED;
     int maxsteps = 0;
     for (int m = 0; m < M; m++) maxsteps = std::max(maxsteps, C->nsteps[m]);
+    if (shared && (C->opt.lock_step || !C->trips.empty() || !C->drops.empty()))
+      throw GpError{GP_ERR_UNSUPPORTED, "shared-factor method: not in lock step, nor with line trips or load drops"};
     Stepper sp(C, method, dt, tol, itermax);   // no fault active at t = 0
     prepare_method(C, method);
+    if (shared) prepare_shared(C, dt);
     // trajectory storage [step][channel][ldM]
     C->d_hist.alloc((size_t)(maxsteps + 1) * C->nrec * L);
     C->hist_steps = maxsteps + 1;
@@ -2929,7 +3174,7 @@ int gp_integrate(gp_ctx *C, int32_t method, double dt, double tol, int32_t iterm
       F.k_first.assign(M, 1);
       F.k_last.assign(C->nsteps.begin(), C->nsteps.begin() + M);
       std::vector<RepivotReq> reqs;
-      F.repivot = &reqs;
+      F.repivot = shared ? nullptr : &reqs;   // (the shared method factorises nothing in the loop)
       S = integrate_free(C, method, dt, tol, itermax, F);
       if (!reqs.empty()) repivot_runs(C, method, dt, tol, itermax, reqs, S);
     }
@@ -2964,6 +3209,8 @@ int gp_adjoint(gp_ctx *C, int32_t method, double dt, double tol, int32_t itermax
                const gp_objective *obj, int32_t n_params, const int32_t *params, double *value, double *grad,
                double *grad_scen, double *dz0, gp_adjoint_stats *ast) {
   if (!C) return fail(GP_ERR_BADARG, "null context");
+  if (method == GP_METHOD_SCHUR_SHARED)
+    return fail(GP_ERR_UNSUPPORTED, "gp_adjoint: the shared-factor method has no adjoint (use a Schur method)");
   if (method < 0 || method > 2) return fail(GP_ERR_BADARG, "unknown method");
   if (!(dt > 0.0) || !(tol > 0.0) || itermax <= 0) return fail(GP_ERR_BADARG, "dt, tol and itermax must be > 0");
   if (!C->state_set) return fail(GP_ERR_BADARG, "gp_set_state was not called");
@@ -3372,7 +3619,7 @@ int gp_eval_jacobian(gp_ctx *C, const double *z, double dt, const uint8_t *fault
 int gp_newton_step(gp_ctx *C, int32_t method, const double *z, const double *zold, double dt,
                    const uint8_t *fault_on, double *dz) {
   if (!C || !z || !zold || !dz) return fail(GP_ERR_BADARG, "gp_newton_step: null argument");
-  if (method < 0 || method > 2) return fail(GP_ERR_BADARG, "unknown method");
+  if (method < 0 || method > 3) return fail(GP_ERR_BADARG, "unknown method");
   if (!C->state_set) return fail(GP_ERR_BADARG, "gp_set_state was not called");
   try {
     const int L = C->ldM;
@@ -3381,11 +3628,26 @@ int gp_newton_step(gp_ctx *C, int32_t method, const double *z, const double *zol
     load_eval_state(C, z, nullptr);
     analyse(C, method);
     if (method != GP_METHOD_MONO_FP64) ensure_schur_buffers(C, method == GP_METHOD_SCHUR_FP32);
+    // shared: P of scenario 0 at z with step dt
+    const bool shared = method == GP_METHOD_SCHUR_SHARED;
+    if (shared) {   // (the parity hook puts each scenario's first events in force at once)
+      std::vector<int> per(C->M, 0);
+      int most = 0;
+      for (const gp_event &e : C->events) most = std::max(most, ++per[e.scenario]);
+      prepare_shared(C, dt, most);
+    }
     std::vector<double> zt((size_t)C->n * L, 0.0);
     for (int m = 0; m < C->M; m++)
       for (int i = 0; i < C->n; i++) zt[(size_t)i * L + m] = zold[(size_t)m * C->n + i];
     HIPCHK(hipMemcpy(C->d_zold.p, zt.data(), zt.size() * sizeof(double), hipMemcpyHostToDevice));
     load_eval_state(C, z, fault_on);
+    if (shared) {   // the Woodbury rows of the faults in force, as load_eval_state set them
+      std::vector<int> fbus;
+      std::vector<double> fg;
+      eval_fault_tables(C, fault_on, false, fbus, fg);
+      for (int m = 0; m < C->M; m++) shared_set(C, m, fbus.data(), fg.data());
+      shared_upload(C);
+    }
     std::vector<uint8_t> all(L, 0);
     for (int m = 0; m < C->M; m++) all[m] = 1;
     (void)newton(C, method, dt, -1.0, 1, all);   // one correction, nothing counts as converged

@@ -68,6 +68,17 @@ struct FaultState {
   double *g;
 };
 
+// Woodbury correction of a shared factorisation P (one per scenario, applied
+// by the solve before its outputs): x -= W_c * s with s = H^{-1} x[pos_c],
+// over the nc[m] columns c of scenario m.  W: [column][n] in the solve's
+// internal (pivot) order; col/pos: [m][kw]; h: [m][kw][kw] row-major.
+// nc == NULL: no correction.
+struct WbTab {
+  const int *nc = nullptr, *col = nullptr, *pos = nullptr;
+  const double *h = nullptr, *W = nullptr;
+  int kw = 0;
+};
+
 struct SpluDev {
   int n, nnz_a, nnz_lu, ns, n_flev, nd, qb, nnz_top;
   const int *a2lu, *a_src;      // per A entry: LU position, value source
@@ -160,7 +171,8 @@ void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *
 template <typename TF>
 void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const double *rhs, int ldM, double sign,
                        const int *scatter, double *zupd, double *xout, const uint8_t *work, const uint8_t *cur, int M,
-                       hipStream_t s);
+                       hipStream_t s, bool shared = false,   // shared: slot 0's factors serve every scenario
+                       const WbTab *wb = nullptr);           // with a Woodbury correction per scenario
 void launch_copy_active(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s);
 // dst[m*n + r] = src[r*ldM + m] for the scenarios in act
 void launch_copy_transpose(int n, int ldM, const double *src, double *dst, const uint8_t *act, hipStream_t s);

@@ -526,6 +526,10 @@ __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict_
     nstall[m] = ns;
     fail = fail || !(v <= 1e10) || ns >= stall;
   }
+  // on the shared factorisation (theta < 0) a correction that raises |F|
+  // gives the solve up: it runs again as exact Newton from the step's start
+  // (an inexact iteration that wanders may settle on another root)
+  if (theta < 0.0 && !first && !(v <= prev)) fail = true;
   if (fail) {
     work[m] = 0;
     failed[m] = isfinite(v) ? 1 : 3;   // bit 1: |F| not finite
@@ -534,9 +538,11 @@ __global__ void k_norm_finish(int M, double tol, unsigned long long *__restrict_
   atomicAdd(n_unconv, 1);
   const bool pending = pend && pend[m];
   // a scenario that waited last round compares against an unchanged |F|:
-  // its new factorisation has just arrived, so it asks for none
-  const int rf = first ? (refac0 ? refac0[m] : 1)
-                       : ((pending || !was_upd) ? 0 : ((theta <= 0.0 || !(v * theta < prev)) ? 1 : 0));
+  // its new factorisation has just arrived, so it asks for none.  theta < 0:
+  // one shared factorisation serves the scenario (only exact Newton factorises)
+  const int rf = theta < 0.0 ? 0
+                 : first ? (refac0 ? refac0[m] : 1)
+                         : ((pending || !was_upd) ? 0 : ((theta <= 0.0 || !(v * theta < prev)) ? 1 : 0));
   // a solve that starts with a factorisation of its own (after an event or
   // a change of dt) does not wait for one asked for under the old system:
   // the host lands and drops that request before this round's factorisation
@@ -2233,13 +2239,14 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
                                                     const double *__restrict__ rhs, int ldM, double sign,
                                                     const int *__restrict__ scatter, double *__restrict__ zupd,
                                                     double *__restrict__ xout, const uint8_t *__restrict__ work,
-                                                    const uint8_t *__restrict__ cur) {
+                                                    const uint8_t *__restrict__ cur, int shared, WbTab wb) {
   const int m = scenario_of_block(blockIdx.x, gridDim.x);
   if (!work[m]) return;
   extern __shared__ __align__(16) char smem_raw[];
   const int n = S.n, nd = S.nd, B = n - nd;
   const int Mb = gridDim.x;
   const int bsel = cur ? cur[m] : 0;
+  const int fm = shared ? 0 : m;   // shared: one factorisation (slot 0) serves every scenario
   const int tid = threadIdx.x, nt = blockDim.x;
   // LDS: x, then the staging area (blob, values, diagonals), which the dense
   // block reuses between the two sides
@@ -2249,7 +2256,7 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   TF *svl = reinterpret_cast<TF *>(area + (size_t)kSolCB * nt * 4);
   TF *sdg = svl + (size_t)kSolCV * nt;
   TF *Dl = reinterpret_cast<TF *>(area);
-  const TF *fv = svv + ((size_t)bsel * Mb + m) * S.ns;
+  const TF *fv = svv + ((size_t)bsel * Mb + fm) * S.ns;
   const TF *bv = fv + S.fw_nvals;
   const TF *bd = bv + S.bw_nvals;
   const int nfs = S.sol_nfw, nbs = S.sol_nbw;
@@ -2284,7 +2291,7 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   }
   if (nbs) sol_fetch<TF>(S.sol_bw_hdr, S.sol_bw_blob, 0, bv, bd, tid, nt, P, pv, pd);
   if (nd) {
-    const TF *dn = dnv + ((size_t)bsel * Mb + m) * nd * nd;
+    const TF *dn = dnv + ((size_t)bsel * Mb + fm) * nd * nd;
     copy16(Dl, dn, (size_t)nd * nd * sizeof(TF), tid, nt);
     __syncthreads();
   }
@@ -2347,6 +2354,29 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
       sol_store<TF>(sb, svl, sdg, tid, nt, P, pv, pd);
       lds_barrier();
     }
+  }
+  // Woodbury correction of the shared factorisation (uniform per workgroup):
+  // s = H^{-1} x[pos] by the first nc threads into the free staging area,
+  // then x -= W s, the columns of W read contiguously
+  const int wnc = wb.nc ? wb.nc[m] : 0;
+  if (wnc > 0) {
+    double *ws = reinterpret_cast<double *>(area);
+    const int *wcol = wb.col + (size_t)m * wb.kw;
+    __syncthreads();
+    if (tid < wnc) {
+      const double *h = wb.h + ((size_t)m * wb.kw + tid) * wb.kw;
+      const int *wpos = wb.pos + (size_t)m * wb.kw;
+      double acc = 0.0;
+      for (int c = 0; c < wnc; c++) acc = fma(h[c], x[wpos[c]], acc);
+      ws[tid] = acc;
+    }
+    __syncthreads();
+    for (int k = tid; k < n; k += nt) {
+      double acc = 0.0;
+      for (int c = 0; c < wnc; c++) acc = fma(wb.W[(size_t)wcol[c] * n + k], ws[c], acc);
+      x[k] -= acc;
+    }
+    __syncthreads();
   }
   // outputs, sixteen rows per thread in flight (the variants without a
   // scatter table or a state update carry no conditional loads, which
@@ -2891,8 +2921,8 @@ void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *
 template <typename TF>
 void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const double *rhs, int ldM, double sign,
                        const int *scatter, double *zupd, double *xout, const uint8_t *work, const uint8_t *cur, int M,
-                       hipStream_t s) {
-  if (S.gx) {
+                       hipStream_t s, bool shared, const WbTab *wb) {
+  if (S.gx) {   // (the host refuses a shared factorisation with x in global memory)
     static std::atomic<unsigned> done_gx{0};
     allow_lds(reinterpret_cast<const void *>(k_splu_solve_gx<TF>), done_gx, S.lds_solve);
     const dim3 lanes(std::min(S.n, 4096), (ldM + WAVE - 1) / WAVE);
@@ -2911,7 +2941,7 @@ void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const doubl
   static std::atomic<unsigned> done_sol{0};
   allow_lds(reinterpret_cast<const void *>(k_splu_solve<TF>), done_sol, S.lds_solve);
   hipLaunchKernelGGL(k_splu_solve<TF>, dim3(M), dim3(S.threads), S.lds_solve, s, S, sv, dn, rhs, ldM, sign, scatter,
-                     zupd, xout, work, cur);
+                     zupd, xout, work, cur, shared ? 1 : 0, wb ? *wb : WbTab{});
 }
 void launch_adj_mask(int ldM, int k, const int *hz, uint8_t *act, hipStream_t s) {
   hipLaunchKernelGGL(k_adj_mask, dim3((ldM + 255) / 256), dim3(256), 0, s, ldM, k, hz, act);
@@ -3042,9 +3072,9 @@ template void launch_splu_factor_impl<float>(const SpluDev &, const double *, co
                                              float *, const uint8_t *, const uint8_t *, int, int *, hipStream_t);
 template void launch_splu_solve<double>(const SpluDev &, const double *, const double *, const double *, int, double,
                                         const int *, double *, double *, const uint8_t *, const uint8_t *, int,
-                                        hipStream_t);
+                                        hipStream_t, bool, const WbTab *);
 template void launch_splu_solve<float>(const SpluDev &, const float *, const float *, const double *, int, double,
                                        const int *, double *, double *, const uint8_t *, const uint8_t *, int,
-                                       hipStream_t);
+                                       hipStream_t, bool, const WbTab *);
 
 }  // namespace gp

@@ -19,7 +19,7 @@ from __future__ import annotations
 import numpy as np
 
 from .desc import build_desc
-from .lib import (HipContext, METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32,
+from .lib import (HipContext, METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32, METHOD_SCHUR_SHARED,
                   REFACTOR_ASYNC, REFACTOR_EVERY_ITER)
 
 
@@ -103,3 +103,23 @@ def integrate_hip_multi(gbl, ps, tf, fault_buses, rfaults, ton, toff, dt=1.0 / 1
     trajs = _run(gbl, method, ev, nsteps, dt, newton_tol, itermax, raise_on_fail, ps)
     tvecs = [np.arange(ns + 1) * dt for ns in nsteps]
     return tvecs, trajs
+
+
+def integrate_hip_shared(gbl, ps, tf, dt=1.0 / 120.0, newton_tol=1e-10, itermax=30, raise_on_fail=True):
+    """Shared-factor Schur Newton with Woodbury fault corrections
+    (integrate_gpu_shared!, ext:2296-2375): one FP64 factorisation of the
+    reduced network at the initial state serves every scenario and
+    iteration; the events of ps.dynamic in every scenario."""
+    nsteps = _step(tf, dt)
+    tvec = np.arange(nsteps + 1) * dt
+    trajs = _run(gbl, METHOD_SCHUR_SHARED, _shared_events(ps, gbl.M, dt), [nsteps] * gbl.M, dt,
+                 newton_tol, itermax, raise_on_fail, ps)
+    return tvec, trajs
+
+
+def integrate_hip_shared_multi(gbl, ps, tf, fault_buses, rfaults, ton, toff, dt=1.0 / 120.0,
+                               newton_tol=1e-10, itermax=30, raise_on_fail=True):
+    """integrate_gpu_shared_multi! (ext:2640-2708): one fault per scenario,
+    the shared factorisation with a rank-two Woodbury correction each."""
+    return integrate_hip_multi(gbl, ps, tf, fault_buses, rfaults, ton, toff, dt=dt, newton_tol=newton_tol,
+                               itermax=itermax, method=METHOD_SCHUR_SHARED, raise_on_fail=raise_on_fail)

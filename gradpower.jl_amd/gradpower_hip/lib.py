@@ -19,7 +19,7 @@ LIBPATH = os.path.join(HERE, "libgradpower_hip.so")
 GP_OK = 0
 GP_ERR_BADARG, GP_ERR_HIP, GP_ERR_OOM, GP_ERR_SINGULAR_PIVOT = -1, -2, -3, -4
 GP_ERR_NEWTON_NOT_CONVERGED, GP_ERR_NAN, GP_ERR_UNSUPPORTED = -5, -6, -7
-METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32 = 0, 1, 2
+METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32, METHOD_SCHUR_SHARED = 0, 1, 2, 3
 REFACTOR_EVERY_ITER, REFACTOR_FIRST_ITER, REFACTOR_ADAPTIVE, REFACTOR_ASYNC = 0, 1, 2, 3
 _NAMES = {-1: "BADARG", -2: "HIP", -3: "OOM", -4: "SINGULAR_PIVOT", -5: "NEWTON_NOT_CONVERGED",
           -6: "NAN", -7: "UNSUPPORTED"}

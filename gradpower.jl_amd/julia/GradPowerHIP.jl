@@ -24,7 +24,7 @@ using GradPower
 using SparseArrays
 using ChainRulesCore
 
-export HipBatchedLayout, integrate_hip!, integrate_hip_schur!, integrate_hip_multi!, hip_adjoint,
+export HipBatchedLayout, integrate_hip!, integrate_hip_schur!, integrate_hip_multi!, integrate_hip_shared!, integrate_hip_shared_multi!, hip_adjoint,
        hip_objective, scenario_status, symbolic_info
 
 const LIB = joinpath(@__DIR__, "..", "gradpower_hip", "libgradpower_hip.so")
@@ -153,7 +153,7 @@ end
 
 const GP_OK = Cint(0)
 const GP_ERR_NEWTON_NOT_CONVERGED = Cint(-5)
-const METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32 = Int32(0), Int32(1), Int32(2)
+const METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32, METHOD_SCHUR_SHARED = Int32(0), Int32(1), Int32(2), Int32(3)
 const REFACTOR_EVERY_ITER, REFACTOR_FIRST_ITER, REFACTOR_ADAPTIVE, REFACTOR_ASYNC = Int32.(0:3)
 
 last_error() = unsafe_string(ccall((:gp_last_error, LIB), Cstring, ()))
@@ -418,6 +418,34 @@ function integrate_hip_multi!(hbl::HipBatchedLayout, ps, tf, fault_buses::Abstra
     trajs = _run!(hbl, ps, Int32(method), ev, nsteps, dt, newton_tol, itermax)
     return [collect(0:Int(nsteps[m])) .* dt for m in 1:M], trajs
 end
+
+"""
+    integrate_hip_shared!(hbl, ps, tf; dt = 1/120, newton_tol = 1e-10, itermax = 30)
+
+`integrate_gpu_shared!` (ext:2296-2375): one FP64 factorisation of the
+reduced network at the initial state shared by every scenario and iteration,
+the faults entering each solve as a Woodbury correction; the events of
+`ps.dynamic` in every scenario.
+"""
+function integrate_hip_shared!(hbl::HipBatchedLayout, ps, tf::Float64; dt::Float64 = 1.0 / 120.0,
+                               newton_tol::Float64 = 1e-10, itermax::Int = 30)
+    ns = steps(tf, dt)
+    trajs = _run!(hbl, ps, METHOD_SCHUR_SHARED, _shared_events(hbl, ps, dt), fill(Int32(ns), hbl.M), dt,
+                  newton_tol, itermax)
+    for e in ps.dynamic.events
+        GradPower.deactivate!(e)
+    end
+    return collect(0:dt:tf), trajs
+end
+
+"""
+    integrate_hip_shared_multi!(hbl, ps, tf, fault_buses, rfaults, ton, toff; ...)
+
+`integrate_gpu_shared_multi!` (ext:2640-2708): `integrate_hip_multi!` with the
+shared factorisation and a rank-two Woodbury correction per scenario.
+"""
+integrate_hip_shared_multi!(hbl::HipBatchedLayout, ps, tf, fault_buses, rfaults, ton, toff; kw...) =
+    integrate_hip_multi!(hbl, ps, tf, fault_buses, rfaults, ton, toff; kw..., method = METHOD_SCHUR_SHARED)
 
 """
     scenario_status(hbl) -> (status, steps_done)

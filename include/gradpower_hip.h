@@ -8,6 +8,8 @@
  *   integrate_gpu_schur_cudss!(gbl, ps, tf; ..) ext:3008-3080    -> gp_integrate(GP_METHOD_SCHUR_*)
  *   integrate_gpu_shared_multi!(..., fault_buses, rfaults, ton, toff)
  *                                               ext:2640-2708    -> gp_set_events (per-scenario faults)
+ *                                                                   + gp_integrate(GP_METHOD_SCHUR_SHARED)
+ *   integrate_gpu_shared!(gbl, ps, tf; ..)      ext:2296-2375    -> gp_integrate(GP_METHOD_SCHUR_SHARED)
  *   TripLineEvent / create_trip_line_event     src/GradPower.jl:140-167,509-570
  *   (CPU integrate! only, src/dynamics.jl:536-540,616-624)        -> gp_set_line_trips (per scenario)
  *   DisconnectDeviceEvent (ZIP loads)          src/GradPower.jl:130-143,681-698
@@ -146,8 +148,22 @@ typedef struct gp_load_drop {
 enum gp_method {
   GP_METHOD_MONO_FP64 = 0,    /* monolithic Newton, sparse LU of J refactored every iteration */
   GP_METHOD_SCHUR_FP64 = 1,   /* cluster Schur reduction, FP64 factors                        */
-  GP_METHOD_SCHUR_FP32 = 2    /* cluster Schur reduction, FP32 block/network factors,
+  GP_METHOD_SCHUR_FP32 = 2,   /* cluster Schur reduction, FP32 block/network factors,
                                  FP64 residual, update and states (inexact Newton)           */
+  GP_METHOD_SCHUR_SHARED = 3  /* cluster Schur reduction with ONE FP64 factorisation of the
+                                 reduced network shared by every scenario: P = S of
+                                 scenario 0 at its state when gp_integrate starts, step dt,
+                                 no event in force; each scenario's faults enter its solves
+                                 exactly as a Woodbury correction of rank two per faulted
+                                 bus (ext:2135-2290, 2409-2495, 2523-2708).  Cluster blocks
+                                 are exact at every iteration; the network part is frozen
+                                 (inexact Newton).  A solve that fails on P (itermax, or
+                                 retry_window corrections without halving |F|inf) runs
+                                 again from the step's start as exact FP64 Schur Newton
+                                 (gp_stats.exact_retries), where the reference keeps the
+                                 unconverged state.  Not with line trips, load drops,
+                                 lock_step, gp_adjoint, or reduced networks whose solution
+                                 does not fit in LDS (GP_ERR_UNSUPPORTED).                 */
 };
 
 enum gp_refactor_policy {
@@ -211,17 +227,20 @@ typedef struct gp_options {
   double async_below;      /* ASYNC: a refactorisation asked for at |F|inf below
                               this runs on the second stream, above it in the
                               lock step (0: every one runs on the second stream) */
-  int32_t predictor;       /* first Newton iterate of a step: 0 (default) the
-                              linear extrapolation 2 z_k - z_(k-1) under the
-                              policies that reuse factorisations (ADAPTIVE,
+  int32_t predictor;       /* first Newton iterate of a step: 0 (default) z_k,
+                              the reference's start (src/dynamics.jl:548-640);
+                              1: the linear extrapolation 2 z_k - z_(k-1) under
+                              the policies that reuse factorisations (ADAPTIVE,
                               ASYNC; not across an event, whose step starts
-                              from z_k), z_k under exact Newton (the
-                              reference's start, src/dynamics.jl:548-640);
-                              2: the quadratic one 3 z_k - 3 z_(k-1) + z_(k-2)
-                              once two steps have been accepted since the
-                              start or the last event (linear before);
-                              -1: z_k always.  A solve that fails restarts
-                              from z_k as exact Newton either way            */
+                              from z_k); 2: the quadratic one
+                              3 z_k - 3 z_(k-1) + z_(k-2) once two steps have
+                              been accepted since the start or the last event
+                              (linear before).  Extrapolated starts save
+                              corrections but may converge to another solution
+                              of a step's equations than the reference's start
+                              (one of 225 scenarios on the 2,000-bus workload).
+                              A solve that fails restarts from z_k as exact
+                              Newton either way                              */
   int32_t pivot_rule;      /* static pivots of gp_integrate's batched LU plans:
                               0 (default) large in the intact matrix and in the
                               matrix of every distinct set of tripped lines; 1

@@ -128,10 +128,10 @@ def test_julia_constants_match_header_enums():
     hdr = _strip_c_comments(open(HEADER).read())
     enum = dict((k, int(v)) for k, v in re.findall(r"\b(GP_(?:METHOD|REFACTOR|ERR)_\w+)\s*=\s*(-?\d+)", hdr))
     t = open(JL).read()
-    m = re.search(r"const METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32 = Int32\((\d)\), Int32\((\d)\), "
-                  r"Int32\((\d)\)", t)
+    m = re.search(r"const METHOD_MONO_FP64, METHOD_SCHUR_FP64, METHOD_SCHUR_FP32, METHOD_SCHUR_SHARED = "
+                  r"Int32\((\d)\), Int32\((\d)\), Int32\((\d)\), Int32\((\d)\)", t)
     assert m and [int(x) for x in m.groups()] == [enum["GP_METHOD_MONO_FP64"], enum["GP_METHOD_SCHUR_FP64"],
-                                                  enum["GP_METHOD_SCHUR_FP32"]]
+                                                  enum["GP_METHOD_SCHUR_FP32"], enum["GP_METHOD_SCHUR_SHARED"]]
     assert "REFACTOR_EVERY_ITER, REFACTOR_FIRST_ITER, REFACTOR_ADAPTIVE, REFACTOR_ASYNC = Int32.(0:3)" in t
     assert [enum[k] for k in ("GP_REFACTOR_EVERY_ITER", "GP_REFACTOR_FIRST_ITER", "GP_REFACTOR_ADAPTIVE",
                               "GP_REFACTOR_ASYNC")] == [0, 1, 2, 3]
