@@ -443,7 +443,8 @@ struct gp_ctx {
   // trip (bus = -2 - t for trip record t, see FaultState)
   std::vector<gp_event> events, faults_user;
   std::vector<gp_line_trip> trips;
-  std::vector<gp_load_drop> drops;
+  std::vector<gp_device_drop> drops;   // ZIP loads and exciters taken offline
+  int nexc = 0;                        // exciter drops among the events (FaultState::nexc)
   // line trips: records (from, to; stamp) and the off-diagonal Y entries
   // that trips change, kept once made variable (see DevTables)
   std::vector<int> tr_bus, tv_p, tv_i;
@@ -986,7 +987,7 @@ void alloc_state(gp_ctx *C) {
   HIPCHK(hipMemset(C->d_fg.p, 0, (size_t)MAXFAULT * L * sizeof(double)));
 }
 
-FaultState faults(gp_ctx *C) { return FaultState{C->d_fbus.p, C->d_fg.p}; }
+FaultState faults(gp_ctx *C) { return FaultState{C->d_fbus.p, C->d_fg.p, C->nexc}; }
 
 void set_mask(gp_ctx *C, DBuf<uint8_t> &d, const std::vector<uint8_t> &h) {
   HIPCHK(hipMemcpyAsync(d.p, h.data(), h.size(), hipMemcpyHostToDevice, C->stream));
@@ -1117,9 +1118,11 @@ void analyse(gp_ctx *C, int method) {
     C->d_dv.alloc((size_t)C->nred * L);
     C->d_X64.alloc((size_t)std::max(1, C->xrows) * 3 * L);
     HIPCHK(hipMemcpyAsync(C->d_zold.p, C->d_z.p, (size_t)C->n * L * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
-    launch_cluster_residual(C->T, C->d_p.p, C->d_z.p, C->d_zold.p, dt, C->d_f.p, C->d_inj.p, C->d_sel.p, C->stream);
+    launch_cluster_residual(C->T, C->d_p.p, C->d_z.p, C->d_zold.p, dt, faults(C),
+                            C->d_f.p, C->d_inj.p, C->d_sel.p, C->stream);
     launch_bus_residual(C->T, C->d_p.p, C->d_z.p, C->d_inj.p, faults(C), C->d_f.p, C->d_sel.p, C->stream);
-    launch_cluster_schur<double>(C->T, C->d_p.p, C->d_z.p, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, C->d_sel.p, C->stream);
+    launch_cluster_schur<double>(C->T, C->d_p.p, C->d_z.p, C->d_f.p, dt, faults(C),
+                                 C->d_X64.p, C->d_Dg.p, C->d_sel.p, C->stream);
     launch_bus_schur(C->T, C->d_p.p, C->d_z.p, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_sel.p,
                      C->stream);
     HIPCHK(hipGetLastError());
@@ -1497,7 +1500,8 @@ struct Newton {
   void eval(int first, const uint8_t *firstv, const uint8_t *refac0, int *cnt, bool read_req) {
     const int L = C->ldM;
     double *z = C->d_z.p;
-    launch_cluster_residual(C->T, C->d_p.p, z, C->d_zold.p, dt, C->d_f.p, C->d_inj.p, C->d_work.p, C->stream, dtv);
+    launch_cluster_residual(C->T, C->d_p.p, z, C->d_zold.p, dt, faults(C),
+                            C->d_f.p, C->d_inj.p, C->d_work.p, C->stream, dtv);
     launch_bus_residual(C->T, C->d_p.p, z, C->d_inj.p, faults(C), C->d_f.p, C->d_work.p, C->stream);
     if (cnt) HIPCHK(hipMemsetAsync(C->d_nunconv.p, 0, 4 * sizeof(int), C->stream));
     launch_rownorm_impl(C->n, C->M, L, C->d_f.p, C->d_work.p, tol, C->d_fnorm.p, C->d_nb.p, C->d_nunconv.p,
@@ -1598,7 +1602,8 @@ struct Newton {
         ref64 = C->d_split.p + 4 * L;
         upd64 = C->d_split.p + 5 * L;
       }
-      launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, C->d_sch.p, C->stream, dtv);
+      launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, faults(C),
+                                   C->d_X64.p, C->d_Dg.p, C->d_sch.p, C->stream, dtv);
       launch_bus_schur(C->T, p, z, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_sch.p, C->stream);
       if (shmix) {
         prof_f = prof_begin(C, C->prof_factor, C->stream);
@@ -1635,9 +1640,10 @@ struct Newton {
       HIPCHK(hipEventRecord(C->ev_fork, C->stream));
       HIPCHK(hipStreamWaitEvent(C->side, C->ev_fork, 0));
     }
-    if (run64) launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, sch64, s64, dtv);
+    if (run64) launch_cluster_schur<double>(C->T, p, z, C->d_f.p, dt, faults(C),
+        C->d_X64.p, C->d_Dg.p, sch64, s64, dtv);
     if (run32)
-      launch_cluster_schur<float>(C->T, p, z, C->d_f.p, dt, C->d_X32.p, C->d_Dg.p, sch32, C->stream, dtv);
+      launch_cluster_schur<float>(C->T, p, z, C->d_f.p, dt, faults(C), C->d_X32.p, C->d_Dg.p, sch32, C->stream, dtv);
     if (mixed) {
       HIPCHK(hipEventRecord(C->ev_side, C->side));
       HIPCHK(hipStreamWaitEvent(C->stream, C->ev_side, 0));
@@ -2410,9 +2416,11 @@ std::vector<double> scenario_matrix(gp_ctx *C, int method, int m) {
     return jv;
   }
   // at dt = 0 the matrix does not depend on zold (z stands in for it)
-  launch_cluster_residual(C->T, C->d_p.p, C->d_z.p, C->d_z.p, 0.0, C->d_f.p, C->d_inj.p, C->d_sel.p, C->stream);
+  launch_cluster_residual(C->T, C->d_p.p, C->d_z.p, C->d_z.p, 0.0, faults(C),
+                          C->d_f.p, C->d_inj.p, C->d_sel.p, C->stream);
   launch_bus_residual(C->T, C->d_p.p, C->d_z.p, C->d_inj.p, faults(C), C->d_f.p, C->d_sel.p, C->stream);
-  launch_cluster_schur<double>(C->T, C->d_p.p, C->d_z.p, C->d_f.p, 0.0, C->d_X64.p, C->d_Dg.p, C->d_sel.p, C->stream);
+  launch_cluster_schur<double>(C->T, C->d_p.p, C->d_z.p, C->d_f.p, 0.0, faults(C),
+                               C->d_X64.p, C->d_Dg.p, C->d_sel.p, C->stream);
   launch_bus_schur(C->T, C->d_p.p, C->d_z.p, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_sel.p,
                    C->stream);
   HIPCHK(hipGetLastError());
@@ -2843,7 +2851,7 @@ int gp_set_state(gp_ctx *C, const double *z0, int32_t per_scenario, const double
 // The event list the time loops read: the faults, then one record per line
 // trip (slot code -2 - t) and per ZIP load disconnect (kDropCode0 - load).
 static std::vector<gp_event> event_list(const std::vector<gp_event> &faults, const std::vector<gp_line_trip> &trips,
-                                        const std::vector<gp_load_drop> &drops) {
+                                        const std::vector<gp_device_drop> &drops) {
   std::vector<gp_event> ev = faults;
   for (size_t i = 0; i < trips.size(); i++) {
     gp_event e{};
@@ -2854,10 +2862,10 @@ static std::vector<gp_event> event_list(const std::vector<gp_event> &faults, con
     e.step_off = trips[i].step_reclose > 0 ? trips[i].step_reclose : std::numeric_limits<int32_t>::max();
     ev.push_back(e);
   }
-  for (const gp_load_drop &d : drops) {
+  for (const gp_device_drop &d : drops) {
     gp_event e{};
     e.scenario = d.scenario;
-    e.bus = kDropCode0 - d.load;
+    e.bus = d.type == GP_ZIPLOAD ? kDropCode0 - d.row : kExcCode0 - (2 * d.row + (d.type == GP_ESDC1A ? 1 : 0));
     e.r_fault = 1.0;
     e.step_on = d.step;
     e.step_off = std::numeric_limits<int32_t>::max();
@@ -2910,9 +2918,11 @@ static void prepare_shared(gp_ctx *C, double dt, int in_force = -1) {
   sel[0] = 1;
   set_mask(C, C->d_sel, sel);
   launch_copy_active(C->n, L, C->d_z.p, C->d_zold.p, C->d_sel.p, C->stream);
-  launch_cluster_residual(C->T, C->d_p.p, C->d_z.p, C->d_zold.p, dt, C->d_f.p, C->d_inj.p, C->d_sel.p, C->stream);
+  launch_cluster_residual(C->T, C->d_p.p, C->d_z.p, C->d_zold.p, dt, faults(C),
+                          C->d_f.p, C->d_inj.p, C->d_sel.p, C->stream);
   launch_bus_residual(C->T, C->d_p.p, C->d_z.p, C->d_inj.p, faults(C), C->d_f.p, C->d_sel.p, C->stream);
-  launch_cluster_schur<double>(C->T, C->d_p.p, C->d_z.p, C->d_f.p, dt, C->d_X64.p, C->d_Dg.p, C->d_sel.p, C->stream);
+  launch_cluster_schur<double>(C->T, C->d_p.p, C->d_z.p, C->d_f.p, dt, faults(C),
+                               C->d_X64.p, C->d_Dg.p, C->d_sel.p, C->stream);
   launch_bus_schur(C->T, C->d_p.p, C->d_z.p, C->d_f.p, C->d_Dg.p, faults(C), C->d_svar.p, C->d_rhs.p, C->d_sel.p,
                    C->stream);
   W.lu.ensure(std::max<size_t>(1, C->luS.lu_elems()));
@@ -2989,7 +2999,11 @@ static void prepare_shared(gp_ctx *C, double dt, int in_force = -1) {
   HIPCHK(hipStreamSynchronize(C->stream));
 }
 
-static void rebuild_events(gp_ctx *C) { C->events = event_list(C->faults_user, C->trips, C->drops); }
+static void rebuild_events(gp_ctx *C) {
+  C->events = event_list(C->faults_user, C->trips, C->drops);
+  C->nexc = 0;
+  for (const gp_device_drop &d : C->drops) C->nexc += d.type != GP_ZIPLOAD;
+}
 
 int gp_set_events(gp_ctx *C, const gp_event *ev, int32_t n_ev, const int32_t *nsteps) {
   if (!C || (n_ev > 0 && !ev) || !nsteps) return fail(GP_ERR_BADARG, "gp_set_events: null argument");
@@ -3024,18 +3038,46 @@ int gp_set_events(gp_ctx *C, const gp_event *ev, int32_t n_ev, const int32_t *ns
 }
 
 
+int gp_set_device_drops(gp_ctx *C, const gp_device_drop *d, int32_t n) {
+  if (!C || (n > 0 && !d) || n < 0) return fail(GP_ERR_BADARG, "gp_set_device_drops: bad argument");
+  for (int i = 0; i < n; i++) {
+    const int t = d[i].type;
+    if (t != GP_ZIPLOAD && t != GP_SEXS && t != GP_ESDC1A)
+      return fail(GP_ERR_UNSUPPORTED, "gp_set_device_drops: only ZIP loads and SEXS / ESDC1A exciters can go offline "
+                                      "(a device with algebraic states would leave them undetermined)");
+    const int rows = (int)(C->dev[t].size() / (t == GP_ZIPLOAD ? GP_COLS_ZIPLOAD
+                                                               : (t == GP_SEXS ? GP_COLS_SEXS : GP_COLS_ESDC1A)));
+    if (d[i].scenario < 0 || d[i].scenario >= C->M || d[i].row < 0 || d[i].row >= rows)
+      return fail(GP_ERR_BADARG, "gp_set_device_drops: scenario or device row out of range");
+    if (d[i].step < 1) return fail(GP_ERR_BADARG, "gp_set_device_drops: step must be >= 1");
+  }
+  if (!events_fit(event_list(C->faults_user, C->trips, std::vector<gp_device_drop>(d, d + n)), C->M))
+    return fail(GP_ERR_UNSUPPORTED, "gp_set_device_drops: more than " + std::to_string(MAXFAULT) +
+                                        " faults, tripped lines and offline devices in force at once in a scenario");
+  C->drops.assign(d, d + n);
+  rebuild_events(C);
+  try {
+    upload_event_slots(C);
+  } catch (const GpError &e) {
+    return fail(e.code, e.msg);
+  }
+  return GP_OK;
+}
+
 int gp_set_load_drops(gp_ctx *C, const gp_load_drop *d, int32_t n) {
   if (!C || (n > 0 && !d) || n < 0) return fail(GP_ERR_BADARG, "gp_set_load_drops: bad argument");
   const int nzip = (int)(C->dev[GP_ZIPLOAD].size() / GP_COLS_ZIPLOAD);
+  std::vector<gp_device_drop> dd(n);
   for (int i = 0; i < n; i++) {
     if (d[i].scenario < 0 || d[i].scenario >= C->M || d[i].load < 0 || d[i].load >= nzip)
       return fail(GP_ERR_BADARG, "gp_set_load_drops: scenario or load out of range");
     if (d[i].step < 1) return fail(GP_ERR_BADARG, "gp_set_load_drops: step must be >= 1");
+    dd[i] = gp_device_drop{d[i].scenario, GP_ZIPLOAD, d[i].load, d[i].step};
   }
-  if (!events_fit(event_list(C->faults_user, C->trips, std::vector<gp_load_drop>(d, d + n)), C->M))
+  if (!events_fit(event_list(C->faults_user, C->trips, dd), C->M))
     return fail(GP_ERR_UNSUPPORTED, "gp_set_load_drops: more than " + std::to_string(MAXFAULT) +
                                         " faults, tripped lines and dropped loads in force at once in a scenario");
-  C->drops.assign(d, d + n);
+  C->drops = dd;
   rebuild_events(C);
   try {
     upload_event_slots(C);
@@ -3360,7 +3402,7 @@ int gp_adjoint(gp_ctx *C, int32_t method, double dt, double tol, int32_t itermax
                                         nullptr, M, C->d_status.p, C->stream);
         launch_splu_solve<double>(JT, C->d_svJT.p, C->d_dnJT.p, C->d_f.p, L, 1.0, nullptr, nullptr, C->d_lam.p, act,
                                   nullptr, M, C->stream);
-        launch_adj_param(C->T, C->d_p.p, ne, C->d_ent.p, zk, C->d_lam.p, dt, C->d_gacc.p, act, C->stream);
+        launch_adj_param(C->T, C->d_p.p, ne, C->d_ent.p, zk, C->d_lam.p, dt, faults(C), C->d_gacc.p, act, C->stream);
         HIPCHK(hipGetLastError());
       }
       for (int k = k0 + 1; k <= k1; k++)
@@ -3537,8 +3579,8 @@ int gp_eval_residual_noalloc(gp_ctx *C, const double *z, const double *zold, dou
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : C->stream;
   launch_prefix_mask((int)L, C->M, mask, s);
   launch_select_faults((int)L, C->d_evbus.p, C->d_evg.p, fault_on, fbus, fg, s);
-  launch_cluster_residual(C->T, C->d_p.p, z, zold, dt, f, inj, mask, s);
-  launch_bus_residual(C->T, C->d_p.p, z, inj, FaultState{fbus, fg}, f, mask, s);
+  launch_cluster_residual(C->T, C->d_p.p, z, zold, dt, FaultState{fbus, fg, C->nexc}, f, inj, mask, s);
+  launch_bus_residual(C->T, C->d_p.p, z, inj, FaultState{fbus, fg, C->nexc}, f, mask, s);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(GP_ERR_HIP, std::string("gp_eval_residual_noalloc: ") + hipGetErrorString(e));
   return GP_OK;
@@ -3556,7 +3598,8 @@ int gp_eval_residual(gp_ctx *C, const double *z, const double *zold, double dt, 
     std::vector<uint8_t> all(L, 0);
     for (int m = 0; m < C->M; m++) all[m] = 1;
     set_mask(C, C->d_sel, all);
-    launch_cluster_residual(C->T, C->d_p.p, C->d_z.p, C->d_zold.p, dt, C->d_f.p, C->d_inj.p, C->d_sel.p, C->stream);
+    launch_cluster_residual(C->T, C->d_p.p, C->d_z.p, C->d_zold.p, dt, faults(C),
+                            C->d_f.p, C->d_inj.p, C->d_sel.p, C->stream);
     launch_bus_residual(C->T, C->d_p.p, C->d_z.p, C->d_inj.p, faults(C), C->d_f.p, C->d_sel.p, C->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(C->stream));

@@ -12,6 +12,7 @@ namespace gp {
 constexpr int WAVE = 64;
 constexpr int MAXFAULT = 8;   // simultaneously active events (faults, tripped lines, dropped loads) per scenario
 constexpr int kDropCode0 = -(1 << 24);   // event slot code of ZIP load 0 disconnected
+constexpr int kExcCode0 = -(1 << 25);    // event slot code of SEXS 0 taken offline (see FaultState)
 
 // Jacobian slot counts per device (order of the slot lists built on the host).
 constexpr int NS_GENROU = 47;   // incl. optional pm, efd slots (-1 when unwired) and the delta diagonal
@@ -61,11 +62,15 @@ struct DevTables {
 
 // Per-scenario event slots: bus[k*ldM+m] >= 0 is a fault on that bus with
 // conductance g[k*ldM+m]; kDropCode0 < bus <= -2 is the trip of line record
-// t = -2 - bus; bus <= kDropCode0 is ZIP load kDropCode0 - bus disconnected;
-// -1 is an empty slot.
+// t = -2 - bus; kExcCode0 < bus <= kDropCode0 is ZIP load kDropCode0 - bus
+// disconnected; bus <= kExcCode0 is exciter x = kExcCode0 - bus taken
+// offline (SEXS row x/2 for even x, ESDC1A row x/2 for odd x); -1 is an
+// empty slot.  nexc: whether any exciter codes are in the tables (0: the
+// cluster kernels skip the search).
 struct FaultState {
   int *bus;
   double *g;
+  int nexc = 0;
 };
 
 // Woodbury correction of a shared factorisation P (one per scenario, applied
@@ -138,7 +143,7 @@ enum AdjKind {
 // ---- kernel launchers (gp_kernels.hip) ----------------------------------
 // dtv (optional): per-scenario step, replacing dt (0 for an event re-solve)
 void launch_cluster_residual(const DevTables &T, const double *p, const double *z, const double *zold,
-                             double dt, double *f, double *inj, const uint8_t *work, hipStream_t s,
+                             double dt, FaultState fs, double *f, double *inj, const uint8_t *work, hipStream_t s,
                              const double *dtv = nullptr);
 void launch_bus_residual(const DevTables &T, const double *p, const double *z, const double *inj,
                          FaultState fs, double *f, const uint8_t *work, hipStream_t s);
@@ -152,7 +157,7 @@ void launch_jac_mono(const DevTables &T, const double *p, const double *z, doubl
                      double *jv, const uint8_t *work, hipStream_t s, const double *dtv = nullptr);
 template <typename TF>
 void launch_cluster_schur(const DevTables &T, const double *p, const double *z, const double *f,
-                          double dt, TF *X, double *Dg, const uint8_t *work, hipStream_t s,
+                          double dt, FaultState fs, TF *X, double *Dg, const uint8_t *work, hipStream_t s,
                           const double *dtv = nullptr);
 void launch_bus_schur(const DevTables &T, const double *p, const double *z, const double *f,
                       const double *Dg, FaultState fs, double *svar, double *rhs, const uint8_t *work,
@@ -205,7 +210,7 @@ void launch_adj_carry(int n, int ldM, const uint8_t *is_diff, const double *lam,
 void launch_adj_objective(int nr, const int *rows, int ldM, double w, double off, const double *z, double *rhs,
                           double *val, const uint8_t *act, hipStream_t s);
 void launch_adj_param(const DevTables &T, const double *p, int ne, const int2 *ent, const double *z,
-                      const double *lam, double dt, double *g, const uint8_t *act, hipStream_t s);
+                      const double *lam, double dt, FaultState fs, double *g, const uint8_t *act, hipStream_t s);
 void launch_gather_traj(int nrec, int len, int ldM, int m, const double *hist, double *out, hipStream_t s);
 void launch_copy_plain(const double *src, double *dst, long long n, hipStream_t s);
 // allocation-free residual: the address of this GPU's scratch array (taken

@@ -207,12 +207,25 @@ __device__ __forceinline__ void ieeest_residual(const double *P, const double *x
   fr[7] = vs - (T5 / T6) * (KS * y4 - s6);
 }
 
+// Exciter `row` of type et (GP_SEXS / GP_ESDC1A) taken offline in scenario m.
+// An offline exciter's rows of F are zero (the reference's kernels skip a
+// device whose online flag is false into a zeroed F, ext:1340,1072-1199),
+// so backward Euler holds its states: F = z - zold, J rows = identity.
+__device__ __forceinline__ bool exc_offline(const FaultState &fs, int ldM, int m, int et, int row) {
+  if (!fs.nexc) return false;
+  const int code = kExcCode0 - (2 * row + (et == GP_ESDC1A ? 1 : 0));
+  bool off = false;
+#pragma unroll
+  for (int s = 0; s < MAXFAULT; s++) off = off || fs.bus[s * ldM + m] == code;
+  return off;
+}
+
 // ---------------------------------------------------------------- residual kernels
 // One wave = one cluster x 64 scenarios.  Writes the backward-Euler residual
 // of every cluster row and the generator's current injection.
 __global__ void __launch_bounds__(64) k_cluster_residual(DevTables T, const double *__restrict__ p,
                                                          const double *__restrict__ z, const double *__restrict__ zold,
-                                                         double dt0, const double *__restrict__ dtv,
+                                                         double dt0, const double *__restrict__ dtv, FaultState fs,
                                                          double *__restrict__ f, double *__restrict__ inj,
                                                          const uint8_t *__restrict__ work) {
   const int c = T.cl_nt[blockIdx.x];
@@ -286,6 +299,7 @@ __global__ void __launch_bounds__(64) k_cluster_residual(DevTables T, const doub
     for (int i = 0; i < ne; i++) x[i] = z[(size_t)(d[0] + i) * ldM + m];
     if (et == GP_SEXS) sexs_residual(p + d[1], x, evr, evi, vs, r);
     else esdc1a_residual(p + d[1], x, evr, evi, vs, r);
+    if (exc_offline(fs, ldM, m, et, cl[GP_CL_EXC])) r[0] = r[1] = r[2] = 0.0;
     for (int i = 0; i < ne; i++) {
       const size_t o = (size_t)(d[0] + i) * ldM + m;
       f[o] = x[i] - zold[o] - dt * r[i];
@@ -574,8 +588,8 @@ __device__ __forceinline__ double be(double v, bool diff, bool diag, double dt) 
 
 __global__ void __launch_bounds__(64) k_cluster_jac_mono(DevTables T, const double *__restrict__ p,
                                                          const double *__restrict__ z, double dt0,
-                                                         const double *__restrict__ dtv, double *__restrict__ jv,
-                                                         const uint8_t *__restrict__ work) {
+                                                         const double *__restrict__ dtv, FaultState fs,
+                                                         double *__restrict__ jv, const uint8_t *__restrict__ work) {
   const int c = T.cl_nt[blockIdx.x];
   const int m = lane_scenario();
   if (m >= T.M || !work[m]) return;
@@ -637,9 +651,10 @@ __global__ void __launch_bounds__(64) k_cluster_jac_mono(DevTables T, const doub
                          K / TE, -1.0 / TE, -K * TA_TB * dr / TE, -K * TA_TB * di / TE, K * TA_TB / TE};
     const bool diag[9] = {1, 0, 0, 0, 0, 1, 0, 0, 0};
     const int *jpd = T.jp_sexs + cl[GP_CL_EXC] * NS_SEXS;
+    const double on = exc_offline(fs, ldM, m, et, cl[GP_CL_EXC]) ? 0.0 : 1.0;
 #pragma unroll
     for (int s = 0; s < 9; s++)
-      if (jpd[s] >= 0) jv[(size_t)jpd[s] * ldM + m] = be(v[s], true, diag[s], dt);
+      if (jpd[s] >= 0) jv[(size_t)jpd[s] * ldM + m] = be(on * v[s], true, diag[s], dt);
   } else if (et == GP_ESDC1A) {
     const int *d = T.esdc1a + cl[GP_CL_EXC] * GP_COLS_ESDC1A;
     const double *P = p + d[1];
@@ -653,9 +668,10 @@ __global__ void __launch_bounds__(64) k_cluster_jac_mono(DevTables T, const doub
                           Ka / Ta, -1.0 / Tf, -Kf / (Tf * Tf), 1.0 / Te, -(Ke + dsat) / Te};
     const bool diag[10] = {1, 0, 0, 0, 0, 0, 1, 0, 0, 1};
     const int *jpd = T.jp_esdc1a + cl[GP_CL_EXC] * NS_ESDC1A;
+    const double on = exc_offline(fs, ldM, m, et, cl[GP_CL_EXC]) ? 0.0 : 1.0;
 #pragma unroll
     for (int s = 0; s < 10; s++)
-      if (jpd[s] >= 0) jv[(size_t)jpd[s] * ldM + m] = be(v[s], true, diag[s], dt);
+      if (jpd[s] >= 0) jv[(size_t)jpd[s] * ldM + m] = be(on * v[s], true, diag[s], dt);
   }
   if (cl[GP_CL_PSS] >= 0) {
     const int *d = T.ieeest + cl[GP_CL_PSS] * GP_COLS_IEEEST;
@@ -933,7 +949,7 @@ __device__ __forceinline__ void ieeest_block(const double *P, double dt, TF (&a)
 // SEXS / ESDC1A (2 or 3 states, padded to 3 with an identity row)
 template <typename TF>
 __device__ __forceinline__ void exciter_block(int et, const double *P, double dt, double e_fd, double vr, double vi,
-                                              TF (&a)[3][3], TF (&av)[3], TF (&bvr)[3], TF (&bvi)[3]) {
+                                              bool off, TF (&a)[3][3], TF (&av)[3], TF (&bvr)[3], TF (&bvi)[3]) {
   double vm = sqrt(vr * vr + vi * vi);
   if (vm == 0.0) vm = 1e-12;
   const double dr = vr / vm, di = vi / vm;
@@ -943,7 +959,9 @@ __device__ __forceinline__ void exciter_block(int et, const double *P, double dt
 #pragma unroll
     for (int j = 0; j < 3; j++) a[i][j] = 0;
   }
-  if (et == GP_SEXS) {
+  if (off) {   // offline: identity rows, no coupling
+    a[0][0] = a[1][1] = a[2][2] = TF(1.0);
+  } else if (et == GP_SEXS) {
     const double TA_TB = P[0], TB = P[1], K = P[2], TE = P[3];
     a[0][0] = TF(1.0 + dt / TB);
     bvr[0] = TF(dt * (1.0 - TA_TB) * dr / TB); bvi[0] = TF(dt * (1.0 - TA_TB) * di / TB);
@@ -975,7 +993,7 @@ __device__ __forceinline__ void exciter_block(int et, const double *P, double dt
 template <typename TF>
 __global__ void __launch_bounds__(64) k_cluster_schur(DevTables T, const double *__restrict__ p,
                                                       const double *__restrict__ z, const double *__restrict__ f,
-                                                      double dt0, const double *__restrict__ dtv,
+                                                      double dt0, const double *__restrict__ dtv, FaultState fs,
                                                       TF *__restrict__ X, double *__restrict__ Dg,
                                                       const uint8_t *__restrict__ work) {
   const int c = T.cl_nt[blockIdx.x];
@@ -1064,10 +1082,11 @@ __global__ void __launch_bounds__(64) k_cluster_schur(DevTables T, const double 
   const int *ed = et == GP_SEXS ? T.sexs + cl[GP_CL_EXC] * GP_COLS_SEXS
                                 : (et == GP_ESDC1A ? T.esdc1a + cl[GP_CL_EXC] * GP_COLS_ESDC1A : nullptr);
   const int ne = et == GP_SEXS ? 2 : 3;
+  const bool eoff = ed && exc_offline(fs, ldM, m, et, cl[GP_CL_EXC]);
   if (ed) {
     TF a[3][3], av[3], bvr[3], bvi[3], r[3][4];
     const double e_fd0 = et == GP_ESDC1A ? z[(size_t)(ed[0] + 2) * ldM + m] : 0.0;
-    exciter_block<TF>(et, p + ed[1], dt, e_fd0, vr, vi, a, av, bvr, bvi);
+    exciter_block<TF>(et, p + ed[1], dt, e_fd0, vr, vi, eoff, a, av, bvr, bvi);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       r[i][0] = bvr[i]; r[i][1] = bvi[i];
@@ -1181,7 +1200,7 @@ __global__ void __launch_bounds__(64) k_cluster_schur(DevTables T, const double 
   if (ed) {
     TF a[3][3], av[3], bvr[3], bvi[3], r[3][3];
     const double e_fd0 = et == GP_ESDC1A ? z[(size_t)(ed[0] + 2) * ldM + m] : 0.0;
-    exciter_block<TF>(et, p + ed[1], dt, e_fd0, vr, vi, a, av, bvr, bvi);
+    exciter_block<TF>(et, p + ed[1], dt, e_fd0, vr, vi, eoff, a, av, bvr, bvi);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       const TF fi = i < ne ? TF(f[(size_t)(ed[0] + i) * ldM + m]) : TF(0);
@@ -2618,7 +2637,8 @@ __global__ void k_adj_objective(int nr, const int *__restrict__ rows, int ldM, d
 //                      dF_6/dKS = -dt y4 / T6,  dg/dKS = -(T5/T6) y4
 __global__ void __launch_bounds__(64) k_adj_param(DevTables T, const double *__restrict__ p, const int2 *__restrict__ ent,
                                                   const double *__restrict__ z, const double *__restrict__ lam,
-                                                  double dt, double *__restrict__ g, const uint8_t *__restrict__ act) {
+                                                  double dt, FaultState fs, double *__restrict__ g,
+                                                  const uint8_t *__restrict__ act) {
   const int e = blockIdx.x;
   const int m = lane_scenario();
   if (m >= T.M || !act[m]) return;
@@ -2652,7 +2672,9 @@ __global__ void __launch_bounds__(64) k_adj_param(DevTables T, const double *__r
     const double vr = Z(d[2]), vi = Z(d[2] + 1), vs = d[3] >= 0 ? Z(d[3]) : 0.0;
     double vm = sqrt(vr * vr + vi * vi);
     if (vm == 0.0) vm = 1e-12;
-    if (kind == GP_ADJ_SEXS_K) {
+    if (exc_offline(fs, ldM, m, kind == GP_ADJ_SEXS_K ? GP_SEXS : GP_ESDC1A, row)) {
+      acc = 0.0;   // offline: its rows of F do not depend on its parameters
+    } else if (kind == GP_ADJ_SEXS_K) {
       const double err = P[6] - vm + vs;
       acc = -LAM(d[0] + 1) * dt * (Z(d[0]) + P[0] * err) / P[3];
     } else {
@@ -2844,10 +2866,11 @@ __global__ void k_copy_plain(const double *__restrict__ src, double *__restrict_
 static inline dim3 lane_grid(int items, int ldM) { return dim3(items, ldM / WAVE); }
 
 void launch_cluster_residual(const DevTables &T, const double *p, const double *z, const double *zold, double dt,
-                             double *f, double *inj, const uint8_t *work, hipStream_t s, const double *dtv) {
+                             FaultState fs, double *f, double *inj, const uint8_t *work, hipStream_t s,
+                             const double *dtv) {
   if (T.n_cl_nt == 0) return;
-  hipLaunchKernelGGL(k_cluster_residual, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, zold, dt, dtv, f, inj,
-                     work);
+  hipLaunchKernelGGL(k_cluster_residual, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, zold, dt, dtv, fs, f,
+                     inj, work);
 }
 void launch_bus_residual(const DevTables &T, const double *p, const double *z, const double *inj, FaultState fs,
                          double *f, const uint8_t *work, hipStream_t s) {
@@ -2868,15 +2891,16 @@ void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, 
 void launch_jac_mono(const DevTables &T, const double *p, const double *z, double dt, FaultState fs, double *jv,
                      const uint8_t *work, hipStream_t s, const double *dtv) {
   if (T.n_cl_nt)
-    hipLaunchKernelGGL(k_cluster_jac_mono, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, dt, dtv, jv, work);
+    hipLaunchKernelGGL(k_cluster_jac_mono, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, dt, dtv, fs, jv,
+                       work);
   hipLaunchKernelGGL(k_bus_jac_mono, lane_grid(T.nbus, T.ldM), dim3(WAVE), 0, s, T, p, z, fs, jv, work);
 }
 template <typename TF>
-void launch_cluster_schur(const DevTables &T, const double *p, const double *z, const double *f, double dt, TF *X,
-                          double *Dg, const uint8_t *work, hipStream_t s, const double *dtv) {
+void launch_cluster_schur(const DevTables &T, const double *p, const double *z, const double *f, double dt,
+                          FaultState fs, TF *X, double *Dg, const uint8_t *work, hipStream_t s, const double *dtv) {
   if (T.n_cl_nt == 0) return;
-  hipLaunchKernelGGL(k_cluster_schur<TF>, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, f, dt, dtv, X, Dg,
-                     work);
+  hipLaunchKernelGGL(k_cluster_schur<TF>, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, f, dt, dtv, fs, X,
+                     Dg, work);
 }
 void launch_bus_schur(const DevTables &T, const double *p, const double *z, const double *f, const double *Dg,
                       FaultState fs, double *svar, double *rhs, const uint8_t *work, hipStream_t s) {
@@ -2956,9 +2980,9 @@ void launch_adj_objective(int nr, const int *rows, int ldM, double w, double off
   hipLaunchKernelGGL(k_adj_objective, dim3((ldM + 63) / 64), dim3(64), 0, s, nr, rows, ldM, w, off, z, rhs, val, act);
 }
 void launch_adj_param(const DevTables &T, const double *p, int ne, const int2 *ent, const double *z,
-                      const double *lam, double dt, double *g, const uint8_t *act, hipStream_t s) {
+                      const double *lam, double dt, FaultState fs, double *g, const uint8_t *act, hipStream_t s) {
   if (ne > 0)
-    hipLaunchKernelGGL(k_adj_param, lane_grid(ne, T.ldM), dim3(WAVE), 0, s, T, p, ent, z, lam, dt, g, act);
+    hipLaunchKernelGGL(k_adj_param, lane_grid(ne, T.ldM), dim3(WAVE), 0, s, T, p, ent, z, lam, dt, fs, g, act);
 }
 void launch_copy_plain(const double *src, double *dst, long long n, hipStream_t s) {
   if (n <= 0) return;
@@ -3058,9 +3082,9 @@ void launch_select_faults(int ldM, const int *ev_bus, const double *ev_g, const 
 }
 
 template void launch_cluster_schur<double>(const DevTables &, const double *, const double *, const double *, double,
-                                           double *, double *, const uint8_t *, hipStream_t, const double *);
+                                             FaultState, double *, double *, const uint8_t *, hipStream_t, const double *);
 template void launch_cluster_schur<float>(const DevTables &, const double *, const double *, const double *, double,
-                                          float *, double *, const uint8_t *, hipStream_t, const double *);
+                                             FaultState, float *, double *, const uint8_t *, hipStream_t, const double *);
 template void launch_cluster_backsub<double>(const DevTables &, const double *, const double *, const double *,
                                              double *, const uint8_t *, hipStream_t);
 template void launch_cluster_backsub<float>(const DevTables &, const float *, const double *, const double *,

@@ -53,9 +53,9 @@ def _run(gbl, method, events, nsteps, dt, newton_tol, itermax, raise_on_fail, ps
                  for m in range(gbl.M) for t in ps.dynamic.trip_events]
         if trips:
             gbl.ctx.set_line_trips(trips)
-        drops = [(m, d.zip_row(ps), _step(d.ton, dt)) for m in range(gbl.M) for d in ps.dynamic.disconnect_events]
+        drops = [(m,) + d.target(ps) + (_step(d.ton, dt),) for m in range(gbl.M) for d in ps.dynamic.disconnect_events]
         if drops:
-            gbl.ctx.set_load_drops(drops)
+            gbl.ctx.set_device_drops(drops)
     gbl.last_stats = gbl.ctx.integrate(method, dt, newton_tol, itermax, raise_on_fail=raise_on_fail)
     trajs = [gbl.ctx.trajectory(m) for m in range(gbl.M)]
     return trajs

@@ -48,6 +48,10 @@ class CLoadDrop(C.Structure):
     _fields_ = [("scenario", C.c_int32), ("load", C.c_int32), ("step", C.c_int32), ("reserved", C.c_int32)]
 
 
+class CDeviceDrop(C.Structure):
+    _fields_ = [("scenario", C.c_int32), ("type", C.c_int32), ("row", C.c_int32), ("step", C.c_int32)]
+
+
 def _p(a, ct):
     return a.ctypes.data_as(C.POINTER(ct))
 

@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-from .desc import SystemDesc, CEvent, CLineTrip, CLoadDrop
+from .desc import SystemDesc, CEvent, CLineTrip, CLoadDrop, CDeviceDrop
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libgradpower_hip.so")
@@ -79,7 +79,7 @@ _up = C.POINTER(C.c_uint8)
 _LIB = None
 
 EXPORTS = ["gp_create", "gp_destroy", "gp_set_state", "gp_set_events", "gp_set_line_trips", "gp_set_load_drops",
-           "gp_integrate",
+           "gp_set_device_drops", "gp_integrate",
            "gp_get_trajectory", "gp_get_state", "gp_eval_residual", "gp_jacobian_pattern",
            "gp_eval_jacobian", "gp_newton_step", "gp_get_symbolic_info", "gp_get_scenario_status",
            "gp_copy_kernel", "gp_adjoint", "gp_analyse_pivots", "gp_eval_residual_noalloc", "gp_last_error",
@@ -102,6 +102,7 @@ def load():
     lib.gp_set_events.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, _ip]
     lib.gp_set_line_trips.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
     lib.gp_set_load_drops.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
+    lib.gp_set_device_drops.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
     lib.gp_integrate.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_void_p]
     lib.gp_get_trajectory.argtypes = [C.c_void_p, C.c_int32, _dp, C.c_int64]
     lib.gp_get_state.argtypes = [C.c_void_p, _dp]
@@ -233,6 +234,17 @@ class HipContext:
         for i, (m, k, step) in enumerate(d):
             arr[i].scenario, arr[i].load, arr[i].step = int(m), int(k), int(step)
         self._check(self.lib.gp_set_load_drops(self.h, C.byref(arr), len(d)))
+
+    def set_device_drops(self, drops):
+        """drops: iterable of (scenario, device type, table row, step): a ZIP
+        load (GP_ZIPLOAD) or an SEXS / ESDC1A exciter goes offline for good
+        at the end of that step (system.DisconnectDeviceEvent).  Call after
+        set_events, which clears them; replaces earlier drops."""
+        d = list(drops)
+        arr = (CDeviceDrop * max(1, len(d)))()
+        for i, (m, t, k, step) in enumerate(d):
+            arr[i].scenario, arr[i].type, arr[i].row, arr[i].step = int(m), int(t), int(k), int(step)
+        self._check(self.lib.gp_set_device_drops(self.h, C.byref(arr), len(d)))
 
     def integrate(self, method, dt, tol=1e-10, itermax=50, raise_on_fail=True):
         st = CStats()

@@ -119,16 +119,26 @@ class TripLineEvent:
 class DisconnectDeviceEvent:
     """Device `device_idx` (0-based index into ps.dynamic.devices) taken
     offline for good at ton (src/GradPower.jl:130-143).  The HIP library
-    takes ZIP loads only (zip_row gives the row of its ZIP table)."""
+    takes ZIP loads and SEXS / ESDC1A exciters (target gives the device
+    type and its table row)."""
     device_idx: int
     ton: float
 
-    def zip_row(self, ps):
+    DROPPABLE = {"zipload": 6, "sexs": 3, "esdc1a": 4}   # gp_device_type of each kind
+
+    def target(self, ps):
         devs = ps.dynamic.devices
-        if devs[self.device_idx].dtype.kind != "zipload":
-            raise NotImplementedError("only ZIP loads can be disconnected: a device with algebraic states "
+        kind = devs[self.device_idx].dtype.kind
+        if kind not in self.DROPPABLE:
+            raise NotImplementedError(f"a {kind} cannot be disconnected: a device with algebraic states "
                                       "left out of F and J leaves them undetermined")
-        return sum(1 for d in devs[:self.device_idx] if d.dtype.kind == "zipload")
+        return self.DROPPABLE[kind], sum(1 for d in devs[:self.device_idx] if d.dtype.kind == kind)
+
+    def zip_row(self, ps):
+        t, row = self.target(ps)
+        if t != self.DROPPABLE["zipload"]:
+            raise NotImplementedError("not a ZIP load")
+        return row
 
 
 def create_trip_line_event(ps, from_bus_ext, to_bus_ext, ton, toff=math.inf):

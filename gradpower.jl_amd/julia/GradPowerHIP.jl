@@ -72,6 +72,13 @@ struct GpLoadDrop
     reserved::Int32
 end
 
+struct GpDeviceDrop
+    scenario::Int32
+    type::Int32              # GP_ZIPLOAD, GP_SEXS or GP_ESDC1A
+    row::Int32
+    step::Int32
+end
+
 struct GpOptions
     record_all::Int32
     n_record::Int32
@@ -329,10 +336,17 @@ function _topology_events!(hbl, ps, dt)
           for m in 1:hbl.M for t in psd.trip_events]
     isempty(tr) || check(ccall((:gp_set_line_trips, LIB), Cint, (Ptr{Cvoid}, Ptr{GpLineTrip}, Int32),
                                hbl.ctx, tr, Int32(length(tr))))
-    zrow(i) = count(d -> d.dtype isa GradPower.ZIPLoad, psd.devices[1:i-1])
-    dr = [GpLoadDrop(Int32(m - 1), Int32(zrow(d.device_idx)), Int32(steps(d.ton, dt)), Int32(0))
+    # DisconnectDeviceEvent: ZIP loads and SEXS / ESDC1A exciters (the
+    # library refuses devices with algebraic states)
+    function target(i)
+        T = typeof(psd.devices[i].dtype)
+        ty = T <: GradPower.ZIPLoad ? GP_ZIPLOAD : T <: GradPower.SEXS ? GP_SEXS :
+             T <: GradPower.ESDC1A ? GP_ESDC1A : error("a $(T) cannot be disconnected (algebraic states)")
+        return Int32(ty), Int32(count(d -> typeof(d.dtype) == T, psd.devices[1:i-1]))
+    end
+    dr = [GpDeviceDrop(Int32(m - 1), target(d.device_idx)..., Int32(steps(d.ton, dt)))
           for m in 1:hbl.M for d in psd.disconnect_events]
-    isempty(dr) || check(ccall((:gp_set_load_drops, LIB), Cint, (Ptr{Cvoid}, Ptr{GpLoadDrop}, Int32),
+    isempty(dr) || check(ccall((:gp_set_device_drops, LIB), Cint, (Ptr{Cvoid}, Ptr{GpDeviceDrop}, Int32),
                                hbl.ctx, dr, Int32(length(dr))))
 end
 

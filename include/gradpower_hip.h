@@ -136,14 +136,32 @@ typedef struct gp_line_trip {
  * src/GradPower.jl:130-143,681-698) at the end of step `step`, for good, with
  * a dt = 0 re-solve after it: its current leaves the bus's rows (the
  * reference skips offline devices in its kernels).  `load` is the row of the
- * ZIP load in desc->dev[GP_ZIPLOAD].  Only ZIP loads: a device with algebraic
- * states left out of F and J leaves them undetermined. */
+ * ZIP load in desc->dev[GP_ZIPLOAD].  reserved: 0. */
 typedef struct gp_load_drop {
   int32_t scenario;
   int32_t load;
   int32_t step;
   int32_t reserved;
 } gp_load_drop;
+
+/* One device of one scenario taken offline (DisconnectDeviceEvent) at the
+ * end of step `step`, for good, with a dt = 0 re-solve after it.  The
+ * reference's kernels skip an offline device into a zeroed F
+ * (ext:1340,1072-1199; the `online` checks of src/kernels/), so:
+ *   GP_ZIPLOAD: row `row` of desc->dev[GP_ZIPLOAD]; its current leaves the
+ *     bus's rows (as gp_load_drop);
+ *   GP_SEXS, GP_ESDC1A: row `row` of that exciter table; its rows of F are
+ *     zero, so backward Euler holds its states (and the generator's field
+ *     voltage) at their values when it went offline.
+ * Other types are refused (GP_ERR_UNSUPPORTED): a device with algebraic
+ * states (GENROU, governors, IEEEST, static generators) whose rows leave F
+ * and J leaves those states undetermined (a singular Jacobian). */
+typedef struct gp_device_drop {
+  int32_t scenario;
+  int32_t type;      /* gp_device_type */
+  int32_t row;
+  int32_t step;
+} gp_device_drop;
 
 enum gp_method {
   GP_METHOD_MONO_FP64 = 0,    /* monolithic Newton, sparse LU of J refactored every iteration */
@@ -326,6 +344,13 @@ int gp_set_line_trips(gp_ctx *ctx, const gp_line_trip *trips, int32_t n_trips);
  * and tripped lines, hold a slot from their step on, and are refused here
  * when a scenario would have more than 8 events in force at once. */
 int gp_set_load_drops(gp_ctx *ctx, const gp_load_drop *drops, int32_t n_drops);
+
+/* Per-scenario device disconnects (ZIP loads and exciters, gp_device_drop;
+ * call after gp_set_events, which clears them; replaces the drops of an
+ * earlier gp_set_load_drops / gp_set_device_drops).  Same event slots and
+ * limits as gp_set_load_drops.  Every method takes them, forward and
+ * gp_adjoint (an offline exciter's parameters then have no pullback). */
+int gp_set_device_drops(gp_ctx *ctx, const gp_device_drop *drops, int32_t n_drops);
 
 /* Integrate all scenarios from the current state to their horizons.
  * tol: Newton tolerance on |F|inf per scenario; itermax per Newton solve. */

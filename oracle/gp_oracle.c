@@ -56,8 +56,9 @@ typedef struct {
   int ntr, ynnz;
   int *tr_fr, *tr_to, *tr_on, *tr_off;
   double *tr_y;
-  /* ZIP load disconnects of the next sweeps (orc_set_drops) and, during a
-   * sweep, the online flag of every ZIP load (NULL: all online) */
+  /* device disconnects of the next sweeps (orc_set_drops) and, during a
+   * sweep, the online flag of every device that can go offline (NULL: all
+   * online), indexed ZIP loads first, then SEXS, then ESDC1A rows */
   int ndrop;
   /* 1: a sweep stops at its first Newton solve that does not converge (the
    * HIP library's rule; the reference itself keeps integrating) */
@@ -294,6 +295,10 @@ static void rhs(const orc_sys *s, const double *z, const double *p, const double
   }
   for (int k = 0; k < s->ndev[GP_SEXS]; k++) {      /* src/kernels/sexs.jl:74-103 */
     int dp = D(3, k, 0), vri = D(3, k, 2), vsi = D(3, k, 3);
+    if (s->zon && !s->zon[s->ndev[GP_ZIPLOAD] + k]) {   /* offline: skipped into a zeroed F (ext:1340) */
+      f[dp] = f[dp + 1] = 0.0;
+      continue;
+    }
     const double *P = p + D(3, k, 1);
     double TA_TB = P[0], TB = P[1], K = P[2], TE = P[3], vref = P[6];
     double x1 = z[dp], e_fd = z[dp + 1], vr = z[vri], vi = z[vri + 1];
@@ -304,6 +309,10 @@ static void rhs(const orc_sys *s, const double *z, const double *p, const double
   }
   for (int k = 0; k < s->ndev[GP_ESDC1A]; k++) {    /* src/kernels/esdc1a.jl:97-131 */
     int dp = D(4, k, 0), vri = D(4, k, 2), vsi = D(4, k, 3);
+    if (s->zon && !s->zon[s->ndev[GP_ZIPLOAD] + s->ndev[GP_SEXS] + k]) {
+      f[dp] = f[dp + 1] = f[dp + 2] = 0.0;
+      continue;
+    }
     const double *P = p + D(4, k, 1);
     double Ka = P[0], Ta = P[1], Kf = P[2], Tf = P[3], Ke = P[4], Te = P[5], sa = P[7], sb = P[8], vref = P[9];
     double vr1 = z[dp], vr2 = z[dp + 1], e_fd = z[dp + 2], vr = z[vri], vi = z[vri + 1];
@@ -497,6 +506,7 @@ static void rhs_jac(const orc_sys *s, const double *z, const double *p, const do
     jset(s, J, ap, ap, 1.0);
   }
   for (int k = 0; k < s->ndev[GP_SEXS]; k++) {      /* src/kernels/sexs.jl:118-156 */
+    if (s->zon && !s->zon[s->ndev[GP_ZIPLOAD] + k]) continue;
     int dp = D(3, k, 0), vri = D(3, k, 2), vsi = D(3, k, 3);
     const double *P = p + D(3, k, 1);
     double TA_TB = P[0], TB = P[1], K = P[2], TE = P[3];
@@ -513,6 +523,7 @@ static void rhs_jac(const orc_sys *s, const double *z, const double *p, const do
     if (vsi >= 0) jset(s, J, dp + 1, vsi, K * TA_TB / TE);
   }
   for (int k = 0; k < s->ndev[GP_ESDC1A]; k++) {    /* src/kernels/esdc1a.jl:150-196 */
+    if (s->zon && !s->zon[s->ndev[GP_ZIPLOAD] + s->ndev[GP_SEXS] + k]) continue;
     int dp = D(4, k, 0), vri = D(4, k, 2), vsi = D(4, k, 3);
     const double *P = p + D(4, k, 1);
     double Ka = P[0], Ta = P[1], Kf = P[2], Tf = P[3], Ke = P[4], Te = P[5], sa = P[7], sb = P[8];
@@ -777,10 +788,13 @@ void orc_destroy(void *h) {
   free(s->drop_k); free(s->drop_step); free(s);
 }
 
-/* ZIP load disconnects of the following sweeps (DisconnectDeviceEvent,
- * src/GradPower.jl:130-143,681-698): ZIP load k[i] (row of the ZIP table)
- * goes offline for good at the end of step step[i]; an offline device is
- * skipped by the residual and Jacobian (src/kernels/zipload.jl, `online`). */
+/* Device disconnects of the following sweeps (DisconnectDeviceEvent,
+ * src/GradPower.jl:130-143,681-698): device k[i] goes offline for good at
+ * the end of step step[i]; k indexes the ZIP table's rows, then the SEXS
+ * rows (nzip + row), then the ESDC1A rows (nzip + nsexs + row).  An offline
+ * device is skipped by the residual and Jacobian (the `online` checks of
+ * src/kernels/); an exciter's rows of F are then zero, as in the GPU
+ * extension's zeroed F (ext:1340), so backward Euler holds its states. */
 void orc_set_stop_on_fail(void *h, int flag) { ((orc_sys *)h)->stop_on_fail = flag != 0; }
 
 void orc_set_drops(void *h, int n, const int *k, const int *step) {
@@ -896,7 +910,7 @@ static int sweep(orc_sys *s, const double *z0, const double *p, double dt, int n
    * (src/dynamics.jl:616-624); Y is restored when the sweep ends */
   double *y_saved = xmalloc((size_t)s->ynnz * sizeof(double) + 8);
   memcpy(y_saved, s->yv, (size_t)s->ynnz * sizeof(double));
-  int nz = s->ndev[GP_ZIPLOAD];
+  int nz = s->ndev[GP_ZIPLOAD] + s->ndev[GP_SEXS] + s->ndev[GP_ESDC1A];   /* devices that can go offline */
   uint8_t *zon = xmalloc((size_t)nz + 1);
   memset(zon, 1, (size_t)nz + 1);
   s->zon = zon;
@@ -1015,7 +1029,7 @@ int orc_adjoint(void *h, const double *z0, const double *p, double dt, int nstep
   double *ys = xmalloc((size_t)(nsteps + 1) * s->ynnz * sizeof(double) + 8);
   double *y_saved = xmalloc((size_t)s->ynnz * sizeof(double) + 8);
   memcpy(y_saved, s->yv, (size_t)s->ynnz * sizeof(double));
-  int nz = s->ndev[GP_ZIPLOAD];
+  int nz = s->ndev[GP_ZIPLOAD] + s->ndev[GP_SEXS] + s->ndev[GP_ESDC1A];   /* devices that can go offline */
   uint8_t *zons = xmalloc((size_t)(nsteps + 1) * (nz + 1));
   double st[4];
   int rc = sweep(s, z0, p, dt, nsteps, ev_bus, ev_r, ev_on, ev_off, n_ev, tol, itermax, traj, fgs, ys, zons, NULL, st);
@@ -1035,7 +1049,7 @@ int orc_adjoint(void *h, const double *z0, const double *p, double dt, int nstep
     kf = kf < 1 ? 1 : (kf > nsteps ? nsteps : kf);
     const double *zk = traj + (size_t)k * n, *zkm = traj + (size_t)(k - 1) * n, *fg = fgs + (size_t)kf * nb;
     memcpy(s->yv, ys + (size_t)kf * s->ynnz, (size_t)s->ynnz * sizeof(double));   /* Y of step kf */
-    s->zon = zons + (size_t)kf * (nz + 1);                                         /* its loads */
+    s->zon = zons + (size_t)kf * (nz + 1);                                         /* its devices */
     for (int i = 0; i < n; i++) a[i] = s->is_diff[i] ? lam[i] : 0.0;
     for (int r = 0; r < n_rows; r++) a[rows[r]] += 2.0 * scale * dt * (zk[rows[r]] - off);
     beuler_jac(s, zk, p, fg, dt, J);

@@ -138,8 +138,12 @@ class Oracle:
         lib().orc_set_trips(self.h, len(tr), _i(fr), _i(to), _i(on), _i(off), _d(y))
 
     def _set_drops(self, drops):
-        """drops: iterable of (ZIP table row, step)."""
-        d = list(drops)
+        """drops: iterable of (ZIP table row, step) or (kind, row, step) with
+        kind "zip", "sexs" or "esdc1a" (row of that device table)."""
+        from gradpower_hip.desc import GP_ESDC1A, GP_SEXS, GP_ZIPLOAD
+        nd = [len(self.desc.dev[t]) for t in (GP_ZIPLOAD, GP_SEXS, GP_ESDC1A)]
+        base = {"zip": 0, "sexs": nd[0], "esdc1a": nd[0] + nd[1]}
+        d = [(base[x[0]] + int(x[1]), int(x[2])) if len(x) == 3 else (int(x[0]), int(x[1])) for x in drops]
         k = np.array([x[0] for x in d] or [0], np.int32)
         st = np.array([x[1] for x in d] or [0], np.int32)
         lib().orc_set_drops(self.h, len(d), _i(k), _i(st))

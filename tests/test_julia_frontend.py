@@ -15,7 +15,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "gradpower_hip.h")
 JL = os.path.join(ROOT, "gradpower.jl_amd", "julia", "GradPowerHIP.jl")
 PAIRS = {"gp_system_desc": "GpSystemDesc", "gp_event": "GpEvent", "gp_line_trip": "GpLineTrip",
-         "gp_load_drop": "GpLoadDrop", "gp_options": "GpOptions", "gp_stats": "GpStats",
+         "gp_load_drop": "GpLoadDrop", "gp_device_drop": "GpDeviceDrop", "gp_options": "GpOptions",
+         "gp_stats": "GpStats",
          "gp_objective": "GpObjective", "gp_adjoint_stats": "GpAdjointStats",
          "gp_symbolic_info": "GpSymbolicInfo"}
 C_SCALARS = {"int32_t": 4, "int64_t": 8, "double": 8, "uint8_t": 1}
@@ -99,7 +100,8 @@ def test_python_ctypes_mirrors_match_header():
     import gradpower_hip.desc as d
     import gradpower_hip.lib as lib
     mirrors = {"gp_system_desc": d.CSystemDesc, "gp_event": d.CEvent, "gp_line_trip": d.CLineTrip,
-               "gp_load_drop": d.CLoadDrop, "gp_options": lib.COptions, "gp_stats": lib.CStats,
+               "gp_load_drop": d.CLoadDrop, "gp_device_drop": d.CDeviceDrop, "gp_options": lib.COptions,
+               "gp_stats": lib.CStats,
                "gp_objective": lib.CObjective, "gp_adjoint_stats": lib.CAdjStats,
                "gp_symbolic_info": lib.CSymInfo}
     cs = _c_structs()
