@@ -390,6 +390,8 @@ def main():
     ap.add_argument("--predictor", type=int, default=0,
                     help="first iterate of a step: 0 z_k (the reference's start), 1 linear extrapolation, "
                          "2 quadratic (under the Schur methods' policies)")
+    ap.add_argument("--solve-levels", type=int, default=0,
+                    help="levels per partition of the LU plans' triangular solves (0 default, 1 unpartitioned)")
     ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete-adjoint measurement")
     ap.add_argument("--no-c4", action="store_true", help="skip the 70,000-bus M=8 measurement")
@@ -422,7 +424,7 @@ def main():
     spol = {"async": REFACTOR_ASYNC, "adaptive": REFACTOR_ADAPTIVE}[args.policy]
     base = {"stall_window": args.stall_window, "refactor_theta": args.theta, "async_below": args.async_below,
             "profile": 0 if args.no_profile else SOLVE_TIMING_STRIDE, "retry_window": args.retry_window,
-            "predictor": args.predictor}
+            "predictor": args.predictor, "solve_levels": args.solve_levels}
     methods = args.methods.split(",")
     runs, mcode = {}, {}
     for name, meth, pol in (("schur32", METHOD_SCHUR_FP32, spol), ("schur64", METHOD_SCHUR_FP64, spol),
@@ -460,6 +462,7 @@ def main():
                    "newton_tol": tol, "itermax": itermax, "stall_window": args.stall_window,
                    "retry_window": args.retry_window or 8,
                    "refactor_policy": args.policy, "refactor_theta": args.theta, "predictor": args.predictor,
+                   "solve_levels": args.solve_levels or 4,
                    "horizon_cap": args.horizon_cap or None,
                    "deviations": {
                        "loads": f"constant impedance (ZIP alpha={case.meta['zip_alpha']}), not the 0.4/0.3/0.3 "

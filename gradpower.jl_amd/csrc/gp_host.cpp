@@ -140,7 +140,7 @@ struct LUPlan {
   SpluDev dev{};
   DBuf<int> a2lu, a_src, op_loc, op_sl, st_o, flp, fac_mode, crec, lu2s, dmap, fw_lvl, fw_chunk, fw_cols, fw_rows,
       bw_lvl, bw_chunk, bw_cols, bw_rows, q, pinv, sf_hdr, sf_blob, sb_hdr,
-      sb_blob;
+      sb_blob, cmp_ph, cmp_tp, cmp_dst, cmp_ops;
   // LDS-resident factorisation plans, one per factor precision (their
   // steps are cut at the precision's value staging capacity)
   struct Res {
@@ -165,6 +165,15 @@ struct LUPlan {
   static std::vector<int> nonempty(const std::vector<int> &v) { return v.empty() ? std::vector<int>{0} : v; }
   // LDS bytes of the solve's x (16-byte aligned)
   int x_bytes() const { return (int)(((size_t)A.n * 8 + 15) / 16 * 16); }
+  // Partitioned solves: up to part_k levels per partition (0 or 1: one
+  // level per step, unpartitioned); on when x, y and the staging area fit LDS
+  int part_k = 4;
+  bool part_on = false;
+  int x_vecs() const { return part_on ? 2 : 1; }
+  bool part_fits() const {
+    return !gx_needed() && 2LL * x_bytes() + (long long)kSolCB * kThreads * 4 +
+                                   (long long)(kSolCV + kSolCD) * kThreads * 8 + 16384 <= kLdsMax;
+  }
   // The solve keeps x in global memory when x and the staging area of FP64
   // values do not fit in LDS together (above about 12k rows)
   bool force_gx = false;   // gp_options.global_x
@@ -177,7 +186,8 @@ struct LUPlan {
   // Largest dense trailing block the solve's LDS allows (x, 8 bytes per row,
   // stays resident beside it unless it lives in global memory).
   int nd_bound() const {
-    const long long room = std::min<long long>(kLdsBudget, (long long)kLdsMax - (gx_needed() ? 0LL : 8LL * A.n) - 16384);
+    const long long room =
+        std::min<long long>(kLdsBudget, (long long)kLdsMax - (gx_needed() ? 0LL : 8LL * x_vecs() * A.n) - 16384);
     int nd_max = 0;
     while ((long long)(nd_max + 16) * (nd_max + 16) * 8 <= room) nd_max += 16;
     return nd_max;
@@ -185,8 +195,9 @@ struct LUPlan {
   // Schedules and uploads the analysis, the dense block at most nd_cap rows
   // (and what the solve's LDS allows).
   void upload(const std::vector<int> &asrc, int nd_cap = 1 << 30) {
-    splu_schedule(A, kThreads, std::min(nd_bound(), nd_cap));
-    if (!gx_needed() && x_bytes() + (long long)A.nd * A.nd * 8 > kLdsMax)
+    part_on = part_k > 1 && part_fits();
+    splu_schedule(A, kThreads, std::min(nd_bound(), nd_cap), part_on ? part_k : 0);
+    if (!gx_needed() && (long long)x_vecs() * x_bytes() + (long long)A.nd * A.nd * 8 > kLdsMax)
       throw GpError{GP_ERR_UNSUPPORTED, "system too large for the in-LDS triangular solve (" +
                                             std::to_string(A.n) + " rows)"};
     a2lu.upload(A.a2lu); a_src.upload(asrc);
@@ -201,6 +212,8 @@ struct LUPlan {
     bw_lvl.upload(nonempty(A.sb.lvl)); bw_chunk.upload(nonempty(A.sb.chunk));
     bw_cols.upload(nonempty(A.sb.cols)); bw_rows.upload(nonempty(A.sb.rows));
     q.upload(A.q); pinv.upload(A.pinv);
+    cmp_ph.upload(nonempty(A.cmp_ph)); cmp_tp.upload(nonempty(A.cmp_tp));
+    cmp_dst.upload(nonempty(A.cmp_dst)); cmp_ops.upload(nonempty(A.cmp_ops));
     if (gx_needed()) {   // the global-memory solve reads the level tables where they lie
       dev.sol_nfw = dev.sol_nbw = 0;
       dev.sol_fw_hdr = dev.sol_fw_blob = dev.sol_bw_hdr = dev.sol_bw_blob = nullptr;
@@ -247,6 +260,9 @@ struct LUPlan {
     d.fw_lvl = fw_lvl.p; d.fw_chunk = fw_chunk.p; d.fw_cols = fw_cols.p; d.fw_rows = fw_rows.p;
     d.bw_lvl = bw_lvl.p; d.bw_chunk = bw_chunk.p; d.bw_cols = bw_cols.p; d.bw_rows = bw_rows.p;
     d.q = q.p; d.pinv = pinv.p;
+    d.part = A.part > 1 ? 1 : 0;
+    d.cmp_nph = (int)A.cmp_ph.size() - 1;
+    d.cmp_ph = cmp_ph.p; d.cmp_tp = cmp_tp.p; d.cmp_dst = cmp_dst.p; d.cmp_ops = cmp_ops.p;
     d.res_xsrc = nullptr;   // set by shape() with the precision's plan
     d.threads = kThreads;
     d.gx = gx_needed() ? 1 : 0;
@@ -282,7 +298,7 @@ struct LUPlan {
     // solve: x, then the staging area, which the dense block reuses; with x
     // in global memory only the dense block
     dev.lds_solve = dev.gx ? (int)std::max<size_t>(16, (size_t)A.nd * A.nd * elem)
-                           : x_bytes() + (int)std::max<size_t>((size_t)kSolCB * kThreads * 4 +
+                           : x_vecs() * x_bytes() + (int)std::max<size_t>((size_t)kSolCB * kThreads * 4 +
                                                                    (size_t)(kSolCV + kSolCD) * kThreads * elem,
                                                                (size_t)A.nd * A.nd * elem);
     if (dev.lds_solve > kLdsMax)
@@ -2741,6 +2757,7 @@ int gp_create(const gp_system_desc *d, int32_t M, const gp_options *opt, gp_ctx 
     if (opt) C->opt = *opt;
     else { C->opt.record_all = 1; C->opt.refactor_policy = GP_REFACTOR_EVERY_ITER; }
     C->luJ.force_gx = C->luS.force_gx = C->luJT.force_gx = C->opt.global_x != 0;
+    C->luJ.part_k = C->luS.part_k = C->luJT.part_k = C->opt.solve_levels > 0 ? C->opt.solve_levels : 4;
     C->prof_solve.every = std::max(1, C->opt.profile);   // profile = k: every k-th solve launch timed
     C->is_diff.assign(d->is_diff, d->is_diff + d->n);
     C->yrp.assign(d->y_rowptr, d->y_rowptr + 2 * d->nbus + 1);

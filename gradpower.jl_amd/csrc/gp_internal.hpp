@@ -98,6 +98,12 @@ struct SpluDev {
   const int *fw_lvl, *fw_chunk, *fw_cols, *fw_rows;
   const int *bw_lvl, *bw_chunk, *bw_cols, *bw_rows;
   const int *q, *pinv;
+  // partitioned solves (part > 1): a second vector y in LDS, levels flagged
+  // A (a partition's rows from the values before it, x -> y forward, y -> x
+  // backward) or B (the other rows' entries in its columns); the compose
+  // program fills the A-level values after every factorisation
+  int part, cmp_nph;
+  const int *cmp_ph, *cmp_tp, *cmp_dst, *cmp_ops;
   int threads;                  // workgroup size of both kernels
   int t_cols;                   // LDS staging per thread (thread mode), 0 = none
   int w_col, w_ops, w_bytes;    // LDS staging per wavefront (wavefront mode)

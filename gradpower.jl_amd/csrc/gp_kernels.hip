@@ -2156,13 +2156,17 @@ __device__ __forceinline__ void sol_store(int *sb, TF *svl, TF *sdg, int tid, in
 
 // The passes of one staged step.  A pass record gives each wavefront one
 // chunk:
-// lane i (i < nl) sums vals[E + i + 64 j] * x[col] over j < pf, the g lanes
-// of a row add up, and the row's first lane updates x[row] (backward: and
-// divides by the diagonal).  The row and x[row] are read together with the
-// entries.  A barrier follows the last pass of each level.
+// lane i (i < nl) sums vals[E + i + 64 j] * v[col] over j < pf, the g lanes
+// of a row add up, and the row's first lane writes v[row] - sum (divided by
+// the diagonal where the level's flags have bit 1).  The row's value is
+// read together with the entries.  Unpartitioned, v is x throughout.
+// Partitioned, the residual vector r (x forward, y backward) and the final
+// one f (the other): an A-level (flag bit 0) reads r for both and writes
+// f[row], a B-level reads f at the columns and updates r[row].  A barrier
+// follows the last pass of each level.
 template <typename TF, bool BW>
 __device__ __forceinline__ void sol_step(const int *h, const int *sb, const TF *svl,
-                                         const TF *sdg, double *x, int tid) {
+                                         const TF *sdg, double *x, double *y, int tid) {
   const int np = ldu(h);
   const int *cols = sb + ldu(h + 8);
   const int *rows = sb + ldu(h + 9);
@@ -2179,9 +2183,13 @@ __device__ __forceinline__ void sol_step(const int *h, const int *sb, const TF *
       rc = L[9 * (k + 1) + 1 + w];
     }
     const int g = __builtin_amdgcn_readfirstlane(hdc.x), E = __builtin_amdgcn_readfirstlane(rcc.x);
+    const int fl = __builtin_amdgcn_readfirstlane(hdc.z);
     const int pf = __builtin_amdgcn_readfirstlane(rcc.y), rs = __builtin_amdgcn_readfirstlane(rcc.z);
     const int nl = __builtin_amdgcn_readfirstlane(rcc.w);
     if (nl > 0) {
+      double *rv = BW ? y : x, *fv = BW ? x : y;   // the same vector unpartitioned
+      const double *cv = (fl & 1) ? rv : fv;
+      double *ov = (fl & 1) ? fv : rv;
       const int uc = min(lane, nl - 1);
       const int base = E + uc;
       const int row = rows[rs + uc];
@@ -2196,8 +2204,8 @@ __device__ __forceinline__ void sol_step(const int *h, const int *sb, const TF *
 #pragma unroll
         for (int q = 0; q < 4; q++) vv[q] = svl[base + WAVE * min(j0 + q, pf - 1)];
 #pragma unroll
-        for (int q = 0; q < 4; q++) xv[q] = x[cc[q]];
-        if (j0 == 0) xr = x[row];
+        for (int q = 0; q < 4; q++) xv[q] = cv[cc[q]];
+        if (j0 == 0) xr = rv[row];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           const double t = (double)vv[q] * xv[q];
@@ -2206,13 +2214,58 @@ __device__ __forceinline__ void sol_step(const int *h, const int *sb, const TF *
       }
       acc = group_sum(acc, g);
       if (lane < nl && (lane & (g - 1)) == 0) {
-        if (BW)
-          x[row] = (xr - acc) / (double)sdg[rs + uc];
+        if (fl & 2)
+          ov[row] = (xr - acc) / (double)sdg[rs + uc];
         else
-          x[row] = xr - acc;
+          ov[row] = xr - acc;
       }
     }
     if (__builtin_amdgcn_readfirstlane(hdc.y)) lds_barrier();
+  }
+}
+
+// Composite values of the partitioned solves after a factorisation (one
+// workgroup per scenario whose refac flag is set, the buffer the
+// factorisation wrote).  The phases run in order, one thread per entry: its
+// terms are taken four at a time with their loads in flight together and
+// added in their fixed order (the result does not depend on timing).  A
+// term reads raw entries, diagonals and composites of earlier phases of the
+// same buffer.
+template <typename TF>
+__global__ void __launch_bounds__(512) k_splu_compose(SpluDev S, TF *__restrict__ svv,
+                                                      const uint8_t *__restrict__ refac,
+                                                      const uint8_t *__restrict__ tgt) {
+  const int m = scenario_of_block(blockIdx.x, gridDim.x);
+  if (!refac[m]) return;
+  const int bsel = tgt ? tgt[m] : 0;
+  TF *v = svv + ((size_t)bsel * gridDim.x + m) * S.ns;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int p = 0; p < S.cmp_nph; p++) {
+    const int e0 = S.cmp_ph[p], e1 = S.cmp_ph[p + 1];
+    for (int e = e0 + tid; e < e1; e += nt) {
+      const int o0 = S.cmp_tp[e], o1 = S.cmp_tp[e + 1];
+      double acc = 0.0;
+      for (int o = o0; o < o1; o += 4) {
+        int4 q[4];
+        TF va[4], vb[4], vd[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = *reinterpret_cast<const int4 *>(S.cmp_ops + 4 * min(o + u, o1 - 1));
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          va[u] = v[q[u].y];
+          vb[u] = v[max(q[u].z, 0)];
+          vd[u] = v[max(q[u].w, 0)];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (o + u < o1) {
+            double t = q[u].w >= 0 ? (double)va[u] / (double)vd[u] : (double)va[u];
+            acc += q[u].z >= 0 ? -t * (double)vb[u] : t;
+          }
+      }
+      v[S.cmp_dst[e]] = (TF)acc;
+    }
+    __syncthreads();
   }
 }
 
@@ -2267,10 +2320,12 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   const int bsel = cur ? cur[m] : 0;
   const int fm = shared ? 0 : m;   // shared: one factorisation (slot 0) serves every scenario
   const int tid = threadIdx.x, nt = blockDim.x;
-  // LDS: x, then the staging area (blob, values, diagonals), which the dense
-  // block reuses between the two sides
+  // LDS: x (and, partitioned, y), then the staging area (blob, values,
+  // diagonals), which the dense block reuses between the two sides
+  const size_t xb = ((size_t)n * 8 + 15) & ~(size_t)15;
   double *x = reinterpret_cast<double *>(smem_raw);
-  char *area = smem_raw + (((size_t)n * 8 + 15) & ~(size_t)15);
+  double *y = S.part ? reinterpret_cast<double *>(smem_raw + xb) : x;
+  char *area = smem_raw + (S.part ? 2 : 1) * xb;
   int *sb = reinterpret_cast<int *>(area);
   TF *svl = reinterpret_cast<TF *>(area + (size_t)kSolCB * nt * 4);
   TF *sdg = svl + (size_t)kSolCV * nt;
@@ -2302,7 +2357,7 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   __syncthreads();
   for (int st = 0; st < nfs; st++) {
     if (st + 1 < nfs) sol_fetch<TF>(S.sol_fw_hdr, S.sol_fw_blob, st + 1, fv, bd, tid, nt, P, pv, pd);
-    sol_step<TF, false>(S.sol_fw_hdr + kSolHdr * st, sb, svl, sdg, x, tid);
+    sol_step<TF, false>(S.sol_fw_hdr + kSolHdr * st, sb, svl, sdg, x, y, tid);
     if (st + 1 < nfs) {
       sol_store<TF>(sb, svl, sdg, tid, nt, P, pv, pd);
       lds_barrier();
@@ -2368,7 +2423,7 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   }
   for (int st = 0; st < nbs; st++) {
     if (st + 1 < nbs) sol_fetch<TF>(S.sol_bw_hdr, S.sol_bw_blob, st + 1, bv, bd, tid, nt, P, pv, pd);
-    sol_step<TF, true>(S.sol_bw_hdr + kSolHdr * st, sb, svl, sdg, x, tid);
+    sol_step<TF, true>(S.sol_bw_hdr + kSolHdr * st, sb, svl, sdg, x, y, tid);
     if (st + 1 < nbs) {
       sol_store<TF>(sb, svl, sdg, tid, nt, P, pv, pd);
       lds_barrier();
@@ -2935,12 +2990,15 @@ void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *
     allow_lds(reinterpret_cast<const void *>(k_splu_factor_res<TF>), done_res, S.lds_factor);
     hipLaunchKernelGGL(k_splu_factor_res<TF>, dim3(M), dim3(S.threads), S.lds_factor, s, S, var, cst, ldM, luv, sv,
                        dn, refac, tgt, status);
-    return;
+  } else {
+    static std::atomic<unsigned> done_fac{0};
+    allow_lds(reinterpret_cast<const void *>(k_splu_factor<TF>), done_fac, S.lds_factor);
+    hipLaunchKernelGGL(k_splu_factor<TF>, dim3(M), dim3(S.threads), S.lds_factor, s, S, var, cst, ldM, luv, sv, dn,
+                       refac, tgt, status);
   }
-  static std::atomic<unsigned> done_fac{0};
-  allow_lds(reinterpret_cast<const void *>(k_splu_factor<TF>), done_fac, S.lds_factor);
-  hipLaunchKernelGGL(k_splu_factor<TF>, dim3(M), dim3(S.threads), S.lds_factor, s, S, var, cst, ldM, luv, sv, dn,
-                     refac, tgt, status);
+  if (S.cmp_nph > 0) {   // the partitioned solves' composites, from the values just written
+    hipLaunchKernelGGL(k_splu_compose<TF>, dim3(M), dim3(S.threads), 0, s, S, sv, refac, tgt);
+  }
 }
 template <typename TF>
 void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const double *rhs, int ldM, double sign,

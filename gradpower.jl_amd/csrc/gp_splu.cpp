@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <queue>
 #include <utility>
 
@@ -387,7 +388,7 @@ long long wave_col_cost(long long nops, long long steps) { return 4000 + 40 * no
 long long dense_cost(int nd) { return nd ? 6000 + 400LL * nd : 0; }
 }  // namespace
 
-void splu_schedule(SpluAnalysis &A, int threads, int nd_max) {
+void splu_schedule(SpluAnalysis &A, int threads, int nd_max, int part_k) {
   const int n = A.n, W = threads / 64;
   const int nl = (int)A.fac_lp.size() - 1;
   // steps: maximal runs of ops with the same source entry, cut at 64
@@ -563,29 +564,27 @@ void splu_schedule(SpluAnalysis &A, int threads, int nd_max) {
   // padded to its longest lane: entry j of lane u of chunk c sits at
   // E_c + 64 j + (u mod 64).  Every address a lane reads is then known
   // without reading the matrix, so a level's loads are issued during the
-  // previous level.
+  // previous level.  An entry's source is an LU position (lu2s maps it to
+  // its slot) or, partitioned, a composite value (-1 - id; cslot gets its
+  // slot).
   A.lu2s.assign(A.nnz_lu, -1);
-  // stream index -> LU position (streams were laid out from lrow/urow order)
-  std::vector<int> lpos(A.nnz_l), upos(A.nnz_u);
-  for (int i = 0; i < n; i++) {
-    int sb = lb[i];
-    for (int e = A.lrow_p[i]; e < A.lrow_p[i + 1]; e++) lpos[sb++] = A.lrow_pos[e];
-    sb = ub[i];
-    for (int e = A.urow_p[i]; e < A.urow_p[i + 1]; e++) upos[sb++] = A.urow_pos[e];
-  }
-  auto build = [&](const std::vector<int> &lp, const std::vector<int> &info, const std::vector<int> &scol,
-                   const std::vector<int> &spos, SpluAnalysis::SolveSide &S, int base) {
+  struct Lrow {
+    int row;
+    std::vector<std::pair<int, int>> ent;   // (column, source)
+  };
+  std::vector<int> cslot;
+  auto build = [&](const std::vector<std::vector<Lrow>> &levels, const std::vector<int> &flags,
+                   SpluAnalysis::SolveSide &S, int base, std::vector<int> *lane_of_a) {
     S = SpluAnalysis::SolveSide{};
-    const int nlev = (int)lp.size() - 1;
     int R = 0;
-    for (int l = 0; l < nlev; l++) {
-      std::vector<int> slots;
-      for (int i = lp[l]; i < lp[l + 1]; i++) slots.push_back(i);
-      std::stable_sort(slots.begin(), slots.end(), [&](int a, int b) {
-        return info[3 * a + 2] - info[3 * a + 1] > info[3 * b + 2] - info[3 * b + 1];
-      });
-      const int nr = (int)slots.size();
-      const int maxlen = nr ? info[3 * slots[0] + 2] - info[3 * slots[0] + 1] : 0;
+    for (size_t l = 0; l < levels.size(); l++) {
+      std::vector<const Lrow *> rows;
+      for (const Lrow &r : levels[l]) rows.push_back(&r);
+      std::stable_sort(rows.begin(), rows.end(),
+                       [](const Lrow *a, const Lrow *b) { return a->ent.size() > b->ent.size(); });
+      const int nr = (int)rows.size();
+      if (nr == 0) continue;
+      const int maxlen = (int)rows[0]->ent.size();
       int g = 1;
       long long bestc = -1;
       for (int gg = 1, lg = 0; gg <= 64; gg *= 2, lg++) {
@@ -596,10 +595,11 @@ void splu_schedule(SpluAnalysis &A, int threads, int nd_max) {
       const int T = nr * g;
       const int nch = (T + 63) / 64;
       S.lvl.insert(S.lvl.end(), {T, g, R, (int)S.chunk.size() / 2});
+      S.lflag.push_back(flags[l]);
       for (int c = 0; c < nch; c++) {
         // longest lane of the chunk: its first row is the longest
         const int r0 = (64 * c) / g;
-        const int len0 = info[3 * slots[r0] + 2] - info[3 * slots[r0] + 1];
+        const int len0 = (int)rows[r0]->ent.size();
         const int pf = std::max(1, (len0 + g - 1) / g);   // >= 1: every lane has a readable slot
         const int E = S.nvals;
         S.chunk.insert(S.chunk.end(), {E, pf});
@@ -607,30 +607,236 @@ void splu_schedule(SpluAnalysis &A, int threads, int nd_max) {
         S.cols.resize(S.nvals, 0);
         for (int u = 64 * c; u < std::min(T, 64 * c + 64); u++) {
           const int r = u / g, sub = u % g;
-          const int slot = slots[r];
-          for (int e = info[3 * slot + 1] + sub, j = 0; e < info[3 * slot + 2]; e += g, j++) {
-            const int at = E + 64 * j + (u - 64 * c);
-            S.cols[at] = scol[e];
-            A.lu2s[spos[e]] = base + at;
+          const auto &ent = rows[r]->ent;
+          for (int e = sub, jj = 0; e < (int)ent.size(); e += g, jj++) {
+            const int at = E + 64 * jj + (u - 64 * c);
+            S.cols[at] = ent[e].first;
+            const int src = ent[e].second;
+            if (src >= 0)
+              A.lu2s[src] = base + at;
+            else
+              cslot[-1 - src] = base + at;
           }
         }
       }
-      for (int u = 0; u < T; u++) S.rows.push_back(info[3 * slots[u / g]]);
+      for (int u = 0; u < T; u++) {
+        S.rows.push_back(rows[u / g]->row);
+        if (lane_of_a && flags[l] && u % g == 0) (*lane_of_a)[rows[u / g]->row] = R + u;
+      }
       R += T;
     }
   };
-  build(A.sfw_lp, A.sfw_info, A.ls_k, lpos, A.sf, 0);
-  build(A.sbw_lp, A.sbw_info, A.us_j, upos, A.sb, A.sf.nvals);
-  // diagonal of row k stored per backward lane (every lane of the row's group)
-  A.ns = A.sf.nvals + A.sb.nvals + (int)A.sb.rows.size();
-  {
-    // the group's first lane carries the diagonal
+  // stream index -> LU position (streams were laid out from lrow/urow order)
+  std::vector<int> lpos(A.nnz_l), upos(A.nnz_u);
+  for (int i = 0; i < n; i++) {
+    int sb = lb[i];
+    for (int e = A.lrow_p[i]; e < A.lrow_p[i + 1]; e++) lpos[sb++] = A.lrow_pos[e];
+    sb = ub[i];
+    for (int e = A.urow_p[i]; e < A.urow_p[i + 1]; e++) upos[sb++] = A.urow_pos[e];
+  }
+  A.part = 0;
+  A.cmp_ph.assign(1, 0);
+  A.cmp_tp.assign(1, 0);
+  A.cmp_dst.clear();
+  A.cmp_ops.clear();
+  int nraw = 0;
+  if (part_k < 2) {
+    std::vector<std::vector<Lrow>> lf, lbk;
+    for (int l = 0; l + 1 < (int)flp.size(); l++) {
+      lf.emplace_back();
+      for (int s = flp[l]; s < flp[l + 1]; s++) {
+        Lrow x{finfo[3 * s], {}};
+        for (int e = finfo[3 * s + 1]; e < finfo[3 * s + 2]; e++) x.ent.push_back({A.ls_k[e], lpos[e]});
+        lf.back().push_back(std::move(x));
+      }
+    }
+    for (int l = 0; l + 1 < (int)blp.size(); l++) {
+      lbk.emplace_back();
+      for (int s = blp[l]; s < blp[l + 1]; s++) {
+        Lrow x{binfo[3 * s], {}};
+        for (int e = binfo[3 * s + 1]; e < binfo[3 * s + 2]; e++) x.ent.push_back({A.us_j[e], upos[e]});
+        lbk.back().push_back(std::move(x));
+      }
+    }
+    build(lf, std::vector<int>(lf.size(), 0), A.sf, 0, nullptr);
+    build(lbk, std::vector<int>(lbk.size(), 2), A.sb, A.sf.nvals, nullptr);   // backward rows divide
+    // diagonal of row k stored per backward lane (the group's first lane carries it)
     const int dbase = A.sf.nvals + A.sb.nvals;
     std::vector<int> first(n, -1);
     for (int u = (int)A.sb.rows.size() - 1; u >= 0; u--) first[A.sb.rows[u]] = u;
     for (int k = 0; k < B; k++)
       if (first[k] >= 0) A.lu2s[A.lu_dp[k]] = dbase + first[k];
+  } else {
+    // Partitioned solves: the levels of rows < B are grouped part_k at a
+    // time (fewer where a row's chain grows long).  Within partition P a
+    // forward row i is finished from the values before P alone:
+    //   y[i] = x[i] - sum_{j in reach(i)} V(i,j) x[j],
+    //   V(i,j) = [j in dep(i)] L(i,j) - sum_{k in dep(i) & P} L(i,k) V(k,j),
+    // (reach(i): the rows of P that i depends on, transitively), written to
+    // the second vector y (A-level); the rows outside P then take their own
+    // entries in P's columns from y (B-level).  Backward alike with
+    //   W(i,j) = [j in dep(i)] U(i,j)/U(j,j) - sum_k U(i,k)/U(k,k) W(k,j)
+    // and the division by U(i,i), from y (the forward result, updated in
+    // place) into x.  A factorisation computes the composites V, W from the
+    // raw entries inside the partitions (the compose program, phases by depth
+    // inside a partition).
+    A.part = part_k;
+    std::vector<int> rawof(A.nnz_lu, -1);   // LU position of an entry inside a partition -> raw index
+    struct Comp { int a, b, d, cid_b; };     // one term: raw a, composite cid_b (-1: direct), diag row d (-1)
+    std::vector<std::vector<Comp>> cterms;   // per composite id
+    std::vector<int> cdepth;                 // per composite id: depth of its row in the partition
+    std::vector<int> a_row_of_diag;          // per composite term: row whose diagonal divides (bw), via lane map
+    auto side = [&](bool fw, std::vector<std::vector<Lrow>> &levels, std::vector<int> &flags) {
+      // rows < B by level
+      std::vector<int> lev(n, -1);
+      int nlev = 0;
+      const std::vector<int> &lp = fw ? flp : blp;
+      const std::vector<int> &inf = fw ? finfo : binfo;
+      for (int l = 0; l + 1 < (int)lp.size(); l++)
+        for (int s = lp[l]; s < lp[l + 1]; s++)
+          if (inf[3 * s] < B) { lev[inf[3 * s]] = l; nlev = std::max(nlev, l + 1); }
+      std::vector<std::vector<int>> byl(nlev);
+      for (int r = 0; r < B; r++)
+        if (lev[r] >= 0) byl[lev[r]].push_back(r);
+      // dependencies (column, LU position) of row r in the sparse part
+      auto deps = [&](int r) {
+        std::vector<std::pair<int, int>> d;
+        if (fw) {
+          for (int e = A.lrow_p[r]; e < A.lrow_p[r + 1]; e++) d.push_back({A.lrow_k[e], A.lrow_pos[e]});
+        } else {
+          for (int e = A.urow_p[r]; e < A.urow_p[r + 1]; e++)
+            if (A.urow_j[e] < B) d.push_back({A.urow_j[e], A.urow_pos[e]});
+        }
+        return d;
+      };
+      if (!fw) {   // the rows' entries in the dense block's columns (final before the sparse part)
+        std::vector<Lrow> lv0;
+        for (int r = 0; r < B; r++) {
+          Lrow x{r, {}};
+          for (int e = A.urow_p[r]; e < A.urow_p[r + 1]; e++)
+            if (A.urow_j[e] >= B) x.ent.push_back({A.urow_j[e], A.urow_pos[e]});
+          if (!x.ent.empty()) lv0.push_back(std::move(x));
+        }
+        if (!lv0.empty()) { levels.push_back(std::move(lv0)); flags.push_back(0); }
+      }
+      std::vector<int> pid(n, -1);
+      std::vector<std::vector<int>> reach(n);   // sorted rows of the partition
+      int l0 = 0, np = 0;
+      while (l0 < nlev) {
+        // grow the partition level by level while the chains stay short
+        int l1 = l0 + 1;
+        for (int r : byl[l0]) { pid[r] = np; reach[r].clear(); }
+        while (l1 < nlev && l1 - l0 < part_k) {
+          bool ok = true;
+          std::vector<std::pair<int, std::vector<int>>> add;
+          for (int r : byl[l1]) {
+            std::vector<int> rc;
+            for (auto &d : deps(r))
+              if (pid[d.first] == np) {
+                rc.push_back(d.first);
+                rc.insert(rc.end(), reach[d.first].begin(), reach[d.first].end());
+              }
+            std::sort(rc.begin(), rc.end());
+            rc.erase(std::unique(rc.begin(), rc.end()), rc.end());
+            if ((int)rc.size() > 48) { ok = false; break; }
+            add.push_back({r, std::move(rc)});
+          }
+          if (!ok) break;
+          for (auto &x : add) { pid[x.first] = np; reach[x.first] = std::move(x.second); }
+          l1++;
+        }
+        // A-level: the partition's rows; composites in depth order
+        std::vector<Lrow> al;
+        for (int l = l0; l < l1; l++)
+          for (int r : byl[l]) {
+            Lrow x{r, {}};
+            for (int j : reach[r]) {
+              const int cid = (int)cterms.size();
+              cterms.emplace_back();
+              cdepth.push_back(l - l0);
+              x.ent.push_back({j, -1 - cid});
+            }
+            al.push_back(std::move(x));
+          }
+        // composite terms (ids of a row's reach in order)
+        {
+          std::map<std::pair<int, int>, int> cid_of;
+          for (auto &x : al)
+            for (auto &e : x.ent) cid_of[{x.row, e.first}] = -1 - e.second;
+          for (auto &x : al)
+            for (auto &e : x.ent) {
+              const int j = e.first, cid = -1 - e.second;
+              for (auto &d : deps(x.row)) {
+                const int k = d.first;
+                if (pid[k] != np) continue;
+                int ra = rawof[d.second];
+                if (ra < 0) ra = rawof[d.second] = nraw++;
+                if (k == j) cterms[cid].push_back({ra, -1, fw ? -1 : k, -1});
+                else if (std::binary_search(reach[k].begin(), reach[k].end(), j))
+                  cterms[cid].push_back({ra, -1, fw ? -1 : k, cid_of.at({k, j})});
+              }
+            }
+        }
+        levels.push_back(std::move(al));
+        flags.push_back(fw ? 1 : 3);   // A-level (backward: dividing by the row's diagonal)
+        // B-level: later rows with entries in the partition's columns
+        std::vector<Lrow> blv;
+        auto outside = [&](int r) {
+          Lrow x{r, {}};
+          if (fw) {
+            for (int e = A.lrow_p[r]; e < A.lrow_p[r + 1]; e++)
+              if (A.lrow_k[e] < B && pid[A.lrow_k[e]] == np) x.ent.push_back({A.lrow_k[e], A.lrow_pos[e]});
+          } else {
+            for (int e = A.urow_p[r]; e < A.urow_p[r + 1]; e++)
+              if (A.urow_j[e] < B && pid[A.urow_j[e]] == np) x.ent.push_back({A.urow_j[e], A.urow_pos[e]});
+          }
+          if (!x.ent.empty()) blv.push_back(std::move(x));
+        };
+        for (int l = l1; l < nlev; l++)
+          for (int r : byl[l]) outside(r);
+        if (fw)
+          for (int r = B; r < n; r++) outside(r);
+        if (!blv.empty()) { levels.push_back(std::move(blv)); flags.push_back(0); }
+        np++;
+        l0 = l1;
+      }
+    };
+    std::vector<std::vector<Lrow>> lf, lbk;
+    std::vector<int> ff, fb;
+    side(true, lf, ff);
+    const int ncf = (int)cterms.size();
+    side(false, lbk, fb);
+    cslot.assign(cterms.size(), -1);
+    std::vector<int> alane(n, -1);
+    build(lf, ff, A.sf, 0, nullptr);
+    build(lbk, fb, A.sb, A.sf.nvals, &alane);
+    const int dbase = A.sf.nvals + A.sb.nvals;
+    for (int k = 0; k < B; k++)
+      if (alane[k] >= 0) A.lu2s[A.lu_dp[k]] = dbase + alane[k];
+    const int rawbase = dbase + (int)A.sb.rows.size();
+    for (int p = 0; p < A.nnz_lu; p++)
+      if (rawof[p] >= 0) A.lu2s[p] = rawbase + rawof[p];
+    // compose program: phases by depth (forward and backward composites
+    // together), per entry its terms (4 ints: 0, a, b, d)
+    int maxd = 0;
+    for (int c = 0; c < (int)cterms.size(); c++) maxd = std::max(maxd, cdepth[c]);
+    A.cmp_tp.assign(1, 0);
+    for (int d = 1; d <= maxd; d++) {
+      for (int c = 0; c < (int)cterms.size(); c++) {
+        if (cdepth[c] != d) continue;
+        A.cmp_dst.push_back(cslot[c]);
+        for (const Comp &t : cterms[c]) {
+          const int dg = t.d >= 0 ? dbase + alane[t.d] : -1;
+          A.cmp_ops.insert(A.cmp_ops.end(), {0, rawbase + t.a, t.cid_b >= 0 ? cslot[t.cid_b] : -1, dg});
+        }
+        A.cmp_tp.push_back((int)A.cmp_ops.size() / 4);
+      }
+      A.cmp_ph.push_back((int)A.cmp_dst.size());
+    }
+    (void)ncf;
   }
+  A.nraw = nraw;
+  A.ns = A.sf.nvals + A.sb.nvals + (int)A.sb.rows.size() + nraw;
   // the dense block's own entries are read from the dense copy, not the streams
   for (int k = B; k < n; k++)
     for (int q = A.lu_cp[k]; q < A.lu_cp[k + 1]; q++)
@@ -929,7 +1135,7 @@ bool solve_plan(const SpluAnalysis::SolveSide &S, bool diag, int dg_base, int bc
       const bool chain = single && k + 1 < j && subs[k + 1].c1 - subs[k + 1].c0 == 1;
       // passes of 8 chunks (one per wavefront); a barrier after the last
       for (int c = sb.c0; c < sb.c1; c += 8, nlp++) {
-        lv.insert(lv.end(), {g, (c + 8 >= sb.c1 && !chain) ? 1 : 0, 0, 0});
+        lv.insert(lv.end(), {g, (c + 8 >= sb.c1 && !chain) ? 1 : 0, S.lflag.empty() ? 0 : S.lflag[l], 0});
         for (int w = 0; w < 8; w++) {
           const int cw = c + w;
           if (cw < sb.c1) {

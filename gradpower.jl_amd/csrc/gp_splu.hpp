@@ -80,20 +80,31 @@ struct SpluAnalysis {
   // Sliced-ELL solve layout (forward side sf over L, backward side sb over U).
   struct SolveSide {
     std::vector<int> lvl;     // 4 ints per level: lanes T, lanes per row g, first lane R, first chunk
+    std::vector<int> lflag;   // per level: bit 0 A-level of a partition (partitioned solves), bit 1 divide
     std::vector<int> chunk;   // 2 ints per chunk of 64 lanes: first value E, entries per lane pf
     std::vector<int> cols;    // [nvals] column of each value slot (0 in padding)
     std::vector<int> rows;    // per lane (all levels): its row
     int nvals = 0;
   };
   SolveSide sf, sb;
-  int ns = 0;                        // solve values per scenario: sf values, sb values, sb diagonals
+  // Partitioned solves (part > 1: up to part levels per partition).  The
+  // A-level values are composites the compose program computes after each
+  // factorisation from the raw entries inside the partitions (nraw slots
+  // after the diagonals): phase p covers entries [cmp_ph[p], cmp_ph[p+1]);
+  // entry e writes solve slot cmp_dst[e], the sum of its terms
+  // [cmp_tp[e], cmp_tp[e+1]), 4 ints each (0, a, b, d):
+  // v[a] / (d >= 0 ? v[d] : 1) for b < 0, -v[a] / (d >= 0 ? v[d] : 1) * v[b]
+  // otherwise.
+  int part = 0, nraw = 0;
+  std::vector<int> cmp_ph, cmp_tp, cmp_dst, cmp_ops;
+  int ns = 0;                        // solve values per scenario: sf values, sb values, sb diagonals, raw
   int t_cols_max = 0;                // largest column in a thread-mode level
   int w_col_max = 0, w_ops_max = 0;  // largest column / op count in a wavefront-mode level
 };
 
 // Build the launch schedule for workgroups of `threads` threads, with a
 // dense trailing block of at most nd_max pivots.
-void splu_schedule(SpluAnalysis &A, int threads, int nd_max = 112);
+void splu_schedule(SpluAnalysis &A, int threads, int nd_max = 112, int part_k = 0);
 
 // Analyse the CSC matrix (cp, ri, representative values ax).  With extra
 // representative matrices of the same pattern (the line trips make one per
@@ -147,9 +158,10 @@ struct ResPlan {
 // than that is cut into sub-levels of whole chunks (its rows are
 // independent).  A (sub-)level runs in passes of 8 chunks, one per
 // wavefront.  Step blob, in dwords: 36 per pass (lanes per row g, 1 if a
-// barrier follows, 0, 0, then per wavefront: first value of its chunk
-// relative to the step, entries per lane pf, first lane relative to the
-// step, active lanes (0: idle)), the step's value columns, then its lanes'
+// barrier follows, the level's flags (SolveSide::lflag), 0, then per
+// wavefront: first value of its chunk relative to the step, entries per
+// lane pf, first lane relative to the step, active lanes (0: idle)), the
+// step's value columns, then its lanes'
 // rows.  The step's values (and, for the backward side, diagonals) are the
 // ranges [val_off, + val_len) and [dg_off, + dg_len) of the solve array.
 // Step header (kSolHdr ints): passes, blob offset, blob length, val_off,

@@ -38,7 +38,8 @@ class COptions(C.Structure):
                 ("stall_window", C.c_int32), ("profile", C.c_int32), ("lock_step", C.c_int32),
                 ("dense_block", C.c_int32), ("retry_window", C.c_int32), ("global_x", C.c_int32),
                 ("refactor_theta", C.c_double),
-                ("async_below", C.c_double), ("predictor", C.c_int32), ("pivot_rule", C.c_int32)]
+                ("async_below", C.c_double), ("predictor", C.c_int32), ("pivot_rule", C.c_int32),
+                ("solve_levels", C.c_int32)]
 
 
 class CStats(C.Structure):
@@ -142,7 +143,7 @@ class HipContext:
     def __init__(self, desc: SystemDesc, M: int, record_idx=None,
                  refactor_policy=REFACTOR_EVERY_ITER, stall_window=0, refactor_theta=0.0, async_below=0.0,
                  profile=False, lock_step=False, dense_block=0, retry_window=0, global_x=False,
-                 predictor=0, pivot_rule=0):
+                 predictor=0, pivot_rule=0, solve_levels=0):
         self.lib = load()
         self.desc = desc
         self.M = int(M)
@@ -169,6 +170,7 @@ class HipContext:
         opt.global_x = 1 if global_x else 0
         opt.predictor = int(predictor)
         opt.pivot_rule = int(pivot_rule)
+        opt.solve_levels = int(solve_levels)
         self._opt = opt
         h = C.c_void_p()
         self._check(self.lib.gp_create(C.byref(self._cs), self.M, C.byref(opt), C.byref(h)))

@@ -94,6 +94,7 @@ struct GpOptions
     async_below::Float64
     predictor::Int32
     pivot_rule::Int32
+    solve_levels::Int32
 end
 
 struct GpStats
@@ -310,7 +311,7 @@ function HipBatchedLayout(dp::GradPower.DynamicProblem, ps::GradPower.PowerSyste
                             ntuple(i -> pointer(t.dev[i]), 8), pointer(t.u0), t.n_clusters, pointer(t.clusters))
         opt = GpOptions(record_idx === nothing ? 1 : 0, length(rec), isempty(rec) ? C_NULL : pointer(rec),
                         Int32(refactor_policy), 0, 0, 0, 0, Int32(retry_window), Int32(0), 0.0, 0.0,
-                        Int32(predictor), Int32(0))
+                        Int32(predictor), Int32(0), Int32(0))
         check(ccall((:gp_create, LIB), Cint, (Ref{GpSystemDesc}, Int32, Ref{GpOptions}, Ref{Ptr{Cvoid}}),
                     desc, Int32(M), opt, h))
     end

@@ -269,6 +269,13 @@ typedef struct gp_options {
                               continues on a pivot sequence chosen for its own
                               matrix (gp_stats.repivots); lock step and the
                               adjoint report GP_ERR_SINGULAR_PIVOT instead    */
+  int32_t solve_levels;    /* triangular solves of the batched LU plans whose
+                              solution fits in LDS: 0 (default) partitioned,
+                              up to 4 levels of the elimination per partition
+                              step; 1 one level per step; k > 1 up to k levels
+                              per partition.  A partition's rows are finished
+                              from the values before it through composite
+                              entries that each factorisation computes */
 } gp_options;
 
 typedef struct gp_stats {
