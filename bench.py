@@ -462,7 +462,7 @@ def main():
                    "newton_tol": tol, "itermax": itermax, "stall_window": args.stall_window,
                    "retry_window": args.retry_window or 8,
                    "refactor_policy": args.policy, "refactor_theta": args.theta, "predictor": args.predictor,
-                   "solve_levels": args.solve_levels or 4,
+                   "solve_levels": args.solve_levels or "autotuned",
                    "horizon_cap": args.horizon_cap or None,
                    "deviations": {
                        "loads": f"constant impedance (ZIP alpha={case.meta['zip_alpha']}), not the 0.4/0.3/0.3 "
@@ -485,6 +485,7 @@ def main():
                   "repivots": st["repivots"], "status": st["status"],
                   "failed_scenarios": int(np.sum(r["status"] != 0)),
                   "lu_dense_block": r["info"]["nd_j" if k == "mono" else "nd_s"],
+                  "lu_solve_levels": r["info"]["solve_levels_j" if k == "mono" else "solve_levels_s"],
                   "lu_factor_lds_resident": bool(r["info"]["res_j" if k == "mono" else "res_s"])}
         if not args.no_profile and k != head_name:
             out[k]["roofline"] = roofline(r, mcode[k])

@@ -139,8 +139,13 @@ struct LUPlan {
   SpluAnalysis A;
   SpluDev dev{};
   DBuf<int> a2lu, a_src, op_loc, op_sl, st_o, flp, fac_mode, crec, lu2s, dmap, fw_lvl, fw_chunk, fw_cols, fw_rows,
-      bw_lvl, bw_chunk, bw_cols, bw_rows, q, pinv, sf_hdr, sf_blob, sb_hdr,
-      sb_blob, cmp_ph, cmp_tp, cmp_dst, cmp_ops;
+      bw_lvl, bw_chunk, bw_cols, bw_rows, q, pinv, cmp_ph, cmp_tp, cmp_dst, cmp_ops;
+  // staged-solve plans, one per factor precision (their steps hold
+  // sol_cv(elem) values per thread)
+  struct Sol {
+    DBuf<int> fh, fb, bh, bb;
+    int nf = 0, nb = 0;
+  } sol[2];   // [0] FP32, [1] FP64
   // LDS-resident factorisation plans, one per factor precision (their
   // steps are cut at the precision's value staging capacity)
   struct Res {
@@ -168,19 +173,20 @@ struct LUPlan {
   // Partitioned solves: up to part_k levels per partition (0 or 1: one
   // level per step, unpartitioned); on when x, y and the staging area fit LDS
   int part_k = 4;
+  bool part_tune = true;   // the autotuner also picks part_k (gp_options.solve_levels 0)
   bool part_on = false;
   int x_vecs() const { return part_on ? 2 : 1; }
-  bool part_fits() const {
-    return !gx_needed() && 2LL * x_bytes() + (long long)kSolCB * kThreads * 4 +
-                                   (long long)(kSolCV + kSolCD) * kThreads * 8 + 16384 <= kLdsMax;
+  // LDS bytes of a solve step's staging area (blob, values, diagonals)
+  static long long stage_bytes(int elem) {
+    return (long long)kSolCB * kThreads * 4 + (long long)(sol_cv(elem) + kSolCD) * kThreads * elem;
   }
+  static long long stage_max() { return std::max(stage_bytes(4), stage_bytes(8)); }
+  bool part_fits() const { return !gx_needed() && 2LL * x_bytes() + stage_max() + 16384 <= kLdsMax; }
   // The solve keeps x in global memory when x and the staging area of FP64
   // values do not fit in LDS together (above about 12k rows)
   bool force_gx = false;   // gp_options.global_x
   bool gx_needed() const {
-    return force_gx || (long long)x_bytes() + (long long)kSolCB * kThreads * 4 +
-                               (long long)(kSolCV + kSolCD) * kThreads * 8 >
-                           kLdsMax;
+    return force_gx || (long long)x_bytes() + stage_max() > kLdsMax;
   }
 
   // Largest dense trailing block the solve's LDS allows (x, 8 bytes per row,
@@ -214,24 +220,23 @@ struct LUPlan {
     q.upload(A.q); pinv.upload(A.pinv);
     cmp_ph.upload(nonempty(A.cmp_ph)); cmp_tp.upload(nonempty(A.cmp_tp));
     cmp_dst.upload(nonempty(A.cmp_dst)); cmp_ops.upload(nonempty(A.cmp_ops));
-    if (gx_needed()) {   // the global-memory solve reads the level tables where they lie
-      dev.sol_nfw = dev.sol_nbw = 0;
-      dev.sol_fw_hdr = dev.sol_fw_blob = dev.sol_bw_hdr = dev.sol_bw_blob = nullptr;
-    } else {
+    // the global-memory solve reads the level tables where they lie; the
+    // LDS one runs a staged plan per factor precision (shape() picks it)
+    for (int pr = 0; pr < 2; pr++) {
+      Sol &P = sol[pr];
+      P.nf = P.nb = 0;
+      if (gx_needed()) continue;
+      const int elem = pr ? 8 : 4;
       SolvePlan pf, pb;
-      if (!solve_plan(A.sf, false, 0, kSolCB * kThreads, kSolCV * kThreads, pf) ||
-          !solve_plan(A.sb, true, 0, kSolCB * kThreads, kSolCV * kThreads, pb))
+      if (!solve_plan(A.sf, false, 0, kSolCB * kThreads, sol_cv(elem) * kThreads, kSolCD * kThreads, pf) ||
+          !solve_plan(A.sb, true, 0, kSolCB * kThreads, sol_cv(elem) * kThreads, kSolCD * kThreads, pb))
         throw GpError{GP_ERR_UNSUPPORTED, "triangular-solve staging plan failed"};
-      sf_hdr.upload(nonempty(pf.hdr));
-      sf_blob.upload(nonempty(pf.blob));
-      sb_hdr.upload(nonempty(pb.hdr));
-      sb_blob.upload(nonempty(pb.blob));
-      dev.sol_nfw = pf.nsteps();
-      dev.sol_nbw = pb.nsteps();
-      dev.sol_fw_hdr = sf_hdr.p;
-      dev.sol_fw_blob = sf_blob.p;
-      dev.sol_bw_hdr = sb_hdr.p;
-      dev.sol_bw_blob = sb_blob.p;
+      P.fh.upload(nonempty(pf.hdr));
+      P.fb.upload(nonempty(pf.blob));
+      P.bh.upload(nonempty(pb.hdr));
+      P.bb.upload(nonempty(pb.blob));
+      P.nf = pf.nsteps();
+      P.nb = pb.nsteps();
     }
     for (int pr = 0; pr < 2; pr++) {
       Res &R = res[pr];
@@ -297,10 +302,18 @@ struct LUPlan {
                                    (int)((((size_t)A.nd * A.nd + 3) / 4 * 4 + 512) * elem));   // dense block, buffers
     // solve: x, then the staging area, which the dense block reuses; with x
     // in global memory only the dense block
+    {
+      const Sol &P = sol[elem == 8 ? 1 : 0];
+      dev.sol_nfw = P.nf;
+      dev.sol_nbw = P.nb;
+      dev.sol_fw_hdr = P.nf ? P.fh.p : nullptr;
+      dev.sol_fw_blob = P.nf ? P.fb.p : nullptr;
+      dev.sol_bw_hdr = P.nb ? P.bh.p : nullptr;
+      dev.sol_bw_blob = P.nb ? P.bb.p : nullptr;
+    }
     dev.lds_solve = dev.gx ? (int)std::max<size_t>(16, (size_t)A.nd * A.nd * elem)
-                           : x_vecs() * x_bytes() + (int)std::max<size_t>((size_t)kSolCB * kThreads * 4 +
-                                                                   (size_t)(kSolCV + kSolCD) * kThreads * elem,
-                                                               (size_t)A.nd * A.nd * elem);
+                           : x_vecs() * x_bytes() + (int)std::max<size_t>((size_t)stage_bytes((int)elem),
+                                                                          (size_t)A.nd * A.nd * elem);
     if (dev.lds_solve > kLdsMax)
       throw GpError{GP_ERR_UNSUPPORTED, "system too large for the in-LDS triangular solve (" +
                                             std::to_string(A.n) + " rows)"};
@@ -329,14 +342,16 @@ struct LUPlan {
     }
   }
 
-  // Dense-block autotuner.  The schedule's cost model picks the dense
-  // trailing block within a bound; the block's size also decides whether the
+  // LU-plan autotuner.  The schedule's cost model picks the dense trailing
+  // block within a bound; the block's size also decides whether the
   // multipliers fit in LDS (the resident factorisation) and how long the
   // solves' two dense substitutions take, which the model does not see.  So
   // each bound in 16-row steps is planned and timed on the device -- one
   // factorisation and one solve of one scenario's matrix (var0: the value
   // array's column of scenario 0), median of 3 launches each -- and the bound
-  // with the least factorisation + w_solve * solve time is kept.
+  // with the least factorisation + w_solve * solve time is kept; then, at
+  // that bound, the levels per partition of the partitioned solves (1, 2, 4,
+  // 6, 8) the same way.
   double tuned_us = 0.0;
   int tuned_cap = -1;
   template <typename TF>
@@ -420,6 +435,29 @@ struct LUPlan {
       }
     }
     if (best_cap < 0) throw GpError{GP_ERR_UNSUPPORTED, "no dense-block size fits the LU kernels (" + why + ")"};
+    // then the levels per partition of the solves at that block size (the
+    // composites' cost is the factorisation's, their benefit the solves')
+    if (part_tune && part_fits()) {
+      const int k0 = part_k;
+      int best_k = k0;
+      for (int k : {1, 2, 4, 6, 8}) {
+        if (k == k0) continue;
+        part_k = k;
+        try {
+          upload(asrc, best_cap);
+          shape(elem);
+        } catch (const GpError &) {
+          continue;
+        }
+        const double t = elem == 4 ? time_one<float>(v1.p, cst, r1.p, x1.p, one.p, st.p, w_solve, s)
+                                   : time_one<double>(v1.p, cst, r1.p, x1.p, one.p, st.p, w_solve, s);
+        if (t < best) {
+          best = t;
+          best_k = k;
+        }
+      }
+      part_k = best_k;
+    }
     upload(asrc, best_cap);
     shape(elem);
     tuned_us = best;
@@ -2758,6 +2796,7 @@ int gp_create(const gp_system_desc *d, int32_t M, const gp_options *opt, gp_ctx 
     else { C->opt.record_all = 1; C->opt.refactor_policy = GP_REFACTOR_EVERY_ITER; }
     C->luJ.force_gx = C->luS.force_gx = C->luJT.force_gx = C->opt.global_x != 0;
     C->luJ.part_k = C->luS.part_k = C->luJT.part_k = C->opt.solve_levels > 0 ? C->opt.solve_levels : 4;
+    C->luJ.part_tune = C->luS.part_tune = C->luJT.part_tune = C->opt.solve_levels <= 0;
     C->prof_solve.every = std::max(1, C->opt.profile);   // profile = k: every k-th solve launch timed
     C->is_diff.assign(d->is_diff, d->is_diff + d->n);
     C->yrp.assign(d->y_rowptr, d->y_rowptr + 2 * d->nbus + 1);
@@ -3739,6 +3778,8 @@ int gp_get_symbolic_info(gp_ctx *C, gp_symbolic_info *info) {
   I.res_j = C->luJ.ready && C->luJ.dev.res_nsteps > 0;
   I.res_s = C->luS.ready && C->luS.dev.res_nsteps > 0;
   I.ld_scen = C->ldM;
+  I.solve_levels_j = C->luJ.ready ? std::max(1, C->luJ.A.part) : 0;
+  I.solve_levels_s = C->luS.ready ? std::max(1, C->luS.A.part) : 0;
   *info = I;
   return GP_OK;
 }

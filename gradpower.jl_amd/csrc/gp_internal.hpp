@@ -138,7 +138,10 @@ constexpr int kResCB = 24, kResCX = 8;
 // kResCX, FP64 half as many (its multipliers and values take twice the LDS)
 constexpr int res_cx(size_t elem) { return elem == 8 ? kResCX / 2 : kResCX; }
 // staging of the triangular solve, per thread: blob dwords, values, diagonals
-constexpr int kSolCB = 8, kSolCV = 8, kSolCD = 4;
+constexpr int kSolCB = 8, kSolCD = 4;
+// values per thread of a solve step: 16 for FP32 factors, 8 for FP64 (the
+// staging area of either fits beside x and y in LDS)
+constexpr int sol_cv(int elem) { return elem == 4 ? 16 : 8; }
 
 // Parameters with a hand-written pullback in the discrete adjoint.
 enum AdjKind {

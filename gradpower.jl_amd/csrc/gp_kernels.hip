@@ -2105,16 +2105,16 @@ __device__ __forceinline__ double group_sum(double acc, int g) {
 // data of step s + 1 (level and chunk tables, value columns, lane rows) and
 // its values (and diagonals) are loaded into registers while step s
 // computes from LDS, and stored into the staging area between two barriers.
-// Per thread: kSolCB blob dwords, kSolCV values, kSolCD diagonals.
+// Per thread: kSolCB blob dwords, sol_cv(sizeof(TF)) values, kSolCD diagonals.
 struct SolStage {
   int b[kSolCB];
 };
 
 // Loads of step st of a side into registers (clamped, always valid).
-template <typename TF>
+template <typename TF, int CV = sol_cv(sizeof(TF))>
 __device__ __forceinline__ void sol_fetch(const int *__restrict__ hdr, const int *__restrict__ blob, int st,
                                           const TF *__restrict__ vals, const TF *__restrict__ dg, int tid, int nt,
-                                          SolStage &P, TF (&pv)[kSolCV], TF (&pd)[kSolCD]) {
+                                          SolStage &P, TF (&pv)[CV], TF (&pd)[kSolCD]) {
   const int *h = hdr + kSolHdr * st;
   const int bo = ldu(h + 1), bl = ldu(h + 2), vo = ldu(h + 3), vl = ldu(h + 4), dgo = ldu(h + 5), dl = ldu(h + 6);
 #pragma unroll
@@ -2127,7 +2127,7 @@ __device__ __forceinline__ void sol_fetch(const int *__restrict__ hdr, const int
   }
   const int vend = ldu(h + 10);   // values actually present (the rest of vl is padding)
 #pragma unroll
-  for (int j = 0; j < kSolCV; j++) pv[j] = vals[vo + min(tid + j * nt, max(vend - 1, 0))];
+  for (int j = 0; j < CV; j++) pv[j] = vals[vo + min(tid + j * nt, max(vend - 1, 0))];
   const int dend = ldu(h + 11);
   // loaded either way and used as loaded: a branch around the load, or a
   // select on its value here, would wait for it before the step computes.
@@ -2141,15 +2141,15 @@ __device__ __forceinline__ void sol_fetch(const int *__restrict__ hdr, const int
   (void)vl;
 }
 
-template <typename TF>
+template <typename TF, int CV = sol_cv(sizeof(TF))>
 __device__ __forceinline__ void sol_store(int *sb, TF *svl, TF *sdg, int tid, int nt, const SolStage &P,
-                                          const TF (&pv)[kSolCV], const TF (&pd)[kSolCD]) {
+                                          const TF (&pv)[CV], const TF (&pd)[kSolCD]) {
 #pragma unroll
   for (int j = 0; j < kSolCB / 4; j++)
     *reinterpret_cast<int4 *>(sb + 4 * (tid + j * nt)) = make_int4(P.b[4 * j], P.b[4 * j + 1], P.b[4 * j + 2],
                                                                    P.b[4 * j + 3]);
 #pragma unroll
-  for (int j = 0; j < kSolCV; j++) svl[tid + j * nt] = pv[j];
+  for (int j = 0; j < CV; j++) svl[tid + j * nt] = pv[j];
 #pragma unroll
   for (int j = 0; j < kSolCD; j++) sdg[tid + j * nt] = pd[j];
 }
@@ -2168,7 +2168,7 @@ template <typename TF, bool BW>
 __device__ __forceinline__ void sol_step(const int *h, const int *sb, const TF *svl,
                                          const TF *sdg, double *x, double *y, int tid) {
   const int np = ldu(h);
-  const int *cols = sb + ldu(h + 8);
+  const uint16_t *cols = reinterpret_cast<const uint16_t *>(sb + ldu(h + 8));
   const int *rows = sb + ldu(h + 9);
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   // pass records from the staged blob (LDS: in-order, unlike scalar loads,
@@ -2226,43 +2226,48 @@ __device__ __forceinline__ void sol_step(const int *h, const int *sb, const TF *
 
 // Composite values of the partitioned solves after a factorisation (one
 // workgroup per scenario whose refac flag is set, the buffer the
-// factorisation wrote).  The phases run in order, one thread per entry: its
-// terms are taken four at a time with their loads in flight together and
-// added in their fixed order (the result does not depend on timing).  A
-// term reads raw entries, diagonals and composites of earlier phases of the
-// same buffer.
+// factorisation wrote).  The blocks run in order (a block's entries are of
+// one depth in their partitions, so it reads only earlier blocks'
+// composites): every term of the block is computed into LDS, four per
+// thread with their loads in flight together, then one thread per entry
+// adds its terms in their fixed order (the result does not depend on
+// timing) and stores the sum.  A term reads raw entries, diagonals and
+// composites of the same buffer.
 template <typename TF>
 __global__ void __launch_bounds__(512) k_splu_compose(SpluDev S, TF *__restrict__ svv,
                                                       const uint8_t *__restrict__ refac,
                                                       const uint8_t *__restrict__ tgt) {
   const int m = scenario_of_block(blockIdx.x, gridDim.x);
   if (!refac[m]) return;
+  __shared__ double tv[kCmpBlock];
   const int bsel = tgt ? tgt[m] : 0;
   TF *v = svv + ((size_t)bsel * gridDim.x + m) * S.ns;
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int p = 0; p < S.cmp_nph; p++) {
     const int e0 = S.cmp_ph[p], e1 = S.cmp_ph[p + 1];
-    for (int e = e0 + tid; e < e1; e += nt) {
-      const int o0 = S.cmp_tp[e], o1 = S.cmp_tp[e + 1];
-      double acc = 0.0;
-      for (int o = o0; o < o1; o += 4) {
-        int4 q[4];
-        TF va[4], vb[4], vd[4];
+    const int t0 = S.cmp_tp[e0], t1 = S.cmp_tp[e1];
+    for (int t = t0 + tid; t < t1; t += 4 * nt) {
+      int4 q[4];
+      TF va[4], vb[4], vd[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) q[u] = *reinterpret_cast<const int4 *>(S.cmp_ops + 4 * min(o + u, o1 - 1));
+      for (int u = 0; u < 4; u++) q[u] = *reinterpret_cast<const int4 *>(S.cmp_ops + 4 * min(t + u * nt, t1 - 1));
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          va[u] = v[q[u].y];
-          vb[u] = v[max(q[u].z, 0)];
-          vd[u] = v[max(q[u].w, 0)];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-          if (o + u < o1) {
-            double t = q[u].w >= 0 ? (double)va[u] / (double)vd[u] : (double)va[u];
-            acc += q[u].z >= 0 ? -t * (double)vb[u] : t;
-          }
+      for (int u = 0; u < 4; u++) {
+        va[u] = v[q[u].y];
+        vb[u] = v[max(q[u].z, 0)];
+        vd[u] = v[max(q[u].w, 0)];
       }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (t + u * nt < t1) {
+          const double a = q[u].w >= 0 ? (double)va[u] / (double)vd[u] : (double)va[u];
+          tv[t + u * nt - t0] = q[u].z >= 0 ? -a * (double)vb[u] : a;
+        }
+    }
+    __syncthreads();
+    for (int e = e0 + tid; e < e1; e += nt) {
+      double acc = 0.0;
+      for (int t = S.cmp_tp[e]; t < S.cmp_tp[e + 1]; t++) acc += tv[t - t0];
       v[S.cmp_dst[e]] = (TF)acc;
     }
     __syncthreads();
@@ -2328,14 +2333,14 @@ __global__ void __launch_bounds__(512) k_splu_solve(SpluDev S, const TF *__restr
   char *area = smem_raw + (S.part ? 2 : 1) * xb;
   int *sb = reinterpret_cast<int *>(area);
   TF *svl = reinterpret_cast<TF *>(area + (size_t)kSolCB * nt * 4);
-  TF *sdg = svl + (size_t)kSolCV * nt;
+  TF *sdg = svl + (size_t)sol_cv(sizeof(TF)) * nt;
   TF *Dl = reinterpret_cast<TF *>(area);
   const TF *fv = svv + ((size_t)bsel * Mb + fm) * S.ns;
   const TF *bv = fv + S.fw_nvals;
   const TF *bd = bv + S.bw_nvals;
   const int nfs = S.sol_nfw, nbs = S.sol_nbw;
   SolStage P;
-  TF pv[kSolCV], pd[kSolCD];
+  TF pv[sol_cv(sizeof(TF))], pd[kSolCD];
   if (nfs) sol_fetch<TF>(S.sol_fw_hdr, S.sol_fw_blob, 0, fv, bd, tid, nt, P, pv, pd);
   {
     // gather b, sixteen rows per thread in flight (one round up to 8k rows)

@@ -817,13 +817,18 @@ void splu_schedule(SpluAnalysis &A, int threads, int nd_max, int part_k) {
     for (int p = 0; p < A.nnz_lu; p++)
       if (rawof[p] >= 0) A.lu2s[p] = rawbase + rawof[p];
     // compose program: phases by depth (forward and backward composites
-    // together), per entry its terms (4 ints: 0, a, b, d)
+    // together), per entry its terms (4 ints: 0, a, b, d); a phase is cut
+    // into blocks of at most kCmpBlock terms at entry boundaries (the blocks
+    // of a phase are independent), cmp_ph listing every block's first entry
     int maxd = 0;
     for (int c = 0; c < (int)cterms.size(); c++) maxd = std::max(maxd, cdepth[c]);
     A.cmp_tp.assign(1, 0);
     for (int d = 1; d <= maxd; d++) {
       for (int c = 0; c < (int)cterms.size(); c++) {
         if (cdepth[c] != d) continue;
+        if ((int)A.cmp_ops.size() / 4 - A.cmp_tp[A.cmp_ph.back()] + (int)cterms[c].size() > kCmpBlock &&
+            (int)A.cmp_dst.size() > A.cmp_ph.back())
+          A.cmp_ph.push_back((int)A.cmp_dst.size());   // a new block
         A.cmp_dst.push_back(cslot[c]);
         for (const Comp &t : cterms[c]) {
           const int dg = t.d >= 0 ? dbase + alane[t.d] : -1;
@@ -831,7 +836,7 @@ void splu_schedule(SpluAnalysis &A, int threads, int nd_max, int part_k) {
         }
         A.cmp_tp.push_back((int)A.cmp_ops.size() / 4);
       }
-      A.cmp_ph.push_back((int)A.cmp_dst.size());
+      if ((int)A.cmp_dst.size() > A.cmp_ph.back()) A.cmp_ph.push_back((int)A.cmp_dst.size());
     }
     (void)ncf;
   }
@@ -1066,10 +1071,14 @@ bool res_plan(const SpluAnalysis &A, int bcap, int xcap, ResPlan &R) {
   return true;
 }
 
-bool solve_plan(const SpluAnalysis::SolveSide &S, bool diag, int dg_base, int bcap, int vcap, SolvePlan &P) {
+bool solve_plan(const SpluAnalysis::SolveSide &S, bool diag, int dg_base, int bcap, int vcap, int dcap,
+                SolvePlan &P) {
   P = SolvePlan{};
   const int nlev = (int)S.lvl.size() / 4;
   auto al4 = [](int v) { return (v + 3) & ~3; };
+  auto half = [](int v) { return (v + 1) / 2; };   // dwords of v 16-bit columns
+  for (int c : S.cols)
+    if (c < 0 || c > 65535) return false;
   // (sub-)levels as chunk ranges [c0, c1) of a level
   struct Sub { int l, c0, c1; };
   std::vector<Sub> subs;
@@ -1080,8 +1089,8 @@ bool solve_plan(const SpluAnalysis::SolveSide &S, bool diag, int dg_base, int bc
     while (c < cb + nch) {
       int c1 = c, v = 0;
       // whole chunks while the values, columns and rows fit one step
-      while (c1 < cb + nch && v + chunk_vals(c1) + (diag ? 64 * (c1 - c + 1) : 0) <= vcap - 8 &&
-             36 * ((c1 - c + 8) / 8 + 1) + v + chunk_vals(c1) + 64 * (c1 - c + 1) <= bcap - 16) {
+      while (c1 < cb + nch && v + chunk_vals(c1) <= vcap - 8 && (!diag || 64 * (c1 - c + 1) <= dcap) &&
+             36 * ((c1 - c + 8) / 8 + 1) + half(v + chunk_vals(c1)) + 64 * (c1 - c + 1) <= bcap - 16) {
         v += chunk_vals(c1);
         c1++;
       }
@@ -1100,9 +1109,8 @@ bool solve_plan(const SpluAnalysis::SolveSide &S, bool diag, int dg_base, int bc
       int v = 0;
       for (int c = subs[j].c0; c < subs[j].c1; c++) v += chunk_vals(c);
       const int lanes = 64 * (nch + ch);   // rows: at most 64 per chunk
-      const int blen = 36 * (nl + (nch + ch + 7) / 8 + 1) + al4(nv + v) + al4(lanes);
-      const int vlen = al4(nv + v) + (diag ? al4(lanes) : 0);
-      if (blen > bcap || vlen > vcap) break;
+      const int blen = 36 * (nl + (nch + ch + 7) / 8 + 1) + al4(half(nv + v)) + al4(lanes);
+      if (blen > bcap || al4(nv + v) > vcap || (diag && al4(lanes) > dcap)) break;
       nch += ch;
       nv += v;
       nl++;
@@ -1149,7 +1157,8 @@ bool solve_plan(const SpluAnalysis::SolveSide &S, bool diag, int dg_base, int bc
       }
     }
     if (S.chunk[2 * (clast - 1)] + chunk_vals(clast - 1) != E1) return false;   // values contiguous
-    cols.assign(S.cols.begin() + E0, S.cols.begin() + E1);
+    cols.assign(half(E1 - E0), 0);   // 16-bit columns, two per dword (low half first)
+    for (int e = E0; e < E1; e++) cols[(e - E0) / 2] |= S.cols[e] << (16 * ((e - E0) & 1));
     rows.assign(S.rows.begin() + R0, S.rows.begin() + R1);
     auto put = [&](std::vector<int> &v) {
       v.resize(al4((int)v.size()), 0);

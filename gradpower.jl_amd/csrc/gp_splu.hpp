@@ -15,6 +15,8 @@
 
 namespace gp {
 
+constexpr int kCmpBlock = 4096;   // terms of one block of the compose program (LDS of the compose kernel)
+
 struct SpluAnalysis {
   int n = 0;
   int nnz_a = 0;
@@ -90,7 +92,8 @@ struct SpluAnalysis {
   // Partitioned solves (part > 1: up to part levels per partition).  The
   // A-level values are composites the compose program computes after each
   // factorisation from the raw entries inside the partitions (nraw slots
-  // after the diagonals): phase p covers entries [cmp_ph[p], cmp_ph[p+1]);
+  // after the diagonals): block p covers entries [cmp_ph[p], cmp_ph[p+1])
+  // (blocks in phase order, each of one depth, at most kCmpBlock terms);
   // entry e writes solve slot cmp_dst[e], the sum of its terms
   // [cmp_tp[e], cmp_tp[e+1]), 4 ints each (0, a, b, d):
   // v[a] / (d >= 0 ? v[d] : 1) for b < 0, -v[a] / (d >= 0 ? v[d] : 1) * v[b]
@@ -161,9 +164,10 @@ struct ResPlan {
 // barrier follows, the level's flags (SolveSide::lflag), 0, then per
 // wavefront: first value of its chunk relative to the step, entries per
 // lane pf, first lane relative to the step, active lanes (0: idle)), the
-// step's value columns, then its lanes'
+// step's value columns (16 bits each, two per dword), then its lanes'
 // rows.  The step's values (and, for the backward side, diagonals) are the
-// ranges [val_off, + val_len) and [dg_off, + dg_len) of the solve array.
+// ranges [val_off, + val_len) and [dg_off, + dg_len) of the solve array;
+// a step holds at most vcap values, dcap diagonals and bcap blob dwords.
 // Step header (kSolHdr ints): passes, blob offset, blob length, val_off,
 // val_len, dg_off, dg_len, 0, cols offset, rows offset, values and
 // diagonals actually present (the lengths before padding to 4).  All
@@ -174,7 +178,8 @@ struct SolvePlan {
   int maxb = 0, maxv = 0;
   int nsteps() const { return (int)hdr.size() / kSolHdr; }
 };
-bool solve_plan(const SpluAnalysis::SolveSide &S, bool diag, int dg_base, int bcap, int vcap, SolvePlan &P);
+bool solve_plan(const SpluAnalysis::SolveSide &S, bool diag, int dg_base, int bcap, int vcap, int dcap,
+                SolvePlan &P);
 
 // Returns false (and leaves R empty) if a column alone exceeds a capacity.
 bool res_plan(const SpluAnalysis &A, int bcap, int xcap, ResPlan &R);

@@ -60,7 +60,8 @@ class CSymInfo(C.Structure):
                 ("levels_j", C.c_int32), ("n_red", C.c_int32), ("nnz_s", C.c_int32),
                 ("nnz_lu_s", C.c_int32), ("levels_s", C.c_int32), ("n_clusters_nt", C.c_int32),
                 ("w_total_nt", C.c_int32), ("nd_j", C.c_int32), ("nd_s", C.c_int32), ("nd_jt", C.c_int32),
-                ("res_j", C.c_int32), ("res_s", C.c_int32), ("ld_scen", C.c_int32)]
+                ("res_j", C.c_int32), ("res_s", C.c_int32), ("ld_scen", C.c_int32),
+                ("solve_levels_j", C.c_int32), ("solve_levels_s", C.c_int32)]
 
 
 class CObjective(C.Structure):

@@ -157,6 +157,8 @@ struct GpSymbolicInfo
     res_j::Int32
     res_s::Int32
     ld_scen::Int32
+    solve_levels_j::Int32
+    solve_levels_s::Int32
 end
 
 const GP_OK = Cint(0)

@@ -479,6 +479,8 @@ typedef struct gp_symbolic_info {
   int32_t res_j, res_s;        /* 1: the plan's factorisation keeps its multipliers in LDS
                                   (at the element size it was last shaped for)       */
   int32_t ld_scen;             /* row stride of the device arrays: M rounded up to 64 */
+  int32_t solve_levels_j;      /* levels per partition of the J plan's solves (1: one level per step) */
+  int32_t solve_levels_s;      /* the same for the S plan */
 } gp_symbolic_info;
 int gp_get_symbolic_info(gp_ctx *ctx, gp_symbolic_info *info);
 
