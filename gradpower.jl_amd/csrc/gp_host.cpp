@@ -571,6 +571,11 @@ struct gp_ctx {
     int every = 1;
   } prof_solve, prof_factor;
   DBuf<unsigned long long> d_nb;
+  // per working scenario, max |F| of each non-trivial cluster's rows and of
+  // each bus's rows, written by the residual kernels ([n_cl_nt + nbus][ldM]);
+  // used for the Newton max-norm when those rows are all n rows (pm_rows)
+  DBuf<double> d_pm;
+  bool pm_rows = false;
   DevTables T{};
   LUPlan luJ, luS;
   // the plans' representative matrices at analysis (scenario 0, dt = 0, no
@@ -1034,6 +1039,21 @@ void alloc_state(gp_ctx *C) {
   C->d_status.alloc(L);
   C->d_nb.alloc(L);
   HIPCHK(hipMemset(C->d_nb.p, 0, L * sizeof(unsigned long long)));
+  {
+    // the residual kernels write every row of F: the non-trivial clusters'
+    // windows, the PV and slack static generators' algebraic rows and two
+    // rows per bus
+    int64_t rows = 2 * (int64_t)C->nbus;
+    for (int c : C->cl_nt) rows += C->cl[(size_t)c * GP_CLUSTER_COLS + GP_CL_WSIZE];
+    const auto &SG = C->dev[GP_STATICGEN];
+    for (size_t k = 0; k < SG.size() / GP_COLS_STATICGEN; k++) {
+      const int bt = SG[k * GP_COLS_STATICGEN + 3];
+      rows += bt == 2 ? 1 : bt == 3 ? 2 : 0;
+    }
+    C->pm_rows = rows == C->n;
+    C->d_pm.alloc((size_t)(C->cl_nt.size() + C->nbus) * L);
+    HIPCHK(hipMemset(C->d_pm.p, 0, (size_t)(C->cl_nt.size() + C->nbus) * L * sizeof(double)));
+  }
   HIPCHK(hipMemset(C->d_status.p, 0, L * sizeof(int)));
   HIPCHK(hipMemset(C->d_z.p, 0, (size_t)C->n * L * sizeof(double)));
   std::vector<int> fb((size_t)MAXFAULT * L, -1);
@@ -1554,11 +1574,13 @@ struct Newton {
   void eval(int first, const uint8_t *firstv, const uint8_t *refac0, int *cnt, bool read_req) {
     const int L = C->ldM;
     double *z = C->d_z.p;
+    double *pm = C->pm_rows ? C->d_pm.p : nullptr;
     launch_cluster_residual(C->T, C->d_p.p, z, C->d_zold.p, dt, faults(C),
-                            C->d_f.p, C->d_inj.p, C->d_work.p, C->stream, dtv);
-    launch_bus_residual(C->T, C->d_p.p, z, C->d_inj.p, faults(C), C->d_f.p, C->d_work.p, C->stream);
+                            C->d_f.p, C->d_inj.p, C->d_work.p, C->stream, dtv, pm);
+    launch_bus_residual(C->T, C->d_p.p, z, C->d_inj.p, faults(C), C->d_f.p, C->d_work.p, C->stream, pm);
     if (cnt) HIPCHK(hipMemsetAsync(C->d_nunconv.p, 0, 4 * sizeof(int), C->stream));
-    launch_rownorm_impl(C->n, C->M, L, C->d_f.p, C->d_work.p, tol, C->d_fnorm.p, C->d_nb.p, C->d_nunconv.p,
+    // max-norm over the kernels' per-block maxima (the same maximum as over F)
+    launch_rownorm_impl(pm ? C->T.n_cl_nt + C->nbus : C->n, C->M, L, pm ? pm : C->d_f.p, C->d_work.p, tol, C->d_fnorm.p, C->d_nb.p, C->d_nunconv.p,
                         C->d_refac.p, reuse ? refac0 : nullptr, first, theta, stall, itermax, C->d_fbest.p,
                         C->d_nstall.p, C->d_iters.p, C->d_failed.p, async ? C->d_pend.p : nullptr, C->d_upd.p,
                         async ? async_below : 0.0, C->d_areq.p, C->d_sch.p, C->stream, firstv, exactv,

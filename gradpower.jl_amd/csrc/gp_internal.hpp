@@ -151,11 +151,14 @@ enum AdjKind {
 
 // ---- kernel launchers (gp_kernels.hip) ----------------------------------
 // dtv (optional): per-scenario step, replacing dt (0 for an event re-solve)
+// pm (optional): per working scenario, max |F| over each non-trivial
+// cluster's rows (row c of [n_cl_nt + nbus][ldM]) and each bus's rows (row
+// n_cl_nt + b), for the max-norm
 void launch_cluster_residual(const DevTables &T, const double *p, const double *z, const double *zold,
                              double dt, FaultState fs, double *f, double *inj, const uint8_t *work, hipStream_t s,
-                             const double *dtv = nullptr);
+                             const double *dtv = nullptr, double *pm = nullptr);
 void launch_bus_residual(const DevTables &T, const double *p, const double *z, const double *inj,
-                         FaultState fs, double *f, const uint8_t *work, hipStream_t s);
+                         FaultState fs, double *f, const uint8_t *work, hipStream_t s, double *pm = nullptr);
 void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, double tol, double *fnorm,
                          unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
                          double theta, int stall, int itermax, double *fbest, int *nstall, int *iters,

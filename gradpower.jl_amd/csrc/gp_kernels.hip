@@ -227,12 +227,20 @@ __global__ void __launch_bounds__(64) k_cluster_residual(DevTables T, const doub
                                                          const double *__restrict__ z, const double *__restrict__ zold,
                                                          double dt0, const double *__restrict__ dtv, FaultState fs,
                                                          double *__restrict__ f, double *__restrict__ inj,
-                                                         const uint8_t *__restrict__ work) {
+                                                         const uint8_t *__restrict__ work, double *__restrict__ nrm) {
   const int c = T.cl_nt[blockIdx.x];
   const int m = lane_scenario();
   if (m >= T.M || !work[m]) return;
   const double dt = dtv ? dtv[m] : dt0;   // per-scenario step (0: event re-solve)
   const int ldM = T.ldM;
+  // |F| of the rows written here, as the bit patterns of non-negative
+  // doubles (whose order is the integers'; a NaN is the largest)
+  unsigned long long bits = 0;
+  auto put = [&](size_t o, double v) {
+    f[o] = v;
+    const unsigned long long u = __double_as_longlong(fabs(v));
+    bits = u > bits ? u : bits;
+  };
   const int *cl = T.cl + c * GP_CLUSTER_COLS;
   const int ws = cl[GP_CL_WSTART], gk = cl[GP_CL_GENROU];
   const int *g = T.genrou + gk * GP_COLS_GENROU;
@@ -252,10 +260,10 @@ __global__ void __launch_bounds__(64) k_cluster_residual(DevTables T, const doub
 #pragma unroll
   for (int i = 0; i < 6; i++) {
     const size_t o = (size_t)(dp + i) * ldM + m;
-    f[o] = zs[i] - zold[o] - dt * fr[i];
+    put(o, zs[i] - zold[o] - dt * fr[i]);
   }
 #pragma unroll
-  for (int i = 0; i < 4; i++) f[(size_t)(ap + i) * ldM + m] = fr[6 + i];
+  for (int i = 0; i < 4; i++) put((size_t)(ap + i) * ldM + m, fr[6 + i]);
   inj[(size_t)(2 * c) * ldM + m] = ir;
   inj[(size_t)(2 * c + 1) * ldM + m] = ii;
   const double w = zs[4];
@@ -271,9 +279,9 @@ __global__ void __launch_bounds__(64) k_cluster_residual(DevTables T, const doub
 #pragma unroll
     for (int i = 0; i < 5; i++) {
       const size_t o = (size_t)(d[0] + i) * ldM + m;
-      f[o] = x[i] - zold[o] - dt * r[i];
+      put(o, x[i] - zold[o] - dt * r[i]);
     }
-    f[(size_t)d[1] * ldM + m] = r[5];
+    put((size_t)d[1] * ldM + m, r[5]);
   } else if (gt == GP_TGOV1) {
     const int *d = T.tgov1 + cl[GP_CL_GOV] * GP_COLS_TGOV1;
     double x[2], r[3];
@@ -285,9 +293,9 @@ __global__ void __launch_bounds__(64) k_cluster_residual(DevTables T, const doub
 #pragma unroll
     for (int i = 0; i < 2; i++) {
       const size_t o = (size_t)(d[0] + i) * ldM + m;
-      f[o] = x[i] - zold[o] - dt * r[i];
+      put(o, x[i] - zold[o] - dt * r[i]);
     }
-    f[(size_t)d[1] * ldM + m] = r[2];
+    put((size_t)d[1] * ldM + m, r[2]);
   }
   const int et = cl[GP_CL_EXCTYPE];
   if (et == GP_SEXS || et == GP_ESDC1A) {
@@ -302,7 +310,7 @@ __global__ void __launch_bounds__(64) k_cluster_residual(DevTables T, const doub
     if (exc_offline(fs, ldM, m, et, cl[GP_CL_EXC])) r[0] = r[1] = r[2] = 0.0;
     for (int i = 0; i < ne; i++) {
       const size_t o = (size_t)(d[0] + i) * ldM + m;
-      f[o] = x[i] - zold[o] - dt * r[i];
+      put(o, x[i] - zold[o] - dt * r[i]);
     }
   }
   if (cl[GP_CL_PSS] >= 0) {
@@ -316,12 +324,13 @@ __global__ void __launch_bounds__(64) k_cluster_residual(DevTables T, const doub
 #pragma unroll
     for (int i = 0; i < 7; i++) {
       const size_t o = (size_t)(d[0] + i) * ldM + m;
-      f[o] = x[i] - zold[o] - dt * r[i];
+      put(o, x[i] - zold[o] - dt * r[i]);
     }
-    f[(size_t)d[1] * ldM + m] = r[7];
+    put((size_t)d[1] * ldM + m, r[7]);
   }
   (void)w;
   (void)ws;
+  if (nrm) nrm[(size_t)blockIdx.x * ldM + m] = __longlong_as_double(bits);
 }
 
 // ZIP load k disconnected in scenario m (DisconnectDeviceEvent,
@@ -385,7 +394,7 @@ __device__ __forceinline__ double trip_offdiag(const DevTables &T, const FaultSt
 __global__ void __launch_bounds__(64) k_bus_residual(DevTables T, const double *__restrict__ p,
                                                      const double *__restrict__ z, const double *__restrict__ inj,
                                                      FaultState fs, double *__restrict__ f,
-                                                     const uint8_t *__restrict__ work) {
+                                                     const uint8_t *__restrict__ work, double *__restrict__ pm) {
   const int b = blockIdx.x;
   const int m = lane_scenario();
   if (m >= T.M || !work[m]) return;
@@ -414,6 +423,7 @@ __global__ void __launch_bounds__(64) k_bus_residual(DevTables T, const double *
     fi -= (1.0 - al) * (ql * vr + pl * vi) / den;
   }
   const int sgk = T.bus_sg[b];
+  double sg0 = 0.0, sg1 = 0.0;   // the static generator's rows (zero without)
   if (sgk >= 0) {
     const int *d = T.sg + sgk * GP_COLS_STATICGEN;
     const int ap = d[1], bt = d[3];
@@ -425,10 +435,14 @@ __global__ void __launch_bounds__(64) k_bus_residual(DevTables T, const double *
     else { pv = P[0]; qv = P[1]; }
     fr += (pv * vr + qv * vi) / vm2;
     fi += (pv * vi - qv * vr) / vm2;
-    if (bt == 2) f[(size_t)ap * ldM + m] = vr * vr + vi * vi - P[2] * P[2];
-    else if (bt == 3) {
-      f[(size_t)ap * ldM + m] = vr - P[2] * cos(P[3]);
-      f[(size_t)(ap + 1) * ldM + m] = vi - P[2] * sin(P[3]);
+    if (bt == 2) {
+      sg0 = vr * vr + vi * vi - P[2] * P[2];
+      f[(size_t)ap * ldM + m] = sg0;
+    } else if (bt == 3) {
+      sg0 = vr - P[2] * cos(P[3]);
+      sg1 = vi - P[2] * sin(P[3]);
+      f[(size_t)ap * ldM + m] = sg0;
+      f[(size_t)(ap + 1) * ldM + m] = sg1;
     }
   }
 #pragma unroll
@@ -442,6 +456,16 @@ __global__ void __launch_bounds__(64) k_bus_residual(DevTables T, const double *
   trip_residual(T, fs, z, b, m, vr, vi, fr, fi);
   f[(size_t)vrz * ldM + m] = fr;
   f[(size_t)(vrz + 1) * ldM + m] = fi;
+  if (pm) {   // |F| of the rows written here (bit patterns, as k_cluster_residual)
+    unsigned long long bits = 0;
+    const double r4[4] = {fr, fi, sg0, sg1};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const unsigned long long u = __double_as_longlong(fabs(r4[k]));
+      bits = u > bits ? u : bits;
+    }
+    pm[(size_t)(T.n_cl_nt + b) * ldM + m] = __longlong_as_double(bits);
+  }
 }
 
 // Max-norm of F per scenario.  Grid (ldM/64, chunks); partial maxima are
@@ -2927,14 +2951,14 @@ static inline dim3 lane_grid(int items, int ldM) { return dim3(items, ldM / WAVE
 
 void launch_cluster_residual(const DevTables &T, const double *p, const double *z, const double *zold, double dt,
                              FaultState fs, double *f, double *inj, const uint8_t *work, hipStream_t s,
-                             const double *dtv) {
+                             const double *dtv, double *pm) {
   if (T.n_cl_nt == 0) return;
   hipLaunchKernelGGL(k_cluster_residual, lane_grid(T.n_cl_nt, T.ldM), dim3(WAVE), 0, s, T, p, z, zold, dt, dtv, fs, f,
-                     inj, work);
+                     inj, work, pm);
 }
 void launch_bus_residual(const DevTables &T, const double *p, const double *z, const double *inj, FaultState fs,
-                         double *f, const uint8_t *work, hipStream_t s) {
-  hipLaunchKernelGGL(k_bus_residual, lane_grid(T.nbus, T.ldM), dim3(WAVE), 0, s, T, p, z, inj, fs, f, work);
+                         double *f, const uint8_t *work, hipStream_t s, double *pm) {
+  hipLaunchKernelGGL(k_bus_residual, lane_grid(T.nbus, T.ldM), dim3(WAVE), 0, s, T, p, z, inj, fs, f, work, pm);
 }
 void launch_rownorm_impl(int n, int M, int ldM, const double *f, uint8_t *work, double tol, double *fnorm,
                          unsigned long long *nb, int *n_unconv, uint8_t *refac, const uint8_t *refac0, int first,
