@@ -1353,6 +1353,14 @@ void prof_end(gp_ctx::KProf &P, int i, hipStream_t s, int64_t scenarios) {
   HIPCHK(hipEventRecord(P.ev[i + 1], s));
   P.scen[i / 2] = scenarios;
 }
+// The end event of pair i (NULL when not timed), for a factorisation
+// launch to record after its factor kernel and before the kernel of the
+// partitioned solves' composite entries: the pair times the factor kernel.
+hipEvent_t prof_end_ev(gp_ctx::KProf &P, int i, int64_t scenarios) {
+  if (i < 0) return nullptr;
+  P.scen[i / 2] = scenarios;
+  return P.ev[i + 1];
+}
 void prof_count(gp_ctx::KProf &P, int i, int64_t scenarios) {
   if (i >= 0) P.scen[i / 2] = scenarios;
 }
@@ -1464,8 +1472,7 @@ void start_batch(gp_ctx *C, int l, TF *luv, TF *sv, TF *dn) {
   const int pi = l == 0 ? prof_begin(C, C->prof_factor, ln.stream) : -1;
   launch_splu_factor_impl<TF>(sizeof(TF) == 8 ? C->luS64 : C->luS.dev, ln.snap.p, C->d_s_cst.p,
                               -std::max(1, C->n_svar), luv, sv, dn,
-                              ln.req, ln.tgt, C->M, C->d_status.p, ln.stream);
-  prof_end(C->prof_factor, pi, ln.stream, nb);
+                              ln.req, ln.tgt, C->M, C->d_status.p, ln.stream, prof_end_ev(C->prof_factor, pi, nb));
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ln.ev_done, ln.stream));
   ln.inflight = true;
@@ -1655,8 +1662,8 @@ struct Newton {
       launch_jac_mono(C->T, p, z, dt, faults(C), C->d_jv.p, C->d_upd.p, C->stream, dtv);
       prof_f = prof_begin(C, C->prof_factor, C->stream);
       launch_splu_factor_impl<double>(C->luJ.dev, C->d_jv.p, nullptr, L, C->d_luJ.p, C->d_svJ.p, C->d_dnJ.p,
-                                      C->d_refac.p, nullptr, C->M, C->d_status.p, C->stream);
-      prof_end(C->prof_factor, prof_f, C->stream, known ? cnt[1] : 0);
+                                      C->d_refac.p, nullptr, C->M, C->d_status.p, C->stream,
+                                      prof_end_ev(C->prof_factor, prof_f, known ? cnt[1] : 0));
       return;
     }
     if (shared) {
@@ -1684,8 +1691,8 @@ struct Newton {
       if (shmix) {
         prof_f = prof_begin(C, C->prof_factor, C->stream);
         launch_splu_factor_impl<double>(C->luS64, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
-                                        C->d_dnS64.p, ref64, C->d_cur.p, C->M, C->d_status.p, C->stream);
-        prof_end(C->prof_factor, prof_f, C->stream, known ? cnt[1] : 0);
+                                        C->d_dnS64.p, ref64, C->d_cur.p, C->M, C->d_status.p, C->stream,
+                                        prof_end_ev(C->prof_factor, prof_f, known ? cnt[1] : 0));
       }
       HIPCHK(hipGetLastError());
       return;
@@ -1769,13 +1776,13 @@ struct Newton {
     const int L = C->ldM;
     // timed: the launches in the method's own precision
     prof_f = (run32 || !fp32) ? prof_begin(C, C->prof_factor, C->stream) : -1;
+    hipEvent_t fe = prof_end_ev(C->prof_factor, prof_f, nscen);
     if (run32)
       launch_splu_factor_impl<float>(C->luS.dev, C->d_svar.p, C->d_s_cst.p, L, C->d_luS32.p, C->d_svS32.p,
-                                     C->d_dnS32.p, ref32_, C->d_cur.p, C->M, C->d_status.p, C->stream);
+                                     C->d_dnS32.p, ref32_, C->d_cur.p, C->M, C->d_status.p, C->stream, fe);
     if (run64 && !mixed)
       launch_splu_factor_impl<double>(C->luS64, C->d_svar.p, C->d_s_cst.p, L, C->d_luS64.p, C->d_svS64.p,
-                                      C->d_dnS64.p, ref64_, C->d_cur.p, C->M, C->d_status.p, C->stream);
-    prof_end(C->prof_factor, prof_f, C->stream, nscen);
+                                      C->d_dnS64.p, ref64_, C->d_cur.p, C->M, C->d_status.p, C->stream, fe);
   }
   void launch_post(const int cnt[4]) {
     const int L = C->ldM;

@@ -181,10 +181,13 @@ void launch_cluster_backsub(const DevTables &T, const TF *X, const double *f, co
 // [2][M][nd*nd] dense trailing block (column-major LU, unit lower part
 // implied).  The factorisation writes buffer tgt[m], the solve reads cur[m]
 // (NULL: buffer 0).  The matrix values var are rows [s][ldM] (ldM > 0) or
-// scenario-major rows [m][-ldM] (ldM < 0).
+// scenario-major rows [m][-ldM] (ldM < 0).  A plan with partitioned solves
+// then computes their composite entries (k_splu_compose); `factored`, when
+// given, is recorded between the two kernels.
 template <typename TF>
 void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv, TF *sv,
-                             TF *dn, const uint8_t *refac, const uint8_t *tgt, int M, int *status, hipStream_t s);
+                             TF *dn, const uint8_t *refac, const uint8_t *tgt, int M, int *status, hipStream_t s,
+                             hipEvent_t factored = nullptr);
 template <typename TF>
 void launch_splu_solve(const SpluDev &S, const TF *sv, const TF *dn, const double *rhs, int ldM, double sign,
                        const int *scatter, double *zupd, double *xout, const uint8_t *work, const uint8_t *cur, int M,

@@ -3013,7 +3013,8 @@ static void allow_lds(const void *f, std::atomic<unsigned> &done, int bytes) {
 
 template <typename TF>
 void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *cst, int ldM, TF *luv, TF *sv, TF *dn,
-                             const uint8_t *refac, const uint8_t *tgt, int M, int *status, hipStream_t s) {
+                             const uint8_t *refac, const uint8_t *tgt, int M, int *status, hipStream_t s,
+                             hipEvent_t factored) {
   if (S.res_nsteps > 0) {
     static std::atomic<unsigned> done_res{0};
     allow_lds(reinterpret_cast<const void *>(k_splu_factor_res<TF>), done_res, S.lds_factor);
@@ -3025,6 +3026,7 @@ void launch_splu_factor_impl(const SpluDev &S, const double *var, const double *
     hipLaunchKernelGGL(k_splu_factor<TF>, dim3(M), dim3(S.threads), S.lds_factor, s, S, var, cst, ldM, luv, sv, dn,
                        refac, tgt, status);
   }
+  if (factored) (void)hipEventRecord(factored, s);
   if (S.cmp_nph > 0) {   // the partitioned solves' composites, from the values just written
     hipLaunchKernelGGL(k_splu_compose<TF>, dim3(M), dim3(S.threads), 0, s, S, sv, refac, tgt);
   }
@@ -3178,9 +3180,10 @@ template void launch_cluster_backsub<float>(const DevTables &, const float *, co
                                             double *, const uint8_t *, hipStream_t);
 template void launch_splu_factor_impl<double>(const SpluDev &, const double *, const double *, int, double *,
                                               double *, double *, const uint8_t *, const uint8_t *, int, int *,
-                                              hipStream_t);
+                                              hipStream_t, hipEvent_t);
 template void launch_splu_factor_impl<float>(const SpluDev &, const double *, const double *, int, float *, float *,
-                                             float *, const uint8_t *, const uint8_t *, int, int *, hipStream_t);
+                                             float *, const uint8_t *, const uint8_t *, int, int *, hipStream_t,
+                                             hipEvent_t);
 template void launch_splu_solve<double>(const SpluDev &, const double *, const double *, const double *, int, double,
                                         const int *, double *, double *, const uint8_t *, const uint8_t *, int,
                                         hipStream_t, bool, const WbTab *);
