@@ -61,10 +61,10 @@ def test_jacobian_vs_finite_differences(name):
 def test_golden_2bus_final_state():
     """Final state of test/test_dynamics.jl:3-28 (2-bus GENROU + constant-Z
     load, fault bus 2 r=0.2 from 0.2 to 0.3 s, tf=1 s).  The rotor-frame
-    states z[1], z[4], z[10] (1-based) agree to ~1e-9; z[14] (bus-2 vi) is
-    not compared: it differs from the archived value by a rigid rotation of
-    all phasors by 0.108 rad, to which every rotor-frame state is invariant,
-    i.e. the archived run used a different angle reference."""
+    states z[1], z[4], z[10] (1-based) agree to ~1e-9; z[14] (bus-2 vi)
+    differs from the archived value by a rigid rotation of all phasors by
+    0.108 rad, to which every rotor-frame state is invariant (the archived
+    run used another angle reference): test_golden_2bus_vi_is_a_rigid_rotation."""
     case = get_case("case_2bus_testdata")
     o = _oracle(case)
     _, zf, st = o.integrate(case.dt, 120, _events(case), tol=1e-10)
@@ -73,6 +73,52 @@ def test_golden_2bus_final_state():
     for i, v in gold.items():
         assert zf[i] == pytest.approx(v, rel=1e-2)
         assert abs(zf[i] - v) < 1e-8   # tighter than the reference's own rtol
+
+
+def test_golden_2bus_vi_is_a_rigid_rotation():
+    """The fourth archived value of test/test_dynamics.jl:28 (z[14], bus-2
+    vi, -0.9344382816186252) is met up to the system's rotation symmetry.
+    The 2-bus case has no static generator to fix an angle reference, and its
+    equations see the bus phasors only through V e^{-i delta} (the machine's
+    rotor frame), |V| (loads) and a real admittance matrix, so rotating every
+    phasor by theta and adding theta to the rotor angle maps trajectories to
+    trajectories: checked here by integrating from the rotated start (every
+    other state equal to 1e-9, the phasors and the angle rotated).  The
+    rotation that carries the final bus-2 phasor onto the archived vi is
+    0.108 rad, and a run started at that rotation meets all four archived
+    values to 1e-8."""
+    case = get_case("case_2bus_testdata")
+    d = case.desc
+    assert len(d.dev[7]) == 0                      # no static generator: no angle reference
+    o = _oracle(case)
+    npt, nb = d.net_ptr, d.nbus
+    dl = int(d.dev[0][0][0] + 5)                   # GENROU rotor angle
+
+    def rotate(z, th):
+        z = z.copy()
+        v = z[npt:npt + 2 * nb:2] + 1j * z[npt + 1:npt + 2 * nb:2]
+        v = v * np.exp(1j * th)
+        z[npt:npt + 2 * nb:2], z[npt + 1:npt + 2 * nb:2] = v.real, v.imag
+        z[dl] += th
+        return z
+
+    _, zf, st = o.integrate(case.dt, 120, _events(case), tol=1e-10)
+    assert st["not_converged"] == 0
+    th = 0.3
+    _, zr, st = o.integrate(case.dt, 120, _events(case), z0=rotate(d.z0, th), tol=1e-10)
+    assert st["not_converged"] == 0
+    assert np.max(np.abs(zr - rotate(zf, th))) < 1e-9
+    # the rotation that meets the archived bus-2 vi (the branch near zero)
+    vi_arch = -0.9344382816186252
+    v2 = zf[npt + 2] + 1j * zf[npt + 3]
+    th_star = np.arcsin(vi_arch / abs(v2)) - np.angle(v2)
+    assert abs(th_star - 0.108) < 1e-3, th_star
+    _, za, st = o.integrate(case.dt, 120, _events(case), z0=rotate(d.z0, th_star), tol=1e-10)
+    gold = {0: 1.0276157414718632, 3: -0.6178842140068552, 9: 0.9337400271910671, 13: vi_arch}
+    for i, v in gold.items():
+        assert abs(za[i] - v) < 1e-8, (i, za[i], v)
+    # unrotated, the archived vi is missed by far more than the rtol of 1e-2
+    assert abs(zf[13] - vi_arch) > 1e-2 * abs(vi_arch)
 
 
 def test_flat_line_no_fault():
