@@ -36,8 +36,8 @@ for _p in (ROOT, os.path.join(ROOT, "gradpower.jl_amd")):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, published
-# the library times every launch of the factorisation and every k-th launch
-# of the triangular solve (an event pair costs the round's stream a few us)
+# the library times every k-th launch of the factorisation and of the
+# triangular solve (an event pair costs the round's stream a few us)
 SOLVE_TIMING_STRIDE = 8
 KERNEL_NAMES = {("solve", 2): "k_splu_solve<float>", ("solve", 1): "k_splu_solve<double>",
                 ("solve", 0): "k_splu_solve<double>", ("factor", 2): "k_splu_factor_res<float>",
@@ -334,8 +334,8 @@ def roofline(run, method):
     """Dominant kernel of one method: HIP-event device time per launch and
     algorithmic bytes per launch (bytes per scenario x scenarios served)."""
     k, st = run["kern"], run["stats"]
-    stride = {"solve": SOLVE_TIMING_STRIDE, "factor": 1}
-    which = "solve" if k["solve_ms"] * SOLVE_TIMING_STRIDE >= k["factor_ms"] else "factor"
+    stride = {"solve": SOLVE_TIMING_STRIDE, "factor": SOLVE_TIMING_STRIDE}
+    which = "solve" if k["solve_ms"] >= k["factor_ms"] else "factor"
     n = k[which + "_n"]
     if n == 0:
         return None

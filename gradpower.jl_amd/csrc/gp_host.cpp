@@ -2826,7 +2826,8 @@ int gp_create(const gp_system_desc *d, int32_t M, const gp_options *opt, gp_ctx 
     C->luJ.force_gx = C->luS.force_gx = C->luJT.force_gx = C->opt.global_x != 0;
     C->luJ.part_k = C->luS.part_k = C->luJT.part_k = C->opt.solve_levels > 0 ? C->opt.solve_levels : 4;
     C->luJ.part_tune = C->luS.part_tune = C->luJT.part_tune = C->opt.solve_levels <= 0;
-    C->prof_solve.every = std::max(1, C->opt.profile);   // profile = k: every k-th solve launch timed
+    // profile = k: every k-th solve launch and every k-th factorisation launch timed
+    C->prof_solve.every = C->prof_factor.every = std::max(1, C->opt.profile);
     C->is_diff.assign(d->is_diff, d->is_diff + d->n);
     C->yrp.assign(d->y_rowptr, d->y_rowptr + 2 * d->nbus + 1);
     C->yci.assign(d->y_colind, d->y_colind + d->y_nnz);

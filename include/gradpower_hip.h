@@ -211,11 +211,11 @@ typedef struct gp_options {
                               1e10, or is not finite, stops at once and is reported
                               as not converged (failed contingencies cost k rounds
                               of the lock step instead of itermax)               */
-  int32_t profile;         /* k >= 1: time every launch of the numeric
-                              factorisation and every k-th launch of the batched
-                              triangular solve with HIP events on the stream it
-                              runs on (gp_stats kernel fields: the solve's cover
-                              the timed launches); 0: no timing                 */
+  int32_t profile;         /* k >= 1: time every k-th launch of the numeric
+                              factorisation and of the batched triangular solve
+                              with HIP events on the stream it runs on
+                              (gp_stats kernel fields cover the timed
+                              launches); 0: no timing                           */
   int32_t lock_step;       /* 0 (default): every scenario advances on its own --
                               a scenario whose step converged moves on while the
                               others iterate; 1: all scenarios take step k
